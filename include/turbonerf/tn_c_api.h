@@ -1,0 +1,174 @@
+/*
+ * C ABI of the TurboNeRF MI355X kernel library (libturbonerf_kernels.so).
+ *
+ * Every entry point takes plain device pointers, element counts, SoA strides
+ * and a hipStream_t, enqueues work on that stream and returns a tn_status
+ * (0 = TN_OK). No entry point allocates device memory: workspaces are passed
+ * in, with their required size reported by a matching *_bytes() query.
+ * Per-ray and per-sample buffers are structure-of-arrays with component
+ * stride `stride` (the reference's batch_size B), e.g. ori[i], ori[i+B],
+ * ori[i+2B].
+ *
+ * Device-side batch descriptor `info` (uint32[4]): info[0] = rays kept in the
+ * batch, info[1] = samples in the batch, info[2] = status of the selection.
+ * It is produced by tn_scan_select() and consumed by later stages, so a
+ * training step needs no host round trip.
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef TURBONERF_TN_C_API_H
+#define TURBONERF_TN_C_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "tn_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* tn_stream_t; /* == hipStream_t */
+
+/* ---------------------------------------------------------------- rays */
+
+/* Seeded uniform floats in [0,1): element j = Philox-4x32-10 word
+ * (offset+j)&3 of counter (offset+j)>>2 under key `seed`.
+ * Replaces curandGenerateUniform (reference src/core/trainer.cu:45-50). */
+int tn_fill_uniform(uint64_t n, uint64_t seed, uint64_t offset, float* out, tn_stream_t stream);
+
+/* Training rays through random pixels, clipped to the training box.
+ * Replaces initialize_training_rays_and_pixels_kernel
+ * (reference src/utils/training-batch-kernels.cuh:82-187). d_n_rays: optional
+ * device count (<= n_rays_max) overriding n_rays_max. */
+int tn_init_training_rays(uint32_t n_rays_max, const uint32_t* d_n_rays, uint32_t stride,
+    uint32_t n_images, uint32_t n_pixels_per_image, int32_t img_w, int32_t img_h,
+    float n_rays_per_image, const tn_bbox_t* bbox, const tn_camera_t* cameras,
+    const uint8_t* images, const float* random, float* pix_rgba, uint32_t* ray_img_id,
+    float* ori, float* dir, float* ray_t, float* ray_tmax, uint8_t* ray_alive,
+    tn_stream_t stream);
+
+/* Replaces deactivate_rays_with_alpha_threshold_kernel
+ * (reference src/utils/training-batch-kernels.cuh:189-208). */
+int tn_deactivate_rays(uint32_t n_rays_max, const uint32_t* d_n_rays, uint32_t stride,
+    float alpha_threshold, float selection_probability, const float* pix_rgba,
+    const float* random, uint8_t* ray_alive, tn_stream_t stream);
+
+/* ------------------------------------------------------------- marching */
+
+/* Steps each ray takes through occupied cells. Replaces
+ * march_and_count_steps_per_ray_kernel (reference
+ * src/utils/training-batch-kernels.cuh:212-298). max_steps caps samples per
+ * ray (0xFFFFFFFF = uncapped, the reference behaviour). */
+int tn_march_count(uint32_t n_rays_max, const uint32_t* d_n_rays, uint32_t stride,
+    tn_bbox_t bbox, tn_occupancy_grid_t grid, float cone_angle, float dt_min, float dt_max,
+    uint32_t max_steps, const float* dir, const float* ray_tmax, uint8_t* ray_alive,
+    const float* ori, const float* ray_t, uint32_t* n_steps, tn_stream_t stream);
+
+/* Exclusive scan of n_steps into ray_offset and selection of the rays that
+ * form the batch; writes info[0..2] on the device. mode 0 (training): keep the
+ * rays before the last one whose offset is below max_samples, the reference's
+ * rule. mode 1 (rendering): keep every ray whose samples end within
+ * max_samples.
+ * Replaces thrust::exclusive_scan + find_last_lt_presorted + the n_samples
+ * device-to-host copy (reference src/core/trainer.cu:143-183,
+ * src/utils/parallel-utils.cuh:30-58). */
+size_t tn_scan_workspace_bytes(uint32_t n_rays);
+int tn_scan_select(int mode, uint32_t n_rays, const uint32_t* n_steps, uint32_t max_samples,
+    uint32_t* ray_offset, uint32_t* info, void* workspace, size_t workspace_bytes,
+    tn_stream_t stream);
+
+/* Re-march and write compacted per-sample SoA buffers. Replaces
+ * march_and_generate_network_positions_kernel (reference
+ * src/utils/training-batch-kernels.cuh:339-481). */
+int tn_march_generate(uint32_t n_rays_max, const uint32_t* info, uint32_t stride,
+    tn_bbox_t bbox, float inv_aabb, tn_occupancy_grid_t grid, float dt_min, float dt_max,
+    float cone_angle, const uint32_t* ray_img_id, const float* ori, const float* dir,
+    const float* ray_t, const uint32_t* ray_offset, const uint8_t* ray_alive,
+    const uint32_t* n_ray_steps, uint32_t* out_img_id, float* out_t, float* out_pos,
+    float* out_dir, float* out_dt, float* out_m_norm, float* out_dt_norm, tn_stream_t stream);
+
+/* Next step's ray count from this step's samples per ray (device-side
+ * replacement of reference src/core/trainer.cu:133-140). */
+int tn_estimate_rays(const uint32_t* info, uint32_t max_samples, uint32_t cap, uint32_t min_rays,
+    uint32_t* n_rays_next, tn_stream_t stream);
+
+/* ---------------------------------------------------------- compositing */
+
+/* Fused compositing + random-background blend + smooth-L1 loss + backward.
+ * Replaces fused_reconstruction_forward_backward_kernel and its memsets
+ * (reference src/utils/training-kernels-fused.cuh:381-498,
+ * src/core/nerf-network.cu:547-603). dtype 0: fp32 density/colour and fp32
+ * colour grads; 1: fp16 in / fp16 colour grads; 2: fp32 with fp64
+ * accumulation (oracle mode). density = raw network output s. Outputs:
+ * loss[4][stride] per ray, dL_ddensity[stride] = dL/ds, dL_dcolor[3][stride]
+ * scaled by grad_scale. */
+size_t tn_composite_scratch_bytes(uint32_t n_rays_max);
+int tn_composite_train(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t stride,
+    float grad_scale, float min_transmittance, const uint32_t* ray_steps,
+    const uint32_t* ray_offset, const float* bg_rgb, const float* target_rgba,
+    const void* density, const void* color, const float* dt, float* loss,
+    float* dL_ddensity, void* dL_dcolor, float* ray_scratch, size_t scratch_bytes,
+    tn_stream_t stream);
+
+/* Forward-only compositing to premultiplied rgba[4][out_stride] per ray
+ * (reference density_to_ray_rgba_forward, training-kernels-fused.cuh:227-264).
+ * raw_density != 0: density holds network outputs, else sigma itself. */
+int tn_composite_forward(int dtype, const uint32_t* info, uint32_t stride, uint32_t max_samples,
+    uint32_t out_stride, float min_transmittance, int raw_density, const uint32_t* ray_steps,
+    const uint32_t* ray_offset, const void* density, const void* color, const float* dt,
+    float* rgba, tn_stream_t stream);
+
+/* (1/info[0]) * sum over rays and n_channels of loss -> out[0] (device).
+ * Replaces the thrust::reduce of the loss buffer (reference
+ * src/core/nerf-network.cu:594-602; its quirk of summing stale entries past
+ * n_rays is not reproduced). */
+int tn_reduce_loss(const uint32_t* info, uint32_t stride, uint32_t n_channels, const float* loss,
+    float* out, tn_stream_t stream);
+
+/* ------------------------------------------------------- occupancy grid */
+
+/* density *= factor (reference decay_occupancy_grid_values_kernel,
+ * src/utils/occupancy-grid-kernels.cu:16-34). */
+int tn_occ_decay(tn_occupancy_grid_t grid, float factor, tn_stream_t stream);
+
+/* Jittered sample points of cells [start_idx, start_idx+n_cells) of `level`
+ * (reference generate_grid_cell_network_sample_points_kernel, :38-76). */
+int tn_occ_sample_points(uint32_t n_cells, uint32_t stride, uint32_t start_idx,
+    tn_occupancy_grid_t grid, int level, float inv_aabb, const float* random, float* pos,
+    tn_stream_t stream);
+
+/* density = max(density, exp(min(s,12)-1)) with the warm-up/random-subset rule
+ * (reference update_occupancy_with_density_kernel, :79-115). raw_dtype 0 =
+ * fp32, 1 = fp16. */
+int tn_occ_update_density(int raw_dtype, uint32_t n_cells, uint32_t start_idx,
+    tn_occupancy_grid_t grid, int level, int sample_all_cells, const float* random,
+    const void* raw_density, tn_stream_t stream);
+
+/* bitfield byte = sum_k [density_k > threshold] << k (reference
+ * update_occupancy_grid_bits_kernel, :118-141). */
+int tn_occ_update_bits(tn_occupancy_grid_t grid, float threshold, tn_stream_t stream);
+
+/* Number of set bits of a byte array -> *out (device u32). Replaces count_1s
+ * (reference src/utils/bit-utils.cuh:30-72). */
+size_t tn_popcount_workspace_bytes(uint64_t n_bytes);
+int tn_popcount(const uint8_t* bytes, uint64_t n_bytes, uint32_t* out, void* workspace,
+    size_t workspace_bytes, tn_stream_t stream);
+
+/* ------------------------------------------- synthetic analytic field */
+
+/* Gaussian-blob field (blob k = {cx,cy,cz,s,p,r,g,b}): density at every cell
+ * centre of every cascade (no reference counterpart: synthetic scenes). */
+int tn_blob_density_grid(tn_occupancy_grid_t grid, const float* blobs, uint32_t n_blobs,
+    tn_stream_t stream);
+
+/* sigma and rgb of the blob field at normalised sample positions. */
+int tn_blob_eval(const uint32_t* info, uint32_t n_max, uint32_t stride, float aabb,
+    const float* blobs, uint32_t n_blobs, const float* pos, float* sigma, float* rgb,
+    tn_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TURBONERF_TN_C_API_H */

@@ -1,0 +1,93 @@
+/*
+ * Plain-C data formats shared by the TurboNeRF MI355X kernel library, its C++
+ * host layer and any FFI binding. Every struct is a POD that is passed by
+ * value (or by device pointer) to the `tn_*` entry points declared in
+ * tn_c_api.h. Layouts follow the reference's device-copyable PODs field by
+ * field where one exists, so a host that already fills the reference structs
+ * can fill these without reinterpretation.
+ */
+#ifndef TURBONERF_TN_TYPES_H
+#define TURBONERF_TN_TYPES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes returned by every entry point (0 = success). */
+enum tn_status {
+    TN_OK = 0,
+    TN_ERR_INVALID_ARGUMENT = 1,   /* null pointer, zero/oversized count, bad stride */
+    TN_ERR_LAUNCH = 2,             /* hipLaunchKernel / hipGetLastError failure */
+    TN_ERR_UNSUPPORTED = 3,        /* feature outside the supported configuration */
+    TN_ERR_OUT_OF_MEMORY = 4,      /* caller-provided workspace too small */
+    TN_ERR_NO_RAYS = 5,            /* "No rays were generated" (reference trainer.cu:162-164) */
+    TN_ERR_NO_SAMPLES = 6          /* "No samples were generated" (reference trainer.cu:181-183) */
+};
+
+/* Axis-aligned box; same field order as BoundingBox
+ * (reference src/models/bounding-box.cuh:8-24). */
+typedef struct tn_bbox {
+    float min_x, min_y, min_z;
+    float max_x, max_y, max_z;
+} tn_bbox_t;
+
+/* Pinhole camera; fields mirror Camera (reference src/models/camera.cuh:45-79).
+ * transform is the 3x4 camera-to-world matrix, row-major
+ * (m00 m01 m02 m03, m10 ..., m20 ...), as Transform4f stores it
+ * (reference src/math/transform4f.cuh). offset_px = shift*resolution - principal_point. */
+typedef struct tn_camera {
+    int32_t resolution[2];
+    float near_plane;
+    float far_plane;
+    float focal_length[2];
+    float resolution_f[2];
+    float principal_point[2];
+    float shift[2];
+    float offset_px[2];
+    float transform[12];
+    float dist_params[5];      /* k1 k2 k3 p1 p2; must be all zero (pinhole only) */
+    int32_t show_image_planes;
+} tn_camera_t;
+
+/* Multi-cascade occupancy grid (reference src/core/occupancy-grid.cuh:22-45).
+ * bitfield: resolution^3 bytes in Morton order; bit k of a byte = cascade k.
+ * density: n_levels * resolution^3 floats, cascade-major. */
+typedef struct tn_occupancy_grid {
+    uint8_t* bitfield;
+    float* density;
+    int32_t n_levels;
+    int32_t resolution;
+} tn_occupancy_grid_t;
+
+/* Multiresolution hash-grid geometry (Instant-NGP encoding configured at
+ * reference src/core/nerf-network.cu:86-104). Filled by tn_hashgrid_layout(). */
+#define TN_HASHGRID_MAX_LEVELS 16
+typedef struct tn_hashgrid {
+    int32_t n_levels;              /* L (16) */
+    int32_t n_features;            /* F (2) */
+    int32_t log2_hashmap_size;     /* log2 T (19) */
+    int32_t base_resolution;       /* N_min (16) */
+    float per_level_scale;         /* b */
+    uint32_t n_params;             /* sum of level sizes * F */
+    float scale[TN_HASHGRID_MAX_LEVELS];          /* N_min * b^l - 1 */
+    uint32_t resolution[TN_HASHGRID_MAX_LEVELS];  /* ceil(scale) + 1 */
+    uint32_t level_size[TN_HASHGRID_MAX_LEVELS];  /* entries in level l */
+    uint32_t level_offset[TN_HASHGRID_MAX_LEVELS];/* first entry of level l */
+    uint32_t level_dense[TN_HASHGRID_MAX_LEVELS]; /* 1 = dense index, 0 = hashed */
+} tn_hashgrid_t;
+
+/* Parameter layout of one NeRF's network (flat buffer, hash grid first):
+ *   [0, n_grid)                        hash-grid table, level-major, F features per entry
+ *   [n_grid, n_grid + 3072)            density MLP: W1[64][32], W2[16][64]
+ *   [n_grid + 3072, n_grid + 3072+6336) colour MLP: W3[64][32], W4[64][64], W5[3][64]
+ * All matrices are row-major [out][in]; the MLPs have no biases. */
+#define TN_DENSITY_MLP_PARAMS (64 * 32 + 16 * 64)
+#define TN_COLOR_MLP_PARAMS (64 * 32 + 64 * 64 + 3 * 64)
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TURBONERF_TN_TYPES_H */

@@ -1,0 +1,427 @@
+// Volume-rendering compositing with random-background blending, smooth-L1
+// loss and its backward pass (reference src/utils/training-kernels-fused.cuh:
+// 227-498, src/core/nerf-network.cu:547-603).
+//
+// Sample-parallel: each wave owns a window of rays (segscan.hpp) and makes two
+// passes over their samples. Pass 1 forms transmittance from a segmented
+// prefix sum of sigma*dt (T = exp(-sum)), the per-ray totals R = sum w c and
+// A = sum w, the loss and dL/dRc, and parks the per-ray totals in `ray_scratch`.
+// Pass 2 recomputes transmittance (nothing per-sample is stored between the
+// passes) and forms, for every sample i of a ray,
+//     g_i       = sum_ch G_ch (c_i,ch A + R_ch - bg_ch)
+//     dL/dsig_i = dt_i (T_{i+1} g_i - sum_{k>i} w_k g_k)
+//     dL/dc_i   = scale G w_i A
+// with the suffix sum taken as the closed-form total A sum_ch G_ch(2R_ch-bg_ch)
+// minus the compensated inclusive prefix. This is the reference's adjoint
+// (its running "cumsum" subtraction) written for a lane-parallel scan.
+// Samples after the early-termination point (T < min_T) get zero gradient,
+// as in the reference.
+
+#include "segscan.hpp"
+
+namespace tn {
+
+template <typename T>
+TN_DEVICE float ldf(const T* p, uint64_t i) { return (float)p[i]; }
+
+template <typename T>
+TN_DEVICE void stf(T* p, uint64_t i, float v) { p[i] = (T)v; }
+
+constexpr int kCompositeWaves = 4;  // waves per workgroup
+
+TN_DEVICE float smooth_l1(float x) {
+    const float ax = fabsf(x);
+    return ax < 1.0f ? 0.5f * x * x : ax - 0.5f;
+}
+TN_DEVICE float smooth_l1_grad(float x) { return fabsf(x) < 1.0f ? x : copysignf(1.0f, x); }
+
+// per-ray scratch rows (stride = ray capacity): A, R[3], G[3]
+constexpr int kRayScratchRows = 7;
+
+// loss and dL/dRc of one ray from its totals (reference :430-477)
+template <typename R>
+TN_DEVICE void ray_loss(uint32_t r, uint32_t B, R rr, R rg, R rb, R ra, const float* __restrict__ bg,
+                        const float* __restrict__ target, float inv_n_rays, float* __restrict__ loss,
+                        float* __restrict__ scratch, uint32_t SB) {
+    const float gr = target[r], gg = target[r + B], gb = target[r + 2 * B], ga = target[r + 3 * B];
+    const float br = bg[r], bgg = bg[r + B], bb = bg[r + 2 * B];
+    const float gac = 1.0f - ga;
+    const float grc = gr * ga + br * gac, ggc = gg * ga + bgg * gac, gbc = gb * ga + bb * gac;
+    const R rac = (R)1 - ra;
+    const R rrc = rr * ra + (R)br * rac, rgc = rg * ra + (R)bgg * rac, rbc = rb * ra + (R)bb * rac;
+    const float dr = (float)(rrc - (R)grc), dg = (float)(rgc - (R)ggc), db = (float)(rbc - (R)gbc);
+    const float da = (float)(rac - (R)gac);
+    loss[r] = smooth_l1(dr);
+    loss[r + B] = smooth_l1(dg);
+    loss[r + 2 * B] = smooth_l1(db);
+    loss[r + 3 * B] = smooth_l1(da);
+    if (scratch) {
+        scratch[r] = (float)ra;
+        scratch[r + SB] = (float)rr;
+        scratch[r + 2 * SB] = (float)rg;
+        scratch[r + 3 * SB] = (float)rb;
+        scratch[r + 4 * SB] = inv_n_rays * smooth_l1_grad(dr);
+        scratch[r + 5 * SB] = inv_n_rays * smooth_l1_grad(dg);
+        scratch[r + 6 * SB] = inv_n_rays * smooth_l1_grad(db);
+    }
+}
+
+// Per-lane recomputation shared by both passes: transmittance from the
+// segmented prefix of sigma*dt, alpha, weight and the early-stop mask.
+template <typename R>
+struct SampleState {
+    R sigma, dt, T, T_next, w;
+    bool inc;
+};
+
+template <typename R, typename TIn>
+TN_DEVICE SampleState<R> sample_state(bool valid, int ray, uint64_t s, const TIn* __restrict__ density,
+                                      const float* __restrict__ dtb, int& carry_ray, Acc<R>& carry_P,
+                                      float min_T, bool raw_density = true) {
+    SampleState<R> st;
+    st.sigma = 0;
+    st.dt = 0;
+    if (valid) {
+        const float d = ldf(density, s);
+        st.sigma = raw_density ? r_exp<R>((R)fminf(d, 12.0f) - (R)1) : (R)d;
+        st.dt = (R)dtb[s];
+    }
+    const R tau = st.sigma * st.dt;
+    Acc<R> P = Acc<R>::of(tau).scan(ray);
+    if (ray == carry_ray) P = P + carry_P;
+    const R p_incl = P.val();
+    const R p_excl = p_incl - tau;
+    st.T = r_exp<R>(-p_excl);
+    st.T_next = r_exp<R>(-p_incl);
+    const R alpha = (R)1 - r_exp<R>(-tau);
+    st.w = st.T * alpha;
+    st.inc = valid && st.T >= (R)min_T;
+    // carry of the prefix for the ray continuing past lane 63
+    const int ray63 = __shfl(ray, 63, 64);
+    carry_P = P.bcast(63);
+    carry_ray = ray63;
+    return st;
+}
+
+template <typename TIn, typename TGrad, typename R>
+__global__ void __launch_bounds__(64 * kCompositeWaves)
+composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t SB, uint32_t S,
+                       float grad_scale, float min_T, const uint32_t* __restrict__ steps,
+                       const uint32_t* __restrict__ offs, const float* __restrict__ bg,
+                       const float* __restrict__ target, const TIn* __restrict__ density,
+                       const TIn* __restrict__ color, const float* __restrict__ dtb, float* __restrict__ loss,
+                       float* __restrict__ dL_ddensity, TGrad* __restrict__ dL_dcolor,
+                       float* __restrict__ scratch) {
+    __shared__ int slots[kCompositeWaves][64];
+    const uint32_t wid = threadIdx.x >> 6;
+    const uint32_t lane = __lane_id();
+    const uint32_t w = blockIdx.x * kCompositeWaves + wid;
+    const uint32_t n_rays = info[0], n_samples = info[1];
+    const WaveRange wr = wave_range(w, S, n_rays, n_samples, offs);
+    if (!wr.active) return;
+    const float inv_n_rays = 1.0f / (float)n_rays;
+    int* slot = slots[wid];
+
+    auto empty_ray = [&](uint32_t r) {
+        ray_loss<float>(r, B, 0.0f, 0.0f, 0.0f, 0.0f, bg, target, inv_n_rays, loss, nullptr, SB);
+    };
+
+    // ---------------- pass 1: forward totals, loss, dL/dRc ----------------
+    {
+        uint32_t r_scan = wr.r0;
+        int carry_ray = -1;
+        Acc<R> carry_P = Acc<R>::zero();
+        Acc<R> cr = Acc<R>::zero(), cg = Acc<R>::zero(), cb = Acc<R>::zero(), ca = Acc<R>::zero();
+        for (uint32_t c = wr.s0; c < wr.s1; c += 64) {
+            const uint64_t s = (uint64_t)c + lane;
+            const bool valid = s < wr.s1;
+            const int prev_carry = carry_ray;
+            int ray = chunk_rays(c, wr.r1, r_scan, offs, steps, slot, carry_ray, empty_ray);
+            if (!valid) ray = -2 - (int)lane;  // unique keys keep padding lanes out of every segment
+            SampleState<R> st = sample_state<R>(valid, ray, s, density, dtb, carry_ray, carry_P, min_T);
+            float cR = 0, cG = 0, cB = 0;
+            if (valid) {
+                cR = ldf(color, s);
+                cG = ldf(color, s + B);
+                cB = ldf(color, s + 2 * (uint64_t)B);
+            }
+            const R wv = st.inc ? st.w : (R)0;
+            Acc<R> sr = Acc<R>::of(wv * (R)cR).scan(ray), sg = Acc<R>::of(wv * (R)cG).scan(ray);
+            Acc<R> sb = Acc<R>::of(wv * (R)cB).scan(ray), sa = Acc<R>::of(wv).scan(ray);
+            if (ray == prev_carry) {
+                sr = sr + cr;
+                sg = sg + cg;
+                sb = sb + cb;
+                sa = sa + ca;
+            }
+            // tail lanes finish their ray
+            bool tail = false;
+            if (valid) {
+                const uint32_t rr = (uint32_t)ray;
+                tail = s + 1 == (uint64_t)offs[rr] + steps[rr];
+                if (tail)
+                    ray_loss<R>(rr, B, sr.val(), sg.val(), sb.val(), sa.val(), bg, target, inv_n_rays, loss,
+                                scratch, SB);
+            }
+            // carry of the running totals for a ray continuing past this chunk
+            const bool tail63 = __shfl((int)tail, 63, 64) != 0;
+            cr = sr.bcast(63);
+            cg = sg.bcast(63);
+            cb = sb.bcast(63);
+            ca = sa.bcast(63);
+            if (tail63 || !__shfl((int)valid, 63, 64)) carry_ray = -1;
+        }
+        // zero-length rays at the end of the window
+        for (uint32_t r = r_scan + lane; r < wr.r1; r += 64) empty_ray(r);
+    }
+    // pass 2 reads the per-ray totals written by other lanes of this wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+    // ---------------- pass 2: gradients ----------------
+    {
+        uint32_t r_scan = wr.r0;
+        int carry_ray = -1;
+        Acc<R> carry_P = Acc<R>::zero();
+        Acc<R> carry_Q = Acc<R>::zero();
+        auto no_op = [](uint32_t) {};
+        for (uint32_t c = wr.s0; c < wr.s1; c += 64) {
+            const uint64_t s = (uint64_t)c + lane;
+            const bool valid = s < wr.s1;
+            const int prev_carry = carry_ray;
+            int ray = chunk_rays(c, wr.r1, r_scan, offs, steps, slot, carry_ray, no_op);
+            if (!valid) ray = -2 - (int)lane;
+            SampleState<R> st = sample_state<R>(valid, ray, s, density, dtb, carry_ray, carry_P, min_T);
+            R g = 0, wg = 0;
+            float cR = 0, cG = 0, cB = 0, A = 0, GR = 0, GG = 0, GB = 0, Wg_tot = 0;
+            if (valid) {
+                const uint32_t rr = (uint32_t)ray;
+                cR = ldf(color, s);
+                cG = ldf(color, s + B);
+                cB = ldf(color, s + 2 * (uint64_t)B);
+                A = scratch[rr];
+                const float RR = scratch[rr + SB], RG = scratch[rr + 2 * SB], RB = scratch[rr + 3 * SB];
+                GR = scratch[rr + 4 * SB];
+                GG = scratch[rr + 5 * SB];
+                GB = scratch[rr + 6 * SB];
+                const float br = bg[rr], bgg = bg[rr + B], bb = bg[rr + 2 * B];
+                g = (R)GR * ((R)cR * (R)A + (R)RR - (R)br) + (R)GG * ((R)cG * (R)A + (R)RG - (R)bgg) +
+                    (R)GB * ((R)cB * (R)A + (R)RB - (R)bb);
+                Wg_tot = A * (GR * (2.0f * RR - br) + GG * (2.0f * RG - bgg) + GB * (2.0f * RB - bb));
+                wg = st.inc ? st.w * g : (R)0;
+            }
+            Acc<R> Q = Acc<R>::of(wg).scan(ray);
+            if (ray == prev_carry) Q = Q + carry_Q;
+            if (valid) {
+                float dsig = 0.0f, gc_r = 0.0f, gc_g = 0.0f, gc_b = 0.0f;
+                if (st.inc) {
+                    const R suffix = (R)Wg_tot - Q.val();
+                    dsig = (float)(st.dt * (st.T_next * g - suffix));
+                    const float k = grad_scale * (float)st.w * A;
+                    gc_r = k * GR;
+                    gc_g = k * GG;
+                    gc_b = k * GB;
+                }
+                dL_ddensity[s] = dsig * (float)st.sigma;
+                stf(dL_dcolor, s, gc_r);
+                stf(dL_dcolor, s + B, gc_g);
+                stf(dL_dcolor, s + 2 * (uint64_t)B, gc_b);
+            }
+            carry_Q = Q.bcast(63);
+            const bool tail63 =
+                __shfl((int)(valid && s + 1 == (uint64_t)offs[valid ? ray : 0] + steps[valid ? ray : 0]), 63, 64) != 0;
+            if (tail63 || !__shfl((int)valid, 63, 64)) carry_ray = -1;
+        }
+    }
+}
+
+// Forward-only compositing: premultiplied rgb and alpha per ray, with the
+// reference's early termination. `raw_density` selects sigma = exp(min(s,12)-1)
+// of a network output, otherwise the input already is sigma.
+template <typename TIn, typename R>
+__global__ void __launch_bounds__(64 * kCompositeWaves)
+composite_forward_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t OB, uint32_t S, float min_T,
+                         int raw_density, const uint32_t* __restrict__ steps, const uint32_t* __restrict__ offs,
+                         const TIn* __restrict__ density, const TIn* __restrict__ color,
+                         const float* __restrict__ dtb, float* __restrict__ rgba) {
+    __shared__ int slots[kCompositeWaves][64];
+    const uint32_t wid = threadIdx.x >> 6;
+    const uint32_t lane = __lane_id();
+    const uint32_t w = blockIdx.x * kCompositeWaves + wid;
+    const uint32_t n_rays = info[0], n_samples = info[1];
+    const WaveRange wr = wave_range(w, S, n_rays, n_samples, offs);
+    if (!wr.active) return;
+    int* slot = slots[wid];
+    auto empty_ray = [&](uint32_t r) {
+        rgba[r] = 0.0f;
+        rgba[r + OB] = 0.0f;
+        rgba[r + 2 * OB] = 0.0f;
+        rgba[r + 3 * OB] = 0.0f;
+    };
+    uint32_t r_scan = wr.r0;
+    int carry_ray = -1;
+    Acc<R> carry_P = Acc<R>::zero();
+    Acc<R> cr = Acc<R>::zero(), cg = Acc<R>::zero(), cb = Acc<R>::zero(), ca = Acc<R>::zero();
+    for (uint32_t c = wr.s0; c < wr.s1; c += 64) {
+        const uint64_t s = (uint64_t)c + lane;
+        const bool valid = s < wr.s1;
+        const int prev_carry = carry_ray;
+        int ray = chunk_rays(c, wr.r1, r_scan, offs, steps, slot, carry_ray, empty_ray);
+        if (!valid) ray = -2 - (int)lane;
+        SampleState<R> st =
+            sample_state<R>(valid, ray, s, density, dtb, carry_ray, carry_P, min_T, raw_density != 0);
+        float cR = 0, cG = 0, cB = 0;
+        if (valid) {
+            cR = ldf(color, s);
+            cG = ldf(color, s + B);
+            cB = ldf(color, s + 2 * (uint64_t)B);
+        }
+        const R wv = st.inc ? st.w : (R)0;
+        Acc<R> sr = Acc<R>::of(wv * (R)cR).scan(ray), sg = Acc<R>::of(wv * (R)cG).scan(ray);
+        Acc<R> sb = Acc<R>::of(wv * (R)cB).scan(ray), sa = Acc<R>::of(wv).scan(ray);
+        if (ray == prev_carry) {
+            sr = sr + cr;
+            sg = sg + cg;
+            sb = sb + cb;
+            sa = sa + ca;
+        }
+        bool tail = false;
+        if (valid) {
+            const uint32_t rr = (uint32_t)ray;
+            tail = s + 1 == (uint64_t)offs[rr] + steps[rr];
+            if (tail) {
+                rgba[rr] = (float)sr.val();
+                rgba[rr + OB] = (float)sg.val();
+                rgba[rr + 2 * OB] = (float)sb.val();
+                rgba[rr + 3 * OB] = (float)sa.val();
+            }
+        }
+        const bool tail63 = __shfl((int)tail, 63, 64) != 0;
+        cr = sr.bcast(63);
+        cg = sg.bcast(63);
+        cb = sb.bcast(63);
+        ca = sa.bcast(63);
+        if (tail63 || !__shfl((int)valid, 63, 64)) carry_ray = -1;
+    }
+    for (uint32_t r = r_scan + lane; r < wr.r1; r += 64) empty_ray(r);
+}
+
+// deterministic mean of the per-ray losses: (1/n_rays) sum_r sum_ch loss
+__global__ void reduce_loss_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t n_ch,
+                                   const float* __restrict__ loss, float* __restrict__ out) {
+    __shared__ double part[16];
+    const uint32_t n = info[0];
+    double acc = 0.0;
+    for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) {
+        float s = 0.0f;
+        for (uint32_t k = 0; k < n_ch; ++k) s += loss[r + (uint64_t)k * B];
+        acc += (double)s;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_down(acc, d, 64);
+    if (__lane_id() == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (uint32_t k = 0; k < blockDim.x / 64; ++k) t += part[k];
+        out[0] = n > 0 ? (float)(t / (double)n) : 0.0f;
+    }
+}
+
+template <typename TIn, typename TGrad, typename R>
+int launch_composite(const uint32_t* info, uint32_t n_rays_max, uint32_t B, uint32_t max_samples, float grad_scale,
+                     float min_T, const uint32_t* steps, const uint32_t* offs, const float* bg, const float* target,
+                     const void* density, const void* color, const float* dt, float* loss, float* dL_ddensity,
+                     void* dL_dcolor, float* scratch, hipStream_t stream) {
+    const uint32_t S = 128;
+    const uint32_t waves = (max_samples + S - 1) / S + 1;
+    const uint32_t blocks = (waves + kCompositeWaves - 1) / kCompositeWaves;
+    composite_train_kernel<TIn, TGrad, R><<<blocks, 64 * kCompositeWaves, 0, stream>>>(
+        info, B, n_rays_max, S, grad_scale, min_T, steps, offs, bg, target, static_cast<const TIn*>(density),
+        static_cast<const TIn*>(color), dt, loss, dL_ddensity, static_cast<TGrad*>(dL_dcolor), scratch);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+}  // namespace tn
+
+using namespace tn;
+
+extern "C" size_t tn_composite_scratch_bytes(uint32_t n_rays_max) {
+    return (size_t)kRayScratchRows * n_rays_max * sizeof(float);
+}
+
+// dtype: 0 = fp32 network outputs and fp32 colour gradients,
+//        1 = fp16 network outputs and fp16 colour gradients,
+//        2 = fp32 in/out with fp64 accumulation (oracle mode)
+extern "C" int tn_composite_train(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t stride,
+                                  float grad_scale, float min_transmittance, const uint32_t* ray_steps,
+                                  const uint32_t* ray_offset, const float* bg_rgb, const float* target_rgba,
+                                  const void* density, const void* color, const float* dt, float* loss,
+                                  float* dL_ddensity, void* dL_dcolor, float* ray_scratch, size_t scratch_bytes,
+                                  hipStream_t stream) {
+    TN_CHECK_ARG(info && ray_steps && ray_offset && bg_rgb && target_rgba && density && color && dt && loss &&
+                 dL_ddensity && dL_dcolor && ray_scratch);
+    TN_CHECK_ARG(n_rays_max > 0 && n_rays_max <= stride);
+    if (scratch_bytes < tn_composite_scratch_bytes(n_rays_max)) return TN_ERR_OUT_OF_MEMORY;
+    switch (dtype) {
+        case 0:
+            return launch_composite<float, float, float>(info, n_rays_max, stride, stride, grad_scale, min_transmittance,
+                                                         ray_steps, ray_offset, bg_rgb, target_rgba, density, color,
+                                                         dt, loss, dL_ddensity, dL_dcolor, ray_scratch, stream);
+        case 1:
+            return launch_composite<_Float16, _Float16, float>(info, n_rays_max, stride, stride, grad_scale,
+                                                               min_transmittance, ray_steps, ray_offset, bg_rgb,
+                                                               target_rgba, density, color, dt, loss, dL_ddensity,
+                                                               dL_dcolor, ray_scratch, stream);
+        case 2:
+            return launch_composite<float, float, double>(info, n_rays_max, stride, stride, grad_scale,
+                                                          min_transmittance, ray_steps, ray_offset, bg_rgb,
+                                                          target_rgba, density, color, dt, loss, dL_ddensity,
+                                                          dL_dcolor, ray_scratch, stream);
+        default:
+            return TN_ERR_INVALID_ARGUMENT;
+    }
+}
+
+extern "C" int tn_reduce_loss(const uint32_t* info, uint32_t stride, uint32_t n_channels, const float* loss,
+                              float* out, hipStream_t stream) {
+    TN_CHECK_ARG(info && loss && out && n_channels >= 1 && n_channels <= 4);
+    reduce_loss_kernel<<<1, 1024, 0, stream>>>(info, stride, n_channels, loss, out);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+// dtype: 0 = fp32 inputs, 1 = fp16 inputs, 2 = fp32 inputs with fp64 accumulation.
+// raw_density != 0: density holds network outputs s (sigma = exp(min(s,12)-1)),
+// otherwise density holds sigma itself. rgba is [4][out_stride], premultiplied.
+extern "C" int tn_composite_forward(int dtype, const uint32_t* info, uint32_t stride, uint32_t max_samples,
+                                    uint32_t out_stride, float min_transmittance, int raw_density,
+                                    const uint32_t* ray_steps, const uint32_t* ray_offset, const void* density,
+                                    const void* color, const float* dt, float* rgba, hipStream_t stream) {
+    TN_CHECK_ARG(info && ray_steps && ray_offset && density && color && dt && rgba && max_samples <= stride);
+    const uint32_t S = 128;
+    const uint32_t waves = (max_samples + S - 1) / S + 1;
+    const uint32_t blocks = (waves + kCompositeWaves - 1) / kCompositeWaves;
+    switch (dtype) {
+        case 0:
+            composite_forward_kernel<float, float><<<blocks, 64 * kCompositeWaves, 0, stream>>>(
+                info, stride, out_stride, S, min_transmittance, raw_density, ray_steps, ray_offset,
+                static_cast<const float*>(density), static_cast<const float*>(color), dt, rgba);
+            break;
+        case 1:
+            composite_forward_kernel<_Float16, float><<<blocks, 64 * kCompositeWaves, 0, stream>>>(
+                info, stride, out_stride, S, min_transmittance, raw_density, ray_steps, ray_offset,
+                static_cast<const _Float16*>(density), static_cast<const _Float16*>(color), dt, rgba);
+            break;
+        case 2:
+            composite_forward_kernel<float, double><<<blocks, 64 * kCompositeWaves, 0, stream>>>(
+                info, stride, out_stride, S, min_transmittance, raw_density, ray_steps, ray_offset,
+                static_cast<const float*>(density), static_cast<const float*>(color), dt, rgba);
+            break;
+        default:
+            return TN_ERR_INVALID_ARGUMENT;
+    }
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}

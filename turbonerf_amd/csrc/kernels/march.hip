@@ -1,0 +1,339 @@
+// Occupancy-grid ray marching and compacted sample generation
+// (reference src/utils/training-batch-kernels.cuh:212-481,
+//  src/core/occupancy-grid.cuh:150-282, src/core/trainer.cu:102-216).
+//
+// The count and generate passes run the same DDA with contraction disabled,
+// so the samples written by the generate pass are exactly the ones counted.
+// The exclusive scan and the prefix selection run on the device and leave
+// (n_rays, n_samples) in a device-side tn_batch_info, so a training step has
+// no host round trip between marching and the network.
+#pragma clang fp contract(off)
+
+#include "common.hpp"
+
+namespace tn {
+
+struct MarchGrid {
+    const uint8_t* bits;
+    int n_levels;
+    float res_f;
+    float inv_res;
+    uint32_t lim;
+};
+
+TN_DEVICE MarchGrid make_grid(const tn_occupancy_grid_t& g) {
+    MarchGrid m;
+    m.bits = g.bitfield;
+    m.n_levels = g.n_levels;
+    m.res_f = (float)g.resolution;
+    m.inv_res = 1.0f / m.res_f;
+    m.lim = (uint32_t)g.resolution - 1u;
+    return m;
+}
+
+TN_DEVICE bool occupied(const MarchGrid& g, int level, float x, float y, float z) {
+    const uint32_t m = voxel_morton_index(g.res_f, g.lim, level, x, y, z);
+    return (g.bits[m] >> level) & 1u;
+}
+
+__global__ void march_count_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays, uint32_t B,
+                                   tn_bbox_t bbox, tn_occupancy_grid_t grid, float cone, float dt_min,
+                                   float dt_max, uint32_t max_steps, const float* __restrict__ dir,
+                                   const float* __restrict__ ray_tmax, uint8_t* __restrict__ ray_alive,
+                                   const float* __restrict__ ori, const float* __restrict__ ray_t,
+                                   uint32_t* __restrict__ n_steps) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_rays = d_n_rays ? min(*d_n_rays, n_rays_max) : n_rays_max;
+    if (i >= n_rays) return;
+    if (!ray_alive[i]) {
+        n_steps[i] = 0;
+        return;
+    }
+    const MarchGrid g = make_grid(grid);
+    const float ox = ori[i], oy = ori[i + B], oz = ori[i + 2 * B];
+    const float dx = dir[i], dy = dir[i + B], dz = dir[i + 2 * B];
+    const float idx = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
+    uint32_t n = 0;
+    float t = ray_t[i];
+    const float t_max = ray_tmax[i];
+    while (t < t_max && n < max_steps) {
+        const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
+        if (!bbox_contains(bbox, x, y, z)) break;
+        const float dt = get_dt(t, cone, dt_min, dt_max);
+        const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
+        if (occupied(g, lvl, x, y, z)) {
+            t += dt;
+            ++n;
+        } else {
+            t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, dx, dy, dz, idx, idy, idz, dt_min, lvl);
+        }
+    }
+    if (n == 0) ray_alive[i] = 0;
+    n_steps[i] = n;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan + prefix selection. Three launches: per-tile sums, a single
+// workgroup scanning the tile sums (at most kScanTile tiles), and a per-tile
+// scan that writes offsets and picks the last ray whose offset is below the
+// sample budget.
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr uint32_t kScanTile = kScanThreads * kScanItems;  // 4096 rays per tile
+
+// inclusive scan of one value per thread across a 256-thread block
+TN_DEVICE uint32_t block_inclusive_scan(uint32_t v, uint32_t* smem /*[kScanThreads/64]*/, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += o;
+    }
+    if (lane == 63) smem[wid] = v;
+    __syncthreads();
+    uint32_t pre = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        const uint32_t s = smem[w];
+        if ((uint32_t)w < wid) pre += s;
+        total += s;
+    }
+    __syncthreads();
+    return v + pre;
+}
+
+__global__ void scan_tile_sums_kernel(uint32_t n, const uint32_t* __restrict__ vals, uint32_t* __restrict__ tile_sums) {
+    __shared__ uint32_t smem[kScanThreads / 64];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t j = base + k;
+        if (j < n) s += vals[j];
+    }
+    uint32_t total;
+    block_inclusive_scan(s, smem, total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void scan_tile_prefix_kernel(uint32_t n_tiles, uint32_t* __restrict__ tile_sums) {
+    __shared__ uint32_t smem[kScanThreads / 64];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n_tiles; base += kScanThreads) {
+        const uint32_t j = base + threadIdx.x;
+        const uint32_t v = j < n_tiles ? tile_sums[j] : 0u;
+        uint32_t total;
+        const uint32_t inc = block_inclusive_scan(v, smem, total);
+        if (j < n_tiles) tile_sums[j] = carry + inc - v;  // exclusive
+        carry += total;
+    }
+}
+
+// info[0] = n_rays kept, info[1] = n_samples, info[2] = status.
+// mode 0 (training, the reference's rule): j = last ray whose offset is below
+//   max_samples; rays [0, j) are kept and n_samples = offset[j].
+// mode 1 (rendering): every ray whose samples end within max_samples is kept.
+__global__ void scan_write_select_kernel(uint32_t n, const uint32_t* __restrict__ vals,
+                                         const uint32_t* __restrict__ tile_prefix, uint32_t max_samples, int mode,
+                                         uint32_t* __restrict__ offsets, uint32_t* __restrict__ info) {
+    __shared__ uint32_t smem[kScanThreads / 64];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t j = base + k;
+        v[k] = j < n ? vals[j] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    const uint32_t inc = block_inclusive_scan(s, smem, total);
+    uint64_t run = (uint64_t)tile_prefix[blockIdx.x] + inc - s;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t j = base + k;
+        if (j < n) {
+            const uint32_t off = (uint32_t)min(run, (uint64_t)0xFFFFFFFFu);
+            offsets[j] = off;
+            const uint64_t next = run + v[k];
+            // offsets are non-decreasing, so exactly one ray is the last with
+            // offset < max_samples; the reference keeps the rays before it.
+            if (mode == 0) {
+                const bool last_lt = off < max_samples && (j + 1 == n || next >= max_samples);
+                if (last_lt) {
+                    info[0] = j;
+                    info[1] = off;
+                    info[2] = j == 0 ? (uint32_t)TN_ERR_NO_RAYS : (off == 0 ? (uint32_t)TN_ERR_NO_SAMPLES : 0u);
+                }
+            } else {
+                // ends are non-decreasing: the last ray ending within budget is unique
+                const uint64_t after = (j + 1 < n) ? next + vals[j + 1] : ~0ull;
+                const bool last_fit = next <= max_samples && (j + 1 == n || after > max_samples);
+                if (last_fit) {
+                    info[0] = j + 1;
+                    info[1] = (uint32_t)next;
+                    info[2] = next == 0 ? (uint32_t)TN_ERR_NO_SAMPLES : 0u;
+                }
+                if (j == 0 && v[k] > max_samples) {
+                    info[0] = 0;
+                    info[1] = 0;
+                    info[2] = (uint32_t)TN_ERR_NO_RAYS;
+                }
+            }
+            run = next;
+        }
+    }
+}
+
+__global__ void march_generate_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ info, uint32_t B,
+                                      tn_bbox_t bbox, float inv_aabb, tn_occupancy_grid_t grid, float dt_min,
+                                      float dt_max, float cone, const uint32_t* __restrict__ ray_img_id,
+                                      const float* __restrict__ ori, const float* __restrict__ dir,
+                                      const float* __restrict__ ray_t, const uint32_t* __restrict__ ray_offset,
+                                      const uint8_t* __restrict__ ray_alive, const uint32_t* __restrict__ n_ray_steps,
+                                      uint32_t* __restrict__ out_img_id, float* __restrict__ out_t,
+                                      float* __restrict__ out_pos, float* __restrict__ out_dir,
+                                      float* __restrict__ out_dt, float* __restrict__ out_m_norm,
+                                      float* __restrict__ out_dt_norm) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_rays = info ? min(info[0], n_rays_max) : n_rays_max;
+    if (i >= n_rays || !ray_alive[i]) return;
+    const MarchGrid g = make_grid(grid);
+    const uint32_t n_steps = n_ray_steps[i];
+    const uint32_t off = ray_offset[i];
+    const uint32_t img = ray_img_id[i];
+    const float ox = ori[i], oy = ori[i + B], oz = ori[i + 2 * B];
+    const float dx = dir[i], dy = dir[i + B], dz = dir[i + 2 * B];
+    const float idx = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
+    const float ndx = 0.5f * dx + 0.5f, ndy = 0.5f * dy + 0.5f, ndz = 0.5f * dz + 0.5f;
+    float t = ray_t[i];
+    const float t_min = t;
+    float t_max = t;
+    uint32_t n = 0;
+    while (n < n_steps) {
+        const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
+        const float dt = get_dt(t, cone, dt_min, dt_max);
+        const bool inside = bbox_contains(bbox, x, y, z);
+        bool emit = !inside;  // outside the box: emit without advancing (unreachable when passes agree)
+        int lvl = 0;
+        if (inside) {
+            lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
+            emit = occupied(g, lvl, x, y, z);
+        }
+        if (emit) {
+            const uint32_t sid = off + n;
+            out_t[sid] = t;
+            out_m_norm[sid] = t + 0.5f * dt;
+            out_dt_norm[sid] = dt;
+            out_img_id[sid] = img;
+            out_dt[sid] = inv_aabb * dt;
+            out_pos[sid] = fminf(fmaxf(x * inv_aabb + 0.5f, 0.0f), 1.0f);
+            out_pos[sid + B] = fminf(fmaxf(y * inv_aabb + 0.5f, 0.0f), 1.0f);
+            out_pos[sid + 2 * B] = fminf(fmaxf(z * inv_aabb + 0.5f, 0.0f), 1.0f);
+            out_dir[sid] = ndx;
+            out_dir[sid + B] = ndy;
+            out_dir[sid + 2 * B] = ndz;
+            if (inside) {
+                t += dt;
+                t_max = t;
+            }
+            ++n;
+        } else {
+            t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, dx, dy, dz, idx, idy, idz, dt_min, lvl);
+        }
+    }
+    const float t_span = fmaxf(t_max - t_min, 1e-6f);
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t sid = off + j;
+        out_m_norm[sid] = fminf(fmaxf((out_m_norm[sid] - t_min) / t_span, 0.0f), 1.0f);
+        out_dt_norm[sid] = out_dt_norm[sid] / t_span;
+    }
+}
+
+// Next step's ray count: the rays that would fill the sample budget at this
+// step's samples-per-ray rate, with 1/8 headroom so the prefix selection
+// usually has rays to spare (replaces the reference's host-side estimate,
+// trainer.cu:133-140).
+__global__ void estimate_rays_kernel(const uint32_t* __restrict__ info, uint32_t max_samples, uint32_t cap,
+                                     uint32_t min_rays, uint32_t* __restrict__ n_rays_next) {
+    const uint32_t n_rays = info[0], n_samples = info[1];
+    uint32_t est = cap;
+    if (n_samples > 0 && n_rays > 0) {
+        const double per = (double)n_rays / (double)n_samples;
+        const double e = per * (double)max_samples * 1.125 + 64.0;
+        est = e >= (double)cap ? cap : (uint32_t)e;
+    }
+    if (est < min_rays) est = min_rays;
+    if (est > cap) est = cap;
+    *n_rays_next = est;
+}
+
+}  // namespace tn
+
+using namespace tn;
+
+extern "C" int tn_march_count(uint32_t n_rays_max, const uint32_t* d_n_rays, uint32_t stride,
+                              tn_bbox_t bbox, tn_occupancy_grid_t grid, float cone_angle, float dt_min,
+                              float dt_max, uint32_t max_steps, const float* dir, const float* ray_tmax,
+                              uint8_t* ray_alive, const float* ori, const float* ray_t, uint32_t* n_steps,
+                              hipStream_t stream) {
+    TN_CHECK_ARG(dir && ray_tmax && ray_alive && ori && ray_t && n_steps && grid.bitfield);
+    TN_CHECK_ARG(n_rays_max <= stride && grid.n_levels >= 1 && grid.n_levels <= 8);
+    TN_CHECK_ARG(grid.resolution >= 2 && grid.resolution <= 1024 && dt_min > 0.0f && dt_max >= dt_min);
+    if (n_rays_max == 0) return TN_OK;
+    march_count_kernel<<<blocks_for(n_rays_max, 64), 64, 0, stream>>>(
+        n_rays_max, d_n_rays, stride, bbox, grid, cone_angle, dt_min, dt_max, max_steps, dir, ray_tmax,
+        ray_alive, ori, ray_t, n_steps);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+extern "C" size_t tn_scan_workspace_bytes(uint32_t n) {
+    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+    return (size_t)(tiles == 0 ? 1 : tiles) * sizeof(uint32_t);
+}
+
+extern "C" int tn_scan_select(int mode, uint32_t n_rays, const uint32_t* n_steps, uint32_t max_samples,
+                              uint32_t* ray_offset, uint32_t* info, void* workspace, size_t workspace_bytes,
+                              hipStream_t stream) {
+    TN_CHECK_ARG(mode == 0 || mode == 1);
+    TN_CHECK_ARG(n_steps && ray_offset && info && workspace && n_rays > 0 && max_samples > 0);
+    if (workspace_bytes < tn_scan_workspace_bytes(n_rays)) return TN_ERR_OUT_OF_MEMORY;
+    const uint32_t tiles = (n_rays + kScanTile - 1) / kScanTile;
+    uint32_t* tile_sums = static_cast<uint32_t*>(workspace);
+    scan_tile_sums_kernel<<<tiles, kScanThreads, 0, stream>>>(n_rays, n_steps, tile_sums);
+    scan_tile_prefix_kernel<<<1, kScanThreads, 0, stream>>>(tiles, tile_sums);
+    scan_write_select_kernel<<<tiles, kScanThreads, 0, stream>>>(n_rays, n_steps, tile_sums, max_samples, mode,
+                                                                 ray_offset, info);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+extern "C" int tn_march_generate(uint32_t n_rays_max, const uint32_t* info, uint32_t stride, tn_bbox_t bbox,
+                                 float inv_aabb, tn_occupancy_grid_t grid, float dt_min, float dt_max,
+                                 float cone_angle, const uint32_t* ray_img_id, const float* ori,
+                                 const float* dir, const float* ray_t, const uint32_t* ray_offset,
+                                 const uint8_t* ray_alive, const uint32_t* n_ray_steps, uint32_t* out_img_id,
+                                 float* out_t, float* out_pos, float* out_dir, float* out_dt,
+                                 float* out_m_norm, float* out_dt_norm, hipStream_t stream) {
+    TN_CHECK_ARG(ray_img_id && ori && dir && ray_t && ray_offset && ray_alive && n_ray_steps && out_img_id &&
+                 out_t && out_pos && out_dir && out_dt && out_m_norm && out_dt_norm && grid.bitfield);
+    TN_CHECK_ARG(n_rays_max <= stride && grid.n_levels >= 1 && grid.n_levels <= 8 && dt_min > 0.0f);
+    if (n_rays_max == 0) return TN_OK;
+    march_generate_kernel<<<blocks_for(n_rays_max, 64), 64, 0, stream>>>(
+        n_rays_max, info, stride, bbox, inv_aabb, grid, dt_min, dt_max, cone_angle, ray_img_id, ori, dir, ray_t,
+        ray_offset, ray_alive, n_ray_steps, out_img_id, out_t, out_pos, out_dir, out_dt, out_m_norm, out_dt_norm);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+extern "C" int tn_estimate_rays(const uint32_t* info, uint32_t max_samples, uint32_t cap, uint32_t min_rays,
+                                uint32_t* n_rays_next, hipStream_t stream) {
+    TN_CHECK_ARG(info && n_rays_next && cap > 0);
+    estimate_rays_kernel<<<1, 1, 0, stream>>>(info, max_samples, cap, min_rays, n_rays_next);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}

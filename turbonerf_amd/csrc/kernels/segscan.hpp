@@ -1,0 +1,213 @@
+// Wave64 machinery for per-ray work over compacted samples.
+//
+// Samples of ray r occupy [offset[r], offset[r] + steps[r]) and rays are laid
+// out back to back. A wave owns the rays whose first sample falls in its
+// window [w*S, (w+1)*S) (the last wave also owns any zero-length rays at the
+// end) and walks their samples in chunks of 64, one sample per lane. Inside a
+// chunk, per-ray prefix sums are segmented inclusive scans keyed by ray
+// index; a ray that continues into the next chunk carries its running sums.
+// Sums in fp32 are compensated (TwoSum, Knuth TAOCP vol. 2 4.2.2) so a
+// 1024-sample ray accumulates O(log n) rather than O(n) rounding error.
+#pragma once
+
+#include "common.hpp"
+
+namespace tn {
+
+// ---------------------------------------------------------------------------
+// compensated fp32 pair
+// ---------------------------------------------------------------------------
+struct CSum {
+    float hi, lo;
+};
+
+TN_DEVICE CSum csum_add(CSum a, CSum b) {
+    const float s = a.hi + b.hi;
+    const float bp = s - a.hi;
+    const float e = (a.hi - (s - bp)) + (b.hi - bp);
+    CSum r;
+    r.hi = s;
+    r.lo = a.lo + b.lo + e;
+    return r;
+}
+
+TN_DEVICE float csum_val(CSum a) { return a.hi + a.lo; }
+
+// Segmented inclusive scan across the wave, segments keyed by `key`
+// (lanes with equal key are contiguous).
+TN_DEVICE CSum seg_scan(CSum v, int key) {
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float oh = __shfl_up(v.hi, d, 64);
+        const float ol = __shfl_up(v.lo, d, 64);
+        const int ok = __shfl_up(key, d, 64);
+        if (lane >= d && ok == key) v = csum_add(v, CSum{oh, ol});
+    }
+    return v;
+}
+
+TN_DEVICE double seg_scan(double v, int key) {
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double o = __shfl_up(v, d, 64);
+        const int ok = __shfl_up(key, d, 64);
+        if (lane >= d && ok == key) v += o;
+    }
+    return v;
+}
+
+// Segmented inclusive scan from the high lanes down (suffix sums).
+TN_DEVICE CSum seg_scan_rev(CSum v, int key) {
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float oh = __shfl_down(v.hi, d, 64);
+        const float ol = __shfl_down(v.lo, d, 64);
+        const int ok = __shfl_down(key, d, 64);
+        if (lane + d < 64 && ok == key) v = csum_add(v, CSum{oh, ol});
+    }
+    return v;
+}
+
+TN_DEVICE double seg_scan_rev(double v, int key) {
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double o = __shfl_down(v, d, 64);
+        const int ok = __shfl_down(key, d, 64);
+        if (lane + d < 64 && ok == key) v += o;
+    }
+    return v;
+}
+
+// accumulator abstraction: fp32 compensated or plain fp64
+template <typename A>
+struct Acc;
+
+template <>
+struct Acc<float> {
+    CSum v;
+    TN_DEVICE static Acc zero() { return Acc{CSum{0.0f, 0.0f}}; }
+    TN_DEVICE static Acc of(float x) { return Acc{CSum{x, 0.0f}}; }
+    TN_DEVICE float val() const { return csum_val(v); }
+    TN_DEVICE Acc operator+(const Acc& o) const { return Acc{csum_add(v, o.v)}; }
+    TN_DEVICE Acc scan(int key) const { return Acc{seg_scan(v, key)}; }
+    TN_DEVICE Acc scan_rev(int key) const { return Acc{seg_scan_rev(v, key)}; }
+    TN_DEVICE Acc bcast(int src) const {
+        return Acc{CSum{__shfl(v.hi, src, 64), __shfl(v.lo, src, 64)}};
+    }
+};
+
+template <>
+struct Acc<double> {
+    double v;
+    TN_DEVICE static Acc zero() { return Acc{0.0}; }
+    TN_DEVICE static Acc of(double x) { return Acc{x}; }
+    TN_DEVICE double val() const { return v; }
+    TN_DEVICE Acc operator+(const Acc& o) const { return Acc{v + o.v}; }
+    TN_DEVICE Acc scan(int key) const { return Acc{seg_scan(v, key)}; }
+    TN_DEVICE Acc scan_rev(int key) const { return Acc{seg_scan_rev(v, key)}; }
+    TN_DEVICE Acc bcast(int src) const { return Acc{__shfl(v, src, 64)}; }
+};
+
+template <typename R>
+TN_DEVICE R r_exp(R x);
+template <>
+TN_DEVICE float r_exp<float>(float x) { return __expf(x); }
+template <>
+TN_DEVICE double r_exp<double>(double x) { return exp(x); }
+
+// ---------------------------------------------------------------------------
+// first index r in [0, n) with offs[r] >= x (n if none), 64-ary search
+// ---------------------------------------------------------------------------
+TN_DEVICE uint32_t wave_lower_bound(const uint32_t* __restrict__ offs, uint32_t n, uint32_t x) {
+    const uint32_t lane = __lane_id();
+    uint32_t lo = 0, hi = n;  // answer in [lo, hi]
+    while (lo < hi) {
+        const uint32_t span = hi - lo;
+        const uint32_t step = (span + 63) / 64;
+        const uint32_t p = lo + lane * step;
+        const bool pred = p < hi && offs[p] >= x;
+        const bool inrange = p < hi;
+        const uint64_t ball = __ballot(pred);
+        const uint64_t inr = __ballot(inrange);
+        if (ball == 0) {
+            const uint32_t last = 63u - (uint32_t)__builtin_clzll(inr);
+            lo = lo + last * step + 1;
+        } else {
+            const uint32_t f = (uint32_t)__builtin_ctzll(ball);
+            if (f == 0) return lo;
+            hi = lo + f * step;
+            lo = lo + (f - 1) * step + 1;
+        }
+    }
+    return lo;
+}
+
+// Ray range and sample window of wave `w` with window S.
+struct WaveRange {
+    uint32_t r0, r1, s0, s1;
+    bool active;
+};
+
+TN_DEVICE WaveRange wave_range(uint32_t w, uint32_t S, uint32_t n_rays, uint32_t n_samples,
+                               const uint32_t* __restrict__ offs) {
+    WaveRange wr;
+    const uint32_t n_waves = n_samples == 0 ? 1 : (n_samples + S - 1) / S;
+    wr.active = w < n_waves && n_rays > 0;
+    if (!wr.active) return wr;
+    wr.r0 = w == 0 ? 0 : wave_lower_bound(offs, n_rays, w * S);
+    wr.r1 = (w + 1 == n_waves) ? n_rays : wave_lower_bound(offs, n_rays, (w + 1) * S);
+    wr.s0 = wr.r0 < n_rays ? min(offs[wr.r0], n_samples) : n_samples;
+    wr.s1 = wr.r1 < n_rays ? min(offs[wr.r1], n_samples) : n_samples;
+    return wr;
+}
+
+// ---------------------------------------------------------------------------
+// Ray of each lane in the chunk [c, c + 64). Rays with index >= *r_scan start
+// at or after c. Heads of rays with samples are scattered into `slot` (one
+// 64-entry LDS row owned by this wave); zero-length rays met on the way are
+// reported through `on_empty(ray)` (called by the lane that holds the ray).
+// Returns the ray index of this lane's sample, or carry_ray for lanes before
+// the first head.
+// ---------------------------------------------------------------------------
+template <typename OnEmpty>
+TN_DEVICE int chunk_rays(uint32_t c, uint32_t r1, uint32_t& r_scan, const uint32_t* __restrict__ offs,
+                         const uint32_t* __restrict__ steps, int* slot, int carry_ray, OnEmpty&& on_empty) {
+    const uint32_t lane = __lane_id();
+    slot[lane] = -1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    while (true) {
+        const uint32_t k = r_scan + lane;
+        bool inchunk = false;
+        if (k < r1) {
+            const uint32_t o = offs[k];
+            inchunk = o < c + 64u;
+            if (inchunk) {
+                if (steps[k] > 0) slot[o - c] = (int)k;
+                else on_empty(k);
+            }
+        }
+        const uint64_t ball = __ballot(inchunk);
+        const uint32_t cnt = (uint32_t)__builtin_popcountll(ball);
+        r_scan += cnt;
+        if (cnt < 64) break;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    int head = slot[lane];
+    // propagate the most recent head to the following lanes (max-scan; ray
+    // indices increase along the chunk)
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(head, d, 64);
+        if ((int)lane >= d) head = max(head, o);
+    }
+    __builtin_amdgcn_wave_barrier();
+    return head < 0 ? carry_ray : head;
+}
+
+}  // namespace tn

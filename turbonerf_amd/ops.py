@@ -1,0 +1,194 @@
+"""Thin torch-facing wrappers of the ``tn_*`` C ABI.
+
+Every function takes CUDA (HIP) torch tensors, launches on torch's current
+stream and returns new or updated tensors. These are the building blocks the
+parity tests and the torch.autograd.Function wrappers (``autograd.py``) use;
+the C++ Trainer/Renderer calls the same C ABI directly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import BBox, OccupancyGrid, call, ptr, query, stream_ptr
+
+FULL = 0xFFFFFFFF
+
+
+def _dev(t):
+    if t is not None and not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    return t
+
+
+def upload_struct(s) -> torch.Tensor:
+    raw = np.frombuffer(C.string_at(C.addressof(s), C.sizeof(s)), np.uint8).copy()
+    return torch.from_numpy(raw).cuda()
+
+
+def fill_uniform(n: int, seed: int, offset: int = 0, out: torch.Tensor | None = None) -> torch.Tensor:
+    out = torch.empty(n, dtype=torch.float32, device="cuda") if out is None else out
+    call("tn_fill_uniform", n, seed, offset, ptr(out), stream_ptr())
+    return out
+
+
+def grid_struct(bitfield: torch.Tensor, density: torch.Tensor | None, n_levels: int, res: int = 128) -> OccupancyGrid:
+    return OccupancyGrid(ptr(bitfield), ptr(density) if density is not None else None, n_levels, res)
+
+
+@dataclass
+class Rays:
+    """Per-ray SoA buffers with component stride `stride`."""
+
+    stride: int
+    pix_rgba: torch.Tensor
+    img_id: torch.Tensor
+    ori: torch.Tensor
+    dir: torch.Tensor
+    t: torch.Tensor
+    tmax: torch.Tensor
+    alive: torch.Tensor
+    n_steps: torch.Tensor
+    offset: torch.Tensor
+
+    @classmethod
+    def empty(cls, stride: int) -> "Rays":
+        f = lambda n: torch.zeros(n, dtype=torch.float32, device="cuda")
+        u = lambda n: torch.zeros(n, dtype=torch.int32, device="cuda")
+        return cls(stride, f(4 * stride), u(stride), f(3 * stride), f(3 * stride), f(stride), f(stride),
+                   torch.zeros(stride, dtype=torch.uint8, device="cuda"), u(stride), u(stride))
+
+
+@dataclass
+class Samples:
+    """Per-sample SoA buffers (stride = sample capacity)."""
+
+    stride: int
+    img_id: torch.Tensor
+    t: torch.Tensor
+    pos: torch.Tensor
+    dir: torch.Tensor
+    dt: torch.Tensor
+    m_norm: torch.Tensor
+    dt_norm: torch.Tensor
+
+    @classmethod
+    def empty(cls, stride: int) -> "Samples":
+        f = lambda n: torch.zeros(n, dtype=torch.float32, device="cuda")
+        return cls(stride, torch.zeros(stride, dtype=torch.int32, device="cuda"), f(stride), f(3 * stride),
+                   f(3 * stride), f(stride), f(stride), f(stride))
+
+
+def init_training_rays(rays: Rays, n_rays: int, n_images: int, img_w: int, img_h: int, n_rays_per_image: float,
+                       bbox_dev: torch.Tensor, cams_dev: torch.Tensor, images_dev: torch.Tensor,
+                       random: torch.Tensor, d_n_rays: torch.Tensor | None = None) -> None:
+    call("tn_init_training_rays", n_rays, ptr(d_n_rays), rays.stride, n_images, img_w * img_h, img_w, img_h,
+         n_rays_per_image, ptr(bbox_dev), ptr(cams_dev), ptr(images_dev), ptr(random), ptr(rays.pix_rgba),
+         ptr(rays.img_id), ptr(rays.ori), ptr(rays.dir), ptr(rays.t), ptr(rays.tmax), ptr(rays.alive),
+         stream_ptr())
+
+
+def march_count(rays: Rays, n_rays: int, bbox: BBox, grid: OccupancyGrid, cone: float, dt_min: float,
+                dt_max: float, max_steps: int = FULL, d_n_rays: torch.Tensor | None = None) -> None:
+    call("tn_march_count", n_rays, ptr(d_n_rays), rays.stride, bbox, grid, cone, dt_min, dt_max, max_steps,
+         ptr(rays.dir), ptr(rays.tmax), ptr(rays.alive), ptr(rays.ori), ptr(rays.t), ptr(rays.n_steps),
+         stream_ptr())
+
+
+def scan_select(rays: Rays, n_rays: int, max_samples: int, info: torch.Tensor, mode: int = 0) -> None:
+    nb = query("tn_scan_workspace_bytes", n_rays)
+    ws = torch.empty(max(nb, 4), dtype=torch.uint8, device="cuda")
+    call("tn_scan_select", mode, n_rays, ptr(rays.n_steps), max_samples, ptr(rays.offset), ptr(info), ptr(ws), nb,
+         stream_ptr())
+
+
+def march_generate(rays: Rays, samples: Samples, n_rays_max: int, info: torch.Tensor, bbox: BBox, inv_aabb: float,
+                   grid: OccupancyGrid, dt_min: float, dt_max: float, cone: float) -> None:
+    if n_rays_max > rays.stride:
+        raise ValueError("n_rays_max exceeds the ray capacity")
+    call("tn_march_generate", n_rays_max, ptr(info), rays.stride, bbox, inv_aabb, grid, dt_min, dt_max, cone,
+         ptr(rays.img_id), ptr(rays.ori), ptr(rays.dir), ptr(rays.t), ptr(rays.offset), ptr(rays.alive),
+         ptr(rays.n_steps), ptr(samples.img_id), ptr(samples.t), ptr(samples.pos), ptr(samples.dir),
+         ptr(samples.dt), ptr(samples.m_norm), ptr(samples.dt_norm), stream_ptr())
+
+
+_DTYPE_CODE = {torch.float32: 0, torch.float16: 1}
+
+
+def composite_train(info: torch.Tensor, n_rays_max: int, stride: int, steps: torch.Tensor, offs: torch.Tensor,
+                    bg: torch.Tensor, target: torch.Tensor, density: torch.Tensor, color: torch.Tensor,
+                    dt: torch.Tensor, grad_scale: float = 1.0, min_T: float = 1e-4, fp64_accumulate: bool = False):
+    """Returns (loss[4*stride], dL_ddensity[stride], dL_dcolor[3*stride])."""
+    code = 2 if fp64_accumulate else _DTYPE_CODE[density.dtype]
+    if fp64_accumulate and density.dtype != torch.float32:
+        raise ValueError("fp64 accumulation takes fp32 inputs")
+    loss = torch.zeros(4 * stride, dtype=torch.float32, device="cuda")
+    dLd = torch.zeros(stride, dtype=torch.float32, device="cuda")
+    dLc = torch.zeros(3 * stride, dtype=density.dtype, device="cuda")
+    nb = query("tn_composite_scratch_bytes", n_rays_max)
+    scratch = torch.empty(nb // 4, dtype=torch.float32, device="cuda")
+    call("tn_composite_train", code, ptr(info), n_rays_max, stride, grad_scale, min_T, ptr(steps), ptr(offs),
+         ptr(bg), ptr(target), ptr(density), ptr(color), ptr(dt), ptr(loss), ptr(dLd), ptr(dLc), ptr(scratch), nb,
+         stream_ptr())
+    return loss, dLd, dLc
+
+
+def composite_forward(info: torch.Tensor, stride: int, max_samples: int, n_rays_out: int, steps: torch.Tensor,
+                      offs: torch.Tensor, density: torch.Tensor, color: torch.Tensor, dt: torch.Tensor,
+                      raw_density: bool = True, min_T: float = 1e-4, fp64_accumulate: bool = False):
+    code = 2 if fp64_accumulate else _DTYPE_CODE[density.dtype]
+    rgba = torch.zeros(4 * n_rays_out, dtype=torch.float32, device="cuda")
+    call("tn_composite_forward", code, ptr(info), stride, max_samples, n_rays_out, min_T, int(raw_density),
+         ptr(steps), ptr(offs), ptr(density), ptr(color), ptr(dt), ptr(rgba), stream_ptr())
+    return rgba
+
+
+def reduce_loss(info: torch.Tensor, stride: int, loss: torch.Tensor, n_channels: int = 4) -> torch.Tensor:
+    out = torch.zeros(1, dtype=torch.float32, device="cuda")
+    call("tn_reduce_loss", ptr(info), stride, n_channels, ptr(loss), ptr(out), stream_ptr())
+    return out
+
+
+def blob_density_grid(grid: OccupancyGrid, blobs: torch.Tensor) -> None:
+    call("tn_blob_density_grid", grid, ptr(blobs), blobs.numel() // 8, stream_ptr())
+
+
+def blob_eval(info, n_max: int, stride: int, aabb: float, blobs: torch.Tensor, pos: torch.Tensor):
+    sigma = torch.zeros(stride, dtype=torch.float32, device="cuda")
+    rgb = torch.zeros(3 * stride, dtype=torch.float32, device="cuda")
+    call("tn_blob_eval", ptr(info), n_max, stride, aabb, ptr(blobs), blobs.numel() // 8, ptr(pos), ptr(sigma),
+         ptr(rgb), stream_ptr())
+    return sigma, rgb
+
+
+def occ_decay(grid: OccupancyGrid, factor: float) -> None:
+    call("tn_occ_decay", grid, factor, stream_ptr())
+
+
+def occ_update_bits(grid: OccupancyGrid, threshold: float) -> None:
+    call("tn_occ_update_bits", grid, threshold, stream_ptr())
+
+
+def occ_sample_points(n_cells: int, stride: int, start: int, grid: OccupancyGrid, level: int, inv_aabb: float,
+                      random: torch.Tensor, pos: torch.Tensor) -> None:
+    call("tn_occ_sample_points", n_cells, stride, start, grid, level, inv_aabb, ptr(random), ptr(pos), stream_ptr())
+
+
+def occ_update_density(n_cells: int, start: int, grid: OccupancyGrid, level: int, sample_all: bool,
+                       random: torch.Tensor, raw: torch.Tensor) -> None:
+    code = _DTYPE_CODE[raw.dtype]
+    call("tn_occ_update_density", code, n_cells, start, grid, level, int(sample_all), ptr(random), ptr(raw),
+         stream_ptr())
+
+
+def popcount(b: torch.Tensor) -> int:
+    n = b.numel()
+    nb = query("tn_popcount_workspace_bytes", n)
+    ws = torch.empty(max(nb, 4), dtype=torch.uint8, device="cuda")
+    out = torch.zeros(1, dtype=torch.int32, device="cuda")
+    call("tn_popcount", ptr(b), n, ptr(out), ptr(ws), nb, stream_ptr())
+    return int(out.item())
