@@ -78,11 +78,12 @@ int tn_scan_select(int mode, uint32_t n_rays, const uint32_t* n_steps, uint32_t 
     uint32_t* ray_offset, uint32_t* info, void* workspace, size_t workspace_bytes,
     tn_stream_t stream);
 
-/* Re-march and write compacted per-sample SoA buffers. Replaces
+/* Re-march and write compacted per-sample SoA buffers (stride sample_stride;
+ * ray buffers have stride `stride`). Replaces
  * march_and_generate_network_positions_kernel (reference
  * src/utils/training-batch-kernels.cuh:339-481). */
 int tn_march_generate(uint32_t n_rays_max, const uint32_t* info, uint32_t stride,
-    tn_bbox_t bbox, float inv_aabb, tn_occupancy_grid_t grid, float dt_min, float dt_max,
+    uint32_t sample_stride, tn_bbox_t bbox, float inv_aabb, tn_occupancy_grid_t grid, float dt_min, float dt_max,
     float cone_angle, const uint32_t* ray_img_id, const float* ori, const float* dir,
     const float* ray_t, const uint32_t* ray_offset, const uint8_t* ray_alive,
     const uint32_t* n_ray_steps, uint32_t* out_img_id, float* out_t, float* out_pos,
@@ -100,15 +101,16 @@ int tn_estimate_rays(const uint32_t* info, uint32_t max_samples, uint32_t cap, u
  * (reference src/utils/training-kernels-fused.cuh:381-498,
  * src/core/nerf-network.cu:547-603). dtype 0: fp32 density/colour and fp32
  * colour grads; 1: fp16 in / fp16 colour grads; 2: fp32 with fp64
- * accumulation (oracle mode). density = raw network output s. Outputs:
- * loss[4][stride] per ray, dL_ddensity[stride] = dL/ds, dL_dcolor[3][stride]
- * scaled by grad_scale. */
+ * accumulation (oracle mode). density = raw network output s. Per-ray
+ * buffers (bg_rgb[3], target_rgba[4], loss[4]) have stride `stride`; per-sample
+ * buffers (density, color[3], dt, dL_ddensity, dL_dcolor[3]) have stride
+ * `sample_stride`. dL_ddensity = dL/ds, dL_dcolor scaled by grad_scale. */
 size_t tn_composite_scratch_bytes(uint32_t n_rays_max);
 int tn_composite_train(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t stride,
-    float grad_scale, float min_transmittance, const uint32_t* ray_steps,
+    uint32_t sample_stride, float grad_scale, float min_transmittance, const uint32_t* ray_steps,
     const uint32_t* ray_offset, const float* bg_rgb, const float* target_rgba,
     const void* density, const void* color, const float* dt, float* loss,
-    float* dL_ddensity, void* dL_dcolor, float* ray_scratch, size_t scratch_bytes,
+    float* dL_ddensity, void* dL_dcolor, void* ray_scratch, size_t scratch_bytes,
     tn_stream_t stream);
 
 /* Forward-only compositing to premultiplied rgba[4][out_stride] per ray
@@ -154,6 +156,69 @@ int tn_occ_update_bits(tn_occupancy_grid_t grid, float threshold, tn_stream_t st
 size_t tn_popcount_workspace_bytes(uint64_t n_bytes);
 int tn_popcount(const uint8_t* bytes, uint64_t n_bytes, uint32_t* out, void* workspace,
     size_t workspace_bytes, tn_stream_t stream);
+
+/* ---------------------------------------------------- hash-grid encoding */
+
+/* Level table of the multiresolution hash grid (reference HashGrid config,
+ * src/core/nerf-network.cu:86-104; tiny-cuda-nn's layout is not vendored, the
+ * convention is documented in hashgrid.hip). */
+int tn_hashgrid_layout(int n_levels, int n_features, int log2_hashmap_size, int base_resolution,
+    double per_level_scale, tn_hashgrid_t* out);
+
+/* Encode pos[3][stride] in [0,1] into features[2L][out_stride]. table_dtype /
+ * feat_dtype: 0 = fp32, 1 = fp16. Replaces the HashGrid forward inside the
+ * density network (reference src/core/nerf-network.cu:461). info (optional):
+ * device batch descriptor, n = min(info[1], n_max). */
+int tn_hashgrid_forward(int table_dtype, int feat_dtype, uint32_t n_max, const uint32_t* info,
+    uint32_t stride, uint32_t out_stride, tn_hashgrid_t grid, const float* pos,
+    const void* table, void* features, tn_stream_t stream);
+
+/* Scatter-add of feature gradients into the fp32 table gradient. Replaces the
+ * HashGrid backward (reference src/core/nerf-network.cu:760). */
+int tn_hashgrid_backward(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
+    uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos, const void* dfeat, float* dtable,
+    tn_stream_t stream);
+
+/* ------------------------------------------------------------------- MLPs */
+
+/* Density MLP (+ SH-4 direction encoding + colour MLP when color != 0) on
+ * MFMA. precision 0 = fp32, 1 = fp16 with fp32 accumulation. params: network
+ * parameters in that precision starting at element net_offset (see
+ * tn_types.h). Replaces the FullyFusedMLP/SphericalHarmonics forward and
+ * inference (reference src/core/nerf-network.cu:319-545). */
+int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* info, uint32_t stride,
+    uint32_t feat_stride, uint32_t out_stride, const void* params, uint32_t net_offset,
+    const void* features, const float* dir, int color, void* density_out, void* rgb_out,
+    tn_stream_t stream);
+
+/* Backward of both MLPs with the reference's gradient plumbing (colour-MLP
+ * backward, density row 0 += loss_scale*(dL_ds + dL_ds_extra), t^2 scaling for
+ * t < 1, density-MLP backward; reference src/core/nerf-network.cu:668-768,
+ * src/utils/training-kernels-fused.cuh:500-515). Writes dfeat and stores
+ * (accumulate=0) or adds the 9408 weight gradients into grad_net (fp32). */
+size_t tn_mlp_backward_workspace_bytes(uint32_t n_max);
+int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* info, uint32_t stride,
+    uint32_t feat_stride, const void* params, uint32_t net_offset, const void* features,
+    const float* dir, const float* sample_t, const void* dL_drgb, uint32_t grad_stride,
+    const float* dL_ds, const float* dL_ds_extra, float loss_scale, void* dfeat, float* grad_net,
+    int accumulate, void* workspace, size_t workspace_bytes, tn_stream_t stream);
+
+/* Dynamic LDS bytes the MLP kernels launch with (for occupancy planning). */
+size_t tn_mlp_lds_bytes(int precision, int backward);
+
+/* ------------------------------------------------------------------ Adam */
+
+/* NGP Adam over all parameters (reference ngp_adam_step,
+ * src/core/adam-optimizer.cuh:48-96). Gradients are fp32 and are zeroed as
+ * they are consumed. lp_dtype 0: no low-precision copy; 1: fp16 copy. */
+int tn_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float loss_scale,
+    float learning_rate, float beta1, float beta2, float epsilon, float l2_reg, float* weights,
+    int lp_dtype, void* weights_lp, float* gradients, float* first_moments, float* second_moments,
+    uint32_t* param_steps, tn_stream_t stream);
+
+/* fp32 -> fp16 parameter copy (reference copy_and_cast,
+ * src/utils/parallel-utils.cuh:60-82). */
+int tn_cast_params(uint64_t n, const float* src, int lp_dtype, void* dst, tn_stream_t stream);
 
 /* ------------------------------------------- synthetic analytic field */
 

@@ -49,11 +49,11 @@ def lib():
             "oc_march_count": (None, [u32, u32, C.c_void_p, u8p, i32, i32, f32, f32, f32, f32p, f32p, u8p, f32p,
                                       f32p, u32p, u32]),
             "oc_scan_select": (i32, [u32, u32p, u32, u32p, C.POINTER(u32), C.POINTER(u32)]),
-            "oc_march_generate": (None, [u32, u32, C.c_void_p, f32, u8p, i32, i32, f32, f32, f32, u32p, f32p, f32p,
+            "oc_march_generate": (None, [u32, u32, u32, C.c_void_p, f32, u8p, i32, i32, f32, f32, f32, u32p, f32p, f32p,
                                          f32p, u32p, u8p, u32p, u32p, f32p, f32p, f32p, f32p, f32p, f32p]),
-            "oc_composite_train_f32": (None, [u32, u32, f32, f32, u32p, u32p, f32p, f32p, f32p, f32p, f32p, f32,
+            "oc_composite_train_f32": (None, [u32, u32, u32, f32, f32, u32p, u32p, f32p, f32p, f32p, f32p, f32p, f32,
                                               f32p, f32p, f32p]),
-            "oc_composite_train_f64": (None, [u32, u32, f64, f64, u32p, u32p, f64p, f64p, f64p, f64p, f64p, f64,
+            "oc_composite_train_f64": (None, [u32, u32, u32, f64, f64, u32p, u32p, f64p, f64p, f64p, f64p, f64p, f64,
                                               f64p, f64p, f64p]),
             "oc_distortion_f32": (None, [u32, f32, u32p, u32p, f32p, f32p, f32p, f32p, f32p]),
             "oc_distortion_f64": (None, [u32, f64, u32p, u32p, f64p, f64p, f64p, f64p, f64p]),
@@ -138,27 +138,30 @@ def scan_select(n_steps, n_rays, max_samples):
     return offs, int(nr.value), int(ns.value), int(rc)
 
 
-def march_generate(n_rays, stride, bbox, inv_aabb, bitfield, n_levels, res, dt_min, dt_max, cone, rays, offs,
-                   alive, n_steps):
-    o = dict(img_id=np.zeros(stride, np.uint32), t=np.zeros(stride, np.float32),
-             pos=np.zeros(3 * stride, np.float32), dir=np.zeros(3 * stride, np.float32),
-             dt=np.zeros(stride, np.float32), m_norm=np.zeros(stride, np.float32),
-             dt_norm=np.zeros(stride, np.float32))
-    lib().oc_march_generate(n_rays, stride, C.addressof(bbox), inv_aabb, c(bitfield, np.uint8), n_levels, res,
+def march_generate(n_rays, stride, sample_stride, bbox, inv_aabb, bitfield, n_levels, res, dt_min, dt_max, cone,
+                   rays, offs, alive, n_steps):
+    """`stride` is the ray buffers' stride, `sample_stride` the sample buffers'."""
+    S = sample_stride
+    o = dict(img_id=np.zeros(S, np.uint32), t=np.zeros(S, np.float32), pos=np.zeros(3 * S, np.float32),
+             dir=np.zeros(3 * S, np.float32), dt=np.zeros(S, np.float32), m_norm=np.zeros(S, np.float32),
+             dt_norm=np.zeros(S, np.float32))
+    lib().oc_march_generate(n_rays, stride, S, C.addressof(bbox), inv_aabb, c(bitfield, np.uint8), n_levels, res,
                             dt_min, dt_max, cone, rays["img_id"], rays["ori"], rays["dir"], rays["t"],
                             c(offs, np.uint32), c(alive, np.uint8), c(n_steps, np.uint32), o["img_id"], o["t"],
                             o["pos"], o["dir"], o["dt"], o["m_norm"], o["dt_norm"])
     return o
 
 
-def composite_train(n_rays, stride, steps, offs, bg, target, density, color, dt, grad_scale=1.0, min_T=1e-4,
-                    fp64=True):
+def composite_train(n_rays, stride, sample_stride, steps, offs, bg, target, density, color, dt, grad_scale=1.0,
+                    min_T=1e-4, fp64=True):
+    """Per-ray arrays (bg, target, loss) use `stride`; per-sample arrays use
+    `sample_stride`."""
     dt_ = np.float64 if fp64 else np.float32
     loss = np.zeros(4 * stride, dt_)
-    dLd = np.zeros(stride, dt_)
-    dLc = np.zeros(3 * stride, dt_)
+    dLd = np.zeros(sample_stride, dt_)
+    dLc = np.zeros(3 * sample_stride, dt_)
     fn = lib().oc_composite_train_f64 if fp64 else lib().oc_composite_train_f32
-    fn(n_rays, stride, 1.0 / n_rays, grad_scale, c(steps, np.uint32), c(offs, np.uint32), c(bg, dt_),
+    fn(n_rays, stride, sample_stride, 1.0 / n_rays, grad_scale, c(steps, np.uint32), c(offs, np.uint32), c(bg, dt_),
        c(target, dt_), c(density, dt_), c(color, dt_), c(dt, dt_), min_T, loss, dLd, dLc)
     return loss, dLd, dLc
 

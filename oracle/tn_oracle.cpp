@@ -135,10 +135,9 @@ extern "C" int oc_init_training_rays(uint32_t n_rays, uint32_t B, uint32_t n_ima
     const float* random, float* pix_rgba, uint32_t* ray_img_id, float* ori, float* dir,
     float* ray_t, float* ray_tmax, uint8_t* ray_alive) {
     (void)img_h;
-    for (uint32_t i = 0; i < n_rays; ++i) {
+    for (uint32_t c = 0; c < n_images; ++c)
         for (int k = 0; k < 5; ++k)
-            if (cams[0].dist_params[k] != 0.0f) return TN_ERR_UNSUPPORTED;
-    }
+            if (cams[c].dist_params[k] != 0.0f) return TN_ERR_UNSUPPORTED;
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)n_rays; ++ii) {
         const uint32_t i = (uint32_t)ii;
@@ -305,19 +304,19 @@ extern "C" int oc_scan_select(uint32_t n_rays, const uint32_t* n_steps, uint32_t
 }
 
 /* training-batch-kernels.cuh:305-331 */
-static inline void assign_normalized(uint32_t B, uint32_t sid, float x, float y, float z, float dx, float dy,
+static inline void assign_normalized(uint32_t SB, uint32_t sid, float x, float y, float z, float dx, float dy,
     float dz, float dt, float inv_aabb, float* pos, float* dir, float* odt) {
     odt[sid] = inv_aabb * dt;
     pos[sid] = std::min(std::max(x * inv_aabb + 0.5f, 0.0f), 1.0f);
-    pos[sid + B] = std::min(std::max(y * inv_aabb + 0.5f, 0.0f), 1.0f);
-    pos[sid + 2 * B] = std::min(std::max(z * inv_aabb + 0.5f, 0.0f), 1.0f);
+    pos[sid + SB] = std::min(std::max(y * inv_aabb + 0.5f, 0.0f), 1.0f);
+    pos[sid + 2 * SB] = std::min(std::max(z * inv_aabb + 0.5f, 0.0f), 1.0f);
     dir[sid] = 0.5f * dx + 0.5f;
-    dir[sid + B] = 0.5f * dy + 0.5f;
-    dir[sid + 2 * B] = 0.5f * dz + 0.5f;
+    dir[sid + SB] = 0.5f * dy + 0.5f;
+    dir[sid + 2 * SB] = 0.5f * dz + 0.5f;
 }
 
 /* training-batch-kernels.cuh:339-481 */
-extern "C" void oc_march_generate(uint32_t n_rays, uint32_t B, const tn_bbox_t* bbox, float inv_aabb,
+extern "C" void oc_march_generate(uint32_t n_rays, uint32_t B, uint32_t SB, const tn_bbox_t* bbox, float inv_aabb,
     const uint8_t* bits, int n_levels, int res, float dt_min, float dt_max, float cone,
     const uint32_t* ray_img_id, const float* ori, const float* dir, const float* ray_t,
     const uint32_t* ray_offset, const uint8_t* ray_alive, const uint32_t* n_ray_steps,
@@ -343,7 +342,7 @@ extern "C" void oc_march_generate(uint32_t n_rays, uint32_t B, const tn_bbox_t* 
             const uint32_t sid = off + n;
             if (!bbox_contains(*bbox, x, y, z)) {  // :422-439 (Q20: unreachable)
                 out_t[sid] = t; out_m_norm[sid] = t + 0.5f * dt; out_dt_norm[sid] = dt; out_img_id[sid] = img;
-                assign_normalized(B, sid, x, y, z, dx, dy, dz, dt, inv_aabb, out_pos, out_dir, out_dt);
+                assign_normalized(SB, sid, x, y, z, dx, dy, dz, dt, inv_aabb, out_pos, out_dir, out_dt);
                 ++n;
                 continue;
             }
@@ -352,7 +351,7 @@ extern "C" void oc_march_generate(uint32_t n_rays, uint32_t B, const tn_bbox_t* 
                 out_t[sid] = t; out_m_norm[sid] = t + 0.5f * dt; out_dt_norm[sid] = dt; out_img_id[sid] = img;
                 t += dt;
                 t_max = t;
-                assign_normalized(B, sid, x, y, z, dx, dy, dz, dt, inv_aabb, out_pos, out_dir, out_dt);
+                assign_normalized(SB, sid, x, y, z, dx, dy, dz, dt, inv_aabb, out_pos, out_dir, out_dt);
                 ++n;
             } else {
                 t += oc_dt_to_next_voxel(res, x, y, z, dx, dy, dz, idx, idy, idz, dt_min, lvl);
@@ -376,7 +375,7 @@ template <typename R> static inline R r_exp(R x) { return std::exp(x); }
 template <> inline float r_exp<float>(float x) { return expf(x); }
 
 template <typename R>
-static void composite_train(uint32_t n_rays, uint32_t B, R inv_n_rays, R grad_scale,
+static void composite_train(uint32_t n_rays, uint32_t B, uint32_t SB, R inv_n_rays, R grad_scale,
     const uint32_t* steps, const uint32_t* offs, const R* bg, const R* target, const R* density,
     const R* color, const R* dtb, R min_T, R* loss, R* dL_ddensity, R* dL_dcolor) {
 #pragma omp parallel for schedule(dynamic, 64)
@@ -384,7 +383,7 @@ static void composite_train(uint32_t n_rays, uint32_t B, R inv_n_rays, R grad_sc
         const uint32_t i = (uint32_t)ii;
         const uint32_t n = steps[i];
         const uint32_t o = offs[i];
-        const R* sr = color + o; const R* sg = color + o + B; const R* sb = color + o + 2 * B;
+        const R* sr = color + o; const R* sg = color + o + SB; const R* sb = color + o + 2 * (size_t)SB;
         const R* sd = density + o; const R* sdt = dtb + o;
         // density_to_ray_rgba_forward (:227-264)
         R r = 0, g = 0, b = 0, a = 0, T = 1;
@@ -427,27 +426,27 @@ static void composite_train(uint32_t n_rays, uint32_t B, R inv_n_rays, R grad_sc
             dL_ddensity[o + k] = dL_dsigma * sigma;
             const R dRdc = w * a;
             dL_dcolor[o + k] = grad_scale * dLr * dRdc;
-            dL_dcolor[o + k + B] = grad_scale * dLg * dRdc;
-            dL_dcolor[o + k + 2 * B] = grad_scale * dLb * dRdc;
+            dL_dcolor[o + k + SB] = grad_scale * dLg * dRdc;
+            dL_dcolor[o + k + 2 * (size_t)SB] = grad_scale * dLb * dRdc;
             T *= (R)1 - alpha;
             if (T < min_T) break;
         }
     }
 }
 
-extern "C" void oc_composite_train_f32(uint32_t n_rays, uint32_t B, float inv_n_rays, float grad_scale,
+extern "C" void oc_composite_train_f32(uint32_t n_rays, uint32_t B, uint32_t SB, float inv_n_rays, float grad_scale,
     const uint32_t* steps, const uint32_t* offs, const float* bg, const float* target,
     const float* density, const float* color, const float* dt, float min_T, float* loss,
     float* dL_ddensity, float* dL_dcolor) {
-    composite_train<float>(n_rays, B, inv_n_rays, grad_scale, steps, offs, bg, target, density, color,
+    composite_train<float>(n_rays, B, SB, inv_n_rays, grad_scale, steps, offs, bg, target, density, color,
         dt, min_T, loss, dL_ddensity, dL_dcolor);
 }
 
-extern "C" void oc_composite_train_f64(uint32_t n_rays, uint32_t B, double inv_n_rays, double grad_scale,
+extern "C" void oc_composite_train_f64(uint32_t n_rays, uint32_t B, uint32_t SB, double inv_n_rays, double grad_scale,
     const uint32_t* steps, const uint32_t* offs, const double* bg, const double* target,
     const double* density, const double* color, const double* dt, double min_T, double* loss,
     double* dL_ddensity, double* dL_dcolor) {
-    composite_train<double>(n_rays, B, inv_n_rays, grad_scale, steps, offs, bg, target, density, color,
+    composite_train<double>(n_rays, B, SB, inv_n_rays, grad_scale, steps, offs, bg, target, density, color,
         dt, min_T, loss, dL_ddensity, dL_dcolor);
 }
 

@@ -49,7 +49,7 @@ void oc_march_count(uint32_t n_rays, uint32_t stride, const tn_bbox_t* bbox,
 int oc_scan_select(uint32_t n_rays, const uint32_t* n_steps, uint32_t max_samples,
     uint32_t* ray_offset, uint32_t* out_n_rays, uint32_t* out_n_samples);
 
-void oc_march_generate(uint32_t n_rays, uint32_t stride, const tn_bbox_t* bbox, float inv_aabb,
+void oc_march_generate(uint32_t n_rays, uint32_t stride, uint32_t sample_stride, const tn_bbox_t* bbox, float inv_aabb,
     const uint8_t* bitfield, int n_levels, int resolution, float dt_min, float dt_max,
     float cone_angle, const uint32_t* ray_img_id, const float* ori, const float* dir,
     const float* ray_t, const uint32_t* ray_offset, const uint8_t* ray_alive,
@@ -57,11 +57,11 @@ void oc_march_generate(uint32_t n_rays, uint32_t stride, const tn_bbox_t* bbox, 
     float* out_dir, float* out_dt, float* out_m_norm, float* out_dt_norm);
 
 /* --- compositing + loss + backward ----------------------------------- */
-void oc_composite_train_f32(uint32_t n_rays, uint32_t stride, float inv_n_rays, float grad_scale,
+void oc_composite_train_f32(uint32_t n_rays, uint32_t stride, uint32_t sample_stride, float inv_n_rays, float grad_scale,
     const uint32_t* ray_steps, const uint32_t* ray_offset, const float* bg_rgb,
     const float* target_rgba, const float* density, const float* color, const float* dt,
     float min_transmittance, float* loss, float* dL_ddensity, float* dL_dcolor);
-void oc_composite_train_f64(uint32_t n_rays, uint32_t stride, double inv_n_rays, double grad_scale,
+void oc_composite_train_f64(uint32_t n_rays, uint32_t stride, uint32_t sample_stride, double inv_n_rays, double grad_scale,
     const uint32_t* ray_steps, const uint32_t* ray_offset, const double* bg_rgb,
     const double* target_rgba, const double* density, const double* color, const double* dt,
     double min_transmittance, double* loss, double* dL_ddensity, double* dL_dcolor);

@@ -1,0 +1,81 @@
+"""fp64 numpy restatement of the density/colour MLPs and their backward pass
+with the reference's density-gradient injection (src/core/nerf-network.cu:
+668-768). Used as the checker of the MFMA kernels; its backward is itself
+checked against torch autograd in tests/test_oracle_network.py."""
+import numpy as np
+
+SH_C = [0.28209479177387814, 0.48860251190291987, 1.0925484305920792, 0.94617469575755997, 0.31539156525251999,
+        0.54627421529603959, 0.59004358992664352, 2.8906114426405538, 0.45704579946446572, 0.3731763325901154,
+        1.4453057213202769]
+
+
+def sh4(dir01):
+    """dir01: (n, 3) in [0,1] (0.5*d + 0.5) -> (n, 16)."""
+    d = 2.0 * np.asarray(dir01, np.float64) - 1.0
+    x, y, z = d[:, 0], d[:, 1], d[:, 2]
+    xx, yy, zz = x * x, y * y, z * z
+    c = SH_C
+    return np.stack([
+        np.full_like(x, c[0]), -c[1] * y, c[1] * z, -c[1] * x, c[2] * x * y, -c[2] * y * z, c[3] * zz - c[4],
+        -c[2] * x * z, c[5] * (xx - yy), c[6] * y * (-3 * xx + yy), c[7] * x * y * z, c[8] * y * (1 - 5 * zz),
+        c[9] * z * (5 * zz - 3), c[8] * x * (1 - 5 * zz), 1.4453057213202769 * z * (xx - yy),
+        c[6] * x * (-xx + 3 * yy)], axis=1)
+
+
+def split(net):
+    net = np.asarray(net, np.float64)
+    W1 = net[0:2048].reshape(64, 32)
+    W2 = net[2048:3072].reshape(16, 64)
+    W3 = net[3072:5120].reshape(64, 32)
+    W4 = net[5120:9216].reshape(64, 64)
+    W5 = net[9216:9408].reshape(3, 64)
+    return W1, W2, W3, W4, W5
+
+
+def forward(net, feat, dir01):
+    W1, W2, W3, W4, W5 = split(net)
+    feat = np.asarray(feat, np.float64)
+    h1 = np.maximum(feat @ W1.T, 0)
+    s16 = h1 @ W2.T
+    xc = np.concatenate([s16, sh4(dir01)], axis=1)
+    h2 = np.maximum(xc @ W3.T, 0)
+    h3 = np.maximum(h2 @ W4.T, 0)
+    o = h3 @ W5.T
+    rgb = 1.0 / (1.0 + np.exp(-o))
+    return dict(h1=h1, s16=s16, xc=xc, h2=h2, h3=h3, o=o, rgb=rgb, feat=feat)
+
+
+def backward(net, fw, G_rgb, dL_ds, t, loss_scale, extra=None):
+    """Returns (dfeat (n,32), grad_net (9408,))."""
+    W1, W2, W3, W4, W5 = split(net)
+    rgb = fw["rgb"]
+    dO = G_rgb * rgb * (1 - rgb)
+    dW5 = dO.T @ fw["h3"]
+    dH3 = (dO @ W5) * (fw["h3"] > 0)
+    dW4 = dH3.T @ fw["h2"]
+    dH2 = (dH3 @ W4) * (fw["h2"] > 0)
+    dW3 = dH2.T @ fw["xc"]
+    dXc = dH2 @ W3
+    dS = dXc[:, :16].copy()
+    g0 = dS[:, 0] + loss_scale * np.asarray(dL_ds, np.float64)
+    if extra is not None:
+        g0 += loss_scale * np.asarray(extra, np.float64)
+    t = np.asarray(t, np.float64)
+    dS[:, 0] = np.where(t < 1.0, g0 * t * t, g0)
+    dW2 = dS.T @ fw["h1"]
+    dH1 = (dS @ W2) * (fw["h1"] > 0)
+    dW1 = dH1.T @ fw["feat"]
+    dX = dH1 @ W1
+    grad = np.concatenate([dW1.ravel(), dW2.ravel(), dW3.ravel(), dW4.ravel(), dW5.ravel()])
+    return dX, grad
+
+
+def xavier(rng, n_out, n_in):
+    a = np.sqrt(6.0 / (n_in + n_out))
+    return rng.uniform(-a, a, size=(n_out, n_in))
+
+
+def random_net(seed=0):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([xavier(rng, 64, 32).ravel(), xavier(rng, 16, 64).ravel(), xavier(rng, 64, 32).ravel(),
+                           xavier(rng, 64, 64).ravel(), xavier(rng, 3, 64).ravel()])

@@ -30,21 +30,16 @@ def norm_rel(a, b):
 
 
 def oracle_f64(b, s=None, color=None):
+    """Per-ray arrays have stride n_rays, per-sample arrays stride ns."""
     import oracle
 
-    # stride for per-ray arrays is n_rays, for per-sample arrays ns: the oracle
-    # uses one stride for both, so per-ray arrays are padded to ns
-    B = b["ns"]
-    nr = b["n_rays"]
-    bg = np.zeros(3 * B); target = np.zeros(4 * B)
-    for k in range(3): bg[k * B:k * B + nr] = b["bg"][k * nr:(k + 1) * nr]
-    for k in range(4): target[k * B:k * B + nr] = b["target"][k * nr:(k + 1) * nr]
+    nr, ns = b["n_rays"], b["ns"]
     s = b["s"] if s is None else s
     color = b["color"] if color is None else color
-    loss, dLd, dLc = oracle.composite_train(nr, B, b["steps"], b["offs"], bg, target, s.astype(np.float64),
+    loss, dLd, dLc = oracle.composite_train(nr, nr, ns, b["steps"], b["offs"], b["bg"].astype(np.float64),
+                                            b["target"].astype(np.float64), s.astype(np.float64),
                                             color.astype(np.float64), b["dt"].astype(np.float64), 1.0, 1e-4, True)
-    loss = loss.reshape(4, B)[:, :nr]
-    return loss, dLd, dLc.reshape(3, B)
+    return loss.reshape(4, nr), dLd, dLc.reshape(3, ns)
 
 
 def run_gpu(b, dtype, fp64_accumulate=False):
@@ -52,21 +47,16 @@ def run_gpu(b, dtype, fp64_accumulate=False):
 
     from turbonerf_amd import ops
 
-    B = b["ns"]
-    nr = b["n_rays"]
-    # per-ray arrays use stride B as in the reference (one stride for rays and samples)
-    bg = np.zeros(3 * B, np.float32); target = np.zeros(4 * B, np.float32)
-    for k in range(3): bg[k * B:k * B + nr] = b["bg"][k * nr:(k + 1) * nr]
-    for k in range(4): target[k * B:k * B + nr] = b["target"][k * nr:(k + 1) * nr]
+    ns, nr = b["ns"], b["n_rays"]
     T = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
-    info = torch.tensor([nr, B, 0, 0], dtype=torch.int32, device="cuda")
-    loss, dLd, dLc = ops.composite_train(info, nr, B, T(b["steps"].view(np.int32), torch.int32),
-                                         T(b["offs"].view(np.int32), torch.int32), T(bg), T(target),
+    info = torch.tensor([nr, ns, 0, 0], dtype=torch.int32, device="cuda")
+    loss, dLd, dLc = ops.composite_train(info, nr, nr, ns, T(b["steps"].view(np.int32), torch.int32),
+                                         T(b["offs"].view(np.int32), torch.int32), T(b["bg"]), T(b["target"]),
                                          T(b["s"], dtype), T(b["color"], dtype), T(b["dt"]), 1.0, 1e-4,
                                          fp64_accumulate)
     torch.cuda.synchronize()
-    return (loss.cpu().numpy().reshape(4, B)[:, :nr], dLd.cpu().numpy().astype(np.float64),
-            dLc.float().cpu().numpy().reshape(3, B).astype(np.float64))
+    return (loss.cpu().numpy().reshape(4, nr), dLd.cpu().numpy().astype(np.float64),
+            dLc.float().cpu().numpy().reshape(3, ns).astype(np.float64))
 
 
 def test_composite_fp32_vs_oracle(cuda):

@@ -116,7 +116,7 @@ def test_c0_pipeline_parity(scene):
     # generate
     samples = ops.Samples.empty(S_CAP)
     ops.march_generate(rays, samples, N_RAYS, info, scene["bbox"], 1.0, grid, DT_MIN, DT_MIN, CONE)
-    so = oracle.march_generate(N_RAYS, S_CAP, scene["bbox"], 1.0, bits_h, 1, 128, DT_MIN, DT_MIN, CONE, rr, excl,
+    so = oracle.march_generate(N_RAYS, N_RAYS, S_CAP, scene["bbox"], 1.0, bits_h, 1, 128, DT_MIN, DT_MIN, CONE, rr, excl,
                                alive_ref, n_ref)
     ns = int(n_ref.sum())
     for k in ("t", "dt", "m_norm", "dt_norm"):

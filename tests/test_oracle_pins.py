@@ -1,0 +1,438 @@
+"""Pins the oracle (oracle/tn_oracle.cpp) to facts independent of its own
+formulas: Random123 known-answer vectors, textbook bit interleaving, exact
+rational arithmetic, brute-force marching with numpy float32, closed forms of
+volume rendering, FP64 finite differences, the O(N^2) mip-NeRF 360 distortion
+sum, the Adam closed form, partition of unity / adjointness of the hash grid,
+SH orthonormality by exact quadrature. The reference ships no tests or golden
+vectors for this path, so these checks are what pin each oracle function."""
+import ctypes
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+
+F32 = np.float32
+
+
+# --------------------------------------------------------------- Philox
+def _philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    oracle.lib().oc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32 with 10 rounds
+    assert _philox([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert _philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert _philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_uniform_stream_definition():
+    seed = 0x1234567890AB
+    u = oracle.uniform(37, seed, offset=5)
+    for j in (0, 1, 2, 3, 17, 36):
+        g = 5 + j
+        words = _philox([(g >> 2) & 0xFFFFFFFF, g >> 34, 0, 0], [seed & 0xFFFFFFFF, seed >> 32])
+        assert u[j] == F32((words[g & 3] >> 8) * 2.0 ** -24)
+    assert u.min() >= 0 and u.max() < 1
+
+
+# --------------------------------------------------------------- Morton
+def test_morton_textbook():
+    rng = np.random.default_rng(0)
+    for x, y, z in rng.integers(0, 1024, size=(200, 3)):
+        m = 0
+        for b in range(10):
+            m |= ((int(x) >> b) & 1) << (3 * b) | ((int(y) >> b) & 1) << (3 * b + 1) | ((int(z) >> b) & 1) << (3 * b + 2)
+        assert oracle.morton3d(int(x), int(y), int(z)) == m
+        assert oracle.morton3d_invert(m) == x and oracle.morton3d_invert(m >> 1) == y and oracle.morton3d_invert(m >> 2) == z
+
+
+# --------------------------------------------------------------- directed division
+def test_fdiv_ru_is_upward_rounding():
+    rng = np.random.default_rng(1)
+    dt = F32(math.sqrt(3) / 1024)
+    vals = np.concatenate([rng.uniform(1e-3, 0.2, 300).astype(F32), [dt, 2 * dt, F32(3) * dt, F32(0.0625)]])
+    for a in vals:
+        for b in (dt, F32(0.0078125), F32(0.3)):
+            q = F32(oracle.fdiv_ru(float(a), float(b)))
+            exact = Fraction(float(a)) / Fraction(float(b))
+            assert Fraction(float(q)) >= exact
+            below = np.nextafter(q, F32(-np.inf))
+            assert Fraction(float(below)) < exact
+
+
+# --------------------------------------------------------------- occupancy queries
+def test_grid_level_and_index_bruteforce():
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        n_levels = int(rng.integers(1, 6))
+        p = rng.uniform(-2 ** (n_levels - 1), 2 ** (n_levels - 1), 3).astype(F32)
+        dt = F32(rng.uniform(0.001, 0.05))
+        lvl = None
+        for k in range(n_levels):
+            half = F32(0.5) * F32(2 ** k)
+            if all(abs(c) < half for c in p) and F32(2 ** k) * F32(1 / 128) > dt:
+                lvl = k
+                break
+        lvl = n_levels - 1 if lvl is None else lvl
+        assert oracle.lib().oc_grid_level_at(n_levels, 128, *map(float, p), float(dt)) == lvl
+        s = F32(1.0) / F32(2 ** lvl)
+        idx = [min(int(max((c * s + F32(0.5)) * F32(128), F32(0))), 127) for c in p]
+        assert oracle.lib().oc_voxel_morton_index(128, lvl, *map(float, p)) == oracle.morton3d(*idx)
+
+
+def _rays(n, seed):
+    from turbonerf_amd import scenes
+
+    cams = scenes.sphere_cameras(seed, 4, 2.0, 64, 64, 50.0)
+    images = np.zeros((4, 64, 64, 4), np.uint8)
+    rnd = oracle.uniform(n, seed)
+    return oracle.init_training_rays(n, n, 4, 64, 64, n / 4, scenes.bbox(1.0), cams, images, rnd)
+
+
+def test_march_count_all_occupied_bruteforce():
+    """All cells occupied, aabb = 1 (constant dt): the count is the number of
+    float32 steps t += dt taken while t < tmax and the point stays in the box."""
+    from turbonerf_amd import scenes
+
+    n = 256
+    rr = _rays(n, 3)
+    bits = np.full(128 ** 3, 0xFF, np.uint8)
+    dt = F32(math.sqrt(3) / 1024)
+    steps, alive = oracle.march_count(n, n, scenes.bbox(1.0), bits, 1, 128, 1 / 256, float(dt), float(dt), rr)
+    for i in range(n):
+        if not rr["alive"][i]:
+            assert steps[i] == 0
+            continue
+        o = rr["ori"].reshape(3, n)[:, i]
+        d = rr["dir"].reshape(3, n)[:, i]
+        t, k = rr["t"][i], 0
+        while t < rr["tmax"][i]:
+            x = o + t * d
+            if np.any(x < F32(-0.5)) or np.any(x > F32(0.5)):
+                break
+            t = F32(t + dt)
+            k += 1
+        assert steps[i] == k
+    bits0 = np.zeros(128 ** 3, np.uint8)
+    s0, a0 = oracle.march_count(n, n, scenes.bbox(1.0), bits0, 1, 128, 1 / 256, float(dt), float(dt), rr)
+    assert s0.sum() == 0 and a0.sum() == 0
+
+
+def test_scan_select_rule():
+    rng = np.random.default_rng(4)
+    steps = rng.integers(0, 50, 1000).astype(np.uint32)
+    budget = 9000
+    offs, nr, ns, rc = oracle.scan_select(steps, 1000, budget)
+    excl = np.concatenate([[0], np.cumsum(steps)[:-1]])
+    assert np.array_equal(offs, excl)
+    j = np.nonzero(excl < budget)[0].max()
+    assert (nr, ns, rc) == (j, excl[j], 0)
+    assert ns < budget
+
+
+def test_march_generate_invariants():
+    from turbonerf_amd import scenes
+
+    n = 512
+    rr = _rays(n, 5)
+    blobs = scenes.blob_field(5, 8, 1.0)
+    ii = np.arange(128 ** 3, dtype=np.uint32)
+    inv = lambda m: sum(((m >> (3 * b)) & 1) << b for b in range(10))
+    xyz = (np.stack([inv(ii), inv(ii >> 1), inv(ii >> 2)], 1).astype(F32) - 64 + 0.5) / 128
+    s, _ = oracle.blob_eval(xyz, blobs)
+    bits = oracle.occ_update_bits(s, 128 ** 3, 1, 0.01)
+    dt = float(F32(math.sqrt(3) / 1024))
+    steps, alive = oracle.march_count(n, n, scenes.bbox(1.0), bits, 1, 128, 1 / 256, dt, dt, rr)
+    offs = np.concatenate([[0], np.cumsum(steps)[:-1]]).astype(np.uint32)
+    S = int(steps.sum())
+    so = oracle.march_generate(n, n, S, scenes.bbox(1.0), 1.0, bits, 1, 128, dt, dt, 1 / 256, rr, offs, alive, steps)
+    pos = so["pos"].reshape(3, S)
+    for i in np.nonzero(steps)[0]:
+        sl = slice(offs[i], offs[i] + steps[i])
+        t = so["t"][sl]
+        assert np.all(np.diff(t) > 0)
+        m = so["m_norm"][sl]
+        assert m.min() >= 0 and m.max() <= 1 and np.all(np.diff(m) >= 0)
+        # the span runs from the ray's entry t (before any empty-space skip)
+        span = max(float(F32(t[-1] + F32(dt)) - rr["t"][i]), 1e-6)
+        np.testing.assert_allclose(so["dt_norm"][sl] * span, dt, rtol=1e-5)
+        # every sample lies in an occupied cell
+        o = rr["ori"].reshape(3, n)[:, i]
+        d = rr["dir"].reshape(3, n)[:, i]
+        for tt in t[:: max(1, len(t) // 5)]:
+            x = o + tt * d
+            assert bits[oracle.lib().oc_voxel_morton_index(128, 0, *map(float, x))] & 1
+        np.testing.assert_allclose(pos[:, sl[0] if False else offs[i]], np.clip(o + t[0] * d + 0.5, 0, 1), atol=1e-6)
+
+
+# --------------------------------------------------------------- compositing
+def _comp(steps, s, c, dt, bg, target, min_T=1e-4):
+    n_rays = len(steps)
+    offs = np.concatenate([[0], np.cumsum(steps)[:-1]]).astype(np.uint32)
+    ns = int(np.sum(steps))
+    return oracle.composite_train(n_rays, n_rays, ns, np.asarray(steps, np.uint32), offs, bg, target, s, c, dt,
+                                  1.0, min_T, True)
+
+
+def test_composite_closed_forms():
+    # one sample, alpha = 1 - exp(-sigma dt), R = alpha c, A = alpha; loss is
+    # smooth-L1 of R*A + bg(1-A) - gt_composite
+    sigma, dt, c = 2.0, 0.3, 0.7
+    s = np.array([math.log(sigma) + 1.0])
+    alpha = 1 - math.exp(-sigma * dt)
+    bg = np.array([0.2, 0.2, 0.2])
+    target = np.array([0.5, 0.5, 0.5, 1.0])
+    loss, _, _ = _comp([1], s, np.array([c, c, c]), np.array([dt]), bg, target)
+    R = alpha * c
+    pred = R * alpha + 0.2 * (1 - alpha)
+    assert abs(loss[0] - 0.5 * (pred - 0.5) ** 2) < 1e-15
+    assert abs(loss[3] - 0.5 * ((1 - alpha) - 0.0) ** 2) < 1e-15
+    # constant sigma, no early stop: sum w + T_final = 1 and T_n = exp(-sigma sum dt)
+    n = 20
+    s = np.full(n, math.log(0.5) + 1.0)
+    dts = np.full(n, 0.1)
+    col = np.zeros(3 * n)
+    loss, _, _ = _comp([n], s, col, dts, np.zeros(3), np.array([0, 0, 0, 0.0]), min_T=0.0)
+    A = 1 - math.exp(-0.5 * 0.1 * n)
+    # alpha channel loss = smoothL1((1-A) - 1)
+    assert abs(loss[3] - 0.5 * A * A) < 1e-14
+
+
+def test_composite_early_termination_index():
+    # opaque samples: T drops below 1e-4 after k samples; later samples get no gradient
+    s = np.full(10, math.log(10.0) + 1.0)
+    dts = np.full(10, 0.3)  # T *= exp(-3) = 0.0498 per sample
+    _, dLd, dLc = _comp([10], s, np.full(30, 0.5), dts, np.zeros(3), np.array([1, 1, 1, 1.0]))
+    T, k = 1.0, 0
+    while True:
+        T *= math.exp(-3.0)
+        k += 1
+        if T < 1e-4:
+            break
+    assert np.all(dLd[:k] != 0) and np.all(dLd[k:] == 0)
+    assert np.all(dLc[k:10] == 0)
+
+
+def test_composite_backward_finite_differences():
+    rng = np.random.default_rng(6)
+    steps = np.clip(rng.geometric(1 / 12, 8), 1, 64)
+    ns = int(steps.sum())
+    s = np.log(rng.lognormal(0, 1.5, ns)) + 1.0
+    c = rng.uniform(0, 1, 3 * ns)
+    dt = rng.uniform(0.02, 0.2, ns)
+    bg = rng.uniform(0, 1, 3 * 8)
+    tg = rng.uniform(0, 1, 4 * 8)
+    _, dLd, dLc = _comp(steps, s, c, dt, bg, tg)
+    h = 1e-6
+
+    def L(s_, c_):
+        loss, _, _ = _comp(steps, s_, c_, dt, bg, tg)
+        return loss.reshape(4, 8)[:3].sum() / 8
+
+    for i in rng.choice(ns, 25, replace=False):
+        e = np.zeros(ns); e[i] = h
+        fd = (L(s + e, c) - L(s - e, c)) / (2 * h)
+        assert abs(fd - dLd[i]) <= 1e-6 * max(1e-3, abs(fd))
+        e3 = np.zeros(3 * ns); e3[i] = h
+        fdc = (L(s, c + e3) - L(s, c - e3)) / (2 * h)
+        assert abs(fdc - dLc[i]) <= 1e-6 * max(1e-3, abs(fdc))
+
+
+# --------------------------------------------------------------- distortion
+def test_distortion_linear_equals_quadratic_and_fd():
+    rng = np.random.default_rng(7)
+    for n in (1, 2, 5, 33, 64):
+        d = np.log(rng.lognormal(0, 1.5, n)) + 1.0
+        dt = rng.uniform(0.5, 1.5, n)
+        t = np.concatenate([[0], np.cumsum(dt)])
+        span = t[-1]
+        m = ((t[:-1] + 0.5 * dt) / span)
+        dtn = dt / span
+        loss, grad = oracle.distortion(1, np.array([n], np.uint32), np.array([0], np.uint32), d, m, dtn, lam=0.01)
+        bf = oracle.distortion_bruteforce(d, m, dtn, lam=0.01)
+        assert abs(loss[0] - bf) <= 1e-12 * max(1.0, abs(bf))
+        h = 1e-6
+        for i in range(min(n, 8)):
+            e = np.zeros(n); e[i] = h
+            fd = (oracle.distortion_bruteforce(d + e, m, dtn, 0.01) - oracle.distortion_bruteforce(d - e, m, dtn, 0.01)) / (2 * h)
+            assert abs(fd - grad[i]) <= 1e-6 * max(1e-6, abs(fd)) + 1e-12
+
+
+# --------------------------------------------------------------- Adam
+def test_adam_closed_form():
+    n = 8
+    w = np.linspace(-1, 1, n).astype(F32)
+    g = np.array([0, 2048, -1024, 0, 512, 0, 256, -128], F32)  # loss scale 1024
+    m = np.zeros(n, F32); v = np.zeros(n, F32); st = np.zeros(n, np.uint32)
+    n_grid, n_net_end = 4, 7
+    lr, b1, b2, eps, l2 = 1e-2, 0.9, 0.99, 1e-15, 1e-6
+    w2, wl, m2, v2, st2 = oracle.adam_step(n_grid, n_net_end, 1024.0, lr, b1, b2, eps, l2, w, g, m, v, st)
+    for i in range(n):
+        gi = float(g[i]) / 1024
+        if i < n_grid and gi == 0:
+            assert w2[i] == w[i] and st2[i] == 0 and m2[i] == 0
+            continue
+        if n_grid <= i < n_net_end:
+            gi += l2 * float(w[i])
+        mi = (1 - b1) * gi
+        vi = (1 - b2) * gi * gi
+        step = lr * math.sqrt(1 - b2) / (1 - b1) * mi / (math.sqrt(vi) + eps)
+        assert st2[i] == 1
+        assert abs(float(w2[i]) - (float(w[i]) - step)) <= 1e-6
+    assert w2[5] != w[5]  # network weight with zero grad still steps (L2 term)
+
+
+# --------------------------------------------------------------- occupancy bits
+def test_occupancy_bits_and_popcount():
+    rng = np.random.default_rng(8)
+    vol, L = 4096, 5
+    dens = rng.uniform(0, 0.02, L * vol).astype(F32)
+    bits = oracle.occ_update_bits(dens, vol, L, 0.01)
+    expect = np.zeros(vol, np.uint8)
+    for l in range(L):
+        expect |= ((dens[l * vol:(l + 1) * vol] > F32(0.01)).astype(np.uint8) << l)
+    assert np.array_equal(bits, expect)
+    assert oracle.popcount(bits) == int(np.unpackbits(bits).sum())
+    d2 = oracle.occ_decay(dens, L, 0.95)
+    assert np.array_equal(d2, dens * F32(0.95))
+
+
+# --------------------------------------------------------------- hash grid
+def _layout():
+    from turbonerf_amd import _capi
+
+    g = _capi.HashGrid()
+    b = math.exp((math.log(2048) - math.log(16)) / 15)
+    oracle.lib().oc_hashgrid_layout(16, 2, 19, 16, b, ctypes.addressof(g))
+    return g
+
+
+def test_hashgrid_partition_of_unity_and_adjoint():
+    g = _layout()
+    rng = np.random.default_rng(9)
+    n = 300
+    pos = rng.uniform(0, 1, 3 * n).astype(F32)
+    ones = np.ones(g.n_params)
+    feat = np.zeros(32 * n)
+    oracle.lib().oc_hashgrid_fwd_f64(ctypes.addressof(g), n, n, pos, ones, feat)
+    np.testing.assert_allclose(feat, 1.0, atol=1e-12)
+    table = rng.normal(size=g.n_params)
+    oracle.lib().oc_hashgrid_fwd_f64(ctypes.addressof(g), n, n, pos, table, feat)
+    G = rng.normal(size=32 * n)
+    dt = np.zeros(g.n_params)
+    oracle.lib().oc_hashgrid_bwd_f64(ctypes.addressof(g), n, n, pos, G, dt)
+    assert abs(float(feat @ G) - float(table @ dt)) <= 1e-9 * abs(float(feat @ G))
+
+
+def test_hashgrid_index_formula_bruteforce():
+    g = _layout()
+    # level 0 is dense: a point inside cell (i,j,k) with frac 0 reads entry i + j r + k r^2
+    r = g.resolution[0]
+    table = np.zeros(g.n_params)
+    i, j, k = 3, 5, 7
+    e = i + j * r + k * r * r
+    table[2 * e] = 1.0
+    s = g.scale[0]
+    p = np.array([(i - 0.5) / s, (j - 0.5) / s, (k - 0.5) / s], F32)
+    feat = np.zeros(32)
+    oracle.lib().oc_hashgrid_fwd_f64(ctypes.addressof(g), 1, 1, p, table, feat)
+    x = p * F32(s) + F32(0.5)
+    w = np.prod([1 - (xx - np.floor(xx)) for xx in x])
+    assert abs(feat[0] - w) < 1e-12
+    # hashed level: the XOR-of-primes formula
+    l = 10
+    assert not g.level_dense[l]
+    cell = (100, 200, 300)
+    h = (cell[0] * 1) ^ ((cell[1] * 2654435761) & 0xFFFFFFFF) ^ ((cell[2] * 805459861) & 0xFFFFFFFF)
+    ent = g.level_offset[l] + (h & 0xFFFFFFFF) % g.level_size[l]
+    table = np.zeros(g.n_params)
+    table[2 * ent + 1] = 1.0
+    sl = g.scale[l]
+    p = ((np.array(cell, np.float64) - 0.5 + 1e-3) / sl).astype(F32)
+    feat = np.zeros(32)
+    oracle.lib().oc_hashgrid_fwd_f64(ctypes.addressof(g), 1, 1, p, table, feat)
+    assert feat[2 * l + 1] > 0.9
+
+
+# --------------------------------------------------------------- spherical harmonics
+def test_sh_orthonormal_quadrature():
+    # Gauss-Legendre in cos(theta) x uniform phi integrates products of two
+    # degree-4 harmonics (degree 8 polynomials) exactly
+    xs, ws = np.polynomial.legendre.leggauss(10)
+    phis = np.arange(20) * 2 * np.pi / 20
+    z = np.repeat(xs, 20)
+    wq = np.repeat(ws, 20) * (2 * np.pi / 20)
+    ph = np.tile(phis, 10)
+    r = np.sqrt(1 - z * z)
+    d = np.stack([r * np.cos(ph), r * np.sin(ph), z], 1)
+    dir01 = (0.5 * d + 0.5).astype(F32)
+    n = len(z)
+    out = np.zeros(16 * n)
+    oracle.lib().oc_sh4_f64(n, n, np.ascontiguousarray(dir01.T.ravel()), out)
+    Y = out.reshape(16, n)
+    gram = (Y * wq) @ Y.T
+    np.testing.assert_allclose(gram, np.eye(16), atol=2e-6)  # inputs are fp32 directions
+
+
+# --------------------------------------------------------------- MLP reference
+def test_numpy_mlp_backward_matches_autograd():
+    import torch
+
+    from nets import backward, forward, random_net, sh4
+
+    rng = np.random.default_rng(11)
+    n = 40
+    net = random_net(3)
+    feat = rng.normal(size=(n, 32))
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    dir01 = 0.5 * d + 0.5
+    G = rng.normal(size=(n, 3))
+    dls = rng.normal(size=n)
+    t = rng.uniform(0.2, 2.0, n)
+    fw = forward(net, feat, dir01)
+    dX, gr = backward(net, fw, G, dls, t, 7.0)
+    W = torch.tensor(net, dtype=torch.float64, requires_grad=True)
+    X = torch.tensor(feat, dtype=torch.float64, requires_grad=True)
+    W1, W2, W3, W4, W5 = W[:2048].view(64, 32), W[2048:3072].view(16, 64), W[3072:5120].view(64, 32), \
+        W[5120:9216].view(64, 64), W[9216:].view(3, 64)
+    h1 = torch.relu(X @ W1.T)
+    s16 = h1 @ W2.T
+    # the t^2 factor scales the gradient reaching s16[:, 0]; express it as a
+    # hook on that column
+    scale = torch.tensor(np.where(t < 1, t * t, 1.0))
+    s16 = s16.clone()
+    col0 = s16[:, 0]
+    s16h = torch.cat([(col0 * 1.0).unsqueeze(1), s16[:, 1:]], 1)
+    s16h.register_hook(lambda gr_: torch.cat([(gr_[:, 0] * scale).unsqueeze(1), gr_[:, 1:]], 1))
+    xc = torch.cat([s16h, torch.tensor(sh4(dir01))], 1)
+    rgb = torch.sigmoid(torch.relu(torch.relu(xc @ W3.T) @ W4.T) @ W5.T)
+    total = (rgb * torch.tensor(G)).sum() + 7.0 * (s16h[:, 0] * torch.tensor(dls)).sum()
+    total.backward()
+    np.testing.assert_allclose(gr, W.grad.numpy(), rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(dX, X.grad.numpy(), rtol=1e-10, atol=1e-12)
+
+
+# --------------------------------------------------------------- C ABI surface
+def test_c_abi_exports_every_declared_symbol():
+    import re
+    from pathlib import Path
+
+    from turbonerf_amd import _capi
+
+    hdr = (Path(__file__).resolve().parent.parent / "include" / "turbonerf" / "tn_c_api.h").read_text()
+    declared = set(re.findall(r"^(?:int|size_t)\s+(tn_\w+)\(", hdr, re.M))
+    assert len(declared) >= 20
+    lib = ctypes.CDLL(str(_capi.LIB_PATH))
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing, missing
+    # and the Python binding covers all of them
+    assert declared <= set(_capi.declared_symbols()), declared - set(_capi.declared_symbols())

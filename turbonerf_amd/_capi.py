@@ -101,11 +101,11 @@ _SIGS: dict[str, tuple] = {
                                  vp, vp]),
     "tn_scan_workspace_bytes": (sz, [u32]),
     "tn_scan_select": (C.c_int, [C.c_int, u32, vp, u32, vp, vp, vp, sz, vp]),
-    "tn_march_generate": (C.c_int, [u32, vp, u32, BBox, f32, OccupancyGrid, f32, f32, f32, vp, vp, vp, vp, vp,
+    "tn_march_generate": (C.c_int, [u32, vp, u32, u32, BBox, f32, OccupancyGrid, f32, f32, f32, vp, vp, vp, vp, vp,
                                     vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "tn_estimate_rays": (C.c_int, [vp, u32, u32, u32, vp, vp]),
     "tn_composite_scratch_bytes": (sz, [u32]),
-    "tn_composite_train": (C.c_int, [C.c_int, vp, u32, u32, f32, f32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+    "tn_composite_train": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, f32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                      vp, sz, vp]),
     "tn_composite_forward": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, C.c_int, vp, vp, vp, vp, vp, vp, vp]),
     "tn_reduce_loss": (C.c_int, [vp, u32, u32, vp, vp, vp]),
@@ -117,6 +117,16 @@ _SIGS: dict[str, tuple] = {
     "tn_popcount": (C.c_int, [vp, u64, vp, vp, sz, vp]),
     "tn_blob_density_grid": (C.c_int, [OccupancyGrid, vp, u32, vp]),
     "tn_blob_eval": (C.c_int, [vp, u32, u32, f32, vp, u32, vp, vp, vp, vp]),
+    "tn_hashgrid_layout": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, vp]),
+    "tn_hashgrid_forward": (C.c_int, [C.c_int, C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp]),
+    "tn_hashgrid_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp]),
+    "tn_mlp_forward": (C.c_int, [C.c_int, u32, vp, u32, u32, u32, vp, u32, vp, vp, C.c_int, vp, vp, vp]),
+    "tn_mlp_backward_workspace_bytes": (sz, [u32]),
+    "tn_mlp_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, vp, u32, vp, vp, vp, vp, u32, vp, vp, f32, vp, vp,
+                                  C.c_int, vp, sz, vp]),
+    "tn_mlp_lds_bytes": (sz, [C.c_int, C.c_int]),
+    "tn_adam_step": (C.c_int, [u32, u32, u32, f32, f32, f32, f32, f32, f32, vp, C.c_int, vp, vp, vp, vp, vp, vp]),
+    "tn_cast_params": (C.c_int, [u64, vp, C.c_int, vp, vp]),
 }
 
 # entry points added by later kernel files register their signatures here

@@ -1,0 +1,106 @@
+// NGP Adam (reference src/core/adam-optimizer.cuh:48-96): per-parameter step
+// counters, hash-table entries with a zero gradient are skipped entirely (no
+// moment decay, no step), L2 regularisation on the network weights only, fp32
+// master weights with a low-precision copy for the network kernels.
+//
+// Gradients are fp32 and are cleared as they are consumed, so the next
+// step's scatter-adds start from zero without a separate memset over the
+// whole table; entries whose gradient is zero are read once and skipped.
+// One thread handles 4 consecutive parameters (16-byte gradient loads).
+#include "common.hpp"
+
+namespace tn {
+
+template <typename TL>
+TN_DEVICE void adam_one(uint32_t i, float g_raw, uint32_t n_grid, uint32_t n_net_end, float inv_loss_scale, float lr,
+                        float beta1, float beta2, float eps, float l2, float* __restrict__ w, TL* __restrict__ w_lp,
+                        float* __restrict__ m, float* __restrict__ v, uint32_t* __restrict__ steps) {
+    float g = g_raw * inv_loss_scale;
+    if (i < n_grid && g == 0.0f) return;
+    const float wf = w[i];
+    if (i >= n_grid && i < n_net_end) g += l2 * wf;
+    const float m1 = beta1 * m[i] + (1.0f - beta1) * g;
+    const float m2 = beta2 * v[i] + (1.0f - beta2) * (g * g);
+    m[i] = m1;
+    v[i] = m2;
+    const uint32_t st = steps[i] + 1;
+    steps[i] = st;
+    const float c = sqrtf(1.0f - powf(beta2, (float)st)) / (1.0f - powf(beta1, (float)st));
+    const float nw = wf - (lr * c) / (sqrtf(m2) + eps) * m1;
+    w[i] = nw;
+    if (w_lp) w_lp[i] = (TL)nw;
+}
+
+template <typename TL>
+__global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float inv_loss_scale, float lr, float beta1,
+                            float beta2, float eps, float l2, float* __restrict__ w, TL* __restrict__ w_lp,
+                            float* __restrict__ grad, float* __restrict__ m, float* __restrict__ v,
+                            uint32_t* __restrict__ steps) {
+    const uint32_t i4 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i4 >= n) return;
+    if (i4 + 3 < n) {
+        const float4 g = *reinterpret_cast<const float4*>(grad + i4);
+        const bool any = g.x != 0.0f || g.y != 0.0f || g.z != 0.0f || g.w != 0.0f;
+        if (!any && i4 + 3 < n_grid) return;  // untouched hash entries: one 16-byte read
+        const float gv[4] = {g.x, g.y, g.z, g.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            adam_one<TL>(i4 + k, gv[k], n_grid, n_net_end, inv_loss_scale, lr, beta1, beta2, eps, l2, w, w_lp, m, v,
+                         steps);
+        if (any) *reinterpret_cast<float4*>(grad + i4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+        for (uint32_t i = i4; i < n; ++i) {
+            const float g = grad[i];
+            adam_one<TL>(i, g, n_grid, n_net_end, inv_loss_scale, lr, beta1, beta2, eps, l2, w, w_lp, m, v, steps);
+            if (g != 0.0f) grad[i] = 0.0f;
+        }
+    }
+}
+
+template <typename TL>
+__global__ void cast_kernel(uint64_t n, const float* __restrict__ src, TL* __restrict__ dst) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (TL)src[i];
+}
+
+}  // namespace tn
+
+using namespace tn;
+
+// lp_dtype: 0 = no low-precision copy (plain fp32 training), 1 = fp16 copy.
+// Parameters [0, n_grid) are hash-table entries, [n_grid, n_net_end) network
+// weights (L2-regularised), the rest (if any) neither.
+extern "C" int tn_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float loss_scale, float learning_rate,
+                            float beta1, float beta2, float epsilon, float l2_reg, float* weights, int lp_dtype,
+                            void* weights_lp, float* gradients, float* first_moments, float* second_moments,
+                            uint32_t* param_steps, hipStream_t stream) {
+    TN_CHECK_ARG(weights && gradients && first_moments && second_moments && param_steps && loss_scale > 0.0f);
+    TN_CHECK_ARG(n_grid <= n_net_end && n_net_end <= n && (lp_dtype == 0 || weights_lp));
+    if (n == 0) return TN_OK;
+    const unsigned blocks = blocks_for((n + 3) / 4, 256);
+    const float inv = 1.0f / loss_scale;
+    if (lp_dtype == 0)
+        adam_kernel<float><<<blocks, 256, 0, stream>>>(n, n_grid, n_net_end, inv, learning_rate, beta1, beta2, epsilon,
+                                                       l2_reg, weights, nullptr, gradients, first_moments,
+                                                       second_moments, param_steps);
+    else if (lp_dtype == 1)
+        adam_kernel<_Float16><<<blocks, 256, 0, stream>>>(n, n_grid, n_net_end, inv, learning_rate, beta1, beta2,
+                                                          epsilon, l2_reg, weights, static_cast<_Float16*>(weights_lp),
+                                                          gradients, first_moments, second_moments, param_steps);
+    else
+        return TN_ERR_INVALID_ARGUMENT;
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+// fp32 -> low-precision copy (reference copy_and_cast, src/utils/parallel-utils.cuh:60-82)
+extern "C" int tn_cast_params(uint64_t n, const float* src, int lp_dtype, void* dst, hipStream_t stream) {
+    TN_CHECK_ARG(src && dst);
+    if (n == 0) return TN_OK;
+    if (lp_dtype == 1)
+        cast_kernel<_Float16><<<blocks_for(n, 256), 256, 0, stream>>>(n, src, static_cast<_Float16*>(dst));
+    else
+        return TN_ERR_INVALID_ARGUMENT;
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}

@@ -38,31 +38,33 @@ TN_DEVICE float smooth_l1_grad(float x) { return fabsf(x) < 1.0f ? x : copysignf
 // per-ray scratch rows (stride = ray capacity): A, R[3], G[3]
 constexpr int kRayScratchRows = 7;
 
-// loss and dL/dRc of one ray from its totals (reference :430-477)
+// loss and dL/dRc of one ray from its totals (reference :430-477); the
+// per-ray totals and dL/dRc go to scratch in the accumulation precision R
 template <typename R>
 TN_DEVICE void ray_loss(uint32_t r, uint32_t B, R rr, R rg, R rb, R ra, const float* __restrict__ bg,
-                        const float* __restrict__ target, float inv_n_rays, float* __restrict__ loss,
-                        float* __restrict__ scratch, uint32_t SB) {
-    const float gr = target[r], gg = target[r + B], gb = target[r + 2 * B], ga = target[r + 3 * B];
-    const float br = bg[r], bgg = bg[r + B], bb = bg[r + 2 * B];
-    const float gac = 1.0f - ga;
-    const float grc = gr * ga + br * gac, ggc = gg * ga + bgg * gac, gbc = gb * ga + bb * gac;
+                        const float* __restrict__ target, R inv_n_rays, float* __restrict__ loss,
+                        R* __restrict__ scratch, uint32_t SB) {
+    const R gr = target[r], gg = target[r + B], gb = target[r + 2 * B], ga = target[r + 3 * B];
+    const R br = bg[r], bgg = bg[r + B], bb = bg[r + 2 * B];
+    const R gac = (R)1 - ga;
+    const R grc = gr * ga + br * gac, ggc = gg * ga + bgg * gac, gbc = gb * ga + bb * gac;
     const R rac = (R)1 - ra;
-    const R rrc = rr * ra + (R)br * rac, rgc = rg * ra + (R)bgg * rac, rbc = rb * ra + (R)bb * rac;
-    const float dr = (float)(rrc - (R)grc), dg = (float)(rgc - (R)ggc), db = (float)(rbc - (R)gbc);
-    const float da = (float)(rac - (R)gac);
-    loss[r] = smooth_l1(dr);
-    loss[r + B] = smooth_l1(dg);
-    loss[r + 2 * B] = smooth_l1(db);
-    loss[r + 3 * B] = smooth_l1(da);
+    const R rrc = rr * ra + br * rac, rgc = rg * ra + bgg * rac, rbc = rb * ra + bb * rac;
+    const R dr = rrc - grc, dg = rgc - ggc, db = rbc - gbc, da = rac - gac;
+    auto sl1 = [](R x) { const R ax = x < (R)0 ? -x : x; return ax < (R)1 ? (R)0.5 * x * x : ax - (R)0.5; };
+    auto sl1g = [](R x) { const R ax = x < (R)0 ? -x : x; return ax < (R)1 ? x : (x < (R)0 ? (R)-1 : (R)1); };
+    loss[r] = (float)sl1(dr);
+    loss[r + B] = (float)sl1(dg);
+    loss[r + 2 * B] = (float)sl1(db);
+    loss[r + 3 * B] = (float)sl1(da);
     if (scratch) {
-        scratch[r] = (float)ra;
-        scratch[r + SB] = (float)rr;
-        scratch[r + 2 * SB] = (float)rg;
-        scratch[r + 3 * SB] = (float)rb;
-        scratch[r + 4 * SB] = inv_n_rays * smooth_l1_grad(dr);
-        scratch[r + 5 * SB] = inv_n_rays * smooth_l1_grad(dg);
-        scratch[r + 6 * SB] = inv_n_rays * smooth_l1_grad(db);
+        scratch[r] = ra;
+        scratch[r + SB] = rr;
+        scratch[r + 2 * SB] = rg;
+        scratch[r + 3 * SB] = rb;
+        scratch[r + 4 * SB] = inv_n_rays * sl1g(dr);
+        scratch[r + 5 * SB] = inv_n_rays * sl1g(dg);
+        scratch[r + 6 * SB] = inv_n_rays * sl1g(db);
     }
 }
 
@@ -105,13 +107,13 @@ TN_DEVICE SampleState<R> sample_state(bool valid, int ray, uint64_t s, const TIn
 
 template <typename TIn, typename TGrad, typename R>
 __global__ void __launch_bounds__(64 * kCompositeWaves)
-composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t SB, uint32_t S,
+composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB, uint32_t SB, uint32_t S,
                        float grad_scale, float min_T, const uint32_t* __restrict__ steps,
                        const uint32_t* __restrict__ offs, const float* __restrict__ bg,
                        const float* __restrict__ target, const TIn* __restrict__ density,
                        const TIn* __restrict__ color, const float* __restrict__ dtb, float* __restrict__ loss,
                        float* __restrict__ dL_ddensity, TGrad* __restrict__ dL_dcolor,
-                       float* __restrict__ scratch) {
+                       R* __restrict__ scratch) {
     __shared__ int slots[kCompositeWaves][64];
     const uint32_t wid = threadIdx.x >> 6;
     const uint32_t lane = __lane_id();
@@ -119,11 +121,11 @@ composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t S
     const uint32_t n_rays = info[0], n_samples = info[1];
     const WaveRange wr = wave_range(w, S, n_rays, n_samples, offs);
     if (!wr.active) return;
-    const float inv_n_rays = 1.0f / (float)n_rays;
+    const R inv_n_rays = (R)1 / (R)n_rays;
     int* slot = slots[wid];
 
     auto empty_ray = [&](uint32_t r) {
-        ray_loss<float>(r, B, 0.0f, 0.0f, 0.0f, 0.0f, bg, target, inv_n_rays, loss, nullptr, SB);
+        ray_loss<R>(r, B, (R)0, (R)0, (R)0, (R)0, bg, target, inv_n_rays, loss, (R*)nullptr, SB);
     };
 
     // ---------------- pass 1: forward totals, loss, dL/dRc ----------------
@@ -142,8 +144,8 @@ composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t S
             float cR = 0, cG = 0, cB = 0;
             if (valid) {
                 cR = ldf(color, s);
-                cG = ldf(color, s + B);
-                cB = ldf(color, s + 2 * (uint64_t)B);
+                cG = ldf(color, s + PB);
+                cB = ldf(color, s + 2 * (uint64_t)PB);
             }
             const R wv = st.inc ? st.w : (R)0;
             Acc<R> sr = Acc<R>::of(wv * (R)cR).scan(ray), sg = Acc<R>::of(wv * (R)cG).scan(ray);
@@ -192,22 +194,20 @@ composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t S
             int ray = chunk_rays(c, wr.r1, r_scan, offs, steps, slot, carry_ray, no_op);
             if (!valid) ray = -2 - (int)lane;
             SampleState<R> st = sample_state<R>(valid, ray, s, density, dtb, carry_ray, carry_P, min_T);
-            R g = 0, wg = 0;
-            float cR = 0, cG = 0, cB = 0, A = 0, GR = 0, GG = 0, GB = 0, Wg_tot = 0;
+            R g = 0, wg = 0, A = 0, GR = 0, GG = 0, GB = 0, Wg_tot = 0;
             if (valid) {
                 const uint32_t rr = (uint32_t)ray;
-                cR = ldf(color, s);
-                cG = ldf(color, s + B);
-                cB = ldf(color, s + 2 * (uint64_t)B);
+                const R cR = ldf(color, s);
+                const R cG = ldf(color, s + PB);
+                const R cB = ldf(color, s + 2 * (uint64_t)PB);
                 A = scratch[rr];
-                const float RR = scratch[rr + SB], RG = scratch[rr + 2 * SB], RB = scratch[rr + 3 * SB];
+                const R RR = scratch[rr + SB], RG = scratch[rr + 2 * SB], RB = scratch[rr + 3 * SB];
                 GR = scratch[rr + 4 * SB];
                 GG = scratch[rr + 5 * SB];
                 GB = scratch[rr + 6 * SB];
-                const float br = bg[rr], bgg = bg[rr + B], bb = bg[rr + 2 * B];
-                g = (R)GR * ((R)cR * (R)A + (R)RR - (R)br) + (R)GG * ((R)cG * (R)A + (R)RG - (R)bgg) +
-                    (R)GB * ((R)cB * (R)A + (R)RB - (R)bb);
-                Wg_tot = A * (GR * (2.0f * RR - br) + GG * (2.0f * RG - bgg) + GB * (2.0f * RB - bb));
+                const R br = bg[rr], bgg = bg[rr + B], bb = bg[rr + 2 * B];
+                g = GR * (cR * A + RR - br) + GG * (cG * A + RG - bgg) + GB * (cB * A + RB - bb);
+                Wg_tot = A * (GR * ((R)2 * RR - br) + GG * ((R)2 * RG - bgg) + GB * ((R)2 * RB - bb));
                 wg = st.inc ? st.w * g : (R)0;
             }
             Acc<R> Q = Acc<R>::of(wg).scan(ray);
@@ -215,17 +215,17 @@ composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t S
             if (valid) {
                 float dsig = 0.0f, gc_r = 0.0f, gc_g = 0.0f, gc_b = 0.0f;
                 if (st.inc) {
-                    const R suffix = (R)Wg_tot - Q.val();
-                    dsig = (float)(st.dt * (st.T_next * g - suffix));
-                    const float k = grad_scale * (float)st.w * A;
-                    gc_r = k * GR;
-                    gc_g = k * GG;
-                    gc_b = k * GB;
+                    const R suffix = Wg_tot - Q.val();
+                    dsig = (float)(st.dt * (st.T_next * g - suffix) * st.sigma);
+                    const R k = (R)grad_scale * st.w * A;
+                    gc_r = (float)(k * GR);
+                    gc_g = (float)(k * GG);
+                    gc_b = (float)(k * GB);
                 }
-                dL_ddensity[s] = dsig * (float)st.sigma;
+                dL_ddensity[s] = dsig;
                 stf(dL_dcolor, s, gc_r);
-                stf(dL_dcolor, s + B, gc_g);
-                stf(dL_dcolor, s + 2 * (uint64_t)B, gc_b);
+                stf(dL_dcolor, s + PB, gc_g);
+                stf(dL_dcolor, s + 2 * (uint64_t)PB, gc_b);
             }
             carry_Q = Q.bcast(63);
             const bool tail63 =
@@ -329,16 +329,16 @@ __global__ void reduce_loss_kernel(const uint32_t* __restrict__ info, uint32_t B
 }
 
 template <typename TIn, typename TGrad, typename R>
-int launch_composite(const uint32_t* info, uint32_t n_rays_max, uint32_t B, uint32_t max_samples, float grad_scale,
+int launch_composite(const uint32_t* info, uint32_t n_rays_max, uint32_t B, uint32_t PB, float grad_scale,
                      float min_T, const uint32_t* steps, const uint32_t* offs, const float* bg, const float* target,
                      const void* density, const void* color, const float* dt, float* loss, float* dL_ddensity,
-                     void* dL_dcolor, float* scratch, hipStream_t stream) {
+                     void* dL_dcolor, void* scratch, hipStream_t stream) {
     const uint32_t S = 128;
-    const uint32_t waves = (max_samples + S - 1) / S + 1;
+    const uint32_t waves = (PB + S - 1) / S + 1;
     const uint32_t blocks = (waves + kCompositeWaves - 1) / kCompositeWaves;
     composite_train_kernel<TIn, TGrad, R><<<blocks, 64 * kCompositeWaves, 0, stream>>>(
-        info, B, n_rays_max, S, grad_scale, min_T, steps, offs, bg, target, static_cast<const TIn*>(density),
-        static_cast<const TIn*>(color), dt, loss, dL_ddensity, static_cast<TGrad*>(dL_dcolor), scratch);
+        info, B, PB, n_rays_max, S, grad_scale, min_T, steps, offs, bg, target, static_cast<const TIn*>(density),
+        static_cast<const TIn*>(color), dt, loss, dL_ddensity, static_cast<TGrad*>(dL_dcolor), static_cast<R*>(scratch));
     TN_CHECK_LAUNCH();
     return TN_OK;
 }
@@ -348,34 +348,34 @@ int launch_composite(const uint32_t* info, uint32_t n_rays_max, uint32_t B, uint
 using namespace tn;
 
 extern "C" size_t tn_composite_scratch_bytes(uint32_t n_rays_max) {
-    return (size_t)kRayScratchRows * n_rays_max * sizeof(float);
+    return (size_t)kRayScratchRows * n_rays_max * sizeof(double);
 }
 
 // dtype: 0 = fp32 network outputs and fp32 colour gradients,
 //        1 = fp16 network outputs and fp16 colour gradients,
 //        2 = fp32 in/out with fp64 accumulation (oracle mode)
 extern "C" int tn_composite_train(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t stride,
-                                  float grad_scale, float min_transmittance, const uint32_t* ray_steps,
+                                  uint32_t sample_stride, float grad_scale, float min_transmittance, const uint32_t* ray_steps,
                                   const uint32_t* ray_offset, const float* bg_rgb, const float* target_rgba,
                                   const void* density, const void* color, const float* dt, float* loss,
-                                  float* dL_ddensity, void* dL_dcolor, float* ray_scratch, size_t scratch_bytes,
+                                  float* dL_ddensity, void* dL_dcolor, void* ray_scratch, size_t scratch_bytes,
                                   hipStream_t stream) {
     TN_CHECK_ARG(info && ray_steps && ray_offset && bg_rgb && target_rgba && density && color && dt && loss &&
                  dL_ddensity && dL_dcolor && ray_scratch);
-    TN_CHECK_ARG(n_rays_max > 0 && n_rays_max <= stride);
+    TN_CHECK_ARG(n_rays_max > 0 && n_rays_max <= stride && sample_stride > 0);
     if (scratch_bytes < tn_composite_scratch_bytes(n_rays_max)) return TN_ERR_OUT_OF_MEMORY;
     switch (dtype) {
         case 0:
-            return launch_composite<float, float, float>(info, n_rays_max, stride, stride, grad_scale, min_transmittance,
+            return launch_composite<float, float, float>(info, n_rays_max, stride, sample_stride, grad_scale, min_transmittance,
                                                          ray_steps, ray_offset, bg_rgb, target_rgba, density, color,
                                                          dt, loss, dL_ddensity, dL_dcolor, ray_scratch, stream);
         case 1:
-            return launch_composite<_Float16, _Float16, float>(info, n_rays_max, stride, stride, grad_scale,
+            return launch_composite<_Float16, _Float16, float>(info, n_rays_max, stride, sample_stride, grad_scale,
                                                                min_transmittance, ray_steps, ray_offset, bg_rgb,
                                                                target_rgba, density, color, dt, loss, dL_ddensity,
                                                                dL_dcolor, ray_scratch, stream);
         case 2:
-            return launch_composite<float, float, double>(info, n_rays_max, stride, stride, grad_scale,
+            return launch_composite<float, float, double>(info, n_rays_max, stride, sample_stride, grad_scale,
                                                           min_transmittance, ray_steps, ray_offset, bg_rgb,
                                                           target_rgba, density, color, dt, loss, dL_ddensity,
                                                           dL_dcolor, ray_scratch, stream);

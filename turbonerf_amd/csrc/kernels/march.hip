@@ -189,7 +189,7 @@ __global__ void scan_write_select_kernel(uint32_t n, const uint32_t* __restrict_
 }
 
 __global__ void march_generate_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ info, uint32_t B,
-                                      tn_bbox_t bbox, float inv_aabb, tn_occupancy_grid_t grid, float dt_min,
+                                      uint32_t SB, tn_bbox_t bbox, float inv_aabb, tn_occupancy_grid_t grid, float dt_min,
                                       float dt_max, float cone, const uint32_t* __restrict__ ray_img_id,
                                       const float* __restrict__ ori, const float* __restrict__ dir,
                                       const float* __restrict__ ray_t, const uint32_t* __restrict__ ray_offset,
@@ -231,11 +231,11 @@ __global__ void march_generate_kernel(uint32_t n_rays_max, const uint32_t* __res
             out_img_id[sid] = img;
             out_dt[sid] = inv_aabb * dt;
             out_pos[sid] = fminf(fmaxf(x * inv_aabb + 0.5f, 0.0f), 1.0f);
-            out_pos[sid + B] = fminf(fmaxf(y * inv_aabb + 0.5f, 0.0f), 1.0f);
-            out_pos[sid + 2 * B] = fminf(fmaxf(z * inv_aabb + 0.5f, 0.0f), 1.0f);
+            out_pos[sid + SB] = fminf(fmaxf(y * inv_aabb + 0.5f, 0.0f), 1.0f);
+            out_pos[sid + 2 * SB] = fminf(fmaxf(z * inv_aabb + 0.5f, 0.0f), 1.0f);
             out_dir[sid] = ndx;
-            out_dir[sid + B] = ndy;
-            out_dir[sid + 2 * B] = ndz;
+            out_dir[sid + SB] = ndy;
+            out_dir[sid + 2 * SB] = ndz;
             if (inside) {
                 t += dt;
                 t_max = t;
@@ -312,7 +312,8 @@ extern "C" int tn_scan_select(int mode, uint32_t n_rays, const uint32_t* n_steps
     return TN_OK;
 }
 
-extern "C" int tn_march_generate(uint32_t n_rays_max, const uint32_t* info, uint32_t stride, tn_bbox_t bbox,
+extern "C" int tn_march_generate(uint32_t n_rays_max, const uint32_t* info, uint32_t stride,
+                                 uint32_t sample_stride, tn_bbox_t bbox,
                                  float inv_aabb, tn_occupancy_grid_t grid, float dt_min, float dt_max,
                                  float cone_angle, const uint32_t* ray_img_id, const float* ori,
                                  const float* dir, const float* ray_t, const uint32_t* ray_offset,
@@ -322,9 +323,10 @@ extern "C" int tn_march_generate(uint32_t n_rays_max, const uint32_t* info, uint
     TN_CHECK_ARG(ray_img_id && ori && dir && ray_t && ray_offset && ray_alive && n_ray_steps && out_img_id &&
                  out_t && out_pos && out_dir && out_dt && out_m_norm && out_dt_norm && grid.bitfield);
     TN_CHECK_ARG(n_rays_max <= stride && grid.n_levels >= 1 && grid.n_levels <= 8 && dt_min > 0.0f);
+    TN_CHECK_ARG(info != nullptr && sample_stride > 0);
     if (n_rays_max == 0) return TN_OK;
     march_generate_kernel<<<blocks_for(n_rays_max, 64), 64, 0, stream>>>(
-        n_rays_max, info, stride, bbox, inv_aabb, grid, dt_min, dt_max, cone_angle, ray_img_id, ori, dir, ray_t,
+        n_rays_max, info, stride, sample_stride, bbox, inv_aabb, grid, dt_min, dt_max, cone_angle, ray_img_id, ori, dir, ray_t,
         ray_offset, ray_alive, n_ray_steps, out_img_id, out_t, out_pos, out_dir, out_dt, out_m_norm, out_dt_norm);
     TN_CHECK_LAUNCH();
     return TN_OK;

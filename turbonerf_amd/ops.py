@@ -110,7 +110,7 @@ def march_generate(rays: Rays, samples: Samples, n_rays_max: int, info: torch.Te
                    grid: OccupancyGrid, dt_min: float, dt_max: float, cone: float) -> None:
     if n_rays_max > rays.stride:
         raise ValueError("n_rays_max exceeds the ray capacity")
-    call("tn_march_generate", n_rays_max, ptr(info), rays.stride, bbox, inv_aabb, grid, dt_min, dt_max, cone,
+    call("tn_march_generate", n_rays_max, ptr(info), rays.stride, samples.stride, bbox, inv_aabb, grid, dt_min, dt_max, cone,
          ptr(rays.img_id), ptr(rays.ori), ptr(rays.dir), ptr(rays.t), ptr(rays.offset), ptr(rays.alive),
          ptr(rays.n_steps), ptr(samples.img_id), ptr(samples.t), ptr(samples.pos), ptr(samples.dir),
          ptr(samples.dt), ptr(samples.m_norm), ptr(samples.dt_norm), stream_ptr())
@@ -119,19 +119,20 @@ def march_generate(rays: Rays, samples: Samples, n_rays_max: int, info: torch.Te
 _DTYPE_CODE = {torch.float32: 0, torch.float16: 1}
 
 
-def composite_train(info: torch.Tensor, n_rays_max: int, stride: int, steps: torch.Tensor, offs: torch.Tensor,
-                    bg: torch.Tensor, target: torch.Tensor, density: torch.Tensor, color: torch.Tensor,
-                    dt: torch.Tensor, grad_scale: float = 1.0, min_T: float = 1e-4, fp64_accumulate: bool = False):
-    """Returns (loss[4*stride], dL_ddensity[stride], dL_dcolor[3*stride])."""
+def composite_train(info: torch.Tensor, n_rays_max: int, stride: int, sample_stride: int, steps: torch.Tensor,
+                    offs: torch.Tensor, bg: torch.Tensor, target: torch.Tensor, density: torch.Tensor,
+                    color: torch.Tensor, dt: torch.Tensor, grad_scale: float = 1.0, min_T: float = 1e-4,
+                    fp64_accumulate: bool = False):
+    """Returns (loss[4*stride], dL_ddensity[sample_stride], dL_dcolor[3*sample_stride])."""
     code = 2 if fp64_accumulate else _DTYPE_CODE[density.dtype]
     if fp64_accumulate and density.dtype != torch.float32:
         raise ValueError("fp64 accumulation takes fp32 inputs")
     loss = torch.zeros(4 * stride, dtype=torch.float32, device="cuda")
-    dLd = torch.zeros(stride, dtype=torch.float32, device="cuda")
-    dLc = torch.zeros(3 * stride, dtype=density.dtype, device="cuda")
+    dLd = torch.zeros(sample_stride, dtype=torch.float32, device="cuda")
+    dLc = torch.zeros(3 * sample_stride, dtype=density.dtype, device="cuda")
     nb = query("tn_composite_scratch_bytes", n_rays_max)
     scratch = torch.empty(nb // 4, dtype=torch.float32, device="cuda")
-    call("tn_composite_train", code, ptr(info), n_rays_max, stride, grad_scale, min_T, ptr(steps), ptr(offs),
+    call("tn_composite_train", code, ptr(info), n_rays_max, stride, sample_stride, grad_scale, min_T, ptr(steps), ptr(offs),
          ptr(bg), ptr(target), ptr(density), ptr(color), ptr(dt), ptr(loss), ptr(dLd), ptr(dLc), ptr(scratch), nb,
          stream_ptr())
     return loss, dLd, dLc
@@ -192,3 +193,71 @@ def popcount(b: torch.Tensor) -> int:
     out = torch.zeros(1, dtype=torch.int32, device="cuda")
     call("tn_popcount", ptr(b), n, ptr(out), ptr(ws), nb, stream_ptr())
     return int(out.item())
+
+
+# ------------------------------------------------------------ hash grid / MLP
+def hashgrid_layout(n_levels: int = 16, n_features: int = 2, log2_T: int = 19, base_res: int = 16,
+                    per_level_scale: float | None = None, max_res: float = 2048.0) -> _capi.HashGrid:
+    """Instant-NGP level table; per_level_scale defaults to
+    exp((ln max_res - ln base_res) / (L - 1)) as the reference configures it
+    (src/core/nerf-network.cu:86-91)."""
+    import math
+
+    if per_level_scale is None:
+        per_level_scale = math.exp((math.log(max_res) - math.log(base_res)) / (n_levels - 1))
+    g = _capi.HashGrid()
+    call("tn_hashgrid_layout", n_levels, n_features, log2_T, base_res, per_level_scale, C.byref(g))
+    return g
+
+
+_PREC = {torch.float32: 0, torch.float16: 1}
+
+
+def hashgrid_forward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: int, table: torch.Tensor,
+                     feat_dtype=None, info: torch.Tensor | None = None, out_stride: int | None = None):
+    feat_dtype = table.dtype if feat_dtype is None else feat_dtype
+    out_stride = stride if out_stride is None else out_stride
+    out = torch.zeros(2 * grid.n_levels * out_stride, dtype=feat_dtype, device="cuda")
+    call("tn_hashgrid_forward", _PREC[table.dtype], _PREC[feat_dtype], n, ptr(info), stride, out_stride, grid,
+         ptr(pos), ptr(table), ptr(out), stream_ptr())
+    return out
+
+
+def hashgrid_backward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: int, dfeat: torch.Tensor,
+                      dtable: torch.Tensor, info: torch.Tensor | None = None, dfeat_stride: int | None = None):
+    dfeat_stride = stride if dfeat_stride is None else dfeat_stride
+    call("tn_hashgrid_backward", _PREC[dfeat.dtype], n, ptr(info), stride, dfeat_stride, grid, ptr(pos), ptr(dfeat),
+         ptr(dtable), stream_ptr())
+
+
+def mlp_forward(params: torch.Tensor, net_offset: int, n: int, feat: torch.Tensor, feat_stride: int,
+                dir: torch.Tensor | None, stride: int, color: bool = True, info: torch.Tensor | None = None):
+    prec = _PREC[params.dtype]
+    dens = torch.zeros(stride, dtype=params.dtype, device="cuda")
+    rgb = torch.zeros(3 * stride, dtype=params.dtype, device="cuda") if color else None
+    call("tn_mlp_forward", prec, n, ptr(info), stride, feat_stride, stride, ptr(params), net_offset, ptr(feat),
+         ptr(dir), int(color), ptr(dens), ptr(rgb), stream_ptr())
+    return dens, rgb
+
+
+def mlp_backward(params: torch.Tensor, net_offset: int, n: int, feat: torch.Tensor, feat_stride: int,
+                 dir: torch.Tensor, stride: int, sample_t: torch.Tensor, dL_drgb: torch.Tensor, dL_ds: torch.Tensor,
+                 loss_scale: float, grad_net: torch.Tensor, dL_ds_extra: torch.Tensor | None = None,
+                 accumulate: bool = False, info: torch.Tensor | None = None):
+    """Returns the feature gradients [32][feat_stride]; weight gradients go to grad_net."""
+    prec = _PREC[params.dtype]
+    dfeat = torch.zeros(32 * feat_stride, dtype=params.dtype, device="cuda")
+    nb = query("tn_mlp_backward_workspace_bytes", n)
+    ws = torch.empty(max(nb, 4), dtype=torch.uint8, device="cuda")
+    call("tn_mlp_backward", prec, n, ptr(info), stride, feat_stride, ptr(params), net_offset, ptr(feat), ptr(dir),
+         ptr(sample_t), ptr(dL_drgb), stride, ptr(dL_ds), ptr(dL_ds_extra), loss_scale, ptr(dfeat), ptr(grad_net),
+         int(accumulate), ptr(ws), nb, stream_ptr())
+    return dfeat
+
+
+def adam_step(n_grid: int, n_net_end: int, loss_scale: float, lr: float, beta1: float, beta2: float, eps: float,
+              l2: float, w: torch.Tensor, w_lp: torch.Tensor | None, grad: torch.Tensor, m: torch.Tensor,
+              v: torch.Tensor, steps: torch.Tensor) -> None:
+    lp = 0 if w_lp is None else _PREC[w_lp.dtype]
+    call("tn_adam_step", w.numel(), n_grid, n_net_end, loss_scale, lr, beta1, beta2, eps, l2, ptr(w), lp, ptr(w_lp),
+         ptr(grad), ptr(m), ptr(v), ptr(steps), stream_ptr())
