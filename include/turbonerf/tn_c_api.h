@@ -74,7 +74,8 @@ int tn_march_count(uint32_t n_rays_max, const uint32_t* d_n_rays, uint32_t strid
  * device-to-host copy (reference src/core/trainer.cu:143-183,
  * src/utils/parallel-utils.cuh:30-58). */
 size_t tn_scan_workspace_bytes(uint32_t n_rays);
-int tn_scan_select(int mode, uint32_t n_rays, const uint32_t* n_steps, uint32_t max_samples,
+int tn_scan_select(int mode, uint32_t n_rays, const uint32_t* d_n_rays, const uint32_t* n_steps,
+    uint32_t max_samples,
     uint32_t* ray_offset, uint32_t* info, void* workspace, size_t workspace_bytes,
     tn_stream_t stream);
 
@@ -120,6 +121,18 @@ int tn_composite_forward(int dtype, const uint32_t* info, uint32_t stride, uint3
     uint32_t out_stride, float min_transmittance, int raw_density, const uint32_t* ray_steps,
     const uint32_t* ray_offset, const void* density, const void* color, const float* dt,
     float* rgba, tn_stream_t stream);
+
+/* mip-NeRF 360 distortion loss and dL/ds (unscaled) from raw densities,
+ * normalised midpoints m_norm and steps dt_norm (reference
+ * mipNeRF360_distortion_loss_forward/backward_kernel,
+ * src/utils/training-kernels-fused.cuh:13-224, and its orchestration
+ * src/core/nerf-network.cu:605-666; lambda is the reference's constant 0.01,
+ * here a parameter). dtype 0 = fp32 density, 1 = fp16, 2 = fp64 accumulation. */
+size_t tn_distortion_scratch_bytes(uint32_t n_rays_max);
+int tn_distortion(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t sample_stride,
+    float lambda, const uint32_t* ray_steps, const uint32_t* ray_offset, const void* density,
+    const float* m_norm, const float* dt_norm, float* ray_loss, float* dL_ds, void* scratch,
+    size_t scratch_bytes, tn_stream_t stream);
 
 /* (1/info[0]) * sum over rays and n_channels of loss -> out[0] (device).
  * Replaces the thrust::reduce of the loss buffer (reference
@@ -219,6 +232,27 @@ int tn_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float loss_sca
 /* fp32 -> fp16 parameter copy (reference copy_and_cast,
  * src/utils/parallel-utils.cuh:60-82). */
 int tn_cast_params(uint64_t n, const float* src, int lp_dtype, void* dst, tn_stream_t stream);
+
+/* -------------------------------------------------------------- render */
+
+/* Row-major pixel rays for pixels [start_pixel, start_pixel + n_rays) of
+ * `camera`, taken to the NeRF frame by `to_local` (optional device 3x4,
+ * world -> local) and clipped to render_bbox there: origins move to the entry
+ * point (ray_t = 0), ray_tmax = exit distance capped by far - near, misses are
+ * not alive, pixel[i] = pixel index. Replaces
+ * generate_linear_buffer_of_rays_kernel + fill_ray_buffers +
+ * prepare_for_linear_raymarching_kernel (reference
+ * src/rendering/ray-batch-coordinators/linear-buffer-ray-batch-coordinator.cu:11-34,
+ * ray-batch-coordinator.cuh:46-84, src/utils/rendering-kernels.cu:175-260). */
+int tn_render_rays(uint32_t n_rays, uint32_t start_pixel, uint32_t stride, tn_camera_t camera,
+    tn_bbox_t render_bbox, const float* to_local, float* ori, float* dir, float* ray_t,
+    float* ray_tmax, uint32_t* pixel, uint8_t* alive, tn_stream_t stream);
+
+/* Interleaved RGBA image writes of the first min(info[0], n_rays) rays from
+ * rgba[4][rgba_stride] (reference copy_packed_rgba_linear_buffer_kernel,
+ * linear-buffer-ray-batch-coordinator.cu:54-79). */
+int tn_render_write_rgba(uint32_t n_rays, const uint32_t* info, uint32_t rgba_stride,
+    const uint32_t* pixel, const uint8_t* alive, const float* rgba, float* out, tn_stream_t stream);
 
 /* ------------------------------------------- synthetic analytic field */
 

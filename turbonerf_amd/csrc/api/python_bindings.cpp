@@ -1,0 +1,236 @@
+// PyTurboNeRF: pybind11 surface mirroring the reference module
+// (src/api/python_bindings.cu:56-694) for the training/rendering path.
+#include <pybind11/functional.h>
+#include <pybind11/numpy.h>
+#include <pybind11/operators.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "controllers.hpp"
+#include "renderer.hpp"
+
+namespace py = pybind11;
+using namespace turbo;
+
+namespace {
+
+int2v to_int2(py::handle h) {
+    auto s = h.cast<py::sequence>();
+    if (py::len(s) != 2) throw std::runtime_error("expected a pair");
+    return int2v{s[0].cast<int>(), s[1].cast<int>()};
+}
+
+float2v to_float2(py::handle h) {
+    auto s = h.cast<py::sequence>();
+    if (py::len(s) != 2) throw std::runtime_error("expected a pair");
+    return float2v{s[0].cast<float>(), s[1].cast<float>()};
+}
+
+Transform4f to_transform(py::array_t<float, py::array::c_style | py::array::forcecast> a) {
+    if (a.ndim() != 2 || a.shape(1) != 4 || (a.shape(0) != 3 && a.shape(0) != 4))
+        throw std::runtime_error("Invalid shape for Transform4f");
+    return Transform4f::from_array(a.data());
+}
+
+}  // namespace
+
+PYBIND11_MODULE(PyTurboNeRF, m) {
+    m.doc() = "TurboNeRF (MI355X) Python bindings";
+    m.attr("__version__") = "0.0.19";
+    m.attr("__build__") = 1;
+    m.def("teardown", []() { DeviceManager::teardown(); });
+
+    py::class_<Transform4f>(m, "Transform4f", py::buffer_protocol())
+        .def(py::init([](py::array_t<float, py::array::c_style | py::array::forcecast> a) { return to_transform(a); }),
+             py::arg("matrix"))
+        .def_static("Identity", &Transform4f::Identity)
+        .def("from_nerf", &Transform4f::from_nerf)
+        .def("to_nerf", &Transform4f::to_nerf)
+        .def("inverse", &Transform4f::inverse)
+        .def("to_matrix", [](const Transform4f& t) {
+            auto a = t.to_array();
+            py::array_t<float> out({3, 4});
+            std::memcpy(out.mutable_data(), a.data(), sizeof(a));
+            return out;
+        })
+        .def(py::self == py::self);
+
+    py::class_<DistortionParams>(m, "DistortionParams")
+        .def(py::init([](float k1, float k2, float k3, float p1, float p2) {
+                 return DistortionParams{k1, k2, k3, p1, p2};
+             }),
+             py::arg("k1") = 0.0f, py::arg("k2") = 0.0f, py::arg("k3") = 0.0f, py::arg("p1") = 0.0f,
+             py::arg("p2") = 0.0f)
+        .def_readwrite("k1", &DistortionParams::k1)
+        .def_readwrite("k2", &DistortionParams::k2)
+        .def_readwrite("k3", &DistortionParams::k3)
+        .def_readwrite("p1", &DistortionParams::p1)
+        .def_readwrite("p2", &DistortionParams::p2);
+
+    py::class_<Camera>(m, "Camera")
+        .def(py::init([](py::object res, float near, float far, py::object focal, py::object pp, py::object shift,
+                         const Transform4f& t, const DistortionParams& dp) {
+                 return Camera(to_int2(res), near, far, to_float2(focal), to_float2(pp), to_float2(shift), t, dp);
+             }),
+             py::arg("resolution"), py::arg("near"), py::arg("far"), py::arg("focal_length"),
+             py::arg("principal_point"), py::arg("shift"), py::arg("transform"),
+             py::arg("dist_params") = DistortionParams())
+        .def_property("resolution", [](const Camera& c) { return py::make_tuple(c.resolution.x, c.resolution.y); },
+                      [](Camera& c, py::object v) { c.resolution = to_int2(v); })
+        .def_readwrite("near", &Camera::near)
+        .def_readwrite("far", &Camera::far)
+        .def_property("focal_length", [](const Camera& c) { return py::make_tuple(c.focal_length.x, c.focal_length.y); },
+                      [](Camera& c, py::object v) { c.focal_length = to_float2(v); })
+        .def_property("principal_point",
+                      [](const Camera& c) { return py::make_tuple(c.principal_point.x, c.principal_point.y); },
+                      [](Camera& c, py::object v) { c.principal_point = to_float2(v); })
+        .def_property("shift", [](const Camera& c) { return py::make_tuple(c.shift.x, c.shift.y); },
+                      [](Camera& c, py::object v) { c.shift = to_float2(v); })
+        .def_readwrite("transform", &Camera::transform)
+        .def_readwrite("dist_params", &Camera::dist_params)
+        .def_readwrite("show_image_planes", &Camera::show_image_planes)
+        .def(py::self == py::self)
+        .def(py::self != py::self);
+
+    py::class_<BoundingBox>(m, "BoundingBox")
+        .def(py::init<>())
+        .def(py::init<float>(), py::arg("size"))
+        .def("size", &BoundingBox::size)
+        .def(py::self == py::self)
+        .def_readwrite("min_x", &BoundingBox::min_x)
+        .def_readwrite("min_y", &BoundingBox::min_y)
+        .def_readwrite("min_z", &BoundingBox::min_z)
+        .def_readwrite("max_x", &BoundingBox::max_x)
+        .def_readwrite("max_y", &BoundingBox::max_y)
+        .def_readwrite("max_z", &BoundingBox::max_z);
+
+    py::class_<Dataset>(m, "Dataset")
+        .def(py::init<const std::string&>(), py::arg("file_path"))
+        .def(py::init([](const BoundingBox& bbox, const std::vector<Camera>& cams,
+                         py::array_t<uint8_t, py::array::c_style | py::array::forcecast> images) {
+                 if (images.ndim() != 4 || images.shape(3) != 4 || (size_t)images.shape(0) != cams.size())
+                     throw std::runtime_error("images must be (n_cameras, height, width, 4) uint8");
+                 return Dataset(bbox, cams, images.data(), (int)images.shape(2), (int)images.shape(1));
+             }),
+             py::arg("bounding_box"), py::arg("cameras"), py::arg("images"))
+        .def("load_transforms", &Dataset::load_transforms)
+        .def("copy", &Dataset::copy)
+        .def("set_camera_at", [](Dataset& ds, int i, const Camera& c) { ds.cameras.at(i) = c; })
+        .def_readwrite("file_path", &Dataset::file_path)
+        .def_readwrite("cameras", &Dataset::cameras)
+        .def_readwrite("bounding_box", &Dataset::bounding_box)
+        .def_property_readonly("image_dimensions",
+                               [](const Dataset& d) { return py::make_tuple(d.image_dimensions.x, d.image_dimensions.y); })
+        .def_property_readonly("n_images", &Dataset::n_images)
+        .def_readonly("contains_multiple_image_dims", &Dataset::contains_multiple_image_dims);
+
+    py::class_<NeRFProxy>(m, "NeRF")
+        .def("attach_dataset", &NeRFProxy::attach_dataset)
+        .def("detach_dataset", &NeRFProxy::detach_dataset)
+        .def("is_dirty", &NeRFProxy::is_dirty)
+        .def("can_train", &NeRFProxy::can_train)
+        .def("is_image_data_loaded", &NeRFProxy::is_image_data_loaded)
+        .def_property("render_bbox", [](const NeRFProxy& p) { return p.render_bbox.get(); },
+                      [](NeRFProxy& p, const BoundingBox& b) { p.render_bbox.set(b); })
+        .def_property("training_bbox", [](const NeRFProxy& p) { return p.training_bbox.get(); },
+                      [](NeRFProxy& p, const BoundingBox& b) { p.training_bbox.set(b); })
+        .def_property("transform", [](const NeRFProxy& p) { return p.transform.get(); },
+                      [](NeRFProxy& p, const Transform4f& t) { p.transform.set(t); })
+        .def_readwrite("is_visible", &NeRFProxy::is_visible)
+        .def_readwrite("is_dataset_dirty", &NeRFProxy::is_dataset_dirty)
+        .def_property_readonly("dataset", [](const NeRFProxy& p) { return p.dataset; })
+        .def_readonly("can_render", &NeRFProxy::can_render)
+        .def_readonly("training_step", &NeRFProxy::training_step)
+        .def_readonly("id", &NeRFProxy::id);
+
+    py::class_<NeRFManager>(m, "NeRFManager")
+        .def(py::init<>())
+        .def("create", &NeRFManager::create, py::return_value_policy::reference)
+        .def("clone", &NeRFManager::clone, py::arg("nerf"), py::return_value_policy::reference)
+        .def("destroy", &NeRFManager::destroy, py::arg("nerf"))
+        .def("get_nerfs", &NeRFManager::get_proxies, py::return_value_policy::reference);
+
+    py::class_<NetworkConfig>(m, "NetworkConfig")
+        .def(py::init<>())
+        .def_readwrite("precision", &NetworkConfig::precision)
+        .def_readwrite("n_levels", &NetworkConfig::n_levels)
+        .def_readwrite("log2_hashmap_size", &NetworkConfig::log2_hashmap_size)
+        .def_readwrite("base_resolution", &NetworkConfig::base_resolution)
+        .def_readwrite("finest_resolution_per_unit", &NetworkConfig::finest_resolution_per_unit)
+        .def_readwrite("hash_init_scale", &NetworkConfig::hash_init_scale)
+        .def_readwrite("seed", &NetworkConfig::seed)
+        .def_readwrite("learning_rate", &NetworkConfig::learning_rate)
+        .def_readwrite("learning_rate_decay", &NetworkConfig::learning_rate_decay)
+        .def_readwrite("beta1", &NetworkConfig::beta1)
+        .def_readwrite("beta2", &NetworkConfig::beta2)
+        .def_readwrite("epsilon", &NetworkConfig::epsilon)
+        .def_readwrite("l2_reg", &NetworkConfig::l2_reg);
+
+    py::class_<TrainingConfig>(m, "TrainingConfig")
+        .def(py::init<>())
+        .def_readwrite("network", &TrainingConfig::network)
+        .def_readwrite("rng_seed", &TrainingConfig::rng_seed)
+        .def_readwrite("distortion_lambda", &TrainingConfig::distortion_lambda)
+        .def_readwrite("occupancy_threshold", &TrainingConfig::occupancy_threshold)
+        .def_readwrite("ray_capacity", &TrainingConfig::ray_capacity);
+
+    py::class_<NeRFTrainingController::TrainingMetrics>(m, "TrainingMetrics")
+        .def_readonly("step", &NeRFTrainingController::TrainingMetrics::step)
+        .def_readonly("loss", &NeRFTrainingController::TrainingMetrics::loss)
+        .def_readonly("n_rays", &NeRFTrainingController::TrainingMetrics::n_rays)
+        .def_readonly("n_samples", &NeRFTrainingController::TrainingMetrics::n_samples);
+
+    py::class_<NeRFTrainingController::OccupancyGridMetrics>(m, "OccupancyGridMetrics")
+        .def_readonly("n_occupied", &NeRFTrainingController::OccupancyGridMetrics::n_occupied)
+        .def_readonly("n_total", &NeRFTrainingController::OccupancyGridMetrics::n_total);
+
+    py::class_<NeRFTrainingController, std::shared_ptr<NeRFTrainingController>>(m, "Trainer")
+        .def(py::init<NeRFProxy*, TrainingConfig>(), py::arg("nerf"), py::arg("config") = TrainingConfig())
+        .def("setup_data", &NeRFTrainingController::setup_data, py::arg("batch_size"),
+             "Call this once before starting training.")
+        .def("teardown", &NeRFTrainingController::teardown, "Call this before destroying the controller.")
+        .def("reset_training", &NeRFTrainingController::reset_training, "Call this to clear the training data.")
+        .def(
+            "load_images",
+            [](NeRFTrainingController& tc, py::object cb) {
+                if (py::isinstance<py::function>(cb)) {
+                    auto f = std::make_shared<py::function>(cb.cast<py::function>());
+                    py::gil_scoped_release release;
+                    tc.load_images([f](int n_loaded, int n_total) {
+                        py::gil_scoped_acquire acquire;
+                        (*f)(n_loaded, n_total);
+                    });
+                } else {
+                    py::gil_scoped_release release;
+                    tc.load_images();
+                }
+            },
+            py::arg("on_image_loaded") = py::none())
+        .def("update_occupancy_grid", &NeRFTrainingController::update_occupancy_grid, py::arg("training_step"))
+        .def("train_step", &NeRFTrainingController::train_step)
+        .def("train_steps", &NeRFTrainingController::train_steps, py::arg("n_steps"), py::arg("read_metrics") = true,
+             py::call_guard<py::gil_scoped_release>())
+        .def("set_occupancy_from_density",
+             [](NeRFTrainingController& tc, py::array_t<float, py::array::c_style | py::array::forcecast> d,
+                float thr) { tc.set_occupancy_from_density(d.data(), thr); },
+             py::arg("density"), py::arg("threshold"))
+        .def("get_params",
+             [](const NeRFTrainingController& tc) {
+                 auto v = tc.get_params();
+                 py::array_t<float> a(v.size());
+                 std::memcpy(a.mutable_data(), v.data(), v.size() * sizeof(float));
+                 return a;
+             })
+        .def("set_params",
+             [](NeRFTrainingController& tc, py::array_t<float, py::array::c_style | py::array::forcecast> a) {
+                 tc.set_params(std::vector<float>(a.data(), a.data() + a.size()));
+             })
+        .def("get_cuda_memory_allocated", &NeRFTrainingController::get_cuda_memory_allocated)
+        .def_readwrite("alpha_selection_threshold", &NeRFTrainingController::alpha_selection_threshold)
+        .def_readwrite("alpha_selection_probability", &NeRFTrainingController::alpha_selection_probability)
+        .def_readwrite("min_step_size", &NeRFTrainingController::min_step_size)
+        .def_readwrite("use_distortion_loss", &NeRFTrainingController::use_distortion_loss)
+        .def_readwrite("nerf", &NeRFTrainingController::proxy);
+
+    register_renderer_bindings(m);
+}

@@ -1,0 +1,81 @@
+// Rendering part of the PyTurboNeRF module (reference python_bindings.cu:292-419).
+#include <pybind11/functional.h>
+#include <pybind11/numpy.h>
+#include <pybind11/operators.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "renderer.hpp"
+
+namespace py = pybind11;
+
+namespace turbo {
+
+void register_renderer_bindings(py::module_& m) {
+    py::class_<NeRFRenderable>(m, "Renderable")
+        .def(py::init([](NeRFProxy* nerf, py::object bbox) {
+                 if (bbox.is_none()) return NeRFRenderable(nerf);
+                 return NeRFRenderable(nerf, bbox.cast<BoundingBox>());
+             }),
+             py::arg("nerf"), py::arg("bounding_box") = py::none())
+        .def_readonly("nerf", &NeRFRenderable::proxy)
+        .def_readwrite("bounding_box", &NeRFRenderable::bounding_box);
+
+    py::enum_<RenderPattern>(m, "RenderPattern")
+        .value("LinearChunks", RenderPattern::LinearChunks)
+        .value("HexagonalGrid", RenderPattern::HexagonalGrid)
+        .value("RectangularGrid", RenderPattern::RectangularGrid);
+
+    py::enum_<RenderFlags>(m, "RenderFlags", py::arithmetic())
+        .value("Preview", RenderFlags::Preview)
+        .value("Final", RenderFlags::Final);
+
+    py::class_<RenderProperties>(m, "RenderProperties")
+        .def(py::init<>())
+        .def_readwrite("show_near_planes", &RenderProperties::show_near_planes)
+        .def_readwrite("show_far_planes", &RenderProperties::show_far_planes);
+
+    py::class_<RenderModifiers>(m, "RenderModifiers")
+        .def(py::init<>())
+        .def_readwrite("properties", &RenderModifiers::properties);
+
+    py::class_<RenderTarget>(m, "RenderTarget")
+        .def("set_size", [](RenderTarget& rt, int w, int h) { rt.set_size(w, h); }, py::arg("width"), py::arg("height"))
+        .def_readonly("width", &RenderTarget::width)
+        .def_readonly("height", &RenderTarget::height)
+        .def("free", [](RenderTarget& rt) { rt.free(); });
+
+    py::class_<CPURenderBuffer, RenderTarget>(m, "CPURenderBuffer")
+        .def(py::init<>())
+        .def("get_rgba", [](py::object self) {
+            auto& rb = self.cast<CPURenderBuffer&>();
+            const py::ssize_t w = rb.width, h = rb.height;
+            // a view into the buffer, as the reference returns (python_bindings.cu:334-349)
+            return py::array_t<float>({h, w, (py::ssize_t)4},
+                                      {(py::ssize_t)(sizeof(float) * 4 * w), (py::ssize_t)(sizeof(float) * 4),
+                                       (py::ssize_t)sizeof(float)},
+                                      rb.get_rgba(), self);
+        });
+
+    py::class_<RenderRequest, std::shared_ptr<RenderRequest>>(m, "RenderRequest")
+        .def(py::init([](const Camera& cam, std::vector<NeRFRenderable> nerfs, RenderTarget* output,
+                         const RenderModifiers& mods, RenderFlags flags, OnCompleteCallback oc, OnProgressCallback op,
+                         OnCancelCallback ocn) {
+                 return std::make_shared<RenderRequest>(cam, std::move(nerfs), output, mods, flags, oc, op, ocn);
+             }),
+             py::arg("camera"), py::arg("nerfs"), py::arg("output"), py::arg("modifiers") = RenderModifiers(),
+             py::arg("flags") = RenderFlags::Final, py::arg("on_complete") = nullptr, py::arg("on_progress") = nullptr,
+             py::arg("on_cancel") = nullptr, py::keep_alive<1, 4>())
+        .def("cancel", &RenderRequest::cancel);
+
+    py::class_<NeRFRenderingController, std::shared_ptr<NeRFRenderingController>>(m, "Renderer")
+        .def(py::init<RenderPattern, uint32_t>(), py::arg("pattern") = RenderPattern::LinearChunks,
+             py::arg("batch_size") = 0)
+        .def("submit", &NeRFRenderingController::submit, py::arg("request"))
+        .def("cancel", &NeRFRenderingController::cancel)
+        .def_readwrite("min_step_size", &NeRFRenderingController::min_step_size)
+        .def_readwrite("max_samples_per_ray", &NeRFRenderingController::max_samples_per_ray)
+        .def_readonly("last_sample_count", &NeRFRenderingController::last_sample_count);
+}
+
+}  // namespace turbo

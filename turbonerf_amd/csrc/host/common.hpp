@@ -1,0 +1,138 @@
+// Host-side infrastructure for the TurboNeRF MI355X C++ layer: error
+// handling, RAII device/pinned buffers and the single device stream.
+// Errors surface as std::runtime_error, as the reference's CUDA_CHECK_THROW
+// and trainer exceptions do (reference src/common.h:28-36).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "turbonerf/tn_c_api.h"
+
+namespace turbo {
+
+inline const char* tn_status_name(int rc) {
+    switch (rc) {
+        case TN_OK: return "TN_OK";
+        case TN_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case TN_ERR_LAUNCH: return "kernel launch failed";
+        case TN_ERR_UNSUPPORTED: return "unsupported configuration";
+        case TN_ERR_OUT_OF_MEMORY: return "workspace too small";
+        case TN_ERR_NO_RAYS: return "No rays were generated for this training batch!";
+        case TN_ERR_NO_SAMPLES: return "No samples were generated for this training batch!";
+        default: return "unknown error";
+    }
+}
+
+#define TN_HIP(expr)                                                                                       \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            throw std::runtime_error(std::string(#expr) + " failed: " + hipGetErrorString(e_));            \
+    } while (0)
+
+#define TN_TRY(expr)                                                                                       \
+    do {                                                                                                   \
+        int rc_ = (expr);                                                                                  \
+        if (rc_ != TN_OK) throw std::runtime_error(std::string(#expr) + ": " + ::turbo::tn_status_name(rc_)); \
+    } while (0)
+
+inline hipStream_t as_hip(tn_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+inline tn_stream_t as_tn(hipStream_t s) { return reinterpret_cast<tn_stream_t>(s); }
+
+// Owning device allocation of n elements of T (hipMalloc / hipFree).
+template <typename T>
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t n) { resize(n); }
+    ~DeviceBuffer() { release(); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), n_(o.n_) { o.ptr_ = nullptr; o.n_ = 0; }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+        if (this != &o) {
+            release();
+            ptr_ = o.ptr_; n_ = o.n_;
+            o.ptr_ = nullptr; o.n_ = 0;
+        }
+        return *this;
+    }
+
+    // (re)allocate to exactly n elements; contents are not preserved
+    void resize(size_t n) {
+        if (n == n_) return;
+        release();
+        if (n > 0) {
+            void* p = nullptr;
+            hipError_t e = hipMalloc(&p, n * sizeof(T));
+            if (e != hipSuccess)
+                throw std::runtime_error("hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
+                                         hipGetErrorString(e));
+            ptr_ = static_cast<T*>(p);
+            n_ = n;
+        }
+    }
+    // grow only
+    void reserve(size_t n) { if (n > n_) resize(n); }
+    void release() {
+        if (ptr_) (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        n_ = 0;
+    }
+    void zero(hipStream_t s) { if (n_) TN_HIP(hipMemsetAsync(ptr_, 0, n_ * sizeof(T), s)); }
+    void fill_bytes(int v, hipStream_t s) { if (n_) TN_HIP(hipMemsetAsync(ptr_, v, n_ * sizeof(T), s)); }
+
+    T* data() const { return ptr_; }
+    size_t size() const { return n_; }
+    size_t bytes() const { return n_ * sizeof(T); }
+
+private:
+    T* ptr_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Pinned host allocation (hipHostMalloc) for asynchronous device-to-host reads.
+template <typename T>
+class PinnedBuffer {
+public:
+    explicit PinnedBuffer(size_t n = 1) {
+        void* p = nullptr;
+        TN_HIP(hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault));
+        ptr_ = static_cast<T*>(p);
+        n_ = n;
+    }
+    ~PinnedBuffer() { if (ptr_) (void)hipHostFree(ptr_); }
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    T* data() const { return ptr_; }
+    T& operator[](size_t i) { return ptr_[i]; }
+
+private:
+    T* ptr_ = nullptr;
+    size_t n_ = 0;
+};
+
+// One device and one stream (reference DeviceManager,
+// src/services/device-manager.cuh:9-113, which also fixes the device count to 1).
+class DeviceManager {
+public:
+    static int get_device_count() { return 1; }
+    static hipStream_t stream() {
+        static hipStream_t s = [] {
+            hipStream_t st;
+            TN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            return st;
+        }();
+        return s;
+    }
+    static void synchronize() { TN_HIP(hipStreamSynchronize(stream())); }
+    static void teardown() { TN_HIP(hipDeviceSynchronize()); }
+};
+
+}  // namespace turbo

@@ -1,0 +1,216 @@
+// NeRFManager and NeRFTrainingController (see controllers.hpp).
+#include "controllers.hpp"
+
+#include <algorithm>
+
+namespace turbo {
+
+NeRFProxy* NeRFManager::create() {
+    for (size_t i = 0; i < proxies_.size(); ++i) {
+        if (!proxies_[i] || !proxies_[i]->is_valid) {
+            proxies_[i] = std::make_unique<NeRFProxy>();
+            proxies_[i]->id = (int)i;
+            proxies_[i]->is_valid = true;
+            return proxies_[i].get();
+        }
+    }
+    throw std::runtime_error("NeRFManager: no free NeRF slots");
+}
+
+NeRFProxy* NeRFManager::clone(NeRFProxy* src) {
+    if (!src) throw std::runtime_error("NeRFManager::clone: null NeRF");
+    NeRFProxy* p = create();
+    // the dataset is not cloned; the trained state is copied (reference nerf-manager.cuh:89-121)
+    p->render_bbox = src->render_bbox;
+    p->training_bbox = src->training_bbox;
+    p->transform = src->transform;
+    p->is_visible = src->is_visible;
+    p->can_render = src->can_render;
+    p->training_step = src->training_step;
+    p->clone_source = src->clone_source ? src->clone_source : src;
+    const hipStream_t s = DeviceManager::stream();
+    NeRF& d = p->nerfs[0];
+    const NeRF& o = src->nerfs[0];
+    d.network.config = o.network.config;
+    d.network.update_aabb_scale_if_needed(o.network.aabb_scale);
+    auto copy = [&](auto& dst, const auto& from) {
+        dst.resize(from.size());
+        if (from.size()) TN_HIP(hipMemcpyAsync(dst.data(), from.data(), from.bytes(), hipMemcpyDeviceToDevice, s));
+    };
+    d.params.n_grid = o.params.n_grid;
+    d.params.n_total = o.params.n_total;
+    copy(d.params.params_fp, o.params.params_fp);
+    copy(d.params.params_lp, o.params.params_lp);
+    d.occupancy_grid.n_levels = o.occupancy_grid.n_levels;
+    d.occupancy_grid.resolution = o.occupancy_grid.resolution;
+    copy(d.occupancy_grid.bitfield, o.occupancy_grid.bitfield);
+    copy(d.occupancy_grid.density, o.occupancy_grid.density);
+    TN_HIP(hipStreamSynchronize(s));
+    return p;
+}
+
+void NeRFManager::destroy(NeRFProxy* p) {
+    for (auto& q : proxies_) {
+        if (q.get() == p) {
+            q->free_device_memory();
+            q->is_valid = false;
+            return;
+        }
+    }
+}
+
+std::vector<NeRFProxy*> NeRFManager::get_proxies() {
+    std::vector<NeRFProxy*> v;
+    for (auto& q : proxies_)
+        if (q && q->is_valid) v.push_back(q.get());
+    return v;
+}
+
+// ------------------------------------------------------------------ training
+NeRFTrainingController::NeRFTrainingController(NeRFProxy* p, TrainingConfig cfg) : proxy(p), config(cfg) {
+    if (!proxy) throw std::runtime_error("NeRFTrainingController: null NeRF");
+}
+
+NeRFTrainingController::~NeRFTrainingController() = default;
+
+void NeRFTrainingController::setup_data(uint32_t batch_size) {
+    if (!proxy->dataset) throw std::runtime_error("setup_data: the NeRF has no dataset attached");
+    const hipStream_t s = DeviceManager::stream();
+    NeRF& nerf = proxy->nerfs[0];
+    const Dataset& ds = *proxy->dataset;
+    if (!ctx_) ctx_ = std::make_unique<Trainer::Context>(s, &ds, &nerf, batch_size);
+    ctx_->dataset = &ds;
+    ctx_->batch_size = batch_size;
+    ctx_->rng_seed = config.rng_seed;
+    ctx_->distortion_lambda = config.distortion_lambda;
+    ctx_->occupancy_threshold = config.occupancy_threshold;
+    nerf.network.config = config.network;
+    const int aabb = std::max(1, (int)ds.bounding_box.size());
+    nerf.network.update_aabb_scale_if_needed(aabb);
+    nerf.occupancy_grid.set_aabb_scale(ds.bounding_box.size());
+    nerf.occupancy_grid.initialize_if_needed(s);
+    const uint32_t R = config.ray_capacity ? config.ray_capacity : batch_size;
+    ctx_->workspace.enlarge(batch_size, R);
+    const uint32_t init = std::min(ctx_->initial_rays, R);
+    TN_HIP(hipMemcpyAsync(ctx_->workspace.n_rays_next.data(), &init, sizeof(init), hipMemcpyHostToDevice, s));
+    nerf.network.workspace.enlarge(batch_size, R, config.network.precision);
+    nerf.network.update_params_if_needed(s, nerf.params);
+    proxy->training_step = 0;
+    TN_HIP(hipStreamSynchronize(s));
+}
+
+void NeRFTrainingController::teardown() {
+    ctx_.reset();
+    alpha_selection_probability = 1.0f;
+    alpha_selection_threshold = 1.0f;
+    min_step_size = NeRFConstants::min_step_size;
+    use_distortion_loss = false;
+}
+
+void NeRFTrainingController::reset_training() {
+    NeRF& nerf = proxy->nerfs[0];
+    nerf.params.free_allocations();
+    nerf.occupancy_grid.free_device_memory();
+    const uint32_t B = ctx_ ? ctx_->batch_size : NeRFConstants::batch_size;
+    ctx_.reset();
+    setup_data(B);
+}
+
+void NeRFTrainingController::load_images(std::function<void(int, int)> on_image_loaded) {
+    if (!proxy->dataset) throw std::runtime_error("load_images: no dataset");
+    NeRF& nerf = proxy->nerfs[0];
+    nerf.dataset_ws.upload(*proxy->dataset, DeviceManager::stream());
+    nerf.is_image_data_loaded = true;
+    proxy->is_dataset_dirty = false;
+    if (on_image_loaded) {
+        const int n = (int)proxy->dataset->n_images();
+        for (int i = 1; i <= n; ++i) on_image_loaded(i, n);
+    }
+}
+
+void NeRFTrainingController::sync_settings() {
+    if (!ctx_) throw std::runtime_error("train_step: call setup_data() first");
+    if (!proxy->nerfs[0].is_image_data_loaded) throw std::runtime_error("train_step: call load_images() first");
+    ctx_->alpha_selection_probability = alpha_selection_probability;
+    ctx_->alpha_selection_threshold = alpha_selection_threshold;
+    ctx_->min_step_size = min_step_size;
+    ctx_->use_distortion_loss = use_distortion_loss;
+}
+
+NeRFTrainingController::TrainingMetrics NeRFTrainingController::train_step() {
+    sync_settings();
+    const float loss = trainer_.train_step(*ctx_, proxy->training_step);
+    proxy->training_step++;
+    TrainingMetrics m;
+    m.loss = loss;
+    m.step = proxy->training_step;
+    m.n_rays = ctx_->n_rays_in_batch;
+    m.n_samples = ctx_->n_samples_in_batch;
+    if (!proxy->can_render && proxy->training_step >= 1) proxy->can_render = true;
+    return m;
+}
+
+NeRFTrainingController::TrainingMetrics NeRFTrainingController::train_steps(uint32_t n, bool read) {
+    sync_settings();
+    for (uint32_t k = 0; k < n; ++k) {
+        trainer_.enqueue_train_step(*ctx_, proxy->training_step);
+        proxy->training_step++;
+    }
+    TrainingMetrics m;
+    m.step = proxy->training_step;
+    if (read) {
+        trainer_.read_metrics(*ctx_, m.loss);
+        m.n_rays = ctx_->n_rays_in_batch;
+        m.n_samples = ctx_->n_samples_in_batch;
+    }
+    if (proxy->training_step >= 1) proxy->can_render = true;
+    return m;
+}
+
+NeRFTrainingController::OccupancyGridMetrics NeRFTrainingController::update_occupancy_grid(uint32_t training_step) {
+    sync_settings();
+    OccupancyGridMetrics m;
+    m.n_occupied = trainer_.update_occupancy_grid(*ctx_, training_step);
+    m.n_total = proxy->nerfs[0].occupancy_grid.get_n_total_elements();
+    return m;
+}
+
+void NeRFTrainingController::set_occupancy_from_density(const float* density_host, float threshold) {
+    NeRF& nerf = proxy->nerfs[0];
+    OccupancyGrid& og = nerf.occupancy_grid;
+    const hipStream_t s = DeviceManager::stream();
+    og.initialize_if_needed(s);
+    TN_HIP(hipMemcpyAsync(og.density.data(), density_host, og.density.bytes(), hipMemcpyHostToDevice, s));
+    TN_TRY(tn_occ_update_bits(og.view(), threshold, as_tn(s)));
+    TN_HIP(hipStreamSynchronize(s));
+}
+
+std::vector<float> NeRFTrainingController::get_params() const {
+    const NeRF& nerf = proxy->nerfs[0];
+    std::vector<float> h(nerf.params.params_fp.size());
+    TN_HIP(hipMemcpy(h.data(), nerf.params.params_fp.data(), h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    return h;
+}
+
+void NeRFTrainingController::set_params(const std::vector<float>& h) {
+    NeRF& nerf = proxy->nerfs[0];
+    if (h.size() != nerf.params.params_fp.size()) throw std::runtime_error("set_params: size mismatch");
+    const hipStream_t s = DeviceManager::stream();
+    TN_HIP(hipMemcpyAsync(nerf.params.params_fp.data(), h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    if (nerf.params.params_lp.size())
+        TN_TRY(tn_cast_params(h.size(), nerf.params.params_fp.data(), 1, nerf.params.params_lp.data(), as_tn(s)));
+    TN_HIP(hipStreamSynchronize(s));
+}
+
+std::vector<size_t> NeRFTrainingController::get_cuda_memory_allocated() const {
+    size_t total = 0;
+    if (ctx_) total += ctx_->workspace.bytes();
+    const NeRF& nerf = proxy->nerfs[0];
+    total += nerf.network.workspace.bytes();
+    total += nerf.params.params_fp.bytes() + nerf.params.params_lp.bytes() + nerf.params.gradients.bytes() +
+             nerf.params.m1.bytes() + nerf.params.m2.bytes() + nerf.params.steps.bytes();
+    total += nerf.occupancy_grid.bitfield.bytes() + nerf.occupancy_grid.density.bytes();
+    return {total};
+}
+
+}  // namespace turbo

@@ -1,0 +1,117 @@
+// Public C++ API for training (and the NeRF slot manager), mirroring the
+// reference's NeRFTrainingController (src/controllers/nerf-training-controller.h:23-89),
+// NeRFProxy (src/models/nerf-proxy.cuh:20-141) and NeRFManager
+// (src/services/nerf-manager.cuh:27-183).
+#pragma once
+
+#include <any>
+#include <functional>
+#include <map>
+#include <memory>
+#include <optional>
+#include <string>
+#include <vector>
+
+#include "core.hpp"
+
+namespace turbo {
+
+struct NeRFProxy {
+    std::vector<NeRF> nerfs;  // one per device (one device here)
+    std::optional<Dataset> dataset;
+    UpdatableProperty<BoundingBox> render_bbox;
+    UpdatableProperty<BoundingBox> training_bbox;
+    UpdatableProperty<Transform4f> transform;
+    bool is_visible = true;
+    bool is_dataset_dirty = true;
+    bool can_render = false;
+    uint32_t training_step = 0;
+    int id = -1;
+    bool is_valid = false;
+    NeRFProxy* clone_source = nullptr;
+
+    NeRFProxy() : nerfs(1) {}
+    NeRFProxy(const NeRFProxy&) = delete;
+    NeRFProxy& operator=(const NeRFProxy&) = delete;
+    void attach_dataset(const Dataset& ds) {
+        dataset = ds;
+        training_bbox.set(ds.bounding_box);
+        render_bbox.set(ds.bounding_box);
+        is_dataset_dirty = true;
+    }
+    void detach_dataset() { dataset.reset(); }
+    bool can_train() const { return dataset.has_value(); }
+    bool is_image_data_loaded() const { return !nerfs.empty() && nerfs[0].is_image_data_loaded; }
+    bool is_dirty() const { return is_dataset_dirty || training_bbox.is_dirty() || transform.is_dirty(); }
+    void free_device_memory() { for (auto& n : nerfs) n.free_device_memory(); }
+};
+
+// Fixed slot pool handing out stable raw pointers (reference nerf-manager.cuh:49-51)
+class NeRFManager {
+public:
+    NeRFManager() : proxies_(NeRFConstants::n_max_nerfs) {}
+    NeRFProxy* create();
+    NeRFProxy* clone(NeRFProxy* src);
+    void destroy(NeRFProxy* p);
+    std::vector<NeRFProxy*> get_proxies();
+
+private:
+    std::vector<std::unique_ptr<NeRFProxy>> proxies_;
+};
+
+// Runtime configuration beyond the reference's public fields (the reference
+// fixes these at compile time; the benchmark configurations change them).
+struct TrainingConfig {
+    NetworkConfig network;
+    uint64_t rng_seed = 0x5eed;
+    float distortion_lambda = NeRFConstants::mipNeRF360_distortion_loss_lambda;
+    float occupancy_threshold = NeRFConstants::occupancy_threshold;
+    uint32_t ray_capacity = 0;  // 0 = batch_size, as the reference sizes ray buffers
+};
+
+struct NeRFTrainingController {
+    struct TrainingMetrics {
+        uint32_t step = 0, n_rays = 0, n_samples = 0;
+        float loss = 0.0f;
+        std::map<std::string, std::any> as_map() const {
+            return {{"step", step}, {"n_rays", n_rays}, {"n_samples", n_samples}, {"loss", loss}};
+        }
+    };
+    struct OccupancyGridMetrics {
+        uint32_t n_occupied = 0, n_total = 0;
+        std::map<std::string, std::any> as_map() const { return {{"n_occupied", n_occupied}, {"n_total", n_total}}; }
+    };
+
+    explicit NeRFTrainingController(NeRFProxy* proxy, TrainingConfig cfg = TrainingConfig());
+    ~NeRFTrainingController();
+
+    void setup_data(uint32_t batch_size = NeRFConstants::batch_size);
+    void teardown();
+    void reset_training();
+    void load_images(std::function<void(int, int)> on_image_loaded = {});
+    TrainingMetrics train_step();
+    // Enqueue n steps without host synchronisation; metrics of the last one.
+    TrainingMetrics train_steps(uint32_t n_steps, bool read_metrics = true);
+    OccupancyGridMetrics update_occupancy_grid(uint32_t training_step);
+    std::vector<size_t> get_cuda_memory_allocated() const;
+
+    // occupancy bitfield from an external density grid (synthetic scenes)
+    void set_occupancy_from_density(const float* density_host, float threshold);
+    // raw parameter access (fp32 master weights), for tests and snapshots
+    std::vector<float> get_params() const;
+    void set_params(const std::vector<float>& p);
+
+    float alpha_selection_threshold = 1.0f;
+    float alpha_selection_probability = 1.0f;
+    float min_step_size = NeRFConstants::min_step_size;
+    bool use_distortion_loss = false;
+    NeRFProxy* proxy = nullptr;
+    TrainingConfig config;
+
+private:
+    std::unique_ptr<Trainer::Context> ctx_;
+    Trainer trainer_;
+    void sync_settings();
+};
+
+}  // namespace turbo

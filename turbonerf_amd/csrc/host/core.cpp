@@ -1,0 +1,309 @@
+// Core engines: OccupancyGrid, NerfNetwork, Trainer (see core.hpp).
+#include "core.hpp"
+
+#include <algorithm>
+#include <random>
+
+namespace turbo {
+
+// ------------------------------------------------------------------ grid
+uint32_t OccupancyGrid::count_occupied(hipStream_t s) {
+    const size_t nb = tn_popcount_workspace_bytes(volume());
+    workspace.reserve(std::max<size_t>(nb, 4));
+    count.reserve(1);
+    TN_TRY(tn_popcount(bitfield.data(), volume(), count.data(), workspace.data(), workspace.size(), as_tn(s)));
+    uint32_t h = 0;
+    TN_HIP(hipMemcpyAsync(&h, count.data(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    TN_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+// ------------------------------------------------------------------ network
+void NetworkWorkspace::enlarge(uint32_t B, uint32_t R, int prec) {
+    if (B <= batch_size && R <= ray_capacity && prec == precision) return;
+    batch_size = std::max(B, batch_size);
+    ray_capacity = std::max(R, ray_capacity);
+    precision = prec;
+    const size_t e = prec == 1 ? 2 : 4;
+    features.resize(32 * (size_t)batch_size * e);
+    density.resize((size_t)batch_size * e);
+    rgb.resize(3 * (size_t)batch_size * e);
+    dL_drgb.resize(3 * (size_t)batch_size * e);
+    dfeat.resize(32 * (size_t)batch_size * e);
+    dL_ds.resize(batch_size);
+    dL_dist.resize(batch_size);
+    ray_loss.resize(4 * (size_t)ray_capacity);
+    ray_dist_loss.resize(ray_capacity);
+    loss_out.resize(2);
+    composite_scratch.resize(tn_composite_scratch_bytes(ray_capacity));
+    distortion_scratch.resize(tn_distortion_scratch_bytes(ray_capacity));
+    mlp_ws.resize(std::max<size_t>(tn_mlp_backward_workspace_bytes(batch_size), 4));
+}
+
+size_t NetworkWorkspace::bytes() const {
+    return features.bytes() + density.bytes() + rgb.bytes() + dL_drgb.bytes() + dfeat.bytes() + dL_ds.bytes() +
+           dL_dist.bytes() + ray_loss.bytes() + ray_dist_loss.bytes() + composite_scratch.bytes() +
+           distortion_scratch.bytes() + mlp_ws.bytes();
+}
+
+void NerfNetwork::update_aabb_scale_if_needed(int aabb) {
+    if (aabb == aabb_scale) return;
+    const double n_min = config.base_resolution;
+    const double n_max = (double)config.finest_resolution_per_unit * (double)aabb;
+    const double b = std::exp((std::log(n_max) - std::log(n_min)) / (config.n_levels - 1.0));
+    TN_TRY(tn_hashgrid_layout(config.n_levels, config.n_features, config.log2_hashmap_size, config.base_resolution, b,
+                              &grid));
+    aabb_scale = aabb;
+}
+
+void NerfNetwork::update_params_if_needed(hipStream_t s, NetworkParamsWorkspace& p) {
+    const uint32_t n_net = TN_DENSITY_MLP_PARAMS + TN_COLOR_MLP_PARAMS;
+    const uint32_t total = grid.n_params + n_net;
+    const bool want_lp = config.precision == 1;
+    if (p.n_total == total && p.params_lp.size() == (want_lp ? (size_t)total * 2 : 0)) return;
+    p.n_grid = grid.n_params;
+    p.n_total = total;
+    // seeded initialisation: hash entries uniform in +-hash_init_scale, MLP
+    // weights Xavier-uniform per matrix
+    std::vector<float> h(total);
+    std::mt19937_64 rng(config.seed);
+    std::uniform_real_distribution<float> u(-1.0f, 1.0f);
+    for (uint32_t i = 0; i < grid.n_params; ++i) h[i] = config.hash_init_scale * u(rng);
+    const int shapes[5][2] = {{64, 32}, {16, 64}, {64, 32}, {64, 64}, {3, 64}};
+    uint32_t off = grid.n_params;
+    for (auto& sh : shapes) {
+        const float a = std::sqrt(6.0f / (float)(sh[0] + sh[1]));
+        for (int k = 0; k < sh[0] * sh[1]; ++k) h[off + k] = a * u(rng);
+        off += sh[0] * sh[1];
+    }
+    p.params_fp.resize(total);
+    TN_HIP(hipMemcpyAsync(p.params_fp.data(), h.data(), total * sizeof(float), hipMemcpyHostToDevice, s));
+    if (want_lp) {
+        p.params_lp.resize((size_t)total * 2);
+        TN_TRY(tn_cast_params(total, p.params_fp.data(), 1, p.params_lp.data(), as_tn(s)));
+    } else {
+        p.params_lp.release();
+    }
+    p.gradients.resize(total);
+    p.gradients.zero(s);
+    p.m1.resize(total);
+    p.m1.zero(s);
+    p.m2.resize(total);
+    p.m2.zero(s);
+    p.steps.resize(total);
+    p.steps.zero(s);
+    TN_HIP(hipStreamSynchronize(s));  // the host staging vector goes out of scope
+}
+
+void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step, uint32_t B, uint32_t R,
+                        const uint32_t* info, const float* random_rgb, const uint32_t* ray_steps,
+                        const uint32_t* ray_offset, const float* t_batch, const float* pos_batch,
+                        const float* dir_batch, const float* dt_batch, const float* m_norm_batch,
+                        const float* dt_norm_batch, const float* target_rgba, const Settings& st) {
+    const int prec = config.precision;
+    workspace.enlarge(B, R, prec);
+    NetworkWorkspace& w = workspace;
+    const void* params = params_for_kernels(p);
+    const tn_stream_t ts = as_tn(s);
+    // forward: encoding, density + colour MLPs (reference :431-545)
+    TN_TRY(tn_hashgrid_forward(prec, prec, B, info, B, B, grid, pos_batch, params, w.features.data(), ts));
+    TN_TRY(tn_mlp_forward(prec, B, info, B, B, B, params, p.n_grid, w.features.data(), dir_batch, 1,
+                          w.density.data(), w.rgb.data(), ts));
+    // distortion loss (reference :273-283)
+    if (st.use_distortion_loss)
+        TN_TRY(tn_distortion(prec, info, R, B, st.distortion_lambda, ray_steps, ray_offset, w.density.data(),
+                             m_norm_batch, dt_norm_batch, w.ray_dist_loss.data(), w.dL_dist.data(),
+                             w.distortion_scratch.data(), w.distortion_scratch.size(), ts));
+    // compositing + loss + its backward (reference :547-603)
+    TN_TRY(tn_composite_train(prec, info, R, R, B, config.loss_scale(), st.min_transmittance, ray_steps, ray_offset,
+                              random_rgb, target_rgba, w.density.data(), w.rgb.data(), dt_batch, w.ray_loss.data(),
+                              w.dL_ds.data(), w.dL_drgb.data(), w.composite_scratch.data(), w.composite_scratch.size(),
+                              ts));
+    // network backward (reference :668-768)
+    TN_TRY(tn_mlp_backward(prec, B, info, B, B, params, p.n_grid, w.features.data(), dir_batch, t_batch,
+                           w.dL_drgb.data(), B, w.dL_ds.data(), st.use_distortion_loss ? w.dL_dist.data() : nullptr,
+                           config.loss_scale(), w.dfeat.data(), p.gradients.data() + p.n_grid, 0, w.mlp_ws.data(),
+                           w.mlp_ws.size(), ts));
+    TN_TRY(tn_hashgrid_backward(prec, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(), ts));
+    // optimiser (reference :770-781, lr schedule :256)
+    const float lr = config.learning_rate / (1.0f + config.learning_rate_decay * (float)step);
+    TN_TRY(tn_adam_step(p.n_total, p.n_grid, p.n_total, config.loss_scale(), lr, config.beta1, config.beta2,
+                        config.epsilon, config.l2_reg, p.params_fp.data(), prec, prec == 1 ? p.params_lp.data() : nullptr,
+                        p.gradients.data(), p.m1.data(), p.m2.data(), p.steps.data(), ts));
+    // losses: mean over rays of the four smooth-L1 channels, mean distortion
+    TN_TRY(tn_reduce_loss(info, R, 4, w.ray_loss.data(), w.loss_out.data(), ts));
+    if (st.use_distortion_loss) TN_TRY(tn_reduce_loss(info, R, 1, w.ray_dist_loss.data(), w.loss_out.data() + 1, ts));
+    else TN_HIP(hipMemsetAsync(w.loss_out.data() + 1, 0, sizeof(float), s));
+}
+
+void NerfNetwork::inference(hipStream_t s, const NetworkParamsWorkspace& p, uint32_t n, const uint32_t* info,
+                            uint32_t stride, const float* pos_batch, const float* dir_batch, void* features,
+                            void* density_out, void* rgb_out, bool use_color_network) {
+    const int prec = config.precision;
+    const void* params = params_for_kernels(p);
+    TN_TRY(tn_hashgrid_forward(prec, prec, n, info, stride, stride, grid, pos_batch, params, features, as_tn(s)));
+    TN_TRY(tn_mlp_forward(prec, n, info, stride, stride, stride, params, p.n_grid, features, dir_batch,
+                          use_color_network ? 1 : 0, density_out, rgb_out, as_tn(s)));
+}
+
+// ------------------------------------------------------------------ workspaces
+void TrainingWorkspace::enlarge(uint32_t B, uint32_t R) {
+    if (B == batch_size && R == ray_capacity) return;
+    batch_size = B;
+    ray_capacity = R;
+    random_float.resize(4 * (size_t)R);
+    pix_rgba.resize(4 * (size_t)R);
+    ray_origin.resize(3 * (size_t)R);
+    ray_dir.resize(3 * (size_t)R);
+    ray_t.resize(R);
+    ray_t_max.resize(R);
+    ray_img_id.resize(R);
+    ray_step.resize(R);
+    ray_offset.resize(R);
+    ray_alive.resize(R);
+    sample_img_id.resize(B);
+    sample_t.resize(B);
+    sample_pos.resize(3 * (size_t)B);
+    sample_dir.resize(3 * (size_t)B);
+    sample_dt.resize(B);
+    sample_m_norm.resize(B);
+    sample_dt_norm.resize(B);
+    info.resize(4);
+    n_rays_next.resize(1);
+    scan_ws.resize(std::max<size_t>(tn_scan_workspace_bytes(R), 4));
+}
+
+size_t TrainingWorkspace::bytes() const {
+    return random_float.bytes() + pix_rgba.bytes() + ray_origin.bytes() + ray_dir.bytes() + ray_t.bytes() +
+           ray_t_max.bytes() + ray_img_id.bytes() + ray_step.bytes() + ray_offset.bytes() + ray_alive.bytes() +
+           sample_img_id.bytes() + sample_t.bytes() + sample_pos.bytes() + sample_dir.bytes() + sample_dt.bytes() +
+           sample_m_norm.bytes() + sample_dt_norm.bytes() + scan_ws.bytes();
+}
+
+void TrainingWorkspace::free_allocations() { *this = TrainingWorkspace(); }
+
+void DatasetWorkspace::upload(const Dataset& ds, hipStream_t s) {
+    n_images = (uint32_t)ds.n_images();
+    image_dims = ds.image_dimensions;
+    n_pixels_per_image = ds.n_pixels_per_image;
+    std::vector<tn_camera_t> cams(n_images);
+    for (uint32_t i = 0; i < n_images; ++i) cams[i] = ds.cameras[i].c();
+    cameras.resize(n_images * sizeof(tn_camera_t));
+    TN_HIP(hipMemcpyAsync(cameras.data(), cams.data(), cameras.size(), hipMemcpyHostToDevice, s));
+    images.resize(ds.image_data.size());
+    TN_HIP(hipMemcpyAsync(images.data(), ds.image_data.data(), images.size(), hipMemcpyHostToDevice, s));
+    const tn_bbox_t b = ds.bounding_box.c();
+    bbox.resize(sizeof(b));
+    TN_HIP(hipMemcpyAsync(bbox.data(), &b, sizeof(b), hipMemcpyHostToDevice, s));
+    TN_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------------ trainer
+void Trainer::generate_next_training_batch(Context& ctx) {
+    TrainingWorkspace& ws = ctx.workspace;
+    NeRF& nerf = *ctx.nerf;
+    const hipStream_t s = ctx.stream;
+    const tn_stream_t ts = as_tn(s);
+    const uint32_t R = ws.ray_capacity, B = ws.batch_size;
+    const Dataset& ds = *ctx.dataset;
+    // seeded uniforms: slice 0 picks pixels, slices 1-3 are the background
+    // colour (reference trainer.cu:45-50,332)
+    TN_TRY(tn_fill_uniform(4 * (uint64_t)R, ctx.rng_seed, ctx.rng_offset, ws.random_float.data(), ts));
+    ctx.rng_offset += 4 * (uint64_t)R;
+    // rays are generated for the device-side estimate of how many fill the
+    // batch and split evenly over the images by that count (-1 below)
+    const DatasetWorkspace& dw = nerf.dataset_ws;
+    TN_TRY(tn_init_training_rays(R, ws.n_rays_next.data(), R, dw.n_images, dw.n_pixels_per_image, dw.image_dims.x,
+                                 dw.image_dims.y, -1.0f,
+                                 reinterpret_cast<const tn_bbox_t*>(dw.bbox.data()),
+                                 reinterpret_cast<const tn_camera_t*>(dw.cameras.data()), dw.images.data(),
+                                 ws.random_float.data(), ws.pix_rgba.data(), ws.ray_img_id.data(), ws.ray_origin.data(),
+                                 ws.ray_dir.data(), ws.ray_t.data(), ws.ray_t_max.data(), ws.ray_alive.data(), ts));
+    if (ctx.alpha_selection_threshold < 1.0f && ctx.alpha_selection_probability < 1.0f)
+        TN_TRY(tn_deactivate_rays(R, ws.n_rays_next.data(), R, ctx.alpha_selection_threshold,
+                                  ctx.alpha_selection_probability, ws.pix_rgba.data(), ws.random_float.data() + R,
+                                  ws.ray_alive.data(), ts));
+    const float dt_min = ctx.min_step_size;
+    const float dt_max = ds.bounding_box.size() * dt_min;
+    const tn_bbox_t bbox = ds.bounding_box.c();
+    const tn_occupancy_grid_t grid = nerf.occupancy_grid.view();
+    TN_TRY(tn_march_count(R, ws.n_rays_next.data(), R, bbox, grid, NeRFConstants::cone_angle, dt_min, dt_max,
+                          0xFFFFFFFFu, ws.ray_dir.data(), ws.ray_t_max.data(), ws.ray_alive.data(), ws.ray_origin.data(),
+                          ws.ray_t.data(), ws.ray_step.data(), ts));
+    TN_TRY(tn_scan_select(0, R, ws.n_rays_next.data(), ws.ray_step.data(), B, ws.ray_offset.data(), ws.info.data(),
+                          ws.scan_ws.data(), ws.scan_ws.size(), ts));
+    TN_TRY(tn_march_generate(R, ws.info.data(), R, B, bbox, 1.0f / ds.bounding_box.size(), grid, dt_min, dt_max,
+                             NeRFConstants::cone_angle, ws.ray_img_id.data(), ws.ray_origin.data(), ws.ray_dir.data(),
+                             ws.ray_t.data(), ws.ray_offset.data(), ws.ray_alive.data(), ws.ray_step.data(),
+                             ws.sample_img_id.data(), ws.sample_t.data(), ws.sample_pos.data(), ws.sample_dir.data(),
+                             ws.sample_dt.data(), ws.sample_m_norm.data(), ws.sample_dt_norm.data(), ts));
+    TN_TRY(tn_estimate_rays(ws.info.data(), B, R, 64, ws.n_rays_next.data(), ts));
+}
+
+void Trainer::enqueue_train_step(Context& ctx, uint32_t training_step) {
+    generate_next_training_batch(ctx);
+    TrainingWorkspace& ws = ctx.workspace;
+    NerfNetwork::Settings st;
+    st.use_distortion_loss = ctx.use_distortion_loss;
+    st.distortion_lambda = ctx.distortion_lambda;
+    ctx.nerf->network.train(ctx.stream, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity,
+                            ws.info.data(), ws.random_float.data() + ws.ray_capacity, ws.ray_step.data(),
+                            ws.ray_offset.data(), ws.sample_t.data(), ws.sample_pos.data(), ws.sample_dir.data(),
+                            ws.sample_dt.data(), ws.sample_m_norm.data(), ws.sample_dt_norm.data(), ws.pix_rgba.data(),
+                            st);
+}
+
+void Trainer::read_metrics(Context& ctx, float& loss) {
+    const hipStream_t s = ctx.stream;
+    TN_HIP(hipMemcpyAsync(ctx.info_host.data(), ctx.workspace.info.data(), 4 * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
+    TN_HIP(hipMemcpyAsync(ctx.loss_host.data(), ctx.nerf->network.workspace.loss_out.data(), 2 * sizeof(float),
+                          hipMemcpyDeviceToHost, s));
+    TN_HIP(hipStreamSynchronize(s));
+    ctx.n_rays_in_batch = ctx.info_host[0];
+    ctx.n_samples_in_batch = ctx.info_host[1];
+    loss = ctx.loss_host[0] + ctx.loss_host[1];
+}
+
+float Trainer::train_step(Context& ctx, uint32_t training_step) {
+    enqueue_train_step(ctx, training_step);
+    float loss = 0.0f;
+    read_metrics(ctx, loss);
+    if (ctx.info_host[2] != 0) throw std::runtime_error(tn_status_name((int)ctx.info_host[2]));
+    return loss;
+}
+
+uint32_t Trainer::update_occupancy_grid(Context& ctx, uint32_t training_step) {
+    NeRF& nerf = *ctx.nerf;
+    OccupancyGrid& og = nerf.occupancy_grid;
+    TrainingWorkspace& ws = ctx.workspace;
+    const hipStream_t s = ctx.stream;
+    const tn_stream_t ts = as_tn(s);
+    const float aabb = ctx.dataset->bounding_box.size();
+    const uint32_t vol = og.volume();
+    const uint32_t B = ws.batch_size;
+    nerf.network.workspace.enlarge(B, ws.ray_capacity, nerf.network.config.precision);
+    NetworkWorkspace& nw = nerf.network.workspace;
+    TN_TRY(tn_occ_decay(og.view(), NeRFConstants::occupancy_decay, ts));
+    // per cascade, per chunk of at most B cells: jittered points, density
+    // inference, max-update (reference trainer.cu:236-295)
+    const uint32_t chunk = std::min(B, ws.ray_capacity);
+    for (int level = 0; level < og.n_levels; ++level) {
+        for (uint32_t start = 0; start < vol; start += chunk) {
+            const uint32_t n = std::min(chunk, vol - start);
+            TN_TRY(tn_fill_uniform(4 * (uint64_t)chunk, ctx.rng_seed, ctx.rng_offset, ws.random_float.data(), ts));
+            ctx.rng_offset += 4 * (uint64_t)chunk;
+            TN_TRY(tn_occ_sample_points(n, chunk, start, og.view(), level, 1.0f / aabb, ws.random_float.data(),
+                                        ws.sample_pos.data(), ts));
+            nerf.network.inference(s, nerf.params, n, nullptr, chunk, ws.sample_pos.data(), nullptr,
+                                   nw.features.data(), nw.density.data(), nullptr, false);
+            TN_TRY(tn_occ_update_density(nerf.network.config.precision, n, start, og.view(), level,
+                                         training_step < 256 ? 1 : 0, ws.random_float.data() + 3 * (size_t)chunk,
+                                         nw.density.data(), ts));
+        }
+    }
+    TN_TRY(tn_occ_update_bits(og.view(), ctx.occupancy_threshold, ts));
+    return og.count_occupied(s);
+}
+
+}  // namespace turbo

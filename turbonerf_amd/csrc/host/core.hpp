@@ -1,0 +1,230 @@
+// Core engines of the training/rendering hot path: OccupancyGrid,
+// NerfNetwork and Trainer, with the reference's class and method names
+// (src/core/occupancy-grid.cuh, nerf-network.cuh, trainer.cuh). Every device
+// computation is a call into the tn_* kernel library.
+#pragma once
+
+#include <cmath>
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "models.hpp"
+
+namespace turbo {
+
+// reference src/utils/nerf-constants.cuh
+namespace NeRFConstants {
+constexpr float min_step_size = 0.00169145586f;  // sqrt(3)/1024
+constexpr float cone_angle = 1.0f / 256.0f;
+constexpr uint32_t batch_size = 2 << 21;
+constexpr float min_transmittance = 1e-4f;
+constexpr float occupancy_decay = 0.95f;
+constexpr float mipNeRF360_distortion_loss_lambda = 0.01f;
+constexpr float occupancy_threshold = 0.01f * min_step_size;
+constexpr uint32_t n_steps_per_render_compaction = 64;
+constexpr float learning_rate_decay = 3.3e-5f;
+constexpr uint32_t n_max_nerfs = 2 << 10;
+}  // namespace NeRFConstants
+
+// Multi-cascade occupancy grid: Morton-ordered bitfield (bit k = cascade k)
+// and per-cascade fp32 densities (reference src/core/occupancy-grid.cuh:22-287).
+struct OccupancyGrid {
+    int n_levels = 1;
+    int resolution = 128;
+    DeviceBuffer<uint8_t> bitfield;
+    DeviceBuffer<float> density;
+    DeviceBuffer<uint8_t> workspace;  // popcount partials
+    DeviceBuffer<uint32_t> count;
+
+    static int get_max_n_levels(float aabb_size) { return (int)std::log2(aabb_size) + 1; }
+    uint32_t volume() const { return (uint32_t)resolution * resolution * resolution; }
+    void set_aabb_scale(float aabb) { n_levels = get_max_n_levels(aabb); }
+
+    // allocate and set every bit (all cells occupied), densities zero
+    void initialize_if_needed(hipStream_t s) {
+        const size_t vol = volume();
+        if (bitfield.size() == vol && density.size() == vol * (size_t)n_levels) return;
+        bitfield.resize(vol);
+        density.resize(vol * (size_t)n_levels);
+        bitfield.fill_bytes(0xFF, s);
+        density.zero(s);
+    }
+    void set_bitfield(int value, hipStream_t s) { bitfield.fill_bytes(value, s); }
+    tn_occupancy_grid_t view() const { return tn_occupancy_grid_t{bitfield.data(), density.data(), n_levels, resolution}; }
+    uint32_t get_n_total_elements() const { return (uint32_t)n_levels * volume(); }
+    // number of set bits (device popcount), synchronises the stream
+    uint32_t count_occupied(hipStream_t s);
+    void free_device_memory() { bitfield.release(); density.release(); workspace.release(); count.release(); }
+};
+
+// Network and optimiser configuration. Defaults follow the reference
+// (src/core/nerf-network.cu:86-121,175-189); beta2 defaults to the tcnn
+// value the reference leaves in place (0.999).
+struct NetworkConfig {
+    int precision = 1;  // 0 = plain fp32, 1 = fp16 MFMA with fp32 accumulation and master weights
+    int n_levels = 16;
+    int n_features = 2;
+    int log2_hashmap_size = 19;
+    int base_resolution = 16;
+    float finest_resolution_per_unit = 2048.0f;  // N_max = 2048 * aabb
+    float hash_init_scale = 1e-4f;
+    uint64_t seed = 72791;
+    float learning_rate = 1e-2f;
+    float learning_rate_decay = NeRFConstants::learning_rate_decay;
+    float beta1 = 0.9f;
+    float beta2 = 0.999f;
+    float epsilon = 1e-15f;
+    float l2_reg = 1e-6f;
+    float loss_scale() const { return precision == 1 ? 1024.0f : 1.0f; }
+};
+
+// Flat parameters (hash table, density MLP, colour MLP), fp32 master copy,
+// optional fp16 copy, fp32 gradients and Adam state
+// (reference src/workspaces/network-params-workspace.cuh,
+//  src/core/adam-optimizer.cuh).
+struct NetworkParamsWorkspace {
+    uint32_t n_grid = 0, n_total = 0;
+    DeviceBuffer<float> params_fp;
+    DeviceBuffer<uint8_t> params_lp;  // fp16 copy when training in mixed precision
+    DeviceBuffer<float> gradients;
+    DeviceBuffer<float> m1, m2;
+    DeviceBuffer<uint32_t> steps;
+    void free_allocations() {
+        params_fp.release(); params_lp.release(); gradients.release(); m1.release(); m2.release(); steps.release();
+        n_total = 0;
+    }
+};
+
+// Per-batch buffers of the network pass (reference NetworkWorkspace,
+// src/workspaces/network-workspace.cuh:43-95).
+struct NetworkWorkspace {
+    uint32_t batch_size = 0, ray_capacity = 0;
+    int precision = -1;
+    DeviceBuffer<uint8_t> features, density, rgb, dL_drgb, dfeat;  // precision-typed
+    DeviceBuffer<float> dL_ds, dL_dist, ray_loss, ray_dist_loss, loss_out;
+    DeviceBuffer<uint8_t> composite_scratch, distortion_scratch, mlp_ws;
+    void enlarge(uint32_t B, uint32_t R, int precision);
+    size_t bytes() const;
+};
+
+struct NerfNetwork {
+    struct Settings {
+        bool use_distortion_loss = false;
+        float distortion_lambda = NeRFConstants::mipNeRF360_distortion_loss_lambda;
+        float min_transmittance = NeRFConstants::min_transmittance;
+    };
+
+    NetworkConfig config;
+    tn_hashgrid_t grid{};
+    int aabb_scale = 0;
+    NetworkWorkspace workspace;
+
+    // (re)build the encoding layout for a scene of side aabb (reference
+    // update_aabb_scale_if_needed, src/core/nerf-network.cu:77-124)
+    void update_aabb_scale_if_needed(int aabb);
+    // allocate and seed parameters, Adam state (reference :127-190)
+    void update_params_if_needed(hipStream_t s, NetworkParamsWorkspace& p);
+    const void* params_for_kernels(const NetworkParamsWorkspace& p) const {
+        return config.precision == 1 ? (const void*)p.params_lp.data() : (const void*)p.params_fp.data();
+    }
+
+    // One training step over a batch already generated on the device. The
+    // losses (reconstruction, distortion) are left in workspace.loss_out.
+    // Reference NerfNetwork::train (src/core/nerf-network.cu:223-317).
+    void train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step, uint32_t batch_size, uint32_t ray_capacity,
+               const uint32_t* info, const float* random_rgb, const uint32_t* ray_steps, const uint32_t* ray_offset,
+               const float* t_batch, const float* pos_batch, const float* dir_batch, const float* dt_batch,
+               const float* m_norm_batch, const float* dt_norm_batch, const float* target_rgba,
+               const Settings& settings);
+
+    // density (and colour) of n points (reference NerfNetwork::inference,
+    // src/core/nerf-network.cu:319-429); outputs in the network precision
+    void inference(hipStream_t s, const NetworkParamsWorkspace& p, uint32_t n, const uint32_t* info, uint32_t stride,
+                   const float* pos_batch, const float* dir_batch, void* features, void* density_out, void* rgb_out,
+                   bool use_color_network);
+    size_t elem_bytes() const { return config.precision == 1 ? 2 : 4; }
+};
+
+// Per-step training buffers (reference TrainingWorkspace,
+// src/workspaces/training-workspace.cuh:20-113).
+struct TrainingWorkspace {
+    uint32_t batch_size = 0, ray_capacity = 0;
+    DeviceBuffer<float> random_float;
+    DeviceBuffer<float> pix_rgba, ray_origin, ray_dir, ray_t, ray_t_max;
+    DeviceBuffer<uint32_t> ray_img_id, ray_step, ray_offset;
+    DeviceBuffer<uint8_t> ray_alive;
+    DeviceBuffer<uint32_t> sample_img_id;
+    DeviceBuffer<float> sample_t, sample_pos, sample_dir, sample_dt, sample_m_norm, sample_dt_norm;
+    DeviceBuffer<uint32_t> info, n_rays_next;
+    DeviceBuffer<uint8_t> scan_ws;
+    void enlarge(uint32_t B, uint32_t R);
+    size_t bytes() const;
+    void free_allocations();
+};
+
+// Device-resident dataset (reference DatasetWorkspace).
+struct DatasetWorkspace {
+    DeviceBuffer<uint8_t> cameras;  // tn_camera_t[n_images]
+    DeviceBuffer<uint8_t> images;   // RGBA8
+    DeviceBuffer<uint8_t> bbox;     // tn_bbox_t
+    uint32_t n_images = 0;
+    int2v image_dims;
+    uint32_t n_pixels_per_image = 0;
+    void upload(const Dataset& ds, hipStream_t s);
+};
+
+// One NeRF's device state (reference src/models/nerf.cuh:17-32).
+struct NeRF {
+    NerfNetwork network;
+    NetworkParamsWorkspace params;
+    OccupancyGrid occupancy_grid;
+    DatasetWorkspace dataset_ws;
+    bool is_image_data_loaded = false;
+    void free_device_memory() {
+        params.free_allocations();
+        occupancy_grid.free_device_memory();
+    }
+};
+
+struct Trainer {
+    struct Context {
+        hipStream_t stream;
+        TrainingWorkspace workspace;
+        const Dataset* dataset;
+        NeRF* nerf;
+        uint32_t batch_size;
+        uint32_t n_rays_in_batch = 0;
+        uint32_t n_samples_in_batch = 0;
+        float n_rays_per_image = 0;
+        float alpha_selection_threshold = 1.0f;
+        float alpha_selection_probability = 1.0f;
+        float min_step_size = NeRFConstants::min_step_size;
+        bool use_distortion_loss = false;
+        float distortion_lambda = NeRFConstants::mipNeRF360_distortion_loss_lambda;
+        float occupancy_threshold = NeRFConstants::occupancy_threshold;
+        uint64_t rng_seed = 0x5eed;
+        uint64_t rng_offset = 0;
+        uint32_t initial_rays = 16384;
+        PinnedBuffer<uint32_t> info_host{4};
+        PinnedBuffer<float> loss_host{2};
+        PinnedBuffer<uint32_t> count_host{1};
+
+        Context(hipStream_t s, const Dataset* ds, NeRF* n, uint32_t B) : stream(s), dataset(ds), nerf(n), batch_size(B) {}
+    };
+
+    // Enqueue one training step; no host synchronisation.
+    void enqueue_train_step(Context& ctx, uint32_t training_step);
+    // One training step, synchronising like the reference: throws when the
+    // batch has no rays/samples, returns the loss (reference trainer.cu:313-349).
+    float train_step(Context& ctx, uint32_t training_step);
+    // Batch generation only (reference trainer.cu:39-217).
+    void generate_next_training_batch(Context& ctx);
+    // Occupancy update; returns the number of occupied bits (reference trainer.cu:221-311).
+    uint32_t update_occupancy_grid(Context& ctx, uint32_t training_step);
+    // Read back batch counts and losses of the last enqueued step.
+    void read_metrics(Context& ctx, float& loss);
+};
+
+}  // namespace turbo

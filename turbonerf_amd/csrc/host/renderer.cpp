@@ -1,0 +1,152 @@
+// Renderer and NeRFRenderingController (see renderer.hpp).
+//
+// Each pass takes a chunk of pixel rays through the same occupancy-skipping
+// march as training (count -> scan -> generate, capped per ray), runs the
+// network on the compacted samples only, and composites them front to back
+// with early termination. The reference instead marches every ray a fixed
+// number of steps per iteration, tests occupancy per sample and synchronises
+// with the host to count and compact (src/core/renderer.cu:298-645); here a
+// pass needs one host read, to learn how many rays fitted the sample budget.
+#include "renderer.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace turbo {
+
+void RenderingWorkspace::enlarge(uint32_t rays, uint32_t samples, size_t e) {
+    if (rays <= ray_capacity && samples <= sample_capacity && features.size() >= 32 * (size_t)samples * e) return;
+    ray_capacity = rays;
+    sample_capacity = samples;
+    ray_ori.resize(3 * (size_t)rays);
+    ray_dir.resize(3 * (size_t)rays);
+    ray_t.resize(rays);
+    ray_tmax.resize(rays);
+    rgba.resize(4 * (size_t)rays);
+    pixel.resize(rays);
+    n_steps.resize(rays);
+    offset.resize(rays);
+    info.resize(4);
+    alive.resize(rays);
+    scan_ws.resize(std::max<size_t>(tn_scan_workspace_bytes(rays), 4));
+    to_local.resize(12 * sizeof(float));
+    s_img.resize(samples);
+    s_t.resize(samples);
+    s_pos.resize(3 * (size_t)samples);
+    s_dir.resize(3 * (size_t)samples);
+    s_dt.resize(samples);
+    s_m.resize(samples);
+    s_dtn.resize(samples);
+    features.resize(32 * (size_t)samples * e);
+    density.resize((size_t)samples * e);
+    color.resize(3 * (size_t)samples * e);
+}
+
+uint64_t Renderer::render_pixels(hipStream_t s, const RenderRequest& req, const NeRFRenderable& r, uint32_t start,
+                                 uint32_t n, float* image, const Settings& st) {
+    NeRF& nerf = r.proxy->nerfs[0];
+    NerfNetwork& net = nerf.network;
+    const tn_stream_t ts = as_tn(s);
+    const uint32_t R = ws.ray_capacity, S = ws.sample_capacity;
+    const tn_camera_t cam = req.camera.c();
+    const tn_bbox_t bbox = r.bounding_box.c();
+    const float aabb = r.proxy->training_bbox.get().size();
+    const tn_occupancy_grid_t grid = nerf.occupancy_grid.view();
+    const float dt_min = st.min_step_size;
+    const float dt_max = 8.0f * dt_min;  // reference renderer.cu:305
+    const Transform4f& tr = r.proxy->transform.get();
+    const float* to_local = nullptr;
+    if (!(tr == Transform4f::Identity())) {
+        const auto a = tr.inverse().to_array();
+        TN_HIP(hipMemcpyAsync(ws.to_local.data(), a.data(), sizeof(a), hipMemcpyHostToDevice, s));
+        to_local = reinterpret_cast<const float*>(ws.to_local.data());
+    }
+    uint64_t n_evals = 0;
+    uint32_t p = start;
+    const uint32_t end = start + n;
+    uint32_t info_h[4];
+    while (p < end) {
+        if (req.is_canceled()) break;
+        const uint32_t m = std::min(end - p, R);
+        TN_TRY(tn_render_rays(m, p, R, cam, bbox, to_local, ws.ray_ori.data(), ws.ray_dir.data(), ws.ray_t.data(),
+                              ws.ray_tmax.data(), ws.pixel.data(), ws.alive.data(), ts));
+        TN_TRY(tn_march_count(m, nullptr, R, bbox, grid, NeRFConstants::cone_angle, dt_min, dt_max,
+                              st.max_samples_per_ray, ws.ray_dir.data(), ws.ray_tmax.data(), ws.alive.data(),
+                              ws.ray_ori.data(), ws.ray_t.data(), ws.n_steps.data(), ts));
+        TN_TRY(tn_scan_select(1, m, nullptr, ws.n_steps.data(), S, ws.offset.data(), ws.info.data(), ws.scan_ws.data(),
+                              ws.scan_ws.size(), ts));
+        TN_TRY(tn_march_generate(m, ws.info.data(), R, S, bbox, 1.0f / aabb, grid, dt_min, dt_max,
+                                 NeRFConstants::cone_angle, ws.pixel.data(), ws.ray_ori.data(), ws.ray_dir.data(),
+                                 ws.ray_t.data(), ws.offset.data(), ws.alive.data(), ws.n_steps.data(), ws.s_img.data(),
+                                 ws.s_t.data(), ws.s_pos.data(), ws.s_dir.data(), ws.s_dt.data(), ws.s_m.data(),
+                                 ws.s_dtn.data(), ts));
+        net.inference(s, nerf.params, S, ws.info.data(), S, ws.s_pos.data(), ws.s_dir.data(), ws.features.data(),
+                      ws.density.data(), ws.color.data(), true);
+        TN_TRY(tn_composite_forward(net.config.precision, ws.info.data(), S, S, R, st.min_transmittance, 1,
+                                    ws.n_steps.data(), ws.offset.data(), ws.density.data(), ws.color.data(),
+                                    ws.s_dt.data(), ws.rgba.data(), ts));
+        TN_TRY(tn_render_write_rgba(m, ws.info.data(), R, ws.pixel.data(), ws.alive.data(), ws.rgba.data(), image, ts));
+        TN_HIP(hipMemcpyAsync(info_h, ws.info.data(), sizeof(info_h), hipMemcpyDeviceToHost, s));
+        TN_HIP(hipStreamSynchronize(s));
+        if (info_h[2] != 0 || info_h[0] == 0) {
+            // mode 1 only fails when a single ray exceeds the sample budget
+            if (info_h[0] == 0) throw std::runtime_error("render: a ray exceeds the per-pass sample budget");
+        }
+        p += info_h[0];
+        n_evals += info_h[1];
+    }
+    return n_evals;
+}
+
+NeRFRenderingController::NeRFRenderingController(RenderPattern pattern, uint32_t batch_size)
+    : pattern_(pattern), batch_size_(batch_size == 0 ? (1u << 20) : batch_size) {}
+
+void NeRFRenderingController::cancel() {
+    if (current_) current_->cancel();
+}
+
+void NeRFRenderingController::submit(std::shared_ptr<RenderRequest> req) {
+    if (!req || !req->output) throw std::runtime_error("submit: request without output");
+    if (pattern_ != RenderPattern::LinearChunks)
+        throw std::runtime_error("submit: only the LinearChunks render pattern is part of this build");
+    current_ = req;
+    const hipStream_t s = DeviceManager::stream();
+    const int W = req->camera.resolution.x, H = req->camera.resolution.y;
+    if (req->output->width != W || req->output->height != H) req->output->set_size(W, H);
+    auto* cpu = dynamic_cast<CPURenderBuffer*>(req->output);
+    if (!cpu) throw std::runtime_error("submit: unsupported render target");
+    const uint32_t n_pixels = (uint32_t)W * (uint32_t)H;
+    // the first visible NeRF that can render (scenes of several NeRFs are not
+    // blended in this build)
+    const NeRFRenderable* r = nullptr;
+    for (auto& x : req->renderables)
+        if (x.proxy && x.proxy->can_render && x.proxy->is_visible) { r = &x; break; }
+    DeviceBuffer<float> image(4 * (size_t)n_pixels);
+    image.zero(s);
+    last_sample_count = 0;
+    if (r) {
+        const size_t e = r->proxy->nerfs[0].network.elem_bytes();
+        // LinearChunks tasks of batch_size/8 rays (reference nerf-rendering-controller.cu:84)
+        const uint32_t rays_per_task = std::max(1u, batch_size_ / 8);
+        renderer_.ws.enlarge(rays_per_task, std::max(batch_size_ * 4u, 1u << 22), e);
+        Renderer::Settings st;
+        st.min_step_size = min_step_size;
+        st.max_samples_per_ray = max_samples_per_ray;
+        for (uint32_t p = 0; p < n_pixels; p += rays_per_task) {
+            if (req->is_canceled()) break;
+            const uint32_t m = std::min(rays_per_task, n_pixels - p);
+            last_sample_count += renderer_.render_pixels(s, *req, *r, p, m, image.data(), st);
+            if (req->on_progress) req->on_progress((float)(p + m) / (float)n_pixels);
+        }
+    }
+    TN_HIP(hipMemcpyAsync(cpu->get_rgba(), image.data(), image.bytes(), hipMemcpyDeviceToHost, s));
+    TN_HIP(hipStreamSynchronize(s));
+    if (req->is_canceled()) {
+        if (req->on_cancel) req->on_cancel();
+    } else if (req->on_complete) {
+        req->on_complete();
+    }
+    current_.reset();
+}
+
+}  // namespace turbo

@@ -1,0 +1,124 @@
+// Rendering API mirroring the reference's NeRFRenderingController
+// (src/controllers/nerf-rendering-controller.h:14-41), RenderRequest
+// (src/models/render-request.cuh:20-87), CPURenderBuffer
+// (src/render-targets/cpu-render-buffer.cuh:11-69) and Renderer
+// (src/core/renderer.cuh:36-64).
+#pragma once
+
+#include <atomic>
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "controllers.hpp"
+
+namespace pybind11 {
+class module_;
+}
+
+namespace turbo {
+
+enum class RenderPattern { LinearChunks = 0, HexagonalGrid = 1, RectangularGrid = 2 };
+enum class RenderFlags : int { Preview = 1 << 0, Final = 1 << 1 };
+
+struct RenderProperties {
+    bool show_near_planes = false;
+    bool show_far_planes = false;
+};
+struct RenderModifiers {
+    RenderProperties properties;
+};
+
+// Output surface: interleaved float RGBA (premultiplied), host memory.
+struct RenderTarget {
+    int width = 0, height = 0;
+    virtual ~RenderTarget() = default;
+    virtual void set_size(int w, int h) = 0;
+    virtual void free() = 0;
+    int n_pixels() const { return width * height; }
+};
+
+struct CPURenderBuffer : RenderTarget {
+    std::vector<float> rgba;
+    void set_size(int w, int h) override {
+        width = w;
+        height = h;
+        rgba.assign((size_t)w * h * 4, 0.0f);
+    }
+    void free() override { rgba.clear(); rgba.shrink_to_fit(); width = height = 0; }
+    float* get_rgba() { return rgba.data(); }
+};
+
+struct NeRFRenderable {
+    NeRFProxy* proxy = nullptr;
+    BoundingBox bounding_box;
+    explicit NeRFRenderable(NeRFProxy* p) : proxy(p), bounding_box(p ? p->render_bbox.get() : BoundingBox()) {}
+    NeRFRenderable(NeRFProxy* p, const BoundingBox& b) : proxy(p), bounding_box(b) {}
+};
+
+using OnCompleteCallback = std::function<void()>;
+using OnProgressCallback = std::function<void(float)>;
+using OnCancelCallback = std::function<void()>;
+
+struct RenderRequest {
+    Camera camera;
+    std::vector<NeRFRenderable> renderables;
+    RenderTarget* output = nullptr;
+    RenderModifiers modifiers;
+    RenderFlags flags = RenderFlags::Final;
+    OnCompleteCallback on_complete;
+    OnProgressCallback on_progress;
+    OnCancelCallback on_cancel;
+    std::atomic<bool> canceled{false};
+    RenderRequest(const Camera& c, std::vector<NeRFRenderable> r, RenderTarget* out,
+                  RenderModifiers m = RenderModifiers(), RenderFlags f = RenderFlags::Final,
+                  OnCompleteCallback oc = nullptr, OnProgressCallback op = nullptr, OnCancelCallback ocn = nullptr)
+        : camera(c), renderables(std::move(r)), output(out), modifiers(m), flags(f), on_complete(std::move(oc)),
+          on_progress(std::move(op)), on_cancel(std::move(ocn)) {}
+    void cancel() { canceled = true; }
+    bool is_canceled() const { return canceled.load(); }
+};
+
+// Device buffers of one render pass.
+struct RenderingWorkspace {
+    uint32_t ray_capacity = 0, sample_capacity = 0;
+    DeviceBuffer<float> ray_ori, ray_dir, ray_t, ray_tmax, rgba, image;
+    DeviceBuffer<uint32_t> pixel, n_steps, offset, info;
+    DeviceBuffer<uint8_t> alive, scan_ws, to_local;
+    DeviceBuffer<uint32_t> s_img;
+    DeviceBuffer<float> s_t, s_pos, s_dir, s_dt, s_m, s_dtn;
+    DeviceBuffer<uint8_t> features, density, color;
+    void enlarge(uint32_t rays, uint32_t samples, size_t elem_bytes);
+};
+
+struct Renderer {
+    struct Settings {
+        float min_step_size = NeRFConstants::min_step_size;
+        uint32_t max_samples_per_ray = 0xFFFFFFFFu;
+        float min_transmittance = NeRFConstants::min_transmittance;
+    };
+    RenderingWorkspace ws;
+    // Render pixels [start, start + n) of the request's camera into `image`
+    // (device, interleaved RGBA). Returns the number of sample evaluations.
+    uint64_t render_pixels(hipStream_t s, const RenderRequest& req, const NeRFRenderable& r, uint32_t start, uint32_t n,
+                           float* image, const Settings& st);
+};
+
+struct NeRFRenderingController {
+    explicit NeRFRenderingController(RenderPattern pattern = RenderPattern::LinearChunks, uint32_t batch_size = 0);
+    void submit(std::shared_ptr<RenderRequest> request);
+    void cancel();
+    float min_step_size = NeRFConstants::min_step_size;
+    uint32_t max_samples_per_ray = 0xFFFFFFFFu;
+    uint64_t last_sample_count = 0;
+
+private:
+    RenderPattern pattern_;
+    uint32_t batch_size_;
+    Renderer renderer_;
+    std::shared_ptr<RenderRequest> current_;
+};
+
+void register_renderer_bindings(pybind11::module_& m);
+
+}  // namespace turbo

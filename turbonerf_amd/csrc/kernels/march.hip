@@ -105,8 +105,10 @@ TN_DEVICE uint32_t block_inclusive_scan(uint32_t v, uint32_t* smem /*[kScanThrea
     return v + pre;
 }
 
-__global__ void scan_tile_sums_kernel(uint32_t n, const uint32_t* __restrict__ vals, uint32_t* __restrict__ tile_sums) {
+__global__ void scan_tile_sums_kernel(uint32_t n, const uint32_t* __restrict__ d_n, const uint32_t* __restrict__ vals,
+                                      uint32_t* __restrict__ tile_sums) {
     __shared__ uint32_t smem[kScanThreads / 64];
+    if (d_n) n = min(*d_n, n);
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     uint32_t s = 0;
 #pragma unroll
@@ -136,10 +138,12 @@ __global__ void scan_tile_prefix_kernel(uint32_t n_tiles, uint32_t* __restrict__
 // mode 0 (training, the reference's rule): j = last ray whose offset is below
 //   max_samples; rays [0, j) are kept and n_samples = offset[j].
 // mode 1 (rendering): every ray whose samples end within max_samples is kept.
-__global__ void scan_write_select_kernel(uint32_t n, const uint32_t* __restrict__ vals,
+__global__ void scan_write_select_kernel(uint32_t n, const uint32_t* __restrict__ d_n,
+                                         const uint32_t* __restrict__ vals,
                                          const uint32_t* __restrict__ tile_prefix, uint32_t max_samples, int mode,
                                          uint32_t* __restrict__ offsets, uint32_t* __restrict__ info) {
     __shared__ uint32_t smem[kScanThreads / 64];
+    if (d_n) n = min(*d_n, n);
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
     uint32_t s = 0;
@@ -296,18 +300,18 @@ extern "C" size_t tn_scan_workspace_bytes(uint32_t n) {
     return (size_t)(tiles == 0 ? 1 : tiles) * sizeof(uint32_t);
 }
 
-extern "C" int tn_scan_select(int mode, uint32_t n_rays, const uint32_t* n_steps, uint32_t max_samples,
-                              uint32_t* ray_offset, uint32_t* info, void* workspace, size_t workspace_bytes,
-                              hipStream_t stream) {
+extern "C" int tn_scan_select(int mode, uint32_t n_rays, const uint32_t* d_n_rays, const uint32_t* n_steps,
+                              uint32_t max_samples, uint32_t* ray_offset, uint32_t* info, void* workspace,
+                              size_t workspace_bytes, hipStream_t stream) {
     TN_CHECK_ARG(mode == 0 || mode == 1);
     TN_CHECK_ARG(n_steps && ray_offset && info && workspace && n_rays > 0 && max_samples > 0);
     if (workspace_bytes < tn_scan_workspace_bytes(n_rays)) return TN_ERR_OUT_OF_MEMORY;
     const uint32_t tiles = (n_rays + kScanTile - 1) / kScanTile;
     uint32_t* tile_sums = static_cast<uint32_t*>(workspace);
-    scan_tile_sums_kernel<<<tiles, kScanThreads, 0, stream>>>(n_rays, n_steps, tile_sums);
+    scan_tile_sums_kernel<<<tiles, kScanThreads, 0, stream>>>(n_rays, d_n_rays, n_steps, tile_sums);
     scan_tile_prefix_kernel<<<1, kScanThreads, 0, stream>>>(tiles, tile_sums);
-    scan_write_select_kernel<<<tiles, kScanThreads, 0, stream>>>(n_rays, n_steps, tile_sums, max_samples, mode,
-                                                                 ray_offset, info);
+    scan_write_select_kernel<<<tiles, kScanThreads, 0, stream>>>(n_rays, d_n_rays, n_steps, tile_sums, max_samples,
+                                                                 mode, ray_offset, info);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }

@@ -61,8 +61,11 @@ __global__ void init_training_rays_kernel(uint32_t n_rays_max, const uint32_t* _
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_rays = d_n_rays ? min(*d_n_rays, n_rays_max) : n_rays_max;
     if (i >= n_rays) return;
+    // rays are split evenly over the images (reference trainer.cu:60); a
+    // non-positive n_rays_per_image asks for that split of the device count
+    const float nrpi = n_rays_per_image > 0.0f ? n_rays_per_image : (float)n_rays / (float)n_images;
 
-    uint32_t image_idx = (uint32_t)((float)i / n_rays_per_image);
+    uint32_t image_idx = (uint32_t)((float)i / nrpi);
     if (image_idx >= n_images) image_idx = n_images - 1;
     const float p = (float)n_pix * random[i];
     const float w = (float)img_w;

@@ -99,10 +99,12 @@ def march_count(rays: Rays, n_rays: int, bbox: BBox, grid: OccupancyGrid, cone: 
          stream_ptr())
 
 
-def scan_select(rays: Rays, n_rays: int, max_samples: int, info: torch.Tensor, mode: int = 0) -> None:
+def scan_select(rays: Rays, n_rays: int, max_samples: int, info: torch.Tensor, mode: int = 0,
+                d_n_rays: torch.Tensor | None = None) -> None:
     nb = query("tn_scan_workspace_bytes", n_rays)
     ws = torch.empty(max(nb, 4), dtype=torch.uint8, device="cuda")
-    call("tn_scan_select", mode, n_rays, ptr(rays.n_steps), max_samples, ptr(rays.offset), ptr(info), ptr(ws), nb,
+    call("tn_scan_select", mode, n_rays, ptr(d_n_rays), ptr(rays.n_steps), max_samples, ptr(rays.offset), ptr(info),
+         ptr(ws), nb,
          stream_ptr())
 
 
