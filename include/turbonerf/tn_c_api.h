@@ -90,6 +90,11 @@ int tn_march_generate(uint32_t n_rays_max, const uint32_t* info, uint32_t stride
     const uint32_t* n_ray_steps, uint32_t* out_img_id, float* out_t, float* out_pos,
     float* out_dir, float* out_dt, float* out_m_norm, float* out_dt_norm, tn_stream_t stream);
 
+/* total[0] += info[1] on the device: a running count of trained samples that
+ * needs no host synchronisation (no reference counterpart; the reference
+ * reads n_samples back every step, src/core/trainer.cu:169-179). */
+int tn_accumulate_samples(const uint32_t* info, unsigned long long* total, tn_stream_t stream);
+
 /* Next step's ray count from this step's samples per ray (device-side
  * replacement of reference src/core/trainer.cu:133-140). */
 int tn_estimate_rays(const uint32_t* info, uint32_t max_samples, uint32_t cap, uint32_t min_rays,

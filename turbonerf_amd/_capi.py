@@ -104,6 +104,7 @@ _SIGS: dict[str, tuple] = {
     "tn_march_generate": (C.c_int, [u32, vp, u32, u32, BBox, f32, OccupancyGrid, f32, f32, f32, vp, vp, vp, vp, vp,
                                     vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "tn_estimate_rays": (C.c_int, [vp, u32, u32, u32, vp, vp]),
+    "tn_accumulate_samples": (C.c_int, [vp, vp, vp]),
     "tn_composite_scratch_bytes": (sz, [u32]),
     "tn_composite_train": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, f32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                      vp, sz, vp]),

@@ -226,6 +226,11 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
                  tc.set_params(std::vector<float>(a.data(), a.data() + a.size()));
              })
         .def("get_cuda_memory_allocated", &NeRFTrainingController::get_cuda_memory_allocated)
+        .def("set_timing", &NeRFTrainingController::set_timing, py::arg("enabled"))
+        .def("phase_timings", &NeRFTrainingController::phase_timings)
+        .def("reset_timings", &NeRFTrainingController::reset_timings)
+        .def("samples_trained", &NeRFTrainingController::samples_trained)
+        .def("synchronize", &NeRFTrainingController::synchronize, py::call_guard<py::gil_scoped_release>())
         .def_readwrite("alpha_selection_threshold", &NeRFTrainingController::alpha_selection_threshold)
         .def_readwrite("alpha_selection_probability", &NeRFTrainingController::alpha_selection_probability)
         .def_readwrite("min_step_size", &NeRFTrainingController::min_step_size)

@@ -93,6 +93,8 @@ void NeRFTrainingController::setup_data(uint32_t batch_size) {
     ctx_->workspace.enlarge(batch_size, R);
     const uint32_t init = std::min(ctx_->initial_rays, R);
     TN_HIP(hipMemcpyAsync(ctx_->workspace.n_rays_next.data(), &init, sizeof(init), hipMemcpyHostToDevice, s));
+    ctx_->samples_total.resize(1);
+    ctx_->samples_total.zero(s);
     nerf.network.workspace.enlarge(batch_size, R, config.network.precision);
     nerf.network.update_params_if_needed(s, nerf.params);
     proxy->training_step = 0;
@@ -201,6 +203,32 @@ void NeRFTrainingController::set_params(const std::vector<float>& h) {
         TN_TRY(tn_cast_params(h.size(), nerf.params.params_fp.data(), 1, nerf.params.params_lp.data(), as_tn(s)));
     TN_HIP(hipStreamSynchronize(s));
 }
+
+void NeRFTrainingController::set_timing(bool on) {
+    if (!ctx_) throw std::runtime_error("set_timing: call setup_data() first");
+    ctx_->timer.enabled = on;
+}
+
+std::map<std::string, std::pair<double, uint64_t>> NeRFTrainingController::phase_timings() {
+    if (!ctx_) return {};
+    ctx_->timer.resolve();
+    return ctx_->timer.totals;
+}
+
+void NeRFTrainingController::reset_timings() {
+    if (ctx_) ctx_->timer.reset();
+}
+
+uint64_t NeRFTrainingController::samples_trained() {
+    if (!ctx_) return 0;
+    unsigned long long h = 0;
+    const hipStream_t s = ctx_->stream;
+    TN_HIP(hipMemcpyAsync(&h, ctx_->samples_total.data(), sizeof(h), hipMemcpyDeviceToHost, s));
+    TN_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+void NeRFTrainingController::synchronize() { TN_HIP(hipStreamSynchronize(DeviceManager::stream())); }
 
 std::vector<size_t> NeRFTrainingController::get_cuda_memory_allocated() const {
     size_t total = 0;

@@ -101,6 +101,14 @@ struct NeRFTrainingController {
     std::vector<float> get_params() const;
     void set_params(const std::vector<float>& p);
 
+    // per-phase HIP-event timing of enqueued steps (name -> ms, launches)
+    void set_timing(bool on);
+    std::map<std::string, std::pair<double, uint64_t>> phase_timings();
+    void reset_timings();
+    // samples trained on since setup_data (device running total; synchronises)
+    uint64_t samples_trained();
+    void synchronize();
+
     float alpha_selection_threshold = 1.0f;
     float alpha_selection_probability = 1.0f;
     float min_step_size = NeRFConstants::min_step_size;

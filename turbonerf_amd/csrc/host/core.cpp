@@ -6,6 +6,48 @@
 
 namespace turbo {
 
+// ------------------------------------------------------------------ timing
+PhaseTimer::~PhaseTimer() {
+    for (auto& r : pending_) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : pool_) (void)hipEventDestroy(e);
+}
+
+hipEvent_t PhaseTimer::get_() {
+    if (!pool_.empty()) {
+        hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    TN_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void PhaseTimer::begin(const char* name, hipStream_t s) {
+    Rec r{name, get_(), get_()};
+    TN_HIP(hipEventRecord(r.a, s));
+    pending_.push_back(r);
+}
+
+void PhaseTimer::end(hipStream_t s) { TN_HIP(hipEventRecord(pending_.back().b, s)); }
+
+void PhaseTimer::resolve() {
+    for (auto& r : pending_) {
+        TN_HIP(hipEventSynchronize(r.b));
+        float ms = 0.0f;
+        TN_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        auto& t = totals[r.name];
+        t.first += ms;
+        t.second += 1;
+        pool_.push_back(r.a);
+        pool_.push_back(r.b);
+    }
+    pending_.clear();
+}
+
 // ------------------------------------------------------------------ grid
 uint32_t OccupancyGrid::count_occupied(hipStream_t s) {
     const size_t nb = tn_popcount_workspace_bytes(volume());
@@ -105,35 +147,62 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
     NetworkWorkspace& w = workspace;
     const void* params = params_for_kernels(p);
     const tn_stream_t ts = as_tn(s);
+    PhaseTimer* tm = st.timer;
     // forward: encoding, density + colour MLPs (reference :431-545)
-    TN_TRY(tn_hashgrid_forward(prec, prec, B, info, B, B, grid, pos_batch, params, w.features.data(), ts));
-    TN_TRY(tn_mlp_forward(prec, B, info, B, B, B, params, p.n_grid, w.features.data(), dir_batch, 1,
-                          w.density.data(), w.rgb.data(), ts));
+    {
+        ScopedPhase ph(tm, "hashgrid_forward", s);
+        TN_TRY(tn_hashgrid_forward(prec, prec, B, info, B, B, grid, pos_batch, params, w.features.data(), ts));
+    }
+    {
+        ScopedPhase ph(tm, "mlp_forward", s);
+        TN_TRY(tn_mlp_forward(prec, B, info, B, B, B, params, p.n_grid, w.features.data(), dir_batch, 1,
+                              w.density.data(), w.rgb.data(), ts));
+    }
     // distortion loss (reference :273-283)
-    if (st.use_distortion_loss)
+    if (st.use_distortion_loss) {
+        ScopedPhase ph(tm, "distortion", s);
         TN_TRY(tn_distortion(prec, info, R, B, st.distortion_lambda, ray_steps, ray_offset, w.density.data(),
                              m_norm_batch, dt_norm_batch, w.ray_dist_loss.data(), w.dL_dist.data(),
                              w.distortion_scratch.data(), w.distortion_scratch.size(), ts));
+    }
     // compositing + loss + its backward (reference :547-603)
-    TN_TRY(tn_composite_train(prec, info, R, R, B, config.loss_scale(), st.min_transmittance, ray_steps, ray_offset,
-                              random_rgb, target_rgba, w.density.data(), w.rgb.data(), dt_batch, w.ray_loss.data(),
-                              w.dL_ds.data(), w.dL_drgb.data(), w.composite_scratch.data(), w.composite_scratch.size(),
-                              ts));
+    {
+        ScopedPhase ph(tm, "composite", s);
+        TN_TRY(tn_composite_train(prec, info, R, R, B, config.loss_scale(), st.min_transmittance, ray_steps,
+                                  ray_offset, random_rgb, target_rgba, w.density.data(), w.rgb.data(), dt_batch,
+                                  w.ray_loss.data(), w.dL_ds.data(), w.dL_drgb.data(), w.composite_scratch.data(),
+                                  w.composite_scratch.size(), ts));
+    }
     // network backward (reference :668-768)
-    TN_TRY(tn_mlp_backward(prec, B, info, B, B, params, p.n_grid, w.features.data(), dir_batch, t_batch,
-                           w.dL_drgb.data(), B, w.dL_ds.data(), st.use_distortion_loss ? w.dL_dist.data() : nullptr,
-                           config.loss_scale(), w.dfeat.data(), p.gradients.data() + p.n_grid, 0, w.mlp_ws.data(),
-                           w.mlp_ws.size(), ts));
-    TN_TRY(tn_hashgrid_backward(prec, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(), ts));
+    {
+        ScopedPhase ph(tm, "mlp_backward", s);
+        TN_TRY(tn_mlp_backward(prec, B, info, B, B, params, p.n_grid, w.features.data(), dir_batch, t_batch,
+                               w.dL_drgb.data(), B, w.dL_ds.data(), st.use_distortion_loss ? w.dL_dist.data() : nullptr,
+                               config.loss_scale(), w.dfeat.data(), p.gradients.data() + p.n_grid, 0, w.mlp_ws.data(),
+                               w.mlp_ws.size(), ts));
+    }
+    {
+        ScopedPhase ph(tm, "hashgrid_backward", s);
+        TN_TRY(tn_hashgrid_backward(prec, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(), ts));
+    }
     // optimiser (reference :770-781, lr schedule :256)
     const float lr = config.learning_rate / (1.0f + config.learning_rate_decay * (float)step);
-    TN_TRY(tn_adam_step(p.n_total, p.n_grid, p.n_total, config.loss_scale(), lr, config.beta1, config.beta2,
-                        config.epsilon, config.l2_reg, p.params_fp.data(), prec, prec == 1 ? p.params_lp.data() : nullptr,
-                        p.gradients.data(), p.m1.data(), p.m2.data(), p.steps.data(), ts));
+    {
+        ScopedPhase ph(tm, "adam", s);
+        TN_TRY(tn_adam_step(p.n_total, p.n_grid, p.n_total, config.loss_scale(), lr, config.beta1, config.beta2,
+                            config.epsilon, config.l2_reg, p.params_fp.data(), prec,
+                            prec == 1 ? p.params_lp.data() : nullptr, p.gradients.data(), p.m1.data(), p.m2.data(),
+                            p.steps.data(), ts));
+    }
     // losses: mean over rays of the four smooth-L1 channels, mean distortion
-    TN_TRY(tn_reduce_loss(info, R, 4, w.ray_loss.data(), w.loss_out.data(), ts));
-    if (st.use_distortion_loss) TN_TRY(tn_reduce_loss(info, R, 1, w.ray_dist_loss.data(), w.loss_out.data() + 1, ts));
-    else TN_HIP(hipMemsetAsync(w.loss_out.data() + 1, 0, sizeof(float), s));
+    {
+        ScopedPhase ph(tm, "loss_reduce", s);
+        TN_TRY(tn_reduce_loss(info, R, 4, w.ray_loss.data(), w.loss_out.data(), ts));
+        if (st.use_distortion_loss)
+            TN_TRY(tn_reduce_loss(info, R, 1, w.ray_dist_loss.data(), w.loss_out.data() + 1, ts));
+        else
+            TN_HIP(hipMemsetAsync(w.loss_out.data() + 1, 0, sizeof(float), s));
+    }
 }
 
 void NerfNetwork::inference(hipStream_t s, const NetworkParamsWorkspace& p, uint32_t n, const uint32_t* info,
@@ -206,38 +275,54 @@ void Trainer::generate_next_training_batch(Context& ctx) {
     const tn_stream_t ts = as_tn(s);
     const uint32_t R = ws.ray_capacity, B = ws.batch_size;
     const Dataset& ds = *ctx.dataset;
-    // seeded uniforms: slice 0 picks pixels, slices 1-3 are the background
-    // colour (reference trainer.cu:45-50,332)
-    TN_TRY(tn_fill_uniform(4 * (uint64_t)R, ctx.rng_seed, ctx.rng_offset, ws.random_float.data(), ts));
-    ctx.rng_offset += 4 * (uint64_t)R;
-    // rays are generated for the device-side estimate of how many fill the
-    // batch and split evenly over the images by that count (-1 below)
+    PhaseTimer* tm = &ctx.timer;
     const DatasetWorkspace& dw = nerf.dataset_ws;
-    TN_TRY(tn_init_training_rays(R, ws.n_rays_next.data(), R, dw.n_images, dw.n_pixels_per_image, dw.image_dims.x,
-                                 dw.image_dims.y, -1.0f,
-                                 reinterpret_cast<const tn_bbox_t*>(dw.bbox.data()),
-                                 reinterpret_cast<const tn_camera_t*>(dw.cameras.data()), dw.images.data(),
-                                 ws.random_float.data(), ws.pix_rgba.data(), ws.ray_img_id.data(), ws.ray_origin.data(),
-                                 ws.ray_dir.data(), ws.ray_t.data(), ws.ray_t_max.data(), ws.ray_alive.data(), ts));
-    if (ctx.alpha_selection_threshold < 1.0f && ctx.alpha_selection_probability < 1.0f)
-        TN_TRY(tn_deactivate_rays(R, ws.n_rays_next.data(), R, ctx.alpha_selection_threshold,
-                                  ctx.alpha_selection_probability, ws.pix_rgba.data(), ws.random_float.data() + R,
-                                  ws.ray_alive.data(), ts));
+    {
+        // seeded uniforms: slice 0 picks pixels, slices 1-3 are the background
+        // colour (reference trainer.cu:45-50,332); rays are generated for the
+        // device-side estimate of how many fill the batch and split evenly
+        // over the images by that count (-1 below)
+        ScopedPhase ph(tm, "rays", s);
+        TN_TRY(tn_fill_uniform(4 * (uint64_t)R, ctx.rng_seed, ctx.rng_offset, ws.random_float.data(), ts));
+        ctx.rng_offset += 4 * (uint64_t)R;
+        TN_TRY(tn_init_training_rays(R, ws.n_rays_next.data(), R, dw.n_images, dw.n_pixels_per_image,
+                                     dw.image_dims.x, dw.image_dims.y, -1.0f,
+                                     reinterpret_cast<const tn_bbox_t*>(dw.bbox.data()),
+                                     reinterpret_cast<const tn_camera_t*>(dw.cameras.data()), dw.images.data(),
+                                     ws.random_float.data(), ws.pix_rgba.data(), ws.ray_img_id.data(),
+                                     ws.ray_origin.data(), ws.ray_dir.data(), ws.ray_t.data(), ws.ray_t_max.data(),
+                                     ws.ray_alive.data(), ts));
+        if (ctx.alpha_selection_threshold < 1.0f && ctx.alpha_selection_probability < 1.0f)
+            TN_TRY(tn_deactivate_rays(R, ws.n_rays_next.data(), R, ctx.alpha_selection_threshold,
+                                      ctx.alpha_selection_probability, ws.pix_rgba.data(), ws.random_float.data() + R,
+                                      ws.ray_alive.data(), ts));
+    }
     const float dt_min = ctx.min_step_size;
     const float dt_max = ds.bounding_box.size() * dt_min;
     const tn_bbox_t bbox = ds.bounding_box.c();
     const tn_occupancy_grid_t grid = nerf.occupancy_grid.view();
-    TN_TRY(tn_march_count(R, ws.n_rays_next.data(), R, bbox, grid, NeRFConstants::cone_angle, dt_min, dt_max,
-                          0xFFFFFFFFu, ws.ray_dir.data(), ws.ray_t_max.data(), ws.ray_alive.data(), ws.ray_origin.data(),
-                          ws.ray_t.data(), ws.ray_step.data(), ts));
-    TN_TRY(tn_scan_select(0, R, ws.n_rays_next.data(), ws.ray_step.data(), B, ws.ray_offset.data(), ws.info.data(),
-                          ws.scan_ws.data(), ws.scan_ws.size(), ts));
-    TN_TRY(tn_march_generate(R, ws.info.data(), R, B, bbox, 1.0f / ds.bounding_box.size(), grid, dt_min, dt_max,
-                             NeRFConstants::cone_angle, ws.ray_img_id.data(), ws.ray_origin.data(), ws.ray_dir.data(),
-                             ws.ray_t.data(), ws.ray_offset.data(), ws.ray_alive.data(), ws.ray_step.data(),
-                             ws.sample_img_id.data(), ws.sample_t.data(), ws.sample_pos.data(), ws.sample_dir.data(),
-                             ws.sample_dt.data(), ws.sample_m_norm.data(), ws.sample_dt_norm.data(), ts));
-    TN_TRY(tn_estimate_rays(ws.info.data(), B, R, 64, ws.n_rays_next.data(), ts));
+    {
+        ScopedPhase ph(tm, "march_count", s);
+        TN_TRY(tn_march_count(R, ws.n_rays_next.data(), R, bbox, grid, NeRFConstants::cone_angle, dt_min, dt_max,
+                              0xFFFFFFFFu, ws.ray_dir.data(), ws.ray_t_max.data(), ws.ray_alive.data(),
+                              ws.ray_origin.data(), ws.ray_t.data(), ws.ray_step.data(), ts));
+    }
+    {
+        ScopedPhase ph(tm, "scan_select", s);
+        TN_TRY(tn_scan_select(0, R, ws.n_rays_next.data(), ws.ray_step.data(), B, ws.ray_offset.data(),
+                              ws.info.data(), ws.scan_ws.data(), ws.scan_ws.size(), ts));
+    }
+    {
+        ScopedPhase ph(tm, "march_generate", s);
+        TN_TRY(tn_march_generate(R, ws.info.data(), R, B, bbox, 1.0f / ds.bounding_box.size(), grid, dt_min, dt_max,
+                                 NeRFConstants::cone_angle, ws.ray_img_id.data(), ws.ray_origin.data(),
+                                 ws.ray_dir.data(), ws.ray_t.data(), ws.ray_offset.data(), ws.ray_alive.data(),
+                                 ws.ray_step.data(), ws.sample_img_id.data(), ws.sample_t.data(), ws.sample_pos.data(),
+                                 ws.sample_dir.data(), ws.sample_dt.data(), ws.sample_m_norm.data(),
+                                 ws.sample_dt_norm.data(), ts));
+        TN_TRY(tn_estimate_rays(ws.info.data(), B, R, 64, ws.n_rays_next.data(), ts));
+        if (ctx.samples_total.size()) TN_TRY(tn_accumulate_samples(ws.info.data(), ctx.samples_total.data(), ts));
+    }
 }
 
 void Trainer::enqueue_train_step(Context& ctx, uint32_t training_step) {
@@ -246,6 +331,7 @@ void Trainer::enqueue_train_step(Context& ctx, uint32_t training_step) {
     NerfNetwork::Settings st;
     st.use_distortion_loss = ctx.use_distortion_loss;
     st.distortion_lambda = ctx.distortion_lambda;
+    st.timer = &ctx.timer;
     ctx.nerf->network.train(ctx.stream, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity,
                             ws.info.data(), ws.random_float.data() + ws.ray_capacity, ws.ray_step.data(),
                             ws.ray_offset.data(), ws.sample_t.data(), ws.sample_pos.data(), ws.sample_dir.data(),

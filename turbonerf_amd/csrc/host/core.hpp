@@ -6,7 +6,9 @@
 
 #include <cmath>
 #include <functional>
+#include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -27,6 +29,37 @@ constexpr uint32_t n_steps_per_render_compaction = 64;
 constexpr float learning_rate_decay = 3.3e-5f;
 constexpr uint32_t n_max_nerfs = 2 << 10;
 }  // namespace NeRFConstants
+
+// Optional per-phase timing with HIP events on the work stream. Records are
+// resolved (and the events recycled) by resolve(), which synchronises.
+class PhaseTimer {
+public:
+    bool enabled = false;
+    ~PhaseTimer();
+    void begin(const char* name, hipStream_t s);
+    void end(hipStream_t s);
+    void resolve();
+    void reset() { resolve(); totals.clear(); }
+    // name -> (total milliseconds, launches)
+    std::map<std::string, std::pair<double, uint64_t>> totals;
+
+private:
+    struct Rec { std::string name; hipEvent_t a, b; };
+    std::vector<Rec> pending_;
+    std::vector<hipEvent_t> pool_;
+    hipEvent_t get_();
+};
+
+struct ScopedPhase {
+    PhaseTimer* t;
+    hipStream_t s;
+    ScopedPhase(PhaseTimer* t_, const char* name, hipStream_t s_) : t(t_), s(s_) {
+        if (t && t->enabled) t->begin(name, s);
+    }
+    ~ScopedPhase() {
+        if (t && t->enabled) t->end(s);
+    }
+};
 
 // Multi-cascade occupancy grid: Morton-ordered bitfield (bit k = cascade k)
 // and per-cascade fp32 densities (reference src/core/occupancy-grid.cuh:22-287).
@@ -114,6 +147,7 @@ struct NerfNetwork {
         bool use_distortion_loss = false;
         float distortion_lambda = NeRFConstants::mipNeRF360_distortion_loss_lambda;
         float min_transmittance = NeRFConstants::min_transmittance;
+        PhaseTimer* timer = nullptr;
     };
 
     NetworkConfig config;
@@ -210,6 +244,8 @@ struct Trainer {
         PinnedBuffer<uint32_t> info_host{4};
         PinnedBuffer<float> loss_host{2};
         PinnedBuffer<uint32_t> count_host{1};
+        PhaseTimer timer;
+        DeviceBuffer<unsigned long long> samples_total;  // running sum of samples trained on
 
         Context(hipStream_t s, const Dataset* ds, NeRF* n, uint32_t B) : stream(s), dataset(ds), nerf(n), batch_size(B) {}
     };

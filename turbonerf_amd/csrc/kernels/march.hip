@@ -275,9 +275,21 @@ __global__ void estimate_rays_kernel(const uint32_t* __restrict__ info, uint32_t
     *n_rays_next = est;
 }
 
+__global__ void accumulate_samples_kernel(const uint32_t* __restrict__ info, unsigned long long* __restrict__ total) {
+    total[0] += (unsigned long long)info[1];
+}
+
 }  // namespace tn
 
 using namespace tn;
+
+// total[0] += info[1] on the device (running count of trained samples)
+extern "C" int tn_accumulate_samples(const uint32_t* info, unsigned long long* total, hipStream_t stream) {
+    TN_CHECK_ARG(info && total);
+    accumulate_samples_kernel<<<1, 1, 0, stream>>>(info, total);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
 
 extern "C" int tn_march_count(uint32_t n_rays_max, const uint32_t* d_n_rays, uint32_t stride,
                               tn_bbox_t bbox, tn_occupancy_grid_t grid, float cone_angle, float dt_min,
