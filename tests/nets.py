@@ -32,28 +32,37 @@ def split(net):
     return W1, W2, W3, W4, W5
 
 
-def forward(net, feat, dir01):
+def _q(a, on):
+    """Round to fp16 where the mixed-precision kernel feeds a matrix-core operand."""
+    return np.asarray(a, np.float16).astype(np.float64) if on else a
+
+
+def forward(net, feat, dir01, round16=False):
+    """round16: round every matrix operand to fp16 as the fp16 MFMA kernels do
+    (accumulation stays exact)."""
     W1, W2, W3, W4, W5 = split(net)
-    feat = np.asarray(feat, np.float64)
+    feat = _q(np.asarray(feat, np.float64), round16)
     h1 = np.maximum(feat @ W1.T, 0)
-    s16 = h1 @ W2.T
-    xc = np.concatenate([s16, sh4(dir01)], axis=1)
+    s16 = _q(h1, round16) @ W2.T
+    xc = np.concatenate([_q(s16, round16), _q(sh4(dir01), round16)], axis=1)
     h2 = np.maximum(xc @ W3.T, 0)
-    h3 = np.maximum(h2 @ W4.T, 0)
-    o = h3 @ W5.T
+    h3 = np.maximum(_q(h2, round16) @ W4.T, 0)
+    o = _q(h3, round16) @ W5.T
     rgb = 1.0 / (1.0 + np.exp(-o))
-    return dict(h1=h1, s16=s16, xc=xc, h2=h2, h3=h3, o=o, rgb=rgb, feat=feat)
+    return dict(h1=h1, s16=s16, xc=xc, h2=h2, h3=h3, o=o, rgb=rgb, feat=feat, round16=round16)
 
 
 def backward(net, fw, G_rgb, dL_ds, t, loss_scale, extra=None):
     """Returns (dfeat (n,32), grad_net (9408,))."""
     W1, W2, W3, W4, W5 = split(net)
+    r = fw.get("round16", False)
     rgb = fw["rgb"]
-    dO = G_rgb * rgb * (1 - rgb)
-    dW5 = dO.T @ fw["h3"]
-    dH3 = (dO @ W5) * (fw["h3"] > 0)
-    dW4 = dH3.T @ fw["h2"]
-    dH2 = (dH3 @ W4) * (fw["h2"] > 0)
+    dO = _q(G_rgb * rgb * (1 - rgb), r)
+    h1, h2, h3 = _q(fw["h1"], r), _q(fw["h2"], r), _q(fw["h3"], r)
+    dW5 = dO.T @ h3
+    dH3 = _q((dO @ W5) * (fw["h3"] > 0), r)
+    dW4 = dH3.T @ h2
+    dH2 = _q((dH3 @ W4) * (fw["h2"] > 0), r)
     dW3 = dH2.T @ fw["xc"]
     dXc = dH2 @ W3
     dS = dXc[:, :16].copy()
@@ -62,8 +71,9 @@ def backward(net, fw, G_rgb, dL_ds, t, loss_scale, extra=None):
         g0 += loss_scale * np.asarray(extra, np.float64)
     t = np.asarray(t, np.float64)
     dS[:, 0] = np.where(t < 1.0, g0 * t * t, g0)
-    dW2 = dS.T @ fw["h1"]
-    dH1 = (dS @ W2) * (fw["h1"] > 0)
+    dS = _q(dS, r)
+    dW2 = dS.T @ h1
+    dH1 = _q((dS @ W2) * (fw["h1"] > 0), r)
     dW1 = dH1.T @ fw["feat"]
     dX = dH1 @ W1
     grad = np.concatenate([dW1.ravel(), dW2.ravel(), dW3.ravel(), dW4.ravel(), dW5.ravel()])

@@ -46,9 +46,11 @@ def run_gpu(b, fp64=False, half=False):
     scratch = torch.empty(nb, dtype=torch.uint8, device="cuda")
     dens = T(b["s"], torch.float16 if half else torch.float32)
     code = 2 if fp64 else (1 if half else 0)
-    call("tn_distortion", code, ptr(info), nr, ns, LAMBDA, ptr(T(b["steps"].view(np.int32), torch.int32)),
-         ptr(T(b["offs"].view(np.int32), torch.int32)), ptr(dens), ptr(T(b["m"])), ptr(T(b["d"])), ptr(loss),
-         ptr(grad), ptr(scratch), nb, stream_ptr())
+    # keep every input tensor referenced until the kernels have run
+    steps, offs = T(b["steps"].view(np.int32), torch.int32), T(b["offs"].view(np.int32), torch.int32)
+    m, d = T(b["m"]), T(b["d"])
+    call("tn_distortion", code, ptr(info), nr, ns, LAMBDA, ptr(steps), ptr(offs), ptr(dens), ptr(m), ptr(d),
+         ptr(loss), ptr(grad), ptr(scratch), nb, stream_ptr())
     torch.cuda.synchronize()
     return loss.cpu().numpy().astype(np.float64), grad.cpu().numpy().astype(np.float64)
 

@@ -3,8 +3,11 @@
 * hash grid (16 levels x 2 features, T = 2^19, N 16 -> 2048): forward vs the
   oracle's fp64 encoding (fp32 table: rtol 1e-5), backward scatter-add vs the
   oracle's fp64 adjoint (norm-wise 1e-5; atomics change the summation order);
-* MLPs: fp32 MFMA path vs fp64 numpy (norm-wise 1e-5), fp16 MFMA path vs fp64
-  numpy on fp16-rounded inputs (norm-wise 1e-2 with fp16 activations);
+* MLPs: fp32 MFMA path vs fp64 numpy (norm-wise 1e-5); fp16 MFMA path: forward
+  vs fp64 numpy on fp16-rounded inputs (1e-2), backward vs fp64 numpy that
+  rounds each matrix operand to fp16 as the kernel does (2e-3; without that
+  rounding, cancellation in zero-mean weight gradients amplifies fp16 operand
+  error to ~2e-2);
 * Adam: vs the oracle (rtol 1e-6), including skipped zero-gradient hash entries.
 """
 import numpy as np
@@ -27,10 +30,13 @@ def test_hashgrid_layout_matches_oracle(cuda):
     import oracle
     from turbonerf_amd import _capi, ops
 
-    g = ops.hashgrid_layout()
+    import math
+
+    b = math.exp((math.log(2048.0) - math.log(16.0)) / 15.0)
+    g = ops.hashgrid_layout(per_level_scale=b)
     o = _capi.HashGrid()
-    oracle.lib().oc_hashgrid_layout(16, 2, 19, 16, float(g.per_level_scale), ctypes.addressof(o))
-    # the oracle recomputes from the same float scale; compare every field
+    oracle.lib().oc_hashgrid_layout(16, 2, 19, 16, b, ctypes.addressof(o))
+    # both derive the table from the same double-precision scale
     for f, _ in _capi.HashGrid._fields_:
         a, b = getattr(g, f), getattr(o, f)
         assert (list(a) == list(b)) if hasattr(a, "__len__") else (a == b), f
@@ -144,11 +150,12 @@ def test_mlp_backward(cuda, prec):
     dfeat = ops.mlp_backward(params, n_grid, n, feat_d, n, dir_d, n, torch.from_numpy(t).cuda(),
                              torch.from_numpy(np.ascontiguousarray(G_rgb.T.ravel())).to(dtype).cuda(),
                              torch.from_numpy(dL_ds).cuda(), ls, grad, torch.from_numpy(extra).cuda())
-    fw = np_forward(net, feat, dir01)
+    # the fp16 reference rounds every matrix operand as the kernel does
+    fw = np_forward(net, feat, dir01, round16=(prec == "fp16"))
     dX, g_ref = np_backward(net, fw, G_rgb, dL_ds, t, ls, extra)
     got_g = grad.cpu().numpy()
     got_x = dfeat.float().cpu().numpy().reshape(32, n).T
-    tol = 1e-5 if prec == "fp32" else 1e-2
+    tol = 1e-5 if prec == "fp32" else 2e-3
     eg, ex = norm_rel(got_g, g_ref), norm_rel(got_x, dX)
     print(f"[mlp bwd {prec}] dW {eg:.2e} dX {ex:.2e}", flush=True)
     for name, sl in (("W1", slice(0, 2048)), ("W2", slice(2048, 3072)), ("W3", slice(3072, 5120)),

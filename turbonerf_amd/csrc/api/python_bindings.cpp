@@ -226,6 +226,22 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
                  tc.set_params(std::vector<float>(a.data(), a.data() + a.size()));
              })
         .def("get_cuda_memory_allocated", &NeRFTrainingController::get_cuda_memory_allocated)
+        .def("get_occupancy_bitfield",
+             [](NeRFTrainingController& tc) {
+                 tc.synchronize();
+                 const auto& og = tc.proxy->nerfs[0].occupancy_grid;
+                 py::array_t<uint8_t> a(og.bitfield.size());
+                 TN_HIP(hipMemcpy(a.mutable_data(), og.bitfield.data(), og.bitfield.size(), hipMemcpyDeviceToHost));
+                 return a;
+             })
+        .def("get_occupancy_density",
+             [](NeRFTrainingController& tc) {
+                 tc.synchronize();
+                 const auto& og = tc.proxy->nerfs[0].occupancy_grid;
+                 py::array_t<float> a(og.density.size());
+                 TN_HIP(hipMemcpy(a.mutable_data(), og.density.data(), og.density.bytes(), hipMemcpyDeviceToHost));
+                 return a;
+             })
         .def("set_timing", &NeRFTrainingController::set_timing, py::arg("enabled"))
         .def("phase_timings", &NeRFTrainingController::phase_timings)
         .def("reset_timings", &NeRFTrainingController::reset_timings)

@@ -14,6 +14,10 @@
 // wave sees 64 consecutive samples of one level. Consecutive samples of a
 // ray share coarse-level cells; the backward pass merges runs of equal corner
 // indices across the wave with a segmented sum before issuing its atomics.
+// Cell selection (x*scale + 0.5, floor) is kept free of contraction so it
+// matches a CPU evaluation of the same expression exactly.
+#pragma clang fp contract(off)
+
 #include "segscan.hpp"
 
 namespace tn {
