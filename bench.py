@@ -1,0 +1,393 @@
+#!/usr/bin/env python3
+"""Training-step throughput of the TurboNeRF hot path on one MI355X.
+
+Workload (the reference's full training step, src/core/trainer.cu:281-406 and
+src/core/nerf-network.cu:547-781): hash grid 16 levels x 2 features, 2^19
+entries per level, resolution 16 -> 2048, table init U(+-1e-4); density MLP
+32 -> 64 -> 16, colour MLP (16 + 16 SH) -> 64 -> 64 -> 3; 2^18 compacted
+samples per step; targets are an 8-blob Gaussian field rendered at 800x800
+from 100 random cameras on a radius-2 sphere (fov 50 deg), composited over a
+random background; smooth-L1 loss; Adam lr 1e-2, betas (0.9, 0.99), eps 1e-15.
+
+Per step: ray generation, occupancy-grid marching (count, scan/select,
+generate), hash-grid forward, fused MLP forward, compositing + loss +
+backward, fused MLP backward, hash-grid scatter-add, Adam. The occupancy grid
+is refreshed every 16 steps during warm-up and timed on its own.
+
+value = samples trained per second (device-side sample counter over the timed
+steps) for fp16-MFMA mixed precision; the plain fp32 path, the per-phase
+breakdown, the occupancy update and render throughput go under "extra".
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+BATCH = 1 << 18
+N_CAMS, RES = 100, 800
+SCENE_SEED = 7
+
+# algorithmic bytes (B) or FLOPs per unit for each trainer phase:
+# (bound, per-sample, per-ray, per-parameter)
+ALGO = {
+    "rays": ("hbm", 0, 60, 0),
+    "march_count": ("hbm", 0, 38, 0),
+    "scan_select": ("hbm", 0, 8, 0),
+    "march_generate": ("hbm", 44, 38, 0),
+    "hashgrid_forward": ("hbm", 588, 0, 0),
+    "mlp_forward": ("mfma", 18816, 0, 0),
+    "composite": ("hbm", 34, 60, 0),
+    "distortion": ("hbm", 24, 8, 0),
+    "mlp_backward": ("mfma", 56448 - 18816, 0, 0),
+    "hashgrid_backward": ("hbm", 1100, 0, 0),
+    "adam": ("hbm", 0, 0, 36),
+    "loss_reduce": ("hbm", 0, 16, 0),
+}
+# the kernel each phase launches (rocprofv3 kernel names)
+KERNEL = {
+    "hashgrid_forward": "hashgrid_forward_kernel",
+    "hashgrid_backward": "hashgrid_backward_kernel",
+    "mlp_forward": "mlp_forward_kernel",
+    "mlp_backward": "mlp_backward_kernel",
+    "composite": "composite_train_kernel",
+    "march_count": "march_count_kernel",
+    "march_generate": "march_generate_kernel",
+    "adam": "adam_kernel",
+}
+METRIC = "train samples/sec (2^18/step) mixed vs FP32 at \u22641e-3 grad err; render Mrays/s; %HBM/MFMA peak"
+PEAK_HBM_GBS = 8000.0
+PEAK_TFLOPS = {"fp16": 2500.0, "fp32": 157.3}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200, help="timed training steps")
+    ap.add_argument("--warmup", type=int, default=256, help="untimed steps (occupancy update every 16)")
+    ap.add_argument("--precision", choices=["mixed", "fp32", "both"], default="both")
+    ap.add_argument("--cpu-samples", type=int, default=1 << 18, help="samples per CPU-baseline step")
+    ap.add_argument("--cpu-runs", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-render", action="store_true")
+    return ap.parse_args()
+
+
+def make_scene():
+    import numpy as np
+
+    from turbonerf_amd import synthetic
+
+    t0 = time.perf_counter()
+    blobs, cams, images = synthetic.blob_dataset(seed=SCENE_SEED, n_cams=N_CAMS, width=RES, height=RES)
+    log(f"[bench] targets: {N_CAMS} x {RES}x{RES} rendered in {time.perf_counter() - t0:.1f} s, "
+        f"alpha mean {images[..., 3].mean() / 255:.3f}")
+    return blobs, cams, np.ascontiguousarray(images)
+
+
+def make_trainer(tn, synthetic, scene, precision):
+    blobs, cams, images = scene
+    mgr = tn.NeRFManager()
+    nerf = mgr.create()
+    ds = tn.Dataset(tn.BoundingBox(1.0), synthetic.to_pyturbonerf_cameras(tn, cams), images)
+    nerf.attach_dataset(ds)
+    cfg = tn.TrainingConfig()
+    cfg.network.precision = precision
+    cfg.network.beta1 = 0.9
+    cfg.network.beta2 = 0.99
+    cfg.network.epsilon = 1e-15
+    cfg.network.learning_rate = 1e-2
+    tr = tn.Trainer(nerf, cfg)
+    tr.setup_data(batch_size=BATCH)
+    tr.load_images()
+    return mgr, nerf, tr
+
+
+def run_precision(tn, synthetic, scene, precision, args):
+    import torch
+
+    mgr, nerf, tr = make_trainer(tn, synthetic, scene, precision)
+    name = "mixed" if precision == 1 else "fp32"
+    # warm-up: the reference's loop, occupancy grid refreshed every 16 steps
+    step = 0
+    while step < args.warmup:
+        k = min(16, args.warmup - step)
+        tr.train_steps(k, read_metrics=False)
+        step += k
+        if step % 16 == 0:
+            tr.update_occupancy_grid(step)
+    m = tr.train_step()
+    step += 1
+    log(f"[bench {name}] warm-up done: loss {m.loss:.4f} rays {m.n_rays} samples {m.n_samples}")
+
+    # timed region: K steps, no host synchronisation inside
+    tr.set_timing(False)
+    s0 = tr.samples_trained()
+    tr.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tr.train_steps(args.steps, read_metrics=False)
+    tr.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    s1 = tr.samples_trained()
+    step += args.steps
+    wall = t1 - t0
+    samples = s1 - s0
+    res = dict(samples_per_s=samples / wall, ms_per_step=1e3 * wall / args.steps,
+               samples_per_step=samples / args.steps)
+
+    # the same K steps again with HIP events around every phase on the
+    # trainer's stream (per-kernel durations for the roofline)
+    tr.reset_timings()
+    tr.set_timing(True)
+    tr.train_steps(args.steps, read_metrics=False)
+    tr.synchronize()
+    phases = tr.phase_timings()
+    tr.set_timing(False)
+    step += args.steps
+    rays = []
+    for _ in range(8):
+        m = tr.train_step()
+        rays.append(m.n_rays)
+        step += 1
+    res["rays_per_step"] = statistics.mean(rays)
+    res["loss"] = m.loss
+    res["phases"] = {k: dict(ms=v[0] / max(v[1], 1), launches=v[1]) for k, v in phases.items()}
+
+    # occupancy update, timed on its own
+    tr.synchronize()
+    t0 = time.perf_counter()
+    n_upd = 8
+    for i in range(n_upd):
+        om = tr.update_occupancy_grid(step + 16 * (i + 1))
+    tr.synchronize()
+    res["occupancy_update_ms"] = 1e3 * (time.perf_counter() - t0) / n_upd
+    res["occupied_fraction"] = om.n_occupied / om.n_total
+    res["bitfield"] = tr.get_occupancy_bitfield()
+    res["trainer"] = (mgr, nerf, tr)
+    log(f"[bench {name}] {res['samples_per_s'] / 1e6:.2f} M samples/s, {res['ms_per_step']:.3f} ms/step, "
+        f"{res['samples_per_step']:.0f} samples/step, {res['rays_per_step']:.0f} rays/step")
+    return res
+
+
+def roofline(res, n_params, dtype):
+    """Achieved rate of every phase from its algorithmic bytes/FLOPs; the
+    roofline object is the phase with the largest share of the step."""
+    S, R = res["samples_per_step"], res["rays_per_step"]
+    table = {}
+    for ph, v in res["phases"].items():
+        if ph not in ALGO:
+            continue
+        bound, ps, pr, pp = ALGO[ph]
+        units = ps * S + pr * R + pp * n_params
+        sec = v["ms"] * 1e-3
+        if bound == "hbm":
+            ach, peak, unit = units / sec / 1e9, PEAK_HBM_GBS, "GB/s"
+        else:
+            ach, peak, unit = units / sec / 1e12, PEAK_TFLOPS[dtype], "TFLOP/s"
+        table[ph] = dict(ms=round(v["ms"], 4), bound=bound, achieved=round(ach, 2), unit=unit,
+                         frac=round(ach / peak, 4), algorithmic=units)
+    dom = max(table, key=lambda k: table[k]["ms"])
+    d = table[dom]
+    peak = PEAK_HBM_GBS if d["bound"] == "hbm" else PEAK_TFLOPS[dtype]
+    traffic = measured_traffic(KERNEL.get(dom), S)
+    return dict(kernel=KERNEL.get(dom, dom), phase=dom, bound=d["bound"], achieved=d["achieved"], peak=peak,
+                unit=d["unit"], frac=d["frac"], traffic=traffic, avg_launch_ms=d["ms"],
+                algorithmic_per_launch=d["algorithmic"]), table
+
+
+def measured_traffic(kernel, samples_per_step):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (profiles/traffic.json), scaled to this run's samples per launch;
+    None when no measurement exists."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    if not kernel or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    e = t.get(kernel)
+    if not e:
+        return None
+    return int(e["bytes_per_sample"] * samples_per_step)
+
+
+def render_rate(tn, synthetic, scene, trainer_tuple, n_frames=5):
+    """Mrays/s of Renderer.submit on one 800x800 frame of the trained NeRF."""
+    blobs, cams, images = scene
+    mgr, nerf, tr = trainer_tuple
+    cam = synthetic.to_pyturbonerf_cameras(tn, cams)[0]
+    buf = tn.CPURenderBuffer()
+    r = tn.Renderer()
+    req = lambda: tn.RenderRequest(cam, [tn.Renderable(nerf)], buf)
+    r.submit(req())  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(n_frames):
+        r.submit(req())
+    dt = (time.perf_counter() - t0) / n_frames
+    return RES * RES / dt / 1e6, 1e3 * dt
+
+
+def cpu_baseline(scene, bitfield, rays_per_step_gpu, args):
+    """The oracle (CPU restatement of the same step, OpenMP over rays and
+    samples, fp32) on a bounded sample: one training step of
+    `args.cpu_samples` samples, 1 warm-up + median of `args.cpu_runs`."""
+    import ctypes as C
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle
+
+    from turbonerf_amd import scenes
+
+    blobs, cams, images = scene
+    lib = oracle.lib()
+    S = args.cpu_samples
+    R = int(math.ceil(1.1 * rays_per_step_gpu * S / BATCH)) + 64
+    g = __import__("turbonerf_amd._capi", fromlist=["HashGrid"]).HashGrid()
+    b = math.exp((math.log(2048.0) - math.log(16.0)) / 15.0)
+    lib.oc_hashgrid_layout(16, 2, 19, 16, b, C.addressof(g))
+    rng = np.random.default_rng(0)
+    table = rng.uniform(-1e-4, 1e-4, g.n_params).astype(np.float32)
+    net = np.concatenate([rng.uniform(-a, a, n).astype(np.float32) for a, n in
+                          ((math.sqrt(6 / 96), 2048), (math.sqrt(6 / 80), 1024), (math.sqrt(6 / 96), 2048),
+                           (math.sqrt(6 / 128), 4096), (math.sqrt(6 / 67), 192))])
+    params = np.concatenate([table, net])
+    m1 = np.zeros_like(params)
+    m2 = np.zeros_like(params)
+    steps = np.zeros(params.size, np.uint32)
+    grads = np.zeros_like(params)
+    bbox = scenes.bbox(1.0)
+    dt = float(np.float32(math.sqrt(3) / 1024))
+    bits = np.ascontiguousarray(bitfield, np.uint8)
+    offset = [0]
+
+    def step():
+        rnd = oracle.uniform(4 * R, 1234, offset[0])
+        offset[0] += 4 * R
+        rays = oracle.init_training_rays(R, R, N_CAMS, RES, RES, R / N_CAMS, bbox, cams, images, rnd)
+        n_steps, alive = oracle.march_count(R, R, bbox, bits, 1, 128, 1 / 256, dt, dt, rays)
+        offs, nr, ns, rc = oracle.scan_select(n_steps, R, S)
+        if rc != 0 or ns == 0:
+            return 0
+        sm = oracle.march_generate(nr, R, ns, bbox, 1.0, bits, 1, 128, dt, dt, 1 / 256, rays, offs, alive, n_steps)
+        feat = np.zeros(32 * ns, np.float32)
+        lib.oc_hashgrid_fwd_f32(C.addressof(g), ns, ns, sm["pos"], params[:g.n_params], feat)
+        feat = feat.reshape(32, ns)
+        dir01 = sm["dir"].reshape(3, ns)
+        dens, rgb, _, _ = oracle.mlp_train(params[g.n_params:], feat, dir01, None, None, None, 1.0, fp64=False)
+        bg = rnd[R:4 * R]
+        loss, dLd, dLc = oracle.composite_train(nr, R, ns, n_steps, offs, bg, rays["pix_rgba"], dens, rgb.ravel(),
+                                                sm["dt"], fp64=False)
+        gnet = np.zeros(9408, np.float32)
+        _, _, dfeat, _ = oracle.mlp_train(params[g.n_params:], feat, dir01, dLc.reshape(3, ns), dLd, sm["t"], 1.0,
+                                          fp64=False, dW=gnet)
+        grads[:] = 0
+        grads[g.n_params:] = gnet
+        lib.oc_hashgrid_bwd_f32(C.addressof(g), ns, ns, sm["pos"], np.ascontiguousarray(dfeat.ravel()),
+                                grads[:g.n_params])
+        lr = 1e-2
+        wl = params.copy()
+        lib.oc_adam_step(params.size, g.n_params, params.size, 1.0, lr, 0.9, 0.99, 1e-15, 1e-6, params, wl, grads,
+                         m1, m2, steps)
+        return ns
+
+    step()  # warm-up
+    times, counts = [], []
+    for _ in range(args.cpu_runs):
+        t0 = time.perf_counter()
+        n = step()
+        times.append(time.perf_counter() - t0)
+        counts.append(n)
+    k = int(np.argsort(times)[len(times) // 2])
+    rate = counts[k] / times[k]
+    cores = oracle.num_threads()
+    log(f"[bench cpu] {rate / 1e6:.3f} M samples/s on {cores} threads ({counts[k]} samples in {times[k]:.2f} s)")
+    return dict(value=round(rate, 1), unit="samples/s", cores=cores, kind="port",
+                sample=f"one training step of {counts[k]} samples ({R} rays) on the same scene and occupancy "
+                       f"grid, fp32, median of {args.cpu_runs} after 1 warm-up")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build has no multi-GPU "
+              f"path; the per-step gradient all-reduce it would need is described in the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a GPU")
+    from turbonerf_amd import PyTurboNeRF as tn
+    from turbonerf_amd import ops, synthetic
+
+    torch.cuda.init()
+    scene = make_scene()
+    precs = {"mixed": [1], "fp32": [0], "both": [1, 0]}[args.precision]
+    results = {}
+    for p in precs:
+        results[p] = run_precision(tn, synthetic, scene, p, args)
+    main_p = precs[0]
+    main = results[main_p]
+    dtype = "fp16" if main_p == 1 else "fp32"
+    n_params = ops.hashgrid_layout().n_params + 9408
+    roof, table = roofline(main, n_params, dtype)
+    extra = dict(phases_ms_per_step=table, occupancy_update_ms=round(main["occupancy_update_ms"], 3),
+                 occupied_fraction=round(main["occupied_fraction"], 4), loss=round(main["loss"], 5),
+                 samples_per_step=round(main["samples_per_step"], 1), rays_per_step=round(main["rays_per_step"], 1))
+    if 0 in results and main_p != 0:
+        f = results[0]
+        extra["fp32_samples_per_s"] = round(f["samples_per_s"], 1)
+        extra["fp32_ms_per_step"] = round(f["ms_per_step"], 4)
+        extra["mixed_over_fp32"] = round(main["samples_per_s"] / f["samples_per_s"], 3)
+        extra["fp32_roofline"] = roofline(f, n_params, "fp32")[0]
+    if not args.no_render:
+        mr, ms = render_rate(tn, synthetic, scene, main["trainer"])
+        extra["render_mrays_per_s"] = round(mr, 2)
+        extra["render_ms_per_frame"] = round(ms, 3)
+        log(f"[bench] render {mr:.1f} Mrays/s ({ms:.2f} ms per {RES}x{RES} frame)")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene, main["bitfield"], main["rays_per_step"], args)
+    line = {
+        "metric": METRIC,
+        "value": round(main["samples_per_s"], 1),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic: 8-blob Gaussian field rendered on the GPU at 800x800 x 100 cameras; random-init network",
+        "config": {"workload": "nerf-train-blob8-800x800x100-batch2^18",
+                   "hash_grid": "16 levels x 2 features, 2^19 entries/level, N 16->2048",
+                   "mlp": "density 32-64-16, colour 32-64-64-3", "batch_samples": BATCH,
+                   "optimizer": "Adam lr 1e-2 betas (0.9, 0.99) eps 1e-15", "loss": "smooth-L1",
+                   "precision": "fp16 MFMA, fp32 accumulate, fp32 master weights" if main_p == 1 else "fp32"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    for p in results:
+        results[p]["trainer"][2].teardown()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

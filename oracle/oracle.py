@@ -69,6 +69,12 @@ def lib():
             "oc_hashgrid_fwd_f64": (None, [C.c_void_p, u32, u32, f32p, f64p, f64p]),
             "oc_hashgrid_bwd_f64": (None, [C.c_void_p, u32, u32, f32p, f64p, f64p]),
             "oc_sh4_f64": (None, [u32, u32, f32p, f64p]),
+            "oc_hashgrid_fwd_f32": (None, [C.c_void_p, u32, u32, f32p, f32p, f32p]),
+            "oc_hashgrid_bwd_f32": (None, [C.c_void_p, u32, u32, f32p, f32p, f32p]),
+            "oc_mlp_train_f32": (None, [u32, u32, f32p, f32p, f32p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        f32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+            "oc_mlp_train_f64": (None, [u32, u32, f64p, f64p, f32p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        f64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
             "oc_nerf_fwd_f64": (None, [C.c_void_p, u32, u32, f32p, f32p, f64p, i32, f64p, f64p]),
             "oc_blob_eval": (None, [u32, f32p, u32, f32p, f32p, f32p]),
             "oc_composite_analytic_f64": (None, [u32, u32p, u32p, f32p, u32, f32, f32p, u32, f32p, f64, f64p]),
@@ -232,6 +238,33 @@ def composite_analytic(n_rays, steps, offs, pos, stride, aabb, dt, blobs, min_T=
     lib().oc_composite_analytic_f64(n_rays, c(steps, np.uint32), c(offs, np.uint32), c(pos, np.float32), stride,
                                     aabb, c(dt, np.float32), len(blobs), c(blobs, np.float32).ravel(), min_T, out)
     return out.reshape(4, n_rays)
+
+
+def mlp_train(net, feat, dir01, G, dL_ds, t, loss_scale, extra=None, fp64=True, dW=None):
+    """MLP forward (+ backward when G is given) over n samples. feat (32, n),
+    dir01 (3, n), G = dL/drgb (3, n) or None, dL_ds / t / extra (n,). Returns
+    (density s0 (n,), rgb (3, n), dfeat (32, n), dW (9408,)); dW accumulates
+    into the given array."""
+    dt_ = np.float64 if fp64 else np.float32
+    n = np.shape(feat)[1]
+    dens = np.zeros(n, dt_)
+    rgb = np.zeros((3, n), dt_)
+    dfeat = np.zeros((32, n), dt_)
+    if dW is None:
+        dW = np.zeros(9408, dt_)
+    keep = []
+
+    def p(a, t_):
+        if a is None:
+            return None
+        a = c(a, t_)
+        keep.append(a)
+        return a.ctypes.data
+
+    fn = lib().oc_mlp_train_f64 if fp64 else lib().oc_mlp_train_f32
+    fn(n, n, c(net, dt_), c(feat, dt_), c(dir01, np.float32), p(G, dt_), p(dL_ds, np.float32), p(extra, np.float32),
+       p(t, np.float32), loss_scale, dens.ctypes.data, rgb.ctypes.data, dfeat.ctypes.data, dW.ctypes.data)
+    return dens, rgb, dfeat, dW
 
 
 def num_threads() -> int:

@@ -774,6 +774,118 @@ extern "C" void oc_nerf_fwd_f64(const tn_hashgrid_t* g, uint32_t n, uint32_t B, 
     }
 }
 
+/* fp32 hash-grid encoding over an fp32 table (the CPU baseline's training
+ * step); backward scatter-adds with atomics, as every parallel CPU port must. */
+extern "C" void oc_hashgrid_fwd_f32(const tn_hashgrid_t* g, uint32_t n, uint32_t B, const float* pos,
+    const float* table, float* feat) {
+    const int F = g->n_features;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        for (int l = 0; l < g->n_levels; ++l) {
+            float acc[8] = {0};
+            hash_visit(g, l, pos[i], pos[i + B], pos[i + 2 * B], [&](uint32_t e, double w) {
+                for (int f = 0; f < F; ++f) acc[f] += (float)w * table[(size_t)e * F + f];
+            });
+            for (int f = 0; f < F; ++f) feat[(size_t)(l * F + f) * B + i] = acc[f];
+        }
+    }
+}
+
+extern "C" void oc_hashgrid_bwd_f32(const tn_hashgrid_t* g, uint32_t n, uint32_t B, const float* pos,
+    const float* dfeat, float* dtable) {
+    const int F = g->n_features;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        for (int l = 0; l < g->n_levels; ++l) {
+            hash_visit(g, l, pos[i], pos[i + B], pos[i + 2 * B], [&](uint32_t e, double w) {
+                for (int f = 0; f < F; ++f) {
+                    const float v = (float)w * dfeat[(size_t)(l * F + f) * B + i];
+#pragma omp atomic
+                    dtable[(size_t)e * F + f] += v;
+                }
+            });
+        }
+    }
+}
+
+/* Density + colour MLP forward and backward for one batch of samples
+ * (reference nerf-network.cu:431-545 forward, :668-768 backward):
+ *   dL/do = G_rgb * rgb * (1 - rgb) through the sigmoid;
+ *   dS = W3^T dH2 restricted to the 16 density outputs;
+ *   dS[0] += loss_scale * (dL/ds + extra); then dS[0] *= t^2 where t < 1;
+ *   weight gradients accumulate into dW (+=), feature gradients go to dfeat.
+ * R is the arithmetic type. OpenMP over samples with one dW accumulator per
+ * thread, summed in thread order. */
+template <typename R>
+static void mlp_train(uint32_t n, uint32_t B, const R* net, const R* feat, const float* dir01, const R* G,
+    const float* dL_ds, const float* extra, const float* t, R loss_scale, R* dens_out, R* rgb_out, R* dfeat, R* dW) {
+    const R* W1 = net;
+    const R* W2 = W1 + 64 * 32;
+    const R* W3 = W2 + 16 * 64;
+    const R* W4 = W3 + 64 * 32;
+    const R* W5 = W4 + 64 * 64;
+    const int NP = 9408;
+    const int nt = omp_get_max_threads();
+    std::vector<R> part((size_t)nt * NP, (R)0);
+#pragma omp parallel
+    {
+        R* gw = part.data() + (size_t)omp_get_thread_num() * NP;
+        R* g1 = gw; R* g2 = g1 + 64 * 32; R* g3 = g2 + 16 * 64; R* g4 = g3 + 64 * 32; R* g5 = g4 + 64 * 64;
+#pragma omp for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+            const uint32_t i = (uint32_t)ii;
+            R x[32], h1[64], s[16], xc[32], h2[64], h3[64], y[3];
+            for (int k = 0; k < 32; ++k) x[k] = feat[(size_t)k * B + i];
+            for (int r = 0; r < 64; ++r) { R a = 0; for (int k = 0; k < 32; ++k) a += W1[r * 32 + k] * x[k]; h1[r] = a > 0 ? a : 0; }
+            for (int r = 0; r < 16; ++r) { R a = 0; for (int k = 0; k < 64; ++k) a += W2[r * 64 + k] * h1[k]; s[r] = a; }
+            double shd[16];
+            const float d3[3] = {dir01[i], dir01[i + B], dir01[i + 2 * B]};
+            oc_sh4_f64(1, 1, d3, shd);
+            for (int k = 0; k < 16; ++k) { xc[k] = s[k]; xc[16 + k] = (R)shd[k]; }
+            for (int r = 0; r < 64; ++r) { R a = 0; for (int k = 0; k < 32; ++k) a += W3[r * 32 + k] * xc[k]; h2[r] = a > 0 ? a : 0; }
+            for (int r = 0; r < 64; ++r) { R a = 0; for (int k = 0; k < 64; ++k) a += W4[r * 64 + k] * h2[k]; h3[r] = a > 0 ? a : 0; }
+            for (int r = 0; r < 3; ++r) { R a = 0; for (int k = 0; k < 64; ++k) a += W5[r * 64 + k] * h3[k]; y[r] = (R)1 / ((R)1 + std::exp(-a)); }
+            if (dens_out) dens_out[i] = s[0];
+            if (rgb_out) for (int r = 0; r < 3; ++r) rgb_out[(size_t)r * B + i] = y[r];
+            if (!G) continue;  // forward only
+            // backward
+            R dO[3], dH3[64], dH2[64], dS[16], dH1[64];
+            for (int r = 0; r < 3; ++r) dO[r] = G[(size_t)r * B + i] * y[r] * ((R)1 - y[r]);
+            for (int r = 0; r < 3; ++r) for (int k = 0; k < 64; ++k) g5[r * 64 + k] += dO[r] * h3[k];
+            for (int k = 0; k < 64; ++k) { R a = 0; for (int r = 0; r < 3; ++r) a += W5[r * 64 + k] * dO[r]; dH3[k] = h3[k] > 0 ? a : 0; }
+            for (int r = 0; r < 64; ++r) for (int k = 0; k < 64; ++k) g4[r * 64 + k] += dH3[r] * h2[k];
+            for (int k = 0; k < 64; ++k) { R a = 0; for (int r = 0; r < 64; ++r) a += W4[r * 64 + k] * dH3[r]; dH2[k] = h2[k] > 0 ? a : 0; }
+            for (int r = 0; r < 64; ++r) for (int k = 0; k < 32; ++k) g3[r * 32 + k] += dH2[r] * xc[k];
+            for (int k = 0; k < 16; ++k) { R a = 0; for (int r = 0; r < 64; ++r) a += W3[r * 32 + k] * dH2[r]; dS[k] = a; }
+            R g0 = dS[0] + loss_scale * (R)dL_ds[i];
+            if (extra) g0 += loss_scale * (R)extra[i];
+            if (t[i] < 1.0f) g0 *= (R)t[i] * (R)t[i];
+            dS[0] = g0;
+            for (int r = 0; r < 16; ++r) for (int k = 0; k < 64; ++k) g2[r * 64 + k] += dS[r] * h1[k];
+            for (int k = 0; k < 64; ++k) { R a = 0; for (int r = 0; r < 16; ++r) a += W2[r * 64 + k] * dS[r]; dH1[k] = h1[k] > 0 ? a : 0; }
+            for (int r = 0; r < 64; ++r) for (int k = 0; k < 32; ++k) g1[r * 32 + k] += dH1[r] * x[k];
+            for (int k = 0; k < 32; ++k) { R a = 0; for (int r = 0; r < 64; ++r) a += W1[r * 32 + k] * dH1[r]; dfeat[(size_t)k * B + i] = a; }
+        }
+    }
+    if (G)
+        for (int th = 0; th < nt; ++th)
+            for (int e = 0; e < NP; ++e) dW[e] += part[(size_t)th * NP + e];
+}
+
+extern "C" void oc_mlp_train_f32(uint32_t n, uint32_t B, const float* net, const float* feat, const float* dir01,
+    const float* G, const float* dL_ds, const float* extra, const float* t, float loss_scale, float* dens_out,
+    float* rgb_out, float* dfeat, float* dW) {
+    mlp_train<float>(n, B, net, feat, dir01, G, dL_ds, extra, t, loss_scale, dens_out, rgb_out, dfeat, dW);
+}
+
+extern "C" void oc_mlp_train_f64(uint32_t n, uint32_t B, const double* net, const double* feat, const float* dir01,
+    const double* G, const float* dL_ds, const float* extra, const float* t, double loss_scale, double* dens_out,
+    double* rgb_out, double* dfeat, double* dW) {
+    mlp_train<double>(n, B, net, feat, dir01, G, dL_ds, extra, t, loss_scale, dens_out, rgb_out, dfeat, dW);
+}
+
 /* ===================================================================== */
 /* Analytic Gaussian-blob field used for synthetic scenes. blob k occupies */
 /* 8 floats: centre xyz (world), sigma s, peak p, rgb.                     */

@@ -102,6 +102,22 @@ void oc_hashgrid_bwd_f64(const tn_hashgrid_t* g, uint32_t n, uint32_t stride, co
 void oc_sh4_f64(uint32_t n, uint32_t stride, const float* dir01, double* out);
 void oc_nerf_fwd_f64(const tn_hashgrid_t* g, uint32_t n, uint32_t stride, const float* pos,
     const float* dir01, const double* params, int color, double* density_out16, double* rgb);
+/* fp32 hash grid over an fp32 table (CPU-baseline training step) */
+void oc_hashgrid_fwd_f32(const tn_hashgrid_t* g, uint32_t n, uint32_t stride, const float* pos,
+    const float* table, float* feat);
+void oc_hashgrid_bwd_f32(const tn_hashgrid_t* g, uint32_t n, uint32_t stride, const float* pos,
+    const float* dfeat, float* dtable);
+/* MLP forward + backward: net = 9408 weights (W1..W5), feat [32][stride],
+ * dir01 [3][stride], G = dL/drgb (loss-scaled) [3][stride] or NULL for the
+ * forward only, dL_ds / extra / t [stride] (extra may be NULL); dens_out (raw
+ * density output s0) [stride] and rgb_out [3][stride] may be NULL; dfeat
+ * [32][stride], dW += [9408]. */
+void oc_mlp_train_f32(uint32_t n, uint32_t stride, const float* net, const float* feat, const float* dir01,
+    const float* G, const float* dL_ds, const float* extra, const float* t, float loss_scale, float* dens_out,
+    float* rgb_out, float* dfeat, float* dW);
+void oc_mlp_train_f64(uint32_t n, uint32_t stride, const double* net, const double* feat, const float* dir01,
+    const double* G, const float* dL_ds, const float* extra, const float* t, double loss_scale, double* dens_out,
+    double* rgb_out, double* dfeat, double* dW);
 
 /* --- analytic blob field (synthetic scenes) --------------------------- */
 void oc_blob_eval(uint32_t n, const float* xyz, uint32_t n_blobs, const float* blobs,

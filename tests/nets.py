@@ -34,30 +34,32 @@ def split(net):
 
 def _q(a, on):
     """Round to fp16 where the mixed-precision kernel feeds a matrix-core operand."""
-    return np.asarray(a, np.float16).astype(np.float64) if on else a
+    return np.asarray(a, np.float16).astype(np.asarray(a).dtype) if on else a
 
 
-def forward(net, feat, dir01, round16=False):
-    """round16: round every matrix operand to fp16 as the fp16 MFMA kernels do
-    (accumulation stays exact)."""
-    W1, W2, W3, W4, W5 = split(net)
-    feat = _q(np.asarray(feat, np.float64), round16)
+def forward(net, feat, dir01, round16=False, acc=np.float64):
+    """round16: round every matrix operand to fp16 as the fp16 MFMA kernels do;
+    acc: the accumulation type (np.float32 emulates the matrix cores' fp32
+    accumulators, np.float64 is exact to working precision)."""
+    W1, W2, W3, W4, W5 = (w.astype(acc) for w in split(net))
+    feat = _q(np.asarray(feat, acc), round16)
     h1 = np.maximum(feat @ W1.T, 0)
     s16 = _q(h1, round16) @ W2.T
-    xc = np.concatenate([_q(s16, round16), _q(sh4(dir01), round16)], axis=1)
+    xc = np.concatenate([_q(s16, round16), _q(sh4(dir01).astype(acc), round16)], axis=1)
     h2 = np.maximum(xc @ W3.T, 0)
     h3 = np.maximum(_q(h2, round16) @ W4.T, 0)
     o = _q(h3, round16) @ W5.T
     rgb = 1.0 / (1.0 + np.exp(-o))
-    return dict(h1=h1, s16=s16, xc=xc, h2=h2, h3=h3, o=o, rgb=rgb, feat=feat, round16=round16)
+    return dict(h1=h1, s16=s16, xc=xc, h2=h2, h3=h3, o=o, rgb=rgb, feat=feat, round16=round16, acc=acc)
 
 
 def backward(net, fw, G_rgb, dL_ds, t, loss_scale, extra=None):
     """Returns (dfeat (n,32), grad_net (9408,))."""
-    W1, W2, W3, W4, W5 = split(net)
+    acc = fw.get("acc", np.float64)
+    W1, W2, W3, W4, W5 = (w.astype(acc) for w in split(net))
     r = fw.get("round16", False)
     rgb = fw["rgb"]
-    dO = _q(G_rgb * rgb * (1 - rgb), r)
+    dO = _q(np.asarray(G_rgb, acc) * rgb * (1 - rgb), r)
     h1, h2, h3 = _q(fw["h1"], r), _q(fw["h2"], r), _q(fw["h3"], r)
     dW5 = dO.T @ h3
     dH3 = _q((dO @ W5) * (fw["h3"] > 0), r)
@@ -66,10 +68,10 @@ def backward(net, fw, G_rgb, dL_ds, t, loss_scale, extra=None):
     dW3 = dH2.T @ fw["xc"]
     dXc = dH2 @ W3
     dS = dXc[:, :16].copy()
-    g0 = dS[:, 0] + loss_scale * np.asarray(dL_ds, np.float64)
+    g0 = dS[:, 0] + acc(loss_scale) * np.asarray(dL_ds, acc)
     if extra is not None:
-        g0 += loss_scale * np.asarray(extra, np.float64)
-    t = np.asarray(t, np.float64)
+        g0 += acc(loss_scale) * np.asarray(extra, acc)
+    t = np.asarray(t, acc)
     dS[:, 0] = np.where(t < 1.0, g0 * t * t, g0)
     dS = _q(dS, r)
     dW2 = dS.T @ h1

@@ -421,6 +421,68 @@ def test_numpy_mlp_backward_matches_autograd():
     np.testing.assert_allclose(dX, X.grad.numpy(), rtol=1e-10, atol=1e-12)
 
 
+def test_oracle_mlp_train_matches_numpy_reference():
+    """The C MLP forward+backward (CPU-baseline training step) against the
+    autograd-pinned numpy restatement: fp64 to 1e-12, fp32 to 1e-4."""
+    from nets import backward, forward, random_net
+
+    rng = np.random.default_rng(12)
+    n = 300
+    net = random_net(5)
+    feat = rng.normal(scale=0.5, size=(n, 32))
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    dir01 = (0.5 * d + 0.5).astype(np.float32)
+    G = rng.normal(size=(n, 3))
+    dls = rng.normal(scale=1e-2, size=n).astype(np.float32)
+    ex = rng.normal(scale=1e-3, size=n).astype(np.float32)
+    t = rng.uniform(0.2, 2.0, n).astype(np.float32)
+    fw = forward(net, feat, dir01)
+    dX, gr = backward(net, fw, G, dls, t, 5.0, ex)
+    dens, rgb, dfeat, dW = oracle.mlp_train(net, feat.T, dir01.T, G.T, dls, t, 5.0, extra=ex)
+    np.testing.assert_allclose(rgb.T, fw["rgb"], rtol=1e-12)
+    np.testing.assert_allclose(dens, fw["s16"][:, 0], rtol=1e-12, atol=1e-14)
+    dens_f, rgb_f, _, dW_f = oracle.mlp_train(net, feat.T, dir01.T, None, None, None, 5.0)
+    assert np.array_equal(dens_f, dens) and np.array_equal(rgb_f, rgb) and not dW_f.any()
+    np.testing.assert_allclose(dW, gr, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(dfeat.T, dX, rtol=1e-9, atol=1e-12)
+    _, _, dfeat32, dW32 = oracle.mlp_train(net, feat.T, dir01.T, G.T, dls, t, 5.0, extra=ex, fp64=False)
+    assert np.linalg.norm(dW32 - gr) / np.linalg.norm(gr) < 1e-4
+    assert np.linalg.norm(dfeat32.T - dX) / np.linalg.norm(dX) < 1e-4
+
+
+def test_oracle_hashgrid_f32_matches_f64():
+    import ctypes as C
+
+    g = _capi_hashgrid()
+    rng = np.random.default_rng(13)
+    n = 2000
+    pos = rng.uniform(0, 1, size=3 * n).astype(np.float32)
+    table = rng.uniform(-1, 1, size=g.n_params).astype(np.float32)
+    f64 = np.zeros(32 * n)
+    f32 = np.zeros(32 * n, np.float32)
+    oracle.lib().oc_hashgrid_fwd_f64(C.addressof(g), n, n, pos, table.astype(np.float64), f64)
+    oracle.lib().oc_hashgrid_fwd_f32(C.addressof(g), n, n, pos, table, f32)
+    np.testing.assert_allclose(f32, f64, rtol=1e-5, atol=1e-6)
+    dfeat = rng.normal(size=32 * n)
+    b64 = np.zeros(g.n_params)
+    b32 = np.zeros(g.n_params, np.float32)
+    oracle.lib().oc_hashgrid_bwd_f64(C.addressof(g), n, n, pos, dfeat, b64)
+    oracle.lib().oc_hashgrid_bwd_f32(C.addressof(g), n, n, pos, dfeat.astype(np.float32), b32)
+    assert np.linalg.norm(b32 - b64) / np.linalg.norm(b64) < 1e-6
+
+
+def _capi_hashgrid():
+    import ctypes as C
+    import math
+
+    from turbonerf_amd import _capi
+
+    g = _capi.HashGrid()
+    b = math.exp((math.log(2048.0) - math.log(16.0)) / 15.0)
+    oracle.lib().oc_hashgrid_layout(16, 2, 19, 16, b, C.addressof(g))
+    return g
+
+
 # --------------------------------------------------------------- C ABI surface
 def test_c_abi_exports_every_declared_symbol():
     import re
