@@ -126,7 +126,9 @@ def run_precision(tn, synthetic, scene, precision, args):
         tr.train_steps(k, read_metrics=False)
         step += k
         if step % 16 == 0:
-            tr.update_occupancy_grid(step)
+            om = tr.update_occupancy_grid(step)
+            if step % 512 == 0:
+                log(f"[bench {name}] warm-up step {step}: occupied {om.n_occupied / om.n_total:.3f}")
     m = tr.train_step()
     step += 1
     log(f"[bench {name}] warm-up done: loss {m.loss:.4f} rays {m.n_rays} samples {m.n_samples}")

@@ -10,10 +10,11 @@
 // modulo the level size. The table is entry-major: entry e's F features are
 // contiguous. Features are written feature-major, out[(l*F+f)*stride + i].
 //
-// Launch shape: one thread per (sample, level), blockIdx.y = level, so a
-// wave sees 64 consecutive samples of one level. Consecutive samples of a
-// ray share coarse-level cells; the backward pass merges runs of equal corner
-// indices across the wave with a segmented sum before issuing its atomics.
+// Launch shape: blockIdx.y = level; the forward pass runs one thread per
+// (sample, level), the backward pass two (one per feature). Consecutive
+// samples of a ray share coarse-level cells; the backward pass merges runs of
+// equal corner indices across the wave with a segmented sum before issuing
+// its atomics.
 // Cell selection (x*scale + 0.5, floor) is kept free of contraction so it
 // matches a CPU evaluation of the same expression exactly.
 #pragma clang fp contract(off)
@@ -89,26 +90,31 @@ __global__ void __launch_bounds__(256) hashgrid_fwd_kernel(uint32_t n_max, const
     out[(uint64_t)(2 * l + 1) * OB + i] = (TO)f1;
 }
 
-// dtable[e] += sum over samples of w * dL/dfeature; runs of lanes that hit
-// the same entry are merged first, then the run's last lane adds its sum.
+// dtable[e] += sum over samples of w * dL/dfeature. Two lanes per sample,
+// one per feature, so each atomic instruction updates both features of an
+// entry in one 8-byte span (half the memory-side requests of one lane per
+// sample). A wave covers 32 consecutive samples; runs of consecutive samples
+// that hit the same entry are merged with a segmented sum first, and a wave
+// whose samples all differ skips the merge.
 template <typename TG>
 __global__ void __launch_bounds__(256) hashgrid_bwd_kernel(uint32_t n_max, const uint32_t* __restrict__ info,
                                                            uint32_t B, uint32_t GB, tn_hashgrid_t g,
                                                            const float* __restrict__ pos, const TG* __restrict__ dfeat,
                                                            float* __restrict__ dtable) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = tid >> 1;
+    const uint32_t f = tid & 1u;
     const int l = blockIdx.y;
     const uint32_t n = info ? min(info[1], n_max) : n_max;
     // whole waves past the end leave; partial waves keep every lane for the shuffles
-    if ((i & ~63u) >= n) return;
+    if ((i & ~31u) >= n) return;
     const bool valid = i < n;
     const uint32_t lane = __lane_id();
     HashCorners hc;
-    float g0 = 0.0f, g1 = 0.0f;
+    float gf = 0.0f;
     if (valid) {
         hash_corners(g, l, pos[i], pos[i + B], pos[i + 2 * (uint64_t)B], hc);
-        g0 = (float)dfeat[(uint64_t)(2 * l) * GB + i];
-        g1 = (float)dfeat[(uint64_t)(2 * l + 1) * GB + i];
+        gf = (float)dfeat[(uint64_t)(2 * l + f) * GB + i];
     } else {
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
@@ -116,34 +122,30 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_kernel(uint32_t n_max, const
             hc.w[c] = 0.0f;
         }
     }
+    const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        // runs are maximal stretches of consecutive lanes with one entry; the
-        // same entry may recur further on (hash collisions), so runs are keyed
-        // by their ordinal, not by the entry
-        const int ent = valid ? (int)hc.idx[c] : -1 - (int)lane;
-        const int prev = __shfl_up(ent, 1, 64);
-        const bool head = lane == 0 || prev != ent;
+        // runs are maximal stretches of consecutive samples with one entry;
+        // the same entry may recur further on (hash collisions), so runs are
+        // keyed by their ordinal, not by the entry
+        const int ent = valid ? (int)hc.idx[c] : -1 - (int)(lane >> 1);
+        const int prev = __shfl_up(ent, 2, 64);
+        const bool head = lane < 2 || prev != ent;
         const uint64_t hm = __ballot(head);
-        const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-        const int key = __builtin_popcountll(hm & le);
-        float a = hc.w[c] * g0, b = hc.w[c] * g1;
+        float a = hc.w[c] * gf;
+        if (hm != ~0ull) {
+            const int key = __builtin_popcountll(hm & le & 0x5555555555555555ull);
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const float oa = __shfl_up(a, d, 64);
-            const float ob = __shfl_up(b, d, 64);
-            const int ok = __shfl_up(key, d, 64);
-            if ((int)lane >= d && ok == key) {
-                a += oa;
-                b += ob;
+            for (int d = 2; d < 64; d <<= 1) {
+                const float o = __shfl_up(a, d, 64);
+                const int ok = __shfl_up(key, d, 64);
+                if ((int)lane >= d && ok == key) a += o;
             }
-        }
-        const int next = __shfl_down(ent, 1, 64);
-        const bool tail = valid && (lane == 63 || next != ent);
-        if (tail && (a != 0.0f || b != 0.0f)) {
-            float* p = dtable + 2 * (uint64_t)hc.idx[c];
-            atomicAdd(p, a);
-            atomicAdd(p + 1, b);
+            const int next = __shfl_down(ent, 2, 64);
+            const bool tail = valid && (lane >= 62 || next != ent);
+            if (tail && a != 0.0f) atomicAdd(dtable + 2 * (uint64_t)hc.idx[c] + f, a);
+        } else if (valid && a != 0.0f) {
+            atomicAdd(dtable + 2 * (uint64_t)hc.idx[c] + f, a);
         }
     }
 }
@@ -220,7 +222,7 @@ extern "C" int tn_hashgrid_backward(int dfeat_dtype, uint32_t n_max, const uint3
     TN_CHECK_ARG(pos && dfeat && dtable && n_max <= stride && n_max <= dfeat_stride && grid.n_features == 2);
     TN_CHECK_ARG(grid.n_levels >= 1 && grid.n_levels <= TN_HASHGRID_MAX_LEVELS);
     if (n_max == 0) return TN_OK;
-    const dim3 blocks(blocks_for(n_max, 256), grid.n_levels);
+    const dim3 blocks(blocks_for(2 * (uint64_t)n_max, 256), grid.n_levels);
     if (dfeat_dtype == 0)
         hashgrid_bwd_kernel<float><<<blocks, 256, 0, stream>>>(n_max, info, stride, dfeat_stride, grid, pos,
                                                                static_cast<const float*>(dfeat), dtable);

@@ -31,9 +31,23 @@ TN_DEVICE MarchGrid make_grid(const tn_occupancy_grid_t& g) {
     return m;
 }
 
-TN_DEVICE bool occupied(const MarchGrid& g, int level, float x, float y, float z) {
+// Occupancy lookup with a one-entry cache: consecutive steps of a ray mostly
+// stay in one cell (a cell spans several steps), so the bitfield byte is
+// loaded only when the (level, cell) pair changes.
+struct CellCache {
+    uint32_t cell = 0xFFFFFFFFu;
+    int level = -1;
+    bool occ = false;
+};
+
+TN_DEVICE bool occupied(const MarchGrid& g, int level, float x, float y, float z, CellCache& cc) {
     const uint32_t m = voxel_morton_index(g.res_f, g.lim, level, x, y, z);
-    return (g.bits[m] >> level) & 1u;
+    if (m != cc.cell || level != cc.level) {
+        cc.cell = m;
+        cc.level = level;
+        cc.occ = (g.bits[m] >> level) & 1u;
+    }
+    return cc.occ;
 }
 
 __global__ void march_count_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays, uint32_t B,
@@ -56,12 +70,13 @@ __global__ void march_count_kernel(uint32_t n_rays_max, const uint32_t* __restri
     uint32_t n = 0;
     float t = ray_t[i];
     const float t_max = ray_tmax[i];
+    CellCache cc;
     while (t < t_max && n < max_steps) {
         const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
         if (!bbox_contains(bbox, x, y, z)) break;
         const float dt = get_dt(t, cone, dt_min, dt_max);
         const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
-        if (occupied(g, lvl, x, y, z)) {
+        if (occupied(g, lvl, x, y, z, cc)) {
             t += dt;
             ++n;
         } else {
@@ -193,68 +208,83 @@ __global__ void scan_write_select_kernel(uint32_t n, const uint32_t* __restrict_
 }
 
 __global__ void march_generate_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ info, uint32_t B,
-                                      uint32_t SB, tn_bbox_t bbox, float inv_aabb, tn_occupancy_grid_t grid, float dt_min,
-                                      float dt_max, float cone, const uint32_t* __restrict__ ray_img_id,
-                                      const float* __restrict__ ori, const float* __restrict__ dir,
+                                      tn_bbox_t bbox, tn_occupancy_grid_t grid, float dt_min, float dt_max,
+                                      float cone, const float* __restrict__ ori, const float* __restrict__ dir,
                                       const float* __restrict__ ray_t, const uint32_t* __restrict__ ray_offset,
                                       const uint8_t* __restrict__ ray_alive, const uint32_t* __restrict__ n_ray_steps,
-                                      uint32_t* __restrict__ out_img_id, float* __restrict__ out_t,
-                                      float* __restrict__ out_pos, float* __restrict__ out_dir,
-                                      float* __restrict__ out_dt, float* __restrict__ out_m_norm,
-                                      float* __restrict__ out_dt_norm) {
+                                      uint32_t* __restrict__ out_ray, float* __restrict__ out_t) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n_rays = info ? min(info[0], n_rays_max) : n_rays_max;
+    const uint32_t n_rays = min(info[0], n_rays_max);
     if (i >= n_rays || !ray_alive[i]) return;
     const MarchGrid g = make_grid(grid);
     const uint32_t n_steps = n_ray_steps[i];
     const uint32_t off = ray_offset[i];
-    const uint32_t img = ray_img_id[i];
     const float ox = ori[i], oy = ori[i + B], oz = ori[i + 2 * B];
     const float dx = dir[i], dy = dir[i + B], dz = dir[i + 2 * B];
     const float idx = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
-    const float ndx = 0.5f * dx + 0.5f, ndy = 0.5f * dy + 0.5f, ndz = 0.5f * dz + 0.5f;
     float t = ray_t[i];
-    const float t_min = t;
-    float t_max = t;
     uint32_t n = 0;
+    CellCache cc;
     while (n < n_steps) {
         const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
         const float dt = get_dt(t, cone, dt_min, dt_max);
+        // the count pass stops at the box boundary, so every counted sample
+        // is inside; a sample outside would be emitted without advancing
         const bool inside = bbox_contains(bbox, x, y, z);
-        bool emit = !inside;  // outside the box: emit without advancing (unreachable when passes agree)
+        bool emit = !inside;
         int lvl = 0;
         if (inside) {
             lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
-            emit = occupied(g, lvl, x, y, z);
+            emit = occupied(g, lvl, x, y, z, cc);
         }
         if (emit) {
-            const uint32_t sid = off + n;
-            out_t[sid] = t;
-            out_m_norm[sid] = t + 0.5f * dt;
-            out_dt_norm[sid] = dt;
-            out_img_id[sid] = img;
-            out_dt[sid] = inv_aabb * dt;
-            out_pos[sid] = fminf(fmaxf(x * inv_aabb + 0.5f, 0.0f), 1.0f);
-            out_pos[sid + SB] = fminf(fmaxf(y * inv_aabb + 0.5f, 0.0f), 1.0f);
-            out_pos[sid + 2 * SB] = fminf(fmaxf(z * inv_aabb + 0.5f, 0.0f), 1.0f);
-            out_dir[sid] = ndx;
-            out_dir[sid + SB] = ndy;
-            out_dir[sid + 2 * SB] = ndz;
-            if (inside) {
-                t += dt;
-                t_max = t;
-            }
+            out_t[off + n] = t;
+            out_ray[off + n] = i;
+            if (inside) t += dt;
             ++n;
         } else {
             t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, dx, dy, dz, idx, idy, idz, dt_min, lvl);
         }
     }
+}
+
+// Sample-parallel second half of generation: from each sample's t and ray
+// (left in out_img_id by the marching pass) write the network inputs and the
+// distortion loss's normalised midpoints and steps. The ray's span ends where
+// its last sample's step ends, fl(t_last + dt(t_last)), which is where the
+// marching pass's t stood after its last emission.
+__global__ void march_expand_kernel(uint32_t n_rays_max, uint32_t SB_max, const uint32_t* __restrict__ info,
+                                    uint32_t B, uint32_t SB, float inv_aabb, float dt_min, float dt_max, float cone,
+                                    const uint32_t* __restrict__ ray_img_id, const float* __restrict__ ori,
+                                    const float* __restrict__ dir, const float* __restrict__ ray_t,
+                                    const uint32_t* __restrict__ ray_offset, const uint32_t* __restrict__ n_ray_steps,
+                                    uint32_t* __restrict__ out_img_id, const float* __restrict__ out_t,
+                                    float* __restrict__ out_pos, float* __restrict__ out_dir, float* __restrict__ out_dt,
+                                    float* __restrict__ out_m_norm, float* __restrict__ out_dt_norm) {
+    const uint32_t sid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_samples = min(info[1], SB_max);
+    if (sid >= n_samples) return;
+    const uint32_t r = out_img_id[sid];
+    if (r >= n_rays_max) return;  // defensive: every sample below n_samples was written by the marching pass
+    const float t = out_t[sid];
+    const float dt = get_dt(t, cone, dt_min, dt_max);
+    const float x = ori[r] + t * dir[r];
+    const float y = ori[r + B] + t * dir[r + B];
+    const float z = ori[r + 2 * B] + t * dir[r + 2 * B];
+    const float t_min = ray_t[r];
+    const float t_last = out_t[ray_offset[r] + n_ray_steps[r] - 1u];
+    const float t_max = t_last + get_dt(t_last, cone, dt_min, dt_max);
     const float t_span = fmaxf(t_max - t_min, 1e-6f);
-    for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t sid = off + j;
-        out_m_norm[sid] = fminf(fmaxf((out_m_norm[sid] - t_min) / t_span, 0.0f), 1.0f);
-        out_dt_norm[sid] = out_dt_norm[sid] / t_span;
-    }
+    out_img_id[sid] = ray_img_id[r];
+    out_dt[sid] = inv_aabb * dt;
+    out_pos[sid] = fminf(fmaxf(x * inv_aabb + 0.5f, 0.0f), 1.0f);
+    out_pos[sid + SB] = fminf(fmaxf(y * inv_aabb + 0.5f, 0.0f), 1.0f);
+    out_pos[sid + 2 * SB] = fminf(fmaxf(z * inv_aabb + 0.5f, 0.0f), 1.0f);
+    out_dir[sid] = 0.5f * dir[r] + 0.5f;
+    out_dir[sid + SB] = 0.5f * dir[r + B] + 0.5f;
+    out_dir[sid + 2 * SB] = 0.5f * dir[r + 2 * B] + 0.5f;
+    out_m_norm[sid] = fminf(fmaxf(((t + 0.5f * dt) - t_min) / t_span, 0.0f), 1.0f);
+    out_dt_norm[sid] = dt / t_span;
 }
 
 // Next step's ray count: the rays that would fill the sample budget at this
@@ -342,8 +372,11 @@ extern "C" int tn_march_generate(uint32_t n_rays_max, const uint32_t* info, uint
     TN_CHECK_ARG(info != nullptr && sample_stride > 0);
     if (n_rays_max == 0) return TN_OK;
     march_generate_kernel<<<blocks_for(n_rays_max, 64), 64, 0, stream>>>(
-        n_rays_max, info, stride, sample_stride, bbox, inv_aabb, grid, dt_min, dt_max, cone_angle, ray_img_id, ori, dir, ray_t,
-        ray_offset, ray_alive, n_ray_steps, out_img_id, out_t, out_pos, out_dir, out_dt, out_m_norm, out_dt_norm);
+        n_rays_max, info, stride, bbox, grid, dt_min, dt_max, cone_angle, ori, dir, ray_t, ray_offset, ray_alive,
+        n_ray_steps, out_img_id, out_t);
+    march_expand_kernel<<<blocks_for(sample_stride, 256), 256, 0, stream>>>(
+        n_rays_max, sample_stride, info, stride, sample_stride, inv_aabb, dt_min, dt_max, cone_angle, ray_img_id, ori,
+        dir, ray_t, ray_offset, n_ray_steps, out_img_id, out_t, out_pos, out_dir, out_dt, out_m_norm, out_dt_norm);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }

@@ -574,14 +574,27 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         dw_partial[(uint64_t)blockIdx.x * kNetParams + e] = red[e];
 }
 
-// grad[e] (+)= sum over partials, fixed order
-__global__ void reduce_partials_kernel(uint32_t n_part, const float* __restrict__ partial, float* __restrict__ grad,
-                                       int accumulate) {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (uint32_t)kNetParams) return;
+// grad[e] (+)= sum over partials in a fixed order: a workgroup owns 64
+// consecutive elements, wave w sums partials w, w + 16, ... (one 256-byte row
+// segment per partial), then wave 0 adds the 16 wave sums in order.
+constexpr int kReduceWaves = 16;
+
+__global__ void __launch_bounds__(64 * kReduceWaves)
+reduce_partials_kernel(uint32_t n_part, const float* __restrict__ partial, float* __restrict__ grad, int accumulate) {
+    __shared__ float red[kReduceWaves][64];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t e = blockIdx.x * 64u + lane;
     float s = 0.0f;
-    for (uint32_t p = 0; p < n_part; ++p) s += partial[(uint64_t)p * kNetParams + e];
-    grad[e] = accumulate ? grad[e] + s : s;
+    if (e < (uint32_t)kNetParams)
+        for (uint32_t p = w; p < n_part; p += kReduceWaves) s += partial[(uint64_t)p * kNetParams + e];
+    red[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && e < (uint32_t)kNetParams) {
+        float t = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kReduceWaves; ++k) t += red[k][lane];
+        grad[e] = accumulate ? grad[e] + t : t;
+    }
 }
 
 template <class P>
@@ -690,7 +703,8 @@ extern "C" int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* in
     } else {
         return TN_ERR_INVALID_ARGUMENT;
     }
-    reduce_partials_kernel<<<blocks_for(kNetParams, 256), 256, 0, stream>>>(blocks, partial, grad_net, accumulate);
+    reduce_partials_kernel<<<blocks_for(kNetParams, 64), 64 * kReduceWaves, 0, stream>>>(blocks, partial, grad_net,
+                                                                                   accumulate);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }
