@@ -90,9 +90,12 @@ void NeRFTrainingController::setup_data(uint32_t batch_size) {
     nerf.occupancy_grid.set_aabb_scale(ds.bounding_box.size());
     nerf.occupancy_grid.initialize_if_needed(s);
     const uint32_t R = config.ray_capacity ? config.ray_capacity : batch_size;
-    ctx_->workspace.enlarge(batch_size, R);
+    TN_HIP(hipStreamSynchronize(ctx_->gen_stream));
+    ctx_->enlarge(batch_size, R);
+    ctx_->cur = ctx_->last = 0;
+    ctx_->prefetched = false;
     const uint32_t init = std::min(ctx_->initial_rays, R);
-    TN_HIP(hipMemcpyAsync(ctx_->workspace.n_rays_next.data(), &init, sizeof(init), hipMemcpyHostToDevice, s));
+    TN_HIP(hipMemcpyAsync(ctx_->n_rays_next.data(), &init, sizeof(init), hipMemcpyHostToDevice, s));
     ctx_->samples_total.resize(1);
     ctx_->samples_total.zero(s);
     nerf.network.workspace.enlarge(batch_size, R, config.network.precision);
@@ -228,11 +231,14 @@ uint64_t NeRFTrainingController::samples_trained() {
     return h;
 }
 
-void NeRFTrainingController::synchronize() { TN_HIP(hipStreamSynchronize(DeviceManager::stream())); }
+void NeRFTrainingController::synchronize() {
+    if (ctx_) TN_HIP(hipStreamSynchronize(ctx_->gen_stream));
+    TN_HIP(hipStreamSynchronize(DeviceManager::stream()));
+}
 
 std::vector<size_t> NeRFTrainingController::get_cuda_memory_allocated() const {
     size_t total = 0;
-    if (ctx_) total += ctx_->workspace.bytes();
+    if (ctx_) total += ctx_->bytes();
     const NeRF& nerf = proxy->nerfs[0];
     total += nerf.network.workspace.bytes();
     total += nerf.params.params_fp.bytes() + nerf.params.params_lp.bytes() + nerf.params.gradients.bytes() +

@@ -238,7 +238,6 @@ void TrainingWorkspace::enlarge(uint32_t B, uint32_t R) {
     sample_m_norm.resize(B);
     sample_dt_norm.resize(B);
     info.resize(4);
-    n_rays_next.resize(1);
     scan_ws.resize(std::max<size_t>(tn_scan_workspace_bytes(R), 4));
 }
 
@@ -268,10 +267,31 @@ void DatasetWorkspace::upload(const Dataset& ds, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ trainer
-void Trainer::generate_next_training_batch(Context& ctx) {
-    TrainingWorkspace& ws = ctx.workspace;
+Trainer::Context::Context(hipStream_t s, const Dataset* ds, NeRF* n, uint32_t B)
+    : stream(s), dataset(ds), nerf(n), batch_size(B) {
+    TN_HIP(hipStreamCreateWithFlags(&gen_stream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+        TN_HIP(hipEventCreateWithFlags(&ev_ready[k], hipEventDisableTiming));
+        TN_HIP(hipEventCreateWithFlags(&ev_consumed[k], hipEventDisableTiming));
+    }
+    TN_HIP(hipEventCreateWithFlags(&ev_grid, hipEventDisableTiming));
+    TN_HIP(hipEventCreateWithFlags(&ev_gen_idle, hipEventDisableTiming));
+}
+
+Trainer::Context::~Context() {
+    (void)hipStreamSynchronize(gen_stream);
+    (void)hipStreamSynchronize(stream);
+    for (int k = 0; k < 2; ++k) {
+        (void)hipEventDestroy(ev_ready[k]);
+        (void)hipEventDestroy(ev_consumed[k]);
+    }
+    (void)hipEventDestroy(ev_grid);
+    (void)hipEventDestroy(ev_gen_idle);
+    (void)hipStreamDestroy(gen_stream);
+}
+
+void Trainer::generate_next_training_batch(Context& ctx, TrainingWorkspace& ws, hipStream_t s) {
     NeRF& nerf = *ctx.nerf;
-    const hipStream_t s = ctx.stream;
     const tn_stream_t ts = as_tn(s);
     const uint32_t R = ws.ray_capacity, B = ws.batch_size;
     const Dataset& ds = *ctx.dataset;
@@ -285,7 +305,7 @@ void Trainer::generate_next_training_batch(Context& ctx) {
         ScopedPhase ph(tm, "rays", s);
         TN_TRY(tn_fill_uniform(4 * (uint64_t)R, ctx.rng_seed, ctx.rng_offset, ws.random_float.data(), ts));
         ctx.rng_offset += 4 * (uint64_t)R;
-        TN_TRY(tn_init_training_rays(R, ws.n_rays_next.data(), R, dw.n_images, dw.n_pixels_per_image,
+        TN_TRY(tn_init_training_rays(R, ctx.n_rays_next.data(), R, dw.n_images, dw.n_pixels_per_image,
                                      dw.image_dims.x, dw.image_dims.y, -1.0f,
                                      reinterpret_cast<const tn_bbox_t*>(dw.bbox.data()),
                                      reinterpret_cast<const tn_camera_t*>(dw.cameras.data()), dw.images.data(),
@@ -293,7 +313,7 @@ void Trainer::generate_next_training_batch(Context& ctx) {
                                      ws.ray_origin.data(), ws.ray_dir.data(), ws.ray_t.data(), ws.ray_t_max.data(),
                                      ws.ray_alive.data(), ts));
         if (ctx.alpha_selection_threshold < 1.0f && ctx.alpha_selection_probability < 1.0f)
-            TN_TRY(tn_deactivate_rays(R, ws.n_rays_next.data(), R, ctx.alpha_selection_threshold,
+            TN_TRY(tn_deactivate_rays(R, ctx.n_rays_next.data(), R, ctx.alpha_selection_threshold,
                                       ctx.alpha_selection_probability, ws.pix_rgba.data(), ws.random_float.data() + R,
                                       ws.ray_alive.data(), ts));
     }
@@ -303,13 +323,13 @@ void Trainer::generate_next_training_batch(Context& ctx) {
     const tn_occupancy_grid_t grid = nerf.occupancy_grid.view();
     {
         ScopedPhase ph(tm, "march_count", s);
-        TN_TRY(tn_march_count(R, ws.n_rays_next.data(), R, bbox, grid, NeRFConstants::cone_angle, dt_min, dt_max,
+        TN_TRY(tn_march_count(R, ctx.n_rays_next.data(), R, bbox, grid, NeRFConstants::cone_angle, dt_min, dt_max,
                               0xFFFFFFFFu, ws.ray_dir.data(), ws.ray_t_max.data(), ws.ray_alive.data(),
                               ws.ray_origin.data(), ws.ray_t.data(), ws.ray_step.data(), ts));
     }
     {
         ScopedPhase ph(tm, "scan_select", s);
-        TN_TRY(tn_scan_select(0, R, ws.n_rays_next.data(), ws.ray_step.data(), B, ws.ray_offset.data(),
+        TN_TRY(tn_scan_select(0, R, ctx.n_rays_next.data(), ws.ray_step.data(), B, ws.ray_offset.data(),
                               ws.info.data(), ws.scan_ws.data(), ws.scan_ws.size(), ts));
     }
     {
@@ -320,28 +340,46 @@ void Trainer::generate_next_training_batch(Context& ctx) {
                                  ws.ray_step.data(), ws.sample_img_id.data(), ws.sample_t.data(), ws.sample_pos.data(),
                                  ws.sample_dir.data(), ws.sample_dt.data(), ws.sample_m_norm.data(),
                                  ws.sample_dt_norm.data(), ts));
-        TN_TRY(tn_estimate_rays(ws.info.data(), B, R, 64, ws.n_rays_next.data(), ts));
-        if (ctx.samples_total.size()) TN_TRY(tn_accumulate_samples(ws.info.data(), ctx.samples_total.data(), ts));
+        TN_TRY(tn_estimate_rays(ws.info.data(), B, R, 64, ctx.n_rays_next.data(), ts));
     }
 }
 
 void Trainer::enqueue_train_step(Context& ctx, uint32_t training_step) {
-    generate_next_training_batch(ctx);
-    TrainingWorkspace& ws = ctx.workspace;
+    const int c = ctx.cur, nx = c ^ 1;
+    const hipStream_t gen = ctx.gen_stream, net = ctx.stream;
+    if (!ctx.prefetched) {
+        // first step, or the grid changed since the batch generated ahead
+        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_grid, 0));
+        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[c], 0));
+        generate_next_training_batch(ctx, ctx.batches[c], gen);
+        TN_HIP(hipEventRecord(ctx.ev_ready[c], gen));
+    }
+    // generate the next step's batch while this one trains
+    TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[nx], 0));
+    generate_next_training_batch(ctx, ctx.batches[nx], gen);
+    TN_HIP(hipEventRecord(ctx.ev_ready[nx], gen));
+
+    TrainingWorkspace& ws = ctx.batches[c];
+    TN_HIP(hipStreamWaitEvent(net, ctx.ev_ready[c], 0));
+    if (ctx.samples_total.size()) TN_TRY(tn_accumulate_samples(ws.info.data(), ctx.samples_total.data(), as_tn(net)));
     NerfNetwork::Settings st;
     st.use_distortion_loss = ctx.use_distortion_loss;
     st.distortion_lambda = ctx.distortion_lambda;
     st.timer = &ctx.timer;
-    ctx.nerf->network.train(ctx.stream, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity,
+    ctx.nerf->network.train(net, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity,
                             ws.info.data(), ws.random_float.data() + ws.ray_capacity, ws.ray_step.data(),
                             ws.ray_offset.data(), ws.sample_t.data(), ws.sample_pos.data(), ws.sample_dir.data(),
                             ws.sample_dt.data(), ws.sample_m_norm.data(), ws.sample_dt_norm.data(), ws.pix_rgba.data(),
                             st);
+    TN_HIP(hipEventRecord(ctx.ev_consumed[c], net));
+    ctx.last = c;
+    ctx.cur = nx;
+    ctx.prefetched = true;
 }
 
 void Trainer::read_metrics(Context& ctx, float& loss) {
     const hipStream_t s = ctx.stream;
-    TN_HIP(hipMemcpyAsync(ctx.info_host.data(), ctx.workspace.info.data(), 4 * sizeof(uint32_t),
+    TN_HIP(hipMemcpyAsync(ctx.info_host.data(), ctx.batches[ctx.last].info.data(), 4 * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, s));
     TN_HIP(hipMemcpyAsync(ctx.loss_host.data(), ctx.nerf->network.workspace.loss_out.data(), 2 * sizeof(float),
                           hipMemcpyDeviceToHost, s));
@@ -362,8 +400,14 @@ float Trainer::train_step(Context& ctx, uint32_t training_step) {
 uint32_t Trainer::update_occupancy_grid(Context& ctx, uint32_t training_step) {
     NeRF& nerf = *ctx.nerf;
     OccupancyGrid& og = nerf.occupancy_grid;
-    TrainingWorkspace& ws = ctx.workspace;
     const hipStream_t s = ctx.stream;
+    // marching of the batch generated ahead reads the grid: let it finish,
+    // then discard that batch (its workspace serves as scratch here) and
+    // make the next generation wait for the new grid
+    TN_HIP(hipEventRecord(ctx.ev_gen_idle, ctx.gen_stream));
+    TN_HIP(hipStreamWaitEvent(s, ctx.ev_gen_idle, 0));
+    ctx.prefetched = false;
+    TrainingWorkspace& ws = ctx.batches[ctx.cur];
     const tn_stream_t ts = as_tn(s);
     const float aabb = ctx.dataset->bounding_box.size();
     const uint32_t vol = og.volume();
@@ -389,6 +433,7 @@ uint32_t Trainer::update_occupancy_grid(Context& ctx, uint32_t training_step) {
         }
     }
     TN_TRY(tn_occ_update_bits(og.view(), ctx.occupancy_threshold, ts));
+    TN_HIP(hipEventRecord(ctx.ev_grid, s));
     return og.count_occupied(s);
 }
 

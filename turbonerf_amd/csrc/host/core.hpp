@@ -191,7 +191,7 @@ struct TrainingWorkspace {
     DeviceBuffer<uint8_t> ray_alive;
     DeviceBuffer<uint32_t> sample_img_id;
     DeviceBuffer<float> sample_t, sample_pos, sample_dir, sample_dt, sample_m_norm, sample_dt_norm;
-    DeviceBuffer<uint32_t> info, n_rays_next;
+    DeviceBuffer<uint32_t> info;
     DeviceBuffer<uint8_t> scan_ws;
     void enlarge(uint32_t B, uint32_t R);
     size_t bytes() const;
@@ -223,9 +223,21 @@ struct NeRF {
 };
 
 struct Trainer {
+    // Batch generation (rays, marching, scan, sample generation) needs only
+    // the occupancy grid and the dataset, not the network, so it runs one
+    // step ahead on its own stream into the other of two batch workspaces
+    // while the network trains on the current one. An occupancy update
+    // discards the batch generated ahead, so every batch is marched through
+    // the grid the reference would use.
     struct Context {
-        hipStream_t stream;
-        TrainingWorkspace workspace;
+        hipStream_t stream;      // network, optimiser and occupancy update
+        hipStream_t gen_stream;  // batch generation
+        TrainingWorkspace batches[2];
+        int cur = 0;              // batch the next step trains on
+        int last = 0;             // batch the last step trained on
+        bool prefetched = false;  // batches[cur] already holds a generated batch
+        hipEvent_t ev_ready[2]{}, ev_consumed[2]{}, ev_grid{}, ev_gen_idle{};
+        DeviceBuffer<uint32_t> n_rays_next;  // device-side ray count for the next batch
         const Dataset* dataset;
         NeRF* nerf;
         uint32_t batch_size;
@@ -247,7 +259,16 @@ struct Trainer {
         PhaseTimer timer;
         DeviceBuffer<unsigned long long> samples_total;  // running sum of samples trained on
 
-        Context(hipStream_t s, const Dataset* ds, NeRF* n, uint32_t B) : stream(s), dataset(ds), nerf(n), batch_size(B) {}
+        Context(hipStream_t s, const Dataset* ds, NeRF* n, uint32_t B);
+        ~Context();
+        Context(const Context&) = delete;
+        Context& operator=(const Context&) = delete;
+        void enlarge(uint32_t B, uint32_t R) {
+            batches[0].enlarge(B, R);
+            batches[1].enlarge(B, R);
+            n_rays_next.resize(1);
+        }
+        size_t bytes() const { return batches[0].bytes() + batches[1].bytes(); }
     };
 
     // Enqueue one training step; no host synchronisation.
@@ -255,8 +276,8 @@ struct Trainer {
     // One training step, synchronising like the reference: throws when the
     // batch has no rays/samples, returns the loss (reference trainer.cu:313-349).
     float train_step(Context& ctx, uint32_t training_step);
-    // Batch generation only (reference trainer.cu:39-217).
-    void generate_next_training_batch(Context& ctx);
+    // Batch generation into `ws` on `s` (reference trainer.cu:39-217).
+    void generate_next_training_batch(Context& ctx, TrainingWorkspace& ws, hipStream_t s);
     // Occupancy update; returns the number of occupied bits (reference trainer.cu:221-311).
     uint32_t update_occupancy_grid(Context& ctx, uint32_t training_step);
     // Read back batch counts and losses of the last enqueued step.
