@@ -610,7 +610,6 @@ constexpr size_t bwd_lds_bytes() {
     return (w + t) > r ? (w + t) : r;
 }
 
-constexpr unsigned kMlpMaxBlocks = 1024;
 
 // kernels using more than the default 64 KiB of dynamic LDS opt in once
 template <typename K>
@@ -620,11 +619,31 @@ inline bool allow_lds(K kernel, size_t bytes) {
                                (int)bytes) == hipSuccess;
 }
 
-inline unsigned mlp_blocks(uint32_t n_max) {
-    const uint32_t tiles = (n_max + 31) / 32;
-    unsigned b = blocks_for(tiles, kMlpWaves);
-    return b > kMlpMaxBlocks ? kMlpMaxBlocks : b;
+// Persistent launches: as many workgroups as fit on the device at once (the
+// backward kernel holds one workgroup per CU, the forward two), each looping
+// over 32-sample tiles. Fewer workgroups also means fewer weight stagings and
+// fewer gradient partials to reduce.
+inline unsigned compute_units() {
+    static int cached_dev = -1;
+    static unsigned cached_cus = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (dev != cached_dev) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        cached_cus = (unsigned)cus;
+        cached_dev = dev;
+    }
+    return cached_cus;
 }
+
+inline unsigned mlp_blocks(uint32_t n_max, unsigned per_cu) {
+    const uint32_t tiles = (n_max + 31) / 32;
+    const unsigned b = blocks_for(tiles, kMlpWaves);
+    const unsigned cap = per_cu * compute_units();
+    return b > cap ? cap : (b == 0 ? 1 : b);
+}
+constexpr unsigned kFwdBlocksPerCU = 2, kBwdBlocksPerCU = 1;
 
 }  // namespace tn
 
@@ -642,7 +661,7 @@ extern "C" int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* inf
     TN_CHECK_ARG(params && features && density_out && (!color || (dir && rgb_out)));
     TN_CHECK_ARG(n_max <= stride && n_max <= feat_stride && n_max <= out_stride);
     if (n_max == 0) return TN_OK;
-    const unsigned blocks = mlp_blocks(n_max);
+    const unsigned blocks = mlp_blocks(n_max, kFwdBlocksPerCU);
     if (precision == 1) {
         using P = PrecF16;
         const auto* w = static_cast<const P::T*>(params) + net_offset;
@@ -663,7 +682,7 @@ extern "C" int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* inf
 }
 
 extern "C" size_t tn_mlp_backward_workspace_bytes(uint32_t n_max) {
-    return (size_t)mlp_blocks(n_max) * kNetParams * sizeof(float);
+    return (size_t)mlp_blocks(n_max, kBwdBlocksPerCU) * kNetParams * sizeof(float);
 }
 
 // Backward of both MLPs for one batch. dL_drgb [3][grad_stride] (loss-scaled,
@@ -682,7 +701,7 @@ extern "C" int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* in
     TN_CHECK_ARG(n_max <= stride && n_max <= feat_stride && n_max <= grad_stride);
     if (workspace_bytes < tn_mlp_backward_workspace_bytes(n_max)) return TN_ERR_OUT_OF_MEMORY;
     if (n_max == 0) return TN_OK;
-    const unsigned blocks = mlp_blocks(n_max);
+    const unsigned blocks = mlp_blocks(n_max, kBwdBlocksPerCU);
     float* partial = static_cast<float*>(workspace);
     if (precision == 1) {
         using P = PrecF16;
