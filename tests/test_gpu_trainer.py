@@ -51,9 +51,10 @@ def test_training_loss_decreases(scene, precision):
     first, last = np.mean(losses[:10]), np.mean(losses[-10:])
     print(f"[trainer prec={precision}] loss {first:.4f} -> {last:.4f}; last n_rays={m.n_rays} "
           f"n_samples={m.n_samples}", flush=True)
-    # the grid starts fully occupied, so early batches hold few rays and the
-    # loss falls slowly; quality after training is checked by the render test
-    assert last < 0.9 * first
+    # rays that miss the box carry a zero target, so the reported loss tracks
+    # the fit; a CPU run of the oracle's step on this scene falls ~8x in 100
+    # steps at a quarter of this batch
+    assert last < 0.5 * first
     assert nerf.training_step == 120 and nerf.can_render
 
 

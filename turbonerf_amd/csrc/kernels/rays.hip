@@ -78,14 +78,19 @@ __global__ void init_training_rays_kernel(uint32_t n_rays_max, const uint32_t* _
     const float idx = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
     float tmin, tmax;
     const tn_bbox_t bb = *bbox;
-    if (!bbox_intersect(bb, ox, oy, oz, idx, idy, idz, tmin, tmax)) {
-        ray_alive[i] = 0;
-        return;
-    }
+    // a ray that misses the box has no samples; its target is cleared so the
+    // reported loss does not pick up a pixel left over from an earlier batch
+    // (the reference leaves the stale value in place; gradients are the same)
+    bool hit = bbox_intersect(bb, ox, oy, oz, idx, idy, idz, tmin, tmax);
     const float t_max = fminf(cam.far_plane - cam.near_plane, tmax);
     const float t = fmaxf(0.0f, tmin + 1e-5f);
-    if (t_max < t) {
+    hit = hit && !(t_max < t);
+    if (!hit) {
         ray_alive[i] = 0;
+        pix_rgba[i] = 0.0f;
+        pix_rgba[i + B] = 0.0f;
+        pix_rgba[i + 2 * B] = 0.0f;
+        pix_rgba[i + 3 * B] = 0.0f;
         return;
     }
     const uint64_t lp = min((uint64_t)p, (uint64_t)n_pix - 1);

@@ -37,7 +37,11 @@ def fill_uniform(n: int, seed: int, offset: int = 0, out: torch.Tensor | None = 
 
 
 def grid_struct(bitfield: torch.Tensor, density: torch.Tensor | None, n_levels: int, res: int = 128) -> OccupancyGrid:
-    return OccupancyGrid(ptr(bitfield), ptr(density) if density is not None else None, n_levels, res)
+    """Occupancy-grid view over device tensors; the struct keeps the tensors
+    alive (it holds raw pointers into them)."""
+    g = OccupancyGrid(ptr(bitfield), ptr(density) if density is not None else None, n_levels, res)
+    g._tensors = (bitfield, density)
+    return g
 
 
 @dataclass
