@@ -31,23 +31,75 @@ TN_DEVICE MarchGrid make_grid(const tn_occupancy_grid_t& g) {
     return m;
 }
 
-// Occupancy lookup with a one-entry cache: consecutive steps of a ray mostly
-// stay in one cell (a cell spans several steps), so the bitfield byte is
-// loaded only when the (level, cell) pair changes.
+// Occupancy lookups with look-ahead. A ray's serial march is bound by the
+// latency of its bitfield loads (one per cell it enters, whether it steps
+// through an occupied cell or skips an empty one). On a miss the lane loads
+// the byte of its current cell together with the next kLookahead-1 cells a
+// grid DDA walk along the ray would enter, as independent loads that overlap
+// in the memory system. The walk is only a prefetch: an entry is used when
+// its (level, Morton index) matches the exact index of the position being
+// tested, so the samples are the same as with one load per step.
+constexpr int kLookahead = 8;
+
 struct CellCache {
-    uint32_t cell = 0xFFFFFFFFu;
+    uint32_t key[kLookahead];
+    uint32_t occ = 0;  // bit j: occupancy of key[j] at `level`
     int level = -1;
-    bool occ = false;
 };
 
-TN_DEVICE bool occupied(const MarchGrid& g, int level, float x, float y, float z, CellCache& cc) {
-    const uint32_t m = voxel_morton_index(g.res_f, g.lim, level, x, y, z);
-    if (m != cc.cell || level != cc.level) {
-        cc.cell = m;
-        cc.level = level;
-        cc.occ = (g.bits[m] >> level) & 1u;
+TN_DEVICE void cache_fill(const MarchGrid& g, int level, float x, float y, float z, float dx, float dy, float dz,
+                          float idx, float idy, float idz, CellCache& cc) {
+    const float s = 1.0f / (float)(1 << level);
+    const float ux = (x * s + 0.5f) * g.res_f, uy = (y * s + 0.5f) * g.res_f, uz = (z * s + 0.5f) * g.res_f;
+    int cx = (int)min((uint32_t)fmaxf(ux, 0.0f), g.lim);
+    int cy = (int)min((uint32_t)fmaxf(uy, 0.0f), g.lim);
+    int cz = (int)min((uint32_t)fmaxf(uz, 0.0f), g.lim);
+    // ray parameter (world units) to the next boundary on each axis, and per cell
+    const float k = 1.0f / (s * g.res_f);  // world size of a cell at this level
+    float tx = ((float)(cx + (dx > 0.0f ? 1 : 0)) - ux) * k * idx;
+    float ty = ((float)(cy + (dy > 0.0f ? 1 : 0)) - uy) * k * idy;
+    float tz = ((float)(cz + (dz > 0.0f ? 1 : 0)) - uz) * k * idz;
+    const float ex = fabsf(k * idx), ey = fabsf(k * idy), ez = fabsf(k * idz);
+    const int sx = dx > 0.0f ? 1 : -1, sy = dy > 0.0f ? 1 : -1, sz = dz > 0.0f ? 1 : -1;
+    const int lim = (int)g.lim;
+    uint32_t bytes[kLookahead];
+    bool live = true;
+#pragma unroll
+    for (int j = 0; j < kLookahead; ++j) {
+        if (j > 0 && live) {
+            if (tx <= ty && tx <= tz) {
+                cx += sx;
+                tx += ex;
+            } else if (ty <= tz) {
+                cy += sy;
+                ty += ey;
+            } else {
+                cz += sz;
+                tz += ez;
+            }
+            live = cx >= 0 && cx <= lim && cy >= 0 && cy <= lim && cz >= 0 && cz <= lim;
+        }
+        cc.key[j] = live ? morton3d((uint32_t)cx, (uint32_t)cy, (uint32_t)cz) : 0xFFFFFFFFu;
+        bytes[j] = live ? g.bits[cc.key[j]] : 0u;
     }
-    return cc.occ;
+    uint32_t occ = 0;
+#pragma unroll
+    for (int j = 0; j < kLookahead; ++j) occ |= ((bytes[j] >> level) & 1u) << j;
+    cc.occ = occ;
+    cc.level = level;
+}
+
+TN_DEVICE bool occupied(const MarchGrid& g, int level, float x, float y, float z, float dx, float dy, float dz,
+                        float idx, float idy, float idz, CellCache& cc) {
+    const uint32_t m = voxel_morton_index(g.res_f, g.lim, level, x, y, z);
+    if (level == cc.level) {
+#pragma unroll
+        for (int j = 0; j < kLookahead; ++j)
+            if (cc.key[j] == m) return (cc.occ >> j) & 1u;
+    }
+    cache_fill(g, level, x, y, z, dx, dy, dz, idx, idy, idz, cc);
+    // entry 0 is the cell of (x, y, z) by the same index computation
+    return cc.occ & 1u;
 }
 
 __global__ void march_count_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays, uint32_t B,
@@ -76,7 +128,7 @@ __global__ void march_count_kernel(uint32_t n_rays_max, const uint32_t* __restri
         if (!bbox_contains(bbox, x, y, z)) break;
         const float dt = get_dt(t, cone, dt_min, dt_max);
         const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
-        if (occupied(g, lvl, x, y, z, cc)) {
+        if (occupied(g, lvl, x, y, z, dx, dy, dz, idx, idy, idz, cc)) {
             t += dt;
             ++n;
         } else {
@@ -235,7 +287,7 @@ __global__ void march_generate_kernel(uint32_t n_rays_max, const uint32_t* __res
         int lvl = 0;
         if (inside) {
             lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
-            emit = occupied(g, lvl, x, y, z, cc);
+            emit = occupied(g, lvl, x, y, z, dx, dy, dz, idx, idy, idz, cc);
         }
         if (emit) {
             out_t[off + n] = t;
