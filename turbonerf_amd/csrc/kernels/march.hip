@@ -31,88 +31,56 @@ TN_DEVICE MarchGrid make_grid(const tn_occupancy_grid_t& g) {
     return m;
 }
 
-// Occupancy lookups with look-ahead. A ray's serial march is bound by the
-// latency of its bitfield loads (one per cell it enters, whether it steps
-// through an occupied cell or skips an empty one). On a miss the lane loads
-// the byte of its current cell together with the next kLookahead-1 cells a
-// grid DDA walk along the ray would enter, as independent loads that overlap
-// in the memory system. The walk is only a prefetch: an entry is used when
-// its (level, Morton index) matches the exact index of the position being
-// tested, so the samples are the same as with one load per step.
-constexpr int kLookahead = 8;
-
-struct CellCache {
-    uint32_t key[kLookahead];
-    uint32_t occ = 0;  // bit j: occupancy of key[j] at `level`
-    int level = -1;
+// Marching runs one wave per ray. The march itself is the reference's serial
+// recurrence (every lane evaluates the same step, so control flow stays
+// uniform); what the 64 lanes buy is the latency of the occupancy lookups: a
+// lookup that misses the wave's cache refills it with 64 parallel loads, lane
+// j taking the cell at j half-cells further along the ray from the current
+// point, and the following steps find their cells there. Entries are keyed by
+// (cascade, Morton index) and hold that cell's own bit, so a prefetched entry
+// can never change a result; lane 0 always holds the exact cell of the point
+// that missed.
+struct WaveCache {
+    uint32_t key = 0xFFFFFFFFu;  // per lane
+    uint32_t occ = 0;            // per lane
+    int level = -1;              // wave-uniform
 };
 
-TN_DEVICE void cache_fill(const MarchGrid& g, int level, float x, float y, float z, float dx, float dy, float dz,
-                          float idx, float idy, float idz, CellCache& cc) {
-    const float s = 1.0f / (float)(1 << level);
-    const float ux = (x * s + 0.5f) * g.res_f, uy = (y * s + 0.5f) * g.res_f, uz = (z * s + 0.5f) * g.res_f;
-    int cx = (int)min((uint32_t)fmaxf(ux, 0.0f), g.lim);
-    int cy = (int)min((uint32_t)fmaxf(uy, 0.0f), g.lim);
-    int cz = (int)min((uint32_t)fmaxf(uz, 0.0f), g.lim);
-    // ray parameter (world units) to the next boundary on each axis, and per cell
-    const float k = 1.0f / (s * g.res_f);  // world size of a cell at this level
-    float tx = ((float)(cx + (dx > 0.0f ? 1 : 0)) - ux) * k * idx;
-    float ty = ((float)(cy + (dy > 0.0f ? 1 : 0)) - uy) * k * idy;
-    float tz = ((float)(cz + (dz > 0.0f ? 1 : 0)) - uz) * k * idz;
-    const float ex = fabsf(k * idx), ey = fabsf(k * idy), ez = fabsf(k * idz);
-    const int sx = dx > 0.0f ? 1 : -1, sy = dy > 0.0f ? 1 : -1, sz = dz > 0.0f ? 1 : -1;
-    const int lim = (int)g.lim;
-    uint32_t bytes[kLookahead];
-    bool live = true;
-#pragma unroll
-    for (int j = 0; j < kLookahead; ++j) {
-        if (j > 0 && live) {
-            if (tx <= ty && tx <= tz) {
-                cx += sx;
-                tx += ex;
-            } else if (ty <= tz) {
-                cy += sy;
-                ty += ey;
-            } else {
-                cz += sz;
-                tz += ez;
-            }
-            live = cx >= 0 && cx <= lim && cy >= 0 && cy <= lim && cz >= 0 && cz <= lim;
-        }
-        cc.key[j] = live ? morton3d((uint32_t)cx, (uint32_t)cy, (uint32_t)cz) : 0xFFFFFFFFu;
-        bytes[j] = live ? g.bits[cc.key[j]] : 0u;
-    }
-    uint32_t occ = 0;
-#pragma unroll
-    for (int j = 0; j < kLookahead; ++j) occ |= ((bytes[j] >> level) & 1u) << j;
-    cc.occ = occ;
-    cc.level = level;
+TN_DEVICE void wave_cache_fill(const MarchGrid& g, int level, float x, float y, float z, float dx, float dy,
+                               float dz, WaveCache& wc) {
+    const float lane = (float)__lane_id();
+    const float half_cell = 0.5f * (float)(1 << level) * g.inv_res;
+    const float tj = lane * half_cell;  // 0 for lane 0: its point is (x, y, z) exactly
+    const uint32_t m = voxel_morton_index(g.res_f, g.lim, level, x + tj * dx, y + tj * dy, z + tj * dz);
+    wc.key = m;
+    wc.occ = (g.bits[m] >> level) & 1u;
+    wc.level = level;
 }
 
-TN_DEVICE bool occupied(const MarchGrid& g, int level, float x, float y, float z, float dx, float dy, float dz,
-                        float idx, float idy, float idz, CellCache& cc) {
+TN_DEVICE bool wave_occupied(const MarchGrid& g, int level, float x, float y, float z, float dx, float dy, float dz,
+                             WaveCache& wc) {
     const uint32_t m = voxel_morton_index(g.res_f, g.lim, level, x, y, z);
-    if (level == cc.level) {
-#pragma unroll
-        for (int j = 0; j < kLookahead; ++j)
-            if (cc.key[j] == m) return (cc.occ >> j) & 1u;
+    uint64_t hit = level == wc.level ? __ballot(wc.key == m) : 0ull;
+    if (hit == 0ull) {
+        wave_cache_fill(g, level, x, y, z, dx, dy, dz, wc);
+        hit = __ballot(wc.key == m);  // lane 0 at least
     }
-    cache_fill(g, level, x, y, z, dx, dy, dz, idx, idy, idz, cc);
-    // entry 0 is the cell of (x, y, z) by the same index computation
-    return cc.occ & 1u;
+    return (__ballot(wc.occ != 0u) & hit) != 0ull;
 }
 
-__global__ void march_count_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays, uint32_t B,
-                                   tn_bbox_t bbox, tn_occupancy_grid_t grid, float cone, float dt_min,
-                                   float dt_max, uint32_t max_steps, const float* __restrict__ dir,
-                                   const float* __restrict__ ray_tmax, uint8_t* __restrict__ ray_alive,
-                                   const float* __restrict__ ori, const float* __restrict__ ray_t,
-                                   uint32_t* __restrict__ n_steps) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+constexpr int kMarchWaves = 4;  // rays (waves) per workgroup
+
+__global__ void __launch_bounds__(64 * kMarchWaves)
+march_count_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays, uint32_t B, tn_bbox_t bbox,
+                   tn_occupancy_grid_t grid, float cone, float dt_min, float dt_max, uint32_t max_steps,
+                   const float* __restrict__ dir, const float* __restrict__ ray_tmax, uint8_t* __restrict__ ray_alive,
+                   const float* __restrict__ ori, const float* __restrict__ ray_t, uint32_t* __restrict__ n_steps) {
+    const uint32_t i = blockIdx.x * kMarchWaves + (threadIdx.x >> 6);
     const uint32_t n_rays = d_n_rays ? min(*d_n_rays, n_rays_max) : n_rays_max;
     if (i >= n_rays) return;
+    const bool lane0 = __lane_id() == 0;
     if (!ray_alive[i]) {
-        n_steps[i] = 0;
+        if (lane0) n_steps[i] = 0;
         return;
     }
     const MarchGrid g = make_grid(grid);
@@ -122,21 +90,23 @@ __global__ void march_count_kernel(uint32_t n_rays_max, const uint32_t* __restri
     uint32_t n = 0;
     float t = ray_t[i];
     const float t_max = ray_tmax[i];
-    CellCache cc;
+    WaveCache wc;
     while (t < t_max && n < max_steps) {
         const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
         if (!bbox_contains(bbox, x, y, z)) break;
         const float dt = get_dt(t, cone, dt_min, dt_max);
         const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
-        if (occupied(g, lvl, x, y, z, dx, dy, dz, idx, idy, idz, cc)) {
+        if (wave_occupied(g, lvl, x, y, z, dx, dy, dz, wc)) {
             t += dt;
             ++n;
         } else {
             t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, dx, dy, dz, idx, idy, idz, dt_min, lvl);
         }
     }
-    if (n == 0) ray_alive[i] = 0;
-    n_steps[i] = n;
+    if (lane0) {
+        if (n == 0) ray_alive[i] = 0;
+        n_steps[i] = n;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -259,15 +229,17 @@ __global__ void scan_write_select_kernel(uint32_t n, const uint32_t* __restrict_
     }
 }
 
-__global__ void march_generate_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ info, uint32_t B,
-                                      tn_bbox_t bbox, tn_occupancy_grid_t grid, float dt_min, float dt_max,
-                                      float cone, const float* __restrict__ ori, const float* __restrict__ dir,
-                                      const float* __restrict__ ray_t, const uint32_t* __restrict__ ray_offset,
-                                      const uint8_t* __restrict__ ray_alive, const uint32_t* __restrict__ n_ray_steps,
-                                      uint32_t* __restrict__ out_ray, float* __restrict__ out_t) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64 * kMarchWaves)
+march_generate_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ info, uint32_t B, tn_bbox_t bbox,
+                      tn_occupancy_grid_t grid, float dt_min, float dt_max, float cone, const float* __restrict__ ori,
+                      const float* __restrict__ dir, const float* __restrict__ ray_t,
+                      const uint32_t* __restrict__ ray_offset, const uint8_t* __restrict__ ray_alive,
+                      const uint32_t* __restrict__ n_ray_steps, uint32_t* __restrict__ out_ray,
+                      float* __restrict__ out_t) {
+    const uint32_t i = blockIdx.x * kMarchWaves + (threadIdx.x >> 6);
     const uint32_t n_rays = min(info[0], n_rays_max);
     if (i >= n_rays || !ray_alive[i]) return;
+    const uint32_t lane = __lane_id();
     const MarchGrid g = make_grid(grid);
     const uint32_t n_steps = n_ray_steps[i];
     const uint32_t off = ray_offset[i];
@@ -275,8 +247,9 @@ __global__ void march_generate_kernel(uint32_t n_rays_max, const uint32_t* __res
     const float dx = dir[i], dy = dir[i + B], dz = dir[i + 2 * B];
     const float idx = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
     float t = ray_t[i];
-    uint32_t n = 0;
-    CellCache cc;
+    uint32_t n = 0, held = 0;  // samples emitted; of those, not yet written
+    float t_held = 0.0f;       // lane k holds the t of pending sample k
+    WaveCache wc;
     while (n < n_steps) {
         const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
         const float dt = get_dt(t, cone, dt_min, dt_max);
@@ -287,16 +260,25 @@ __global__ void march_generate_kernel(uint32_t n_rays_max, const uint32_t* __res
         int lvl = 0;
         if (inside) {
             lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
-            emit = occupied(g, lvl, x, y, z, dx, dy, dz, idx, idy, idz, cc);
+            emit = wave_occupied(g, lvl, x, y, z, dx, dy, dz, wc);
         }
         if (emit) {
-            out_t[off + n] = t;
-            out_ray[off + n] = i;
-            if (inside) t += dt;
+            if (lane == held) t_held = t;
+            ++held;
             ++n;
+            if (held == 64) {  // one coalesced row of 64 samples
+                out_t[off + n - 64 + lane] = t_held;
+                out_ray[off + n - 64 + lane] = i;
+                held = 0;
+            }
+            if (inside) t += dt;
         } else {
             t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, dx, dy, dz, idx, idy, idz, dt_min, lvl);
         }
+    }
+    if (lane < held) {
+        out_t[off + n - held + lane] = t_held;
+        out_ray[off + n - held + lane] = i;
     }
 }
 
@@ -382,7 +364,7 @@ extern "C" int tn_march_count(uint32_t n_rays_max, const uint32_t* d_n_rays, uin
     TN_CHECK_ARG(n_rays_max <= stride && grid.n_levels >= 1 && grid.n_levels <= 8);
     TN_CHECK_ARG(grid.resolution >= 2 && grid.resolution <= 1024 && dt_min > 0.0f && dt_max >= dt_min);
     if (n_rays_max == 0) return TN_OK;
-    march_count_kernel<<<blocks_for(n_rays_max, 64), 64, 0, stream>>>(
+    march_count_kernel<<<blocks_for(n_rays_max, kMarchWaves), 64 * kMarchWaves, 0, stream>>>(
         n_rays_max, d_n_rays, stride, bbox, grid, cone_angle, dt_min, dt_max, max_steps, dir, ray_tmax,
         ray_alive, ori, ray_t, n_steps);
     TN_CHECK_LAUNCH();
@@ -423,7 +405,7 @@ extern "C" int tn_march_generate(uint32_t n_rays_max, const uint32_t* info, uint
     TN_CHECK_ARG(n_rays_max <= stride && grid.n_levels >= 1 && grid.n_levels <= 8 && dt_min > 0.0f);
     TN_CHECK_ARG(info != nullptr && sample_stride > 0);
     if (n_rays_max == 0) return TN_OK;
-    march_generate_kernel<<<blocks_for(n_rays_max, 64), 64, 0, stream>>>(
+    march_generate_kernel<<<blocks_for(n_rays_max, kMarchWaves), 64 * kMarchWaves, 0, stream>>>(
         n_rays_max, info, stride, bbox, grid, dt_min, dt_max, cone_angle, ori, dir, ray_t, ray_offset, ray_alive,
         n_ray_steps, out_img_id, out_t);
     march_expand_kernel<<<blocks_for(sample_stride, 256), 256, 0, stream>>>(
