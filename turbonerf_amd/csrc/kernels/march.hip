@@ -31,41 +31,106 @@ TN_DEVICE MarchGrid make_grid(const tn_occupancy_grid_t& g) {
     return m;
 }
 
-// Marching runs one wave per ray. The march itself is the reference's serial
-// recurrence (every lane evaluates the same step, so control flow stays
-// uniform); what the 64 lanes buy is the latency of the occupancy lookups: a
-// lookup that misses the wave's cache refills it with 64 parallel loads, lane
-// j taking the cell at j half-cells further along the ray from the current
-// point, and the following steps find their cells there. Entries are keyed by
-// (cascade, Morton index) and hold that cell's own bit, so a prefetched entry
-// can never change a result; lane 0 always holds the exact cell of the point
-// that missed.
+// Marching runs one wave per ray and reproduces the reference's serial
+// recurrence exactly (contraction off, same operation order). The 64 lanes
+// buy two things:
+//  * occupancy lookups: a lookup that misses the wave's cache refills it with
+//    64 parallel loads, lane k taking the cell at k half-cells further along
+//    the ray from the current point (t0). Entries are keyed by (cascade,
+//    Morton index) and hold that cell's own bit, so a prefetched entry can
+//    never change a result; lane 0 holds the exact cell of the point that
+//    missed;
+//  * runs of occupied steps: once the step at t is known to be occupied, lane
+//    j forms t_j, the t after j more occupied steps (the same serial sums),
+//    and tests its own step; the leading lanes whose steps are inside, within
+//    the limits and occupied are taken at once. The first lane that fails is
+//    re-evaluated by the serial step, which handles the box exit, an empty
+//    cell (the DDA skip) or a cache miss exactly as the reference would.
 struct WaveCache {
     uint32_t key = 0xFFFFFFFFu;  // per lane
     uint32_t occ = 0;            // per lane
-    int level = -1;              // wave-uniform
+    int level = -1;              // wave-uniform from here on
+    float t0 = 0.0f;             // ray t of lane 0's point
+    float inv_h = 0.0f;          // 1 / (half a cell of `level`, in ray t)
 };
 
-TN_DEVICE void wave_cache_fill(const MarchGrid& g, int level, float x, float y, float z, float dx, float dy,
+TN_DEVICE void wave_cache_fill(const MarchGrid& g, int level, float t, float x, float y, float z, float dx, float dy,
                                float dz, WaveCache& wc) {
-    const float lane = (float)__lane_id();
     const float half_cell = 0.5f * (float)(1 << level) * g.inv_res;
-    const float tj = lane * half_cell;  // 0 for lane 0: its point is (x, y, z) exactly
+    const float tj = (float)__lane_id() * half_cell;  // 0 for lane 0: its point is (x, y, z) exactly
     const uint32_t m = voxel_morton_index(g.res_f, g.lim, level, x + tj * dx, y + tj * dy, z + tj * dz);
     wc.key = m;
     wc.occ = (g.bits[m] >> level) & 1u;
     wc.level = level;
+    wc.t0 = t;
+    wc.inv_h = 1.0f / half_cell;
 }
 
-TN_DEVICE bool wave_occupied(const MarchGrid& g, int level, float x, float y, float z, float dx, float dy, float dz,
-                             WaveCache& wc) {
+TN_DEVICE bool wave_occupied(const MarchGrid& g, int level, float t, float x, float y, float z, float dx, float dy,
+                             float dz, WaveCache& wc) {
     const uint32_t m = voxel_morton_index(g.res_f, g.lim, level, x, y, z);
     uint64_t hit = level == wc.level ? __ballot(wc.key == m) : 0ull;
     if (hit == 0ull) {
-        wave_cache_fill(g, level, x, y, z, dx, dy, dz, wc);
+        wave_cache_fill(g, level, t, x, y, z, dx, dy, dz, wc);
         hit = __ballot(wc.key == m);  // lane 0 at least
     }
     return (__ballot(wc.occ != 0u) & hit) != 0ull;
+}
+
+struct MarchRay {
+    float ox, oy, oz, dx, dy, dz;
+};
+
+// Leading lanes (at least 0, at most 64) whose step, j occupied steps after
+// t, is inside the box, below t_max, within `budget` and found occupied in
+// the cache; t_lane = t_j.
+TN_DEVICE uint32_t occupied_run(const MarchGrid& g, const tn_bbox_t& bbox, const MarchRay& r, float t, float t_max,
+                                uint32_t budget, float cone, float dt_min, float dt_max, const WaveCache& wc,
+                                float& t_lane) {
+    const uint32_t lane = __lane_id();
+    float tj = t;
+    if (dt_min == dt_max) {
+        for (uint32_t k = 0; k < 63; ++k) {
+            const float tn = tj + dt_min;
+            tj = lane > k ? tn : tj;
+        }
+    } else {
+        for (uint32_t k = 0; k < 63; ++k) {
+            const float tn = tj + get_dt(tj, cone, dt_min, dt_max);
+            tj = lane > k ? tn : tj;
+        }
+    }
+    t_lane = tj;
+    const float x = r.ox + tj * r.dx, y = r.oy + tj * r.dy, z = r.oz + tj * r.dz;
+    bool good = lane < budget && tj < t_max && bbox_contains(bbox, x, y, z);
+    const float dtj = get_dt(tj, cone, dt_min, dt_max);
+    const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dtj);
+    good = good && lvl == wc.level;
+    const uint32_t m = voxel_morton_index(g.res_f, g.lim, lvl, x, y, z);
+    // the cache lanes whose points bracket this one along the ray
+    const float kf = fminf(fmaxf((tj - wc.t0) * wc.inv_h, -2.0f), 66.0f);
+    const int k0 = (int)floorf(kf) - 1;
+    bool found = false, occ = false;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int src = min(max(k0 + c, 0), 63);
+        const uint32_t key = (uint32_t)__shfl((int)wc.key, src, 64);
+        const uint32_t o = (uint32_t)__shfl((int)wc.occ, src, 64);
+        if (key == m) {
+            found = true;
+            occ = o != 0u;
+        }
+    }
+    good = good && found && occ;
+    const uint64_t bad = __ballot(!good);
+    return bad == 0ull ? 64u : (uint32_t)__builtin_ctzll(bad);
+}
+
+// t after the run of f steps: lane f's t_f, or one more step after lane 63's
+TN_DEVICE float after_run(uint32_t f, float t_lane, float cone, float dt_min, float dt_max) {
+    if (f < 64u) return __shfl(t_lane, (int)f, 64);
+    const float t63 = __shfl(t_lane, 63, 64);
+    return t63 + get_dt(t63, cone, dt_min, dt_max);
 }
 
 constexpr int kMarchWaves = 4;  // rays (waves) per workgroup
@@ -84,23 +149,29 @@ march_count_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays, u
         return;
     }
     const MarchGrid g = make_grid(grid);
-    const float ox = ori[i], oy = ori[i + B], oz = ori[i + 2 * B];
-    const float dx = dir[i], dy = dir[i + B], dz = dir[i + 2 * B];
-    const float idx = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
+    const MarchRay r = {ori[i], ori[i + B], ori[i + 2 * B], dir[i], dir[i + B], dir[i + 2 * B]};
+    const float idx = 1.0f / r.dx, idy = 1.0f / r.dy, idz = 1.0f / r.dz;
     uint32_t n = 0;
     float t = ray_t[i];
     const float t_max = ray_tmax[i];
     WaveCache wc;
     while (t < t_max && n < max_steps) {
-        const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
+        const float x = r.ox + t * r.dx, y = r.oy + t * r.dy, z = r.oz + t * r.dz;
         if (!bbox_contains(bbox, x, y, z)) break;
         const float dt = get_dt(t, cone, dt_min, dt_max);
         const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
-        if (wave_occupied(g, lvl, x, y, z, dx, dy, dz, wc)) {
-            t += dt;
-            ++n;
+        if (wave_occupied(g, lvl, t, x, y, z, r.dx, r.dy, r.dz, wc)) {
+            float t_lane;
+            const uint32_t f = occupied_run(g, bbox, r, t, t_max, max_steps - n, cone, dt_min, dt_max, wc, t_lane);
+            if (f == 0u) {  // this step's cell lies outside the run's window
+                t += dt;
+                ++n;
+            } else {
+                t = after_run(f, t_lane, cone, dt_min, dt_max);
+                n += f;
+            }
         } else {
-            t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, dx, dy, dz, idx, idy, idz, dt_min, lvl);
+            t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, r.dx, r.dy, r.dz, idx, idy, idz, dt_min, lvl);
         }
     }
     if (lane0) {
@@ -243,42 +314,47 @@ march_generate_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ info, ui
     const MarchGrid g = make_grid(grid);
     const uint32_t n_steps = n_ray_steps[i];
     const uint32_t off = ray_offset[i];
-    const float ox = ori[i], oy = ori[i + B], oz = ori[i + 2 * B];
-    const float dx = dir[i], dy = dir[i + B], dz = dir[i + 2 * B];
-    const float idx = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
+    const MarchRay r = {ori[i], ori[i + B], ori[i + 2 * B], dir[i], dir[i + B], dir[i + 2 * B]};
+    const float idx = 1.0f / r.dx, idy = 1.0f / r.dy, idz = 1.0f / r.dz;
     float t = ray_t[i];
-    uint32_t n = 0, held = 0;  // samples emitted; of those, not yet written
-    float t_held = 0.0f;       // lane k holds the t of pending sample k
+    uint32_t n = 0;
     WaveCache wc;
     while (n < n_steps) {
-        const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
+        const float x = r.ox + t * r.dx, y = r.oy + t * r.dy, z = r.oz + t * r.dz;
         const float dt = get_dt(t, cone, dt_min, dt_max);
-        // the count pass stops at the box boundary, so every counted sample
-        // is inside; a sample outside would be emitted without advancing
-        const bool inside = bbox_contains(bbox, x, y, z);
-        bool emit = !inside;
-        int lvl = 0;
-        if (inside) {
-            lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
-            emit = wave_occupied(g, lvl, x, y, z, dx, dy, dz, wc);
-        }
-        if (emit) {
-            if (lane == held) t_held = t;
-            ++held;
-            ++n;
-            if (held == 64) {  // one coalesced row of 64 samples
-                out_t[off + n - 64 + lane] = t_held;
-                out_ray[off + n - 64 + lane] = i;
-                held = 0;
+        if (!bbox_contains(bbox, x, y, z)) {
+            // the count pass stops at the box boundary, so every counted
+            // sample is inside; one outside would be emitted without advancing
+            if (lane == 0) {
+                out_t[off + n] = t;
+                out_ray[off + n] = i;
             }
-            if (inside) t += dt;
-        } else {
-            t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, dx, dy, dz, idx, idy, idz, dt_min, lvl);
+            ++n;
+            continue;
         }
-    }
-    if (lane < held) {
-        out_t[off + n - held + lane] = t_held;
-        out_ray[off + n - held + lane] = i;
+        const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
+        if (wave_occupied(g, lvl, t, x, y, z, r.dx, r.dy, r.dz, wc)) {
+            float t_lane;
+            const uint32_t f =
+                occupied_run(g, bbox, r, t, INFINITY, n_steps - n, cone, dt_min, dt_max, wc, t_lane);
+            if (f == 0u) {
+                if (lane == 0) {
+                    out_t[off + n] = t;
+                    out_ray[off + n] = i;
+                }
+                t += dt;
+                ++n;
+            } else {
+                if (lane < f) {  // one coalesced row of samples
+                    out_t[off + n + lane] = t_lane;
+                    out_ray[off + n + lane] = i;
+                }
+                t = after_run(f, t_lane, cone, dt_min, dt_max);
+                n += f;
+            }
+        } else {
+            t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, r.dx, r.dy, r.dz, idx, idy, idz, dt_min, lvl);
+        }
     }
 }
 
