@@ -75,13 +75,29 @@ def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed training steps")
-    ap.add_argument("--warmup", type=int, default=256, help="untimed steps (occupancy update every 16)")
+    ap.add_argument("--warmup", type=int, default=2048, help="untimed steps (occupancy update every 16)")
     ap.add_argument("--precision", choices=["mixed", "fp32", "both"], default="both")
     ap.add_argument("--cpu-samples", type=int, default=1 << 18, help="samples per CPU-baseline step")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB 10 times (calibrates HBM counters)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """A plain copy kernel of known size, for scaling rocprofv3's FETCH_SIZE /
+    WRITE_SIZE readings to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_launch": 2 * n * 4, "read_bytes": n * 4, "write_bytes": n * 4}))
 
 
 def make_scene():
@@ -333,6 +349,9 @@ def main():
 
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a GPU")
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
     from turbonerf_amd import PyTurboNeRF as tn
     from turbonerf_amd import ops, synthetic
 
