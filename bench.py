@@ -53,8 +53,8 @@ ALGO = {
 }
 # the kernel each phase launches (rocprofv3 kernel names)
 KERNEL = {
-    "hashgrid_forward": "hashgrid_forward_kernel",
-    "hashgrid_backward": "hashgrid_backward_kernel",
+    "hashgrid_forward": "hashgrid_fwd_kernel",
+    "hashgrid_backward": "hashgrid_bwd_kernel",
     "mlp_forward": "mlp_forward_kernel",
     "mlp_backward": "mlp_backward_kernel",
     "composite": "composite_train_kernel",
