@@ -81,6 +81,8 @@ def parse():
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--profile-region", action="store_true",
+                    help="bracket the first timed region with roctxProfilerResume/Pause (rocprofv3 --selected-regions)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB 10 times (calibrates HBM counters)")
     return ap.parse_args()
@@ -98,6 +100,27 @@ def copy_probe(mib):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_per_launch": 2 * n * 4, "read_bytes": n * 4, "write_bytes": n * 4}))
+
+
+class ProfileRegion:
+    """roctxProfilerResume/Pause around a region, so that rocprofv3
+    --selected-regions records only the timed training steps."""
+
+    def __init__(self, enabled):
+        self.lib = None
+        if enabled:
+            import ctypes
+
+            self.lib = ctypes.CDLL("librocprofiler-sdk-roctx.so")
+
+    def resume(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+
+    def pause(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+            self.lib = None  # one region per run
 
 
 def make_scene():
@@ -130,7 +153,7 @@ def make_trainer(tn, synthetic, scene, precision):
     return mgr, nerf, tr
 
 
-def run_precision(tn, synthetic, scene, precision, args):
+def run_precision(tn, synthetic, scene, precision, args, region=None):
     import torch
 
     mgr, nerf, tr = make_trainer(tn, synthetic, scene, precision)
@@ -154,11 +177,15 @@ def run_precision(tn, synthetic, scene, precision, args):
     s0 = tr.samples_trained()
     tr.synchronize()
     torch.cuda.synchronize()
+    if region is not None:
+        region.resume()
     t0 = time.perf_counter()
     tr.train_steps(args.steps, read_metrics=False)
     tr.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    if region is not None:
+        region.pause()
     s1 = tr.samples_trained()
     step += args.steps
     wall = t1 - t0
@@ -359,8 +386,9 @@ def main():
     scene = make_scene()
     precs = {"mixed": [1], "fp32": [0], "both": [1, 0]}[args.precision]
     results = {}
+    region = ProfileRegion(args.profile_region)
     for p in precs:
-        results[p] = run_precision(tn, synthetic, scene, p, args)
+        results[p] = run_precision(tn, synthetic, scene, p, args, region)
     main_p = precs[0]
     main = results[main_p]
     dtype = "fp16" if main_p == 1 else "fp32"
