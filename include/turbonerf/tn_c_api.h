@@ -228,7 +228,9 @@ size_t tn_mlp_lds_bytes(int precision, int backward);
 
 /* NGP Adam over all parameters (reference ngp_adam_step,
  * src/core/adam-optimizer.cuh:48-96). Gradients are fp32 and are zeroed as
- * they are consumed. lp_dtype 0: no low-precision copy; 1: fp16 copy. */
+ * they are consumed. lp_dtype 0: no low-precision copy; 1: fp16 copy.
+ * weights, gradients, moments and steps must be 16-byte aligned, weights_lp
+ * 8-byte aligned. */
 int tn_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float loss_scale,
     float learning_rate, float beta1, float beta2, float epsilon, float l2_reg, float* weights,
     int lp_dtype, void* weights_lp, float* gradients, float* first_moments, float* second_moments,

@@ -6,7 +6,6 @@
 // Gradients are fp32 and are cleared as they are consumed, so the next
 // step's scatter-adds start from zero without a separate memset over the
 // whole table; entries whose gradient is zero are read once and skipped.
-// One thread handles 4 consecutive parameters (16-byte gradient loads).
 #include "common.hpp"
 
 namespace tn {
@@ -32,6 +31,29 @@ TN_DEVICE void adam_one(uint32_t i, float g_raw, uint32_t n_grid, uint32_t n_net
 }
 
 template <typename TL>
+struct LowPrecision4;
+template <>
+struct LowPrecision4<float> {
+    TN_DEVICE static void store(float*, uint32_t, const float (&)[4]) {}
+};
+typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+template <>
+struct LowPrecision4<_Float16> {
+    TN_DEVICE static void store(_Float16* p, uint32_t i4, const float (&w)[4]) {
+        half4v h;
+        h.x = (_Float16)w[0];
+        h.y = (_Float16)w[1];
+        h.z = (_Float16)w[2];
+        h.w = (_Float16)w[3];
+        *reinterpret_cast<half4v*>(p + i4) = h;
+    }
+};
+
+// One thread per 4 consecutive parameters with 16-byte loads and stores of
+// the gradient, weights, moments and step counters (8 bytes for the fp16
+// copy); a group whose gradients are all zero inside the hash table costs one
+// 16-byte read.
+template <typename TL>
 __global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float inv_loss_scale, float lr, float beta1,
                             float beta2, float eps, float l2, float* __restrict__ w, TL* __restrict__ w_lp,
                             float* __restrict__ grad, float* __restrict__ m, float* __restrict__ v,
@@ -39,14 +61,36 @@ __global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, flo
     const uint32_t i4 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (i4 >= n) return;
     if (i4 + 3 < n) {
-        const float4 g = *reinterpret_cast<const float4*>(grad + i4);
-        const bool any = g.x != 0.0f || g.y != 0.0f || g.z != 0.0f || g.w != 0.0f;
+        const float4 g4 = *reinterpret_cast<const float4*>(grad + i4);
+        const bool any = g4.x != 0.0f || g4.y != 0.0f || g4.z != 0.0f || g4.w != 0.0f;
         if (!any && i4 + 3 < n_grid) return;  // untouched hash entries: one 16-byte read
-        const float gv[4] = {g.x, g.y, g.z, g.w};
+        const float4 w4 = *reinterpret_cast<const float4*>(w + i4);
+        const float4 m4 = *reinterpret_cast<const float4*>(m + i4);
+        const float4 v4 = *reinterpret_cast<const float4*>(v + i4);
+        const uint4 s4 = *reinterpret_cast<const uint4*>(steps + i4);
+        const float gv[4] = {g4.x, g4.y, g4.z, g4.w};
+        float wv[4] = {w4.x, w4.y, w4.z, w4.w}, mv[4] = {m4.x, m4.y, m4.z, m4.w}, vv[4] = {v4.x, v4.y, v4.z, v4.w};
+        uint32_t sv[4] = {s4.x, s4.y, s4.z, s4.w};
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            adam_one<TL>(i4 + k, gv[k], n_grid, n_net_end, inv_loss_scale, lr, beta1, beta2, eps, l2, w, w_lp, m, v,
-                         steps);
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t i = i4 + k;
+            float g = gv[k] * inv_loss_scale;
+            if (i < n_grid && g == 0.0f) continue;  // skipped entry keeps its state
+            if (i >= n_grid && i < n_net_end) g += l2 * wv[k];
+            const float m1 = beta1 * mv[k] + (1.0f - beta1) * g;
+            const float m2 = beta2 * vv[k] + (1.0f - beta2) * (g * g);
+            const uint32_t st = sv[k] + 1;
+            const float c = sqrtf(1.0f - powf(beta2, (float)st)) / (1.0f - powf(beta1, (float)st));
+            wv[k] = wv[k] - (lr * c) / (sqrtf(m2) + eps) * m1;
+            mv[k] = m1;
+            vv[k] = m2;
+            sv[k] = st;
+        }
+        *reinterpret_cast<float4*>(w + i4) = make_float4(wv[0], wv[1], wv[2], wv[3]);
+        *reinterpret_cast<float4*>(m + i4) = make_float4(mv[0], mv[1], mv[2], mv[3]);
+        *reinterpret_cast<float4*>(v + i4) = make_float4(vv[0], vv[1], vv[2], vv[3]);
+        *reinterpret_cast<uint4*>(steps + i4) = make_uint4(sv[0], sv[1], sv[2], sv[3]);
+        if (w_lp) LowPrecision4<TL>::store(w_lp, i4, wv);
         if (any) *reinterpret_cast<float4*>(grad + i4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     } else {
         for (uint32_t i = i4; i < n; ++i) {
@@ -76,6 +120,10 @@ extern "C" int tn_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, flo
                             uint32_t* param_steps, hipStream_t stream) {
     TN_CHECK_ARG(weights && gradients && first_moments && second_moments && param_steps && loss_scale > 0.0f);
     TN_CHECK_ARG(n_grid <= n_net_end && n_net_end <= n && (lp_dtype == 0 || weights_lp));
+    // 16-byte vector access (8 bytes for the fp16 copy)
+    auto aligned = [](const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0; };
+    TN_CHECK_ARG(aligned(weights, 16) && aligned(gradients, 16) && aligned(first_moments, 16) &&
+                 aligned(second_moments, 16) && aligned(param_steps, 16) && (lp_dtype == 0 || aligned(weights_lp, 8)));
     if (n == 0) return TN_OK;
     const unsigned blocks = blocks_for((n + 3) / 4, 256);
     const float inv = 1.0f / loss_scale;

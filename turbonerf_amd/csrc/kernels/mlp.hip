@@ -412,6 +412,38 @@ TN_DEVICE f16v relu_mask(f16v d, const f16v& act) {
     return d;
 }
 
+// An activation tile kept for the backward pass in the network precision
+// (half the registers of the fp32 accumulator tile in mixed precision).
+template <class P>
+struct ActTile {
+    typename P::T v[16];
+};
+
+template <class P>
+TN_DEVICE ActTile<P> keep(const f16v& D) {
+    ActTile<P> a;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) a.v[r] = (typename P::T)D[r];
+    return a;
+}
+
+template <class P>
+TN_DEVICE f16v relu_mask(f16v d, const ActTile<P>& act) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) d[r] = act.v[r] > (typename P::T)0.0f ? d[r] : 0.0f;
+    return d;
+}
+
+template <class P>
+TN_DEVICE void put_tile(typename P::T* buf, int t, const ActTile<P>& A, int lane) {
+    const int n = lane & 31, h = lane >> 5;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3);
+        buf[row * kTW + n] = A.v[r];
+    }
+}
+
 // add tile C (rows 32*mt.., cols 32*nt..) of a [n_out][n_in] gradient into LDS
 TN_DEVICE void add_dw_tile(float* acc, const f16v& C, int mt, int nt, int n_out, int n_in, int lane) {
     const int col = 32 * nt + (lane & 31), h = lane >> 5;
@@ -466,6 +498,11 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         Tile<P> tl;
         load_inputs<P>(tl, valid, sample, h, feat, FS, dir, B, true);
         forward_tile<P>(tl, W, lane, true);
+        // activations the backward pass needs, in the network precision
+        const ActTile<P> a_h1[2] = {keep<P>(tl.h1[0]), keep<P>(tl.h1[1])};
+        const ActTile<P> a_s = keep<P>(tl.s);
+        const ActTile<P> a_h2[2] = {keep<P>(tl.h2[0]), keep<P>(tl.h2[1])};
+        const ActTile<P> a_h3[2] = {keep<P>(tl.h3[0]), keep<P>(tl.h3[1])};
 
         // dL/do through the sigmoid (rows 0..2 live in lanes 0..31, registers 0..2)
         f16v dO = zero16();
@@ -480,21 +517,21 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         f16v dH3[2];
         f16v dOt[1] = {dO};
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) dH3[mt] = relu_mask(gemm_dt<P, 1>(WB.b5, N::B5, mt, dOt, lane, N::B5.n_ks), tl.h3[mt]);
+        for (int mt = 0; mt < 2; ++mt) dH3[mt] = relu_mask<P>(gemm_dt<P, 1>(WB.b5, N::B5, mt, dOt, lane, N::B5.n_ks), a_h3[mt]);
         put_tile<P>(dbuf, 0, dO, lane);
-        put_tile<P>(abuf, 0, tl.h3[0], lane);
-        put_tile<P>(abuf, 1, tl.h3[1], lane);
+        put_tile<P>(abuf, 0, a_h3[0], lane);
+        put_tile<P>(abuf, 1, a_h3[1], lane);
         wave_sync();
         accumulate_dw<P, 1, 2>(dW5, dbuf, abuf, lane);
         wave_sync();
         // dH2 = relu'(h2) (W4^T dH3); dW4 += dH3 h2^T
         f16v dH2[2];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) dH2[mt] = relu_mask(gemm_dt<P, 2>(WB.b4, N::B4, mt, dH3, lane, N::B4.n_ks), tl.h2[mt]);
+        for (int mt = 0; mt < 2; ++mt) dH2[mt] = relu_mask<P>(gemm_dt<P, 2>(WB.b4, N::B4, mt, dH3, lane, N::B4.n_ks), a_h2[mt]);
         put_tile<P>(dbuf, 0, dH3[0], lane);
         put_tile<P>(dbuf, 1, dH3[1], lane);
-        put_tile<P>(abuf, 0, tl.h2[0], lane);
-        put_tile<P>(abuf, 1, tl.h2[1], lane);
+        put_tile<P>(abuf, 0, a_h2[0], lane);
+        put_tile<P>(abuf, 1, a_h2[1], lane);
         wave_sync();
         accumulate_dw<P, 2, 2>(dW4, dbuf, abuf, lane);
         wave_sync();
@@ -502,7 +539,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         f16v dXc = gemm_dt<P, 2>(WB.b3, N::B3, 0, dH2, lane, N::B3.n_ks);
         put_tile<P>(dbuf, 0, dH2[0], lane);
         put_tile<P>(dbuf, 1, dH2[1], lane);
-        put_tile<P>(abuf, 0, tl.s, lane);  // rows 0..15 = density outputs (rows 16..31 overwritten next)
+        put_tile<P>(abuf, 0, a_s, lane);  // rows 0..15 = density outputs (rows 16..31 overwritten next)
         put_nat<P, 16 / P::KC>(abuf, 16, tl.sh, lane);
         wave_sync();
         accumulate_dw<P, 2, 1>(dW3, dbuf, abuf, lane);
@@ -524,10 +561,10 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         f16v dSt[1] = {dS};
         f16v dH1[2];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) dH1[mt] = relu_mask(gemm_dt<P, 1>(WB.b2, N::B2, mt, dSt, lane, N::B2.n_ks), tl.h1[mt]);
+        for (int mt = 0; mt < 2; ++mt) dH1[mt] = relu_mask<P>(gemm_dt<P, 1>(WB.b2, N::B2, mt, dSt, lane, N::B2.n_ks), a_h1[mt]);
         put_tile<P>(dbuf, 0, dS, lane);
-        put_tile<P>(abuf, 0, tl.h1[0], lane);
-        put_tile<P>(abuf, 1, tl.h1[1], lane);
+        put_tile<P>(abuf, 0, a_h1[0], lane);
+        put_tile<P>(abuf, 1, a_h1[1], lane);
         wave_sync();
         accumulate_dw<P, 1, 2>(dW2, dbuf, abuf, lane);
         wave_sync();
