@@ -10,13 +10,13 @@
 // modulo the level size. The table is entry-major: entry e's F features are
 // contiguous. Features are written feature-major, out[(l*F+f)*stride + i].
 //
-// Launch shape: a 1-D grid whose block b serves level b % L and sample chunk
-// b / L. Workgroups are dispatched to the 8 XCDs round-robin, so with L = 16
-// each XCD works on two levels only: its L2 holds those levels' tables (fp16,
-// 2 MiB per hashed level) and gradients instead of all of them, and blocks in
-// flight at any time cover every level, which spreads the scatter-add over
-// the whole table. The forward pass runs one thread per (sample, level), the
-// backward pass two (one per feature). Consecutive
+// Launch shape. Forward: one thread per (sample, level), blockIdx.y = level,
+// so the blocks in flight gather from one level's table at a time. Backward:
+// two threads per (sample, level), one per feature, in a 1-D grid whose
+// block b serves level b % L and sample chunk b / L; workgroups go to the 8
+// XCDs round-robin, so with L = 16 each XCD scatters into two levels'
+// gradients only, and the blocks in flight cover every level, which spreads
+// the atomics over the whole table. Consecutive
 // samples of a ray share coarse-level cells; the backward pass merges runs of
 // equal corner indices across the wave with a segmented sum before issuing
 // its atomics.
@@ -78,9 +78,8 @@ __global__ void __launch_bounds__(256) hashgrid_fwd_kernel(uint32_t n_max, const
                                                            uint32_t B, uint32_t OB, tn_hashgrid_t g,
                                                            const float* __restrict__ pos, const TT* __restrict__ table,
                                                            TO* __restrict__ out) {
-    const uint32_t L = (uint32_t)g.n_levels;
-    const int l = (int)(blockIdx.x % L);
-    const uint32_t i = (blockIdx.x / L) * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
     const uint32_t n = info ? min(info[1], n_max) : n_max;
     if (i >= n) return;
     HashCorners hc;
@@ -202,7 +201,7 @@ extern "C" int tn_hashgrid_forward(int table_dtype, int feat_dtype, uint32_t n_m
     TN_CHECK_ARG(pos && table && features && n_max <= stride && n_max <= out_stride && grid.n_features == 2);
     TN_CHECK_ARG(grid.n_levels >= 1 && grid.n_levels <= TN_HASHGRID_MAX_LEVELS);
     if (n_max == 0) return TN_OK;
-    const unsigned blocks = blocks_for(n_max, 256) * (unsigned)grid.n_levels;
+    const dim3 blocks(blocks_for(n_max, 256), grid.n_levels);
     if (table_dtype == 0 && feat_dtype == 0)
         hashgrid_fwd_kernel<float, float><<<blocks, 256, 0, stream>>>(n_max, info, stride, out_stride, grid, pos,
                                                                       static_cast<const float*>(table),

@@ -346,7 +346,9 @@ TN_DEVICE BwdLds<P> stage_backward(typename P::T* lds, const typename P::T* __re
     return b;
 }
 
-constexpr int kTW = 33;  // LDS transpose row stride (32 samples + 1 to spread banks)
+// LDS transpose buffers: row stride of 40 elements, so that a row's 8
+// consecutive samples (one fp16 MFMA operand) are one aligned 16-byte read
+constexpr int kTW = 40;
 
 // write an accumulator tile as rows [32*t, 32*t+32) x 32 samples
 template <class P>
@@ -377,10 +379,7 @@ TN_DEVICE void put_nat(typename P::T* buf, int base, const typename P::Op (&x)[N
 template <class P>
 TN_DEVICE typename P::Op get_rows(const typename P::T* buf, int t, int ks, int lane) {
     const int row = 32 * t + (lane & 31), h = lane >> 5;
-    float v[P::SL];
-#pragma unroll
-    for (int j = 0; j < P::SL; ++j) v[j] = (float)buf[row * kTW + ks * P::KC + h * P::SL + j];
-    return P::from(v);
+    return *reinterpret_cast<const typename P::Op*>(buf + row * kTW + ks * P::KC + h * P::SL);
 }
 
 TN_DEVICE void wave_sync() {
