@@ -346,9 +346,12 @@ TN_DEVICE BwdLds<P> stage_backward(typename P::T* lds, const typename P::T* __re
     return b;
 }
 
-// LDS transpose buffers: row stride of 40 elements, so that a row's 8
-// consecutive samples (one fp16 MFMA operand) are one aligned 16-byte read
-constexpr int kTW = 40;
+// LDS transpose buffers: row stride 40 elements in mixed precision, so that
+// a row's 8 consecutive samples (one fp16 MFMA operand) are one aligned
+// 16-byte read; 33 in fp32 (scalar operands; the +1 spreads banks and the
+// fp32 weights leave no room for more)
+template <class P>
+constexpr int kTW = P::SL == 8 ? 40 : 33;
 
 // write an accumulator tile as rows [32*t, 32*t+32) x 32 samples
 template <class P>
@@ -357,7 +360,7 @@ TN_DEVICE void put_tile(typename P::T* buf, int t, const f16v& D, int lane) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int row = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3);
-        buf[row * kTW + n] = (typename P::T)D[r];
+        buf[row * kTW<P> + n] = (typename P::T)D[r];
     }
 }
 
@@ -370,8 +373,8 @@ TN_DEVICE void put_nat(typename P::T* buf, int base, const typename P::Op (&x)[N
 #pragma unroll
         for (int j = 0; j < P::SL; ++j) {
             const int row = base + nat_in<P>(ks, h, j);
-            if constexpr (P::SL == 1) buf[row * kTW + n] = x[ks];
-            else buf[row * kTW + n] = x[ks][j];
+            if constexpr (P::SL == 1) buf[row * kTW<P> + n] = x[ks];
+            else buf[row * kTW<P> + n] = x[ks][j];
         }
 }
 
@@ -379,7 +382,7 @@ TN_DEVICE void put_nat(typename P::T* buf, int base, const typename P::Op (&x)[N
 template <class P>
 TN_DEVICE typename P::Op get_rows(const typename P::T* buf, int t, int ks, int lane) {
     const int row = 32 * t + (lane & 31), h = lane >> 5;
-    return *reinterpret_cast<const typename P::Op*>(buf + row * kTW + ks * P::KC + h * P::SL);
+    return *reinterpret_cast<const typename P::Op*>(buf + row * kTW<P> + ks * P::KC + h * P::SL);
 }
 
 TN_DEVICE void wave_sync() {
@@ -439,7 +442,7 @@ TN_DEVICE void put_tile(typename P::T* buf, int t, const ActTile<P>& A, int lane
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int row = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3);
-        buf[row * kTW + n] = A.v[r];
+        buf[row * kTW<P> + n] = A.v[r];
     }
 }
 
@@ -468,9 +471,9 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
     const FwdLds<P> W = stage_forward<P>(lds, params);
     const BwdLds<P> WB = stage_backward<P>(lds + N::kFwdElems, params);
     const int wid = threadIdx.x >> 6;
-    T* tbuf = lds + N::kFwdElems + N::kBwdElems + wid * (2 * 64 * kTW);
+    T* tbuf = lds + N::kFwdElems + N::kBwdElems + wid * (2 * 64 * kTW<P>);
     T* dbuf = tbuf;               // deltas, up to 64 rows
-    T* abuf = tbuf + 64 * kTW;    // activations, up to 64 rows
+    T* abuf = tbuf + 64 * kTW<P>;    // activations, up to 64 rows
     __syncthreads();
 
     const uint32_t n = info ? min(info[1], n_max) : n_max;
@@ -641,18 +644,24 @@ constexpr size_t fwd_lds_bytes() {
 template <class P>
 constexpr size_t bwd_lds_bytes() {
     const size_t w = (size_t)(Net<P>::kFwdElems + Net<P>::kBwdElems) * sizeof(typename P::T);
-    const size_t t = (size_t)kMlpWaves * 2 * 64 * kTW * sizeof(typename P::T);
+    const size_t t = (size_t)kMlpWaves * 2 * 64 * kTW<P> * sizeof(typename P::T);
     const size_t r = (size_t)kNetParams * sizeof(float);
     return (w + t) > r ? (w + t) : r;
 }
 
 
+static_assert(bwd_lds_bytes<PrecF16>() <= 160 * 1024 && bwd_lds_bytes<PrecF32>() <= 160 * 1024,
+              "the backward kernel's LDS must fit one CU (160 KiB)");
+
 // kernels using more than the default 64 KiB of dynamic LDS opt in once
 template <typename K>
 inline bool allow_lds(K kernel, size_t bytes) {
     if (bytes <= 65536) return true;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)bytes) == hipSuccess;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes) == hipSuccess)
+        return true;
+    (void)hipGetLastError();  // do not leave the failure for the next launch check
+    return false;
 }
 
 // Persistent launches: as many workgroups as fit on the device at once (the
