@@ -81,8 +81,13 @@ def parse():
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
-    ap.add_argument("--profile-region", action="store_true",
-                    help="bracket the first timed region with roctxProfilerResume/Pause (rocprofv3 --selected-regions)")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="profiling run: render the targets, then only --warmup and this many mixed-precision "
+                         "training steps (no occupancy updates, second pass, render or CPU leg), and exit")
+    ap.add_argument("--summarize", nargs="+", metavar="CSV",
+                    help="summarise rocprofv3 CSVs of a --profile-steps run: kernel_trace.csv [FETCH_SIZE "
+                         "counter_collection.csv WRITE_SIZE counter_collection.csv copy-probe FETCH_SIZE csv "
+                         "copy-probe WRITE_SIZE csv]; prints markdown (no GPU needed)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB 10 times (calibrates HBM counters)")
     return ap.parse_args()
@@ -100,27 +105,6 @@ def copy_probe(mib):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_per_launch": 2 * n * 4, "read_bytes": n * 4, "write_bytes": n * 4}))
-
-
-class ProfileRegion:
-    """roctxProfilerResume/Pause around a region, so that rocprofv3
-    --selected-regions records only the timed training steps."""
-
-    def __init__(self, enabled):
-        self.lib = None
-        if enabled:
-            import ctypes
-
-            self.lib = ctypes.CDLL("librocprofiler-sdk-roctx.so")
-
-    def resume(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerResume(0)
-
-    def pause(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerPause(0)
-            self.lib = None  # one region per run
 
 
 def make_scene():
@@ -153,7 +137,7 @@ def make_trainer(tn, synthetic, scene, precision):
     return mgr, nerf, tr
 
 
-def run_precision(tn, synthetic, scene, precision, args, region=None):
+def run_precision(tn, synthetic, scene, precision, args):
     import torch
 
     mgr, nerf, tr = make_trainer(tn, synthetic, scene, precision)
@@ -177,15 +161,11 @@ def run_precision(tn, synthetic, scene, precision, args, region=None):
     s0 = tr.samples_trained()
     tr.synchronize()
     torch.cuda.synchronize()
-    if region is not None:
-        region.resume()
     t0 = time.perf_counter()
     tr.train_steps(args.steps, read_metrics=False)
     tr.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    if region is not None:
-        region.pause()
     s1 = tr.samples_trained()
     step += args.steps
     wall = t1 - t0
@@ -365,8 +345,81 @@ def cpu_baseline(scene, bitfield, rays_per_step_gpu, args):
                        f"grid, fp32, median of {args.cpu_runs} after 1 warm-up")
 
 
+def profile_steps(tn, synthetic, scene, args):
+    """Training steps only (after the target renders), for rocprofv3: every
+    dispatch after the last write_rgba_kernel belongs to a training step."""
+    import torch
+
+    mgr, nerf, tr = make_trainer(tn, synthetic, scene, 1)
+    tr.train_steps(args.warmup + args.profile_steps, read_metrics=False)
+    tr.synchronize()
+    torch.cuda.synchronize()
+    log(f"[bench profile] {args.warmup + args.profile_steps} training steps, {tr.samples_trained()} samples")
+    tr.teardown()
+
+
+def _short(name):
+    base = name.split("(")[0]
+    for key in sorted(set(KERNEL.values()) | {"march_expand_kernel", "reduce_partials_kernel", "scan_tile_sums_kernel",
+                                               "scan_tile_prefix_kernel", "scan_write_select_kernel",
+                                               "uniform_kernel", "init_training_rays_kernel", "estimate_rays_kernel",
+                                               "accumulate_samples_kernel", "reduce_loss_kernel"}):
+        if key in base:
+            return key
+    return base[:48]
+
+
+def summarize(paths):
+    """Markdown summary of a --profile-steps run from rocprofv3 CSVs."""
+    import csv
+    import statistics as st
+
+    def training_rows(path, ts_key):
+        rows = list(csv.DictReader(open(path)))
+        marks = [int(r[ts_key]) for r in rows if "write_rgba_kernel" in r["Kernel_Name"]]
+        start = max(marks) if marks else 0
+        return [r for r in rows if int(r[ts_key]) > start]
+
+    trace = training_rows(paths[0], "Start_Timestamp")
+    per = {}
+    for r in trace:
+        per.setdefault(_short(r["Kernel_Name"]), []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    steps = len(per.get("adam_kernel", [])) or 1
+    total = sum(sum(v) for v in per.values())
+    out = ["| kernel | calls | avg us | total us per step | share |", "|---|---:|---:|---:|---:|"]
+    for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        out.append(f"| {k} | {len(v)} | {st.mean(v) / 1e3:.2f} | {sum(v) / 1e3 / steps:.2f} | "
+                   f"{100 * sum(v) / total:.1f}% |")
+    print("\n".join(out))
+    if len(paths) >= 5:
+        def counter(path):
+            vals = {}
+            for r in training_rows(path, "Start_Timestamp"):
+                vals.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+            return {k: st.mean(v) for k, v in vals.items()}
+
+        def copy_counter(path):
+            rows = [r for r in csv.DictReader(open(path)) if "elementwise" in r["Kernel_Name"] or
+                    "copy" in r["Kernel_Name"].lower()]
+            return max(float(r["Counter_Value"]) for r in rows)
+
+        fetch, write = counter(paths[1]), counter(paths[2])
+        cf, cw = copy_counter(paths[3]), copy_counter(paths[4])
+        probe = 256 * (1 << 20)  # --copy-probe-mib 256: bytes read and written per launch
+        kf, kw = probe / cf, probe / cw
+        print(f"\nCounter scale from the 256 MiB copy probe: FETCH_SIZE x {kf:.1f} B, WRITE_SIZE x {kw:.1f} B "
+              f"per unit\n")
+        print("| kernel | HBM read MB/launch | HBM write MB/launch |")
+        print("|---|---:|---:|")
+        for k in sorted(fetch, key=lambda k: -fetch[k])[:12]:
+            print(f"| {k} | {fetch[k] * kf / 1e6:.2f} | {write.get(k, 0.0) * kw / 1e6:.2f} |")
+
+
 def main():
     args = parse()
+    if args.summarize:
+        summarize(args.summarize)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build has no multi-GPU "
               f"path; the per-step gradient all-reduce it would need is described in the design notes)",
@@ -384,11 +437,13 @@ def main():
 
     torch.cuda.init()
     scene = make_scene()
+    if args.profile_steps:
+        profile_steps(tn, synthetic, scene, args)
+        return
     precs = {"mixed": [1], "fp32": [0], "both": [1, 0]}[args.precision]
     results = {}
-    region = ProfileRegion(args.profile_region)
     for p in precs:
-        results[p] = run_precision(tn, synthetic, scene, p, args, region)
+        results[p] = run_precision(tn, synthetic, scene, p, args)
     main_p = precs[0]
     main = results[main_p]
     dtype = "fp16" if main_p == 1 else "fp32"
