@@ -88,6 +88,10 @@ def parse():
                     help="summarise rocprofv3 CSVs of a --profile-steps run: kernel_trace.csv [FETCH_SIZE "
                          "counter_collection.csv WRITE_SIZE counter_collection.csv copy-probe FETCH_SIZE csv "
                          "copy-probe WRITE_SIZE csv]; prints markdown (no GPU needed)")
+    ap.add_argument("--summarize-last", type=int, default=0,
+                    help="with --summarize: only the last N training steps (after the warm-up)")
+    ap.add_argument("--samples-per-step", type=float, default=0.0,
+                    help="with --summarize: samples per step, for the per-sample traffic column")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB 10 times (calibrates HBM counters)")
     return ap.parse_args()
@@ -233,19 +237,24 @@ def roofline(res, n_params, dtype):
                 algorithmic_per_launch=d["algorithmic"]), table
 
 
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "train_step_blob8_800x800_b262144.md")
+
+
 def measured_traffic(kernel, samples_per_step):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (profiles/traffic.json), scaled to this run's samples per launch;
-    None when no measurement exists."""
-    path = os.path.join(HERE, "profiles", "traffic.json")
-    if not kernel or not os.path.exists(path):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (its "HBM B/sample" column), scaled to
+    this run's samples per launch; None when no measurement exists."""
+    if not kernel or not os.path.exists(TRAFFIC_PROFILE):
         return None
-    with open(path) as f:
-        t = json.load(f)
-    e = t.get(kernel)
-    if not e:
-        return None
-    return int(e["bytes_per_sample"] * samples_per_step)
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 4 and cells[0] == kernel:
+                try:
+                    return int(float(cells[3]) * samples_per_step)
+                except ValueError:
+                    return None
+    return None
 
 
 def render_rate(tn, synthetic, scene, trainer_tuple, n_frames=5):
@@ -369,8 +378,10 @@ def _short(name):
     return base[:48]
 
 
-def summarize(paths):
-    """Markdown summary of a --profile-steps run from rocprofv3 CSVs."""
+def summarize(paths, last=0, sps=0.0):
+    """Markdown summary of a --profile-steps run from rocprofv3 CSVs: the
+    dispatches after the last target-render kernel (write_rgba_kernel), or,
+    with `last`, after the start of the last `last` Adam steps' window."""
     import csv
     import statistics as st
 
@@ -378,6 +389,10 @@ def summarize(paths):
         rows = list(csv.DictReader(open(path)))
         marks = [int(r[ts_key]) for r in rows if "write_rgba_kernel" in r["Kernel_Name"]]
         start = max(marks) if marks else 0
+        if last:
+            adam = sorted(int(r["End_Timestamp"]) for r in rows if "adam_kernel" in r["Kernel_Name"])
+            if len(adam) > last:
+                start = max(start, adam[-last - 1])
         return [r for r in rows if int(r[ts_key]) > start]
 
     trace = training_rows(paths[0], "Start_Timestamp")
@@ -409,16 +424,19 @@ def summarize(paths):
         kf, kw = probe / cf, probe / cw
         print(f"\nCounter scale from the 256 MiB copy probe: FETCH_SIZE x {kf:.1f} B, WRITE_SIZE x {kw:.1f} B "
               f"per unit\n")
-        print("| kernel | HBM read MB/launch | HBM write MB/launch |")
-        print("|---|---:|---:|")
+        head = "| kernel | HBM read MB/launch | HBM write MB/launch |"
+        print(head + (" HBM B/sample |" if sps else ""))
+        print("|---|---:|---:|" + ("---:|" if sps else ""))
         for k in sorted(fetch, key=lambda k: -fetch[k])[:12]:
-            print(f"| {k} | {fetch[k] * kf / 1e6:.2f} | {write.get(k, 0.0) * kw / 1e6:.2f} |")
+            rd, wr = fetch[k] * kf, write.get(k, 0.0) * kw
+            row = f"| {k} | {rd / 1e6:.2f} | {wr / 1e6:.2f} |"
+            print(row + (f" {(rd + wr) / sps:.0f} |" if sps else ""))
 
 
 def main():
     args = parse()
     if args.summarize:
-        summarize(args.summarize)
+        summarize(args.summarize, args.summarize_last, args.samples_per_step)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build has no multi-GPU "
