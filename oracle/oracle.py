@@ -78,6 +78,9 @@ def lib():
             "oc_nerf_fwd_f64": (None, [C.c_void_p, u32, u32, f32p, f32p, f64p, i32, f64p, f64p]),
             "oc_blob_eval": (None, [u32, f32p, u32, f32p, f32p, f32p]),
             "oc_composite_analytic_f64": (None, [u32, u32p, u32p, f32p, u32, f32, f32p, u32, f32p, f64, f64p]),
+            "oc_render_rays": (None, [u32, u32, u32, C.c_void_p, C.c_void_p, f32p, f32p, f32p, f32p, u8p]),
+            "oc_render_fixed_step_analytic_f64": (None, [u32, u32, f32p, f32p, f32p, f32p, C.c_void_p, u8p, i32, i32,
+                                                         f32, f32, f32, u32, u32, f32p, f64, f64p]),
             "oc_num_threads": (i32, []),
         }
         for name, (res, args) in sig.items():
@@ -265,6 +268,26 @@ def mlp_train(net, feat, dir01, G, dL_ds, t, loss_scale, extra=None, fp64=True, 
     fn(n, n, c(net, dt_), c(feat, dt_), c(dir01, np.float32), p(G, dt_), p(dL_ds, np.float32), p(extra, np.float32),
        p(t, np.float32), loss_scale, dens.ctypes.data, rgb.ctypes.data, dfeat.ctypes.data, dW.ctypes.data)
     return dens, rgb, dfeat, dW
+
+
+def render_rays(n, start, stride, camera, bbox):
+    """Render rays of pixels [start, start + n): dict of SoA arrays (stride)."""
+    o = dict(ori=np.zeros(3 * stride, np.float32), dir=np.zeros(3 * stride, np.float32),
+             t=np.zeros(stride, np.float32), tmax=np.zeros(stride, np.float32), alive=np.zeros(stride, np.uint8))
+    lib().oc_render_rays(n, start, stride, C.addressof(camera), C.addressof(bbox), o["ori"], o["dir"], o["t"],
+                         o["tmax"], o["alive"])
+    return o
+
+
+def render_fixed_step_analytic(n_rays, stride, ori, dir, t, tmax, bbox, bitfield, n_levels, res, cone, dt_min, dt_max,
+                               max_steps, blobs, min_T=1e-4):
+    """(4, n_rays) rgba of the reference's fixed-step render loop over the blob field."""
+    out = np.zeros(4 * n_rays, np.float64)
+    lib().oc_render_fixed_step_analytic_f64(n_rays, stride, c(ori, np.float32), c(dir, np.float32), c(t, np.float32),
+                                            c(tmax, np.float32), C.addressof(bbox), c(bitfield, np.uint8), n_levels,
+                                            res, cone, dt_min, dt_max, max_steps, len(blobs),
+                                            c(blobs, np.float32).ravel(), min_T, out)
+    return out.reshape(4, n_rays)
 
 
 def num_threads() -> int:

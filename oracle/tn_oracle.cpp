@@ -916,8 +916,10 @@ extern "C" void oc_blob_eval(uint32_t n, const float* xyz, uint32_t nb, const fl
 }
 
 /* Front-to-back compositing of the analytic field over marched samples,
- * with the reference's early termination (training-kernels-fused.cuh:243-258).
- * Output rgba[4][n_rays]: premultiplied rgb and accumulated alpha. */
+ * with the reference's early termination (training-kernels-fused.cuh:243-258)
+ * and its render rule that a sample's colour counts only when its alpha
+ * exceeds 1e-4 (rendering-kernels.cu:570-578). Output rgba[4][n_rays]:
+ * premultiplied rgb and accumulated alpha. */
 extern "C" void oc_composite_analytic_f64(uint32_t n_rays, const uint32_t* steps, const uint32_t* offs,
     const float* pos, uint32_t B, float aabb, const float* sdt, uint32_t nb, const float* bl, double min_T,
     double* rgba) {
@@ -932,9 +934,88 @@ extern "C" void oc_composite_analytic_f64(uint32_t n_rays, const uint32_t* steps
             blob_eval_d(xw, nb, bl, sig, c);
             const double alpha = 1.0 - std::exp(-sig * (double)sdt[s]);
             const double w = T * alpha;
-            r += w * c[0]; g += w * c[1]; b += w * c[2]; a += w;
+            // colour only when alpha > 1e-4 (rendering-kernels.cu:570-578)
+            if (alpha > 1e-4) { r += w * c[0]; g += w * c[1]; b += w * c[2]; }
+            a += w;
             T *= 1.0 - alpha;
             if (T < min_T) break;
+        }
+        rgba[i] = r; rgba[i + n_rays] = g; rgba[i + 2 * n_rays] = b; rgba[i + 3 * n_rays] = a;
+    }
+}
+
+/* Render rays of pixels [start, start + n) for one NeRF with an identity
+ * transform: generate_linear_buffer_of_rays_kernel + fill_ray_buffers
+ * (linear-buffer-ray-batch-coordinator.cu:11-34, ray-batch-coordinator.cuh:46-84)
+ * then prepare_for_linear_raymarching_kernel (rendering-kernels.cu:174-260):
+ * the origin moves to the box entry max(tmin, 1e-4), t starts at 0 and
+ * t_max = min(box exit, far - near) measured from the original origin. */
+extern "C" void oc_render_rays(uint32_t n, uint32_t start, uint32_t B, const tn_camera_t* cam,
+    const tn_bbox_t* bbox, float* ori, float* dir, float* ray_t, float* ray_tmax, uint8_t* alive) {
+    const int w = cam->resolution[0];
+    for (uint32_t i = 0; i < n; ++i) {
+        const int ii = (int)(i + start);
+        const int iy = ii / w, ix = ii - iy * w;
+        f3 o, d;
+        camera_ray(*cam, (float)ix, (float)iy, o, d);
+        float tmax = cam->far_plane - cam->near_plane;
+        const f3 id = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+        const float vol = (bbox->max_x - bbox->min_x) * (bbox->max_y - bbox->min_y) * (bbox->max_z - bbox->min_z);
+        float t0, t1;
+        ray_t[i] = 0.0f;
+        if (!(vol > 0.0f) || !bbox_intersect(*bbox, o, id, t0, t1)) {
+            alive[i] = 0;
+            ori[i] = o.x; ori[i + B] = o.y; ori[i + 2 * B] = o.z;
+            dir[i] = d.x; dir[i + B] = d.y; dir[i + 2 * B] = d.z;
+            ray_tmax[i] = tmax;
+            continue;
+        }
+        t0 = fmaxf(t0, 1e-4f);
+        alive[i] = 1;
+        ray_tmax[i] = fminf(t1, tmax);
+        ori[i] = o.x + t0 * d.x; ori[i + B] = o.y + t0 * d.y; ori[i + 2 * B] = o.z + t0 * d.z;
+        dir[i] = d.x; dir[i + B] = d.y; dir[i + 2 * B] = d.z;
+    }
+}
+
+/* The reference's render loop for one NeRF, restated over the analytic
+ * field (rendering-kernels.cu:264-330 march, :404-478 occupancy filter,
+ * :480-528 accumulate, :530-618 composite; renderer.cu:215,305 steps). The
+ * ray starts at its box entry (t = 0) and steps by get_dt(t, cone, dt_min,
+ * dt_max) with no empty-space skipping; a sample counts when it is inside
+ * the box and its cell (cascade chosen by dt) is occupied; alpha =
+ * 1 - exp(-sigma dt), colour counted only when alpha > 1e-4 (the reference
+ * divides the accumulated colour by alpha with that guard); stop once T <
+ * min_T, past t_max or after max_steps steps. fp64 accumulation. */
+extern "C" void oc_render_fixed_step_analytic_f64(uint32_t n_rays, uint32_t B, const float* ori, const float* dir,
+    const float* ray_t, const float* ray_tmax, const tn_bbox_t* bbox, const uint8_t* bits, int n_levels, int res,
+    float cone, float dt_min, float dt_max, uint32_t max_steps, uint32_t nb, const float* bl, double min_T,
+    double* rgba) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t ii = 0; ii < (int64_t)n_rays; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const float ox = ori[i], oy = ori[i + B], oz = ori[i + 2 * B];
+        const float dx = dir[i], dy = dir[i + B], dz = dir[i + 2 * B];
+        double r = 0, g = 0, b = 0, a = 0, T = 1;
+        float t = ray_t[i];
+        for (uint32_t k = 0; k < max_steps && t <= ray_tmax[i]; ++k) {
+            const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
+            const float dt = get_dt(t, cone, dt_min, dt_max);
+            if (bbox_contains(*bbox, x, y, z)) {
+                const int lvl = oc_grid_level_at(n_levels, res, x, y, z, dt);
+                if (grid_occupied(bits, res, lvl, x, y, z)) {
+                    const float xw[3] = {x, y, z};
+                    double sig, c[3];
+                    blob_eval_d(xw, nb, bl, sig, c);
+                    const double alpha = 1.0 - std::exp(-sig * (double)dt);
+                    const double w = T * alpha;
+                    if (alpha > 1e-4) { r += w * c[0]; g += w * c[1]; b += w * c[2]; }
+                    a += w;
+                    T *= 1.0 - alpha;
+                    if (T < min_T) break;
+                }
+            }
+            t += dt;
         }
         rgba[i] = r; rgba[i + n_rays] = g; rgba[i + 2 * n_rays] = b; rgba[i + 3 * n_rays] = a;
     }

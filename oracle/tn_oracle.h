@@ -127,6 +127,15 @@ void oc_composite_analytic_f64(uint32_t n_rays, const uint32_t* ray_steps,
     const float* sample_dt, uint32_t n_blobs, const float* blobs, double min_transmittance,
     double* rgba);
 
+/* render rays of a pinhole camera (identity NeRF transform) */
+void oc_render_rays(uint32_t n, uint32_t start, uint32_t stride, const tn_camera_t* camera,
+    const tn_bbox_t* render_bbox, float* ori, float* dir, float* ray_t, float* ray_tmax, uint8_t* alive);
+/* the reference's fixed-step render loop over the analytic field (fp64) */
+void oc_render_fixed_step_analytic_f64(uint32_t n_rays, uint32_t stride, const float* ori, const float* dir,
+    const float* ray_t, const float* ray_tmax, const tn_bbox_t* bbox, const uint8_t* bitfield, int n_levels,
+    int resolution, float cone, float dt_min, float dt_max, uint32_t max_steps, uint32_t n_blobs,
+    const float* blobs, double min_transmittance, double* rgba);
+
 /* --- timing of the CPU baseline (OpenMP over rays) -------------------- */
 int oc_num_threads(void);
 

@@ -174,6 +174,121 @@ def test_march_generate_invariants():
         np.testing.assert_allclose(pos[:, sl[0] if False else offs[i]], np.clip(o + t[0] * d + 0.5, 0, 1), atol=1e-6)
 
 
+# --------------------------------------------------------------- render loop
+def _render_cam(w=40, h=32):
+    from turbonerf_amd import scenes
+
+    return scenes.sphere_cameras(9, 2, 2.0, w, h, 50.0)[0]
+
+
+def test_render_rays_pinhole_and_slab():
+    """Pixel rays against a float64 pinhole model and slab intersection: the
+    origin moves to the box entry (at least 1e-4 along the ray), t = 0, and
+    t_max = min(exit, far - near) from the camera origin."""
+    from turbonerf_amd import scenes
+
+    W, H = 40, 32
+    cam = _render_cam(W, H)
+    n = W * H
+    rr = oracle.render_rays(n, 0, n, cam, scenes.bbox(1.0))
+    M = np.array(cam.transform[:], np.float64).reshape(3, 4)
+    ix, iy = np.arange(n) % W, np.arange(n) // W
+    v = np.stack([(ix + cam.offset_px[0] + 0.5) / cam.focal_length[0],
+                  (iy + cam.offset_px[1] + 0.5) / cam.focal_length[1], np.ones(n)])
+    d = M[:, :3] @ v
+    d /= np.linalg.norm(d, axis=0)
+    o = M[:, 3:4] + cam.near_plane * np.linalg.norm(v, axis=0) * d
+    t_a, t_b = (-0.5 - o) / d, (0.5 - o) / d
+    t0, t1 = np.minimum(t_a, t_b).max(0), np.maximum(t_a, t_b).min(0)
+    hit = t1 >= t0
+    clear = np.abs(t1 - t0) > 1e-4  # skip grazing rays
+    assert np.array_equal(rr["alive"][clear].astype(bool), hit[clear])
+    assert 0 < hit.sum() < n
+    t0c = np.maximum(t0, 1e-4)
+    m = hit & clear
+    np.testing.assert_allclose(rr["dir"].reshape(3, n)[:, m], d[:, m], atol=1e-6)
+    np.testing.assert_allclose(rr["ori"].reshape(3, n)[:, m], (o + t0c * d)[:, m], atol=1e-5)
+    np.testing.assert_allclose(rr["tmax"][m], np.minimum(t1, cam.far_plane - cam.near_plane)[m], atol=1e-5)
+    assert np.all(rr["t"] == 0)
+
+
+def _fixed_steps(rr, i, dt, n):
+    """float32 brute force of the fixed-step loop: samples inside the box."""
+    o = rr["ori"].reshape(3, n)[:, i]
+    d = rr["dir"].reshape(3, n)[:, i]
+    t, k = F32(0), 0
+    while t <= rr["tmax"][i]:
+        x = o + t * d
+        if np.all(x >= F32(-0.5)) and np.all(x <= F32(0.5)):
+            k += 1
+        t = F32(t + F32(dt))
+    return k
+
+
+def test_render_fixed_step_closed_forms():
+    """A blob 1000x wider than the box is a constant density p with colour c:
+    after k in-box samples of constant dt, A = 1 - exp(-p k dt) and rgb = c A;
+    below alpha 1e-4 a sample adds alpha but no colour; an opaque first sample
+    ends the ray (T < 1e-4); an empty grid renders nothing."""
+    from turbonerf_amd import scenes
+
+    W, H = 8, 8
+    cam = _render_cam(W, H)
+    n = W * H
+    bbox = scenes.bbox(1.0)
+    rr = oracle.render_rays(n, 0, n, cam, bbox)
+    tmax = np.where(rr["alive"] != 0, rr["tmax"], F32(-1)).astype(F32)
+    full = np.full(128 ** 3, 0xFF, np.uint8)
+    dt = float(F32(math.sqrt(3) / 1024))
+    c = np.array([0.2, 0.5, 0.9])
+
+    def render(p, bits=full):
+        blobs = np.array([[0, 0, 0, 1000.0, p, *c]], F32)
+        return oracle.render_fixed_step_analytic(n, n, rr["ori"], rr["dir"], rr["t"], tmax, bbox, bits, 1, 128,
+                                                 1 / 256, dt, dt, 1 << 16, blobs)
+
+    k = np.array([_fixed_steps(rr, i, dt, n) if rr["alive"][i] else 0 for i in range(n)])
+    assert k.max() > 300
+    for p in (2.0, 0.01):
+        out = render(p)
+        A = 1 - np.exp(-float(F32(p)) * k * dt)
+        np.testing.assert_allclose(out[3], A, rtol=1e-5, atol=1e-12)
+        if p > 1:  # alpha 3.4e-3 per sample: colour counts
+            np.testing.assert_allclose(out[:3], c[:, None].astype(F32) * A, rtol=1e-5, atol=1e-12)
+        else:  # alpha 1.7e-5 per sample: alpha only
+            assert np.all(out[:3] == 0) and out[3].max() > 1e-3
+    out = render(1e4)
+    a1 = 1 - math.exp(-1e4 * dt)
+    hitk = k > 0
+    np.testing.assert_allclose(out[3][hitk], a1, rtol=1e-6)
+    assert np.all(out[3][~hitk] == 0)
+    assert np.all(render(2.0, np.zeros(128 ** 3, np.uint8)) == 0)
+
+
+def test_render_fixed_step_matches_marched_compositing():
+    """With every cell occupied and a constant dt the reference's fixed-step
+    loop visits the same samples as the marcher, so the two compositings of
+    the blob field agree (up to the float32 rounding of normalised positions)."""
+    from turbonerf_amd import scenes
+
+    n = 256
+    rr = _rays(n, 21)
+    bbox = scenes.bbox(1.0)
+    full = np.full(128 ** 3, 0xFF, np.uint8)
+    dt = float(F32(math.sqrt(3) / 1024))
+    blobs = scenes.blob_field(21, 8, 1.0)
+    steps, alive = oracle.march_count(n, n, bbox, full, 1, 128, 1 / 256, dt, dt, rr, 4096)
+    offs = np.concatenate([[0], np.cumsum(steps)[:-1]]).astype(np.uint32)
+    S = int(steps.sum())
+    so = oracle.march_generate(n, n, S, bbox, 1.0, full, 1, 128, dt, dt, 1 / 256, rr, offs, alive, steps)
+    ref = oracle.composite_analytic(n, steps, offs, so["pos"], S, 1.0, so["dt"], blobs)
+    tmax = np.where(rr["alive"] != 0, rr["tmax"], F32(-1)).astype(F32)
+    out = oracle.render_fixed_step_analytic(n, n, rr["ori"], rr["dir"], rr["t"], tmax, bbox, full, 1, 128, 1 / 256,
+                                            dt, dt, 4096, blobs)
+    assert ref[3].max() > 0.5
+    np.testing.assert_allclose(out, ref, rtol=0, atol=2e-6)
+
+
 # --------------------------------------------------------------- compositing
 def _comp(steps, s, c, dt, bg, target, min_T=1e-4):
     n_rays = len(steps)

@@ -72,7 +72,7 @@ TN_DEVICE void ray_loss(uint32_t r, uint32_t B, R rr, R rg, R rb, R ra, const fl
 // segmented prefix of sigma*dt, alpha, weight and the early-stop mask.
 template <typename R>
 struct SampleState {
-    R sigma, dt, T, T_next, w;
+    R sigma, dt, T, T_next, alpha, w;
     bool inc;
 };
 
@@ -95,8 +95,8 @@ TN_DEVICE SampleState<R> sample_state(bool valid, int ray, uint64_t s, const TIn
     const R p_excl = p_incl - tau;
     st.T = r_exp<R>(-p_excl);
     st.T_next = r_exp<R>(-p_incl);
-    const R alpha = (R)1 - r_exp<R>(-tau);
-    st.w = st.T * alpha;
+    st.alpha = (R)1 - r_exp<R>(-tau);
+    st.w = st.T * st.alpha;
     st.inc = valid && st.T >= (R)min_T;
     // carry of the prefix for the ray continuing past lane 63
     const int ray63 = __shfl(ray, 63, 64);
@@ -237,7 +237,10 @@ composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t P
 
 // Forward-only compositing: premultiplied rgb and alpha per ray, with the
 // reference's early termination. `raw_density` selects sigma = exp(min(s,12)-1)
-// of a network output, otherwise the input already is sigma.
+// of a network output, otherwise the input already is sigma. As in the
+// reference's render compositing (rendering-kernels.cu:570-578, colour divided
+// by alpha only when alpha > 1e-4, else zero), a sample's colour counts only
+// when its alpha exceeds 1e-4; its alpha always counts.
 template <typename TIn, typename R>
 __global__ void __launch_bounds__(64 * kCompositeWaves)
 composite_forward_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t OB, uint32_t S, float min_T,
@@ -277,8 +280,9 @@ composite_forward_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t
             cB = ldf(color, s + 2 * (uint64_t)B);
         }
         const R wv = st.inc ? st.w : (R)0;
-        Acc<R> sr = Acc<R>::of(wv * (R)cR).scan(ray), sg = Acc<R>::of(wv * (R)cG).scan(ray);
-        Acc<R> sb = Acc<R>::of(wv * (R)cB).scan(ray), sa = Acc<R>::of(wv).scan(ray);
+        const R wc = st.alpha > (R)1e-4f ? wv : (R)0;
+        Acc<R> sr = Acc<R>::of(wc * (R)cR).scan(ray), sg = Acc<R>::of(wc * (R)cG).scan(ray);
+        Acc<R> sb = Acc<R>::of(wc * (R)cB).scan(ray), sa = Acc<R>::of(wv).scan(ray);
         if (ray == prev_carry) {
             sr = sr + cr;
             sg = sg + cg;

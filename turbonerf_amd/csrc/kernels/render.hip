@@ -73,9 +73,11 @@ __global__ void render_rays_kernel(uint32_t n, uint32_t start, uint32_t B, tn_ca
     }
     tmin = fmaxf(tmin, 1e-4f);
     const float far_t = cam.far_plane - cam.near_plane;
-    ori[i] = lox + tmin * ldx;
-    ori[i + B] = loy + tmin * ldy;
-    ori[i + 2 * B] = loz + tmin * ldz;
+    // the entry point, clamped into the box: rounding can leave it an ulp
+    // outside, where the march would end the ray before its first sample
+    ori[i] = fminf(fmaxf(lox + tmin * ldx, bbox.min_x), bbox.max_x);
+    ori[i + B] = fminf(fmaxf(loy + tmin * ldy, bbox.min_y), bbox.max_y);
+    ori[i + 2 * B] = fminf(fmaxf(loz + tmin * ldz, bbox.min_z), bbox.max_z);
     dir[i] = ldx;
     dir[i + B] = ldy;
     dir[i + 2 * B] = ldz;
