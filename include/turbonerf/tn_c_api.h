@@ -197,6 +197,17 @@ int tn_hashgrid_backward(int dfeat_dtype, uint32_t n_max, const uint32_t* info, 
     uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos, const void* dfeat, float* dtable,
     tn_stream_t stream);
 
+/* The same scatter-add through a counting sort of the contributions into
+ * 4096-entry bins summed in LDS (64-bit fixed point per bin), for batches of
+ * up to n_max samples; same replacement as tn_hashgrid_backward (reference
+ * src/core/nerf-network.cu:760). The workspace (256-byte aligned,
+ * tn_hashgrid_backward_workspace_bytes, 0 = layout beyond the bin limits) is
+ * scratch; without it the atomic kernel runs. */
+size_t tn_hashgrid_backward_workspace_bytes(uint32_t n_max, tn_hashgrid_t grid);
+int tn_hashgrid_backward_binned(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
+    uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos, const void* dfeat, float* dtable,
+    void* workspace, size_t workspace_bytes, tn_stream_t stream);
+
 /* ------------------------------------------------------------------- MLPs */
 
 /* Density MLP (+ SH-4 direction encoding + colour MLP when color != 0) on

@@ -1,0 +1,153 @@
+"""The binned hash-grid backward (counting sort into 4096-entry bins, 64-bit
+fixed-point sums in LDS) against the oracle's fp64 scatter-add and the atomic
+kernel, on the default 16-level layout (T = 2^19):
+
+* samples along rays (runs of equal coarse entries merged before sorting),
+  fp32 and fp16 feature gradients, a share of samples with zero gradient
+  (skipped), batch size from the device count (info);
+* random points (no merging: level 0's single bin holds 16 chunks of
+  records, summed in parts);
+* accumulation into a non-zero table, values spanning 1e-6..1e3 (the
+  fixed-point scale follows each bin's largest value), and a non-finite
+  gradient (the bin falls back to float sums and the inf reaches the table).
+
+Bar: norm-wise 1e-6 against fp64; for the binned kernel also per entry
+|err| <= 2e-6 sum|w g| (fp32 products and run sums, summed exactly) plus
+6.4e-8 max|g| of the level (the fixed-point rounding: 2^-S per record, at
+most 2^-31 of a bin's largest record, which is a run of at most 64 samples)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rays(rng, n_rays, per, spacing):
+    o = rng.normal(size=(n_rays, 3))
+    o = 0.5 + 1.2 * o / np.linalg.norm(o, axis=1, keepdims=True)
+    tgt = rng.uniform(0.3, 0.7, size=(n_rays, 3))
+    d = tgt - o
+    ln = np.linalg.norm(d, axis=1, keepdims=True)
+    d /= ln
+    t = (ln - 0.45) + spacing * np.arange(per)[None, :]
+    p = o[:, None, :] + t[..., None] * d[:, None, :]
+    return np.clip(p.reshape(-1, 3), 0.0, 1.0).astype(np.float32).T.copy()  # [3][n]
+
+
+def _oracle_bwd(g, pos, dfeat):
+    import oracle
+
+    n = pos.shape[1]
+    out = np.zeros(g.n_params)
+    oracle.lib().oc_hashgrid_bwd_f64(ctypes.addressof(g), n, n, np.ascontiguousarray(pos.ravel()),
+                                     np.ascontiguousarray(dfeat, np.float64).ravel(), out)
+    return out
+
+
+def _level_of_entry(g):
+    lv = np.zeros(g.n_params // 2, np.int64)
+    for l in range(g.n_levels):
+        lv[g.level_offset[l]:g.level_offset[l] + g.level_size[l]] = l
+    return np.repeat(lv, 2)
+
+
+def _check(got, ref, tag, g=None, ref_abs=None, dfeat=None):
+    rel = np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300)
+    msg = f"[hash bwd {tag}] rel {rel:.2e}"
+    assert rel < 1e-6, msg
+    if ref_abs is not None:
+        gmax = np.array([np.abs(dfeat[2 * l:2 * l + 2]).max() for l in range(g.n_levels)])
+        bar = 2e-6 * ref_abs + 6.4e-8 * gmax[_level_of_entry(g)]
+        worst = float(np.max(np.abs(got - ref) / np.maximum(bar, 1e-300)))
+        msg += f" worst/bar {worst:.3f}"
+        assert worst <= 1.0, msg
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_binned_rays_vs_oracle(cuda, half):
+    import torch
+
+    from turbonerf_amd import ops
+
+    rng = np.random.default_rng(11)
+    g = ops.hashgrid_layout()
+    n_rays, per = 600, 160
+    pos = _rays(rng, n_rays, per, 0.0017)
+    n = pos.shape[1]
+    dfeat = rng.normal(0.0, 0.01, size=(32, n)).astype(np.float16 if half else np.float32)
+    dfeat[:, (np.arange(n) % per) >= 100] = 0  # samples behind the surface carry no gradient
+    dfeat_used = dfeat.astype(np.float64)
+    ref = _oracle_bwd(g, pos, dfeat_used)
+    ref_abs = _oracle_bwd(g, pos, np.abs(dfeat_used))
+    dt = torch.float16 if half else torch.float32
+    B = n + 77  # capacity above the batch; the count comes from info
+    pos_d = torch.zeros(3, B, dtype=torch.float32, device="cuda")
+    pos_d[:, :n] = torch.from_numpy(pos).cuda()
+    df_d = torch.zeros(32, B, dtype=dt, device="cuda")
+    df_d[:, :n] = torch.from_numpy(dfeat).cuda()
+    info = torch.tensor([0, n, 0, 0], dtype=torch.int32, device="cuda")
+    got = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
+    ops.hashgrid_backward(g, B, pos_d.view(-1), B, df_d.view(-1), got, info=info, binned=True)
+    atom = torch.zeros_like(got)
+    ops.hashgrid_backward(g, B, pos_d.view(-1), B, df_d.view(-1), atom, info=info)
+    got_h, atom_h = got.cpu().numpy().astype(np.float64), atom.cpu().numpy().astype(np.float64)
+    _check(got_h, ref, f"rays {'fp16' if half else 'fp32'}", g, ref_abs, dfeat_used)
+    _check(atom_h, ref, "rays atomic")
+    # untouched entries stay exactly zero in both
+    assert np.all(got_h[ref == 0] == 0)
+
+
+def test_binned_random_points_chunked_bins(cuda):
+    import torch
+
+    from turbonerf_amd import ops
+
+    rng = np.random.default_rng(12)
+    g = ops.hashgrid_layout()
+    n = 65536  # level 0 (4096 entries, one bin) receives 8 records per sample: 16 chunks
+    pos = rng.uniform(0.0, 1.0, size=(3, n)).astype(np.float32)
+    dfeat = rng.normal(0.0, 1.0, size=(32, n)).astype(np.float32)
+    ref = _oracle_bwd(g, pos, dfeat)
+    got = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
+    ops.hashgrid_backward(g, n, torch.from_numpy(pos).cuda().view(-1), n, torch.from_numpy(dfeat).cuda().view(-1),
+                          got, binned=True)
+    _check(got.cpu().numpy().astype(np.float64), ref, "random", g, _oracle_bwd(g, pos, np.abs(dfeat)),
+           dfeat.astype(np.float64))
+
+
+def test_binned_accumulates_wide_range_and_nonfinite(cuda):
+    import torch
+
+    from turbonerf_amd import ops
+
+    rng = np.random.default_rng(13)
+    g = ops.hashgrid_layout()
+    pos = _rays(rng, 200, 100, 0.003)
+    n = pos.shape[1]
+    # magnitudes from 1e-6 to 1e3 across samples
+    dfeat = (rng.normal(size=(32, n)) * 10.0 ** rng.uniform(-6, 3, size=(1, n))).astype(np.float32)
+    ref = _oracle_bwd(g, pos, dfeat)
+    pos_d = torch.from_numpy(pos).cuda().view(-1)
+    df_d = torch.from_numpy(dfeat).cuda().view(-1)
+    got = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
+    ops.hashgrid_backward(g, n, pos_d, n, df_d, got, binned=True)
+    _check(got.cpu().numpy().astype(np.float64), ref, "wide range", g, _oracle_bwd(g, pos, np.abs(dfeat)),
+           dfeat.astype(np.float64))
+    # accumulation into a non-zero table: table + increment
+    base = rng.normal(0.0, 1.0, size=g.n_params).astype(np.float32)
+    acc = torch.from_numpy(base.copy()).cuda()
+    ops.hashgrid_backward(g, n, pos_d, n, df_d, acc, binned=True)
+    _check(acc.cpu().numpy().astype(np.float64) - base, ref, "accumulate")
+    assert np.all(acc.cpu().numpy()[ref == 0] == base[ref == 0])
+    # one infinite gradient: its entries become inf, the rest stay finite
+    dinf = dfeat.copy()
+    dinf[0, 5] = np.inf
+    got2 = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
+    ops.hashgrid_backward(g, n, pos_d, n, torch.from_numpy(dinf).cuda().view(-1), got2, binned=True)
+    g2 = got2.cpu().numpy()
+    ref2 = _oracle_bwd(g, pos, dinf.astype(np.float64))
+    assert np.array_equal(np.isinf(g2), np.isinf(ref2)) and np.isinf(g2).sum() > 0
+    fin = np.isfinite(ref2)
+    _check(np.where(fin, g2, 0.0).astype(np.float64), np.where(fin, ref2, 0.0), "with inf")

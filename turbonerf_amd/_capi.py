@@ -121,6 +121,8 @@ _SIGS: dict[str, tuple] = {
     "tn_hashgrid_layout": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, vp]),
     "tn_hashgrid_forward": (C.c_int, [C.c_int, C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp]),
     "tn_hashgrid_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp]),
+    "tn_hashgrid_backward_workspace_bytes": (C.c_size_t, [u32, HashGrid]),
+    "tn_hashgrid_backward_binned": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp, C.c_size_t, vp]),
     "tn_mlp_forward": (C.c_int, [C.c_int, u32, vp, u32, u32, u32, vp, u32, vp, vp, C.c_int, vp, vp, vp]),
     "tn_mlp_backward_workspace_bytes": (sz, [u32]),
     "tn_mlp_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, vp, u32, vp, vp, vp, vp, u32, vp, vp, f32, vp, vp,

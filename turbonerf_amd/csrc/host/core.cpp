@@ -85,7 +85,7 @@ void NetworkWorkspace::enlarge(uint32_t B, uint32_t R, int prec) {
 size_t NetworkWorkspace::bytes() const {
     return features.bytes() + density.bytes() + rgb.bytes() + dL_drgb.bytes() + dfeat.bytes() + dL_ds.bytes() +
            dL_dist.bytes() + ray_loss.bytes() + ray_dist_loss.bytes() + composite_scratch.bytes() +
-           distortion_scratch.bytes() + mlp_ws.bytes();
+           distortion_scratch.bytes() + mlp_ws.bytes() + hash_ws.bytes();
 }
 
 void NerfNetwork::update_aabb_scale_if_needed(int aabb) {
@@ -183,7 +183,11 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
     }
     {
         ScopedPhase ph(tm, "hashgrid_backward", s);
-        TN_TRY(tn_hashgrid_backward(prec, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(), ts));
+        // the bin-sort scatter (its workspace follows the batch size and the layout)
+        const size_t hb = tn_hashgrid_backward_workspace_bytes(B, grid);
+        w.hash_ws.reserve(hb);
+        TN_TRY(tn_hashgrid_backward_binned(prec, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(),
+                                           w.hash_ws.data(), w.hash_ws.size(), ts));
     }
     // optimiser (reference :770-781, lr schedule :256)
     const float lr = config.learning_rate / (1.0f + config.learning_rate_decay * (float)step);

@@ -137,7 +137,7 @@ struct NetworkWorkspace {
     int precision = -1;
     DeviceBuffer<uint8_t> features, density, rgb, dL_drgb, dfeat;  // precision-typed
     DeviceBuffer<float> dL_ds, dL_dist, ray_loss, ray_dist_loss, loss_out;
-    DeviceBuffer<uint8_t> composite_scratch, distortion_scratch, mlp_ws;
+    DeviceBuffer<uint8_t> composite_scratch, distortion_scratch, mlp_ws, hash_ws;
     void enlarge(uint32_t B, uint32_t R, int precision);
     size_t bytes() const;
 };

@@ -156,6 +156,452 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_kernel(uint32_t n_max, const
     }
 }
 
+// ---------------------------------------------------------------------------
+// Binned backward. Global float atomics on this GPU are performed on the
+// memory side (none of them hit in L2), and their rate, not bandwidth, bounds
+// the scatter-add above. The binned variant turns the scatter into a counting
+// sort: the table is cut into bins of 4096 entries; (1) every block counts
+// the records it will emit per bin of its level (and the largest |value| per
+// bin), (2) one block per level turns the counts into per-(block, bin)
+// offsets and bin totals, and one block scans the totals into bin starts,
+// (3) every block recomputes its records and writes them to their bins, (4)
+// one block per bin (per chunk of at most kHbChunk records of a large bin)
+// sums its records in LDS and adds the sums to the table. A record is one
+// sample's contribution to both features of one corner entry, after runs of
+// consecutive samples on one entry have been merged. Sums in LDS are 64-bit
+// fixed point (integer LDS atomics run several times faster than float ones
+// here): with every |v| < 2^E in the bin and m records, the scale 2^S with
+// S = 62 - E - ceil(log2(m + 1)) cannot overflow, and the sum is exact up to
+// the rounding of each value to a multiple of 2^-S (relative 2^-38 of the
+// bin's largest value or better). Blocks of count and scatter work on one
+// level each: block b serves level b % L and chunks b / L + k * (blocks / L)
+// of 256 samples.
+// ---------------------------------------------------------------------------
+constexpr int kHbBinLog = 12;
+constexpr uint32_t kHbBinEntries = 1u << kHbBinLog;
+constexpr uint32_t kHbMaxLevelBins = 512;   // bins per level (level size <= 2^21)
+constexpr uint32_t kHbMaxBins = 2048;       // bins in all
+constexpr uint32_t kHbChunk = 32768;        // records per accumulating block at most
+constexpr uint32_t kHbBlocksPerLevel = 128; // count/scatter blocks per level at most
+
+struct HbPlan {
+    uint32_t base[TN_HASHGRID_MAX_LEVELS];   // first bin of level l
+    uint32_t count[TN_HASHGRID_MAX_LEVELS];  // bins of level l
+    uint32_t n_bins;
+    uint32_t row;         // per-block stride of the count/offset matrices (max bins of a level)
+    uint32_t blocks;      // count/scatter grid (a multiple of the level count)
+    uint32_t acc_blocks;  // accumulate grid: an upper bound on the number of chunks
+    uint64_t cap;         // record capacity: n_max * L * 8
+};
+
+struct HbRecord {
+    uint32_t e;  // entry within its bin
+    float v0, v1;
+};
+
+// One sample per lane, level l: the eight corner contributions of both
+// features, with runs of consecutive lanes on one entry summed into the run's
+// last lane (emit). Lanes past the batch and samples whose feature gradients
+// are both zero emit nothing.
+struct HbCorners {
+    uint32_t idx[8];
+    float a0[8], a1[8];
+    bool emit[8];
+};
+
+// DPP segmented inclusive sum over the wave of lanes with equal key (runs are
+// contiguous): rows of 16 first (row_shr 1, 2, 4, 8), then the last lane of
+// rows 0 and 2 into rows 1 and 3 (row_bcast:15) and of row 1 into rows 2 and
+// 3 (row_bcast:31). A lane adds the shifted value only when the source lane
+// has its key; sources outside the row or rows not selected read key 0,
+// which no run has.
+template <int CTRL, int ROWS>
+TN_DEVICE void seg_step2(float& a0, float& a1, int key) {
+    const int k = __builtin_amdgcn_update_dpp(0, key, CTRL, ROWS, 0xF, false);
+    const float o0 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a0), CTRL, ROWS, 0xF, false));
+    const float o1 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a1), CTRL, ROWS, 0xF, false));
+    if (k == key) {
+        a0 += o0;
+        a1 += o1;
+    }
+}
+
+TN_DEVICE void seg_sum2(float& a0, float& a1, int key) {
+    seg_step2<0x111, 0xF>(a0, a1, key);
+    seg_step2<0x112, 0xF>(a0, a1, key);
+    seg_step2<0x114, 0xF>(a0, a1, key);
+    seg_step2<0x118, 0xF>(a0, a1, key);
+    seg_step2<0x142, 0xA>(a0, a1, key);
+    seg_step2<0x143, 0xC>(a0, a1, key);
+}
+
+template <typename TG>
+TN_DEVICE void hb_corners(const tn_hashgrid_t& g, int l, uint32_t i, uint32_t n, uint32_t B, uint32_t GB,
+                          const float* __restrict__ pos, const TG* __restrict__ dfeat, HbCorners& hc) {
+    const uint32_t lane = __lane_id();
+    float g0 = 0.0f, g1 = 0.0f;
+    if (i < n) {
+        g0 = (float)dfeat[(uint64_t)(2 * l) * GB + i];
+        g1 = (float)dfeat[(uint64_t)(2 * l + 1) * GB + i];
+    }
+    const bool active = i < n && (g0 != 0.0f || g1 != 0.0f);
+    HashCorners c8;
+    if (active) {
+        hash_corners(g, l, pos[i], pos[i + B], pos[i + 2 * (uint64_t)B], c8);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            c8.idx[c] = 0;
+            c8.w[c] = 0.0f;
+        }
+    }
+    const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        // runs are keyed by their ordinal, not by the entry: the same entry
+        // may recur further on (hash collisions)
+        const int ent = active ? (int)c8.idx[c] : -1 - (int)lane;
+        const int prev = __builtin_amdgcn_update_dpp(-1, ent, 0x138, 0xF, 0xF, false);  // wave_shr:1
+        const bool head = lane == 0 || prev != ent;
+        const uint64_t hm = __ballot(head);
+        float a0 = c8.w[c] * g0, a1 = c8.w[c] * g1;
+        bool emit = active;
+        if (hm != ~0ull) {
+            seg_sum2(a0, a1, __builtin_popcountll(hm & le));
+            emit = active && (lane == 63 || ((hm >> (lane + 1)) & 1ull) != 0);
+        }
+        hc.idx[c] = c8.idx[c];
+        hc.a0[c] = a0;
+        hc.a1[c] = a1;
+        hc.emit[c] = emit;
+    }
+}
+
+TN_DEVICE uint32_t abs_bits(float a, float b) {
+    return max(__float_as_uint(fabsf(a)), __float_as_uint(fabsf(b)));
+}
+
+// (1) per-block record counts and largest |value| per bin of the block's level
+template <typename TG>
+__global__ void __launch_bounds__(256) hashgrid_bwd_count_kernel(uint32_t n_max, const uint32_t* __restrict__ info,
+                                                                 uint32_t B, uint32_t GB, tn_hashgrid_t g, HbPlan p,
+                                                                 const float* __restrict__ pos,
+                                                                 const TG* __restrict__ dfeat,
+                                                                 uint32_t* __restrict__ counts,
+                                                                 uint32_t* __restrict__ maxes) {
+    __shared__ uint32_t hist[kHbMaxLevelBins], bmax[kHbMaxLevelBins];
+    const uint32_t L = (uint32_t)g.n_levels;
+    const int l = (int)(blockIdx.x % L);
+    const uint32_t nb = p.count[l];
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
+        hist[k] = 0;
+        bmax[k] = 0;
+    }
+    __syncthreads();
+    const uint32_t n = info ? min(info[1], n_max) : n_max;
+    const uint32_t chunks = (n + 255) / 256, step = p.blocks / L;
+    for (uint32_t c = blockIdx.x / L; c < chunks; c += step) {
+        HbCorners hc;
+        hb_corners<TG>(g, l, c * 256 + threadIdx.x, n, B, GB, pos, dfeat, hc);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (hc.emit[k]) {
+                const uint32_t b = (hc.idx[k] - g.level_offset[l]) >> kHbBinLog;
+                atomicAdd(&hist[b], 1u);
+                atomicMax(&bmax[b], abs_bits(hc.a0[k], hc.a1[k]));
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
+        counts[(uint64_t)blockIdx.x * p.row + k] = hist[k];
+        maxes[(uint64_t)blockIdx.x * p.row + k] = bmax[k];
+    }
+}
+
+// (2a) one block per level: bin totals, largest values, and each block's
+// record offset within its bins (blocks in order), from the level's count
+// columns staged in LDS
+__global__ void __launch_bounds__(1024) hashgrid_bwd_reduce_kernel(tn_hashgrid_t g, HbPlan p,
+                                                                   const uint32_t* __restrict__ counts,
+                                                                   const uint32_t* __restrict__ maxes,
+                                                                   uint32_t* __restrict__ offsets,
+                                                                   uint32_t* __restrict__ totals,
+                                                                   uint32_t* __restrict__ binmax) {
+    __shared__ uint32_t cnt[kHbBlocksPerLevel * 64];
+    __shared__ uint32_t mx[kHbMaxLevelBins];
+    const uint32_t L = (uint32_t)g.n_levels, l = blockIdx.x, step = p.blocks / L, nb = p.count[l];
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) mx[k] = 0;
+    __syncthreads();
+    // columns in slices of 64 bins: cnt[j][c] for block j of the level
+    for (uint32_t k0 = 0; k0 < nb; k0 += 64) {
+        const uint32_t w = min(64u, nb - k0);
+        for (uint32_t e = threadIdx.x; e < step * 64; e += blockDim.x) {
+            const uint32_t j = e >> 6, c = e & 63u;
+            uint32_t v = 0;
+            if (c < w) {
+                const uint64_t b = (uint64_t)(l + L * j) * p.row + k0 + c;
+                v = counts[b];
+                atomicMax(&mx[k0 + c], maxes[b]);
+            }
+            cnt[e] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < w) {
+            const uint32_t c = threadIdx.x;
+            uint32_t run = 0;
+            for (uint32_t j = 0; j < step; ++j) {
+                offsets[(uint64_t)(l + L * j) * p.row + k0 + c] = run;
+                run += cnt[j * 64 + c];
+            }
+            totals[p.base[l] + k0 + c] = run;
+        }
+        __syncthreads();
+    }
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) binmax[p.base[l] + k] = mx[k];
+}
+
+// (2b) one block of 1024: exclusive scans of the bin totals (record starts)
+// and of the chunks per bin (chunk starts)
+__global__ void __launch_bounds__(1024) hashgrid_bwd_scan_kernel(HbPlan p, const uint32_t* __restrict__ totals,
+                                                                 uint32_t* __restrict__ start,
+                                                                 uint32_t* __restrict__ chunk_start) {
+    __shared__ uint32_t part[2][1024];
+    const uint32_t k0 = 2 * threadIdx.x;
+    const uint32_t t0 = k0 < p.n_bins ? totals[k0] : 0u, t1 = k0 + 1 < p.n_bins ? totals[k0 + 1] : 0u;
+    const uint32_t c0 = k0 < p.n_bins ? max(1u, (t0 + kHbChunk - 1) / kHbChunk) : 0u;
+    const uint32_t c1 = k0 + 1 < p.n_bins ? max(1u, (t1 + kHbChunk - 1) / kHbChunk) : 0u;
+    part[0][threadIdx.x] = t0 + t1;
+    part[1][threadIdx.x] = c0 + c1;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const uint32_t a = threadIdx.x >= d ? part[0][threadIdx.x - d] : 0u;
+        const uint32_t b = threadIdx.x >= d ? part[1][threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[0][threadIdx.x] += a;
+        part[1][threadIdx.x] += b;
+        __syncthreads();
+    }
+    const uint32_t ex_t = part[0][threadIdx.x] - (t0 + t1), ex_c = part[1][threadIdx.x] - (c0 + c1);
+    if (k0 < p.n_bins) {
+        start[k0] = ex_t;
+        chunk_start[k0] = ex_c;
+    }
+    if (k0 + 1 < p.n_bins) {
+        start[k0 + 1] = ex_t + t0;
+        chunk_start[k0 + 1] = ex_c + c0;
+    }
+    if (threadIdx.x == 1023) {
+        start[p.n_bins] = part[0][1023];
+        chunk_start[p.n_bins] = part[1][1023];
+    }
+}
+
+// (3) records to their bins, in the same order as the count pass
+template <typename TG>
+__global__ void __launch_bounds__(256) hashgrid_bwd_scatter_kernel(uint32_t n_max, const uint32_t* __restrict__ info,
+                                                                   uint32_t B, uint32_t GB, tn_hashgrid_t g, HbPlan p,
+                                                                   const float* __restrict__ pos,
+                                                                   const TG* __restrict__ dfeat,
+                                                                   const uint32_t* __restrict__ offsets,
+                                                                   const uint32_t* __restrict__ start,
+                                                                   HbRecord* __restrict__ rec) {
+    __shared__ uint32_t cur[kHbMaxLevelBins];
+    const uint32_t L = (uint32_t)g.n_levels;
+    const int l = (int)(blockIdx.x % L);
+    const uint32_t nb = p.count[l];
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x)
+        cur[k] = start[p.base[l] + k] + offsets[(uint64_t)blockIdx.x * p.row + k];
+    __syncthreads();
+    const uint32_t n = info ? min(info[1], n_max) : n_max;
+    const uint32_t chunks = (n + 255) / 256, step = p.blocks / L;
+    for (uint32_t c = blockIdx.x / L; c < chunks; c += step) {
+        HbCorners hc;
+        hb_corners<TG>(g, l, c * 256 + threadIdx.x, n, B, GB, pos, dfeat, hc);
+        uint32_t slot[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t local = hc.idx[k] - g.level_offset[l];
+            slot[k] = hc.emit[k] ? atomicAdd(&cur[local >> kHbBinLog], 1u) : ~0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (hc.emit[k] && slot[k] < p.cap) {
+                HbRecord r;
+                r.e = (hc.idx[k] - g.level_offset[l]) & (kHbBinEntries - 1u);
+                r.v0 = hc.a0[k];
+                r.v1 = hc.a1[k];
+                rec[slot[k]] = r;
+            }
+        }
+    }
+}
+
+TN_DEVICE long long hb_fixed(float v, int S) { return (long long)rintf(ldexpf(v, S)); }
+
+// (4) one block per chunk of a bin: fixed-point sums in LDS, then dtable +=
+// (atomically when the bin is split over several chunks). A bin holding a
+// non-finite value sums in float instead, so inf and NaN reach the table.
+__global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgrid_t g, HbPlan p,
+                                                                      const uint32_t* __restrict__ start,
+                                                                      const uint32_t* __restrict__ chunk_start,
+                                                                      const uint32_t* __restrict__ binmax,
+                                                                      const HbRecord* __restrict__ rec,
+                                                                      float* __restrict__ dtable) {
+    __shared__ unsigned long long acc[2 * kHbBinEntries];
+    const uint32_t j = blockIdx.x;
+    if (j >= chunk_start[p.n_bins]) return;
+    uint32_t lo = 0, hi = p.n_bins;  // the bin k with chunk_start[k] <= j < chunk_start[k + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (chunk_start[mid] <= j) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t k = lo;
+    const uint32_t part = j - chunk_start[k], nparts = chunk_start[k + 1] - chunk_start[k];
+    uint32_t l = 0;
+    while (l + 1 < (uint32_t)g.n_levels && p.base[l + 1] <= k) ++l;
+    const uint32_t local0 = (k - p.base[l]) << kHbBinLog;
+    const uint32_t ne = min(kHbBinEntries, g.level_size[l] - local0);
+    for (uint32_t e = threadIdx.x; e < 2 * ne; e += blockDim.x) acc[e] = 0ull;
+    __syncthreads();
+    const uint64_t r0 = (uint64_t)start[k] + (uint64_t)part * kHbChunk;
+    const uint64_t r1 = min(min((uint64_t)start[k + 1], r0 + kHbChunk), p.cap);
+    const uint32_t bits = binmax[k];
+    const bool finite = bits < 0x7F800000u;
+    const int E = (int)(bits >> 23) - 126;  // every |v| < 2^E
+    const int S = 62 - E - (32 - __builtin_clz((uint32_t)(r1 - r0) + 1u));
+    float* accf = reinterpret_cast<float*>(acc);
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += 4 * blockDim.x) {
+        HbRecord q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t rr = r + (uint64_t)u * blockDim.x;
+            q[u] = rr < r1 ? rec[rr] : HbRecord{~0u, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (q[u].e >= ne) continue;
+            if (finite) {
+                atomicAdd(&acc[2 * q[u].e], (unsigned long long)hb_fixed(q[u].v0, S));
+                atomicAdd(&acc[2 * q[u].e + 1], (unsigned long long)hb_fixed(q[u].v1, S));
+            } else {
+                atomicAdd(&accf[2 * q[u].e], q[u].v0);
+                atomicAdd(&accf[2 * q[u].e + 1], q[u].v1);
+            }
+        }
+    }
+    __syncthreads();
+    float2* dt2 = reinterpret_cast<float2*>(dtable) + g.level_offset[l] + local0;
+    for (uint32_t e = threadIdx.x; e < ne; e += blockDim.x) {
+        float v0, v1;
+        if (finite) {
+            const long long q0 = (long long)acc[2 * e], q1 = (long long)acc[2 * e + 1];
+            if (q0 == 0 && q1 == 0) continue;
+            v0 = (float)ldexp((double)q0, -S);
+            v1 = (float)ldexp((double)q1, -S);
+        } else {
+            v0 = accf[2 * e];
+            v1 = accf[2 * e + 1];
+            if (v0 == 0.0f && v1 == 0.0f) continue;
+        }
+        if (nparts == 1) {
+            float2 o = dt2[e];
+            o.x += v0;
+            o.y += v1;
+            dt2[e] = o;
+        } else {
+            atomicAdd(&dt2[e].x, v0);
+            atomicAdd(&dt2[e].y, v1);
+        }
+    }
+}
+
+// Bin plan for a layout and batch capacity; false when the table does not
+// fit the bin limits (the caller then takes the atomic kernel).
+inline bool hb_plan(const tn_hashgrid_t& g, uint32_t n_max, HbPlan& p) {
+    p = HbPlan{};
+    uint32_t n_bins = 0, row = 0;
+    for (int l = 0; l < g.n_levels; ++l) {
+        p.base[l] = n_bins;
+        p.count[l] = (g.level_size[l] + kHbBinEntries - 1) / kHbBinEntries;
+        n_bins += p.count[l];
+        row = row > p.count[l] ? row : p.count[l];
+    }
+    if (row > kHbMaxLevelBins || n_bins > kHbMaxBins || n_bins == 0) return false;
+    const uint64_t L = (uint64_t)g.n_levels;
+    const uint64_t chunks = ((uint64_t)n_max + 255) / 256;
+    const uint64_t per_level = chunks < kHbBlocksPerLevel ? (chunks > 0 ? chunks : 1) : kHbBlocksPerLevel;
+    p.n_bins = n_bins;
+    p.row = row;
+    p.blocks = (uint32_t)(per_level * L);
+    p.cap = (uint64_t)n_max * L * 8;
+    if (p.cap >= (1ull << 32)) return false;  // record offsets are 32-bit
+    p.acc_blocks = (uint32_t)((uint64_t)n_bins + (p.cap + kHbChunk - 1) / kHbChunk);
+    return true;
+}
+
+struct HbWorkspace {
+    uint32_t *counts, *maxes, *offsets, *totals, *start, *chunk_start, *binmax;
+    HbRecord* rec;
+};
+
+inline size_t hb_align(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// workspace bytes for a plan, and the carve-up of a workspace pointer
+inline size_t hb_workspace(const HbPlan& p, void* base, HbWorkspace* ws) {
+    const size_t mat = (size_t)p.blocks * p.row * sizeof(uint32_t);
+    const size_t bins1 = ((size_t)p.n_bins + 1) * sizeof(uint32_t);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += hb_align(bytes);
+        return reinterpret_cast<unsigned char*>(base) + o;
+    };
+    unsigned char* c = take(mat);
+    unsigned char* m = take(mat);
+    unsigned char* o = take(mat);
+    unsigned char* t = take(bins1);
+    unsigned char* s = take(bins1);
+    unsigned char* cs = take(bins1);
+    unsigned char* bm = take(bins1);
+    unsigned char* r = take((size_t)p.cap * sizeof(HbRecord));
+    if (ws) {
+        ws->counts = reinterpret_cast<uint32_t*>(c);
+        ws->maxes = reinterpret_cast<uint32_t*>(m);
+        ws->offsets = reinterpret_cast<uint32_t*>(o);
+        ws->totals = reinterpret_cast<uint32_t*>(t);
+        ws->start = reinterpret_cast<uint32_t*>(s);
+        ws->chunk_start = reinterpret_cast<uint32_t*>(cs);
+        ws->binmax = reinterpret_cast<uint32_t*>(bm);
+        ws->rec = reinterpret_cast<HbRecord*>(r);
+    }
+    return off;
+}
+
+template <typename TG>
+int hashgrid_backward_binned(uint32_t n_max, const uint32_t* info, uint32_t B, uint32_t GB, const tn_hashgrid_t& g,
+                             const HbPlan& p, const float* pos, const TG* dfeat, float* dtable, void* workspace,
+                             hipStream_t stream) {
+    HbWorkspace ws;
+    hb_workspace(p, workspace, &ws);
+    hashgrid_bwd_count_kernel<TG><<<p.blocks, 256, 0, stream>>>(n_max, info, B, GB, g, p, pos, dfeat, ws.counts,
+                                                                ws.maxes);
+    TN_CHECK_LAUNCH();
+    hashgrid_bwd_reduce_kernel<<<g.n_levels, 1024, 0, stream>>>(g, p, ws.counts, ws.maxes, ws.offsets, ws.totals,
+                                                                ws.binmax);
+    TN_CHECK_LAUNCH();
+    hashgrid_bwd_scan_kernel<<<1, 1024, 0, stream>>>(p, ws.totals, ws.start, ws.chunk_start);
+    TN_CHECK_LAUNCH();
+    hashgrid_bwd_scatter_kernel<TG><<<p.blocks, 256, 0, stream>>>(n_max, info, B, GB, g, p, pos, dfeat, ws.offsets,
+                                                                  ws.start, ws.rec);
+    TN_CHECK_LAUNCH();
+    hashgrid_bwd_accumulate_kernel<<<p.acc_blocks, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax, ws.rec,
+                                                                     dtable);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
 }  // namespace tn
 
 using namespace tn;
@@ -239,4 +685,34 @@ extern "C" int tn_hashgrid_backward(int dfeat_dtype, uint32_t n_max, const uint3
         return TN_ERR_INVALID_ARGUMENT;
     TN_CHECK_LAUNCH();
     return TN_OK;
+}
+
+// Workspace of tn_hashgrid_backward_binned for batches of up to n_max
+// samples; 0 when the layout exceeds the bin limits.
+extern "C" size_t tn_hashgrid_backward_workspace_bytes(uint32_t n_max, tn_hashgrid_t grid) {
+    if (grid.n_levels < 1 || grid.n_levels > TN_HASHGRID_MAX_LEVELS || grid.n_features != 2 || n_max == 0) return 0;
+    HbPlan p;
+    if (!hb_plan(grid, n_max, p)) return 0;
+    return hb_workspace(p, nullptr, nullptr);
+}
+
+// The scatter-add of tn_hashgrid_backward through the bin sort (same
+// arguments and result up to summation order); a missing or short workspace,
+// or a layout beyond the bin limits, takes the atomic kernel.
+extern "C" int tn_hashgrid_backward_binned(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
+                                           uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos,
+                                           const void* dfeat, float* dtable, void* workspace, size_t workspace_bytes,
+                                           hipStream_t stream) {
+    TN_CHECK_ARG(pos && dfeat && dtable && n_max <= stride && n_max <= dfeat_stride && grid.n_features == 2);
+    TN_CHECK_ARG(grid.n_levels >= 1 && grid.n_levels <= TN_HASHGRID_MAX_LEVELS && (dfeat_dtype == 0 || dfeat_dtype == 1));
+    if (n_max == 0) return TN_OK;
+    HbPlan p;
+    if (!workspace || !hb_plan(grid, n_max, p) || workspace_bytes < hb_workspace(p, nullptr, nullptr))
+        return tn_hashgrid_backward(dfeat_dtype, n_max, info, stride, dfeat_stride, grid, pos, dfeat, dtable, stream);
+    TN_CHECK_ARG((reinterpret_cast<uintptr_t>(workspace) & 255) == 0 && (reinterpret_cast<uintptr_t>(dtable) & 7) == 0);
+    if (dfeat_dtype == 0)
+        return hashgrid_backward_binned<float>(n_max, info, stride, dfeat_stride, grid, p, pos,
+                                               static_cast<const float*>(dfeat), dtable, workspace, stream);
+    return hashgrid_backward_binned<_Float16>(n_max, info, stride, dfeat_stride, grid, p, pos,
+                                              static_cast<const _Float16*>(dfeat), dtable, workspace, stream);
 }

@@ -230,8 +230,19 @@ def hashgrid_forward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: in
 
 
 def hashgrid_backward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: int, dfeat: torch.Tensor,
-                      dtable: torch.Tensor, info: torch.Tensor | None = None, dfeat_stride: int | None = None):
+                      dtable: torch.Tensor, info: torch.Tensor | None = None, dfeat_stride: int | None = None,
+                      binned: bool = False):
+    """dtable += the table gradient. binned: the bin-sort kernels (with their
+    workspace) instead of the atomic scatter-add."""
     dfeat_stride = stride if dfeat_stride is None else dfeat_stride
+    if binned:
+        nb = query("tn_hashgrid_backward_workspace_bytes", n, grid)
+        if nb == 0:
+            raise ValueError("hash-grid layout exceeds the bin limits")
+        ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        call("tn_hashgrid_backward_binned", _PREC[dfeat.dtype], n, ptr(info), stride, dfeat_stride, grid, ptr(pos),
+             ptr(dfeat), ptr(dtable), ptr(ws), nb, stream_ptr())
+        return
     call("tn_hashgrid_backward", _PREC[dfeat.dtype], n, ptr(info), stride, dfeat_stride, grid, ptr(pos), ptr(dfeat),
          ptr(dtable), stream_ptr())
 
