@@ -200,6 +200,40 @@ TN_DEVICE f16v gemm_dt(const typename P::T* lds, const LayerDesc& L, int mt, con
     return acc;
 }
 
+// An activation tile kept as the next layer's B operands (network precision,
+// two fp16 values per register in mixed precision): K-step ks holds
+// accumulator registers [SL*ks, SL*ks + SL) of the fp32 tile.
+template <class P>
+struct OpTile {
+    typename P::Op k[kKsPerTile<P>];
+};
+
+template <class P>
+TN_DEVICE OpTile<P> to_ops(const f16v& D) {
+    OpTile<P> t;
+#pragma unroll
+    for (int ks = 0; ks < kKsPerTile<P>; ++ks) t.k[ks] = dt_op<P>(D, ks);
+    return t;
+}
+
+// accumulator register r of a kept tile
+template <class P>
+TN_DEVICE float op_get(const OpTile<P>& t, int r) {
+    if constexpr (P::SL == 8) return (float)t.k[r / 8][r % 8];
+    else return t.k[r];
+}
+
+// y = A . x over K-steps from kept tiles x[]
+template <class P, int NT_IN>
+TN_DEVICE f16v gemm_op(const typename P::T* lds, const LayerDesc& L, int mt, const OpTile<P> (&x)[NT_IN], int lane,
+                       int n_ks) {
+    f16v acc = zero16();
+#pragma unroll
+    for (int ks = 0; ks < n_ks; ++ks)
+        acc = P::mma(load_a<P>(lds, L, mt, ks, lane), x[ks / kKsPerTile<P>].k[ks % kKsPerTile<P>], acc);
+    return acc;
+}
+
 // Real SH basis to degree 4 (16 coefficients) of direction d = 2*in - 1
 // (Sloan, "Efficient Spherical Harmonic Evaluation", JCGT 2013).
 TN_DEVICE void sh4(float ix, float iy, float iz, float* o) {
@@ -229,26 +263,33 @@ template <class P>
 struct Tile {
     typename P::Op x[32 / P::KC];   // hash features, natural order
     typename P::Op sh[16 / P::KC];  // SH coefficients, natural order
-    f16v h1[2], s, h2[2], h3[2], o;
+    OpTile<P> h1[2], s, h2[2], h3[2];
+    float s0;  // raw density output (row 0 of s)
+    f16v o;    // colour logits (rows 0..2)
 };
 
+// Inputs of one tile. Lanes past the batch load sample 0 and zero what they
+// loaded (no divergent branches around the loads); element offsets are
+// 32-bit (the host checks 32 * stride < 2^32).
 template <class P, typename TF>
-TN_DEVICE void load_inputs(Tile<P>& tl, bool valid, uint64_t sample, int h, const TF* __restrict__ feat,
+TN_DEVICE void load_inputs(Tile<P>& tl, bool valid, uint32_t sample, int h, const TF* __restrict__ feat,
                            uint32_t FS, const float* __restrict__ dir, uint32_t B, bool need_sh) {
+    const uint32_t sl = valid ? sample : 0u;
 #pragma unroll
     for (int ks = 0; ks < 32 / P::KC; ++ks) {
         float v[P::SL];
 #pragma unroll
         for (int j = 0; j < P::SL; ++j) {
-            const int row = nat_in<P>(ks, h, j);
-            v[j] = valid ? (float)feat[(uint64_t)row * FS + sample] : 0.0f;
+            const uint32_t row = (uint32_t)nat_in<P>(ks, h, j);
+            const float x = (float)feat[row * FS + sl];
+            v[j] = valid ? x : 0.0f;
         }
         tl.x[ks] = P::from(v);
     }
     if (need_sh) {
         float shv[16];
-        if (valid) sh4(dir[sample], dir[sample + B], dir[sample + 2 * (uint64_t)B], shv);
-        else
+        sh4(dir[sl], dir[sl + B], dir[sl + 2 * B], shv);
+        if (!valid)
             for (int k = 0; k < 16; ++k) shv[k] = 0.0f;
 #pragma unroll
         for (int ks = 0; ks < 16 / P::KC; ++ks) {
@@ -268,24 +309,26 @@ TN_DEVICE void forward_tile(Tile<P>& tl, const FwdLds<P>& W, int lane, bool colo
         f16v acc = zero16();
 #pragma unroll
         for (int ks = 0; ks < N::L1.n_ks; ++ks) acc = P::mma(load_a<P>(W.l1, N::L1, mt, ks, lane), tl.x[ks], acc);
-        tl.h1[mt] = relu(acc);
+        tl.h1[mt] = to_ops<P>(relu(acc));
     }
-    tl.s = gemm_dt<P, 2>(W.l2, N::L2, 0, tl.h1, lane, N::L2.n_ks);
+    const f16v s = gemm_op<P, 2>(W.l2, N::L2, 0, tl.h1, lane, N::L2.n_ks);
+    tl.s0 = s[0];
+    tl.s = to_ops<P>(s);
     if (!color) return;
     constexpr int half = kKsPerTile<P> / 2;
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
         f16v acc = zero16();
 #pragma unroll
-        for (int ks = 0; ks < half; ++ks) acc = P::mma(load_a<P>(W.l3, N::L3, mt, ks, lane), dt_op<P>(tl.s, ks), acc);
+        for (int ks = 0; ks < half; ++ks) acc = P::mma(load_a<P>(W.l3, N::L3, mt, ks, lane), tl.s.k[ks], acc);
 #pragma unroll
         for (int ks = 0; ks < 16 / P::KC; ++ks)
             acc = P::mma(load_a<P>(W.l3, N::L3, mt, half + ks, lane), tl.sh[ks], acc);
-        tl.h2[mt] = relu(acc);
+        tl.h2[mt] = to_ops<P>(relu(acc));
     }
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) tl.h3[mt] = relu(gemm_dt<P, 2>(W.l4, N::L4, mt, tl.h2, lane, N::L4.n_ks));
-    tl.o = gemm_dt<P, 2>(W.l5, N::L5, 0, tl.h3, lane, N::L5.n_ks);
+    for (int mt = 0; mt < 2; ++mt) tl.h3[mt] = to_ops<P>(relu(gemm_op<P, 2>(W.l4, N::L4, mt, tl.h2, lane, N::L4.n_ks)));
+    tl.o = gemm_op<P, 2>(W.l5, N::L5, 0, tl.h3, lane, N::L5.n_ks);
 }
 
 TN_DEVICE float sigmoidf(float x) { return 1.0f / (1.0f + __expf(-x)); }
@@ -309,17 +352,17 @@ mlp_forward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t B
     const uint32_t wave = blockIdx.x * kMlpWaves + (threadIdx.x >> 6);
     const uint32_t n_waves = gridDim.x * kMlpWaves;
     for (uint32_t t = wave; t < n_tiles; t += n_waves) {
-        const uint64_t sample = (uint64_t)t * 32 + (lane & 31);
+        const uint32_t sample = t * 32 + (lane & 31);
         const bool valid = sample < n;
         Tile<P> tl;
         load_inputs<P>(tl, valid, sample, h, feat, FS, dir, B, color != 0);
         forward_tile<P>(tl, W, lane, color != 0);
         if (valid && h == 0) {
-            density_out[sample] = (typename P::T)tl.s[0];
+            density_out[sample] = (typename P::T)tl.s0;
             if (color) {
                 rgb_out[sample] = (typename P::T)sigmoidf(tl.o[0]);
                 rgb_out[sample + OS] = (typename P::T)sigmoidf(tl.o[1]);
-                rgb_out[sample + 2 * (uint64_t)OS] = (typename P::T)sigmoidf(tl.o[2]);
+                rgb_out[sample + 2 * OS] = (typename P::T)sigmoidf(tl.o[2]);
             }
         }
     }
@@ -346,104 +389,78 @@ TN_DEVICE BwdLds<P> stage_backward(typename P::T* lds, const typename P::T* __re
     return b;
 }
 
-// LDS transpose buffers: row stride 40 elements in mixed precision, so that
-// a row's 8 consecutive samples (one fp16 MFMA operand) are one aligned
-// 16-byte read; 33 in fp32 (scalar operands; the +1 spreads banks and the
-// fp32 weights leave no room for more)
-template <class P>
-constexpr int kTW = P::SL == 8 ? 40 : 33;
+// Backward. A workgroup of kMlpWaves = 4 waves takes 128 samples at a time,
+// wave w the 32-sample tile w. Each wave recomputes its tile's forward pass,
+// keeps the activations as operand tiles and runs the delta chain down to
+// dfeat. Weight gradients dW = delta * act^T take the samples as K: each
+// layer's deltas and activations of all four tiles go to LDS in
+// [row][sample] layout, and the four waves share the layer's 32x32 dW tiles
+// -- with four tiles (W4) one each, with two tiles (W1, W2, W3, W5) each
+// tile's 128 samples split between two waves -- so a wave accumulates five
+// dW tiles across the batch instead of twelve.
+constexpr int kGrp = 32 * kMlpWaves;  // samples per workgroup step
+static_assert(kMlpWaves == 4, "the dW split assumes four waves");
 
-// write an accumulator tile as rows [32*t, 32*t+32) x 32 samples
+// LDS row stride of the group buffers: 136 fp16 (a row's 8 consecutive
+// samples, one MFMA operand, are one aligned 16-byte read), 129 fp32 (the +1
+// spreads banks)
 template <class P>
-TN_DEVICE void put_tile(typename P::T* buf, int t, const f16v& D, int lane) {
-    const int n = lane & 31, h = lane >> 5;
+constexpr int kGW = P::SL == 8 ? kGrp + 8 : kGrp + 1;
+
+// row of accumulator register r in lane half h of a 32-row tile t
+TN_DEVICE int acc_row(int t, int r, int h) { return 32 * t + 8 * (r >> 2) + 4 * h + (r & 3); }
+
+// tile t of a wave's samples (columns 32w..32w+31); HALF: rows 32t..32t+15
+// only (registers 0..7)
+template <class P, bool HALF = false>
+TN_DEVICE void put_grp(typename P::T* buf, int t, int w, const OpTile<P>& A, int lane) {
+    const int n = 32 * w + (lane & 31), h = (lane >> 5) & 1;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int row = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3);
-        buf[row * kTW<P> + n] = (typename P::T)D[r];
-    }
+    for (int r = 0; r < (HALF ? 8 : 16); ++r) buf[acc_row(t, r, h) * kGW<P> + n] = (typename P::T)op_get<P>(A, r);
 }
 
-// write natural-order operands rows [base, base + 32/P::KC*P::KC)
+template <class P>
+TN_DEVICE void put_grp(typename P::T* buf, int t, int w, const f16v& D, int lane) {
+    const int n = 32 * w + (lane & 31), h = (lane >> 5) & 1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) buf[acc_row(t, r, h) * kGW<P> + n] = (typename P::T)D[r];
+}
+
+// natural-order operands x[ks] as rows base + ks*KC + h*SL + j
 template <class P, int NKS>
-TN_DEVICE void put_nat(typename P::T* buf, int base, const typename P::Op (&x)[NKS], int lane) {
-    const int n = lane & 31, h = lane >> 5;
+TN_DEVICE void put_grp_nat(typename P::T* buf, int base, int w, const typename P::Op (&x)[NKS], int lane) {
+    const int n = 32 * w + (lane & 31), h = (lane >> 5) & 1;
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
         for (int j = 0; j < P::SL; ++j) {
             const int row = base + nat_in<P>(ks, h, j);
-            if constexpr (P::SL == 1) buf[row * kTW<P> + n] = x[ks];
-            else buf[row * kTW<P> + n] = x[ks][j];
+            if constexpr (P::SL == 1) buf[row * kGW<P> + n] = x[ks];
+            else buf[row * kGW<P> + n] = x[ks][j];
         }
 }
 
-// operand holding samples [ks*KC + h*SL, +SL) of row 32*t + (lane & 31)
+// operand holding group samples [ks*KC + h*SL, +SL) of row 32*t + (lane & 31)
 template <class P>
-TN_DEVICE typename P::Op get_rows(const typename P::T* buf, int t, int ks, int lane) {
+TN_DEVICE typename P::Op grp_rows(const typename P::T* buf, int t, int ks, int lane) {
     const int row = 32 * t + (lane & 31), h = lane >> 5;
-    return *reinterpret_cast<const typename P::Op*>(buf + row * kTW<P> + ks * P::KC + h * P::SL);
+    return *reinterpret_cast<const typename P::Op*>(buf + row * kGW<P> + ks * P::KC + h * P::SL);
 }
 
-TN_DEVICE void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// C += delta rows (tile mt) x act rows (tile nt)^T over group K-steps [ks0, ks1)
+template <class P>
+TN_DEVICE f16v dw_tile(f16v c, const typename P::T* dbuf, const typename P::T* abuf, int mt, int nt, int ks0,
+                       int ks1, int lane) {
+#pragma unroll 4
+    for (int ks = ks0; ks < ks1; ++ks) c = P::mma(grp_rows<P>(dbuf, mt, ks, lane), grp_rows<P>(abuf, nt, ks, lane), c);
+    return c;
 }
 
-// dW[mt][nt] += delta rows (tile mt) x act rows (tile nt), samples as K
-template <class P, int MT, int NT>
-TN_DEVICE void accumulate_dw(f16v (&dw)[MT][NT], const typename P::T* dbuf, const typename P::T* abuf, int lane) {
+template <class P>
+TN_DEVICE f16v relu_mask(f16v d, const OpTile<P>& act) {
 #pragma unroll
-    for (int ks = 0; ks < 32 / P::KC; ++ks) {
-        typename P::Op b[NT];
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) b[nt] = get_rows<P>(abuf, nt, ks, lane);
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            const typename P::Op a = get_rows<P>(dbuf, mt, ks, lane);
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) dw[mt][nt] = P::mma(a, b[nt], dw[mt][nt]);
-        }
-    }
-}
-
-TN_DEVICE f16v relu_mask(f16v d, const f16v& act) {
-#pragma unroll
-    for (int r = 0; r < 16; ++r) d[r] = act[r] > 0.0f ? d[r] : 0.0f;
+    for (int r = 0; r < 16; ++r) d[r] = op_get<P>(act, r) > 0.0f ? d[r] : 0.0f;
     return d;
-}
-
-// An activation tile kept for the backward pass in the network precision
-// (half the registers of the fp32 accumulator tile in mixed precision).
-template <class P>
-struct ActTile {
-    typename P::T v[16];
-};
-
-template <class P>
-TN_DEVICE ActTile<P> keep(const f16v& D) {
-    ActTile<P> a;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) a.v[r] = (typename P::T)D[r];
-    return a;
-}
-
-template <class P>
-TN_DEVICE f16v relu_mask(f16v d, const ActTile<P>& act) {
-#pragma unroll
-    for (int r = 0; r < 16; ++r) d[r] = act.v[r] > (typename P::T)0.0f ? d[r] : 0.0f;
-    return d;
-}
-
-template <class P>
-TN_DEVICE void put_tile(typename P::T* buf, int t, const ActTile<P>& A, int lane) {
-    const int n = lane & 31, h = lane >> 5;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int row = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3);
-        buf[row * kTW<P> + n] = A.v[r];
-    }
 }
 
 // add tile C (rows 32*mt.., cols 32*nt..) of a [n_out][n_in] gradient into LDS
@@ -451,7 +468,7 @@ TN_DEVICE void add_dw_tile(float* acc, const f16v& C, int mt, int nt, int n_out,
     const int col = 32 * nt + (lane & 31), h = lane >> 5;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        const int row = 32 * mt + 8 * (r >> 2) + 4 * h + (r & 3);
+        const int row = acc_row(mt, r, h);
         if (row < n_out && col < n_in) acc[row * n_in + col] += C[r];
     }
 }
@@ -467,44 +484,38 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     using T = typename P::T;
     using N = Net<P>;
+    constexpr int KS = kGrp / P::KC;  // K-steps of a group
     T* lds = reinterpret_cast<T*>(smem_raw);
     const FwdLds<P> W = stage_forward<P>(lds, params);
     const BwdLds<P> WB = stage_backward<P>(lds + N::kFwdElems, params);
-    const int wid = threadIdx.x >> 6;
-    T* tbuf = lds + N::kFwdElems + N::kBwdElems + wid * (2 * 64 * kTW<P>);
-    T* dbuf = tbuf;               // deltas, up to 64 rows
-    T* abuf = tbuf + 64 * kTW<P>;    // activations, up to 64 rows
+    T* dbuf = lds + N::kFwdElems + N::kBwdElems;  // deltas, up to 64 rows x 128 samples
+    T* abuf = dbuf + 64 * kGW<P>;                 // activations, up to 64 rows x 128 samples
     __syncthreads();
 
     const uint32_t n = info ? min(info[1], n_max) : n_max;
-    const uint32_t n_tiles = (n + 31) / 32;
+    const uint32_t n_grp = (n + kGrp - 1) / kGrp;
     const int lane = (int)__lane_id();
     const int h = lane >> 5;
-    const uint32_t wave = blockIdx.x * kMlpWaves + wid;
-    const uint32_t n_waves = gridDim.x * kMlpWaves;
+    const int wid = threadIdx.x >> 6;
+    // this wave's share of the dW tiles: a four-tile layer's tile (wid >> 1,
+    // wid & 1); a two-tile layer's tile wid & 1 over K half wid >> 1
+    const int t2 = wid & 1, k0 = (wid >> 1) * (KS / 2), k1 = k0 + KS / 2;
+    f16v dW1 = zero16(), dW2 = zero16(), dW3 = zero16(), dW4 = zero16(), dW5 = zero16();
 
-    f16v dW1[2][1], dW2[1][2], dW3[2][1], dW4[2][2], dW5[1][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        dW1[a][0] = zero16();
-        dW3[a][0] = zero16();
-        dW2[0][a] = zero16();
-        dW5[0][a] = zero16();
-        dW4[a][0] = zero16();
-        dW4[a][1] = zero16();
-    }
-
-    for (uint32_t t = wave; t < n_tiles; t += n_waves) {
-        const uint64_t sample = (uint64_t)t * 32 + (lane & 31);
+    for (uint32_t grp = blockIdx.x; grp < n_grp; grp += gridDim.x) {
+        const uint32_t sample = grp * kGrp + 32 * wid + (lane & 31);
         const bool valid = sample < n;
+        const uint32_t sl = valid ? sample : 0u;
         Tile<P> tl;
         load_inputs<P>(tl, valid, sample, h, feat, FS, dir, B, true);
+        // per-sample gradients from the compositing, loaded up front
+        float g_rgb[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g_rgb[c] = (float)dL_drgb[sl + c * GS];
+        const float g_s = loss_scale * dL_ds[sl];
+        const float g_x = dL_ds_extra ? loss_scale * dL_ds_extra[sl] : 0.0f;
+        const float t_s = sample_t[sl];
         forward_tile<P>(tl, W, lane, true);
-        // activations the backward pass needs, in the network precision
-        const ActTile<P> a_h1[2] = {keep<P>(tl.h1[0]), keep<P>(tl.h1[1])};
-        const ActTile<P> a_s = keep<P>(tl.s);
-        const ActTile<P> a_h2[2] = {keep<P>(tl.h2[0]), keep<P>(tl.h2[1])};
-        const ActTile<P> a_h3[2] = {keep<P>(tl.h3[0]), keep<P>(tl.h3[1])};
 
         // dL/do through the sigmoid (rows 0..2 live in lanes 0..31, registers 0..2)
         f16v dO = zero16();
@@ -512,40 +523,43 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float y = sigmoidf(tl.o[c]);
-                dO[c] = (float)dL_drgb[sample + (uint64_t)c * GS] * y * (1.0f - y);
+                dO[c] = g_rgb[c] * y * (1.0f - y);
             }
         }
         // dH3 = relu'(h3) (W5^T dO); dW5 += dO h3^T
-        f16v dH3[2];
-        f16v dOt[1] = {dO};
+        const f16v dOt[1] = {dO};
+        OpTile<P> dH3[2];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) dH3[mt] = relu_mask<P>(gemm_dt<P, 1>(WB.b5, N::B5, mt, dOt, lane, N::B5.n_ks), a_h3[mt]);
-        put_tile<P>(dbuf, 0, dO, lane);
-        put_tile<P>(abuf, 0, a_h3[0], lane);
-        put_tile<P>(abuf, 1, a_h3[1], lane);
-        wave_sync();
-        accumulate_dw<P, 1, 2>(dW5, dbuf, abuf, lane);
-        wave_sync();
+        for (int mt = 0; mt < 2; ++mt)
+            dH3[mt] = to_ops<P>(relu_mask<P>(gemm_dt<P, 1>(WB.b5, N::B5, mt, dOt, lane, N::B5.n_ks), tl.h3[mt]));
+        put_grp<P>(dbuf, 0, wid, dO, lane);
+        put_grp<P>(abuf, 0, wid, tl.h3[0], lane);
+        put_grp<P>(abuf, 1, wid, tl.h3[1], lane);
+        __syncthreads();
+        dW5 = dw_tile<P>(dW5, dbuf, abuf, 0, t2, k0, k1, lane);
+        __syncthreads();
         // dH2 = relu'(h2) (W4^T dH3); dW4 += dH3 h2^T
-        f16v dH2[2];
+        OpTile<P> dH2[2];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) dH2[mt] = relu_mask<P>(gemm_dt<P, 2>(WB.b4, N::B4, mt, dH3, lane, N::B4.n_ks), a_h2[mt]);
-        put_tile<P>(dbuf, 0, dH3[0], lane);
-        put_tile<P>(dbuf, 1, dH3[1], lane);
-        put_tile<P>(abuf, 0, a_h2[0], lane);
-        put_tile<P>(abuf, 1, a_h2[1], lane);
-        wave_sync();
-        accumulate_dw<P, 2, 2>(dW4, dbuf, abuf, lane);
-        wave_sync();
-        // dXc = W3^T dH2 (rows 0..15 = d density outputs); dW3 += dH2 xc^T
-        f16v dXc = gemm_dt<P, 2>(WB.b3, N::B3, 0, dH2, lane, N::B3.n_ks);
-        put_tile<P>(dbuf, 0, dH2[0], lane);
-        put_tile<P>(dbuf, 1, dH2[1], lane);
-        put_tile<P>(abuf, 0, a_s, lane);  // rows 0..15 = density outputs (rows 16..31 overwritten next)
-        put_nat<P, 16 / P::KC>(abuf, 16, tl.sh, lane);
-        wave_sync();
-        accumulate_dw<P, 2, 1>(dW3, dbuf, abuf, lane);
-        wave_sync();
+        for (int mt = 0; mt < 2; ++mt)
+            dH2[mt] = to_ops<P>(relu_mask<P>(gemm_op<P, 2>(WB.b4, N::B4, mt, dH3, lane, N::B4.n_ks), tl.h2[mt]));
+        put_grp<P>(dbuf, 0, wid, dH3[0], lane);
+        put_grp<P>(dbuf, 1, wid, dH3[1], lane);
+        put_grp<P>(abuf, 0, wid, tl.h2[0], lane);
+        put_grp<P>(abuf, 1, wid, tl.h2[1], lane);
+        __syncthreads();
+        dW4 = dw_tile<P>(dW4, dbuf, abuf, wid >> 1, t2, 0, KS, lane);
+        __syncthreads();
+        // dXc = W3^T dH2 (rows 0..15 = d density outputs); dW3 += dH2 xc^T,
+        // xc = [density outputs (rows 0..15) | SH (rows 16..31)]
+        const f16v dXc = gemm_op<P, 2>(WB.b3, N::B3, 0, dH2, lane, N::B3.n_ks);
+        put_grp<P>(dbuf, 0, wid, dH2[0], lane);
+        put_grp<P>(dbuf, 1, wid, dH2[1], lane);
+        put_grp<P, true>(abuf, 0, wid, tl.s, lane);
+        put_grp_nat<P, 16 / P::KC>(abuf, 16, wid, tl.sh, lane);
+        __syncthreads();
+        dW3 = dw_tile<P>(dW3, dbuf, abuf, t2, 0, k0, k1, lane);
+        __syncthreads();
         // density-output gradient: row 0 also receives the loss-scaled
         // compositing (and distortion) gradient, then the whole row is scaled
         // by t^2 where t < 1 (reference nerf-network.cu:734-758)
@@ -553,59 +567,49 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
 #pragma unroll
         for (int r = 0; r < 8; ++r) dS[r] = dXc[r];  // rows < 16: registers 0..7
         if (valid && h == 0) {
-            float g0 = dS[0] + loss_scale * dL_ds[sample];
-            if (dL_ds_extra) g0 += loss_scale * dL_ds_extra[sample];
-            const float tt = sample_t[sample];
-            if (tt < 1.0f) g0 *= tt * tt;
+            float g0 = dS[0] + g_s;
+            if (dL_ds_extra) g0 += g_x;
+            if (t_s < 1.0f) g0 *= t_s * t_s;
             dS[0] = g0;
         }
         // dH1 = relu'(h1) (W2^T dS); dW2 += dS h1^T
-        f16v dSt[1] = {dS};
-        f16v dH1[2];
+        const f16v dSt[1] = {dS};
+        OpTile<P> dH1[2];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) dH1[mt] = relu_mask<P>(gemm_dt<P, 1>(WB.b2, N::B2, mt, dSt, lane, N::B2.n_ks), a_h1[mt]);
-        put_tile<P>(dbuf, 0, dS, lane);
-        put_tile<P>(abuf, 0, a_h1[0], lane);
-        put_tile<P>(abuf, 1, a_h1[1], lane);
-        wave_sync();
-        accumulate_dw<P, 1, 2>(dW2, dbuf, abuf, lane);
-        wave_sync();
+        for (int mt = 0; mt < 2; ++mt)
+            dH1[mt] = to_ops<P>(relu_mask<P>(gemm_dt<P, 1>(WB.b2, N::B2, mt, dSt, lane, N::B2.n_ks), tl.h1[mt]));
+        put_grp<P>(dbuf, 0, wid, dS, lane);
+        put_grp<P>(abuf, 0, wid, tl.h1[0], lane);
+        put_grp<P>(abuf, 1, wid, tl.h1[1], lane);
+        __syncthreads();
+        dW2 = dw_tile<P>(dW2, dbuf, abuf, 0, t2, k0, k1, lane);
+        __syncthreads();
         // dX = W1^T dH1 -> feature gradients; dW1 += dH1 x^T
-        const f16v dX = gemm_dt<P, 2>(WB.b1, N::B1, 0, dH1, lane, N::B1.n_ks);
+        const f16v dX = gemm_op<P, 2>(WB.b1, N::B1, 0, dH1, lane, N::B1.n_ks);
         if (valid) {
-            const int nn = lane & 31;
-            (void)nn;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = 8 * (r >> 2) + 4 * h + (r & 3);
-                dfeat[(uint64_t)row * FS + sample] = (TF)dX[r];
-            }
+            for (int r = 0; r < 16; ++r) dfeat[(uint32_t)acc_row(0, r, h) * FS + sample] = (TF)dX[r];
         }
-        put_tile<P>(dbuf, 0, dH1[0], lane);
-        put_tile<P>(dbuf, 1, dH1[1], lane);
-        put_nat<P, 32 / P::KC>(abuf, 0, tl.x, lane);
-        wave_sync();
-        accumulate_dw<P, 2, 1>(dW1, dbuf, abuf, lane);
-        wave_sync();
+        put_grp<P>(dbuf, 0, wid, dH1[0], lane);
+        put_grp<P>(dbuf, 1, wid, dH1[1], lane);
+        put_grp_nat<P, 32 / P::KC>(abuf, 0, wid, tl.x, lane);
+        __syncthreads();
+        dW1 = dw_tile<P>(dW1, dbuf, abuf, t2, 0, k0, k1, lane);
+        __syncthreads();
     }
 
-    // workgroup reduction of the weight gradients (waves in turn, so the sum
-    // order is fixed), then one partial per workgroup
-    __syncthreads();
+    // workgroup sum of the weight gradients (waves in turn, so the sum order
+    // is fixed), then one partial per workgroup
     float* red = reinterpret_cast<float*>(smem_raw);
     for (int e = threadIdx.x; e < kNetParams; e += blockDim.x) red[e] = 0.0f;
     __syncthreads();
     for (int w = 0; w < kMlpWaves; ++w) {
         if (wid == w) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                add_dw_tile(red + kW1, dW1[a][0], a, 0, 64, 32, lane);
-                add_dw_tile(red + kW2, dW2[0][a], 0, a, 16, 64, lane);
-                add_dw_tile(red + kW3, dW3[a][0], a, 0, 64, 32, lane);
-                add_dw_tile(red + kW4, dW4[a][0], a, 0, 64, 64, lane);
-                add_dw_tile(red + kW4, dW4[a][1], a, 1, 64, 64, lane);
-                add_dw_tile(red + kW5, dW5[0][a], 0, a, 3, 64, lane);
-            }
+            add_dw_tile(red + kW5, dW5, 0, t2, 3, 64, lane);
+            add_dw_tile(red + kW4, dW4, wid >> 1, t2, 64, 64, lane);
+            add_dw_tile(red + kW3, dW3, t2, 0, 64, 32, lane);
+            add_dw_tile(red + kW2, dW2, 0, t2, 16, 64, lane);
+            add_dw_tile(red + kW1, dW1, t2, 0, 64, 32, lane);
         }
         __syncthreads();
     }
@@ -644,7 +648,7 @@ constexpr size_t fwd_lds_bytes() {
 template <class P>
 constexpr size_t bwd_lds_bytes() {
     const size_t w = (size_t)(Net<P>::kFwdElems + Net<P>::kBwdElems) * sizeof(typename P::T);
-    const size_t t = (size_t)kMlpWaves * 2 * 64 * kTW<P> * sizeof(typename P::T);
+    const size_t t = (size_t)2 * 64 * kGW<P> * sizeof(typename P::T);
     const size_t r = (size_t)kNetParams * sizeof(float);
     return (w + t) > r ? (w + t) : r;
 }
@@ -705,6 +709,7 @@ extern "C" int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* inf
                               hipStream_t stream) {
     TN_CHECK_ARG(params && features && density_out && (!color || (dir && rgb_out)));
     TN_CHECK_ARG(n_max <= stride && n_max <= feat_stride && n_max <= out_stride);
+    TN_CHECK_ARG(32ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * out_stride < (1ull << 32));
     if (n_max == 0) return TN_OK;
     const unsigned blocks = mlp_blocks(n_max, kFwdBlocksPerCU);
     if (precision == 1) {
@@ -744,6 +749,7 @@ extern "C" int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* in
                                hipStream_t stream) {
     TN_CHECK_ARG(params && features && dir && sample_t && dL_drgb && dL_ds && dfeat && grad_net && workspace);
     TN_CHECK_ARG(n_max <= stride && n_max <= feat_stride && n_max <= grad_stride);
+    TN_CHECK_ARG(32ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * grad_stride < (1ull << 32));
     if (workspace_bytes < tn_mlp_backward_workspace_bytes(n_max)) return TN_ERR_OUT_OF_MEMORY;
     if (n_max == 0) return TN_OK;
     const unsigned blocks = mlp_blocks(n_max, kBwdBlocksPerCU);
