@@ -245,6 +245,15 @@ TN_DEVICE void hb_corners(const tn_hashgrid_t& g, int l, uint32_t i, uint32_t n,
         g1 = (float)dfeat[(uint64_t)(2 * l + 1) * GB + i];
     }
     const bool active = i < n && (g0 != 0.0f || g1 != 0.0f);
+    if (__ballot(active) == 0) {  // a wave of samples without gradient emits nothing
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            hc.idx[c] = 0;
+            hc.a0[c] = hc.a1[c] = 0.0f;
+            hc.emit[c] = false;
+        }
+        return;
+    }
     HashCorners c8;
     if (active) {
         hash_corners(g, l, pos[i], pos[i + B], pos[i + 2 * (uint64_t)B], c8);
