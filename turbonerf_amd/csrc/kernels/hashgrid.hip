@@ -31,6 +31,7 @@ namespace tn {
 struct HashCorners {
     uint32_t idx[8];
     float w[8];
+    uint32_t gx, gy, gz;  // the cell (its lowest corner)
 };
 
 TN_DEVICE void hash_corners(const tn_hashgrid_t& g, int l, float px, float py, float pz, HashCorners& hc) {
@@ -39,6 +40,9 @@ TN_DEVICE void hash_corners(const tn_hashgrid_t& g, int l, float px, float py, f
     const float fx0 = floorf(x), fy0 = floorf(y), fz0 = floorf(z);
     const float fx = x - fx0, fy = y - fy0, fz = z - fz0;
     const uint32_t gx = (uint32_t)(int32_t)fx0, gy = (uint32_t)(int32_t)fy0, gz = (uint32_t)(int32_t)fz0;
+    hc.gx = gx;
+    hc.gy = gy;
+    hc.gz = gz;
     const uint32_t size = g.level_size[l];
     const uint32_t off = g.level_offset[l];
     const bool dense = g.level_dense[l] != 0;
@@ -168,7 +172,7 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_kernel(uint32_t n_max, const
 // one block per bin (per chunk of at most kHbChunk records of a large bin)
 // sums its records in LDS and adds the sums to the table. A record is one
 // sample's contribution to both features of one corner entry, after runs of
-// consecutive samples on one entry have been merged. Sums in LDS are 64-bit
+// consecutive samples in one cell have been merged. Sums in LDS are 64-bit
 // fixed point (integer LDS atomics run several times faster than float ones
 // here): with every |v| < 2^E in the bin and m records, the scale 2^S with
 // S = 62 - E - ceil(log2(m + 1)) cannot overflow, and the sum is exact up to
@@ -214,25 +218,28 @@ struct HbCorners {
 // rows 0 and 2 into rows 1 and 3 (row_bcast:15) and of row 1 into rows 2 and
 // 3 (row_bcast:31). A lane adds the shifted value only when the source lane
 // has its key; sources outside the row or rows not selected read key 0,
-// which no run has.
+// which no run has. Here for the eight corners' two features under one key.
 template <int CTRL, int ROWS>
-TN_DEVICE void seg_step2(float& a0, float& a1, int key) {
-    const int k = __builtin_amdgcn_update_dpp(0, key, CTRL, ROWS, 0xF, false);
-    const float o0 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a0), CTRL, ROWS, 0xF, false));
-    const float o1 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a1), CTRL, ROWS, 0xF, false));
-    if (k == key) {
-        a0 += o0;
-        a1 += o1;
+TN_DEVICE void seg_step16(float (&a0)[8], float (&a1)[8], int key) {
+    const bool take = __builtin_amdgcn_update_dpp(0, key, CTRL, ROWS, 0xF, false) == key;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const float o0 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a0[c]), CTRL, ROWS, 0xF, false));
+        const float o1 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a1[c]), CTRL, ROWS, 0xF, false));
+        if (take) {
+            a0[c] += o0;
+            a1[c] += o1;
+        }
     }
 }
 
-TN_DEVICE void seg_sum2(float& a0, float& a1, int key) {
-    seg_step2<0x111, 0xF>(a0, a1, key);
-    seg_step2<0x112, 0xF>(a0, a1, key);
-    seg_step2<0x114, 0xF>(a0, a1, key);
-    seg_step2<0x118, 0xF>(a0, a1, key);
-    seg_step2<0x142, 0xA>(a0, a1, key);
-    seg_step2<0x143, 0xC>(a0, a1, key);
+TN_DEVICE void seg_sum16(float (&a0)[8], float (&a1)[8], int key) {
+    seg_step16<0x111, 0xF>(a0, a1, key);
+    seg_step16<0x112, 0xF>(a0, a1, key);
+    seg_step16<0x114, 0xF>(a0, a1, key);
+    seg_step16<0x118, 0xF>(a0, a1, key);
+    seg_step16<0x142, 0xA>(a0, a1, key);
+    seg_step16<0x143, 0xC>(a0, a1, key);
 }
 
 template <typename TG>
@@ -264,24 +271,33 @@ TN_DEVICE void hb_corners(const tn_hashgrid_t& g, int l, uint32_t i, uint32_t n,
             c8.w[c] = 0.0f;
         }
     }
-    const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    // Consecutive samples in one cell share all eight corner entries; runs of
+    // them (keyed by their ordinal, since a cell may recur further on) are
+    // summed into the run's last lane for all corners at once. Inactive lanes
+    // carry a cell no sample has, so they stay runs of their own.
+    const uint32_t cx = active ? c8.gx : 0xFFFFF000u + lane, cy = active ? c8.gy : 0u, cz = active ? c8.gz : 0u;
+    const uint32_t px = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)cx, 0x138, 0xF, 0xF, false);  // wave_shr:1
+    const uint32_t py = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)cy, 0x138, 0xF, 0xF, false);
+    const uint32_t pz = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)cz, 0x138, 0xF, 0xF, false);
+    const bool head = lane == 0 || px != cx || py != cy || pz != cz;
+    const uint64_t hm = __ballot(head);
+    float a0[8], a1[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        // runs are keyed by their ordinal, not by the entry: the same entry
-        // may recur further on (hash collisions)
-        const int ent = active ? (int)c8.idx[c] : -1 - (int)lane;
-        const int prev = __builtin_amdgcn_update_dpp(-1, ent, 0x138, 0xF, 0xF, false);  // wave_shr:1
-        const bool head = lane == 0 || prev != ent;
-        const uint64_t hm = __ballot(head);
-        float a0 = c8.w[c] * g0, a1 = c8.w[c] * g1;
-        bool emit = active;
-        if (hm != ~0ull) {
-            seg_sum2(a0, a1, __builtin_popcountll(hm & le));
-            emit = active && (lane == 63 || ((hm >> (lane + 1)) & 1ull) != 0);
-        }
+        a0[c] = c8.w[c] * g0;
+        a1[c] = c8.w[c] * g1;
+    }
+    bool emit = active;
+    if (hm != ~0ull) {
+        const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+        seg_sum16(a0, a1, __builtin_popcountll(hm & le));
+        emit = active && (lane == 63 || ((hm >> (lane + 1)) & 1ull) != 0);
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
         hc.idx[c] = c8.idx[c];
-        hc.a0[c] = a0;
-        hc.a1[c] = a1;
+        hc.a0[c] = a0[c];
+        hc.a1[c] = a1[c];
         hc.emit[c] = emit;
     }
 }
