@@ -71,6 +71,10 @@ __global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, flo
         const float gv[4] = {g4.x, g4.y, g4.z, g4.w};
         float wv[4] = {w4.x, w4.y, w4.z, w4.w}, mv[4] = {m4.x, m4.y, m4.z, m4.w}, vv[4] = {v4.x, v4.y, v4.z, v4.w};
         uint32_t sv[4] = {s4.x, s4.y, s4.z, s4.w};
+        // the bias correction depends on the step count only; neighbours
+        // mostly share it, so it is recomputed only when the count changes
+        uint32_t c_st = 0;
+        float c = 0.0f;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t i = i4 + k;
@@ -80,7 +84,10 @@ __global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, flo
             const float m1 = beta1 * mv[k] + (1.0f - beta1) * g;
             const float m2 = beta2 * vv[k] + (1.0f - beta2) * (g * g);
             const uint32_t st = sv[k] + 1;
-            const float c = sqrtf(1.0f - powf(beta2, (float)st)) / (1.0f - powf(beta1, (float)st));
+            if (st != c_st) {
+                c = sqrtf(1.0f - powf(beta2, (float)st)) / (1.0f - powf(beta1, (float)st));
+                c_st = st;
+            }
             wv[k] = wv[k] - (lr * c) / (sqrtf(m2) + eps) * m1;
             mv[k] = m1;
             vv[k] = m2;

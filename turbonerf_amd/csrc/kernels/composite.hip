@@ -76,17 +76,36 @@ struct SampleState {
     bool inc;
 };
 
-template <typename R, typename TIn>
-TN_DEVICE SampleState<R> sample_state(bool valid, int ray, uint64_t s, const TIn* __restrict__ density,
-                                      const float* __restrict__ dtb, int& carry_ray, Acc<R>& carry_P,
+// Per-sample inputs of one 64-sample chunk, loaded one chunk ahead of their
+// use. Lanes past the window read the window's last sample (in bounds) and
+// are masked where the values are used.
+struct ChunkIn {
+    float d, dt, cr, cg, cb;
+};
+
+template <typename TIn>
+TN_DEVICE ChunkIn load_chunk(uint32_t c, uint32_t s1, uint32_t lane, const TIn* __restrict__ density,
+                             const float* __restrict__ dtb, const TIn* __restrict__ color, uint32_t CB) {
+    const uint32_t s = c + lane;
+    const uint32_t sl = s < s1 ? s : s1 - 1;
+    ChunkIn x;
+    x.d = ldf(density, sl);
+    x.dt = dtb[sl];
+    x.cr = ldf(color, sl);
+    x.cg = ldf(color, sl + CB);
+    x.cb = ldf(color, sl + 2 * (uint64_t)CB);
+    return x;
+}
+
+template <typename R>
+TN_DEVICE SampleState<R> sample_state(bool valid, int ray, const ChunkIn& in, int& carry_ray, Acc<R>& carry_P,
                                       float min_T, bool raw_density = true) {
     SampleState<R> st;
     st.sigma = 0;
     st.dt = 0;
     if (valid) {
-        const float d = ldf(density, s);
-        st.sigma = raw_density ? r_exp<R>((R)fminf(d, 12.0f) - (R)1) : (R)d;
-        st.dt = (R)dtb[s];
+        st.sigma = raw_density ? r_exp<R>((R)fminf(in.d, 12.0f) - (R)1) : (R)in.d;
+        st.dt = (R)in.dt;
     }
     const R tau = st.sigma * st.dt;
     Acc<R> P = Acc<R>::of(tau).scan(ray);
@@ -99,7 +118,7 @@ TN_DEVICE SampleState<R> sample_state(bool valid, int ray, uint64_t s, const TIn
     st.w = st.T * st.alpha;
     st.inc = valid && st.T >= (R)min_T;
     // carry of the prefix for the ray continuing past lane 63
-    const int ray63 = __shfl(ray, 63, 64);
+    const int ray63 = __builtin_amdgcn_readlane(ray, 63);
     carry_P = P.bcast(63);
     carry_ray = ray63;
     return st;
@@ -134,19 +153,18 @@ composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t P
         int carry_ray = -1;
         Acc<R> carry_P = Acc<R>::zero();
         Acc<R> cr = Acc<R>::zero(), cg = Acc<R>::zero(), cb = Acc<R>::zero(), ca = Acc<R>::zero();
+        ChunkIn in = {};
+        if (wr.s0 < wr.s1) in = load_chunk(wr.s0, wr.s1, lane, density, dtb, color, PB);
         for (uint32_t c = wr.s0; c < wr.s1; c += 64) {
             const uint64_t s = (uint64_t)c + lane;
             const bool valid = s < wr.s1;
+            const ChunkIn cur = in;
+            if (c + 64 < wr.s1) in = load_chunk(c + 64, wr.s1, lane, density, dtb, color, PB);
             const int prev_carry = carry_ray;
             int ray = chunk_rays(c, wr.r1, r_scan, offs, steps, slot, carry_ray, empty_ray);
             if (!valid) ray = -2 - (int)lane;  // unique keys keep padding lanes out of every segment
-            SampleState<R> st = sample_state<R>(valid, ray, s, density, dtb, carry_ray, carry_P, min_T);
-            float cR = 0, cG = 0, cB = 0;
-            if (valid) {
-                cR = ldf(color, s);
-                cG = ldf(color, s + PB);
-                cB = ldf(color, s + 2 * (uint64_t)PB);
-            }
+            SampleState<R> st = sample_state<R>(valid, ray, cur, carry_ray, carry_P, min_T);
+            const float cR = valid ? cur.cr : 0.0f, cG = valid ? cur.cg : 0.0f, cB = valid ? cur.cb : 0.0f;
             const R wv = st.inc ? st.w : (R)0;
             Acc<R> sr = Acc<R>::of(wv * (R)cR).scan(ray), sg = Acc<R>::of(wv * (R)cG).scan(ray);
             Acc<R> sb = Acc<R>::of(wv * (R)cB).scan(ray), sa = Acc<R>::of(wv).scan(ray);
@@ -166,12 +184,12 @@ composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t P
                                 scratch, SB);
             }
             // carry of the running totals for a ray continuing past this chunk
-            const bool tail63 = __shfl((int)tail, 63, 64) != 0;
+            const bool tail63 = __builtin_amdgcn_readlane((int)tail, 63) != 0;
             cr = sr.bcast(63);
             cg = sg.bcast(63);
             cb = sb.bcast(63);
             ca = sa.bcast(63);
-            if (tail63 || !__shfl((int)valid, 63, 64)) carry_ray = -1;
+            if (tail63 || !__builtin_amdgcn_readlane((int)valid, 63)) carry_ray = -1;
         }
         // zero-length rays at the end of the window
         for (uint32_t r = r_scan + lane; r < wr.r1; r += 64) empty_ray(r);
@@ -187,19 +205,23 @@ composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t P
         Acc<R> carry_P = Acc<R>::zero();
         Acc<R> carry_Q = Acc<R>::zero();
         auto no_op = [](uint32_t) {};
+        ChunkIn in = {};
+        if (wr.s0 < wr.s1) in = load_chunk(wr.s0, wr.s1, lane, density, dtb, color, PB);
         for (uint32_t c = wr.s0; c < wr.s1; c += 64) {
             const uint64_t s = (uint64_t)c + lane;
             const bool valid = s < wr.s1;
+            const ChunkIn cur = in;
+            if (c + 64 < wr.s1) in = load_chunk(c + 64, wr.s1, lane, density, dtb, color, PB);
             const int prev_carry = carry_ray;
             int ray = chunk_rays(c, wr.r1, r_scan, offs, steps, slot, carry_ray, no_op);
             if (!valid) ray = -2 - (int)lane;
-            SampleState<R> st = sample_state<R>(valid, ray, s, density, dtb, carry_ray, carry_P, min_T);
+            SampleState<R> st = sample_state<R>(valid, ray, cur, carry_ray, carry_P, min_T);
             R g = 0, wg = 0, A = 0, GR = 0, GG = 0, GB = 0, Wg_tot = 0;
             if (valid) {
                 const uint32_t rr = (uint32_t)ray;
-                const R cR = ldf(color, s);
-                const R cG = ldf(color, s + PB);
-                const R cB = ldf(color, s + 2 * (uint64_t)PB);
+                const R cR = cur.cr;
+                const R cG = cur.cg;
+                const R cB = cur.cb;
                 A = scratch[rr];
                 const R RR = scratch[rr + SB], RG = scratch[rr + 2 * SB], RB = scratch[rr + 3 * SB];
                 GR = scratch[rr + 4 * SB];
@@ -229,8 +251,8 @@ composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t P
             }
             carry_Q = Q.bcast(63);
             const bool tail63 =
-                __shfl((int)(valid && s + 1 == (uint64_t)offs[valid ? ray : 0] + steps[valid ? ray : 0]), 63, 64) != 0;
-            if (tail63 || !__shfl((int)valid, 63, 64)) carry_ray = -1;
+                __builtin_amdgcn_readlane((int)(valid && s + 1 == (uint64_t)offs[valid ? ray : 0] + steps[valid ? ray : 0]), 63) != 0;
+            if (tail63 || !__builtin_amdgcn_readlane((int)valid, 63)) carry_ray = -1;
         }
     }
 }
@@ -265,20 +287,18 @@ composite_forward_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t
     int carry_ray = -1;
     Acc<R> carry_P = Acc<R>::zero();
     Acc<R> cr = Acc<R>::zero(), cg = Acc<R>::zero(), cb = Acc<R>::zero(), ca = Acc<R>::zero();
+    ChunkIn in = {};
+    if (wr.s0 < wr.s1) in = load_chunk(wr.s0, wr.s1, lane, density, dtb, color, B);
     for (uint32_t c = wr.s0; c < wr.s1; c += 64) {
         const uint64_t s = (uint64_t)c + lane;
         const bool valid = s < wr.s1;
+        const ChunkIn cur = in;
+        if (c + 64 < wr.s1) in = load_chunk(c + 64, wr.s1, lane, density, dtb, color, B);
         const int prev_carry = carry_ray;
         int ray = chunk_rays(c, wr.r1, r_scan, offs, steps, slot, carry_ray, empty_ray);
         if (!valid) ray = -2 - (int)lane;
-        SampleState<R> st =
-            sample_state<R>(valid, ray, s, density, dtb, carry_ray, carry_P, min_T, raw_density != 0);
-        float cR = 0, cG = 0, cB = 0;
-        if (valid) {
-            cR = ldf(color, s);
-            cG = ldf(color, s + B);
-            cB = ldf(color, s + 2 * (uint64_t)B);
-        }
+        SampleState<R> st = sample_state<R>(valid, ray, cur, carry_ray, carry_P, min_T, raw_density != 0);
+        const float cR = valid ? cur.cr : 0.0f, cG = valid ? cur.cg : 0.0f, cB = valid ? cur.cb : 0.0f;
         const R wv = st.inc ? st.w : (R)0;
         const R wc = st.alpha > (R)1e-4f ? wv : (R)0;
         Acc<R> sr = Acc<R>::of(wc * (R)cR).scan(ray), sg = Acc<R>::of(wc * (R)cG).scan(ray);
@@ -300,12 +320,12 @@ composite_forward_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t
                 rgba[rr + 3 * OB] = (float)sa.val();
             }
         }
-        const bool tail63 = __shfl((int)tail, 63, 64) != 0;
+        const bool tail63 = __builtin_amdgcn_readlane((int)tail, 63) != 0;
         cr = sr.bcast(63);
         cg = sg.bcast(63);
         cb = sb.bcast(63);
         ca = sa.bcast(63);
-        if (tail63 || !__shfl((int)valid, 63, 64)) carry_ray = -1;
+        if (tail63 || !__builtin_amdgcn_readlane((int)valid, 63)) carry_ray = -1;
     }
     for (uint32_t r = r_scan + lane; r < wr.r1; r += 64) empty_ray(r);
 }

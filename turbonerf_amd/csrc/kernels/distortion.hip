@@ -130,9 +130,9 @@ distortion_kernel(const uint32_t* __restrict__ info, uint32_t S, float lambda, c
                 }
                 cL = L.bcast(63);
             }
-            const bool tail63 = __shfl((int)tail, 63, 64) != 0;
-            carry_ray = __shfl(ray, 63, 64);
-            if (tail63 || !__shfl((int)valid, 63, 64)) carry_ray = -1;
+            const bool tail63 = __builtin_amdgcn_readlane((int)tail, 63) != 0;
+            carry_ray = __builtin_amdgcn_readlane(ray, 63);
+            if (tail63 || !__builtin_amdgcn_readlane((int)valid, 63)) carry_ray = -1;
         }
         if (pass == 0)
             for (uint32_t r = r_scan + lane; r < wr.r1; r += 64) empty_ray(r);

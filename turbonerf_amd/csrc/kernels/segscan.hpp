@@ -12,6 +12,8 @@
 
 #include "common.hpp"
 
+#include <climits>
+
 namespace tn {
 
 // ---------------------------------------------------------------------------
@@ -33,28 +35,61 @@ TN_DEVICE CSum csum_add(CSum a, CSum b) {
 
 TN_DEVICE float csum_val(CSum a) { return a.hi + a.lo; }
 
-// Segmented inclusive scan across the wave, segments keyed by `key`
-// (lanes with equal key are contiguous).
-TN_DEVICE CSum seg_scan(CSum v, int key) {
-    const int lane = (int)__lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const float oh = __shfl_up(v.hi, d, 64);
-        const float ol = __shfl_up(v.lo, d, 64);
-        const int ok = __shfl_up(key, d, 64);
-        if (lane >= d && ok == key) v = csum_add(v, CSum{oh, ol});
-    }
+// Segmented inclusive scan across the wave, segments keyed by `key` (lanes
+// with equal key are contiguous), on DPP lane moves: rows of 16 first
+// (row_shr 1, 2, 4, 8), then the last lane of rows 0 and 2 into rows 1 and 3
+// (row_bcast:15) and the last lane of row 1 into rows 2 and 3
+// (row_bcast:31). A lane combines the moved value only when the source lane
+// has its key; sources outside the row and unselected rows read key
+// INT_MIN, which no segment uses.
+template <int CTRL, int ROWS>
+TN_DEVICE int dpp_i(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+TN_DEVICE float dpp_f(float v) {
+    return __int_as_float(dpp_i<CTRL, ROWS>(0, __float_as_int(v)));
+}
+template <int CTRL, int ROWS>
+TN_DEVICE double dpp_d(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)dpp_i<CTRL, ROWS>(0, (int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)dpp_i<CTRL, ROWS>(0, (int)(uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+template <int CTRL, int ROWS>
+TN_DEVICE void seg_step(CSum& v, int key) {
+    const bool take = dpp_i<CTRL, ROWS>(INT_MIN, key) == key;
+    const float oh = dpp_f<CTRL, ROWS>(v.hi), ol = dpp_f<CTRL, ROWS>(v.lo);
+    if (take) v = csum_add(v, CSum{oh, ol});
+}
+template <int CTRL, int ROWS>
+TN_DEVICE void seg_step(double& v, int key) {
+    const bool take = dpp_i<CTRL, ROWS>(INT_MIN, key) == key;
+    const double o = dpp_d<CTRL, ROWS>(v);
+    if (take) v += o;
+}
+
+template <typename V>
+TN_DEVICE V seg_scan(V v, int key) {
+    seg_step<0x111, 0xF>(v, key);
+    seg_step<0x112, 0xF>(v, key);
+    seg_step<0x114, 0xF>(v, key);
+    seg_step<0x118, 0xF>(v, key);
+    seg_step<0x142, 0xA>(v, key);
+    seg_step<0x143, 0xC>(v, key);
     return v;
 }
 
-TN_DEVICE double seg_scan(double v, int key) {
-    const int lane = (int)__lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const double o = __shfl_up(v, d, 64);
-        const int ok = __shfl_up(key, d, 64);
-        if (lane >= d && ok == key) v += o;
-    }
+// inclusive max-scan of a non-negative-or--1 int across the wave (DPP)
+TN_DEVICE int max_scan(int v) {
+    v = max(v, dpp_i<0x111, 0xF>(-1, v));
+    v = max(v, dpp_i<0x112, 0xF>(-1, v));
+    v = max(v, dpp_i<0x114, 0xF>(-1, v));
+    v = max(v, dpp_i<0x118, 0xF>(-1, v));
+    v = max(v, dpp_i<0x142, 0xA>(-1, v));
+    v = max(v, dpp_i<0x143, 0xC>(-1, v));
     return v;
 }
 
@@ -96,7 +131,8 @@ struct Acc<float> {
     TN_DEVICE Acc scan(int key) const { return Acc{seg_scan(v, key)}; }
     TN_DEVICE Acc scan_rev(int key) const { return Acc{seg_scan_rev(v, key)}; }
     TN_DEVICE Acc bcast(int src) const {
-        return Acc{CSum{__shfl(v.hi, src, 64), __shfl(v.lo, src, 64)}};
+        return Acc{CSum{__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.hi), src)),
+                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.lo), src))}};
     }
 };
 
@@ -109,7 +145,12 @@ struct Acc<double> {
     TN_DEVICE Acc operator+(const Acc& o) const { return Acc{v + o.v}; }
     TN_DEVICE Acc scan(int key) const { return Acc{seg_scan(v, key)}; }
     TN_DEVICE Acc scan_rev(int key) const { return Acc{seg_scan_rev(v, key)}; }
-    TN_DEVICE Acc bcast(int src) const { return Acc{__shfl(v, src, 64)}; }
+    TN_DEVICE Acc bcast(int src) const {
+        const uint64_t b = (uint64_t)__double_as_longlong(v);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, src);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), src);
+        return Acc{__longlong_as_double((long long)(((uint64_t)hi << 32) | lo))};
+    }
 };
 
 template <typename R>
@@ -198,14 +239,9 @@ TN_DEVICE int chunk_rays(uint32_t c, uint32_t r1, uint32_t& r_scan, const uint32
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    int head = slot[lane];
     // propagate the most recent head to the following lanes (max-scan; ray
     // indices increase along the chunk)
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int o = __shfl_up(head, d, 64);
-        if ((int)lane >= d) head = max(head, o);
-    }
+    const int head = max_scan(slot[lane]);
     __builtin_amdgcn_wave_barrier();
     return head < 0 ? carry_ray : head;
 }
