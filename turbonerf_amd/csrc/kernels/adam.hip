@@ -49,6 +49,28 @@ struct LowPrecision4<_Float16> {
     }
 };
 
+// Streaming access to the fp32 master weights, moments and step counters,
+// which the next kernels do not read (non-temporal, so they do not displace
+// the gradient and low-precision weights from the caches).
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t nt_u4 __attribute__((ext_vector_type(4)));
+TN_DEVICE float4 ld_nt(const float* p) {
+    const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+TN_DEVICE uint4 ld_nt(const uint32_t* p) {
+    const nt_u4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_u4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+TN_DEVICE void st_nt(float* p, const float (&a)[4]) {
+    const nt_f4 v = {a[0], a[1], a[2], a[3]};
+    __builtin_nontemporal_store(v, reinterpret_cast<nt_f4*>(p));
+}
+TN_DEVICE void st_nt(uint32_t* p, const uint32_t (&a)[4]) {
+    const nt_u4 v = {a[0], a[1], a[2], a[3]};
+    __builtin_nontemporal_store(v, reinterpret_cast<nt_u4*>(p));
+}
+
 // One thread per 4 consecutive parameters with 16-byte loads and stores of
 // the gradient, weights, moments and step counters (8 bytes for the fp16
 // copy); a group whose gradients are all zero inside the hash table costs one
@@ -64,10 +86,10 @@ __global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, flo
         const float4 g4 = *reinterpret_cast<const float4*>(grad + i4);
         const bool any = g4.x != 0.0f || g4.y != 0.0f || g4.z != 0.0f || g4.w != 0.0f;
         if (!any && i4 + 3 < n_grid) return;  // untouched hash entries: one 16-byte read
-        const float4 w4 = *reinterpret_cast<const float4*>(w + i4);
-        const float4 m4 = *reinterpret_cast<const float4*>(m + i4);
-        const float4 v4 = *reinterpret_cast<const float4*>(v + i4);
-        const uint4 s4 = *reinterpret_cast<const uint4*>(steps + i4);
+        const float4 w4 = ld_nt(w + i4);
+        const float4 m4 = ld_nt(m + i4);
+        const float4 v4 = ld_nt(v + i4);
+        const uint4 s4 = ld_nt(steps + i4);
         const float gv[4] = {g4.x, g4.y, g4.z, g4.w};
         float wv[4] = {w4.x, w4.y, w4.z, w4.w}, mv[4] = {m4.x, m4.y, m4.z, m4.w}, vv[4] = {v4.x, v4.y, v4.z, v4.w};
         uint32_t sv[4] = {s4.x, s4.y, s4.z, s4.w};
@@ -93,10 +115,10 @@ __global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, flo
             vv[k] = m2;
             sv[k] = st;
         }
-        *reinterpret_cast<float4*>(w + i4) = make_float4(wv[0], wv[1], wv[2], wv[3]);
-        *reinterpret_cast<float4*>(m + i4) = make_float4(mv[0], mv[1], mv[2], mv[3]);
-        *reinterpret_cast<float4*>(v + i4) = make_float4(vv[0], vv[1], vv[2], vv[3]);
-        *reinterpret_cast<uint4*>(steps + i4) = make_uint4(sv[0], sv[1], sv[2], sv[3]);
+        st_nt(w + i4, wv);
+        st_nt(m + i4, mv);
+        st_nt(v + i4, vv);
+        st_nt(steps + i4, sv);
         if (w_lp) LowPrecision4<TL>::store(w_lp, i4, wv);
         if (any) *reinterpret_cast<float4*>(grad + i4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     } else {
