@@ -51,17 +51,22 @@ ALGO = {
     "adam": ("hbm", 0, 0, 36),
     "loss_reduce": ("hbm", 0, 16, 0),
 }
-# the kernel each phase launches (rocprofv3 kernel names)
+# the kernels each phase launches (rocprofv3 kernel names)
 KERNEL = {
-    "hashgrid_forward": "hashgrid_fwd_kernel",
-    "hashgrid_backward": "hashgrid_bwd_kernel",
-    "mlp_forward": "mlp_forward_kernel",
-    "mlp_backward": "mlp_backward_kernel",
-    "composite": "composite_train_kernel",
-    "march_count": "march_count_kernel",
-    "march_generate": "march_generate_kernel",
-    "adam": "adam_kernel",
+    "hashgrid_forward": ("hashgrid_fwd_kernel",),
+    "hashgrid_backward": ("hashgrid_bwd_count_kernel", "hashgrid_bwd_reduce_kernel", "hashgrid_bwd_scan_kernel",
+                          "hashgrid_bwd_scatter_kernel", "hashgrid_bwd_accumulate_kernel"),
+    "mlp_forward": ("mlp_forward_kernel",),
+    "mlp_backward": ("mlp_backward_kernel", "reduce_partials_kernel"),
+    "composite": ("composite_train_kernel",),
+    "march_count": ("march_count_kernel",),
+    "march_generate": ("march_generate_kernel", "march_expand_kernel"),
+    "adam": ("adam_kernel",),
 }
+# phases on the network stream: the step's critical path (batch generation
+# runs on its own stream, overlapped with them)
+NETWORK_PHASES = ("hashgrid_forward", "mlp_forward", "distortion", "composite", "mlp_backward", "hashgrid_backward",
+                  "adam", "loss_reduce")
 METRIC = "train samples/sec (2^18/step) mixed vs FP32 at \u22641e-3 grad err; render Mrays/s; %HBM/MFMA peak"
 PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {"fp16": 2500.0, "fp32": 157.3}
@@ -228,11 +233,12 @@ def roofline(res, n_params, dtype):
             ach, peak, unit = units / sec / 1e12, PEAK_TFLOPS[dtype], "TFLOP/s"
         table[ph] = dict(ms=round(v["ms"], 4), bound=bound, achieved=round(ach, 2), unit=unit,
                          frac=round(ach / peak, 4), algorithmic=units)
-    dom = max(table, key=lambda k: table[k]["ms"])
+    dom = max((k for k in table if k in NETWORK_PHASES), key=lambda k: table[k]["ms"])
     d = table[dom]
     peak = PEAK_HBM_GBS if d["bound"] == "hbm" else PEAK_TFLOPS[dtype]
-    traffic = measured_traffic(KERNEL.get(dom), S)
-    return dict(kernel=KERNEL.get(dom, dom), phase=dom, bound=d["bound"], achieved=d["achieved"], peak=peak,
+    kernels = KERNEL.get(dom, (dom,))
+    traffic = measured_traffic(kernels, S)
+    return dict(kernel="+".join(kernels), phase=dom, bound=d["bound"], achieved=d["achieved"], peak=peak,
                 unit=d["unit"], frac=d["frac"], traffic=traffic, avg_launch_ms=d["ms"],
                 algorithmic_per_launch=d["algorithmic"]), table
 
@@ -240,21 +246,25 @@ def roofline(res, n_params, dtype):
 TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "train_step_blob8_800x800_b262144.md")
 
 
-def measured_traffic(kernel, samples_per_step):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (its "HBM B/sample" column), scaled to
-    this run's samples per launch; None when no measurement exists."""
-    if not kernel or not os.path.exists(TRAFFIC_PROFILE):
+def measured_traffic(kernels, samples_per_step):
+    """HBM bytes per launch of the phase's kernels from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (its "HBM B/sample" column),
+    scaled to this run's samples per launch; None when a kernel has no
+    measurement."""
+    if not kernels or not os.path.exists(TRAFFIC_PROFILE):
         return None
+    per = {}
     with open(TRAFFIC_PROFILE) as f:
         for line in f:
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) == 4 and cells[0] == kernel:
+            if len(cells) == 4 and cells[0] in kernels:
                 try:
-                    return int(float(cells[3]) * samples_per_step)
+                    per[cells[0]] = float(cells[3])
                 except ValueError:
-                    return None
-    return None
+                    pass
+    if any(k not in per for k in kernels):
+        return None
+    return int(sum(per.values()) * samples_per_step)
 
 
 def render_rate(tn, synthetic, scene, trainer_tuple, n_frames=5):
@@ -369,10 +379,11 @@ def profile_steps(tn, synthetic, scene, args):
 
 def _short(name):
     base = name.split("(")[0]
-    for key in sorted(set(KERNEL.values()) | {"march_expand_kernel", "reduce_partials_kernel", "scan_tile_sums_kernel",
-                                               "scan_tile_prefix_kernel", "scan_write_select_kernel",
-                                               "uniform_kernel", "init_training_rays_kernel", "estimate_rays_kernel",
-                                               "accumulate_samples_kernel", "reduce_loss_kernel"}):
+    keys = {k for ks in KERNEL.values() for k in ks} | {
+        "scan_tile_sums_kernel", "scan_tile_prefix_kernel", "scan_write_select_kernel", "uniform_kernel",
+        "init_training_rays_kernel", "estimate_rays_kernel", "accumulate_samples_kernel", "reduce_loss_kernel",
+        "write_rgba_kernel", "cast_kernel"}
+    for key in sorted(keys, key=len, reverse=True):
         if key in base:
             return key
     return base[:48]
@@ -427,7 +438,7 @@ def summarize(paths, last=0, sps=0.0):
         head = "| kernel | HBM read MB/launch | HBM write MB/launch |"
         print(head + (" HBM B/sample |" if sps else ""))
         print("|---|---:|---:|" + ("---:|" if sps else ""))
-        for k in sorted(fetch, key=lambda k: -fetch[k])[:12]:
+        for k in sorted(fetch, key=lambda k: -fetch[k]):
             rd, wr = fetch[k] * kf, write.get(k, 0.0) * kw
             row = f"| {k} | {rd / 1e6:.2f} | {wr / 1e6:.2f} |"
             print(row + (f" {(rd + wr) / sps:.0f} |" if sps else ""))
