@@ -151,3 +151,25 @@ def test_binned_accumulates_wide_range_and_nonfinite(cuda):
     assert np.array_equal(np.isinf(g2), np.isinf(ref2)) and np.isinf(g2).sum() > 0
     fin = np.isfinite(ref2)
     _check(np.where(fin, g2, 0.0).astype(np.float64), np.where(fin, ref2, 0.0), "with inf")
+
+
+@pytest.mark.parametrize("log2_T", [15, 21])
+def test_binned_table_sizes(cuda, log2_T):
+    """Tables of 2^15 (most levels dense-sized) and 2^21 entries per level
+    (11 hashed levels of 512 bins: 5,717 bins in all)."""
+    import torch
+
+    from turbonerf_amd import _capi, ops
+
+    rng = np.random.default_rng(14 + log2_T)
+    g = ops.hashgrid_layout(log2_T=log2_T)
+    pos = _rays(rng, 300, 120, 0.002)
+    n = pos.shape[1]
+    assert _capi.query("tn_hashgrid_backward_workspace_bytes", n, g) > 0
+    dfeat = rng.normal(0.0, 0.05, size=(32, n)).astype(np.float32)
+    ref = _oracle_bwd(g, pos, dfeat)
+    got = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
+    ops.hashgrid_backward(g, n, torch.from_numpy(pos).cuda().view(-1), n, torch.from_numpy(dfeat).cuda().view(-1),
+                          got, binned=True)
+    _check(got.cpu().numpy().astype(np.float64), ref, f"T=2^{log2_T}", g, _oracle_bwd(g, pos, np.abs(dfeat)),
+           dfeat.astype(np.float64))

@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_kernel(uint32_t n_max, const
 constexpr int kHbBinLog = 12;
 constexpr uint32_t kHbBinEntries = 1u << kHbBinLog;
 constexpr uint32_t kHbMaxLevelBins = 512;   // bins per level (level size <= 2^21)
-constexpr uint32_t kHbMaxBins = 2048;       // bins in all
+constexpr uint32_t kHbMaxBins = 8192;       // bins in all (16 levels of 2^21 entries)
 constexpr uint32_t kHbChunk = 32768;        // records per accumulating block at most
 constexpr uint32_t kHbBlocksPerLevel = 128; // count/scatter blocks per level at most
 
@@ -387,17 +387,25 @@ __global__ void __launch_bounds__(1024) hashgrid_bwd_reduce_kernel(tn_hashgrid_t
 }
 
 // (2b) one block of 1024: exclusive scans of the bin totals (record starts)
-// and of the chunks per bin (chunk starts)
+// and of the chunks per bin (chunk starts); thread j scans bins
+// [8j, 8j + 8) serially, then the block scans the thread sums
+constexpr uint32_t kHbScanPer = kHbMaxBins / 1024;
 __global__ void __launch_bounds__(1024) hashgrid_bwd_scan_kernel(HbPlan p, const uint32_t* __restrict__ totals,
                                                                  uint32_t* __restrict__ start,
                                                                  uint32_t* __restrict__ chunk_start) {
     __shared__ uint32_t part[2][1024];
-    const uint32_t k0 = 2 * threadIdx.x;
-    const uint32_t t0 = k0 < p.n_bins ? totals[k0] : 0u, t1 = k0 + 1 < p.n_bins ? totals[k0 + 1] : 0u;
-    const uint32_t c0 = k0 < p.n_bins ? max(1u, (t0 + kHbChunk - 1) / kHbChunk) : 0u;
-    const uint32_t c1 = k0 + 1 < p.n_bins ? max(1u, (t1 + kHbChunk - 1) / kHbChunk) : 0u;
-    part[0][threadIdx.x] = t0 + t1;
-    part[1][threadIdx.x] = c0 + c1;
+    const uint32_t k0 = kHbScanPer * threadIdx.x;
+    uint32_t t[kHbScanPer], c[kHbScanPer], st = 0, sc = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kHbScanPer; ++j) {
+        const uint32_t k = k0 + j;
+        t[j] = k < p.n_bins ? totals[k] : 0u;
+        c[j] = k < p.n_bins ? max(1u, (t[j] + kHbChunk - 1) / kHbChunk) : 0u;
+        st += t[j];
+        sc += c[j];
+    }
+    part[0][threadIdx.x] = st;
+    part[1][threadIdx.x] = sc;
     __syncthreads();
     for (uint32_t d = 1; d < 1024; d <<= 1) {
         const uint32_t a = threadIdx.x >= d ? part[0][threadIdx.x - d] : 0u;
@@ -407,14 +415,16 @@ __global__ void __launch_bounds__(1024) hashgrid_bwd_scan_kernel(HbPlan p, const
         part[1][threadIdx.x] += b;
         __syncthreads();
     }
-    const uint32_t ex_t = part[0][threadIdx.x] - (t0 + t1), ex_c = part[1][threadIdx.x] - (c0 + c1);
-    if (k0 < p.n_bins) {
-        start[k0] = ex_t;
-        chunk_start[k0] = ex_c;
-    }
-    if (k0 + 1 < p.n_bins) {
-        start[k0 + 1] = ex_t + t0;
-        chunk_start[k0 + 1] = ex_c + c0;
+    uint32_t et = part[0][threadIdx.x] - st, ec = part[1][threadIdx.x] - sc;
+#pragma unroll
+    for (uint32_t j = 0; j < kHbScanPer; ++j) {
+        const uint32_t k = k0 + j;
+        if (k < p.n_bins) {
+            start[k] = et;
+            chunk_start[k] = ec;
+        }
+        et += t[j];
+        ec += c[j];
     }
     if (threadIdx.x == 1023) {
         start[p.n_bins] = part[0][1023];
