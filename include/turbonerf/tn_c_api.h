@@ -208,6 +208,7 @@ int tn_hashgrid_backward_binned(int dfeat_dtype, uint32_t n_max, const uint32_t*
     uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos, const void* dfeat, float* dtable,
     void* workspace, size_t workspace_bytes, tn_stream_t stream);
 
+
 /* ------------------------------------------------------------------- MLPs */
 
 /* Density MLP (+ SH-4 direction encoding + colour MLP when color != 0) on

@@ -173,3 +173,23 @@ def test_binned_table_sizes(cuda, log2_T):
                           got, binned=True)
     _check(got.cpu().numpy().astype(np.float64), ref, f"T=2^{log2_T}", g, _oracle_bwd(g, pos, np.abs(dfeat)),
            dfeat.astype(np.float64))
+
+
+def test_binned_is_reproducible(cuda):
+    """Two runs over random points (level 0's bin split over 16 blocks) give
+    the same gradient bit for bit."""
+    import torch
+
+    from turbonerf_amd import ops
+
+    rng = np.random.default_rng(16)
+    g = ops.hashgrid_layout()
+    n = 65536
+    pos = torch.from_numpy(rng.uniform(0, 1, size=(3, n)).astype(np.float32)).cuda().view(-1)
+    df = torch.from_numpy(rng.normal(0.0, 1.0, size=(32, n)).astype(np.float32)).cuda().view(-1)
+    runs = []
+    for _ in range(2):
+        got = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
+        ops.hashgrid_backward(g, n, pos, n, df, got, binned=True)
+        runs.append(got.cpu().numpy())
+    assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))

@@ -16,18 +16,15 @@ TN_DEVICE void adam_one(uint32_t i, float g_raw, uint32_t n_grid, uint32_t n_net
                         float* __restrict__ m, float* __restrict__ v, uint32_t* __restrict__ steps) {
     float g = g_raw * inv_loss_scale;
     if (i < n_grid && g == 0.0f) return;
-    const float wf = w[i];
+    float wf = w[i], mf = m[i], vf = v[i];
     if (i >= n_grid && i < n_net_end) g += l2 * wf;
-    const float m1 = beta1 * m[i] + (1.0f - beta1) * g;
-    const float m2 = beta2 * v[i] + (1.0f - beta2) * (g * g);
-    m[i] = m1;
-    v[i] = m2;
     const uint32_t st = steps[i] + 1;
     steps[i] = st;
-    const float c = sqrtf(1.0f - powf(beta2, (float)st)) / (1.0f - powf(beta1, (float)st));
-    const float nw = wf - (lr * c) / (sqrtf(m2) + eps) * m1;
-    w[i] = nw;
-    if (w_lp) w_lp[i] = (TL)nw;
+    adam_apply(g, adam_bias(st, beta1, beta2), lr, beta1, beta2, eps, wf, mf, vf);
+    m[i] = mf;
+    v[i] = vf;
+    w[i] = wf;
+    if (w_lp) w_lp[i] = (TL)wf;
 }
 
 template <typename TL>
@@ -103,16 +100,12 @@ __global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, flo
             float g = gv[k] * inv_loss_scale;
             if (i < n_grid && g == 0.0f) continue;  // skipped entry keeps its state
             if (i >= n_grid && i < n_net_end) g += l2 * wv[k];
-            const float m1 = beta1 * mv[k] + (1.0f - beta1) * g;
-            const float m2 = beta2 * vv[k] + (1.0f - beta2) * (g * g);
             const uint32_t st = sv[k] + 1;
             if (st != c_st) {
-                c = sqrtf(1.0f - powf(beta2, (float)st)) / (1.0f - powf(beta1, (float)st));
+                c = adam_bias(st, beta1, beta2);
                 c_st = st;
             }
-            wv[k] = wv[k] - (lr * c) / (sqrtf(m2) + eps) * m1;
-            mv[k] = m1;
-            vv[k] = m2;
+            adam_apply(g, c, lr, beta1, beta2, eps, wv[k], mv[k], vv[k]);
             sv[k] = st;
         }
         st_nt(w + i4, wv);

@@ -190,4 +190,24 @@ TN_DEVICE T wave_shfl_up(T v, int d) {
 
 TN_DEVICE uint32_t lane_id() { return __lane_id(); }
 
+// ---------------------------------------------------------------------------
+// NGP Adam on one parameter (reference src/core/adam-optimizer.cuh:48-96).
+// ---------------------------------------------------------------------------
+// bias-corrected step-size factor sqrt(1 - beta2^t) / (1 - beta1^t)
+TN_DEVICE float adam_bias(uint32_t t, float beta1, float beta2) {
+    return sqrtf(1.0f - powf(beta2, (float)t)) / (1.0f - powf(beta1, (float)t));
+}
+
+// moments and weight after gradient g with bias factor c (no contraction, so
+// every kernel that inlines it rounds identically)
+TN_DEVICE void adam_apply(float g, float c, float lr, float beta1, float beta2, float eps, float& w, float& m,
+                          float& v) {
+#pragma clang fp contract(off)
+    const float m1 = beta1 * m + (1.0f - beta1) * g;
+    const float m2 = beta2 * v + (1.0f - beta2) * (g * g);
+    w = w - (lr * c) / (sqrtf(m2) + eps) * m1;
+    m = m1;
+    v = m2;
+}
+
 }  // namespace tn

@@ -170,7 +170,9 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_kernel(uint32_t n_max, const
 // offsets and bin totals, and one block scans the totals into bin starts,
 // (3) every block recomputes its records and writes them to their bins, (4)
 // one block per bin (per chunk of at most kHbChunk records of a large bin)
-// sums its records in LDS and adds the sums to the table. A record is one
+// sums its records in LDS and adds the sums to the table, (5) the partial
+// sums of large bins are added up in a fixed order, so the gradient is the
+// same bit for bit from run to run. A record is one
 // sample's contribution to both features of one corner entry, after runs of
 // consecutive samples in one cell have been merged. Sums in LDS are 64-bit
 // fixed point (integer LDS atomics run several times faster than float ones
@@ -195,6 +197,7 @@ struct HbPlan {
     uint32_t row;         // per-block stride of the count/offset matrices (max bins of a level)
     uint32_t blocks;      // count/scatter grid (a multiple of the level count)
     uint32_t acc_blocks;  // accumulate grid: an upper bound on the number of chunks
+    uint32_t split_slots; // partial-sum slots for bins summed by several blocks (upper bound)
     uint64_t cap;         // record capacity: n_max * L * 8
 };
 
@@ -386,49 +389,63 @@ __global__ void __launch_bounds__(1024) hashgrid_bwd_reduce_kernel(tn_hashgrid_t
     for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) binmax[p.base[l] + k] = mx[k];
 }
 
-// (2b) one block of 1024: exclusive scans of the bin totals (record starts)
-// and of the chunks per bin (chunk starts); thread j scans bins
-// [8j, 8j + 8) serially, then the block scans the thread sums
+// (2b) one block of 1024: exclusive scans of the bin totals (record starts),
+// of the chunks per bin (chunk starts) and, for bins summed by several
+// blocks, of their chunks (partial-sum slots) and of the bins themselves (the
+// list the finishing kernel walks; its length goes to split_list[n_bins]).
+// Thread j scans bins [8j, 8j + 8) serially, then the block scans the sums.
 constexpr uint32_t kHbScanPer = kHbMaxBins / 1024;
 __global__ void __launch_bounds__(1024) hashgrid_bwd_scan_kernel(HbPlan p, const uint32_t* __restrict__ totals,
                                                                  uint32_t* __restrict__ start,
-                                                                 uint32_t* __restrict__ chunk_start) {
-    __shared__ uint32_t part[2][1024];
+                                                                 uint32_t* __restrict__ chunk_start,
+                                                                 uint32_t* __restrict__ slot_base,
+                                                                 uint32_t* __restrict__ split_list) {
+    __shared__ uint32_t part[4][1024];
     const uint32_t k0 = kHbScanPer * threadIdx.x;
-    uint32_t t[kHbScanPer], c[kHbScanPer], st = 0, sc = 0;
+    uint32_t t[kHbScanPer], c[kHbScanPer], sum[4] = {0, 0, 0, 0};
 #pragma unroll
     for (uint32_t j = 0; j < kHbScanPer; ++j) {
         const uint32_t k = k0 + j;
         t[j] = k < p.n_bins ? totals[k] : 0u;
         c[j] = k < p.n_bins ? max(1u, (t[j] + kHbChunk - 1) / kHbChunk) : 0u;
-        st += t[j];
-        sc += c[j];
+        sum[0] += t[j];
+        sum[1] += c[j];
+        sum[2] += c[j] > 1 ? c[j] : 0u;
+        sum[3] += c[j] > 1 ? 1u : 0u;
     }
-    part[0][threadIdx.x] = st;
-    part[1][threadIdx.x] = sc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) part[q][threadIdx.x] = sum[q];
     __syncthreads();
     for (uint32_t d = 1; d < 1024; d <<= 1) {
-        const uint32_t a = threadIdx.x >= d ? part[0][threadIdx.x - d] : 0u;
-        const uint32_t b = threadIdx.x >= d ? part[1][threadIdx.x - d] : 0u;
+        uint32_t o[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) o[q] = threadIdx.x >= d ? part[q][threadIdx.x - d] : 0u;
         __syncthreads();
-        part[0][threadIdx.x] += a;
-        part[1][threadIdx.x] += b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[q][threadIdx.x] += o[q];
         __syncthreads();
     }
-    uint32_t et = part[0][threadIdx.x] - st, ec = part[1][threadIdx.x] - sc;
+    uint32_t ex[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ex[q] = part[q][threadIdx.x] - sum[q];
 #pragma unroll
     for (uint32_t j = 0; j < kHbScanPer; ++j) {
         const uint32_t k = k0 + j;
         if (k < p.n_bins) {
-            start[k] = et;
-            chunk_start[k] = ec;
+            start[k] = ex[0];
+            chunk_start[k] = ex[1];
+            slot_base[k] = ex[2];
+            if (c[j] > 1) split_list[ex[3]] = k;
         }
-        et += t[j];
-        ec += c[j];
+        ex[0] += t[j];
+        ex[1] += c[j];
+        ex[2] += c[j] > 1 ? c[j] : 0u;
+        ex[3] += c[j] > 1 ? 1u : 0u;
     }
     if (threadIdx.x == 1023) {
         start[p.n_bins] = part[0][1023];
         chunk_start[p.n_bins] = part[1][1023];
+        split_list[p.n_bins] = part[3][1023];
     }
 }
 
@@ -474,14 +491,47 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_scatter_kernel(uint32_t n_ma
 
 TN_DEVICE long long hb_fixed(float v, int S) { return (long long)rintf(ldexpf(v, S)); }
 
-// (4) one block per chunk of a bin: fixed-point sums in LDS, then dtable +=
-// (atomically when the bin is split over several chunks). A bin holding a
-// non-finite value sums in float instead, so inf and NaN reach the table.
+// the complete sums (v0, v1) of table entry `entry` added to dtable
+TN_DEVICE void hb_finish_entry(float* __restrict__ dtable, uint32_t entry, float v0, float v1) {
+    float2* d = reinterpret_cast<float2*>(dtable) + entry;
+    float2 o = *d;
+    o.x += v0;
+    o.y += v1;
+    *d = o;
+}
+
+// bin k's level, first entry and entry count
+struct HbBin {
+    uint32_t l, local0, ne;
+};
+TN_DEVICE HbBin hb_bin(const tn_hashgrid_t& g, const HbPlan& p, uint32_t k) {
+    HbBin b;
+    b.l = 0;
+    while (b.l + 1 < (uint32_t)g.n_levels && p.base[b.l + 1] <= k) ++b.l;
+    b.local0 = (k - p.base[b.l]) << kHbBinLog;
+    b.ne = min(kHbBinEntries, g.level_size[b.l] - b.local0);
+    return b;
+}
+
+// fixed-point scale of bin k: every |v| < 2^E with m records in the bin
+TN_DEVICE int hb_scale(uint32_t bits, uint64_t m) {
+    const int E = (int)(bits >> 23) - 126;
+    return 62 - E - (32 - __builtin_clz((uint32_t)m + 1u));
+}
+
+// (4) one block per chunk of a bin: fixed-point sums in LDS. A bin summed by
+// one block adds its sums to dtable here; a bin split over several
+// blocks leaves each block's exact 64-bit partial sums in its slot for (5),
+// so the result does not depend on the order the blocks run in. A bin
+// holding a non-finite value sums in float and adds into dtable atomically
+// instead, so inf and NaN reach the table.
 __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgrid_t g, HbPlan p,
                                                                       const uint32_t* __restrict__ start,
                                                                       const uint32_t* __restrict__ chunk_start,
                                                                       const uint32_t* __restrict__ binmax,
+                                                                      const uint32_t* __restrict__ slot_base,
                                                                       const HbRecord* __restrict__ rec,
+                                                                      unsigned long long* __restrict__ slots,
                                                                       float* __restrict__ dtable) {
     __shared__ unsigned long long acc[2 * kHbBinEntries];
     const uint32_t j = blockIdx.x;
@@ -494,18 +544,16 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgri
     }
     const uint32_t k = lo;
     const uint32_t part = j - chunk_start[k], nparts = chunk_start[k + 1] - chunk_start[k];
-    uint32_t l = 0;
-    while (l + 1 < (uint32_t)g.n_levels && p.base[l + 1] <= k) ++l;
-    const uint32_t local0 = (k - p.base[l]) << kHbBinLog;
-    const uint32_t ne = min(kHbBinEntries, g.level_size[l] - local0);
+    const HbBin bin = hb_bin(g, p, k);
+    const uint32_t ne = bin.ne;
     for (uint32_t e = threadIdx.x; e < 2 * ne; e += blockDim.x) acc[e] = 0ull;
     __syncthreads();
-    const uint64_t r0 = (uint64_t)start[k] + (uint64_t)part * kHbChunk;
-    const uint64_t r1 = min(min((uint64_t)start[k + 1], r0 + kHbChunk), p.cap);
+    const uint64_t b0 = start[k], b1 = min((uint64_t)start[k + 1], p.cap);
+    const uint64_t r0 = b0 + (uint64_t)part * kHbChunk;
+    const uint64_t r1 = min(b1, r0 + kHbChunk);
     const uint32_t bits = binmax[k];
     const bool finite = bits < 0x7F800000u;
-    const int E = (int)(bits >> 23) - 126;  // every |v| < 2^E
-    const int S = 62 - E - (32 - __builtin_clz((uint32_t)(r1 - r0) + 1u));
+    const int S = hb_scale(bits, b1 > b0 ? b1 - b0 : 0);
     float* accf = reinterpret_cast<float*>(acc);
     for (uint64_t r = r0 + threadIdx.x; r < r1; r += 4 * blockDim.x) {
         HbRecord q[4];
@@ -527,7 +575,15 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgri
         }
     }
     __syncthreads();
-    float2* dt2 = reinterpret_cast<float2*>(dtable) + g.level_offset[l] + local0;
+    const uint32_t entry0 = g.level_offset[bin.l] + bin.local0;
+    // all parts of a split bin use slots, or none (the bound on slots covers
+    // every batch, so "none" is only a guard)
+    const bool to_slots = nparts > 1 && finite && (uint64_t)slot_base[k] + nparts <= p.split_slots;
+    if (to_slots) {
+        unsigned long long* dst = slots + (uint64_t)(slot_base[k] + part) * 2 * kHbBinEntries;
+        for (uint32_t e = threadIdx.x; e < 2 * ne; e += blockDim.x) dst[e] = acc[e];
+        return;
+    }
     for (uint32_t e = threadIdx.x; e < ne; e += blockDim.x) {
         float v0, v1;
         if (finite) {
@@ -541,14 +597,43 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgri
             if (v0 == 0.0f && v1 == 0.0f) continue;
         }
         if (nparts == 1) {
-            float2 o = dt2[e];
-            o.x += v0;
-            o.y += v1;
-            dt2[e] = o;
-        } else {
-            atomicAdd(&dt2[e].x, v0);
-            atomicAdd(&dt2[e].y, v1);
+            hb_finish_entry(dtable, entry0 + e, v0, v1);
+        } else {  // non-finite split bin: float partials
+            atomicAdd(&dtable[2 * (entry0 + e)], v0);
+            atomicAdd(&dtable[2 * (entry0 + e) + 1], v1);
         }
+    }
+}
+
+// (5) one block per bin split over several blocks: the sum of its partial
+// sums, in slot order (exact in 64 bits), finished like a one-block bin
+__global__ void __launch_bounds__(512) hashgrid_bwd_finish_kernel(tn_hashgrid_t g, HbPlan p,
+                                                                  const uint32_t* __restrict__ start,
+                                                                  const uint32_t* __restrict__ chunk_start,
+                                                                  const uint32_t* __restrict__ binmax,
+                                                                  const uint32_t* __restrict__ slot_base,
+                                                                  const uint32_t* __restrict__ split_list,
+                                                                  const unsigned long long* __restrict__ slots,
+                                                                  float* __restrict__ dtable) {
+    if (blockIdx.x >= split_list[p.n_bins]) return;
+    const uint32_t k = split_list[blockIdx.x];
+    const uint32_t bits = binmax[k];
+    if (bits >= 0x7F800000u) return;  // non-finite: added atomically in (4)
+    const uint32_t nparts = chunk_start[k + 1] - chunk_start[k], s0 = slot_base[k];
+    if ((uint64_t)s0 + nparts > p.split_slots) return;  // beyond the slots: (4) added it atomically
+    const HbBin bin = hb_bin(g, p, k);
+    const uint64_t b0 = start[k], b1 = min((uint64_t)start[k + 1], p.cap);
+    const int S = hb_scale(bits, b1 > b0 ? b1 - b0 : 0);
+    const uint32_t entry0 = g.level_offset[bin.l] + bin.local0;
+    for (uint32_t e = threadIdx.x; e < bin.ne; e += blockDim.x) {
+        long long q0 = 0, q1 = 0;
+        for (uint32_t j = 0; j < nparts; ++j) {
+            const unsigned long long* src = slots + (uint64_t)(s0 + j) * 2 * kHbBinEntries;
+            q0 += (long long)src[2 * e];
+            q1 += (long long)src[2 * e + 1];
+        }
+        if (q0 == 0 && q1 == 0) continue;
+        hb_finish_entry(dtable, entry0 + e, (float)ldexp((double)q0, -S), (float)ldexp((double)q1, -S));
     }
 }
 
@@ -573,11 +658,14 @@ inline bool hb_plan(const tn_hashgrid_t& g, uint32_t n_max, HbPlan& p) {
     p.cap = (uint64_t)n_max * L * 8;
     if (p.cap >= (1ull << 32)) return false;  // record offsets are 32-bit
     p.acc_blocks = (uint32_t)((uint64_t)n_bins + (p.cap + kHbChunk - 1) / kHbChunk);
+    // a bin with t > kHbChunk records has ceil(t / kHbChunk) <= 2 t / kHbChunk chunks
+    p.split_slots = (uint32_t)(2 * ((p.cap + kHbChunk - 1) / kHbChunk) + 2);
     return true;
 }
 
 struct HbWorkspace {
-    uint32_t *counts, *maxes, *offsets, *totals, *start, *chunk_start, *binmax;
+    uint32_t *counts, *maxes, *offsets, *totals, *start, *chunk_start, *binmax, *slot_base, *split_list;
+    unsigned long long* slots;
     HbRecord* rec;
 };
 
@@ -600,6 +688,9 @@ inline size_t hb_workspace(const HbPlan& p, void* base, HbWorkspace* ws) {
     unsigned char* s = take(bins1);
     unsigned char* cs = take(bins1);
     unsigned char* bm = take(bins1);
+    unsigned char* sb = take(bins1);
+    unsigned char* sl = take(bins1 + sizeof(uint32_t));
+    unsigned char* sp = take((size_t)p.split_slots * 2 * kHbBinEntries * sizeof(unsigned long long));
     unsigned char* r = take((size_t)p.cap * sizeof(HbRecord));
     if (ws) {
         ws->counts = reinterpret_cast<uint32_t*>(c);
@@ -609,6 +700,9 @@ inline size_t hb_workspace(const HbPlan& p, void* base, HbWorkspace* ws) {
         ws->start = reinterpret_cast<uint32_t*>(s);
         ws->chunk_start = reinterpret_cast<uint32_t*>(cs);
         ws->binmax = reinterpret_cast<uint32_t*>(bm);
+        ws->slot_base = reinterpret_cast<uint32_t*>(sb);
+        ws->split_list = reinterpret_cast<uint32_t*>(sl);
+        ws->slots = reinterpret_cast<unsigned long long*>(sp);
         ws->rec = reinterpret_cast<HbRecord*>(r);
     }
     return off;
@@ -626,13 +720,17 @@ int hashgrid_backward_binned(uint32_t n_max, const uint32_t* info, uint32_t B, u
     hashgrid_bwd_reduce_kernel<<<g.n_levels, 1024, 0, stream>>>(g, p, ws.counts, ws.maxes, ws.offsets, ws.totals,
                                                                 ws.binmax);
     TN_CHECK_LAUNCH();
-    hashgrid_bwd_scan_kernel<<<1, 1024, 0, stream>>>(p, ws.totals, ws.start, ws.chunk_start);
+    hashgrid_bwd_scan_kernel<<<1, 1024, 0, stream>>>(p, ws.totals, ws.start, ws.chunk_start, ws.slot_base,
+                                                     ws.split_list);
     TN_CHECK_LAUNCH();
     hashgrid_bwd_scatter_kernel<TG><<<p.blocks, 256, 0, stream>>>(n_max, info, B, GB, g, p, pos, dfeat, ws.offsets,
                                                                   ws.start, ws.rec);
     TN_CHECK_LAUNCH();
-    hashgrid_bwd_accumulate_kernel<<<p.acc_blocks, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax, ws.rec,
-                                                                     dtable);
+    hashgrid_bwd_accumulate_kernel<<<p.acc_blocks, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax,
+                                                                     ws.slot_base, ws.rec, ws.slots, dtable);
+    TN_CHECK_LAUNCH();
+    hashgrid_bwd_finish_kernel<<<p.split_slots / 2, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax,
+                                                                      ws.slot_base, ws.split_list, ws.slots, dtable);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }
