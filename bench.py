@@ -55,7 +55,8 @@ ALGO = {
 KERNEL = {
     "hashgrid_forward": ("hashgrid_fwd_kernel",),
     "hashgrid_backward": ("hashgrid_bwd_count_kernel", "hashgrid_bwd_reduce_kernel", "hashgrid_bwd_scan_kernel",
-                          "hashgrid_bwd_scatter_kernel", "hashgrid_bwd_accumulate_kernel"),
+                          "hashgrid_bwd_scatter_kernel", "hashgrid_bwd_accumulate_kernel",
+                          "hashgrid_bwd_finish_kernel"),
     "mlp_forward": ("mlp_forward_kernel",),
     "mlp_backward": ("mlp_backward_kernel", "reduce_partials_kernel"),
     "composite": ("composite_train_kernel",),

@@ -133,22 +133,28 @@ TN_DEVICE float after_run(uint32_t f, float t_lane, float cone, float dt_min, fl
     return t63 + get_dt(t63, cone, dt_min, dt_max);
 }
 
-constexpr int kMarchWaves = 4;  // rays (waves) per workgroup
+constexpr int kMarchWaves = 4;  // waves per workgroup
+// The marching kernels are launched on at most this many workgroups (eight
+// waves per SIMD over 256 CUs) and stride over the live rays, one ray per wave
+// at a time: the ray count lives on the device, and a grid sized for the
+// largest batch would launch a wave per possible ray, nearly all of which exit
+// at once yet occupy the dispatcher beside the network kernels.
+constexpr uint32_t kMarchMaxBlocks = 2048;
 
-__global__ void __launch_bounds__(64 * kMarchWaves)
-march_count_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays, uint32_t B, tn_bbox_t bbox,
-                   tn_occupancy_grid_t grid, float cone, float dt_min, float dt_max, uint32_t max_steps,
-                   const float* __restrict__ dir, const float* __restrict__ ray_tmax, uint8_t* __restrict__ ray_alive,
-                   const float* __restrict__ ori, const float* __restrict__ ray_t, uint32_t* __restrict__ n_steps) {
-    const uint32_t i = blockIdx.x * kMarchWaves + (threadIdx.x >> 6);
-    const uint32_t n_rays = d_n_rays ? min(*d_n_rays, n_rays_max) : n_rays_max;
-    if (i >= n_rays) return;
+inline unsigned march_blocks(uint32_t n_rays_max) {
+    return std::min(blocks_for(n_rays_max, kMarchWaves), (unsigned)kMarchMaxBlocks);
+}
+
+TN_DEVICE void march_count_ray(uint32_t i, uint32_t B, const tn_bbox_t& bbox, const MarchGrid& g, float cone,
+                               float dt_min, float dt_max, uint32_t max_steps, const float* __restrict__ dir,
+                               const float* __restrict__ ray_tmax, uint8_t* __restrict__ ray_alive,
+                               const float* __restrict__ ori, const float* __restrict__ ray_t,
+                               uint32_t* __restrict__ n_steps) {
     const bool lane0 = __lane_id() == 0;
     if (!ray_alive[i]) {
         if (lane0) n_steps[i] = 0;
         return;
     }
-    const MarchGrid g = make_grid(grid);
     const MarchRay r = {ori[i], ori[i + B], ori[i + 2 * B], dir[i], dir[i + B], dir[i + 2 * B]};
     const float idx = 1.0f / r.dx, idy = 1.0f / r.dy, idz = 1.0f / r.dz;
     uint32_t n = 0;
@@ -178,6 +184,18 @@ march_count_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays, u
         if (n == 0) ray_alive[i] = 0;
         n_steps[i] = n;
     }
+}
+
+__global__ void __launch_bounds__(64 * kMarchWaves)
+march_count_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays, uint32_t B, tn_bbox_t bbox,
+                   tn_occupancy_grid_t grid, float cone, float dt_min, float dt_max, uint32_t max_steps,
+                   const float* __restrict__ dir, const float* __restrict__ ray_tmax, uint8_t* __restrict__ ray_alive,
+                   const float* __restrict__ ori, const float* __restrict__ ray_t, uint32_t* __restrict__ n_steps) {
+    const uint32_t n_rays = d_n_rays ? min(*d_n_rays, n_rays_max) : n_rays_max;
+    const uint32_t n_waves = gridDim.x * kMarchWaves;
+    const MarchGrid g = make_grid(grid);
+    for (uint32_t i = blockIdx.x * kMarchWaves + (threadIdx.x >> 6); i < n_rays; i += n_waves)
+        march_count_ray(i, B, bbox, g, cone, dt_min, dt_max, max_steps, dir, ray_tmax, ray_alive, ori, ray_t, n_steps);
 }
 
 // ---------------------------------------------------------------------------
@@ -300,18 +318,12 @@ __global__ void scan_write_select_kernel(uint32_t n, const uint32_t* __restrict_
     }
 }
 
-__global__ void __launch_bounds__(64 * kMarchWaves)
-march_generate_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ info, uint32_t B, tn_bbox_t bbox,
-                      tn_occupancy_grid_t grid, float dt_min, float dt_max, float cone, const float* __restrict__ ori,
-                      const float* __restrict__ dir, const float* __restrict__ ray_t,
-                      const uint32_t* __restrict__ ray_offset, const uint8_t* __restrict__ ray_alive,
-                      const uint32_t* __restrict__ n_ray_steps, uint32_t* __restrict__ out_ray,
-                      float* __restrict__ out_t) {
-    const uint32_t i = blockIdx.x * kMarchWaves + (threadIdx.x >> 6);
-    const uint32_t n_rays = min(info[0], n_rays_max);
-    if (i >= n_rays || !ray_alive[i]) return;
+TN_DEVICE void march_generate_ray(uint32_t i, uint32_t B, const tn_bbox_t& bbox, const MarchGrid& g, float dt_min,
+                                  float dt_max, float cone, const float* __restrict__ ori,
+                                  const float* __restrict__ dir, const float* __restrict__ ray_t,
+                                  const uint32_t* __restrict__ ray_offset, const uint32_t* __restrict__ n_ray_steps,
+                                  uint32_t* __restrict__ out_ray, float* __restrict__ out_t) {
     const uint32_t lane = __lane_id();
-    const MarchGrid g = make_grid(grid);
     const uint32_t n_steps = n_ray_steps[i];
     const uint32_t off = ray_offset[i];
     const MarchRay r = {ori[i], ori[i + B], ori[i + 2 * B], dir[i], dir[i + B], dir[i + 2 * B]};
@@ -355,6 +367,23 @@ march_generate_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ info, ui
         } else {
             t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, r.dx, r.dy, r.dz, idx, idy, idz, dt_min, lvl);
         }
+    }
+}
+
+__global__ void __launch_bounds__(64 * kMarchWaves)
+march_generate_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ info, uint32_t B, tn_bbox_t bbox,
+                      tn_occupancy_grid_t grid, float dt_min, float dt_max, float cone, const float* __restrict__ ori,
+                      const float* __restrict__ dir, const float* __restrict__ ray_t,
+                      const uint32_t* __restrict__ ray_offset, const uint8_t* __restrict__ ray_alive,
+                      const uint32_t* __restrict__ n_ray_steps, uint32_t* __restrict__ out_ray,
+                      float* __restrict__ out_t) {
+    const uint32_t n_rays = min(info[0], n_rays_max);
+    const uint32_t n_waves = gridDim.x * kMarchWaves;
+    const MarchGrid g = make_grid(grid);
+    for (uint32_t i = blockIdx.x * kMarchWaves + (threadIdx.x >> 6); i < n_rays; i += n_waves) {
+        if (ray_alive[i])
+            march_generate_ray(i, B, bbox, g, dt_min, dt_max, cone, ori, dir, ray_t, ray_offset, n_ray_steps,
+                               out_ray, out_t);
     }
 }
 
@@ -440,7 +469,7 @@ extern "C" int tn_march_count(uint32_t n_rays_max, const uint32_t* d_n_rays, uin
     TN_CHECK_ARG(n_rays_max <= stride && grid.n_levels >= 1 && grid.n_levels <= 8);
     TN_CHECK_ARG(grid.resolution >= 2 && grid.resolution <= 1024 && dt_min > 0.0f && dt_max >= dt_min);
     if (n_rays_max == 0) return TN_OK;
-    march_count_kernel<<<blocks_for(n_rays_max, kMarchWaves), 64 * kMarchWaves, 0, stream>>>(
+    march_count_kernel<<<march_blocks(n_rays_max), 64 * kMarchWaves, 0, stream>>>(
         n_rays_max, d_n_rays, stride, bbox, grid, cone_angle, dt_min, dt_max, max_steps, dir, ray_tmax,
         ray_alive, ori, ray_t, n_steps);
     TN_CHECK_LAUNCH();
@@ -481,7 +510,7 @@ extern "C" int tn_march_generate(uint32_t n_rays_max, const uint32_t* info, uint
     TN_CHECK_ARG(n_rays_max <= stride && grid.n_levels >= 1 && grid.n_levels <= 8 && dt_min > 0.0f);
     TN_CHECK_ARG(info != nullptr && sample_stride > 0);
     if (n_rays_max == 0) return TN_OK;
-    march_generate_kernel<<<blocks_for(n_rays_max, kMarchWaves), 64 * kMarchWaves, 0, stream>>>(
+    march_generate_kernel<<<march_blocks(n_rays_max), 64 * kMarchWaves, 0, stream>>>(
         n_rays_max, info, stride, bbox, grid, dt_min, dt_max, cone_angle, ori, dir, ray_t, ray_offset, ray_alive,
         n_ray_steps, out_img_id, out_t);
     march_expand_kernel<<<blocks_for(sample_stride, 256), 256, 0, stream>>>(
