@@ -34,18 +34,17 @@ TN_DEVICE MarchGrid make_grid(const tn_occupancy_grid_t& g) {
 // Marching runs one wave per ray and reproduces the reference's serial
 // recurrence exactly (contraction off, same operation order). The 64 lanes
 // buy two things:
-//  * occupancy lookups: a lookup that misses the wave's cache refills it with
-//    64 parallel loads, lane k taking the cell at k half-cells further along
-//    the ray from the current point (t0). Entries are keyed by (cascade,
-//    Morton index) and hold that cell's own bit, so a prefetched entry can
-//    never change a result; lane 0 holds the exact cell of the point that
-//    missed;
-//  * runs of occupied steps: once the step at t is known to be occupied, lane
-//    j forms t_j, the t after j more occupied steps (the same serial sums),
-//    and tests its own step; the leading lanes whose steps are inside, within
-//    the limits and occupied are taken at once. The first lane that fails is
-//    re-evaluated by the serial step, which handles the box exit, an empty
-//    cell (the DDA skip) or a cache miss exactly as the reference would.
+//  * runs of occupied steps: from a point t inside the box, lane j forms t_j,
+//    the t after j more occupied steps (the same serial sums), and reads the
+//    occupancy bit of its own cell at its own cascade; the leading lanes whose
+//    steps are inside, within the limits and occupied are taken at once, for
+//    one gather per 64 steps. If lane 0 fails, the cell at t is empty;
+//  * empty-space skipping: the serial DDA skip steps look cells up in a wave
+//    cache. A lookup that misses refills it with 64 parallel loads, lane k
+//    taking the cell at k half-cells further along the ray from the point that
+//    missed (t0). Entries are keyed by (cascade, Morton index) and hold that
+//    cell's own bit, so a prefetched entry can never change a result; lane 0
+//    holds the exact cell of the point that missed.
 struct WaveCache {
     uint32_t key = 0xFFFFFFFFu;  // per lane
     uint32_t occ = 0;            // per lane
@@ -82,11 +81,13 @@ struct MarchRay {
 };
 
 // Leading lanes (at least 0, at most 64) whose step, j occupied steps after
-// t, is inside the box, below t_max, within `budget` and found occupied in
-// the cache; t_lane = t_j.
+// t, is inside the box, below t_max, within `budget` and occupied; t_lane =
+// t_j. Each lane still in the running reads its own cell's bit. `empty_stop`
+// tells whether the first lane that failed did so only for an empty cell
+// (the serial loop's next step is then a DDA skip from that lane's t).
 TN_DEVICE uint32_t occupied_run(const MarchGrid& g, const tn_bbox_t& bbox, const MarchRay& r, float t, float t_max,
-                                uint32_t budget, float cone, float dt_min, float dt_max, const WaveCache& wc,
-                                float& t_lane) {
+                                uint32_t budget, float cone, float dt_min, float dt_max, float& t_lane,
+                                bool& empty_stop) {
     const uint32_t lane = __lane_id();
     float tj = t;
     if (dt_min == dt_max) {
@@ -102,28 +103,38 @@ TN_DEVICE uint32_t occupied_run(const MarchGrid& g, const tn_bbox_t& bbox, const
     }
     t_lane = tj;
     const float x = r.ox + tj * r.dx, y = r.oy + tj * r.dy, z = r.oz + tj * r.dz;
-    bool good = lane < budget && tj < t_max && bbox_contains(bbox, x, y, z);
-    const float dtj = get_dt(tj, cone, dt_min, dt_max);
-    const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dtj);
-    good = good && lvl == wc.level;
-    const uint32_t m = voxel_morton_index(g.res_f, g.lim, lvl, x, y, z);
-    // the cache lanes whose points bracket this one along the ray
-    const float kf = fminf(fmaxf((tj - wc.t0) * wc.inv_h, -2.0f), 66.0f);
-    const int k0 = (int)floorf(kf) - 1;
-    bool found = false, occ = false;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int src = min(max(k0 + c, 0), 63);
-        const uint32_t key = (uint32_t)__shfl((int)wc.key, src, 64);
-        const uint32_t o = (uint32_t)__shfl((int)wc.occ, src, 64);
-        if (key == m) {
-            found = true;
-            occ = o != 0u;
-        }
+    const bool within = lane < budget && tj < t_max && bbox_contains(bbox, x, y, z);
+    bool good = within;
+    if (within) {
+        const float dtj = get_dt(tj, cone, dt_min, dt_max);
+        const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dtj);
+        const uint32_t m = voxel_morton_index(g.res_f, g.lim, lvl, x, y, z);
+        good = ((g.bits[m] >> lvl) & 1u) != 0u;
     }
-    good = good && found && occ;
     const uint64_t bad = __ballot(!good);
-    return bad == 0ull ? 64u : (uint32_t)__builtin_ctzll(bad);
+    const uint64_t empty = __ballot(within && !good);
+    const uint32_t f = bad == 0ull ? 64u : (uint32_t)__builtin_ctzll(bad);
+    empty_stop = f < 64u && ((empty >> f) & 1ull) != 0ull;
+    return f;
+}
+
+// DDA skip steps from t, whose cell is empty, until the point is outside the
+// box, at or beyond t_max, or in an occupied cell (the reference's loop takes
+// the same steps, testing each point in turn); returns that point's t.
+TN_DEVICE float skip_empty(const MarchGrid& g, const tn_bbox_t& bbox, const MarchRay& r, float t, float t_max,
+                           float cone, float dt_min, float dt_max, float idx, float idy, float idz, WaveCache& wc) {
+    float x = r.ox + t * r.dx, y = r.oy + t * r.dy, z = r.oz + t * r.dz;
+    int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(t, cone, dt_min, dt_max));
+    for (;;) {
+        t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, r.dx, r.dy, r.dz, idx, idy, idz, dt_min, lvl);
+        if (!(t < t_max)) return t;
+        x = r.ox + t * r.dx;
+        y = r.oy + t * r.dy;
+        z = r.oz + t * r.dz;
+        if (!bbox_contains(bbox, x, y, z)) return t;
+        lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(t, cone, dt_min, dt_max));
+        if (wave_occupied(g, lvl, t, x, y, z, r.dx, r.dy, r.dz, wc)) return t;
+    }
 }
 
 // t after the run of f steps: lane f's t_f, or one more step after lane 63's
@@ -164,21 +175,15 @@ TN_DEVICE void march_count_ray(uint32_t i, uint32_t B, const tn_bbox_t& bbox, co
     while (t < t_max && n < max_steps) {
         const float x = r.ox + t * r.dx, y = r.oy + t * r.dy, z = r.oz + t * r.dz;
         if (!bbox_contains(bbox, x, y, z)) break;
-        const float dt = get_dt(t, cone, dt_min, dt_max);
-        const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
-        if (wave_occupied(g, lvl, t, x, y, z, r.dx, r.dy, r.dz, wc)) {
-            float t_lane;
-            const uint32_t f = occupied_run(g, bbox, r, t, t_max, max_steps - n, cone, dt_min, dt_max, wc, t_lane);
-            if (f == 0u) {  // this step's cell lies outside the run's window
-                t += dt;
-                ++n;
-            } else {
-                t = after_run(f, t_lane, cone, dt_min, dt_max);
-                n += f;
-            }
-        } else {
-            t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, r.dx, r.dy, r.dz, idx, idy, idz, dt_min, lvl);
+        float t_lane;
+        bool empty;
+        const uint32_t f = occupied_run(g, bbox, r, t, t_max, max_steps - n, cone, dt_min, dt_max, t_lane, empty);
+        if (f != 0u) {
+            t = after_run(f, t_lane, cone, dt_min, dt_max);
+            n += f;
         }
+        // f == 0: lane 0 is inside, below t_max and within budget, so its cell is empty
+        if (f == 0u || empty) t = skip_empty(g, bbox, r, t, t_max, cone, dt_min, dt_max, idx, idy, idz, wc);
     }
     if (lane0) {
         if (n == 0) ray_alive[i] = 0;
@@ -333,7 +338,6 @@ TN_DEVICE void march_generate_ray(uint32_t i, uint32_t B, const tn_bbox_t& bbox,
     WaveCache wc;
     while (n < n_steps) {
         const float x = r.ox + t * r.dx, y = r.oy + t * r.dy, z = r.oz + t * r.dz;
-        const float dt = get_dt(t, cone, dt_min, dt_max);
         if (!bbox_contains(bbox, x, y, z)) {
             // the count pass stops at the box boundary, so every counted
             // sample is inside; one outside would be emitted without advancing
@@ -344,29 +348,18 @@ TN_DEVICE void march_generate_ray(uint32_t i, uint32_t B, const tn_bbox_t& bbox,
             ++n;
             continue;
         }
-        const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dt);
-        if (wave_occupied(g, lvl, t, x, y, z, r.dx, r.dy, r.dz, wc)) {
-            float t_lane;
-            const uint32_t f =
-                occupied_run(g, bbox, r, t, INFINITY, n_steps - n, cone, dt_min, dt_max, wc, t_lane);
-            if (f == 0u) {
-                if (lane == 0) {
-                    out_t[off + n] = t;
-                    out_ray[off + n] = i;
-                }
-                t += dt;
-                ++n;
-            } else {
-                if (lane < f) {  // one coalesced row of samples
-                    out_t[off + n + lane] = t_lane;
-                    out_ray[off + n + lane] = i;
-                }
-                t = after_run(f, t_lane, cone, dt_min, dt_max);
-                n += f;
+        float t_lane;
+        bool empty;
+        const uint32_t f = occupied_run(g, bbox, r, t, INFINITY, n_steps - n, cone, dt_min, dt_max, t_lane, empty);
+        if (f != 0u) {
+            if (lane < f) {  // one coalesced row of samples
+                out_t[off + n + lane] = t_lane;
+                out_ray[off + n + lane] = i;
             }
-        } else {
-            t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, r.dx, r.dy, r.dz, idx, idy, idz, dt_min, lvl);
+            t = after_run(f, t_lane, cone, dt_min, dt_max);
+            n += f;
         }
+        if (f == 0u || empty) t = skip_empty(g, bbox, r, t, INFINITY, cone, dt_min, dt_max, idx, idy, idz, wc);
     }
 }
 
