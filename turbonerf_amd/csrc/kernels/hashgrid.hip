@@ -165,23 +165,23 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_kernel(uint32_t n_max, const
 // memory side (none of them hit in L2), and their rate, not bandwidth, bounds
 // the scatter-add above. The binned variant turns the scatter into a counting
 // sort: the table is cut into bins of 4096 entries; (1) every block counts
-// the records it will emit per bin of its level (and the largest |value| per
-// bin), (2) one block per level turns the counts into per-(block, bin)
-// offsets and bin totals, and one block scans the totals into bin starts,
-// (3) every block recomputes its records and writes them to their bins, (4)
-// one block per bin (per chunk of at most kHbChunk records of a large bin)
+// the records it will emit per bin of its level, (2) one block per level
+// turns the counts into per-(block, bin) offsets and bin totals, and one
+// block scans the totals into bin starts, (3) every block recomputes its
+// records, writes them to their bins and notes its largest |value| per bin,
+// (4) one block per bin (per chunk of at most kHbChunk records of a large bin)
 // sums its records in LDS and adds the sums to the table, (5) the partial
 // sums of large bins are added up in a fixed order, so the gradient is the
-// same bit for bit from run to run. A record is one
-// sample's contribution to both features of one corner entry, after runs of
-// consecutive samples in one cell have been merged. Sums in LDS are 64-bit
-// fixed point (integer LDS atomics run several times faster than float ones
-// here): with every |v| < 2^E in the bin and m records, the scale 2^S with
-// S = 62 - E - ceil(log2(m + 1)) cannot overflow, and the sum is exact up to
-// the rounding of each value to a multiple of 2^-S (relative 2^-38 of the
-// bin's largest value or better). Blocks of count and scatter work on one
-// level each: block b serves level b % L and chunks b / L + k * (blocks / L)
-// of 256 samples.
+// same bit for bit from run to run. A record is one sample's contribution to
+// both features of one corner entry, after runs of consecutive samples in one
+// cell have been merged; samples whose feature gradients are both zero emit
+// none. Sums in LDS are 64-bit fixed point (integer LDS atomics run several
+// times faster than float ones here): with every |v| < 2^E in the bin and m
+// records, the scale 2^S with S = 62 - E - ceil(log2(m + 1)) cannot overflow,
+// and the sum is exact up to the rounding of each value to a multiple of 2^-S
+// (relative 2^-38 of the bin's largest value or better). Blocks of count and
+// scatter work on one level each: block b serves level b % L and chunks
+// b / L + k * (blocks / L) of 256 samples.
 // ---------------------------------------------------------------------------
 constexpr int kHbBinLog = 12;
 constexpr uint32_t kHbBinEntries = 1u << kHbBinLog;
@@ -196,6 +196,7 @@ struct HbPlan {
     uint32_t n_bins;
     uint32_t row;         // per-block stride of the count/offset matrices (max bins of a level)
     uint32_t blocks;      // count/scatter grid (a multiple of the level count)
+    uint32_t level_blocks; // count/scatter blocks per level: blocks / L
     uint32_t acc_blocks;  // accumulate grid: an upper bound on the number of chunks
     uint32_t split_slots; // partial-sum slots for bins summed by several blocks (upper bound)
     uint64_t cap;         // record capacity: n_max * L * 8
@@ -213,7 +214,7 @@ struct HbRecord {
 struct HbCorners {
     uint32_t idx[8];
     float a0[8], a1[8];
-    bool emit[8];
+    bool emit;
 };
 
 // DPP segmented inclusive sum over the wave of lanes with equal key (runs are
@@ -245,26 +246,14 @@ TN_DEVICE void seg_sum16(float (&a0)[8], float (&a1)[8], int key) {
     seg_step16<0x143, 0xC>(a0, a1, key);
 }
 
-template <typename TG>
-TN_DEVICE void hb_corners(const tn_hashgrid_t& g, int l, uint32_t i, uint32_t n, uint32_t B, uint32_t GB,
-                          const float* __restrict__ pos, const TG* __restrict__ dfeat, HbCorners& hc) {
+// Corner entries and weights of an active sample i at level l, and the mask
+// of lanes that start a run: consecutive samples in one cell share all eight
+// corner entries, and a run of them (keyed by its ordinal, since a cell may
+// recur further on) emits one record per corner from its last lane. Inactive
+// lanes carry a cell no sample has, so they stay runs of their own.
+TN_DEVICE void hb_cells(const tn_hashgrid_t& g, int l, uint32_t i, bool active, uint32_t B,
+                        const float* __restrict__ pos, HashCorners& c8, uint64_t& heads) {
     const uint32_t lane = __lane_id();
-    float g0 = 0.0f, g1 = 0.0f;
-    if (i < n) {
-        g0 = (float)dfeat[(uint64_t)(2 * l) * GB + i];
-        g1 = (float)dfeat[(uint64_t)(2 * l + 1) * GB + i];
-    }
-    const bool active = i < n && (g0 != 0.0f || g1 != 0.0f);
-    if (__ballot(active) == 0) {  // a wave of samples without gradient emits nothing
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            hc.idx[c] = 0;
-            hc.a0[c] = hc.a1[c] = 0.0f;
-            hc.emit[c] = false;
-        }
-        return;
-    }
-    HashCorners c8;
     if (active) {
         hash_corners(g, l, pos[i], pos[i + B], pos[i + 2 * (uint64_t)B], c8);
     } else {
@@ -274,105 +263,113 @@ TN_DEVICE void hb_corners(const tn_hashgrid_t& g, int l, uint32_t i, uint32_t n,
             c8.w[c] = 0.0f;
         }
     }
-    // Consecutive samples in one cell share all eight corner entries; runs of
-    // them (keyed by their ordinal, since a cell may recur further on) are
-    // summed into the run's last lane for all corners at once. Inactive lanes
-    // carry a cell no sample has, so they stay runs of their own.
     const uint32_t cx = active ? c8.gx : 0xFFFFF000u + lane, cy = active ? c8.gy : 0u, cz = active ? c8.gz : 0u;
     const uint32_t px = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)cx, 0x138, 0xF, 0xF, false);  // wave_shr:1
     const uint32_t py = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)cy, 0x138, 0xF, 0xF, false);
     const uint32_t pz = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)cz, 0x138, 0xF, 0xF, false);
-    const bool head = lane == 0 || px != cx || py != cy || pz != cz;
-    const uint64_t hm = __ballot(head);
+    heads = __ballot(lane == 0 || px != cx || py != cy || pz != cz);
+}
+
+// the feature gradients of sample i at level l (zero past the batch)
+template <typename TG>
+TN_DEVICE void hb_grads(int l, uint32_t i, uint32_t n, uint32_t GB, const TG* __restrict__ dfeat, float& g0,
+                        float& g1) {
+    g0 = g1 = 0.0f;
+    if (i < n) {
+        g0 = (float)dfeat[(uint64_t)(2 * l) * GB + i];
+        g1 = (float)dfeat[(uint64_t)(2 * l + 1) * GB + i];
+    }
+}
+
+// whether this lane ends its run (emits the run's records)
+TN_DEVICE bool hb_emits(bool active, uint64_t heads) {
+    const uint32_t lane = __lane_id();
+    return active && (lane == 63 || ((heads >> (lane + 1)) & 1ull) != 0);
+}
+
+template <typename TG>
+TN_DEVICE void hb_corners(const tn_hashgrid_t& g, int l, uint32_t i, uint32_t n, uint32_t B, uint32_t GB,
+                          const float* __restrict__ pos, const TG* __restrict__ dfeat, HbCorners& hc) {
+    const uint32_t lane = __lane_id();
+    float g0, g1;
+    hb_grads<TG>(l, i, n, GB, dfeat, g0, g1);
+    const bool active = i < n && (g0 != 0.0f || g1 != 0.0f);
+    hc.emit = false;
+    if (__ballot(active) == 0) return;  // a wave of samples without gradient emits nothing
+    HashCorners c8;
+    uint64_t hm;
+    hb_cells(g, l, i, active, B, pos, c8, hm);
     float a0[8], a1[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         a0[c] = c8.w[c] * g0;
         a1[c] = c8.w[c] * g1;
     }
-    bool emit = active;
     if (hm != ~0ull) {
         const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
         seg_sum16(a0, a1, __builtin_popcountll(hm & le));
-        emit = active && (lane == 63 || ((hm >> (lane + 1)) & 1ull) != 0);
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         hc.idx[c] = c8.idx[c];
         hc.a0[c] = a0[c];
         hc.a1[c] = a1[c];
-        hc.emit[c] = emit;
     }
+    hc.emit = hb_emits(active, hm);
 }
 
 TN_DEVICE uint32_t abs_bits(float a, float b) {
     return max(__float_as_uint(fabsf(a)), __float_as_uint(fabsf(b)));
 }
 
-// (1) per-block record counts and largest |value| per bin of the block's level
+// (1) per-block record counts per bin of the block's level (the run
+// structure only: no values are formed here)
 template <typename TG>
 __global__ void __launch_bounds__(256) hashgrid_bwd_count_kernel(uint32_t n_max, const uint32_t* __restrict__ info,
                                                                  uint32_t B, uint32_t GB, tn_hashgrid_t g, HbPlan p,
                                                                  const float* __restrict__ pos,
                                                                  const TG* __restrict__ dfeat,
-                                                                 uint32_t* __restrict__ counts,
-                                                                 uint32_t* __restrict__ maxes) {
-    __shared__ uint32_t hist[kHbMaxLevelBins], bmax[kHbMaxLevelBins];
+                                                                 uint32_t* __restrict__ counts) {
+    __shared__ uint32_t hist[kHbMaxLevelBins];
     const uint32_t L = (uint32_t)g.n_levels;
     const int l = (int)(blockIdx.x % L);
     const uint32_t nb = p.count[l];
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
-        hist[k] = 0;
-        bmax[k] = 0;
-    }
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) hist[k] = 0;
     __syncthreads();
     const uint32_t n = info ? min(info[1], n_max) : n_max;
     const uint32_t chunks = (n + 255) / 256, step = p.blocks / L;
     for (uint32_t c = blockIdx.x / L; c < chunks; c += step) {
-        HbCorners hc;
-        hb_corners<TG>(g, l, c * 256 + threadIdx.x, n, B, GB, pos, dfeat, hc);
+        const uint32_t i = c * 256 + threadIdx.x;
+        float g0, g1;
+        hb_grads<TG>(l, i, n, GB, dfeat, g0, g1);
+        const bool active = i < n && (g0 != 0.0f || g1 != 0.0f);
+        if (__ballot(active) == 0) continue;
+        HashCorners c8;
+        uint64_t hm;
+        hb_cells(g, l, i, active, B, pos, c8, hm);
+        if (hb_emits(active, hm)) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (hc.emit[k]) {
-                const uint32_t b = (hc.idx[k] - g.level_offset[l]) >> kHbBinLog;
-                atomicAdd(&hist[b], 1u);
-                atomicMax(&bmax[b], abs_bits(hc.a0[k], hc.a1[k]));
-            }
+            for (int k = 0; k < 8; ++k) atomicAdd(&hist[(c8.idx[k] - g.level_offset[l]) >> kHbBinLog], 1u);
         }
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
-        counts[(uint64_t)blockIdx.x * p.row + k] = hist[k];
-        maxes[(uint64_t)blockIdx.x * p.row + k] = bmax[k];
-    }
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) counts[(uint64_t)blockIdx.x * p.row + k] = hist[k];
 }
 
-// (2a) one block per level: bin totals, largest values, and each block's
-// record offset within its bins (blocks in order), from the level's count
-// columns staged in LDS
+// (2a) one block per level: bin totals and each block's record offset within
+// its bins (blocks in order), from the level's count columns staged in LDS
 __global__ void __launch_bounds__(1024) hashgrid_bwd_reduce_kernel(tn_hashgrid_t g, HbPlan p,
                                                                    const uint32_t* __restrict__ counts,
-                                                                   const uint32_t* __restrict__ maxes,
                                                                    uint32_t* __restrict__ offsets,
-                                                                   uint32_t* __restrict__ totals,
-                                                                   uint32_t* __restrict__ binmax) {
+                                                                   uint32_t* __restrict__ totals) {
     __shared__ uint32_t cnt[kHbBlocksPerLevel * 64];
-    __shared__ uint32_t mx[kHbMaxLevelBins];
     const uint32_t L = (uint32_t)g.n_levels, l = blockIdx.x, step = p.blocks / L, nb = p.count[l];
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) mx[k] = 0;
-    __syncthreads();
     // columns in slices of 64 bins: cnt[j][c] for block j of the level
     for (uint32_t k0 = 0; k0 < nb; k0 += 64) {
         const uint32_t w = min(64u, nb - k0);
         for (uint32_t e = threadIdx.x; e < step * 64; e += blockDim.x) {
             const uint32_t j = e >> 6, c = e & 63u;
-            uint32_t v = 0;
-            if (c < w) {
-                const uint64_t b = (uint64_t)(l + L * j) * p.row + k0 + c;
-                v = counts[b];
-                atomicMax(&mx[k0 + c], maxes[b]);
-            }
-            cnt[e] = v;
+            cnt[e] = c < w ? counts[(uint64_t)(l + L * j) * p.row + k0 + c] : 0u;
         }
         __syncthreads();
         if (threadIdx.x < w) {
@@ -386,7 +383,6 @@ __global__ void __launch_bounds__(1024) hashgrid_bwd_reduce_kernel(tn_hashgrid_t
         }
         __syncthreads();
     }
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) binmax[p.base[l] + k] = mx[k];
 }
 
 // (2b) one block of 1024: exclusive scans of the bin totals (record starts),
@@ -449,7 +445,9 @@ __global__ void __launch_bounds__(1024) hashgrid_bwd_scan_kernel(HbPlan p, const
     }
 }
 
-// (3) records to their bins, in the same order as the count pass
+// (3) records to their bins, in the same order as the count pass, and the
+// block's largest |value| per bin into column j (the block's index within its
+// level) of that bin's row of binmax [n_bins][blocks / L]
 template <typename TG>
 __global__ void __launch_bounds__(256) hashgrid_bwd_scatter_kernel(uint32_t n_max, const uint32_t* __restrict__ info,
                                                                    uint32_t B, uint32_t GB, tn_hashgrid_t g, HbPlan p,
@@ -457,28 +455,34 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_scatter_kernel(uint32_t n_ma
                                                                    const TG* __restrict__ dfeat,
                                                                    const uint32_t* __restrict__ offsets,
                                                                    const uint32_t* __restrict__ start,
+                                                                   uint32_t* __restrict__ binmax,
                                                                    HbRecord* __restrict__ rec) {
-    __shared__ uint32_t cur[kHbMaxLevelBins];
+    __shared__ uint32_t cur[kHbMaxLevelBins], bmax[kHbMaxLevelBins];
     const uint32_t L = (uint32_t)g.n_levels;
     const int l = (int)(blockIdx.x % L);
     const uint32_t nb = p.count[l];
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x)
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
         cur[k] = start[p.base[l] + k] + offsets[(uint64_t)blockIdx.x * p.row + k];
+        bmax[k] = 0u;
+    }
     __syncthreads();
     const uint32_t n = info ? min(info[1], n_max) : n_max;
     const uint32_t chunks = (n + 255) / 256, step = p.blocks / L;
     for (uint32_t c = blockIdx.x / L; c < chunks; c += step) {
         HbCorners hc;
         hb_corners<TG>(g, l, c * 256 + threadIdx.x, n, B, GB, pos, dfeat, hc);
+        if (!hc.emit) continue;
         uint32_t slot[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const uint32_t local = hc.idx[k] - g.level_offset[l];
-            slot[k] = hc.emit[k] ? atomicAdd(&cur[local >> kHbBinLog], 1u) : ~0u;
+            const uint32_t bin = (hc.idx[k] - g.level_offset[l]) >> kHbBinLog;
+            slot[k] = atomicAdd(&cur[bin], 1u);
+            const uint32_t m = abs_bits(hc.a0[k], hc.a1[k]);
+            if (m != 0u) atomicMax(&bmax[bin], m);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            if (hc.emit[k] && slot[k] < p.cap) {
+            if (slot[k] < p.cap) {
                 HbRecord r;
                 r.e = (hc.idx[k] - g.level_offset[l]) & (kHbBinEntries - 1u);
                 r.v0 = hc.a0[k];
@@ -487,6 +491,22 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_scatter_kernel(uint32_t n_ma
             }
         }
     }
+    __syncthreads();
+    const uint32_t j = blockIdx.x / L;
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x)
+        binmax[(uint64_t)(p.base[l] + k) * p.level_blocks + j] = bmax[k];
+}
+
+// the largest |value| bits of bin k: the maximum over its row of binmax
+TN_DEVICE uint32_t hb_bin_max(const HbPlan& p, const uint32_t* __restrict__ binmax, uint32_t k, uint32_t* smax) {
+    if (threadIdx.x == 0) *smax = 0u;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < p.level_blocks; j += blockDim.x) {
+        const uint32_t v = binmax[(uint64_t)k * p.level_blocks + j];
+        if (v != 0u) atomicMax(smax, v);
+    }
+    __syncthreads();
+    return *smax;
 }
 
 TN_DEVICE long long hb_fixed(float v, int S) { return (long long)rintf(ldexpf(v, S)); }
@@ -534,6 +554,7 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgri
                                                                       unsigned long long* __restrict__ slots,
                                                                       float* __restrict__ dtable) {
     __shared__ unsigned long long acc[2 * kHbBinEntries];
+    __shared__ uint32_t smax;
     const uint32_t j = blockIdx.x;
     if (j >= chunk_start[p.n_bins]) return;
     uint32_t lo = 0, hi = p.n_bins;  // the bin k with chunk_start[k] <= j < chunk_start[k + 1]
@@ -547,11 +568,10 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgri
     const HbBin bin = hb_bin(g, p, k);
     const uint32_t ne = bin.ne;
     for (uint32_t e = threadIdx.x; e < 2 * ne; e += blockDim.x) acc[e] = 0ull;
-    __syncthreads();
+    const uint32_t bits = hb_bin_max(p, binmax, k, &smax);  // its barriers also order the clearing
     const uint64_t b0 = start[k], b1 = min((uint64_t)start[k + 1], p.cap);
     const uint64_t r0 = b0 + (uint64_t)part * kHbChunk;
     const uint64_t r1 = min(b1, r0 + kHbChunk);
-    const uint32_t bits = binmax[k];
     const bool finite = bits < 0x7F800000u;
     const int S = hb_scale(bits, b1 > b0 ? b1 - b0 : 0);
     float* accf = reinterpret_cast<float*>(acc);
@@ -615,9 +635,10 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_finish_kernel(tn_hashgrid_t 
                                                                   const uint32_t* __restrict__ split_list,
                                                                   const unsigned long long* __restrict__ slots,
                                                                   float* __restrict__ dtable) {
+    __shared__ uint32_t smax;
     if (blockIdx.x >= split_list[p.n_bins]) return;
     const uint32_t k = split_list[blockIdx.x];
-    const uint32_t bits = binmax[k];
+    const uint32_t bits = hb_bin_max(p, binmax, k, &smax);
     if (bits >= 0x7F800000u) return;  // non-finite: added atomically in (4)
     const uint32_t nparts = chunk_start[k + 1] - chunk_start[k], s0 = slot_base[k];
     if ((uint64_t)s0 + nparts > p.split_slots) return;  // beyond the slots: (4) added it atomically
@@ -655,6 +676,7 @@ inline bool hb_plan(const tn_hashgrid_t& g, uint32_t n_max, HbPlan& p) {
     p.n_bins = n_bins;
     p.row = row;
     p.blocks = (uint32_t)(per_level * L);
+    p.level_blocks = (uint32_t)per_level;
     p.cap = (uint64_t)n_max * L * 8;
     if (p.cap >= (1ull << 32)) return false;  // record offsets are 32-bit
     p.acc_blocks = (uint32_t)((uint64_t)n_bins + (p.cap + kHbChunk - 1) / kHbChunk);
@@ -664,75 +686,60 @@ inline bool hb_plan(const tn_hashgrid_t& g, uint32_t n_max, HbPlan& p) {
 }
 
 struct HbWorkspace {
-    uint32_t *counts, *maxes, *offsets, *totals, *start, *chunk_start, *binmax, *slot_base, *split_list;
+    uint32_t *counts, *offsets, *binmax, *totals, *start, *chunk_start, *slot_base, *split_list;
     unsigned long long* slots;
     HbRecord* rec;
 };
 
 inline size_t hb_align(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// workspace bytes for a plan, and the carve-up of a workspace pointer
+// workspace bytes for a plan, and its carve-up when `base` is given
 inline size_t hb_workspace(const HbPlan& p, void* base, HbWorkspace* ws) {
     const size_t mat = (size_t)p.blocks * p.row * sizeof(uint32_t);
     const size_t bins1 = ((size_t)p.n_bins + 1) * sizeof(uint32_t);
+    unsigned char* b = static_cast<unsigned char*>(base);
     size_t off = 0;
     auto take = [&](size_t bytes) {
-        const size_t o = off;
+        unsigned char* q = b ? b + off : nullptr;
         off += hb_align(bytes);
-        return reinterpret_cast<unsigned char*>(base) + o;
+        return q;
     };
-    unsigned char* c = take(mat);
-    unsigned char* m = take(mat);
-    unsigned char* o = take(mat);
-    unsigned char* t = take(bins1);
-    unsigned char* s = take(bins1);
-    unsigned char* cs = take(bins1);
-    unsigned char* bm = take(bins1);
-    unsigned char* sb = take(bins1);
-    unsigned char* sl = take(bins1 + sizeof(uint32_t));
-    unsigned char* sp = take((size_t)p.split_slots * 2 * kHbBinEntries * sizeof(unsigned long long));
-    unsigned char* r = take((size_t)p.cap * sizeof(HbRecord));
-    if (ws) {
-        ws->counts = reinterpret_cast<uint32_t*>(c);
-        ws->maxes = reinterpret_cast<uint32_t*>(m);
-        ws->offsets = reinterpret_cast<uint32_t*>(o);
-        ws->totals = reinterpret_cast<uint32_t*>(t);
-        ws->start = reinterpret_cast<uint32_t*>(s);
-        ws->chunk_start = reinterpret_cast<uint32_t*>(cs);
-        ws->binmax = reinterpret_cast<uint32_t*>(bm);
-        ws->slot_base = reinterpret_cast<uint32_t*>(sb);
-        ws->split_list = reinterpret_cast<uint32_t*>(sl);
-        ws->slots = reinterpret_cast<unsigned long long*>(sp);
-        ws->rec = reinterpret_cast<HbRecord*>(r);
-    }
+    HbWorkspace w;
+    w.counts = reinterpret_cast<uint32_t*>(take(mat));
+    w.offsets = reinterpret_cast<uint32_t*>(take(mat));
+    w.binmax = reinterpret_cast<uint32_t*>(take((size_t)p.n_bins * p.level_blocks * sizeof(uint32_t)));
+    w.totals = reinterpret_cast<uint32_t*>(take(bins1));
+    w.start = reinterpret_cast<uint32_t*>(take(bins1));
+    w.chunk_start = reinterpret_cast<uint32_t*>(take(bins1));
+    w.slot_base = reinterpret_cast<uint32_t*>(take(bins1));
+    w.split_list = reinterpret_cast<uint32_t*>(take(bins1 + sizeof(uint32_t)));
+    w.slots = reinterpret_cast<unsigned long long*>(
+        take((size_t)p.split_slots * 2 * kHbBinEntries * sizeof(unsigned long long)));
+    w.rec = reinterpret_cast<HbRecord*>(take((size_t)p.cap * sizeof(HbRecord)));
+    if (ws) *ws = w;
     return off;
 }
 
 template <typename TG>
-int hashgrid_backward_binned(uint32_t n_max, const uint32_t* info, uint32_t B, uint32_t GB, const tn_hashgrid_t& g,
-                             const HbPlan& p, const float* pos, const TG* dfeat, float* dtable, void* workspace,
-                             hipStream_t stream) {
+void hashgrid_backward_binned(uint32_t n_max, const uint32_t* info, uint32_t B, uint32_t GB, const tn_hashgrid_t& g,
+                              const HbPlan& p, const float* pos, const TG* dfeat, float* dtable, void* workspace,
+                              hipStream_t stream) {
     HbWorkspace ws;
     hb_workspace(p, workspace, &ws);
-    hashgrid_bwd_count_kernel<TG><<<p.blocks, 256, 0, stream>>>(n_max, info, B, GB, g, p, pos, dfeat, ws.counts,
-                                                                ws.maxes);
-    TN_CHECK_LAUNCH();
-    hashgrid_bwd_reduce_kernel<<<g.n_levels, 1024, 0, stream>>>(g, p, ws.counts, ws.maxes, ws.offsets, ws.totals,
-                                                                ws.binmax);
-    TN_CHECK_LAUNCH();
+    hashgrid_bwd_count_kernel<TG><<<p.blocks, 256, 0, stream>>>(n_max, info, B, GB, g, p, pos, dfeat, ws.counts);
+    hashgrid_bwd_reduce_kernel<<<g.n_levels, 1024, 0, stream>>>(g, p, ws.counts, ws.offsets, ws.totals);
     hashgrid_bwd_scan_kernel<<<1, 1024, 0, stream>>>(p, ws.totals, ws.start, ws.chunk_start, ws.slot_base,
                                                      ws.split_list);
-    TN_CHECK_LAUNCH();
     hashgrid_bwd_scatter_kernel<TG><<<p.blocks, 256, 0, stream>>>(n_max, info, B, GB, g, p, pos, dfeat, ws.offsets,
-                                                                  ws.start, ws.rec);
-    TN_CHECK_LAUNCH();
+                                                                  ws.start, ws.binmax, ws.rec);
     hashgrid_bwd_accumulate_kernel<<<p.acc_blocks, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax,
                                                                      ws.slot_base, ws.rec, ws.slots, dtable);
-    TN_CHECK_LAUNCH();
     hashgrid_bwd_finish_kernel<<<p.split_slots / 2, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax,
                                                                       ws.slot_base, ws.split_list, ws.slots, dtable);
-    TN_CHECK_LAUNCH();
-    return TN_OK;
+}
+
+inline bool hb_layout_ok(const tn_hashgrid_t& g) {
+    return g.n_levels >= 1 && g.n_levels <= TN_HASHGRID_MAX_LEVELS && g.n_features == 2;
 }
 
 }  // namespace tn
@@ -823,9 +830,8 @@ extern "C" int tn_hashgrid_backward(int dfeat_dtype, uint32_t n_max, const uint3
 // Workspace of tn_hashgrid_backward_binned for batches of up to n_max
 // samples; 0 when the layout exceeds the bin limits.
 extern "C" size_t tn_hashgrid_backward_workspace_bytes(uint32_t n_max, tn_hashgrid_t grid) {
-    if (grid.n_levels < 1 || grid.n_levels > TN_HASHGRID_MAX_LEVELS || grid.n_features != 2 || n_max == 0) return 0;
     HbPlan p;
-    if (!hb_plan(grid, n_max, p)) return 0;
+    if (!hb_layout_ok(grid) || n_max == 0 || !hb_plan(grid, n_max, p)) return 0;
     return hb_workspace(p, nullptr, nullptr);
 }
 
@@ -836,16 +842,19 @@ extern "C" int tn_hashgrid_backward_binned(int dfeat_dtype, uint32_t n_max, cons
                                            uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos,
                                            const void* dfeat, float* dtable, void* workspace, size_t workspace_bytes,
                                            hipStream_t stream) {
-    TN_CHECK_ARG(pos && dfeat && dtable && n_max <= stride && n_max <= dfeat_stride && grid.n_features == 2);
-    TN_CHECK_ARG(grid.n_levels >= 1 && grid.n_levels <= TN_HASHGRID_MAX_LEVELS && (dfeat_dtype == 0 || dfeat_dtype == 1));
+    TN_CHECK_ARG(pos && dfeat && dtable && n_max <= stride && n_max <= dfeat_stride && hb_layout_ok(grid));
+    TN_CHECK_ARG(dfeat_dtype == 0 || dfeat_dtype == 1);
     if (n_max == 0) return TN_OK;
     HbPlan p;
     if (!workspace || !hb_plan(grid, n_max, p) || workspace_bytes < hb_workspace(p, nullptr, nullptr))
         return tn_hashgrid_backward(dfeat_dtype, n_max, info, stride, dfeat_stride, grid, pos, dfeat, dtable, stream);
     TN_CHECK_ARG((reinterpret_cast<uintptr_t>(workspace) & 255) == 0 && (reinterpret_cast<uintptr_t>(dtable) & 7) == 0);
     if (dfeat_dtype == 0)
-        return hashgrid_backward_binned<float>(n_max, info, stride, dfeat_stride, grid, p, pos,
-                                               static_cast<const float*>(dfeat), dtable, workspace, stream);
-    return hashgrid_backward_binned<_Float16>(n_max, info, stride, dfeat_stride, grid, p, pos,
-                                              static_cast<const _Float16*>(dfeat), dtable, workspace, stream);
+        hashgrid_backward_binned<float>(n_max, info, stride, dfeat_stride, grid, p, pos,
+                                        static_cast<const float*>(dfeat), dtable, workspace, stream);
+    else
+        hashgrid_backward_binned<_Float16>(n_max, info, stride, dfeat_stride, grid, p, pos,
+                                           static_cast<const _Float16*>(dfeat), dtable, workspace, stream);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
 }
