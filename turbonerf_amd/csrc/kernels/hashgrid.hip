@@ -356,33 +356,41 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_count_kernel(uint32_t n_max,
     for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) counts[(uint64_t)blockIdx.x * p.row + k] = hist[k];
 }
 
-// (2a) one block per level: bin totals and each block's record offset within
-// its bins (blocks in order), from the level's count columns staged in LDS
+// (2a) one block per (level, slice of 64 bins): bin totals and each block's
+// record offset within its bins (blocks in order). Thread (q, c) sums rows
+// [8q, 8q + 8) of column c (a bin) of the level's count matrix, the 16 row
+// groups of a column are scanned through LDS, and each thread writes the
+// offsets of its rows.
+static_assert(kHbBlocksPerLevel <= 16 * 8, "eight rows per thread");
 __global__ void __launch_bounds__(1024) hashgrid_bwd_reduce_kernel(tn_hashgrid_t g, HbPlan p,
                                                                    const uint32_t* __restrict__ counts,
                                                                    uint32_t* __restrict__ offsets,
                                                                    uint32_t* __restrict__ totals) {
-    __shared__ uint32_t cnt[kHbBlocksPerLevel * 64];
-    const uint32_t L = (uint32_t)g.n_levels, l = blockIdx.x, step = p.blocks / L, nb = p.count[l];
-    // columns in slices of 64 bins: cnt[j][c] for block j of the level
-    for (uint32_t k0 = 0; k0 < nb; k0 += 64) {
-        const uint32_t w = min(64u, nb - k0);
-        for (uint32_t e = threadIdx.x; e < step * 64; e += blockDim.x) {
-            const uint32_t j = e >> 6, c = e & 63u;
-            cnt[e] = c < w ? counts[(uint64_t)(l + L * j) * p.row + k0 + c] : 0u;
-        }
-        __syncthreads();
-        if (threadIdx.x < w) {
-            const uint32_t c = threadIdx.x;
-            uint32_t run = 0;
-            for (uint32_t j = 0; j < step; ++j) {
-                offsets[(uint64_t)(l + L * j) * p.row + k0 + c] = run;
-                run += cnt[j * 64 + c];
-            }
-            totals[p.base[l] + k0 + c] = run;
-        }
-        __syncthreads();
+    __shared__ uint32_t part[16][64];
+    const uint32_t L = (uint32_t)g.n_levels, l = blockIdx.x % L, k0 = (blockIdx.x / L) * 64, nb = p.count[l];
+    if (k0 >= nb) return;
+    const uint32_t c = threadIdx.x & 63u, q = threadIdx.x >> 6;
+    const uint32_t rows = (p.level_blocks + 15) / 16, j0 = q * rows, j1 = min(p.level_blocks, j0 + rows);
+    const bool col = k0 + c < nb;
+    uint32_t v[8], sum = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < 8; ++r) {
+        const uint32_t j = j0 + r;
+        v[r] = col && j < j1 ? counts[(uint64_t)(l + L * j) * p.row + k0 + c] : 0u;
+        sum += v[r];
     }
+    part[q][c] = sum;
+    __syncthreads();
+    uint32_t run = 0;
+    for (uint32_t t = 0; t < q; ++t) run += part[t][c];
+    if (!col) return;
+#pragma unroll
+    for (uint32_t r = 0; r < 8; ++r) {
+        const uint32_t j = j0 + r;
+        if (j < j1) offsets[(uint64_t)(l + L * j) * p.row + k0 + c] = run;
+        run += v[r];
+    }
+    if (q == 15) totals[p.base[l] + k0 + c] = run;
 }
 
 // (2b) one block of 1024: exclusive scans of the bin totals (record starts),
@@ -625,36 +633,51 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgri
     }
 }
 
-// (5) one block per bin split over several blocks: the sum of its partial
-// sums, in slot order (exact in 64 bits), finished like a one-block bin
-__global__ void __launch_bounds__(512) hashgrid_bwd_finish_kernel(tn_hashgrid_t g, HbPlan p,
-                                                                  const uint32_t* __restrict__ start,
-                                                                  const uint32_t* __restrict__ chunk_start,
-                                                                  const uint32_t* __restrict__ binmax,
-                                                                  const uint32_t* __restrict__ slot_base,
-                                                                  const uint32_t* __restrict__ split_list,
-                                                                  const unsigned long long* __restrict__ slots,
-                                                                  float* __restrict__ dtable) {
+// (5) bins split over several blocks: the sum of their partial sums, in slot
+// order (exact in 64 bits), finished like a one-block bin. Work items are
+// (split bin, 512 entries of it); blocks stride over them.
+constexpr uint32_t kHbFinishThreads = 512;
+constexpr uint32_t kHbFinishItems = kHbBinEntries / kHbFinishThreads;  // per bin
+constexpr uint32_t kHbFinishBlocks = 256;
+__global__ void __launch_bounds__(kHbFinishThreads) hashgrid_bwd_finish_kernel(
+    tn_hashgrid_t g, HbPlan p, const uint32_t* __restrict__ start, const uint32_t* __restrict__ chunk_start,
+    const uint32_t* __restrict__ binmax, const uint32_t* __restrict__ slot_base, const uint32_t* __restrict__ split_list,
+    const unsigned long long* __restrict__ slots, float* __restrict__ dtable) {
     __shared__ uint32_t smax;
-    if (blockIdx.x >= split_list[p.n_bins]) return;
-    const uint32_t k = split_list[blockIdx.x];
-    const uint32_t bits = hb_bin_max(p, binmax, k, &smax);
-    if (bits >= 0x7F800000u) return;  // non-finite: added atomically in (4)
-    const uint32_t nparts = chunk_start[k + 1] - chunk_start[k], s0 = slot_base[k];
-    if ((uint64_t)s0 + nparts > p.split_slots) return;  // beyond the slots: (4) added it atomically
-    const HbBin bin = hb_bin(g, p, k);
-    const uint64_t b0 = start[k], b1 = min((uint64_t)start[k + 1], p.cap);
-    const int S = hb_scale(bits, b1 > b0 ? b1 - b0 : 0);
-    const uint32_t entry0 = g.level_offset[bin.l] + bin.local0;
-    for (uint32_t e = threadIdx.x; e < bin.ne; e += blockDim.x) {
+    const uint32_t items = split_list[p.n_bins] * kHbFinishItems;
+    for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
+        const uint32_t k = split_list[w / kHbFinishItems];
+        const uint32_t bits = hb_bin_max(p, binmax, k, &smax);
+        __syncthreads();  // smax is rewritten by the next item
+        if (bits >= 0x7F800000u) continue;  // non-finite: added atomically in (4)
+        const uint32_t nparts = chunk_start[k + 1] - chunk_start[k], s0 = slot_base[k];
+        if ((uint64_t)s0 + nparts > p.split_slots) continue;  // beyond the slots: (4) added it atomically
+        const HbBin bin = hb_bin(g, p, k);
+        const uint32_t e = (w % kHbFinishItems) * kHbFinishThreads + threadIdx.x;
+        if (e >= bin.ne) continue;
+        const uint64_t b0 = start[k], b1 = min((uint64_t)start[k + 1], p.cap);
+        const int S = hb_scale(bits, b1 > b0 ? b1 - b0 : 0);
+        const ulonglong2* src = reinterpret_cast<const ulonglong2*>(slots) + (uint64_t)s0 * kHbBinEntries + e;
         long long q0 = 0, q1 = 0;
-        for (uint32_t j = 0; j < nparts; ++j) {
-            const unsigned long long* src = slots + (uint64_t)(s0 + j) * 2 * kHbBinEntries;
-            q0 += (long long)src[2 * e];
-            q1 += (long long)src[2 * e + 1];
+        uint32_t j = 0;
+        for (; j + 4 <= nparts; j += 4) {  // four loads in flight, added in slot order
+            ulonglong2 t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) t[u] = src[(uint64_t)(j + u) * kHbBinEntries];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                q0 += (long long)t[u].x;
+                q1 += (long long)t[u].y;
+            }
+        }
+        for (; j < nparts; ++j) {
+            const ulonglong2 t = src[(uint64_t)j * kHbBinEntries];
+            q0 += (long long)t.x;
+            q1 += (long long)t.y;
         }
         if (q0 == 0 && q1 == 0) continue;
-        hb_finish_entry(dtable, entry0 + e, (float)ldexp((double)q0, -S), (float)ldexp((double)q1, -S));
+        hb_finish_entry(dtable, g.level_offset[bin.l] + bin.local0 + e, (float)ldexp((double)q0, -S),
+                        (float)ldexp((double)q1, -S));
     }
 }
 
@@ -727,15 +750,16 @@ void hashgrid_backward_binned(uint32_t n_max, const uint32_t* info, uint32_t B, 
     HbWorkspace ws;
     hb_workspace(p, workspace, &ws);
     hashgrid_bwd_count_kernel<TG><<<p.blocks, 256, 0, stream>>>(n_max, info, B, GB, g, p, pos, dfeat, ws.counts);
-    hashgrid_bwd_reduce_kernel<<<g.n_levels, 1024, 0, stream>>>(g, p, ws.counts, ws.offsets, ws.totals);
+    hashgrid_bwd_reduce_kernel<<<g.n_levels * ((p.row + 63) / 64), 1024, 0, stream>>>(g, p, ws.counts, ws.offsets,
+                                                                                     ws.totals);
     hashgrid_bwd_scan_kernel<<<1, 1024, 0, stream>>>(p, ws.totals, ws.start, ws.chunk_start, ws.slot_base,
                                                      ws.split_list);
     hashgrid_bwd_scatter_kernel<TG><<<p.blocks, 256, 0, stream>>>(n_max, info, B, GB, g, p, pos, dfeat, ws.offsets,
                                                                   ws.start, ws.binmax, ws.rec);
     hashgrid_bwd_accumulate_kernel<<<p.acc_blocks, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax,
                                                                      ws.slot_base, ws.rec, ws.slots, dtable);
-    hashgrid_bwd_finish_kernel<<<p.split_slots / 2, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax,
-                                                                      ws.slot_base, ws.split_list, ws.slots, dtable);
+    hashgrid_bwd_finish_kernel<<<kHbFinishBlocks, kHbFinishThreads, 0, stream>>>(
+        g, p, ws.start, ws.chunk_start, ws.binmax, ws.slot_base, ws.split_list, ws.slots, dtable);
 }
 
 inline bool hb_layout_ok(const tn_hashgrid_t& g) {
