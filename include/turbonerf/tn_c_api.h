@@ -110,8 +110,10 @@ int tn_estimate_rays(const uint32_t* info, uint32_t max_samples, uint32_t cap, u
  * accumulation (oracle mode). density = raw network output s. Per-ray
  * buffers (bg_rgb[3], target_rgba[4], loss[4]) have stride `stride`; per-sample
  * buffers (density, color[3], dt, dL_ddensity, dL_dcolor[3]) have stride
- * `sample_stride`. dL_ddensity = dL/ds, dL_dcolor scaled by grad_scale. */
-size_t tn_composite_scratch_bytes(uint32_t n_rays_max);
+ * `sample_stride`. dL_ddensity = dL/ds, dL_dcolor scaled by grad_scale.
+ * ray_scratch (256-byte aligned, tn_composite_scratch_bytes for the ray and
+ * sample capacities) holds per-ray totals and per-64-sample-chunk sums. */
+size_t tn_composite_scratch_bytes(uint32_t n_rays_max, uint32_t sample_stride);
 int tn_composite_train(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t stride,
     uint32_t sample_stride, float grad_scale, float min_transmittance, const uint32_t* ray_steps,
     const uint32_t* ray_offset, const float* bg_rgb, const float* target_rgba,

@@ -124,3 +124,23 @@ def test_composite_fd(cuda):
     es, ec = norm_rel(an_s, fd_s), norm_rel(an_c, fd_c)
     print(f"[composite fd] rel err dL/ds={es:.2e} dL/dc={ec:.2e}", flush=True)
     assert es < 1e-3 and ec < 1e-3
+
+
+@pytest.mark.parametrize("dt_scale", [0.5 / 400, 0.015])
+def test_composite_long_rays(cuda, dt_scale):
+    """Rays of about 400 samples (up to 1024), each spanning many 64-sample
+    chunks: with little optical depth (no ray stops) and with transmittance
+    falling below 1e-4 a few hundred samples in, inside a chunk the ray
+    entered from an earlier one."""
+    import torch
+
+    b = make_batch(seed=21, n_rays=24, mean=400, dt_scale=dt_scale)
+    l_ref, d_ref, c_ref = oracle_f64(b)
+    l, d, c = run_gpu(b, torch.float32)
+    np.testing.assert_allclose(l, l_ref, rtol=1e-5, atol=1e-7)
+    ed, ec = norm_rel(d, d_ref), norm_rel(c, c_ref)
+    print(f"[composite long dt={dt_scale:.4g}] rel err dL/ds={ed:.2e} dL/dc={ec:.2e}", flush=True)
+    assert ed < 1e-4 and ec < 1e-4
+    l, d, c = run_gpu(b, torch.float32, fp64_accumulate=True)
+    np.testing.assert_allclose(l, l_ref, rtol=1e-6, atol=1e-9)
+    assert norm_rel(d, d_ref) < 1e-6 and norm_rel(c, c_ref) < 1e-6

@@ -105,7 +105,7 @@ _SIGS: dict[str, tuple] = {
                                     vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "tn_estimate_rays": (C.c_int, [vp, u32, u32, u32, vp, vp]),
     "tn_accumulate_samples": (C.c_int, [vp, vp, vp]),
-    "tn_composite_scratch_bytes": (sz, [u32]),
+    "tn_composite_scratch_bytes": (sz, [u32, u32]),
     "tn_composite_train": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, f32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                      vp, sz, vp]),
     "tn_composite_forward": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, C.c_int, vp, vp, vp, vp, vp, vp, vp]),

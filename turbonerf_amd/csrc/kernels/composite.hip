@@ -2,20 +2,32 @@
 // loss and its backward pass (reference src/utils/training-kernels-fused.cuh:
 // 227-498, src/core/nerf-network.cu:547-603).
 //
-// Sample-parallel: each wave owns a window of rays (segscan.hpp) and makes two
-// passes over their samples. Pass 1 forms transmittance from a segmented
-// prefix sum of sigma*dt (T = exp(-sum)), the per-ray totals R = sum w c and
-// A = sum w, the loss and dL/dRc, and parks the per-ray totals in `ray_scratch`.
-// Pass 2 recomputes transmittance (nothing per-sample is stored between the
-// passes) and forms, for every sample i of a ray,
-//     g_i       = sum_ch G_ch (c_i,ch A + R_ch - bg_ch)
-//     dL/dsig_i = dt_i (T_{i+1} g_i - sum_{k>i} w_k g_k)
-//     dL/dc_i   = scale G w_i A
-// with the suffix sum taken as the closed-form total A sum_ch G_ch(2R_ch-bg_ch)
-// minus the compensated inclusive prefix. This is the reference's adjoint
-// (its running "cumsum" subtraction) written for a lane-parallel scan.
-// Samples after the early-termination point (T < min_T) get zero gradient,
-// as in the reference.
+// Training compositing runs in three launches, none of which walks a long ray
+// serially. The batch is cut into chunks of 64 samples, one per wave:
+//  (1) per chunk, segmented scans keyed by ray form each sample's prefix of
+//      sigma*dt from its segment's start, and the segment sums of w c and w
+//      with transmittance from that start. Rays that start and end in the
+//      chunk are complete: their loss and dL/dRc are written here. For the
+//      segments of lane 0 and lane 63 (rays entering or leaving the chunk)
+//      the sums go to a per-chunk record;
+//  (2) per ray crossing chunks, its records are chained: the sigma*dt prefix
+//      at each chunk start, the totals R = sum w c and A = sum w (a segment's
+//      sums scaled by the transmittance at its start), the loss and dL/dRc,
+//      and the prefix of w g at each chunk start (from the same segment sums,
+//      since g is linear in c). If transmittance falls below min_T inside a
+//      segment, that segment is summed sample by sample up to the stop and
+//      later segments count nothing, as in the reference's loop;
+//  (3) per chunk, the prefixes are recomputed with the chained carry-ins and
+//      give, for every sample i of a ray,
+//        g_i       = sum_ch G_ch (c_i,ch A + R_ch - bg_ch)
+//        dL/dsig_i = dt_i (T_{i+1} g_i - sum_{k>i} w_k g_k)
+//        dL/dc_i   = scale G w_i A
+//      with the suffix sum taken as the closed-form total A sum_ch
+//      G_ch(2R_ch-bg_ch) minus the compensated inclusive prefix. This is the
+//      reference's adjoint (its running "cumsum" subtraction) written for a
+//      lane-parallel scan. Samples after the early-termination point
+//      (T < min_T) get zero gradient, as in the reference.
+// Forward-only compositing (rendering) walks each wave's window of rays.
 
 #include "segscan.hpp"
 
@@ -73,6 +85,7 @@ TN_DEVICE void ray_loss(uint32_t r, uint32_t B, R rr, R rg, R rb, R ra, const fl
 template <typename R>
 struct SampleState {
     R sigma, dt, T, T_next, alpha, w;
+    Acc<R> P;  // inclusive prefix of sigma * dt along the ray
     bool inc;
 };
 
@@ -114,9 +127,10 @@ TN_DEVICE SampleState<R> sample_state(bool valid, int ray, const ChunkIn& in, in
     const R p_excl = p_incl - tau;
     st.T = r_exp<R>(-p_excl);
     st.T_next = r_exp<R>(-p_incl);
-    st.alpha = (R)1 - r_exp<R>(-tau);
+    st.alpha = r_alpha<R>(tau);
     st.w = st.T * st.alpha;
     st.inc = valid && st.T >= (R)min_T;
+    st.P = P;
     // carry of the prefix for the ray continuing past lane 63
     const int ray63 = __builtin_amdgcn_readlane(ray, 63);
     carry_P = P.bcast(63);
@@ -124,137 +138,252 @@ TN_DEVICE SampleState<R> sample_state(bool valid, int ray, const ChunkIn& in, in
     return st;
 }
 
-template <typename TIn, typename TGrad, typename R>
+// per chunk: two segment records of kChunkRec sums (sigma*dt, w c_r, w c_g,
+// w c_b, w) and a carry-in pair (sigma*dt prefix, w g prefix)
+constexpr int kChunkRec = 5;
+
+// Walk state at chunk start c: the ray holding sample c is carried into the
+// chunk when it started earlier; otherwise the walk starts at the first ray
+// beginning at c (zero-length rays there included).
+TN_DEVICE RayWalk chunk_walk(uint32_t c, uint32_t n_rays, const uint32_t* __restrict__ offs,
+                             const uint32_t* __restrict__ steps) {
+    RayWalk wk;
+    const uint32_t r_next = wave_lower_bound(offs, n_rays, c + 1);  // offs[0] = 0 <= c, so r_next >= 1
+    const uint32_t rf = r_next - 1;
+    const uint32_t of = offs[rf];
+    if (of < c) {
+        wk.carry_ray = (int)rf;
+        wk.carry_end = (int)(of + steps[rf]);
+        wk.r_scan = r_next;
+    } else {
+        wk.r_scan = wave_lower_bound(offs, n_rays, c);
+    }
+    return wk;
+}
+
+template <typename R>
+TN_DEVICE void put_rec(Acc<R>* __restrict__ dst, const Acc<R>& P, const Acc<R>& sr, const Acc<R>& sg,
+                       const Acc<R>& sb, const Acc<R>& sa) {
+    dst[0] = P;
+    dst[1] = sr;
+    dst[2] = sg;
+    dst[3] = sb;
+    dst[4] = sa;
+}
+
+template <typename R>
+TN_DEVICE Acc<R> acc_scaled(const Acc<R>& a, R k);
+template <>
+TN_DEVICE Acc<float> acc_scaled<float>(const Acc<float>& a, float k) {
+    return Acc<float>{CSum{a.v.hi * k, a.v.lo * k}};
+}
+template <>
+TN_DEVICE Acc<double> acc_scaled<double>(const Acc<double>& a, double k) {
+    return Acc<double>{a.v * k};
+}
+
+// (1) chunk sums, complete rays' losses
+template <typename TIn, typename R>
 __global__ void __launch_bounds__(64 * kCompositeWaves)
-composite_train_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB, uint32_t SB, uint32_t S,
-                       float grad_scale, float min_T, const uint32_t* __restrict__ steps,
-                       const uint32_t* __restrict__ offs, const float* __restrict__ bg,
-                       const float* __restrict__ target, const TIn* __restrict__ density,
+composite_chunk_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB, uint32_t SB, float min_T,
+                       const uint32_t* __restrict__ steps, const uint32_t* __restrict__ offs,
+                       const float* __restrict__ bg, const float* __restrict__ target, const TIn* __restrict__ density,
                        const TIn* __restrict__ color, const float* __restrict__ dtb, float* __restrict__ loss,
-                       float* __restrict__ dL_ddensity, TGrad* __restrict__ dL_dcolor,
-                       R* __restrict__ scratch) {
-    __shared__ int slots[kCompositeWaves][64];
+                       R* __restrict__ scratch, Acc<R>* __restrict__ rec) {
+    __shared__ int slots[kCompositeWaves][128];
     const uint32_t wid = threadIdx.x >> 6;
     const uint32_t lane = __lane_id();
-    const uint32_t w = blockIdx.x * kCompositeWaves + wid;
+    const uint32_t q = blockIdx.x * kCompositeWaves + wid;
     const uint32_t n_rays = info[0], n_samples = info[1];
-    const WaveRange wr = wave_range(w, S, n_rays, n_samples, offs);
-    if (!wr.active) return;
+    const uint32_t n_chunks = (n_samples + 63) / 64;
+    if (n_rays == 0 || q >= max(n_chunks, 1u)) return;
     const R inv_n_rays = (R)1 / (R)n_rays;
     int* slot = slots[wid];
-
     auto empty_ray = [&](uint32_t r) {
         ray_loss<R>(r, B, (R)0, (R)0, (R)0, (R)0, bg, target, inv_n_rays, loss, (R*)nullptr, SB);
     };
-
-    // ---------------- pass 1: forward totals, loss, dL/dRc ----------------
-    {
-        uint32_t r_scan = wr.r0;
-        int carry_ray = -1;
-        Acc<R> carry_P = Acc<R>::zero();
-        Acc<R> cr = Acc<R>::zero(), cg = Acc<R>::zero(), cb = Acc<R>::zero(), ca = Acc<R>::zero();
-        ChunkIn in = {};
-        if (wr.s0 < wr.s1) in = load_chunk(wr.s0, wr.s1, lane, density, dtb, color, PB);
-        for (uint32_t c = wr.s0; c < wr.s1; c += 64) {
-            const uint64_t s = (uint64_t)c + lane;
-            const bool valid = s < wr.s1;
-            const ChunkIn cur = in;
-            if (c + 64 < wr.s1) in = load_chunk(c + 64, wr.s1, lane, density, dtb, color, PB);
-            const int prev_carry = carry_ray;
-            int ray = chunk_rays(c, wr.r1, r_scan, offs, steps, slot, carry_ray, empty_ray);
-            if (!valid) ray = -2 - (int)lane;  // unique keys keep padding lanes out of every segment
-            SampleState<R> st = sample_state<R>(valid, ray, cur, carry_ray, carry_P, min_T);
-            const float cR = valid ? cur.cr : 0.0f, cG = valid ? cur.cg : 0.0f, cB = valid ? cur.cb : 0.0f;
-            const R wv = st.inc ? st.w : (R)0;
-            Acc<R> sr = Acc<R>::of(wv * (R)cR).scan(ray), sg = Acc<R>::of(wv * (R)cG).scan(ray);
-            Acc<R> sb = Acc<R>::of(wv * (R)cB).scan(ray), sa = Acc<R>::of(wv).scan(ray);
-            if (ray == prev_carry) {
-                sr = sr + cr;
-                sg = sg + cg;
-                sb = sb + cb;
-                sa = sa + ca;
-            }
-            // tail lanes finish their ray
-            bool tail = false;
-            if (valid) {
-                const uint32_t rr = (uint32_t)ray;
-                tail = s + 1 == (uint64_t)offs[rr] + steps[rr];
-                if (tail)
-                    ray_loss<R>(rr, B, sr.val(), sg.val(), sb.val(), sa.val(), bg, target, inv_n_rays, loss,
-                                scratch, SB);
-            }
-            // carry of the running totals for a ray continuing past this chunk
-            const bool tail63 = __builtin_amdgcn_readlane((int)tail, 63) != 0;
-            cr = sr.bcast(63);
-            cg = sg.bcast(63);
-            cb = sb.bcast(63);
-            ca = sa.bcast(63);
-            if (tail63 || !__builtin_amdgcn_readlane((int)valid, 63)) carry_ray = -1;
-        }
-        // zero-length rays at the end of the window
-        for (uint32_t r = r_scan + lane; r < wr.r1; r += 64) empty_ray(r);
+    if (n_chunks == 0) {  // every ray is empty
+        for (uint32_t r = lane; r < n_rays; r += 64) empty_ray(r);
+        return;
     }
-    // pass 2 reads the per-ray totals written by other lanes of this wave
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const uint32_t c = 64 * q;
+    RayWalk wk = chunk_walk(c, n_rays, offs, steps);
+    const int carried = wk.carry_ray;
+    const uint64_t s = (uint64_t)c + lane;
+    const bool valid = s < n_samples;
+    const ChunkIn in = load_chunk(c, n_samples, lane, density, dtb, color, PB);
+    const LaneRay lr = chunk_rays(c, n_rays, wk, offs, steps, slot, empty_ray);
+    const int ray = valid ? lr.ray : -2 - (int)lane;  // unique keys keep padding lanes out of every segment
+    int no_carry = -1;
+    Acc<R> no_P = Acc<R>::zero();
+    const SampleState<R> st = sample_state<R>(valid, ray, in, no_carry, no_P, min_T);
+    // Rays starting in this chunk see their true transmittance and stop as
+    // the reference does; the carried ray's segment is summed whole (its
+    // stop, if any, is found per ray).
+    const bool own = valid && ray != carried;
+    const R wv = (own ? st.inc : valid) ? st.w : (R)0;
+    const float cR = valid ? in.cr : 0.0f, cG = valid ? in.cg : 0.0f, cB = valid ? in.cb : 0.0f;
+    const Acc<R> sr = Acc<R>::of(wv * (R)cR).scan(ray), sg = Acc<R>::of(wv * (R)cG).scan(ray);
+    const Acc<R> sb = Acc<R>::of(wv * (R)cB).scan(ray), sa = Acc<R>::of(wv).scan(ray);
+    const bool tail = valid && s + 1 == (uint64_t)lr.end;
+    if (tail && own)
+        ray_loss<R>((uint32_t)ray, B, sr.val(), sg.val(), sb.val(), sa.val(), bg, target, inv_n_rays, loss, scratch,
+                    SB);
+    // the records: lane 0's segment at its last lane, lane 63's segment
+    const int ray_next = __shfl_down(ray, 1, 64);
+    const bool seg_last = lane == 63 || ray_next != ray;
+    Acc<R>* rq = rec + (uint64_t)q * 2 * kChunkRec;
+    if (valid && seg_last && ray == __builtin_amdgcn_readlane(ray, 0)) put_rec<R>(rq, st.P, sr, sg, sb, sa);
+    if (valid && lane == 63) put_rec<R>(rq + kChunkRec, st.P, sr, sg, sb, sa);
+    // zero-length rays after the last sample
+    if (q + 1 == n_chunks)
+        for (uint32_t r = wk.r_scan + lane; r < n_rays; r += 64) empty_ray(r);
+}
 
-    // ---------------- pass 2: gradients ----------------
-    {
-        uint32_t r_scan = wr.r0;
-        int carry_ray = -1;
-        Acc<R> carry_P = Acc<R>::zero();
-        Acc<R> carry_Q = Acc<R>::zero();
-        auto no_op = [](uint32_t) {};
-        ChunkIn in = {};
-        if (wr.s0 < wr.s1) in = load_chunk(wr.s0, wr.s1, lane, density, dtb, color, PB);
-        for (uint32_t c = wr.s0; c < wr.s1; c += 64) {
-            const uint64_t s = (uint64_t)c + lane;
-            const bool valid = s < wr.s1;
-            const ChunkIn cur = in;
-            if (c + 64 < wr.s1) in = load_chunk(c + 64, wr.s1, lane, density, dtb, color, PB);
-            const int prev_carry = carry_ray;
-            int ray = chunk_rays(c, wr.r1, r_scan, offs, steps, slot, carry_ray, no_op);
-            if (!valid) ray = -2 - (int)lane;
-            SampleState<R> st = sample_state<R>(valid, ray, cur, carry_ray, carry_P, min_T);
-            R g = 0, wg = 0, A = 0, GR = 0, GG = 0, GB = 0, Wg_tot = 0;
-            if (valid) {
-                const uint32_t rr = (uint32_t)ray;
-                const R cR = cur.cr;
-                const R cG = cur.cg;
-                const R cB = cur.cb;
-                A = scratch[rr];
-                const R RR = scratch[rr + SB], RG = scratch[rr + 2 * SB], RB = scratch[rr + 3 * SB];
-                GR = scratch[rr + 4 * SB];
-                GG = scratch[rr + 5 * SB];
-                GB = scratch[rr + 6 * SB];
-                const R br = bg[rr], bgg = bg[rr + B], bb = bg[rr + 2 * B];
-                g = GR * (cR * A + RR - br) + GG * (cG * A + RG - bgg) + GB * (cB * A + RB - bb);
-                Wg_tot = A * (GR * ((R)2 * RR - br) + GG * ((R)2 * RG - bgg) + GB * ((R)2 * RB - bb));
-                wg = st.inc ? st.w * g : (R)0;
-            }
-            Acc<R> Q = Acc<R>::of(wg).scan(ray);
-            if (ray == prev_carry) Q = Q + carry_Q;
-            if (valid) {
-                float dsig = 0.0f, gc_r = 0.0f, gc_g = 0.0f, gc_b = 0.0f;
-                if (st.inc) {
-                    const R suffix = Wg_tot - Q.val();
-                    dsig = (float)(st.dt * (st.T_next * g - suffix) * st.sigma);
-                    const R k = (R)grad_scale * st.w * A;
-                    gc_r = (float)(k * GR);
-                    gc_g = (float)(k * GG);
-                    gc_b = (float)(k * GB);
+// (2) one thread per ray spanning several chunks: totals, loss, carry-ins
+template <typename TIn, typename R>
+__global__ void __launch_bounds__(256)
+composite_ray_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB, uint32_t SB, float min_T,
+                     const uint32_t* __restrict__ steps, const uint32_t* __restrict__ offs,
+                     const float* __restrict__ bg, const float* __restrict__ target, const TIn* __restrict__ density,
+                     const TIn* __restrict__ color, const float* __restrict__ dtb, float* __restrict__ loss,
+                     R* __restrict__ scratch, const Acc<R>* __restrict__ rec, Acc<R>* __restrict__ carry) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_rays = info[0];
+    if (r >= n_rays) return;
+    const uint32_t o = offs[r], n = steps[r];
+    if (n == 0) return;
+    const uint32_t qs = o / 64, qe = (o + n - 1) / 64;
+    if (qs == qe) return;  // complete in one chunk
+    auto rec_of = [&](uint32_t k) { return rec + ((uint64_t)k * 2 + (k == qs ? 1u : 0u)) * kChunkRec; };
+    // totals
+    Acc<R> Pin = Acc<R>::zero();
+    Acc<R> tr = Acc<R>::zero(), tg = Acc<R>::zero(), tb = Acc<R>::zero(), ta = Acc<R>::zero();
+    Acc<R> pr = Acc<R>::zero(), pg = Acc<R>::zero(), pb = Acc<R>::zero(), pa = Acc<R>::zero();
+    uint32_t q_stop = 0xFFFFFFFFu;  // chunk whose segment holds the stop, if any
+    for (uint32_t k = qs; k <= qe; ++k) {
+        const Acc<R>* rk = rec_of(k);
+        if (k > qs) carry[(uint64_t)k * 2] = Pin;
+        const Acc<R> Pout = Pin + rk[0];
+        if (q_stop == 0xFFFFFFFFu) {
+            if (r_exp<R>(-Pout.val()) >= (R)min_T) {  // every sample of the segment sees T >= min_T
+                const R Tin = r_exp<R>(-Pin.val());
+                tr = tr + acc_scaled<R>(rk[1], Tin);
+                tg = tg + acc_scaled<R>(rk[2], Tin);
+                tb = tb + acc_scaled<R>(rk[3], Tin);
+                ta = ta + acc_scaled<R>(rk[4], Tin);
+            } else {  // the stop lies in this segment: its samples in order
+                const uint32_t s0 = max(o, 64 * k), s1 = min(o + n, 64 * k + 64);
+                Acc<R> p = Pin;
+                for (uint32_t sm = s0; sm < s1; ++sm) {
+                    const R T = r_exp<R>(-p.val());
+                    if (!(T >= (R)min_T)) break;
+                    const R sigma = r_exp<R>((R)fminf(ldf(density, sm), 12.0f) - (R)1);
+                    const R tau = sigma * (R)dtb[sm];
+                    const R w = T * r_alpha<R>(tau);
+                    pr = pr + Acc<R>::of(w * (R)ldf(color, sm));
+                    pg = pg + Acc<R>::of(w * (R)ldf(color, sm + PB));
+                    pb = pb + Acc<R>::of(w * (R)ldf(color, sm + 2 * (uint64_t)PB));
+                    pa = pa + Acc<R>::of(w);
+                    p = p + Acc<R>::of(tau);
                 }
-                dL_ddensity[s] = dsig;
-                stf(dL_dcolor, s, gc_r);
-                stf(dL_dcolor, s + PB, gc_g);
-                stf(dL_dcolor, s + 2 * (uint64_t)PB, gc_b);
+                tr = tr + pr;
+                tg = tg + pg;
+                tb = tb + pb;
+                ta = ta + pa;
+                q_stop = k;
             }
-            carry_Q = Q.bcast(63);
-            const bool tail63 =
-                __builtin_amdgcn_readlane((int)(valid && s + 1 == (uint64_t)offs[valid ? ray : 0] + steps[valid ? ray : 0]), 63) != 0;
-            if (tail63 || !__builtin_amdgcn_readlane((int)valid, 63)) carry_ray = -1;
         }
+        Pin = Pout;
     }
+    const R inv_n_rays = (R)1 / (R)n_rays;
+    ray_loss<R>(r, B, tr.val(), tg.val(), tb.val(), ta.val(), bg, target, inv_n_rays, loss, scratch, SB);
+    const R A = ta.val(), RR = tr.val(), RG = tg.val(), RB = tb.val();
+    const R GR = scratch[r + 4 * SB], GG = scratch[r + 5 * SB], GB = scratch[r + 6 * SB];
+    const R br = bg[r], bgg = bg[r + B], bb = bg[r + 2 * B];
+    // prefix of w g at each chunk start: sum_seg w g = sum_ch G_ch (A sum w c_ch + (R_ch - bg_ch) sum w)
+    auto wg_of = [&](R swr, R swg, R swb, R swa) {
+        return Acc<R>::of(GR * A * swr) + Acc<R>::of(GR * (RR - br) * swa) + Acc<R>::of(GG * A * swg) +
+               Acc<R>::of(GG * (RG - bgg) * swa) + Acc<R>::of(GB * A * swb) + Acc<R>::of(GB * (RB - bb) * swa);
+    };
+    Acc<R> Q = Acc<R>::zero();
+    Pin = Acc<R>::zero();
+    for (uint32_t k = qs; k <= qe; ++k) {
+        const Acc<R>* rk = rec_of(k);
+        if (k > qs) carry[(uint64_t)k * 2 + 1] = Q;
+        if (k < q_stop) {
+            const R Tin = r_exp<R>(-Pin.val());
+            Q = Q + wg_of(Tin * rk[1].val(), Tin * rk[2].val(), Tin * rk[3].val(), Tin * rk[4].val());
+        } else if (k == q_stop) {
+            Q = Q + wg_of(pr.val(), pg.val(), pb.val(), pa.val());
+        }
+        Pin = Pin + rk[0];
+    }
+}
+
+// (3) gradients per chunk with the chained carry-ins
+template <typename TIn, typename TGrad, typename R>
+__global__ void __launch_bounds__(64 * kCompositeWaves)
+composite_grad_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB, uint32_t SB, float grad_scale,
+                      float min_T, const uint32_t* __restrict__ steps, const uint32_t* __restrict__ offs,
+                      const float* __restrict__ bg, const TIn* __restrict__ density, const TIn* __restrict__ color,
+                      const float* __restrict__ dtb, float* __restrict__ dL_ddensity, TGrad* __restrict__ dL_dcolor,
+                      const R* __restrict__ scratch, const Acc<R>* __restrict__ carry) {
+    __shared__ int slots[kCompositeWaves][128];
+    const uint32_t wid = threadIdx.x >> 6;
+    const uint32_t lane = __lane_id();
+    const uint32_t q = blockIdx.x * kCompositeWaves + wid;
+    const uint32_t n_rays = info[0], n_samples = info[1];
+    const uint32_t n_chunks = (n_samples + 63) / 64;
+    if (n_rays == 0 || q >= n_chunks) return;
+    int* slot = slots[wid];
+    const uint32_t c = 64 * q;
+    RayWalk wk = chunk_walk(c, n_rays, offs, steps);
+    const int carried = wk.carry_ray;
+    Acc<R> Pin = Acc<R>::zero(), Qin = Acc<R>::zero();
+    if (carried >= 0) {
+        Pin = carry[(uint64_t)q * 2];
+        Qin = carry[(uint64_t)q * 2 + 1];
+    }
+    const uint64_t s = (uint64_t)c + lane;
+    const bool valid = s < n_samples;
+    const ChunkIn in = load_chunk(c, n_samples, lane, density, dtb, color, PB);
+    auto no_op = [](uint32_t) {};
+    const LaneRay lr = chunk_rays(c, n_rays, wk, offs, steps, slot, no_op);
+    const int ray = valid ? lr.ray : -2 - (int)lane;
+    int carry_ray = carried;
+    Acc<R> carry_P = Pin;
+    const SampleState<R> st = sample_state<R>(valid, ray, in, carry_ray, carry_P, min_T);
+    R g = 0, wg = 0, A = 0, GR = 0, GG = 0, GB = 0, Wg_tot = 0;
+    if (valid) {
+        const uint32_t rr = (uint32_t)ray;
+        A = scratch[rr];
+        const R RR = scratch[rr + SB], RG = scratch[rr + 2 * SB], RB = scratch[rr + 3 * SB];
+        GR = scratch[rr + 4 * SB];
+        GG = scratch[rr + 5 * SB];
+        GB = scratch[rr + 6 * SB];
+        const R br = bg[rr], bgg = bg[rr + B], bb = bg[rr + 2 * B];
+        const R cR = in.cr, cG = in.cg, cB = in.cb;
+        g = GR * (cR * A + RR - br) + GG * (cG * A + RG - bgg) + GB * (cB * A + RB - bb);
+        Wg_tot = A * (GR * ((R)2 * RR - br) + GG * ((R)2 * RG - bgg) + GB * ((R)2 * RB - bb));
+        wg = st.inc ? st.w * g : (R)0;
+    }
+    Acc<R> Q = Acc<R>::of(wg).scan(ray);
+    if (ray == carried) Q = Q + Qin;
+    if (!valid) return;
+    float dsig = 0.0f, gc_r = 0.0f, gc_g = 0.0f, gc_b = 0.0f;
+    if (st.inc) {
+        const R suffix = Wg_tot - Q.val();
+        dsig = (float)(st.dt * (st.T_next * g - suffix) * st.sigma);
+        const R k = (R)grad_scale * st.w * A;
+        gc_r = (float)(k * GR);
+        gc_g = (float)(k * GG);
+        gc_b = (float)(k * GB);
+    }
+    dL_ddensity[s] = dsig;
+    stf(dL_dcolor, s, gc_r);
+    stf(dL_dcolor, s + PB, gc_g);
+    stf(dL_dcolor, s + 2 * (uint64_t)PB, gc_b);
 }
 
 // Forward-only compositing: premultiplied rgb and alpha per ray, with the
@@ -269,7 +398,7 @@ composite_forward_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t
                          int raw_density, const uint32_t* __restrict__ steps, const uint32_t* __restrict__ offs,
                          const TIn* __restrict__ density, const TIn* __restrict__ color,
                          const float* __restrict__ dtb, float* __restrict__ rgba) {
-    __shared__ int slots[kCompositeWaves][64];
+    __shared__ int slots[kCompositeWaves][128];
     const uint32_t wid = threadIdx.x >> 6;
     const uint32_t lane = __lane_id();
     const uint32_t w = blockIdx.x * kCompositeWaves + wid;
@@ -283,8 +412,8 @@ composite_forward_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t
         rgba[r + 2 * OB] = 0.0f;
         rgba[r + 3 * OB] = 0.0f;
     };
-    uint32_t r_scan = wr.r0;
-    int carry_ray = -1;
+    RayWalk wk;
+    wk.r_scan = wr.r0;
     Acc<R> carry_P = Acc<R>::zero();
     Acc<R> cr = Acc<R>::zero(), cg = Acc<R>::zero(), cb = Acc<R>::zero(), ca = Acc<R>::zero();
     ChunkIn in = {};
@@ -294,10 +423,10 @@ composite_forward_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t
         const bool valid = s < wr.s1;
         const ChunkIn cur = in;
         if (c + 64 < wr.s1) in = load_chunk(c + 64, wr.s1, lane, density, dtb, color, B);
-        const int prev_carry = carry_ray;
-        int ray = chunk_rays(c, wr.r1, r_scan, offs, steps, slot, carry_ray, empty_ray);
-        if (!valid) ray = -2 - (int)lane;
-        SampleState<R> st = sample_state<R>(valid, ray, cur, carry_ray, carry_P, min_T, raw_density != 0);
+        const int prev_carry = wk.carry_ray;
+        const LaneRay lr = chunk_rays(c, wr.r1, wk, offs, steps, slot, empty_ray);
+        const int ray = valid ? lr.ray : -2 - (int)lane;
+        SampleState<R> st = sample_state<R>(valid, ray, cur, wk.carry_ray, carry_P, min_T, raw_density != 0);
         const float cR = valid ? cur.cr : 0.0f, cG = valid ? cur.cg : 0.0f, cB = valid ? cur.cb : 0.0f;
         const R wv = st.inc ? st.w : (R)0;
         const R wc = st.alpha > (R)1e-4f ? wv : (R)0;
@@ -309,25 +438,23 @@ composite_forward_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t
             sb = sb + cb;
             sa = sa + ca;
         }
-        bool tail = false;
-        if (valid) {
+        const bool tail = valid && s + 1 == (uint64_t)lr.end;
+        if (tail) {
             const uint32_t rr = (uint32_t)ray;
-            tail = s + 1 == (uint64_t)offs[rr] + steps[rr];
-            if (tail) {
-                rgba[rr] = (float)sr.val();
-                rgba[rr + OB] = (float)sg.val();
-                rgba[rr + 2 * OB] = (float)sb.val();
-                rgba[rr + 3 * OB] = (float)sa.val();
-            }
+            rgba[rr] = (float)sr.val();
+            rgba[rr + OB] = (float)sg.val();
+            rgba[rr + 2 * OB] = (float)sb.val();
+            rgba[rr + 3 * OB] = (float)sa.val();
         }
-        const bool tail63 = __builtin_amdgcn_readlane((int)tail, 63) != 0;
+        const bool tail63 = lane63((int)tail) != 0;
         cr = sr.bcast(63);
         cg = sg.bcast(63);
         cb = sb.bcast(63);
         ca = sa.bcast(63);
-        if (tail63 || !__builtin_amdgcn_readlane((int)valid, 63)) carry_ray = -1;
+        wk.carry_end = lane63(lr.end);
+        if (tail63 || !lane63((int)valid)) wk.carry_ray = -1;
     }
-    for (uint32_t r = r_scan + lane; r < wr.r1; r += 64) empty_ray(r);
+    for (uint32_t r = wk.r_scan + lane; r < wr.r1; r += 64) empty_ray(r);
 }
 
 // deterministic mean of the per-ray losses: (1/n_rays) sum_r sum_ch loss
@@ -352,17 +479,37 @@ __global__ void reduce_loss_kernel(const uint32_t* __restrict__ info, uint32_t B
     }
 }
 
+// composite scratch: per-ray rows, then per-chunk records and carry-ins
+inline size_t comp_align(size_t x) { return (x + 255) & ~(size_t)255; }
+inline uint32_t comp_chunks(uint32_t sample_stride) { return sample_stride / 64 + 1; }
+inline size_t comp_rays_bytes(uint32_t n_rays_max) { return comp_align((size_t)kRayScratchRows * n_rays_max * 8); }
+inline size_t comp_rec_bytes(uint32_t sample_stride) {
+    return comp_align((size_t)comp_chunks(sample_stride) * 2 * kChunkRec * 8);
+}
+inline size_t comp_carry_bytes(uint32_t sample_stride) {
+    return comp_align((size_t)comp_chunks(sample_stride) * 2 * 8);
+}
+static_assert(sizeof(Acc<float>) == 8 && sizeof(Acc<double>) == 8, "8-byte accumulators");
+
 template <typename TIn, typename TGrad, typename R>
 int launch_composite(const uint32_t* info, uint32_t n_rays_max, uint32_t B, uint32_t PB, float grad_scale,
                      float min_T, const uint32_t* steps, const uint32_t* offs, const float* bg, const float* target,
                      const void* density, const void* color, const float* dt, float* loss, float* dL_ddensity,
                      void* dL_dcolor, void* scratch, hipStream_t stream) {
-    const uint32_t S = 128;
-    const uint32_t waves = (PB + S - 1) / S + 1;
-    const uint32_t blocks = (waves + kCompositeWaves - 1) / kCompositeWaves;
-    composite_train_kernel<TIn, TGrad, R><<<blocks, 64 * kCompositeWaves, 0, stream>>>(
-        info, B, PB, n_rays_max, S, grad_scale, min_T, steps, offs, bg, target, static_cast<const TIn*>(density),
-        static_cast<const TIn*>(color), dt, loss, dL_ddensity, static_cast<TGrad*>(dL_dcolor), static_cast<R*>(scratch));
+    unsigned char* base = static_cast<unsigned char*>(scratch);
+    R* rays = reinterpret_cast<R*>(base);
+    Acc<R>* rec = reinterpret_cast<Acc<R>*>(base + comp_rays_bytes(n_rays_max));
+    Acc<R>* carry = reinterpret_cast<Acc<R>*>(base + comp_rays_bytes(n_rays_max) + comp_rec_bytes(PB));
+    const uint32_t chunk_blocks = (comp_chunks(PB) + kCompositeWaves - 1) / kCompositeWaves;
+    const TIn* d = static_cast<const TIn*>(density);
+    const TIn* c = static_cast<const TIn*>(color);
+    composite_chunk_kernel<TIn, R><<<chunk_blocks, 64 * kCompositeWaves, 0, stream>>>(
+        info, B, PB, n_rays_max, min_T, steps, offs, bg, target, d, c, dt, loss, rays, rec);
+    composite_ray_kernel<TIn, R><<<blocks_for(n_rays_max, 256), 256, 0, stream>>>(
+        info, B, PB, n_rays_max, min_T, steps, offs, bg, target, d, c, dt, loss, rays, rec, carry);
+    composite_grad_kernel<TIn, TGrad, R><<<chunk_blocks, 64 * kCompositeWaves, 0, stream>>>(
+        info, B, PB, n_rays_max, grad_scale, min_T, steps, offs, bg, d, c, dt, dL_ddensity,
+        static_cast<TGrad*>(dL_dcolor), rays, carry);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }
@@ -371,8 +518,8 @@ int launch_composite(const uint32_t* info, uint32_t n_rays_max, uint32_t B, uint
 
 using namespace tn;
 
-extern "C" size_t tn_composite_scratch_bytes(uint32_t n_rays_max) {
-    return (size_t)kRayScratchRows * n_rays_max * sizeof(double);
+extern "C" size_t tn_composite_scratch_bytes(uint32_t n_rays_max, uint32_t sample_stride) {
+    return comp_rays_bytes(n_rays_max) + comp_rec_bytes(sample_stride) + comp_carry_bytes(sample_stride);
 }
 
 // dtype: 0 = fp32 network outputs and fp32 colour gradients,
@@ -386,8 +533,9 @@ extern "C" int tn_composite_train(int dtype, const uint32_t* info, uint32_t n_ra
                                   hipStream_t stream) {
     TN_CHECK_ARG(info && ray_steps && ray_offset && bg_rgb && target_rgba && density && color && dt && loss &&
                  dL_ddensity && dL_dcolor && ray_scratch);
-    TN_CHECK_ARG(n_rays_max > 0 && n_rays_max <= stride && sample_stride > 0);
-    if (scratch_bytes < tn_composite_scratch_bytes(n_rays_max)) return TN_ERR_OUT_OF_MEMORY;
+    TN_CHECK_ARG(n_rays_max > 0 && n_rays_max <= stride && sample_stride > 0 && sample_stride < (1u << 31));
+    if (scratch_bytes < tn_composite_scratch_bytes(n_rays_max, sample_stride)) return TN_ERR_OUT_OF_MEMORY;
+    TN_CHECK_ARG((reinterpret_cast<uintptr_t>(ray_scratch) & 255) == 0);
     switch (dtype) {
         case 0:
             return launch_composite<float, float, float>(info, n_rays_max, stride, sample_stride, grad_scale, min_transmittance,
@@ -424,6 +572,7 @@ extern "C" int tn_composite_forward(int dtype, const uint32_t* info, uint32_t st
                                     const uint32_t* ray_steps, const uint32_t* ray_offset, const void* density,
                                     const void* color, const float* dt, float* rgba, hipStream_t stream) {
     TN_CHECK_ARG(info && ray_steps && ray_offset && density && color && dt && rgba && max_samples <= stride);
+    TN_CHECK_ARG(max_samples < (1u << 31));  // sample offsets fit an int
     const uint32_t S = 128;
     const uint32_t waves = (max_samples + S - 1) / S + 1;
     const uint32_t blocks = (waves + kCompositeWaves - 1) / kCompositeWaves;

@@ -46,7 +46,7 @@ TN_DEVICE DistState<R> dist_state(bool valid, int ray, int carry_ray_in, uint64_
     const R p_incl = P.val();
     const R T = r_exp<R>(-(p_incl - tau));
     st.T_next = r_exp<R>(-p_incl);
-    st.w = valid ? T * ((R)1 - r_exp<R>(-tau)) : (R)0;
+    st.w = valid ? T * r_alpha<R>(tau) : (R)0;
     Acc<R> W = Acc<R>::of(st.w).scan(ray);
     Acc<R> WM = Acc<R>::of(st.w * st.m).scan(ray);
     if (ray == carry_ray_in) {
@@ -68,7 +68,7 @@ distortion_kernel(const uint32_t* __restrict__ info, uint32_t S, float lambda, c
                   const uint32_t* __restrict__ offs, const TIn* __restrict__ density, const float* __restrict__ mn,
                   const float* __restrict__ dtn, float* __restrict__ ray_loss, float* __restrict__ dL_ds,
                   R* __restrict__ scratch, uint32_t SB) {
-    __shared__ int slots[kDistWaves][64];
+    __shared__ int slots[kDistWaves][128];
     const uint32_t wid = threadIdx.x >> 6;
     const uint32_t lane = __lane_id();
     const uint32_t w = blockIdx.x * kDistWaves + wid;
@@ -78,8 +78,8 @@ distortion_kernel(const uint32_t* __restrict__ info, uint32_t S, float lambda, c
     int* slot = slots[wid];
 
     for (int pass = 0; pass < 3; ++pass) {
-        uint32_t r_scan = wr.r0;
-        int carry_ray = -1;
+        RayWalk wk;
+        wk.r_scan = wr.r0;
         Acc<R> cP = Acc<R>::zero(), cW = Acc<R>::zero(), cWM = Acc<R>::zero(), cL = Acc<R>::zero();
         auto empty_ray = [&](uint32_t r) {
             if (pass == 0) ray_loss[r] = 0.0f;
@@ -87,12 +87,12 @@ distortion_kernel(const uint32_t* __restrict__ info, uint32_t S, float lambda, c
         for (uint32_t c = wr.s0; c < wr.s1; c += 64) {
             const uint64_t s = (uint64_t)c + lane;
             const bool valid = s < wr.s1;
-            const int prev = carry_ray;
-            int ray = chunk_rays(c, wr.r1, r_scan, offs, steps, slot, carry_ray, empty_ray);
-            if (!valid) ray = -2 - (int)lane;
+            const int prev = wk.carry_ray;
+            const LaneRay lr = chunk_rays(c, wr.r1, wk, offs, steps, slot, empty_ray);
+            const int ray = valid ? lr.ray : -2 - (int)lane;
             const DistState<R> st = dist_state<R>(valid, ray, prev, s, density, mn, dtn, cP, cW, cWM);
             const uint32_t rr = valid ? (uint32_t)ray : 0u;
-            const bool tail = valid && s + 1 == (uint64_t)offs[rr] + steps[rr];
+            const bool tail = valid && s + 1 == (uint64_t)lr.end;
             const R kf = valid ? (R)lambda / (R)steps[rr] : (R)0;
             R v = 0;
             if (pass == 0) {
@@ -130,12 +130,13 @@ distortion_kernel(const uint32_t* __restrict__ info, uint32_t S, float lambda, c
                 }
                 cL = L.bcast(63);
             }
-            const bool tail63 = __builtin_amdgcn_readlane((int)tail, 63) != 0;
-            carry_ray = __builtin_amdgcn_readlane(ray, 63);
-            if (tail63 || !__builtin_amdgcn_readlane((int)valid, 63)) carry_ray = -1;
+            const bool tail63 = lane63((int)tail) != 0;
+            wk.carry_ray = lane63(ray);
+            wk.carry_end = lane63(lr.end);
+            if (tail63 || !lane63((int)valid)) wk.carry_ray = -1;
         }
         if (pass == 0)
-            for (uint32_t r = r_scan + lane; r < wr.r1; r += 64) empty_ray(r);
+            for (uint32_t r = wk.r_scan + lane; r < wr.r1; r += 64) empty_ray(r);
         // the next pass reads totals this wave's tail lanes wrote
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -156,7 +157,7 @@ extern "C" int tn_distortion(int dtype, const uint32_t* info, uint32_t n_rays_ma
                              const float* m_norm, const float* dt_norm, float* ray_loss, float* dL_ds, void* scratch,
                              size_t scratch_bytes, hipStream_t stream) {
     TN_CHECK_ARG(info && ray_steps && ray_offset && density && m_norm && dt_norm && ray_loss && dL_ds && scratch);
-    TN_CHECK_ARG(n_rays_max > 0 && sample_stride > 0);
+    TN_CHECK_ARG(n_rays_max > 0 && sample_stride > 0 && sample_stride < (1u << 31));
     if (scratch_bytes < tn_distortion_scratch_bytes(n_rays_max)) return TN_ERR_OUT_OF_MEMORY;
     const uint32_t S = 128;
     const uint32_t waves = (sample_stride + S - 1) / S + 1;

@@ -160,6 +160,15 @@ TN_DEVICE float r_exp<float>(float x) { return __expf(x); }
 template <>
 TN_DEVICE double r_exp<double>(double x) { return exp(x); }
 
+// alpha = 1 - exp(-tau) without the cancellation of the subtraction when
+// tau is small (fine steps through thin media)
+template <typename R>
+TN_DEVICE R r_alpha(R tau);
+template <>
+TN_DEVICE float r_alpha<float>(float tau) { return -expm1f(-tau); }
+template <>
+TN_DEVICE double r_alpha<double>(double tau) { return -expm1(-tau); }
+
 // ---------------------------------------------------------------------------
 // first index r in [0, n) with offs[r] >= x (n if none), 64-ary search
 // ---------------------------------------------------------------------------
@@ -207,43 +216,77 @@ TN_DEVICE WaveRange wave_range(uint32_t w, uint32_t S, uint32_t n_rays, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// Ray of each lane in the chunk [c, c + 64). Rays with index >= *r_scan start
-// at or after c. Heads of rays with samples are scattered into `slot` (one
-// 64-entry LDS row owned by this wave); zero-length rays met on the way are
-// reported through `on_empty(ray)` (called by the lane that holds the ray).
-// Returns the ray index of this lane's sample, or carry_ray for lanes before
-// the first head.
+// Walking a window's rays chunk by chunk. Rays with index >= r_scan start at
+// or after the current chunk; carry_ray (or -1) continues from the previous
+// chunk and ends at sample carry_end. Each lane caches the offset and step
+// count of ray r_cache + lane, reloaded only when r_scan moves (when rays
+// start), so a chunk inside a long ray issues no loads.
 // ---------------------------------------------------------------------------
+struct RayWalk {
+    uint32_t r_scan;
+    uint32_t r_cache = UINT_MAX;
+    uint32_t o = 0, n = 0;  // per lane
+    int carry_ray = -1;
+    int carry_end = 0;
+};
+
+// the ray of a lane's sample and the end (one past the last sample) of it
+struct LaneRay {
+    int ray;
+    int end;
+};
+
+// Ray of each lane in the chunk [c, c + 64). Heads of rays with samples are
+// scattered into `slot` (a 128-entry LDS row owned by this wave: ray indices,
+// then ray ends); zero-length rays met on the way are reported through
+// `on_empty(ray)` (called by the lane that holds the ray). Lanes before the
+// first head take the carried ray. Sample offsets are below 2^31.
 template <typename OnEmpty>
-TN_DEVICE int chunk_rays(uint32_t c, uint32_t r1, uint32_t& r_scan, const uint32_t* __restrict__ offs,
-                         const uint32_t* __restrict__ steps, int* slot, int carry_ray, OnEmpty&& on_empty) {
+TN_DEVICE LaneRay chunk_rays(uint32_t c, uint32_t r1, RayWalk& wk, const uint32_t* __restrict__ offs,
+                             const uint32_t* __restrict__ steps, int* slot, OnEmpty&& on_empty) {
     const uint32_t lane = __lane_id();
     slot[lane] = -1;
+    slot[64 + lane] = -1;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     while (true) {
-        const uint32_t k = r_scan + lane;
-        bool inchunk = false;
-        if (k < r1) {
-            const uint32_t o = offs[k];
-            inchunk = o < c + 64u;
-            if (inchunk) {
-                if (steps[k] > 0) slot[o - c] = (int)k;
-                else on_empty(k);
+        const uint32_t k = wk.r_scan + lane;
+        if (wk.r_scan != wk.r_cache) {
+            wk.o = k < r1 ? offs[k] : UINT_MAX;
+            wk.n = k < r1 ? steps[k] : 0u;
+            wk.r_cache = wk.r_scan;
+        }
+        const bool inchunk = k < r1 && wk.o < c + 64u;
+        if (inchunk) {
+            if (wk.n > 0) {
+                slot[wk.o - c] = (int)k;
+                slot[64 + wk.o - c] = (int)(wk.o + wk.n);
+            } else {
+                on_empty(k);
             }
         }
-        const uint64_t ball = __ballot(inchunk);
-        const uint32_t cnt = (uint32_t)__builtin_popcountll(ball);
-        r_scan += cnt;
+        const uint32_t cnt = (uint32_t)__builtin_popcountll(__ballot(inchunk));
+        wk.r_scan += cnt;
         if (cnt < 64) break;
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    // propagate the most recent head to the following lanes (max-scan; ray
-    // indices increase along the chunk)
+    // propagate the most recent head to the following lanes (max-scans: ray
+    // indices and ray ends increase along the chunk)
     const int head = max_scan(slot[lane]);
+    const int end = max_scan(slot[64 + lane]);
     __builtin_amdgcn_wave_barrier();
-    return head < 0 ? carry_ray : head;
+    return head < 0 ? LaneRay{wk.carry_ray, wk.carry_end} : LaneRay{head, end};
+}
+
+// value of lane 63
+TN_DEVICE float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+TN_DEVICE int lane63(int v) { return __builtin_amdgcn_readlane(v, 63); }
+TN_DEVICE double lane63(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 }  // namespace tn

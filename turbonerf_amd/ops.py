@@ -136,7 +136,7 @@ def composite_train(info: torch.Tensor, n_rays_max: int, stride: int, sample_str
     loss = torch.zeros(4 * stride, dtype=torch.float32, device="cuda")
     dLd = torch.zeros(sample_stride, dtype=torch.float32, device="cuda")
     dLc = torch.zeros(3 * sample_stride, dtype=density.dtype, device="cuda")
-    nb = query("tn_composite_scratch_bytes", n_rays_max)
+    nb = query("tn_composite_scratch_bytes", n_rays_max, sample_stride)
     scratch = torch.empty(nb // 4, dtype=torch.float32, device="cuda")
     call("tn_composite_train", code, ptr(info), n_rays_max, stride, sample_stride, grad_scale, min_T, ptr(steps), ptr(offs),
          ptr(bg), ptr(target), ptr(density), ptr(color), ptr(dt), ptr(loss), ptr(dLd), ptr(dLc), ptr(scratch), nb,
