@@ -148,6 +148,15 @@ int tn_distortion(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t
 int tn_reduce_loss(const uint32_t* info, uint32_t stride, uint32_t n_channels, const float* loss,
     float* out, tn_stream_t stream);
 
+/* A training step's metrics in one launch: out[0] = mean over the info[0]
+ * rays of the four loss channels (stride `stride`), out[1] = mean of
+ * dist_loss (0 when dist_loss is null), and samples_total[0] += info[1] when
+ * samples_total is given. Replaces the loss reductions at the end of the
+ * reference's step (src/core/nerf-network.cu:594-602, :649-666) together with
+ * tn_accumulate_samples. */
+int tn_step_metrics(const uint32_t* info, uint32_t stride, const float* loss, const float* dist_loss,
+    float* out, unsigned long long* samples_total, tn_stream_t stream);
+
 /* ------------------------------------------------------- occupancy grid */
 
 /* density *= factor (reference decay_occupancy_grid_values_kernel,

@@ -144,3 +144,28 @@ def test_composite_long_rays(cuda, dt_scale):
     l, d, c = run_gpu(b, torch.float32, fp64_accumulate=True)
     np.testing.assert_allclose(l, l_ref, rtol=1e-6, atol=1e-9)
     assert norm_rel(d, d_ref) < 1e-6 and norm_rel(c, c_ref) < 1e-6
+
+
+def test_step_metrics(cuda):
+    """Mean loss over info[0] rays (entries past it ignored), mean distortion
+    loss or 0, and the device sample counter."""
+    import torch
+
+    from turbonerf_amd._capi import call, ptr, stream_ptr
+
+    rng = np.random.default_rng(4)
+    stride, n = 3000, 2345
+    loss = rng.uniform(0, 1, 4 * stride).astype(np.float32)
+    dist = rng.uniform(0, 1, stride).astype(np.float32)
+    info = torch.tensor([n, 123456, 0, 0], dtype=torch.int32, device="cuda")
+    total = torch.tensor([1000], dtype=torch.int64, device="cuda")
+    out = torch.full((2,), 7.0, dtype=torch.float32, device="cuda")
+    ld, dd = torch.from_numpy(loss).cuda(), torch.from_numpy(dist).cuda()
+    call("tn_step_metrics", ptr(info), stride, ptr(ld), ptr(dd), ptr(out), ptr(total), stream_ptr())
+    o = out.cpu().numpy()
+    ref = loss.astype(np.float64).reshape(4, stride)[:, :n].sum() / n
+    assert abs(o[0] - ref) <= 1e-6 * ref and abs(o[1] - dist[:n].astype(np.float64).mean()) <= 1e-6
+    assert int(total.cpu()[0]) == 1000 + 123456
+    call("tn_step_metrics", ptr(info), stride, ptr(ld), None, ptr(out), None, stream_ptr())
+    o = out.cpu().numpy()
+    assert o[1] == 0.0 and abs(o[0] - ref) <= 1e-6 * ref and int(total.cpu()[0]) == 1000 + 123456

@@ -110,6 +110,7 @@ _SIGS: dict[str, tuple] = {
                                      vp, sz, vp]),
     "tn_composite_forward": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, C.c_int, vp, vp, vp, vp, vp, vp, vp]),
     "tn_reduce_loss": (C.c_int, [vp, u32, u32, vp, vp, vp]),
+    "tn_step_metrics": (C.c_int, [vp, u32, vp, vp, vp, vp, vp]),
     "tn_occ_decay": (C.c_int, [OccupancyGrid, f32, vp]),
     "tn_occ_sample_points": (C.c_int, [u32, u32, u32, OccupancyGrid, C.c_int, f32, vp, vp, vp]),
     "tn_occ_update_density": (C.c_int, [C.c_int, u32, u32, OccupancyGrid, C.c_int, C.c_int, vp, vp, vp]),

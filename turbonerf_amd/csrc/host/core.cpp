@@ -198,14 +198,12 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
                             prec == 1 ? p.params_lp.data() : nullptr, p.gradients.data(), p.m1.data(), p.m2.data(),
                             p.steps.data(), ts));
     }
-    // losses: mean over rays of the four smooth-L1 channels, mean distortion
+    // losses: mean over rays of the four smooth-L1 channels, mean distortion,
+    // and the running count of trained samples
     {
         ScopedPhase ph(tm, "loss_reduce", s);
-        TN_TRY(tn_reduce_loss(info, R, 4, w.ray_loss.data(), w.loss_out.data(), ts));
-        if (st.use_distortion_loss)
-            TN_TRY(tn_reduce_loss(info, R, 1, w.ray_dist_loss.data(), w.loss_out.data() + 1, ts));
-        else
-            TN_HIP(hipMemsetAsync(w.loss_out.data() + 1, 0, sizeof(float), s));
+        TN_TRY(tn_step_metrics(info, R, w.ray_loss.data(), st.use_distortion_loss ? w.ray_dist_loss.data() : nullptr,
+                               w.loss_out.data(), st.samples_total, ts));
     }
 }
 
@@ -365,11 +363,11 @@ void Trainer::enqueue_train_step(Context& ctx, uint32_t training_step) {
 
     TrainingWorkspace& ws = ctx.batches[c];
     TN_HIP(hipStreamWaitEvent(net, ctx.ev_ready[c], 0));
-    if (ctx.samples_total.size()) TN_TRY(tn_accumulate_samples(ws.info.data(), ctx.samples_total.data(), as_tn(net)));
     NerfNetwork::Settings st;
     st.use_distortion_loss = ctx.use_distortion_loss;
     st.distortion_lambda = ctx.distortion_lambda;
     st.timer = &ctx.timer;
+    st.samples_total = ctx.samples_total.size() ? ctx.samples_total.data() : nullptr;
     ctx.nerf->network.train(net, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity,
                             ws.info.data(), ws.random_float.data() + ws.ray_capacity, ws.ray_step.data(),
                             ws.ray_offset.data(), ws.sample_t.data(), ws.sample_pos.data(), ws.sample_dir.data(),

@@ -148,6 +148,7 @@ struct NerfNetwork {
         float distortion_lambda = NeRFConstants::mipNeRF360_distortion_loss_lambda;
         float min_transmittance = NeRFConstants::min_transmittance;
         PhaseTimer* timer = nullptr;
+        unsigned long long* samples_total = nullptr;  // device counter of trained samples, if kept
     };
 
     NetworkConfig config;

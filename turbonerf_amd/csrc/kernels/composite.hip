@@ -491,6 +491,43 @@ inline size_t comp_carry_bytes(uint32_t sample_stride) {
 }
 static_assert(sizeof(Acc<float>) == 8 && sizeof(Acc<double>) == 8, "8-byte accumulators");
 
+// A training step's metrics in one launch: out[0] = mean over rays of the
+// four loss channels, out[1] = mean distortion loss (0 without one), and
+// total[0] += info[1] when a counter is given.
+__global__ void step_metrics_kernel(const uint32_t* __restrict__ info, uint32_t B, const float* __restrict__ loss,
+                                    const float* __restrict__ dist_loss, float* __restrict__ out,
+                                    unsigned long long* __restrict__ total) {
+    __shared__ double part[2][16];
+    const uint32_t n = info[0];
+    double acc = 0.0, accd = 0.0;
+    for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) {
+        float s = 0.0f;
+        for (uint32_t k = 0; k < 4; ++k) s += loss[r + (uint64_t)k * B];
+        acc += (double)s;
+        if (dist_loss) accd += (double)dist_loss[r];
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        acc += __shfl_down(acc, d, 64);
+        accd += __shfl_down(accd, d, 64);
+    }
+    if (__lane_id() == 0) {
+        part[0][threadIdx.x >> 6] = acc;
+        part[1][threadIdx.x >> 6] = accd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0, td = 0.0;
+        for (uint32_t k = 0; k < blockDim.x / 64; ++k) {
+            t += part[0][k];
+            td += part[1][k];
+        }
+        out[0] = n > 0 ? (float)(t / (double)n) : 0.0f;
+        out[1] = n > 0 && dist_loss ? (float)(td / (double)n) : 0.0f;
+        if (total) *total += info[1];
+    }
+}
+
 template <typename TIn, typename TGrad, typename R>
 int launch_composite(const uint32_t* info, uint32_t n_rays_max, uint32_t B, uint32_t PB, float grad_scale,
                      float min_T, const uint32_t* steps, const uint32_t* offs, const float* bg, const float* target,
@@ -560,6 +597,14 @@ extern "C" int tn_reduce_loss(const uint32_t* info, uint32_t stride, uint32_t n_
                               float* out, hipStream_t stream) {
     TN_CHECK_ARG(info && loss && out && n_channels >= 1 && n_channels <= 4);
     reduce_loss_kernel<<<1, 1024, 0, stream>>>(info, stride, n_channels, loss, out);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+extern "C" int tn_step_metrics(const uint32_t* info, uint32_t stride, const float* loss, const float* dist_loss,
+                               float* out, unsigned long long* samples_total, hipStream_t stream) {
+    TN_CHECK_ARG(info && loss && out);
+    step_metrics_kernel<<<1, 1024, 0, stream>>>(info, stride, loss, dist_loss, out, samples_total);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }
