@@ -59,7 +59,8 @@ KERNEL = {
                           "hashgrid_bwd_finish_kernel"),
     "mlp_forward": ("mlp_forward_kernel",),
     "mlp_backward": ("mlp_backward_kernel", "reduce_partials_kernel"),
-    "composite": ("composite_train_kernel",),
+    "composite": ("composite_chunk_kernel", "composite_ray_kernel", "composite_grad_kernel"),
+    "loss_reduce": ("step_metrics_kernel",),
     "march_count": ("march_count_kernel",),
     "march_generate": ("march_generate_kernel", "march_expand_kernel"),
     "adam": ("adam_kernel",),
@@ -382,7 +383,7 @@ def _short(name):
     base = name.split("(")[0]
     keys = {k for ks in KERNEL.values() for k in ks} | {
         "scan_tile_sums_kernel", "scan_tile_prefix_kernel", "scan_write_select_kernel", "uniform_kernel",
-        "init_training_rays_kernel", "estimate_rays_kernel", "accumulate_samples_kernel", "reduce_loss_kernel",
+        "init_training_rays_kernel", "estimate_rays_kernel", "reduce_partials_kernel",
         "write_rgba_kernel", "cast_kernel"}
     for key in sorted(keys, key=len, reverse=True):
         if key in base:

@@ -70,8 +70,11 @@ TN_DEVICE void st_nt(uint32_t* p, const uint32_t (&a)[4]) {
 
 // One thread per 4 consecutive parameters with 16-byte loads and stores of
 // the gradient, weights, moments and step counters (8 bytes for the fp16
-// copy); a group whose gradients are all zero inside the hash table costs one
-// 16-byte read.
+// copy). The skip of zero-gradient hash entries is decided per 8 lanes, one
+// 128-byte line of each fp32 array: a line without any gradient costs its
+// gradient read only; otherwise all 32 parameters of the line are read and
+// written back whole (skipped entries keep their values bit for bit), since
+// partially written lines cost more than the bytes they save.
 template <typename TL>
 __global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float inv_loss_scale, float lr, float beta1,
                             float beta2, float eps, float l2, float* __restrict__ w, TL* __restrict__ w_lp,
@@ -82,7 +85,9 @@ __global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, flo
     if (i4 + 3 < n) {
         const float4 g4 = *reinterpret_cast<const float4*>(grad + i4);
         const bool any = g4.x != 0.0f || g4.y != 0.0f || g4.z != 0.0f || g4.w != 0.0f;
-        if (!any && i4 + 3 < n_grid) return;  // untouched hash entries: one 16-byte read
+        const uint64_t touched = __ballot(any);
+        const bool line = ((touched >> (__lane_id() & 56u)) & 0xFFull) != 0ull;
+        if (!line && i4 + 3 < n_grid) return;  // a line of untouched hash entries: its gradient read only
         const float4 w4 = ld_nt(w + i4);
         const float4 m4 = ld_nt(m + i4);
         const float4 v4 = ld_nt(v + i4);
@@ -113,7 +118,7 @@ __global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, flo
         st_nt(v + i4, vv);
         st_nt(steps + i4, sv);
         if (w_lp) LowPrecision4<TL>::store(w_lp, i4, wv);
-        if (any) *reinterpret_cast<float4*>(grad + i4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        *reinterpret_cast<float4*>(grad + i4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     } else {
         for (uint32_t i = i4; i < n; ++i) {
             const float g = grad[i];
