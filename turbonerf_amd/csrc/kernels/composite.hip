@@ -50,12 +50,18 @@ TN_DEVICE float smooth_l1_grad(float x) { return fabsf(x) < 1.0f ? x : copysignf
 // per-ray scratch rows (stride = ray capacity): A, R[3], G[3]
 constexpr int kRayScratchRows = 7;
 
+template <typename R>
+struct RayGrad {
+    R GR, GG, GB;  // dL/dRc per channel
+};
+
 // loss and dL/dRc of one ray from its totals (reference :430-477); the
 // per-ray totals and dL/dRc go to scratch in the accumulation precision R
+// (stored only when `store`; dL/dRc is returned either way)
 template <typename R>
-TN_DEVICE void ray_loss(uint32_t r, uint32_t B, R rr, R rg, R rb, R ra, const float* __restrict__ bg,
-                        const float* __restrict__ target, R inv_n_rays, float* __restrict__ loss,
-                        R* __restrict__ scratch, uint32_t SB) {
+TN_DEVICE RayGrad<R> ray_loss(uint32_t r, uint32_t B, R rr, R rg, R rb, R ra, const float* __restrict__ bg,
+                              const float* __restrict__ target, R inv_n_rays, float* __restrict__ loss,
+                              R* __restrict__ scratch, uint32_t SB, bool store = true) {
     const R gr = target[r], gg = target[r + B], gb = target[r + 2 * B], ga = target[r + 3 * B];
     const R br = bg[r], bgg = bg[r + B], bb = bg[r + 2 * B];
     const R gac = (R)1 - ga;
@@ -65,6 +71,8 @@ TN_DEVICE void ray_loss(uint32_t r, uint32_t B, R rr, R rg, R rb, R ra, const fl
     const R dr = rrc - grc, dg = rgc - ggc, db = rbc - gbc, da = rac - gac;
     auto sl1 = [](R x) { const R ax = x < (R)0 ? -x : x; return ax < (R)1 ? (R)0.5 * x * x : ax - (R)0.5; };
     auto sl1g = [](R x) { const R ax = x < (R)0 ? -x : x; return ax < (R)1 ? x : (x < (R)0 ? (R)-1 : (R)1); };
+    const RayGrad<R> G = {inv_n_rays * sl1g(dr), inv_n_rays * sl1g(dg), inv_n_rays * sl1g(db)};
+    if (!store) return G;
     loss[r] = (float)sl1(dr);
     loss[r + B] = (float)sl1(dg);
     loss[r + 2 * B] = (float)sl1(db);
@@ -74,10 +82,11 @@ TN_DEVICE void ray_loss(uint32_t r, uint32_t B, R rr, R rg, R rb, R ra, const fl
         scratch[r + SB] = rr;
         scratch[r + 2 * SB] = rg;
         scratch[r + 3 * SB] = rb;
-        scratch[r + 4 * SB] = inv_n_rays * sl1g(dr);
-        scratch[r + 5 * SB] = inv_n_rays * sl1g(dg);
-        scratch[r + 6 * SB] = inv_n_rays * sl1g(db);
+        scratch[r + 4 * SB] = G.GR;
+        scratch[r + 5 * SB] = G.GG;
+        scratch[r + 6 * SB] = G.GB;
     }
+    return G;
 }
 
 // Per-lane recomputation shared by both passes: transmittance from the
@@ -240,15 +249,33 @@ composite_chunk_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t P
         for (uint32_t r = wk.r_scan + lane; r < n_rays; r += 64) empty_ray(r);
 }
 
-// (2) one thread per ray spanning several chunks: totals, loss, carry-ins
+// exclusive scan of an accumulator across the wave (lane 0 gets zero)
+template <typename R>
+TN_DEVICE Acc<R> acc_exclusive(const Acc<R>& v) {
+    const Acc<R> inc = v.scan(0);
+    Acc<R> ex;
+    if constexpr (sizeof(R) == 4) {
+        ex = Acc<R>{CSum{__shfl_up(inc.v.hi, 1, 64), __shfl_up(inc.v.lo, 1, 64)}};
+    } else {
+        ex = Acc<R>{__shfl_up(inc.v, 1, 64)};
+    }
+    return __lane_id() == 0 ? Acc<R>::zero() : ex;
+}
+
+// (2) one wave per ray spanning several chunks, a lane per chunk of the ray
+// (64 at a time): the sigma*dt prefixes at the chunk starts by a lane scan,
+// the stop (the first chunk whose end sees T < min_T: its samples are summed
+// in order up to the stop by that lane), the totals, loss and dL/dRc, and the
+// prefixes of w g at the chunk starts by a second lane scan
 template <typename TIn, typename R>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * kCompositeWaves)
 composite_ray_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB, uint32_t SB, float min_T,
                      const uint32_t* __restrict__ steps, const uint32_t* __restrict__ offs,
                      const float* __restrict__ bg, const float* __restrict__ target, const TIn* __restrict__ density,
                      const TIn* __restrict__ color, const float* __restrict__ dtb, float* __restrict__ loss,
                      R* __restrict__ scratch, const Acc<R>* __restrict__ rec, Acc<R>* __restrict__ carry) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r = blockIdx.x * kCompositeWaves + (threadIdx.x >> 6);
+    const uint32_t lane = __lane_id();
     const uint32_t n_rays = info[0];
     if (r >= n_rays) return;
     const uint32_t o = offs[r], n = steps[r];
@@ -256,23 +283,31 @@ composite_ray_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB,
     const uint32_t qs = o / 64, qe = (o + n - 1) / 64;
     if (qs == qe) return;  // complete in one chunk
     auto rec_of = [&](uint32_t k) { return rec + ((uint64_t)k * 2 + (k == qs ? 1u : 0u)) * kChunkRec; };
-    // totals
-    Acc<R> Pin = Acc<R>::zero();
+    // totals: lanes before the stop add their segment sums scaled by the
+    // transmittance at the segment start, the stop lane its partial sums
+    Acc<R> P0 = Acc<R>::zero();  // sigma*dt before this pass's first chunk
     Acc<R> tr = Acc<R>::zero(), tg = Acc<R>::zero(), tb = Acc<R>::zero(), ta = Acc<R>::zero();
-    Acc<R> pr = Acc<R>::zero(), pg = Acc<R>::zero(), pb = Acc<R>::zero(), pa = Acc<R>::zero();
     uint32_t q_stop = 0xFFFFFFFFu;  // chunk whose segment holds the stop, if any
-    for (uint32_t k = qs; k <= qe; ++k) {
-        const Acc<R>* rk = rec_of(k);
-        if (k > qs) carry[(uint64_t)k * 2] = Pin;
-        const Acc<R> Pout = Pin + rk[0];
+    Acc<R> pr = Acc<R>::zero(), pg = Acc<R>::zero(), pb = Acc<R>::zero(), pa = Acc<R>::zero();  // stop lane's
+    for (uint32_t k0 = qs; k0 <= qe; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        const bool act = k <= qe;
+        const Acc<R>* rk = rec_of(act ? k : qe);
+        const Acc<R> Pk = act ? rk[0] : Acc<R>::zero();
+        const Acc<R> Pin = P0 + acc_exclusive<R>(Pk);
+        const Acc<R> Pout = Pin + Pk;
+        if (act && k > qs) carry[(uint64_t)k * 2] = Pin;
+        const uint64_t low = __ballot(act && !(r_exp<R>(-Pout.val()) >= (R)min_T));
+        const uint32_t stop_lane = low ? (uint32_t)__builtin_ctzll(low) : 64u;
+        Acc<R> cr = Acc<R>::zero(), cg = Acc<R>::zero(), cb = Acc<R>::zero(), ca = Acc<R>::zero();
         if (q_stop == 0xFFFFFFFFu) {
-            if (r_exp<R>(-Pout.val()) >= (R)min_T) {  // every sample of the segment sees T >= min_T
+            if (act && lane < stop_lane) {
                 const R Tin = r_exp<R>(-Pin.val());
-                tr = tr + acc_scaled<R>(rk[1], Tin);
-                tg = tg + acc_scaled<R>(rk[2], Tin);
-                tb = tb + acc_scaled<R>(rk[3], Tin);
-                ta = ta + acc_scaled<R>(rk[4], Tin);
-            } else {  // the stop lies in this segment: its samples in order
+                cr = acc_scaled<R>(rk[1], Tin);
+                cg = acc_scaled<R>(rk[2], Tin);
+                cb = acc_scaled<R>(rk[3], Tin);
+                ca = acc_scaled<R>(rk[4], Tin);
+            } else if (lane == stop_lane) {  // the stop lies in this segment: its samples in order
                 const uint32_t s0 = max(o, 64 * k), s1 = min(o + n, 64 * k + 64);
                 Acc<R> p = Pin;
                 for (uint32_t sm = s0; sm < s1; ++sm) {
@@ -287,37 +322,55 @@ composite_ray_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB,
                     pa = pa + Acc<R>::of(w);
                     p = p + Acc<R>::of(tau);
                 }
-                tr = tr + pr;
-                tg = tg + pg;
-                tb = tb + pb;
-                ta = ta + pa;
-                q_stop = k;
+                cr = pr;
+                cg = pg;
+                cb = pb;
+                ca = pa;
             }
+            tr = tr + cr.scan(0).bcast(63);
+            tg = tg + cg.scan(0).bcast(63);
+            tb = tb + cb.scan(0).bcast(63);
+            ta = ta + ca.scan(0).bcast(63);
+            if (stop_lane < 64) q_stop = k0 + stop_lane;
         }
-        Pin = Pout;
+        P0 = Pout.bcast(63);
+    }
+    // the stop lane's partial sums, for the w g prefixes
+    if (q_stop != 0xFFFFFFFFu) {
+        const int src = (int)((q_stop - qs) & 63u);
+        pr = pr.bcast(src);
+        pg = pg.bcast(src);
+        pb = pb.bcast(src);
+        pa = pa.bcast(src);
     }
     const R inv_n_rays = (R)1 / (R)n_rays;
-    ray_loss<R>(r, B, tr.val(), tg.val(), tb.val(), ta.val(), bg, target, inv_n_rays, loss, scratch, SB);
     const R A = ta.val(), RR = tr.val(), RG = tg.val(), RB = tb.val();
-    const R GR = scratch[r + 4 * SB], GG = scratch[r + 5 * SB], GB = scratch[r + 6 * SB];
+    const RayGrad<R> G = ray_loss<R>(r, B, RR, RG, RB, A, bg, target, inv_n_rays, loss, scratch, SB, lane == 0);
     const R br = bg[r], bgg = bg[r + B], bb = bg[r + 2 * B];
     // prefix of w g at each chunk start: sum_seg w g = sum_ch G_ch (A sum w c_ch + (R_ch - bg_ch) sum w)
     auto wg_of = [&](R swr, R swg, R swb, R swa) {
-        return Acc<R>::of(GR * A * swr) + Acc<R>::of(GR * (RR - br) * swa) + Acc<R>::of(GG * A * swg) +
-               Acc<R>::of(GG * (RG - bgg) * swa) + Acc<R>::of(GB * A * swb) + Acc<R>::of(GB * (RB - bb) * swa);
+        return Acc<R>::of(G.GR * A * swr) + Acc<R>::of(G.GR * (RR - br) * swa) + Acc<R>::of(G.GG * A * swg) +
+               Acc<R>::of(G.GG * (RG - bgg) * swa) + Acc<R>::of(G.GB * A * swb) + Acc<R>::of(G.GB * (RB - bb) * swa);
     };
-    Acc<R> Q = Acc<R>::zero();
-    Pin = Acc<R>::zero();
-    for (uint32_t k = qs; k <= qe; ++k) {
-        const Acc<R>* rk = rec_of(k);
-        if (k > qs) carry[(uint64_t)k * 2 + 1] = Q;
-        if (k < q_stop) {
+    Acc<R> Q0 = Acc<R>::zero();
+    P0 = Acc<R>::zero();
+    for (uint32_t k0 = qs; k0 <= qe; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        const bool act = k <= qe;
+        const Acc<R>* rk = rec_of(act ? k : qe);
+        const Acc<R> Pk = act ? rk[0] : Acc<R>::zero();
+        const Acc<R> Pin = P0 + acc_exclusive<R>(Pk);
+        Acc<R> wg = Acc<R>::zero();
+        if (act && k < q_stop) {
             const R Tin = r_exp<R>(-Pin.val());
-            Q = Q + wg_of(Tin * rk[1].val(), Tin * rk[2].val(), Tin * rk[3].val(), Tin * rk[4].val());
-        } else if (k == q_stop) {
-            Q = Q + wg_of(pr.val(), pg.val(), pb.val(), pa.val());
+            wg = wg_of(Tin * rk[1].val(), Tin * rk[2].val(), Tin * rk[3].val(), Tin * rk[4].val());
+        } else if (act && k == q_stop) {
+            wg = wg_of(pr.val(), pg.val(), pb.val(), pa.val());
         }
-        Pin = Pin + rk[0];
+        const Acc<R> Qin = Q0 + acc_exclusive<R>(wg);
+        if (act && k > qs) carry[(uint64_t)k * 2 + 1] = Qin;
+        Q0 = (Qin + wg).bcast(63);
+        P0 = (Pin + Pk).bcast(63);
     }
 }
 
@@ -542,7 +595,7 @@ int launch_composite(const uint32_t* info, uint32_t n_rays_max, uint32_t B, uint
     const TIn* c = static_cast<const TIn*>(color);
     composite_chunk_kernel<TIn, R><<<chunk_blocks, 64 * kCompositeWaves, 0, stream>>>(
         info, B, PB, n_rays_max, min_T, steps, offs, bg, target, d, c, dt, loss, rays, rec);
-    composite_ray_kernel<TIn, R><<<blocks_for(n_rays_max, 256), 256, 0, stream>>>(
+    composite_ray_kernel<TIn, R><<<blocks_for(n_rays_max, kCompositeWaves), 64 * kCompositeWaves, 0, stream>>>(
         info, B, PB, n_rays_max, min_T, steps, offs, bg, target, d, c, dt, loss, rays, rec, carry);
     composite_grad_kernel<TIn, TGrad, R><<<chunk_blocks, 64 * kCompositeWaves, 0, stream>>>(
         info, B, PB, n_rays_max, grad_scale, min_T, steps, offs, bg, d, c, dt, dL_ddensity,
