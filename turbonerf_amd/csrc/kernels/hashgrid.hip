@@ -404,8 +404,9 @@ __global__ void __launch_bounds__(1024) hashgrid_bwd_scan_kernel(HbPlan p, const
                                                                  uint32_t* __restrict__ chunk_start,
                                                                  uint32_t* __restrict__ slot_base,
                                                                  uint32_t* __restrict__ split_list) {
-    __shared__ uint32_t part[4][1024];
+    __shared__ uint32_t wsum[4][16];
     const uint32_t k0 = kHbScanPer * threadIdx.x;
+    const uint32_t lane = __lane_id(), wid = threadIdx.x >> 6;
     uint32_t t[kHbScanPer], c[kHbScanPer], sum[4] = {0, 0, 0, 0};
 #pragma unroll
     for (uint32_t j = 0; j < kHbScanPer; ++j) {
@@ -417,21 +418,31 @@ __global__ void __launch_bounds__(1024) hashgrid_bwd_scan_kernel(HbPlan p, const
         sum[2] += c[j] > 1 ? c[j] : 0u;
         sum[3] += c[j] > 1 ? 1u : 0u;
     }
+    // inclusive scans within each wave, then across the 16 wave totals
+    uint32_t inc[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) part[q][threadIdx.x] = sum[q];
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024; d <<= 1) {
-        uint32_t o[4];
+    for (int q = 0; q < 4; ++q) {
+        inc[q] = sum[q];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) o[q] = threadIdx.x >= d ? part[q][threadIdx.x - d] : 0u;
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) part[q][threadIdx.x] += o[q];
-        __syncthreads();
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(inc[q], d, 64);
+            if (lane >= (uint32_t)d) inc[q] += o;
+        }
+        if (lane == 63) wsum[q][wid] = inc[q];
     }
-    uint32_t ex[4];
+    __syncthreads();
+    uint32_t ex[4], all[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ex[q] = part[q][threadIdx.x] - sum[q];
+    for (int q = 0; q < 4; ++q) {
+        uint32_t pre = 0, tot = 0;
+        for (uint32_t w = 0; w < 16; ++w) {
+            const uint32_t v = wsum[q][w];
+            pre += w < wid ? v : 0u;
+            tot += v;
+        }
+        ex[q] = pre + inc[q] - sum[q];
+        all[q] = tot;
+    }
 #pragma unroll
     for (uint32_t j = 0; j < kHbScanPer; ++j) {
         const uint32_t k = k0 + j;
@@ -446,10 +457,10 @@ __global__ void __launch_bounds__(1024) hashgrid_bwd_scan_kernel(HbPlan p, const
         ex[2] += c[j] > 1 ? c[j] : 0u;
         ex[3] += c[j] > 1 ? 1u : 0u;
     }
-    if (threadIdx.x == 1023) {
-        start[p.n_bins] = part[0][1023];
-        chunk_start[p.n_bins] = part[1][1023];
-        split_list[p.n_bins] = part[3][1023];
+    if (threadIdx.x == 0) {
+        start[p.n_bins] = all[0];
+        chunk_start[p.n_bins] = all[1];
+        split_list[p.n_bins] = all[3];
     }
 }
 
