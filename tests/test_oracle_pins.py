@@ -3,7 +3,9 @@ formulas: Random123 known-answer vectors, textbook bit interleaving, exact
 rational arithmetic, brute-force marching with numpy float32, closed forms of
 volume rendering, FP64 finite differences, the O(N^2) mip-NeRF 360 distortion
 sum, the Adam closed form, partition of unity / adjointness of the hash grid,
-SH orthonormality by exact quadrature. The reference ships no tests or golden
+SH orthonormality by exact quadrature, the camera model run backwards, Morton
+de-interleaving of occupancy points and known answers at the occupancy update
+rule's boundaries. The reference ships no tests or golden
 vectors for this path, so these checks are what pin each oracle function."""
 import ctypes
 import math
@@ -419,6 +421,131 @@ def test_occupancy_bits_and_popcount():
     assert oracle.popcount(bits) == int(np.unpackbits(bits).sum())
     d2 = oracle.occ_decay(dens, L, 0.95)
     assert np.array_equal(d2, dens * F32(0.95))
+
+
+def test_init_training_rays_camera_model_backwards():
+    """Training rays pinned by running the camera model backwards in float64:
+    the direction rotated into the camera frame and divided by its z gives the
+    pixel's ((x + offset + 0.5) / fx, (y + offset + 0.5) / fy), x = fmod(p, W)
+    and y = p / W left unfloored, p = n_pix * u; the origin lies on the near
+    plane along that direction; t and t_max meet the box (slab test) or
+    far - near; the target is texel floor(p) / 255 of image floor(i / rays per
+    image); rays that miss the box are dead."""
+    from turbonerf_amd import scenes
+
+    NC, W, H, R = 6, 48, 40, 3000
+    seed = 31
+    rng = np.random.default_rng(seed)
+    cams = scenes.sphere_cameras(seed, NC, 2.0, W, H, 50.0)
+    images = rng.integers(0, 256, size=(NC, H, W, 4), dtype=np.uint8)
+    rnd = rng.uniform(0, 1, R).astype(F32)
+    rr = oracle.init_training_rays(R, R, NC, W, H, R / NC, scenes.bbox(1.0), cams, images, rnd)
+    alive = rr["alive"].astype(bool)
+    ori, d = rr["ori"].reshape(3, R).astype(np.float64), rr["dir"].reshape(3, R).astype(np.float64)
+    img = (np.arange(R) * NC) // R
+    assert np.array_equal(rr["img_id"][alive], img[alive])
+    p = F32(W * H) * rnd
+    x, y = np.fmod(p, F32(W)).astype(np.float64), (p / F32(W)).astype(np.float64)
+    assert np.any(y[alive] != np.floor(y[alive]))
+    # which rays hit the box: the pinhole model in float64 and a slab test
+    hit = np.zeros(R, bool)
+    clear = np.zeros(R, bool)
+    for i in range(R):
+        cam = cams[img[i]]
+        M = np.array(cam.transform[:], np.float64).reshape(3, 4)
+        v = np.array([(x[i] + cam.offset_px[0] + 0.5) / cam.focal_length[0],
+                      (y[i] + cam.offset_px[1] + 0.5) / cam.focal_length[1], 1.0])
+        dm = M[:, :3] @ v
+        dm /= np.linalg.norm(dm)
+        om = M[:, 3] + cam.near_plane * np.linalg.norm(v) * dm
+        with np.errstate(divide="ignore"):
+            ta, tb = (-0.5 - om) / dm, (0.5 - om) / dm
+        a0, a1 = np.minimum(ta, tb).max(), np.maximum(ta, tb).min()
+        hit[i] = a1 >= a0 and min(cam.far_plane - cam.near_plane, a1) >= max(0.0, a0 + 1e-5)
+        clear[i] = abs(a1 - a0) > 1e-4
+    assert np.array_equal(alive[clear], hit[clear])
+    # t and t_max from a slab test on the oracle's own origin and direction
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t_a, t_b = (-0.5 - ori) / d, (0.5 - ori) / d
+    t0, t1 = np.minimum(t_a, t_b).max(0), np.maximum(t_a, t_b).min(0)
+    checked = 0
+    for i in np.nonzero(clear & alive)[0]:
+        cam = cams[img[i]]
+        M = np.array(cam.transform[:], np.float64).reshape(3, 4)
+        checked += 1
+        dc = M[:, :3].T @ d[:, i]
+        want = np.array([(x[i] + cam.offset_px[0] + 0.5) / cam.focal_length[0],
+                         (y[i] + cam.offset_px[1] + 0.5) / cam.focal_length[1]])
+        np.testing.assert_allclose(dc[:2] / dc[2], want, rtol=1e-5, atol=2e-6)
+        oc = M[:, :3].T @ (ori[:, i] - M[:, 3])
+        assert abs(oc[2] - cam.near_plane) < 1e-6
+        np.testing.assert_allclose(oc[:2] / oc[2], want, rtol=1e-4, atol=1e-5)
+        assert abs(rr["t"][i] - max(0.0, t0[i] + 1e-5)) < 2e-6
+        assert abs(rr["tmax"][i] - min(cam.far_plane - cam.near_plane, t1[i])) < 2e-6
+        lp = min(int(np.floor(p[i])), W * H - 1)
+        np.testing.assert_array_equal(rr["pix_rgba"].reshape(4, R)[:, i],
+                                      images[img[i]].reshape(-1, 4)[lp].astype(F32) / F32(255))
+    assert checked > R // 8 and (~alive).sum() > 0
+
+
+def _morton_decode(m):
+    m = np.asarray(m, np.uint64)
+    out = []
+    for axis in range(3):
+        v = np.zeros_like(m)
+        for b in range(10):
+            v |= ((m >> np.uint64(3 * b + axis)) & np.uint64(1)) << np.uint64(b)
+        out.append(v.astype(np.int64))
+    return out
+
+
+def test_occ_sample_points_lie_in_their_cells():
+    """Jittered occupancy points: for cell idx (Morton, decoded here by bit
+    de-interleaving) of cascade k, the world point (pos - 0.5) * aabb minus the
+    cell corner ((i - res/2) * 2^k / res per axis), over the cell size, is the
+    uniform draw; positions are not clamped to [0, 1]."""
+    res, n, start, level, aabb = 128, 4096, 777777, 3, 4.0
+    rng = np.random.default_rng(9)
+    rnd = rng.uniform(0, 1, 3 * n).astype(F32)
+    pos = oracle.occ_sample_points(n, n, start, res, level, 1.0 / aabb, rnd).reshape(3, n).astype(np.float64)
+    idx = np.arange(start, start + n)
+    ijk = _morton_decode(idx)
+    voxel = 2.0 ** level / res
+    for a in range(3):
+        corner = (ijk[a] - res // 2) * voxel
+        u = ((pos[a] - 0.5) * aabb - corner) / voxel
+        np.testing.assert_allclose(u, rnd[a * n:(a + 1) * n], atol=2e-5)
+    assert np.any(pos < 0.0) or np.any(pos > 1.0)
+
+
+def test_occ_update_density_subset_rule():
+    """Known answers at the rule's boundaries: after warm-up a cell is updated
+    when r < 0.25, or r <= 0.5 and its bit at this cascade is set, never when
+    r > 0.5; during warm-up always; the update is max(old, exp(min(s, 12) - 1))
+    at this cascade only."""
+    vol, L, level, start, n = 4096, 3, 1, 100, 2000
+    rng = np.random.default_rng(10)
+    r = rng.uniform(0, 1, n).astype(F32)
+    r[:8] = [0.25, 0.5, np.nextafter(F32(0.5), F32(1)), np.nextafter(F32(0.25), F32(0)), 0.0, 1.0, 0.5, 0.25]
+    bits = rng.integers(0, 8, vol).astype(np.uint8)
+    bit = np.uint8(1 << level)
+    for k, on in ((0, False), (1, True), (2, True), (6, False), (7, True)):  # boundary cases, both ways
+        bits[start + k] = (bits[start + k] | bit) if on else (bits[start + k] & ~bit)
+    raw = rng.uniform(-3, 15, n).astype(F32)
+    dens = rng.uniform(0, 20, L * vol).astype(F32)
+    sig = np.exp(np.minimum(raw.astype(np.float64), 12.0) - 1.0)
+    occ = ((bits[start:start + n] >> level) & 1).astype(bool)
+    for warm in (True, False):
+        got = oracle.occ_update_density(n, start, vol, level, warm, r, raw, bits, dens)
+        sel = np.ones(n, bool) if warm else (r < 0.25) | ((r <= 0.5) & occ)
+        if not warm:
+            assert list(sel[:8]) == [False, True, False, True, True, False, False, True]
+        old = dens[level * vol + start: level * vol + start + n].astype(np.float64)
+        want = np.where(sel, np.maximum(old, sig), old)
+        np.testing.assert_allclose(got[level * vol + start: level * vol + start + n], want, rtol=1e-6)
+        keep = np.ones(L * vol, bool)
+        keep[level * vol + start: level * vol + start + n] = False
+        assert np.array_equal(got[keep], dens[keep])
 
 
 # --------------------------------------------------------------- hash grid
