@@ -285,6 +285,70 @@ def render_rate(tn, synthetic, scene, trainer_tuple, n_frames=5):
     return RES * RES / dt / 1e6, 1e3 * dt
 
 
+def large_scene(tn, synthetic):
+    """Large-scene inference: a 16-unit box (5 cascades of 128^3) with 256
+    small Gaussian blobs. The occupancy update of the random-init network
+    (5 x 2^21 jittered points, density inference, the random-subset rule)
+    is timed; the grid is then set from the blob field at cell centres
+    (threshold 0.01) and a 1920x1080 frame of the network is rendered from
+    a camera 24 units out."""
+    import numpy as np
+    import torch
+
+    from turbonerf_amd import ops, scenes
+
+    aabb, levels, seed = 16.0, 5, 404
+    blobs = scenes.blob_field(seed, 256, aabb, sigma_range=(0.004, 0.012))
+    bits = torch.zeros(128 ** 3, dtype=torch.uint8, device="cuda")
+    dens = torch.zeros(levels * 128 ** 3, dtype=torch.float32, device="cuda")
+    ops.blob_density_grid(ops.grid_struct(bits, dens, levels), torch.from_numpy(blobs.ravel()).cuda())
+    dens_h = dens.cpu().numpy()
+    cams = scenes.sphere_cameras(seed, 4, 24.0, 64, 64, 50.0, near=0.05, far=100.0)
+    mgr = tn.NeRFManager()
+    nerf = mgr.create()
+    ds = tn.Dataset(tn.BoundingBox(aabb), synthetic.to_pyturbonerf_cameras(tn, cams),
+                    np.zeros((4, 64, 64, 4), np.uint8))
+    nerf.attach_dataset(ds)
+    cfg = tn.TrainingConfig()
+    cfg.network.precision = 1
+    tr = tn.Trainer(nerf, cfg)
+    tr.setup_data(batch_size=BATCH)
+    tr.load_images()
+    tr.train_step()  # a NeRF renders once it has trained a step
+    tr.update_occupancy_grid(256)  # warm-up (allocations)
+    tr.synchronize()
+    t0 = time.perf_counter()
+    n_upd = 4
+    for k in range(n_upd):
+        tr.update_occupancy_grid(272 + 16 * k)
+    tr.synchronize()
+    upd_ms = 1e3 * (time.perf_counter() - t0) / n_upd
+    tr.set_occupancy_from_density(dens_h, 0.01)
+    occ = float((tr.get_occupancy_bitfield() != 0).mean())
+    W, H = 1920, 1080
+    cam = synthetic.to_pyturbonerf_cameras(tn, scenes.sphere_cameras(seed + 1, 1, 24.0, W, H, 50.0))[0]
+    buf = tn.CPURenderBuffer()
+    r = tn.Renderer()
+    req = lambda: tn.RenderRequest(cam, [tn.Renderable(nerf)], buf)
+    r.submit(req())  # warm-up
+    n_frames = 3
+    t0 = time.perf_counter()
+    for _ in range(n_frames):
+        r.submit(req())
+    dt = (time.perf_counter() - t0) / n_frames
+    samples = int(r.last_sample_count)
+    alpha = float(np.asarray(buf.get_rgba())[..., 3].mean())
+    tr.teardown()
+    out = dict(occupancy_update_ms=round(upd_ms, 3), occupied_cells=round(occ, 4),
+               render_mrays_per_s=round(W * H / dt / 1e6, 2), render_ms_per_frame=round(1e3 * dt, 2),
+               render_samples_per_frame=samples, render_msamples_per_s=round(samples / dt / 1e6, 1),
+               mean_alpha=round(alpha, 4))
+    log(f"[bench large scene] occupancy update {upd_ms:.2f} ms (5 cascades x 2^21 points); render "
+        f"{W * H / dt / 1e6:.1f} Mrays/s ({1e3 * dt:.1f} ms per {W}x{H} frame, {samples} samples, "
+        f"{occ:.3f} of cells occupied, mean alpha {alpha:.3f})")
+    return out
+
+
 def cpu_baseline(scene, bitfield, rays_per_step_gpu, args):
     """The oracle (CPU restatement of the same step, OpenMP over rays and
     samples, fp32) on a bounded sample: one training step of
@@ -494,6 +558,8 @@ def main():
         extra["render_mrays_per_s"] = round(mr, 2)
         extra["render_ms_per_frame"] = round(ms, 3)
         log(f"[bench] render {mr:.1f} Mrays/s ({ms:.2f} ms per {RES}x{RES} frame)")
+    if not args.no_render:
+        extra["large_scene_aabb16_5cascades_1920x1080"] = large_scene(tn, synthetic)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, main["bitfield"], main["rays_per_step"], args)

@@ -67,7 +67,11 @@ uint64_t Renderer::render_pixels(hipStream_t s, const RenderRequest& req, const 
     uint32_t info_h[4];
     while (p < end) {
         if (req.is_canceled()) break;
-        const uint32_t m = std::min(end - p, R);
+        // a pass marches only about as many rays as the sample budget takes
+        // (the rays past the kept prefix would be marched again next pass)
+        uint32_t m = std::min(end - p, R);
+        if (samples_per_ray > 0.0)
+            m = std::min(m, (uint32_t)std::clamp(1.05 * (double)S / samples_per_ray, 256.0, (double)R));
         TN_TRY(tn_render_rays(m, p, R, cam, bbox, to_local, ws.ray_ori.data(), ws.ray_dir.data(), ws.ray_t.data(),
                               ws.ray_tmax.data(), ws.pixel.data(), ws.alive.data(), ts));
         TN_TRY(tn_march_count(m, nullptr, R, bbox, grid, NeRFConstants::cone_angle, dt_min, dt_max,
@@ -94,6 +98,7 @@ uint64_t Renderer::render_pixels(hipStream_t s, const RenderRequest& req, const 
         }
         p += info_h[0];
         n_evals += info_h[1];
+        samples_per_ray = std::max(1.0, (double)info_h[1] / (double)info_h[0]);
     }
     return n_evals;
 }

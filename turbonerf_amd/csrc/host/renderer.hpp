@@ -98,6 +98,9 @@ struct Renderer {
         float min_transmittance = NeRFConstants::min_transmittance;
     };
     RenderingWorkspace ws;
+    // samples per ray of the last pass, to size the next pass's rays to the
+    // sample budget (0: not known yet)
+    double samples_per_ray = 0.0;
     // Render pixels [start, start + n) of the request's camera into `image`
     // (device, interleaved RGBA). Returns the number of sample evaluations.
     uint64_t render_pixels(hipStream_t s, const RenderRequest& req, const NeRFRenderable& r, uint32_t start, uint32_t n,
