@@ -7,7 +7,9 @@
   density activation goes through the device exp); threshold 0.01, the
   5-bit bitfield and its popcount bit-exact given identical densities;
 * training rays through the cascades (dt from sqrt(3)/1024 to 16x that,
-  cone 1/256): per-ray counts, offsets and every per-sample output bit-exact.
+  cone 1/256): per-ray counts, offsets and every per-sample output bit-exact,
+  with few rays (a wave per ray) and with many (lanes skip empty space for
+  their own rays).
 """
 import math
 
@@ -94,13 +96,17 @@ def test_cascade_bitfield_and_update(field):
     assert ops.popcount(bits) == oracle.popcount(bits_ref)
 
 
-def test_cascade_marching(field):
+@pytest.mark.parametrize("n_rays", [2048, 40960])
+def test_cascade_marching(field, n_rays):
+    """2048 rays: a wave marches one ray at a time. 40960 rays (five per wave
+    of the bounded grid): lanes take their own rays through empty cells and
+    the wave takes each ray's occupied runs together. Both bit-exact."""
     import torch
 
     import oracle
     from turbonerf_amd import ops, scenes
 
-    n_rays, n_cams, W = 2048, 8, 128
+    n_cams, W = 8, 128
     cams = scenes.sphere_cameras(SEED, n_cams, 0.9 * AABB, W, W, 60.0)
     rng = np.random.default_rng(SEED)
     images = rng.integers(0, 256, size=(n_cams, W, W, 4), dtype=np.uint8)

@@ -191,6 +191,118 @@ TN_DEVICE void march_count_ray(uint32_t i, uint32_t B, const tn_bbox_t& bbox, co
     }
 }
 
+TN_DEVICE float lane_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+TN_DEVICE uint32_t lane_u(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+
+// Many rays per wave (a render pass or a sparse scene): lanes [0, K) own rays
+// first + lane. Each lane takes its ray through empty cells by the serial DDA
+// itself (reading its cells' bits), so the long skips of rays through mostly
+// empty space run side by side; a ray that reaches an occupied cell is then
+// taken by the whole wave for its occupied runs (one ray at a time, as in
+// march_count_ray), after which its lane resumes. Every ray follows the same
+// recurrence as march_count_ray, step for step.
+struct LaneRayState {
+    MarchRay r;
+    float idx, idy, idz, t, t_max;
+    uint32_t n;
+    bool done, ready;
+};
+
+TN_DEVICE void lane_skip_step(const MarchGrid& g, const tn_bbox_t& bbox, float cone, float dt_min, float dt_max,
+                              uint32_t max_steps, LaneRayState& q) {
+    if (!(q.t < q.t_max && q.n < max_steps)) {
+        q.done = true;
+        return;
+    }
+    const float x = q.r.ox + q.t * q.r.dx, y = q.r.oy + q.t * q.r.dy, z = q.r.oz + q.t * q.r.dz;
+    if (!bbox_contains(bbox, x, y, z)) {
+        q.done = true;
+        return;
+    }
+    const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(q.t, cone, dt_min, dt_max));
+    if ((g.bits[voxel_morton_index(g.res_f, g.lim, lvl, x, y, z)] >> lvl) & 1u)
+        q.ready = true;
+    else
+        q.t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, q.r.dx, q.r.dy, q.r.dz, q.idx, q.idy, q.idz, dt_min, lvl);
+}
+
+TN_DEVICE LaneRayState lane_ray_state(bool active, uint32_t i, uint32_t B, const float* __restrict__ ori,
+                                      const float* __restrict__ dir, const float* __restrict__ ray_t, float t_max) {
+    LaneRayState q;
+    q.r = active ? MarchRay{ori[i], ori[i + B], ori[i + 2 * B], dir[i], dir[i + B], dir[i + 2 * B]}
+                 : MarchRay{0.0f, 0.0f, 0.0f, 1.0f, 1.0f, 1.0f};
+    q.idx = 1.0f / q.r.dx;
+    q.idy = 1.0f / q.r.dy;
+    q.idz = 1.0f / q.r.dz;
+    q.t = active ? ray_t[i] : 0.0f;
+    q.t_max = t_max;
+    q.n = 0;
+    q.done = !active;
+    q.ready = false;
+    return q;
+}
+
+TN_DEVICE MarchRay lane_ray(const MarchRay& r, int l) {
+    return MarchRay{lane_f(r.ox, l), lane_f(r.oy, l), lane_f(r.oz, l), lane_f(r.dx, l), lane_f(r.dy, l),
+                    lane_f(r.dz, l)};
+}
+
+TN_DEVICE void march_count_lanes(uint32_t first, uint32_t K, uint32_t n_rays, uint32_t B, const tn_bbox_t& bbox,
+                                 const MarchGrid& g, float cone, float dt_min, float dt_max, uint32_t max_steps,
+                                 const float* __restrict__ dir, const float* __restrict__ ray_tmax,
+                                 uint8_t* __restrict__ ray_alive, const float* __restrict__ ori,
+                                 const float* __restrict__ ray_t, uint32_t* __restrict__ n_steps) {
+    const uint32_t lane = __lane_id();
+    const uint32_t i = first + lane;
+    const bool own = lane < K && i < n_rays;
+    const bool active = own && ray_alive[i] != 0;
+    LaneRayState q = lane_ray_state(active, i, B, ori, dir, ray_t, active ? ray_tmax[i] : 0.0f);
+    while (__ballot(!q.done) != 0ull) {
+        if (!q.done && !q.ready) lane_skip_step(g, bbox, cone, dt_min, dt_max, max_steps, q);
+        uint64_t rb = __ballot(q.ready);
+        while (rb != 0ull) {
+            const int L = __builtin_ctzll(rb);
+            rb &= rb - 1ull;
+            const MarchRay r = lane_ray(q.r, L);
+            float t = lane_f(q.t, L);
+            const float t_max = lane_f(q.t_max, L);
+            uint32_t n = lane_u(q.n, L);
+            for (;;) {  // the point at t is occupied and within the limits, so the first run takes a step
+                float t_lane;
+                bool empty;
+                const uint32_t f = occupied_run(g, bbox, r, t, t_max, max_steps - n, cone, dt_min, dt_max, t_lane, empty);
+                if (f == 0u) break;
+                t = after_run(f, t_lane, cone, dt_min, dt_max);
+                n += f;
+                if (f < 64u || !(t < t_max && n < max_steps)) break;
+            }
+            if ((int)lane == L) {
+                q.t = t;
+                q.n = n;
+                q.ready = false;  // the lane's next step re-tests the point where the run stopped
+            }
+        }
+    }
+    if (own) {
+        if (!active) {
+            n_steps[i] = 0;
+        } else {
+            if (q.n == 0) ray_alive[i] = 0;
+            n_steps[i] = q.n;
+        }
+    }
+}
+
+// Rays per wave from which lanes take rays through empty space themselves;
+// below it a wave marches one ray at a time (march_count_ray), skipping with
+// its wave cache, which is faster per ray.
+constexpr uint32_t kLaneRaysMin = 4;
+
+inline TN_DEVICE uint32_t rays_per_wave(uint32_t n_rays, uint32_t n_waves) {
+    const uint32_t k = (n_rays + n_waves - 1) / n_waves;
+    return k < kLaneRaysMin ? 1u : min(k, 64u);
+}
+
 __global__ void __launch_bounds__(64 * kMarchWaves)
 march_count_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays, uint32_t B, tn_bbox_t bbox,
                    tn_occupancy_grid_t grid, float cone, float dt_min, float dt_max, uint32_t max_steps,
@@ -198,9 +310,18 @@ march_count_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays, u
                    const float* __restrict__ ori, const float* __restrict__ ray_t, uint32_t* __restrict__ n_steps) {
     const uint32_t n_rays = d_n_rays ? min(*d_n_rays, n_rays_max) : n_rays_max;
     const uint32_t n_waves = gridDim.x * kMarchWaves;
+    const uint32_t wave = blockIdx.x * kMarchWaves + (threadIdx.x >> 6);
     const MarchGrid g = make_grid(grid);
-    for (uint32_t i = blockIdx.x * kMarchWaves + (threadIdx.x >> 6); i < n_rays; i += n_waves)
-        march_count_ray(i, B, bbox, g, cone, dt_min, dt_max, max_steps, dir, ray_tmax, ray_alive, ori, ray_t, n_steps);
+    const uint32_t K = rays_per_wave(n_rays, n_waves);
+    if (K == 1) {
+        for (uint32_t i = wave; i < n_rays; i += n_waves)
+            march_count_ray(i, B, bbox, g, cone, dt_min, dt_max, max_steps, dir, ray_tmax, ray_alive, ori, ray_t,
+                            n_steps);
+    } else {
+        for (uint32_t first = wave * K; first < n_rays; first += n_waves * K)
+            march_count_lanes(first, K, n_rays, B, bbox, g, cone, dt_min, dt_max, max_steps, dir, ray_tmax, ray_alive,
+                              ori, ray_t, n_steps);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -363,6 +484,71 @@ TN_DEVICE void march_generate_ray(uint32_t i, uint32_t B, const tn_bbox_t& bbox,
     }
 }
 
+// march_generate_ray for rays first + lane, lanes [0, K) (see march_count_lanes)
+TN_DEVICE void march_generate_lanes(uint32_t first, uint32_t K, uint32_t n_rays, uint32_t B, const tn_bbox_t& bbox,
+                                    const MarchGrid& g, float dt_min, float dt_max, float cone,
+                                    const float* __restrict__ ori, const float* __restrict__ dir,
+                                    const float* __restrict__ ray_t, const uint32_t* __restrict__ ray_offset,
+                                    const uint8_t* __restrict__ ray_alive, const uint32_t* __restrict__ n_ray_steps,
+                                    uint32_t* __restrict__ out_ray, float* __restrict__ out_t) {
+    const uint32_t lane = __lane_id();
+    const uint32_t i = first + lane;
+    const bool active = lane < K && i < n_rays && ray_alive[i] != 0;
+    const uint32_t ns = active ? n_ray_steps[i] : 0u;
+    const uint32_t off = active ? ray_offset[i] : 0u;
+    LaneRayState q = lane_ray_state(active, i, B, ori, dir, ray_t, INFINITY);
+    while (__ballot(!q.done) != 0ull) {
+        if (!q.done && !q.ready) {
+            if (!(q.n < ns)) {
+                q.done = true;
+            } else {
+                const float x = q.r.ox + q.t * q.r.dx, y = q.r.oy + q.t * q.r.dy, z = q.r.oz + q.t * q.r.dz;
+                if (!bbox_contains(bbox, x, y, z)) {
+                    // the count pass stops at the box boundary, so every counted
+                    // sample is inside; one outside would be emitted without advancing
+                    out_t[off + q.n] = q.t;
+                    out_ray[off + q.n] = i;
+                    ++q.n;
+                } else {
+                    const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(q.t, cone, dt_min, dt_max));
+                    if ((g.bits[voxel_morton_index(g.res_f, g.lim, lvl, x, y, z)] >> lvl) & 1u)
+                        q.ready = true;
+                    else
+                        q.t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, q.r.dx, q.r.dy, q.r.dz, q.idx, q.idy,
+                                                q.idz, dt_min, lvl);
+                }
+            }
+        }
+        uint64_t rb = __ballot(q.ready);
+        while (rb != 0ull) {
+            const int L = __builtin_ctzll(rb);
+            rb &= rb - 1ull;
+            const MarchRay r = lane_ray(q.r, L);
+            float t = lane_f(q.t, L);
+            uint32_t n = lane_u(q.n, L);
+            const uint32_t nsL = lane_u(ns, L), offL = lane_u(off, L), iL = lane_u(i, L);
+            for (;;) {
+                float t_lane;
+                bool empty;
+                const uint32_t f = occupied_run(g, bbox, r, t, INFINITY, nsL - n, cone, dt_min, dt_max, t_lane, empty);
+                if (f == 0u) break;
+                if (lane < f) {  // one coalesced row of samples
+                    out_t[offL + n + lane] = t_lane;
+                    out_ray[offL + n + lane] = iL;
+                }
+                t = after_run(f, t_lane, cone, dt_min, dt_max);
+                n += f;
+                if (f < 64u || n >= nsL) break;
+            }
+            if ((int)lane == L) {
+                q.t = t;
+                q.n = n;
+                q.ready = false;
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(64 * kMarchWaves)
 march_generate_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ info, uint32_t B, tn_bbox_t bbox,
                       tn_occupancy_grid_t grid, float dt_min, float dt_max, float cone, const float* __restrict__ ori,
@@ -372,11 +558,19 @@ march_generate_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ info, ui
                       float* __restrict__ out_t) {
     const uint32_t n_rays = min(info[0], n_rays_max);
     const uint32_t n_waves = gridDim.x * kMarchWaves;
+    const uint32_t wave = blockIdx.x * kMarchWaves + (threadIdx.x >> 6);
     const MarchGrid g = make_grid(grid);
-    for (uint32_t i = blockIdx.x * kMarchWaves + (threadIdx.x >> 6); i < n_rays; i += n_waves) {
-        if (ray_alive[i])
-            march_generate_ray(i, B, bbox, g, dt_min, dt_max, cone, ori, dir, ray_t, ray_offset, n_ray_steps,
-                               out_ray, out_t);
+    const uint32_t K = rays_per_wave(n_rays, n_waves);
+    if (K == 1) {
+        for (uint32_t i = wave; i < n_rays; i += n_waves) {
+            if (ray_alive[i])
+                march_generate_ray(i, B, bbox, g, dt_min, dt_max, cone, ori, dir, ray_t, ray_offset, n_ray_steps,
+                                   out_ray, out_t);
+        }
+    } else {
+        for (uint32_t first = wave * K; first < n_rays; first += n_waves * K)
+            march_generate_lanes(first, K, n_rays, B, bbox, g, dt_min, dt_max, cone, ori, dir, ray_t, ray_offset,
+                                 ray_alive, n_ray_steps, out_ray, out_t);
     }
 }
 
