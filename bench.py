@@ -88,6 +88,8 @@ def parse():
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--no-extra-configs", action="store_true",
+                    help="skip the distortion-loss and eight-scene training measurements")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="profiling run: render the targets, then only --warmup and this many mixed-precision "
                          "training steps (no occupancy updates, second pass, render or CPU leg), and exit")
@@ -283,6 +285,78 @@ def render_rate(tn, synthetic, scene, trainer_tuple, n_frames=5):
         r.submit(req())
     dt = (time.perf_counter() - t0) / n_frames
     return RES * RES / dt / 1e6, 1e3 * dt
+
+
+def timed_steps(tr, steps):
+    """(samples/s, ms/step) of `steps` training steps without host
+    synchronisation inside."""
+    tr.synchronize()
+    s0 = tr.samples_trained()
+    t0 = time.perf_counter()
+    tr.train_steps(steps, read_metrics=False)
+    tr.synchronize()
+    wall = time.perf_counter() - t0
+    return (tr.samples_trained() - s0) / wall, 1e3 * wall / steps
+
+
+def distortion_rate(tr, steps):
+    """The trained mixed-precision NeRF with the mip-NeRF 360 distortion loss
+    switched on (the reference's lambda 0.01; its forward and backward run
+    every step)."""
+    tr.use_distortion_loss = True
+    tr.train_steps(16, read_metrics=False)
+    sps, ms = timed_steps(tr, steps)
+    m = tr.train_step()
+    tr.use_distortion_loss = False
+    log(f"[bench distortion] {sps / 1e6:.2f} M samples/s, {ms:.3f} ms/step (loss with distortion {m.loss:.5f})")
+    return dict(samples_per_s=round(sps, 1), ms_per_step=round(ms, 4))
+
+
+def multi_scene(tn, synthetic, steps):
+    """Eight NeRFs in one manager, hash tables of 2^15 ... 2^21 entries per
+    level, blob fields of 4 to 32 blobs, batches summing to 2^19 samples,
+    trained in interleaved steps: aggregate samples/s after 128 warm-up
+    steps each."""
+    log2_t = [15, 16, 17, 18, 19, 20, 21, 21]
+    n_blobs = [4, 8, 12, 16, 20, 24, 28, 32]
+    batch = [16384, 32768, 65536, 131072, 131072, 65536, 49152, 32768]
+    mgr = tn.NeRFManager()
+    trainers = []
+    for k in range(8):
+        blobs, cams, images = synthetic.blob_dataset(seed=100 + k, n_cams=16, width=128, height=128,
+                                                     n_blobs=n_blobs[k])
+        nerf = mgr.create()
+        nerf.attach_dataset(tn.Dataset(tn.BoundingBox(1.0), synthetic.to_pyturbonerf_cameras(tn, cams), images))
+        cfg = tn.TrainingConfig()
+        cfg.network.precision = 1
+        cfg.network.log2_hashmap_size = log2_t[k]
+        cfg.network.beta2 = 0.99
+        cfg.rng_seed = 7 + k
+        tr = tn.Trainer(nerf, cfg)
+        tr.setup_data(batch_size=batch[k])
+        tr.load_images()
+        trainers.append(tr)
+    for i in range(8):
+        for tr in trainers:
+            tr.train_steps(16, read_metrics=False)
+            tr.update_occupancy_grid(16 * (i + 1))
+    for tr in trainers:
+        tr.synchronize()
+    s0 = sum(tr.samples_trained() for tr in trainers)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        for tr in trainers:
+            tr.train_steps(1, read_metrics=False)
+    for tr in trainers:
+        tr.synchronize()
+    wall = time.perf_counter() - t0
+    samples = sum(tr.samples_trained() for tr in trainers) - s0
+    for tr in trainers:
+        tr.teardown()
+    log(f"[bench multi-scene] {samples / wall / 1e6:.2f} M samples/s over 8 scenes ({1e3 * wall / steps:.3f} ms per "
+        f"round of 8 steps, {samples / steps:.0f} samples per round)")
+    return dict(samples_per_s=round(samples / wall, 1), ms_per_round=round(1e3 * wall / steps, 4),
+                samples_per_round=round(samples / steps, 1), log2_hashmap_size=log2_t, batch=batch)
 
 
 def large_scene(tn, synthetic):
@@ -560,6 +634,10 @@ def main():
         log(f"[bench] render {mr:.1f} Mrays/s ({ms:.2f} ms per {RES}x{RES} frame)")
     if not args.no_render:
         extra["large_scene_aabb16_5cascades_1920x1080"] = large_scene(tn, synthetic)
+    if not args.no_extra_configs:
+        if main_p == 1:
+            extra["distortion_loss_mixed"] = distortion_rate(main["trainer"][2], args.steps)
+        extra["multi_scene_8"] = multi_scene(tn, synthetic, 50)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, main["bitfield"], main["rays_per_step"], args)
