@@ -134,8 +134,10 @@ int tn_composite_forward(int dtype, const uint32_t* info, uint32_t stride, uint3
  * mipNeRF360_distortion_loss_forward/backward_kernel,
  * src/utils/training-kernels-fused.cuh:13-224, and its orchestration
  * src/core/nerf-network.cu:605-666; lambda is the reference's constant 0.01,
- * here a parameter). dtype 0 = fp32 density, 1 = fp16, 2 = fp64 accumulation. */
-size_t tn_distortion_scratch_bytes(uint32_t n_rays_max);
+ * here a parameter). dtype 0 = fp32 density, 1 = fp16, 2 = fp64 accumulation.
+ * The scratch (256-byte aligned, tn_distortion_scratch_bytes for the ray and
+ * sample capacities) holds per-ray totals and per-64-sample-chunk sums. */
+size_t tn_distortion_scratch_bytes(uint32_t n_rays_max, uint32_t sample_stride);
 int tn_distortion(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t sample_stride,
     float lambda, const uint32_t* ray_steps, const uint32_t* ray_offset, const void* density,
     const float* m_norm, const float* dt_norm, float* ray_loss, float* dL_ds, void* scratch,

@@ -11,9 +11,9 @@ pytestmark = pytest.mark.gpu
 LAMBDA = 1e-3
 
 
-def make(seed=21, n_rays=64):
+def make(seed=21, n_rays=64, mean=48):
     rng = np.random.default_rng(seed)
-    steps = np.clip(rng.geometric(1 / 48, n_rays), 1, 1024).astype(np.uint32)
+    steps = np.clip(rng.geometric(1 / mean, n_rays), 1, 1024).astype(np.uint32)
     offs = np.concatenate([[0], np.cumsum(steps)[:-1]]).astype(np.uint32)
     ns = int(steps.sum())
     s = (np.log(rng.lognormal(0, 1.5, ns)) + 1.0).astype(np.float32)
@@ -42,7 +42,7 @@ def run_gpu(b, fp64=False, half=False):
     info = torch.tensor([nr, ns, 0, 0], dtype=torch.int32, device="cuda")
     loss = torch.zeros(nr, dtype=torch.float32, device="cuda")
     grad = torch.zeros(ns, dtype=torch.float32, device="cuda")
-    nb = query("tn_distortion_scratch_bytes", nr)
+    nb = query("tn_distortion_scratch_bytes", nr, ns)
     scratch = torch.empty(nb, dtype=torch.uint8, device="cuda")
     dens = T(b["s"], torch.float16 if half else torch.float32)
     code = 2 if fp64 else (1 if half else 0)
@@ -79,6 +79,31 @@ def test_distortion_vs_oracle(cuda):
     l, g = run_gpu(b, half=True)
     np.testing.assert_allclose(l, l_ref, rtol=1e-4)
     assert norm_rel(g, g_ref) < 1e-4
+
+
+def test_distortion_long_rays(cuda):
+    """Rays of about 400 samples (up to 1024), each spanning many 64-sample
+    chunks, and a zero-length ray: loss and gradient against the oracle in
+    fp32 and fp64 accumulation."""
+    import oracle
+
+    b = make(seed=23, n_rays=24, mean=400)
+    b["steps"][5] = 0
+    b["offs"] = np.concatenate([[0], np.cumsum(b["steps"])[:-1]]).astype(np.uint32)
+    ns = int(b["steps"].sum())
+    for k in ("s", "m", "d"):
+        b[k] = b[k][:ns]
+    b["ns"] = ns
+    l_ref, g_ref = oracle.distortion(b["n_rays"], b["steps"], b["offs"], b["s"].astype(np.float64),
+                                     b["m"].astype(np.float64), b["d"].astype(np.float64), LAMBDA)
+    assert l_ref[5] == 0.0
+    for fp64 in (False, True):
+        l, g = run_gpu(b, fp64)
+        eg = norm_rel(g, g_ref)
+        print(f"[distortion long fp64={fp64}] loss max rel "
+              f"{np.max(np.abs(l - l_ref) / np.maximum(np.abs(l_ref), 1e-30)):.2e} grad {eg:.2e}", flush=True)
+        np.testing.assert_allclose(l, l_ref, rtol=1e-5, atol=1e-12)
+        assert eg < (1e-7 if fp64 else 1e-4)
 
 
 def test_distortion_fd(cuda):

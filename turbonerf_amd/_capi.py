@@ -133,7 +133,7 @@ _SIGS: dict[str, tuple] = {
     "tn_cast_params": (C.c_int, [u64, vp, C.c_int, vp, vp]),
     "tn_render_rays": (C.c_int, [u32, u32, u32, Camera, BBox, vp, vp, vp, vp, vp, vp, vp, vp]),
     "tn_render_write_rgba": (C.c_int, [u32, vp, u32, vp, vp, vp, vp, vp]),
-    "tn_distortion_scratch_bytes": (sz, [u32]),
+    "tn_distortion_scratch_bytes": (sz, [u32, u32]),
     "tn_distortion": (C.c_int, [C.c_int, vp, u32, u32, f32, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]),
 }
 

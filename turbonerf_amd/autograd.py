@@ -123,7 +123,7 @@ class DistortionLoss(torch.autograd.Function):
         info = torch.tensor([n_rays, S, 0, 0], dtype=torch.int32, device=density.device)
         loss = torch.zeros(n_rays, dtype=torch.float32, device=density.device)
         grad = torch.zeros(S, dtype=torch.float32, device=density.device)
-        nb = query("tn_distortion_scratch_bytes", n_rays)
+        nb = query("tn_distortion_scratch_bytes", n_rays, S)
         scratch = torch.empty(max(nb, 4), dtype=torch.uint8, device=density.device)
         call("tn_distortion", code, ptr(info), n_rays, S, lam, ptr(steps), ptr(offsets), ptr(density), ptr(m_norm),
              ptr(dt_norm), ptr(loss), ptr(grad), ptr(scratch), nb, stream_ptr())

@@ -78,7 +78,7 @@ void NetworkWorkspace::enlarge(uint32_t B, uint32_t R, int prec) {
     ray_dist_loss.resize(ray_capacity);
     loss_out.resize(2);
     composite_scratch.resize(tn_composite_scratch_bytes(ray_capacity, batch_size));
-    distortion_scratch.resize(tn_distortion_scratch_bytes(ray_capacity));
+    distortion_scratch.resize(tn_distortion_scratch_bytes(ray_capacity, batch_size));
     mlp_ws.resize(std::max<size_t>(tn_mlp_backward_workspace_bytes(batch_size), 4));
 }
 

@@ -151,25 +151,6 @@ TN_DEVICE SampleState<R> sample_state(bool valid, int ray, const ChunkIn& in, in
 // w c_b, w) and a carry-in pair (sigma*dt prefix, w g prefix)
 constexpr int kChunkRec = 5;
 
-// Walk state at chunk start c: the ray holding sample c is carried into the
-// chunk when it started earlier; otherwise the walk starts at the first ray
-// beginning at c (zero-length rays there included).
-TN_DEVICE RayWalk chunk_walk(uint32_t c, uint32_t n_rays, const uint32_t* __restrict__ offs,
-                             const uint32_t* __restrict__ steps) {
-    RayWalk wk;
-    const uint32_t r_next = wave_lower_bound(offs, n_rays, c + 1);  // offs[0] = 0 <= c, so r_next >= 1
-    const uint32_t rf = r_next - 1;
-    const uint32_t of = offs[rf];
-    if (of < c) {
-        wk.carry_ray = (int)rf;
-        wk.carry_end = (int)(of + steps[rf]);
-        wk.r_scan = r_next;
-    } else {
-        wk.r_scan = wave_lower_bound(offs, n_rays, c);
-    }
-    return wk;
-}
-
 template <typename R>
 TN_DEVICE void put_rec(Acc<R>* __restrict__ dst, const Acc<R>& P, const Acc<R>& sr, const Acc<R>& sg,
                        const Acc<R>& sb, const Acc<R>& sa) {
@@ -178,17 +159,6 @@ TN_DEVICE void put_rec(Acc<R>* __restrict__ dst, const Acc<R>& P, const Acc<R>& 
     dst[2] = sg;
     dst[3] = sb;
     dst[4] = sa;
-}
-
-template <typename R>
-TN_DEVICE Acc<R> acc_scaled(const Acc<R>& a, R k);
-template <>
-TN_DEVICE Acc<float> acc_scaled<float>(const Acc<float>& a, float k) {
-    return Acc<float>{CSum{a.v.hi * k, a.v.lo * k}};
-}
-template <>
-TN_DEVICE Acc<double> acc_scaled<double>(const Acc<double>& a, double k) {
-    return Acc<double>{a.v * k};
 }
 
 // (1) chunk sums, complete rays' losses
@@ -247,19 +217,6 @@ composite_chunk_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t P
     // zero-length rays after the last sample
     if (q + 1 == n_chunks)
         for (uint32_t r = wk.r_scan + lane; r < n_rays; r += 64) empty_ray(r);
-}
-
-// exclusive scan of an accumulator across the wave (lane 0 gets zero)
-template <typename R>
-TN_DEVICE Acc<R> acc_exclusive(const Acc<R>& v) {
-    const Acc<R> inc = v.scan(0);
-    Acc<R> ex;
-    if constexpr (sizeof(R) == 4) {
-        ex = Acc<R>{CSum{__shfl_up(inc.v.hi, 1, 64), __shfl_up(inc.v.lo, 1, 64)}};
-    } else {
-        ex = Acc<R>{__shfl_up(inc.v, 1, 64)};
-    }
-    return __lane_id() == 0 ? Acc<R>::zero() : ex;
 }
 
 // (2) one wave per ray spanning several chunks, a lane per chunk of the ray

@@ -279,6 +279,53 @@ TN_DEVICE LaneRay chunk_rays(uint32_t c, uint32_t r1, RayWalk& wk, const uint32_
     return head < 0 ? LaneRay{wk.carry_ray, wk.carry_end} : LaneRay{head, end};
 }
 
+// ---------------------------------------------------------------------------
+// Chunk-parallel passes over a batch cut into chunks of 64 samples, one per
+// wave, with per-ray chaining of the chunk sums between them.
+// ---------------------------------------------------------------------------
+// Walk state at chunk start c: the ray holding sample c is carried into the
+// chunk when it started earlier; otherwise the walk starts at the first ray
+// beginning at c (zero-length rays there included).
+TN_DEVICE RayWalk chunk_walk(uint32_t c, uint32_t n_rays, const uint32_t* __restrict__ offs,
+                             const uint32_t* __restrict__ steps) {
+    RayWalk wk;
+    const uint32_t r_next = wave_lower_bound(offs, n_rays, c + 1);  // offs[0] = 0 <= c, so r_next >= 1
+    const uint32_t rf = r_next - 1;
+    const uint32_t of = offs[rf];
+    if (of < c) {
+        wk.carry_ray = (int)rf;
+        wk.carry_end = (int)(of + steps[rf]);
+        wk.r_scan = r_next;
+    } else {
+        wk.r_scan = wave_lower_bound(offs, n_rays, c);
+    }
+    return wk;
+}
+
+template <typename R>
+TN_DEVICE Acc<R> acc_scaled(const Acc<R>& a, R k);
+template <>
+TN_DEVICE Acc<float> acc_scaled<float>(const Acc<float>& a, float k) {
+    return Acc<float>{CSum{a.v.hi * k, a.v.lo * k}};
+}
+template <>
+TN_DEVICE Acc<double> acc_scaled<double>(const Acc<double>& a, double k) {
+    return Acc<double>{a.v * k};
+}
+
+// exclusive scan of an accumulator across the wave (lane 0 gets zero)
+template <typename R>
+TN_DEVICE Acc<R> acc_exclusive(const Acc<R>& v) {
+    const Acc<R> inc = v.scan(0);
+    Acc<R> ex;
+    if constexpr (sizeof(R) == 4) {
+        ex = Acc<R>{CSum{__shfl_up(inc.v.hi, 1, 64), __shfl_up(inc.v.lo, 1, 64)}};
+    } else {
+        ex = Acc<R>{__shfl_up(inc.v, 1, 64)};
+    }
+    return __lane_id() == 0 ? Acc<R>::zero() : ex;
+}
+
 // value of lane 63
 TN_DEVICE float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
 TN_DEVICE int lane63(int v) { return __builtin_amdgcn_readlane(v, 63); }
