@@ -7,6 +7,7 @@
 #include <pybind11/stl.h>
 
 #include "controllers.hpp"
+#include "file_manager.hpp"
 #include "renderer.hpp"
 
 namespace py = pybind11;
@@ -123,6 +124,15 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
                                [](const Dataset& d) { return py::make_tuple(d.image_dimensions.x, d.image_dimensions.y); })
         .def_property_readonly("n_images", &Dataset::n_images)
         .def_readonly("contains_multiple_image_dims", &Dataset::contains_multiple_image_dims);
+
+    // snapshots (reference python_bindings.cu:647-664)
+    py::class_<FileManager>(m, "FileManager")
+        .def_static(
+            "load", [](NeRFProxy* nerf, const std::string& path) { FileManager::load(nerf, path); },
+            py::arg("nerf"), py::arg("path"))
+        .def_static(
+            "save", [](NeRFProxy* nerf, const std::string& path) { FileManager::save(nerf, path); },
+            py::arg("nerf"), py::arg("path"));
 
     py::class_<NeRFProxy>(m, "NeRF")
         .def("attach_dataset", &NeRFProxy::attach_dataset)
