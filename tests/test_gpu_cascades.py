@@ -96,11 +96,12 @@ def test_cascade_bitfield_and_update(field):
     assert ops.popcount(bits) == oracle.popcount(bits_ref)
 
 
-@pytest.mark.parametrize("n_rays", [2048, 40960])
-def test_cascade_marching(field, n_rays):
+@pytest.mark.parametrize("n_rays,max_steps", [(2048, 0xFFFFFFFF), (40960, 0xFFFFFFFF), (40960, 48)])
+def test_cascade_marching(field, n_rays, max_steps):
     """2048 rays: a wave marches one ray at a time. 40960 rays (five per wave
     of the bounded grid): lanes take their own rays through empty cells and
-    the wave takes each ray's occupied runs together. Both bit-exact."""
+    the wave takes each ray's occupied runs together; also with a cap of 48
+    samples per ray, which ends runs part-way. All bit-exact."""
     import torch
 
     import oracle
@@ -120,8 +121,9 @@ def test_cascade_marching(field, n_rays):
     assert np.array_equal(rays.alive.cpu().numpy(), rr["alive"])
     bits_h = field["bits"].cpu().numpy()
     dt_max = AABB * DT_MIN
-    ops.march_count(rays, n_rays, bbox, field["grid"], CONE, DT_MIN, dt_max)
-    n_ref, alive_ref = oracle.march_count(n_rays, n_rays, bbox, bits_h, LEVELS, RES, CONE, DT_MIN, dt_max, rr)
+    ops.march_count(rays, n_rays, bbox, field["grid"], CONE, DT_MIN, dt_max, max_steps)
+    n_ref, alive_ref = oracle.march_count(n_rays, n_rays, bbox, bits_h, LEVELS, RES, CONE, DT_MIN, dt_max, rr,
+                                          max_steps)
     n_gpu = _u32(rays.n_steps)
     assert np.array_equal(n_gpu, n_ref)
     assert np.array_equal(rays.alive.cpu().numpy(), alive_ref)
