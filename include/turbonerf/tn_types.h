@@ -33,7 +33,7 @@ typedef struct tn_bbox {
     float max_x, max_y, max_z;
 } tn_bbox_t;
 
-/* Pinhole camera; fields mirror Camera (reference src/models/camera.cuh:45-79).
+/* Camera; fields mirror Camera (reference src/models/camera.cuh:45-79).
  * transform is the 3x4 camera-to-world matrix, row-major
  * (m00 m01 m02 m03, m10 ..., m20 ...), as Transform4f stores it
  * (reference src/math/transform4f.cuh). offset_px = shift*resolution - principal_point. */
@@ -47,7 +47,7 @@ typedef struct tn_camera {
     float shift[2];
     float offset_px[2];
     float transform[12];
-    float dist_params[5];      /* k1 k2 k3 p1 p2; must be all zero (pinhole only) */
+    float dist_params[5];      /* k1 k2 k3 p1 p2 lens model; all zero: pinhole */
     int32_t show_image_planes;
 } tn_camera_t;
 

@@ -78,6 +78,8 @@ def lib():
             "oc_nerf_fwd_f64": (None, [C.c_void_p, u32, u32, f32p, f32p, f64p, i32, f64p, f64p]),
             "oc_blob_eval": (None, [u32, f32p, u32, f32p, f32p, f32p]),
             "oc_composite_analytic_f64": (None, [u32, u32p, u32p, f32p, u32, f32, f32p, u32, f32p, f64, f64p]),
+            "oc_lens_residual_jacobian": (None, [f32, f32, f32, f32, f32p, f32p]),
+            "oc_lens_undistort": (None, [f32, f32, f32p, C.POINTER(f32), C.POINTER(f32)]),
             "oc_render_rays": (None, [u32, u32, u32, C.c_void_p, C.c_void_p, f32p, f32p, f32p, f32p, u8p]),
             "oc_render_fixed_step_analytic_f64": (None, [u32, u32, f32p, f32p, f32p, f32p, C.c_void_p, u8p, i32, i32,
                                                          f32, f32, f32, u32, u32, f32p, f64, f64p]),
@@ -268,6 +270,20 @@ def mlp_train(net, feat, dir01, G, dL_ds, t, loss_scale, extra=None, fp64=True, 
     fn(n, n, c(net, dt_), c(feat, dt_), c(dir01, np.float32), p(G, dt_), p(dL_ds, np.float32), p(extra, np.float32),
        p(t, np.float32), loss_scale, dens.ctypes.data, rgb.ctypes.data, dfeat.ctypes.data, dW.ctypes.data)
     return dens, rgb, dfeat, dW
+
+
+def lens_residual_jacobian(x, y, xd, yd, k):
+    """(fx, fy, fx_x, fx_y, fy_x, fy_y) of the lens model at (x, y) as float32."""
+    out = np.zeros(6, np.float32)
+    lib().oc_lens_residual_jacobian(x, y, xd, yd, np.asarray(k, np.float32), out)
+    return out
+
+
+def lens_undistort(xd, yd, k):
+    """Undistorted normalised point of the distorted (xd, yd), float32 pair."""
+    xu, yu = f32(), f32()
+    lib().oc_lens_undistort(xd, yd, np.asarray(k, np.float32), C.byref(xu), C.byref(yu))
+    return np.float32(xu.value), np.float32(yu.value)
 
 
 def render_rays(n, start, stride, camera, bbox):

@@ -84,12 +84,65 @@ extern "C" uint32_t oc_morton3d_invert(uint32_t m) {
 }
 
 /* ===================================================================== */
-/* Camera rays (reference src/models/camera.cuh:94-141, pinhole branch)    */
+/* Camera rays (reference src/models/camera.cuh:94-141)                     */
 /* and training ray init (src/utils/training-batch-kernels.cuh:82-187).    */
 /* ===================================================================== */
 struct f3 { float x, y, z; };
 
+/* Residual and Jacobian of the radial (k1 k2 k3) + tangential (p1 p2) lens
+ * model at the undistorted point (x, y), against the distorted (xd, yd)
+ * (reference src/utils/camera-kernels.cuh:12-54). The reference's C++ types
+ * are kept: terms written with a double literal are evaluated in double and
+ * rounded to float once; the rest is float. out = fx fy fx_x fx_y fy_x fy_y. */
+extern "C" void oc_lens_residual_jacobian(float x, float y, float xd, float yd, const float k[5], float out[6]) {
+    const float k1 = k[0], k2 = k[1], k3 = k[2], p1 = k[3], p2 = k[4];
+    const float r = x * x + y * y;
+    const float d = (float)(1.0 + (double)(r * (k1 + r * (k2 + r * k3))));
+    out[0] = d * x + 2.0f * p1 * x * y + p2 * (r + 2.0f * x * x) - xd;
+    out[1] = d * y + 2.0f * p2 * x * y + p1 * (r + 2.0f * y * y) - yd;
+    const float d_r = (float)((double)k1 + (double)r * (2.0 * (double)k2 + (double)r * (3.0 * (double)k3)));
+    const float d_x = (float)(2.0 * (double)x * (double)d_r);
+    const float d_y = (float)(2.0 * (double)y * (double)d_r);
+    out[2] = (float)((double)(d + d_x * x) + 2.0 * (double)p1 * (double)y + 6.0 * (double)p2 * (double)x);
+    out[3] = (float)((double)(d_y * x) + 2.0 * (double)p1 * (double)x + 2.0 * (double)p2 * (double)y);
+    out[4] = (float)((double)(d_x * y) + 2.0 * (double)p2 * (double)y + 2.0 * (double)p1 * (double)x);
+    out[5] = (float)((double)(d + d_y * y) + 2.0 * (double)p2 * (double)x + 6.0 * (double)p1 * (double)y);
+}
+
+/* Newton's method for the undistorted point: start at (xd, yd), at most 10
+ * steps, stop when |det| < 1e-9 or both steps are below 1e-9
+ * (reference camera-kernels.cuh:58-105, called with those constants from
+ * camera.cuh:100-115). */
+extern "C" void oc_lens_undistort(float xd, float yd, const float k[5], float* xu, float* yu) {
+    const float eps = 1e-9f;
+    float x = xd, y = yd;
+    for (int it = 0; it < 10; ++it) {
+        float f[6];
+        oc_lens_residual_jacobian(x, y, xd, yd, k, f);
+        const float det = f[3] * f[4] - f[2] * f[5];
+        if (fabsf(det) < eps) break;
+        const float dx = (f[0] * f[5] - f[1] * f[3]) / det;
+        const float dy = (f[1] * f[2] - f[0] * f[4]) / det;
+        x += dx;
+        y += dy;
+        if (fabsf(dx) < eps && fabsf(dy) < eps) break;
+    }
+    *xu = x;
+    *yu = y;
+}
+
 static inline void camera_ray(const tn_camera_t& cam, float x, float y, f3& ori, f3& dir) {
+    // camera.cuh:100-116: pixel coordinates of a lens-distorted camera are
+    // undistorted first, in units of the focal length about the principal point
+    const float* k = cam.dist_params;
+    if (k[0] != 0.0f || k[1] != 0.0f || k[2] != 0.0f || k[3] != 0.0f || k[4] != 0.0f) {
+        const float xd = (x - cam.principal_point[0]) / cam.focal_length[0];
+        const float yd = (y - cam.principal_point[1]) / cam.focal_length[1];
+        float xu, yu;
+        oc_lens_undistort(xd, yd, k, &xu, &yu);
+        x = xu * cam.focal_length[0] + cam.principal_point[0];
+        y = yu * cam.focal_length[1] + cam.principal_point[1];
+    }
     // camera.cuh:119-123
     const f3 v = {
         (x + cam.offset_px[0] + 0.5f) / cam.focal_length[0],
@@ -135,9 +188,6 @@ extern "C" int oc_init_training_rays(uint32_t n_rays, uint32_t B, uint32_t n_ima
     const float* random, float* pix_rgba, uint32_t* ray_img_id, float* ori, float* dir,
     float* ray_t, float* ray_tmax, uint8_t* ray_alive) {
     (void)img_h;
-    for (uint32_t c = 0; c < n_images; ++c)
-        for (int k = 0; k < 5; ++k)
-            if (cams[c].dist_params[k] != 0.0f) return TN_ERR_UNSUPPORTED;
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)n_rays; ++ii) {
         const uint32_t i = (uint32_t)ii;

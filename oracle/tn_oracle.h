@@ -29,6 +29,8 @@ uint32_t oc_morton3d(uint32_t x, uint32_t y, uint32_t z);
 uint32_t oc_morton3d_invert(uint32_t m);
 
 /* --- rays and marching (float, bit-exact target) -------------------- */
+void oc_lens_residual_jacobian(float x, float y, float xd, float yd, const float k[5], float out[6]);
+void oc_lens_undistort(float xd, float yd, const float k[5], float* xu, float* yu);
 int oc_init_training_rays(uint32_t n_rays, uint32_t stride, uint32_t n_images,
     uint32_t n_pixels_per_image, int32_t img_w, int32_t img_h, float n_rays_per_image,
     const tn_bbox_t* bbox, const tn_camera_t* cameras, const uint8_t* images,

@@ -3,7 +3,8 @@ formulas: Random123 known-answer vectors, textbook bit interleaving, exact
 rational arithmetic, brute-force marching with numpy float32, closed forms of
 volume rendering, FP64 finite differences, the O(N^2) mip-NeRF 360 distortion
 sum, the Adam closed form, partition of unity / adjointness of the hash grid,
-SH orthonormality by exact quadrature, the camera model run backwards, Morton
+SH orthonormality by exact quadrature, the camera model run backwards, the
+lens model against its documented forward form and finite differences, Morton
 de-interleaving of occupancy points and known answers at the occupancy update
 rule's boundaries. The reference ships no tests or golden
 vectors for this path, so these checks are what pin each oracle function."""
@@ -486,6 +487,104 @@ def test_init_training_rays_camera_model_backwards():
         np.testing.assert_array_equal(rr["pix_rgba"].reshape(4, R)[:, i],
                                       images[img[i]].reshape(-1, 4)[lp].astype(F32) / F32(255))
     assert checked > R // 8 and (~alive).sum() > 0
+
+
+# --------------------------------------------------------------- lens model
+LENSES = [(-0.28, 0.07, 0.0, 1e-4, -2e-4),   # barrel, typical wide-angle calibration
+          (0.12, -0.02, 0.003, -5e-4, 8e-4),   # pincushion with a cubic term
+          (0.0, 0.0, 0.0, 2e-3, 0.0)]          # tangential only
+
+
+def _lens_forward(x, y, k):
+    """Radial-tangential (Brown-Conrady) distortion as OpenCV documents it,
+    in float64: r = x^2 + y^2, radial factor 1 + k1 r + k2 r^2 + k3 r^3,
+    tangential terms 2 p1 x y + p2 (r + 2 x^2) and p1 (r + 2 y^2) + 2 p2 x y."""
+    k1, k2, k3, p1, p2 = (float(F32(v)) for v in k)
+    r = x * x + y * y
+    rad = 1.0 + k1 * r + k2 * r * r + k3 * r * r * r
+    return x * rad + 2.0 * p1 * x * y + p2 * (r + 2.0 * x * x), y * rad + p1 * (r + 2.0 * y * y) + 2.0 * p2 * x * y
+
+
+@pytest.mark.parametrize("k", LENSES)
+def test_lens_residual_and_jacobian_against_forward_model(k):
+    """The residual is the forward model minus the target and the Jacobian
+    matches float64 central differences of the forward model."""
+    rng = np.random.default_rng(41)
+    h = 1e-6
+    for x, y, xd, yd in rng.uniform(-0.6, 0.6, size=(200, 4)).astype(F32):
+        f = oracle.lens_residual_jacobian(x, y, xd, yd, k)
+        X, Y = float(x), float(y)
+        fx, fy = _lens_forward(X, Y, k)
+        np.testing.assert_allclose(f[:2], [fx - float(xd), fy - float(yd)], rtol=0, atol=3e-7)
+        ax, ay = _lens_forward(X + h, Y, k)
+        bx, by = _lens_forward(X - h, Y, k)
+        cx, cy = _lens_forward(X, Y + h, k)
+        ex, ey = _lens_forward(X, Y - h, k)
+        jac = [(ax - bx) / (2 * h), (cx - ex) / (2 * h), (ay - by) / (2 * h), (cy - ey) / (2 * h)]
+        np.testing.assert_allclose(f[2:], jac, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("k", LENSES)
+def test_lens_undistort_inverts_forward_model(k):
+    """Newton's answer, pushed through the forward model, lands on the
+    distorted point to float32 rounding; with no distortion it is the point."""
+    rng = np.random.default_rng(42)
+    pts = rng.uniform(-0.55, 0.55, size=(400, 2)).astype(F32)
+    worst = 0.0
+    for xd, yd in pts:
+        xu, yu = oracle.lens_undistort(xd, yd, k)
+        gx, gy = _lens_forward(float(xu), float(yu), k)
+        worst = max(worst, abs(gx - float(xd)), abs(gy - float(yd)))
+        assert (xu, yu) != (xd, yd) or k[0] == 0.0
+    assert worst < 3e-7, worst
+    assert oracle.lens_undistort(F32(0.3), F32(-0.2), (0, 0, 0, 0, 0)) == (F32(0.3), F32(-0.2))
+
+
+def test_distorted_camera_rays_model_backwards():
+    """Training and render rays of lens-distorted cameras: the direction in
+    the camera frame gives the undistorted pixel (u f - offset - 0.5 per
+    axis), and that pixel, normalised about the principal point and pushed
+    through the forward lens model, is the sampled pixel normalised the same
+    way (training: x = fmod(p, W), y = p / W; render: pixel centres' corners
+    ix, iy)."""
+    from turbonerf_amd import scenes
+
+    NC, W, H, R = 3, 64, 48, 2000
+    rng = np.random.default_rng(43)
+    images = rng.integers(0, 256, size=(NC, H, W, 4), dtype=np.uint8)
+    rnd = rng.uniform(0, 1, R).astype(F32)
+    img = (np.arange(R) * NC) // R
+    p = F32(W * H) * rnd
+    x, y = np.fmod(p, F32(W)).astype(np.float64), (p / F32(W)).astype(np.float64)
+
+    def check(cam, d, px, py):
+        M = np.array(cam.transform[:], np.float64).reshape(3, 4)
+        dc = M[:, :3].T @ d
+        f = np.array(cam.focal_length[:], np.float64)
+        pp = np.array(cam.principal_point[:], np.float64)
+        off = np.array(cam.offset_px[:], np.float64)
+        u = (dc[:2] / dc[2] * f - off - 0.5 - pp) / f
+        g = _lens_forward(u[0], u[1], cam.dist_params[:])
+        np.testing.assert_allclose(g, [(px - pp[0]) / f[0], (py - pp[1]) / f[1]], rtol=0, atol=2e-5)
+        return u
+
+    for k in LENSES:
+        cams = scenes.sphere_cameras(44, NC, 2.0, W, H, 60.0, dist_params=k)
+        rr = oracle.init_training_rays(R, R, NC, W, H, R / NC, scenes.bbox(1.0), cams, images, rnd)
+        alive = rr["alive"].astype(bool)
+        d = rr["dir"].reshape(3, R).astype(np.float64)
+        assert alive.sum() > R // 8
+        moved = 0.0
+        for i in np.nonzero(alive)[0]:
+            u = check(cams[img[i]], d[:, i], x[i], y[i])
+            moved = max(moved, abs(u[0] * cams[0].focal_length[0] + cams[0].principal_point[0] - x[i]))
+        assert moved > 0.01  # pixels move 20x the tolerance: the pinhole model fails the check
+        cam = cams[0]
+        n = W * H
+        rd = oracle.render_rays(n, 0, n, cam, scenes.bbox(1.0))
+        dr = rd["dir"].reshape(3, n).astype(np.float64)
+        for i in range(0, n, 37):
+            check(cam, dr[:, i], float(i % W), float(i // W))
 
 
 def _morton_decode(m):

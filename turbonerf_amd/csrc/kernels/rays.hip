@@ -1,8 +1,9 @@
-// Training-ray generation: seeded uniforms and pinhole rays clipped to the
+// Training-ray generation: seeded uniforms and camera rays clipped to the
 // training box (reference src/utils/training-batch-kernels.cuh:82-187,
 // src/models/camera.cuh:94-141, src/core/trainer.cu:45-62).
 #pragma clang fp contract(off)
 
+#include "camera.hpp"
 #include "common.hpp"
 
 namespace tn {
@@ -22,31 +23,6 @@ __global__ void uniform_kernel(uint64_t n, uint32_t k0, uint32_t k1, uint64_t of
         const uint64_t j = g - offset;
         if (j < n) out[j] = u32_to_unit(c[w]);
     }
-}
-
-// Pinhole ray through pixel (x, y) in world space: origin on the near plane,
-// unit direction (same operation order as Camera::global_ray_at_pixel_xy).
-TN_DEVICE void camera_ray(const tn_camera_t& cam, float x, float y, float& ox, float& oy, float& oz,
-                          float& dx, float& dy, float& dz) {
-    const float vx = (x + cam.offset_px[0] + 0.5f) / cam.focal_length[0];
-    const float vy = (y + cam.offset_px[1] + 0.5f) / cam.focal_length[1];
-    const float vz = 1.0f;
-    const float v_len = sqrtf(vx * vx + vy * vy + vz * vz);
-    const float* m = cam.transform;
-    float gx = m[0] * vx + m[1] * vy + m[2] * vz;
-    float gy = m[4] * vx + m[5] * vy + m[6] * vz;
-    float gz = m[8] * vx + m[9] * vy + m[10] * vz;
-    const float gl = sqrtf(gx * gx + gy * gy + gz * gz);
-    gx = gx / gl;
-    gy = gy / gl;
-    gz = gz / gl;
-    const float sx = v_len * gx, sy = v_len * gy, sz = v_len * gz;
-    ox = m[3] + cam.near_plane * sx;
-    oy = m[7] + cam.near_plane * sy;
-    oz = m[11] + cam.near_plane * sz;
-    dx = gx;
-    dy = gy;
-    dz = gz;
 }
 
 __global__ void init_training_rays_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays,

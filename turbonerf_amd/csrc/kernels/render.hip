@@ -8,33 +8,10 @@
 // linear-buffer-ray-batch-coordinator.cu:54-79).
 #pragma clang fp contract(off)
 
+#include "camera.hpp"
 #include "common.hpp"
 
 namespace tn {
-
-// same operation order as the training rays (rays.hip)
-TN_DEVICE void pinhole_ray(const tn_camera_t& cam, float x, float y, float& ox, float& oy, float& oz, float& dx,
-                           float& dy, float& dz) {
-    const float vx = (x + cam.offset_px[0] + 0.5f) / cam.focal_length[0];
-    const float vy = (y + cam.offset_px[1] + 0.5f) / cam.focal_length[1];
-    const float vz = 1.0f;
-    const float v_len = sqrtf(vx * vx + vy * vy + vz * vz);
-    const float* m = cam.transform;
-    float gx = m[0] * vx + m[1] * vy + m[2] * vz;
-    float gy = m[4] * vx + m[5] * vy + m[6] * vz;
-    float gz = m[8] * vx + m[9] * vy + m[10] * vz;
-    const float gl = sqrtf(gx * gx + gy * gy + gz * gz);
-    gx = gx / gl;
-    gy = gy / gl;
-    gz = gz / gl;
-    const float sx = v_len * gx, sy = v_len * gy, sz = v_len * gz;
-    ox = m[3] + cam.near_plane * sx;
-    oy = m[7] + cam.near_plane * sy;
-    oz = m[11] + cam.near_plane * sz;
-    dx = gx;
-    dy = gy;
-    dz = gz;
-}
 
 // Pixel rays for pixels [start, start + n) of the camera image, transformed
 // by `to_local` (3x4, world -> NeRF frame), clipped to `bbox` there: the
@@ -50,7 +27,7 @@ __global__ void render_rays_kernel(uint32_t n, uint32_t start, uint32_t B, tn_ca
     const uint32_t w = (uint32_t)cam.resolution[0];
     const uint32_t iy = p / w, ix = p - iy * w;
     float ox, oy, oz, dx, dy, dz;
-    pinhole_ray(cam, (float)ix, (float)iy, ox, oy, oz, dx, dy, dz);
+    camera_ray(cam, (float)ix, (float)iy, ox, oy, oz, dx, dy, dz);
     float M[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) M[k] = to_local_ptr ? to_local_ptr[k] : ((k == 0 || k == 5 || k == 10) ? 1.0f : 0.0f);
@@ -109,7 +86,6 @@ extern "C" int tn_render_rays(uint32_t n_rays, uint32_t start_pixel, uint32_t st
                               tn_bbox_t render_bbox, const float* to_local, float* ori, float* dir, float* ray_t,
                               float* ray_tmax, uint32_t* pixel, uint8_t* alive, hipStream_t stream) {
     TN_CHECK_ARG(ori && dir && ray_t && ray_tmax && pixel && alive && n_rays <= stride && camera.resolution[0] > 0);
-    for (int k = 0; k < 5; ++k) TN_CHECK_ARG(camera.dist_params[k] == 0.0f);
     if (n_rays == 0) return TN_OK;
     render_rays_kernel<<<blocks_for(n_rays, 256), 256, 0, stream>>>(n_rays, start_pixel, stride, camera, render_bbox,
                                                                     to_local, ori, dir, ray_t, ray_tmax, pixel, alive);

@@ -54,10 +54,11 @@ def look_at_transform(eye: np.ndarray) -> np.ndarray:
 
 
 def make_camera(transform34: np.ndarray, width: int, height: int, fov_deg: float, near: float = 0.05,
-                far: float = 100.0) -> Camera:
-    """Pinhole camera with the principal point at the image centre; fields as
+                far: float = 100.0, dist_params=(0.0, 0.0, 0.0, 0.0, 0.0)) -> Camera:
+    """Camera with the principal point at the image centre; fields as
     Camera(resolution, near, far, focal_length, principal_point, shift,
-    transform) builds them (src/models/camera.cuh:59-79)."""
+    transform, dist_params) builds them (src/models/camera.cuh:59-79).
+    dist_params = (k1, k2, k3, p1, p2) of the lens model; all zero: pinhole."""
     f = 0.5 * width / math.tan(math.radians(fov_deg) * 0.5)
     cam = Camera()
     cam.resolution[0], cam.resolution[1] = width, height
@@ -72,20 +73,20 @@ def make_camera(transform34: np.ndarray, width: int, height: int, fov_deg: float
     for k, v in enumerate(np.asarray(transform34, np.float32).ravel()):
         cam.transform[k] = float(v)
     for k in range(5):
-        cam.dist_params[k] = 0.0
+        cam.dist_params[k] = float(dist_params[k])
     cam.show_image_planes = 1
     return cam
 
 
 def sphere_cameras(seed: int, n: int, radius: float, width: int, height: int, fov_deg: float,
-                   near: float = 0.05, far: float = 100.0):
+                   near: float = 0.05, far: float = 100.0, dist_params=(0.0, 0.0, 0.0, 0.0, 0.0)):
     """n cameras at seeded uniform points of a sphere, looking at the origin."""
     rng = np.random.default_rng(seed)
     v = rng.normal(size=(n, 3))
     v /= np.linalg.norm(v, axis=1, keepdims=True)
     cams = (Camera * n)()
     for i in range(n):
-        cams[i] = make_camera(look_at_transform(v[i] * radius), width, height, fov_deg, near, far)
+        cams[i] = make_camera(look_at_transform(v[i] * radius), width, height, fov_deg, near, far, dist_params)
     return cams
 
 

@@ -75,10 +75,11 @@ def render_blob_images(blobs_np: np.ndarray, cams, width: int, height: int, aabb
 
 
 def blob_dataset(seed: int, n_cams: int, width: int, height: int, fov_deg: float = 50.0, radius: float = 2.0,
-                 aabb: float = 1.0, n_blobs: int = 8):
-    """(blobs, ctypes camera array, images) for a seeded blob scene."""
+                 aabb: float = 1.0, n_blobs: int = 8, dist_params=(0.0, 0.0, 0.0, 0.0, 0.0)):
+    """(blobs, ctypes camera array, images) for a seeded blob scene; the
+    images are rendered through the cameras' lens model (dist_params)."""
     blobs = scenes.blob_field(seed, n_blobs, aabb)
-    cams = scenes.sphere_cameras(seed + 1, n_cams, radius * aabb, width, height, fov_deg)
+    cams = scenes.sphere_cameras(seed + 1, n_cams, radius * aabb, width, height, fov_deg, dist_params=dist_params)
     images = render_blob_images(blobs, cams, width, height, aabb)
     return blobs, cams, images
 
@@ -90,5 +91,5 @@ def to_pyturbonerf_cameras(tn, cams):
         t = np.array(list(c.transform), np.float32).reshape(3, 4)
         out.append(tn.Camera((c.resolution[0], c.resolution[1]), c.near_plane, c.far_plane,
                              (c.focal_length[0], c.focal_length[1]), (c.principal_point[0], c.principal_point[1]),
-                             (c.shift[0], c.shift[1]), tn.Transform4f(t)))
+                             (c.shift[0], c.shift[1]), tn.Transform4f(t), tn.DistortionParams(*c.dist_params)))
     return out
