@@ -1,0 +1,279 @@
+"""File-backed datasets (host code, no GPU): the PNG decoder against PIL's
+decoder and against scanlines filtered here with each PNG filter type, and
+Dataset.load_transforms against the transforms.json rules of the reference
+loader (src/models/dataset.cu:24-188) stated independently: the NeRF-to-this-
+frame change of basis as a matrix product P M F, focal lengths from the field
+of view, per-frame overrides, scene scale, missing files skipped, the most
+common image size kept, and the error paths."""
+import json
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+PIL = pytest.importorskip("PIL.Image")
+
+
+def tn():
+    from turbonerf_amd import PyTurboNeRF
+
+    return PyTurboNeRF
+
+
+def _write_transforms(path, doc):
+    path.write_text(json.dumps(doc))
+    return str(path)
+
+
+def _decode(tmp_path, files, w, h):
+    """Decode image files through a file-backed Dataset (one frame each)."""
+    frames = [{"file_path": f, "transform_matrix": np.eye(4).tolist()} for f in files]
+    p = _write_transforms(tmp_path / "transforms.json", {"w": w, "h": h, "frames": frames})
+    ds = tn().Dataset(p)
+    ds.load_transforms()
+    ds.load_images()
+    return np.array(ds.image_data)
+
+
+# ------------------------------------------------------------------ PNG
+def _pil_cases(rng, w, h):
+    rgba = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    rgba[0, 0, 3] = 0
+    g16 = rng.integers(0, 65536, (h, w), dtype=np.uint16)
+    pal = rng.integers(0, 256, (16, 3), dtype=np.uint8)
+    idx = rng.integers(0, 16, (h, w), dtype=np.uint8)
+    bits = rng.integers(0, 2, (h, w), dtype=np.uint8)
+    cases = {}
+    # image, expected RGBA8 under stb_image's conversion rules
+    cases["rgba8"] = (PIL.fromarray(rgba, "RGBA"), rgba)
+    cases["rgb8"] = (PIL.fromarray(rgba[..., :3], "RGB"),
+                     np.concatenate([rgba[..., :3], np.full((h, w, 1), 255, np.uint8)], -1))
+    g = rgba[..., 0]
+    cases["grey8"] = (PIL.fromarray(g, "L"), np.stack([g, g, g, np.full_like(g, 255)], -1))
+    cases["grey_alpha8"] = (PIL.fromarray(rgba[..., :2], "LA"), np.stack([g, g, g, rgba[..., 1]], -1))
+    gh = (g16 >> 8).astype(np.uint8)
+    cases["grey16"] = (PIL.fromarray(g16), np.stack([gh, gh, gh, np.full_like(gh, 255)], -1))
+    b = (bits * 255).astype(np.uint8)
+    cases["grey1"] = (PIL.fromarray(bits.astype(bool)), np.stack([b, b, b, np.full_like(b, 255)], -1))
+    alpha = rng.integers(0, 256, 16, dtype=np.uint8)
+    im = PIL.fromarray(idx, "P")
+    im.putpalette(pal.ravel().tolist())
+    im.info["transparency"] = bytes(alpha)
+    cases["palette_trns"] = (im, np.concatenate([pal[idx], alpha[idx][..., None]], -1))
+    key = tuple(int(v) for v in rgba[1, 1, :3])
+    exp = np.concatenate([rgba[..., :3], np.full((h, w, 1), 255, np.uint8)], -1)
+    exp[..., 3][np.all(rgba[..., :3] == key, -1)] = 0
+    cases["rgb_key"] = (PIL.fromarray(rgba[..., :3], "RGB"), exp, key)
+    return cases
+
+
+def test_png_decoder_matches_pil_and_stb_rules(tmp_path):
+    rng = np.random.default_rng(5)
+    w, h = 37, 23
+    names = []
+    expected = []
+    for name, case in _pil_cases(rng, w, h).items():
+        fn = f"{name}.png"
+        if name == "rgb_key":
+            case[0].save(tmp_path / fn, transparency=case[2])
+        else:
+            case[0].save(tmp_path / fn)
+        names.append(fn)
+        expected.append(case[1])
+        if name in ("rgba8", "rgb8", "grey8", "grey_alpha8", "palette_trns"):
+            # PIL's own decoder agrees with the stated rule
+            ref = np.array(PIL.open(tmp_path / fn).convert("RGBA"))
+            assert np.array_equal(ref, case[1]), name
+    got = _decode(tmp_path, names, w, h)
+    for i, name in enumerate(names):
+        assert np.array_equal(got[i], expected[i]), name
+
+
+def _paeth(a, b, c):
+    p = a + b - c
+    pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+    return a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
+
+
+def _png_bytes(w, h, ctype, depth, rows, filters, plte=None, interlace=0):
+    """A PNG written here: each scanline filtered with the given type as
+    ISO/IEC 15948 section 9 defines the filters."""
+    ch = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    bpp = max(1, ch * depth // 8)
+    out = bytearray()
+    prior = bytes(len(rows[0]))
+    for y, raw in enumerate(rows):
+        ft = filters[y % len(filters)]
+        f = bytearray(len(raw))
+        for i in range(len(raw)):
+            a = raw[i - bpp] if i >= bpp else 0
+            b = prior[i]
+            c = prior[i - bpp] if i >= bpp else 0
+            pred = [0, a, b, (a + b) // 2, _paeth(a, b, c)][ft]
+            f[i] = (raw[i] - pred) & 0xFF
+        out += bytes([ft]) + f
+        prior = raw
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+
+    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, interlace))
+    if plte is not None:
+        png += chunk(b"PLTE", bytes(plte))
+    return png + chunk(b"IDAT", zlib.compress(bytes(out), 9)) + chunk(b"IEND", b"")
+
+
+def test_png_every_filter_and_bit_depth(tmp_path):
+    rng = np.random.default_rng(6)
+    w, h = 29, 10
+    names, expected = [], []
+    rgba = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    rows = [bytes(rgba[y].ravel()) for y in range(h)]
+    for ft in range(5):
+        fn = f"rgba_f{ft}.png"
+        (tmp_path / fn).write_bytes(_png_bytes(w, h, 6, 8, rows, [ft]))
+        names.append(fn)
+        expected.append(rgba)
+    # 16-bit RGBA, filters mixed per row: high bytes
+    v16 = rng.integers(0, 65536, (h, w, 4), dtype=np.uint16)
+    rows16 = [v16[y].astype(">u2").tobytes() for y in range(h)]
+    (tmp_path / "rgba16.png").write_bytes(_png_bytes(w, h, 6, 16, rows16, [0, 1, 2, 3, 4]))
+    names.append("rgba16.png")
+    expected.append((v16 >> 8).astype(np.uint8))
+    # 2- and 4-bit grey, packed from the high bit, scaled by 85 and 17
+    for depth, scale in ((2, 85), (4, 17)):
+        g = rng.integers(0, 1 << depth, (h, w), dtype=np.uint8)
+        per = 8 // depth
+        rows_g = []
+        for y in range(h):
+            packed = bytearray((w + per - 1) // per)
+            for x in range(w):
+                packed[x // per] |= int(g[y, x]) << (8 - depth * (x % per + 1))
+            rows_g.append(bytes(packed))
+        fn = f"grey{depth}.png"
+        (tmp_path / fn).write_bytes(_png_bytes(w, h, 0, depth, rows_g, [4, 3, 1]))
+        names.append(fn)
+        gg = (g * scale).astype(np.uint8)
+        expected.append(np.stack([gg, gg, gg, np.full_like(gg, 255)], -1))
+    got = _decode(tmp_path, names, w, h)
+    for i, name in enumerate(names):
+        assert np.array_equal(got[i], expected[i]), name
+
+
+def test_png_errors(tmp_path):
+    rng = np.random.default_rng(7)
+    w, h = 8, 4
+    PIL.fromarray(rng.integers(0, 256, (h, w, 3), dtype=np.uint8), "RGB").save(tmp_path / "x.jpg", format="JPEG")
+    good = _png_bytes(w, h, 6, 8, [bytes(w * 4)] * h, [0])
+    (tmp_path / "i.png").write_bytes(_png_bytes(w, h, 6, 8, [bytes(w * 4)] * h, [0], interlace=1))
+    bad_crc = bytearray(good)
+    bad_crc[30] ^= 1  # inside IHDR
+    (tmp_path / "crc.png").write_bytes(bytes(bad_crc))
+    (tmp_path / "trunc.png").write_bytes(good[:-20])
+    PIL.fromarray(np.zeros((h + 1, w, 4), np.uint8), "RGBA").save(tmp_path / "size.png")
+    for fn, msg in (("x.jpg", "JPEG"), ("i.png", "interlaced"), ("crc.png", "CRC"), ("trunc.png", "truncated"),
+                    ("size.png", "declares")):
+        with pytest.raises(RuntimeError, match=msg):
+            _decode(tmp_path, [fn], w, h)
+
+
+# ------------------------------------------------------------ transforms.json
+P = np.array([[0, 1, 0], [0, 0, 1], [1, 0, 0]], np.float32)  # rows xyz <- yzx
+F = np.diag([1, -1, -1, 1]).astype(np.float32)               # flip the y and z columns
+
+
+def _expected_transform(m44, scale):
+    return (np.float32(scale) * (P @ (np.asarray(m44, np.float32)[:3] @ F))).astype(np.float32)
+
+
+def test_load_transforms_rules(tmp_path):
+    t = tn()
+    rng = np.random.default_rng(8)
+    for name, (w, h) in (("a", (40, 30)), ("b", (40, 30)), ("c", (40, 30)), ("d", (20, 10)), ("e", (20, 10))):
+        PIL.fromarray(rng.integers(0, 256, (h, w, 4), dtype=np.uint8), "RGBA").save(tmp_path / f"{name}.png")
+    mats = [rng.normal(size=(4, 4)).round(3).tolist() for _ in range(6)]
+    doc = {
+        "camera_angle_x": 0.9, "camera_angle_y": 0.7, "w": 40, "h": 30, "aabb_scale": 256, "scene_scale": 0.5,
+        "near": 0.1, "far": 20.0, "k1": -0.1, "p2": 0.002,
+        "frames": [
+            {"file_path": "a.png", "transform_matrix": mats[0]},
+            {"file_path": "b.png", "transform_matrix": mats[1], "fl_x": 33.5, "cx": 19.0, "k1": 0.05, "near": 0.2},
+            {"file_path": "gone.png", "transform_matrix": mats[2]},
+            {"file_path": "c.png", "transform_matrix": mats[3]},
+            {"file_path": "d.png", "transform_matrix": mats[4], "w": 20, "h": 10},
+            {"file_path": "e.png", "transform_matrix": mats[5], "w": 20, "h": 10},
+        ]}
+    ds = t.Dataset(_write_transforms(tmp_path / "transforms.json", doc))
+    ds.load_transforms()
+    # the missing file is skipped; 40x30 (three frames) outnumbers 20x10 (two)
+    assert ds.n_images == 3 and ds.contains_multiple_image_dims and ds.image_dimensions == (40, 30)
+    assert [p.rsplit("/", 1)[-1] for p in ds.image_paths] == ["a.png", "b.png", "c.png"]
+    assert ds.bounding_box.max_x == 64.0 and ds.bounding_box.min_z == -64.0  # aabb_scale capped at 128
+    fx = np.float32(0.5) * np.float32(40) / np.tan(np.float32(0.5) * np.float32(0.9), dtype=np.float32)
+    fy = np.float32(0.5) * np.float32(30) / np.tan(np.float32(0.5) * np.float32(0.7), dtype=np.float32)
+    for cam, m, over in zip(ds.cameras, (mats[0], mats[1], mats[3]), (False, True, False)):
+        np.testing.assert_array_equal(cam.transform.to_matrix(), _expected_transform(m, 0.5))
+        np.testing.assert_allclose(cam.focal_length, (33.5 if over else fx, fy), rtol=1e-6)
+        assert cam.principal_point == ((19.0 if over else 20.0), 15.0)
+        assert cam.near == np.float32(0.5) * np.float32(0.2 if over else 0.1)
+        assert cam.far == np.float32(10.0)
+        d = cam.dist_params
+        assert (d.k1, d.k2, d.p2) == (np.float32(0.05 if over else -0.1), 0.0, np.float32(0.002))
+        assert cam.resolution == (40, 30)
+
+
+def test_load_transforms_tie_fl_defaults_and_errors(tmp_path):
+    t = tn()
+    rng = np.random.default_rng(9)
+    for name, (w, h) in (("a", (12, 8)), ("b", (10, 9))):
+        PIL.fromarray(rng.integers(0, 256, (h, w, 4), dtype=np.uint8), "RGBA").save(tmp_path / f"{name}.png")
+    # a tie between sizes keeps the smaller width; no w/h given: sizes from the PNG headers;
+    # no field of view: fl_x defaults to 1000 and fl_y to fl_x
+    doc = {"fl_x": 500.0, "frames": [{"file_path": "a.png", "transform_matrix": np.eye(4).tolist()},
+                                     {"file_path": "b.png", "transform_matrix": np.eye(4).tolist()}]}
+    ds = t.Dataset(_write_transforms(tmp_path / "t1.json", doc))
+    ds.load_transforms()
+    assert ds.n_images == 1 and ds.image_dimensions == (10, 9)
+    assert ds.cameras[0].focal_length == (500.0, 500.0) and ds.cameras[0].principal_point == (0.0, 0.0)
+    assert ds.bounding_box.max_x == 8.0  # aabb_scale defaults to 16
+    doc2 = {"frames": [{"file_path": "a.png", "transform_matrix": np.eye(4).tolist()}]}
+    ds = t.Dataset(_write_transforms(tmp_path / "t2.json", doc2))
+    ds.load_transforms()
+    assert ds.cameras[0].focal_length == (1000.0, 1000.0)
+    with pytest.raises(RuntimeError, match="No valid images"):
+        d = t.Dataset(_write_transforms(tmp_path / "t3.json", {"frames": [{"file_path": "no.png",
+                                                                            "transform_matrix": []}]}))
+        d.load_transforms()
+    (tmp_path / "bad.json").write_text('{"frames": [1, 2,, 3]}')
+    with pytest.raises(RuntimeError, match="byte"):
+        t.Dataset(str(tmp_path / "bad.json")).load_transforms()
+    with pytest.raises(RuntimeError, match="Could not open"):
+        t.Dataset(str(tmp_path / "absent.json")).load_transforms()
+    (tmp_path / "t4.json").write_text(json.dumps({"w": "wide", "frames": []}))
+    with pytest.raises(RuntimeError, match="not a number"):
+        t.Dataset(str(tmp_path / "t4.json")).load_transforms()
+
+
+def test_round_trip_of_this_frames_cameras(tmp_path):
+    """Cameras written as NeRF matrices (the inverse change of basis,
+    P^T M' F) and read back are the same float32 transforms."""
+    from turbonerf_amd import scenes
+
+    t = tn()
+    cams = scenes.sphere_cameras(10, 5, 2.0, 16, 12, 50.0)
+    PIL.fromarray(np.zeros((12, 16, 4), np.uint8), "RGBA").save(tmp_path / "z.png")
+    frames = []
+    for c in cams:
+        m = np.array(c.transform[:], np.float32).reshape(3, 4)
+        nerf = np.eye(4, dtype=np.float32)
+        nerf[:3] = P.T @ m @ F
+        frames.append({"file_path": "z.png", "transform_matrix": nerf.astype(np.float64).tolist(),
+                       "fl_x": float(c.focal_length[0]), "fl_y": float(c.focal_length[1])})
+    doc = {"w": 16, "h": 12, "near": float(cams[0].near_plane), "far": float(cams[0].far_plane), "frames": frames}
+    ds = t.Dataset(_write_transforms(tmp_path / "transforms.json", doc))
+    ds.load_transforms()
+    for c, got in zip(cams, ds.cameras):
+        np.testing.assert_array_equal(got.transform.to_matrix(), np.array(c.transform[:], np.float32).reshape(3, 4))
+        assert got.near == np.float32(c.near_plane) and got.far == np.float32(c.far_plane)
+        assert got.principal_point == (8.0, 6.0)

@@ -103,7 +103,7 @@ def build_host(jobs: int = 8) -> Path | None:
     out = LIB / "libturbonerf.so"
     tmp = LIB / "libturbonerf.so.tmp"
     _run([cxx, "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),
-          f"-L{LIB}", "-lturbonerf_kernels", f"-L{ROCM / 'lib'}", "-lamdhip64",
+          f"-L{LIB}", "-lturbonerf_kernels", f"-L{ROCM / 'lib'}", "-lamdhip64", "-lz",
           "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, out)
     return out

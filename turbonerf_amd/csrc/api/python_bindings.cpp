@@ -123,6 +123,18 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
         .def_property_readonly("image_dimensions",
                                [](const Dataset& d) { return py::make_tuple(d.image_dimensions.x, d.image_dimensions.y); })
         .def_property_readonly("n_images", &Dataset::n_images)
+        .def_readonly("image_paths", &Dataset::image_paths)
+        .def("is_loaded_cpu", &Dataset::is_loaded_cpu)
+        .def("load_images", &Dataset::load_images, "Decode the image files (file-backed datasets).")
+        .def("unload_images", &Dataset::unload_images)
+        .def_property_readonly("image_data",
+                               [](const Dataset& d) {
+                                   if (!d.is_loaded_cpu()) throw std::runtime_error("images are not loaded");
+                                   return py::array_t<uint8_t>({(py::ssize_t)d.n_images(),
+                                                                (py::ssize_t)d.image_dimensions.y,
+                                                                (py::ssize_t)d.image_dimensions.x, (py::ssize_t)4},
+                                                               d.image_data.data());
+                               })
         .def_readonly("contains_multiple_image_dims", &Dataset::contains_multiple_image_dims);
 
     // snapshots (reference python_bindings.cu:647-664)
@@ -155,7 +167,8 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
 
     py::class_<NeRFManager>(m, "NeRFManager")
         .def(py::init<>())
-        .def("create", &NeRFManager::create, py::return_value_policy::reference)
+        // the returned NeRF keeps its manager (the owner of its storage) alive
+        .def("create", &NeRFManager::create, py::return_value_policy::reference_internal)
         .def("clone", &NeRFManager::clone, py::arg("nerf"), py::return_value_policy::reference)
         .def("destroy", &NeRFManager::destroy, py::arg("nerf"))
         .def("get_nerfs", &NeRFManager::get_proxies, py::return_value_policy::reference);
@@ -195,7 +208,8 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
         .def_readonly("n_total", &NeRFTrainingController::OccupancyGridMetrics::n_total);
 
     py::class_<NeRFTrainingController, std::shared_ptr<NeRFTrainingController>>(m, "Trainer")
-        .def(py::init<NeRFProxy*, TrainingConfig>(), py::arg("nerf"), py::arg("config") = TrainingConfig())
+        .def(py::init<NeRFProxy*, TrainingConfig>(), py::arg("nerf"), py::arg("config") = TrainingConfig(),
+             py::keep_alive<1, 2>())  // the trainer keeps its NeRF alive
         .def("setup_data", &NeRFTrainingController::setup_data, py::arg("batch_size"),
              "Call this once before starting training.")
         .def("teardown", &NeRFTrainingController::teardown, "Call this before destroying the controller.")

@@ -17,7 +17,7 @@ void register_renderer_bindings(py::module_& m) {
                  if (bbox.is_none()) return NeRFRenderable(nerf);
                  return NeRFRenderable(nerf, bbox.cast<BoundingBox>());
              }),
-             py::arg("nerf"), py::arg("bounding_box") = py::none())
+             py::arg("nerf"), py::arg("bounding_box") = py::none(), py::keep_alive<1, 2>())
         .def_readonly("nerf", &NeRFRenderable::proxy)
         .def_readwrite("bounding_box", &NeRFRenderable::bounding_box);
 

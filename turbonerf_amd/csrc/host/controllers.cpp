@@ -123,8 +123,15 @@ void NeRFTrainingController::reset_training() {
 
 void NeRFTrainingController::load_images(std::function<void(int, int)> on_image_loaded) {
     if (!proxy->dataset) throw std::runtime_error("load_images: no dataset");
+    Dataset& ds = *proxy->dataset;
+    // a file-backed dataset decodes its images now and drops the CPU copies
+    // once they are on the device (reference nerf-training-controller.cu:122-183)
+    const bool from_files = !ds.is_loaded_cpu() && !ds.image_paths.empty();
+    if (from_files) ds.load_images();
+    if (!ds.is_loaded_cpu()) throw std::runtime_error("load_images: the dataset holds no images");
     NeRF& nerf = proxy->nerfs[0];
-    nerf.dataset_ws.upload(*proxy->dataset, DeviceManager::stream());
+    nerf.dataset_ws.upload(ds, DeviceManager::stream());
+    if (from_files) ds.unload_images();
     nerf.is_image_data_loaded = true;
     proxy->is_dataset_dirty = false;
     if (on_image_loaded) {

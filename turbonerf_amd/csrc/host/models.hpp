@@ -151,9 +151,12 @@ struct Camera {
 // Training images and cameras (reference src/models/dataset.h). Images are
 // RGBA8, all of image_dimensions, concatenated. Loading transforms.json and
 // image files is outside this build; datasets are constructed in memory.
+// A training set: cameras and their RGBA8 images, either given in memory or
+// described by a transforms.json file (reference src/models/dataset.h:19-43).
 struct Dataset {
     std::vector<Camera> cameras;
-    std::vector<uint8_t> image_data;
+    std::vector<uint8_t> image_data;        // n_images x H x W x RGBA8, once loaded
+    std::vector<std::string> image_paths;   // per camera, for file-backed datasets
     int2v image_dimensions;
     uint32_t n_pixels_per_image = 0;
     BoundingBox bounding_box;
@@ -168,9 +171,14 @@ struct Dataset {
         image_data.assign(rgba, rgba + (size_t)cams.size() * n_pixels_per_image * 4);
     }
     size_t n_images() const { return cameras.size(); }
-    void load_transforms() {
-        throw std::runtime_error("Dataset::load_transforms: reading transforms.json is not part of this build");
-    }
+    // Reads cameras, image paths and the bounding box from file_path
+    // (reference src/models/dataset.cu:24-188).
+    void load_transforms();
+    // Decodes every image file into image_data, on several threads
+    // (reference dataset.cu:202-225).
+    void load_images();
+    void unload_images() { std::vector<uint8_t>().swap(image_data); }
+    bool is_loaded_cpu() const { return image_data.size() == (size_t)n_images() * n_pixels_per_image * 4; }
     Dataset copy() const { return *this; }
 };
 
