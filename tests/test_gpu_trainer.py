@@ -91,3 +91,32 @@ def test_render_after_training(scene):
           f"vs {gt[..., 3].mean():.3f}", flush=True)
     assert rgba[..., 3].max() > 0.3
     assert err < 0.1
+
+
+def test_empty_batch_raises_and_leaves_the_gpu_usable(scene):
+    """Every camera looks away from the box: every training ray misses, the
+    step runs its kernels over an empty batch (zero samples) and train_step
+    raises the reference's "No samples were generated" error
+    (src/core/trainer.cu:181-183). A normal trainer in the same process then
+    trains as before."""
+    from turbonerf_amd import PyTurboNeRF as tn
+    from turbonerf_amd import synthetic
+
+    blobs, cams, images = scene
+    away = synthetic.to_pyturbonerf_cameras(tn, cams)
+    for c in away:
+        m = c.transform.to_matrix().copy()
+        m[:, 1:3] *= -1.0  # flip down and forward: the camera faces outwards
+        c.transform = tn.Transform4f(m)
+    mgr = tn.NeRFManager()
+    nerf = mgr.create()
+    nerf.attach_dataset(tn.Dataset(tn.BoundingBox(1.0), away, images))
+    tr = tn.Trainer(nerf)
+    tr.setup_data(batch_size=1 << 16)
+    tr.load_images()
+    print("[trainer] empty batch", flush=True)
+    with pytest.raises(RuntimeError, match="No samples"):
+        tr.train_step()
+    mgr2, nerf2, tr2 = make_trainer(scene, 1)
+    m = tr2.train_step()
+    assert np.isfinite(m.loss) and m.n_samples > 0
