@@ -277,3 +277,37 @@ def test_round_trip_of_this_frames_cameras(tmp_path):
         np.testing.assert_array_equal(got.transform.to_matrix(), np.array(c.transform[:], np.float32).reshape(3, 4))
         assert got.near == np.float32(c.near_plane) and got.far == np.float32(c.far_plane)
         assert got.principal_point == (8.0, 6.0)
+
+
+def test_to_json_round_trip_and_images(tmp_path):
+    """Dataset.to_json writes NeRF-convention matrices, posix paths relative
+    to the dataset file and scene scale 1; reading it back gives the same
+    cameras, paths and box (scene scale and per-frame overrides already
+    applied). Dataset.images lists the image files with their size."""
+    t = tn()
+    rng = np.random.default_rng(11)
+    (tmp_path / "imgs").mkdir()
+    for name in ("a", "b"):
+        PIL.fromarray(rng.integers(0, 256, (9, 14, 4), dtype=np.uint8), "RGBA").save(tmp_path / "imgs" / f"{name}.png")
+    doc = {"camera_angle_x": 0.8, "w": 14, "h": 9, "aabb_scale": 8, "scene_scale": 0.3, "k2": 0.01,
+           "frames": [{"file_path": "imgs/a.png", "transform_matrix": rng.normal(size=(4, 4)).tolist()},
+                      {"file_path": "imgs/b.png", "transform_matrix": rng.normal(size=(4, 4)).tolist(),
+                       "fl_y": 21.5, "p1": 0.003}]}
+    ds = t.Dataset(_write_transforms(tmp_path / "transforms.json", doc))
+    ds.load_transforms()
+    assert [(im.file_path.rsplit("/", 2)[-2:], im.dimensions) for im in ds.images] == \
+        [(["imgs", "a.png"], (14, 9)), (["imgs", "b.png"], (14, 9))]
+    out = ds.to_json()
+    assert out["scene_scale"] == 1.0 and out["aabb_scale"] == 8 and (out["w"], out["h"]) == (14, 9)
+    assert [f["file_path"] for f in out["frames"]] == ["imgs/a.png", "imgs/b.png"]
+    ds2 = t.Dataset(_write_transforms(tmp_path / "again.json", out))
+    ds2.load_transforms()
+    assert ds2.image_paths == ds.image_paths and ds2.bounding_box == ds.bounding_box
+    for a, b in zip(ds.cameras, ds2.cameras):
+        np.testing.assert_array_equal(a.transform.to_matrix(), b.transform.to_matrix())
+        assert (a.focal_length, a.principal_point, a.near, a.far, a.resolution) == \
+            (b.focal_length, b.principal_point, b.near, b.far, b.resolution)
+        da, db = a.dist_params, b.dist_params
+        assert (da.k1, da.k2, da.k3, da.p1, da.p2) == (db.k1, db.k2, db.k3, db.p1, db.p2)
+    with pytest.raises(RuntimeError, match="to_json"):
+        t.Dataset(t.BoundingBox(1.0), list(ds.cameras), np.zeros((2, 9, 14, 4), np.uint8)).to_json()

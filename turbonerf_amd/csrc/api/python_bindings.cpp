@@ -105,6 +105,14 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
         .def_readwrite("max_y", &BoundingBox::max_y)
         .def_readwrite("max_z", &BoundingBox::max_z);
 
+    py::class_<TrainingImage>(m, "TrainingImage")
+        .def(py::init([](const std::string& path, py::object dims) { return TrainingImage{path, to_int2(dims)}; }),
+             py::arg("file_path"), py::arg("dimensions"))
+        .def_readonly("file_path", &TrainingImage::file_path)
+        .def_property_readonly("dimensions", [](const TrainingImage& t) {
+            return py::make_tuple(t.dimensions.x, t.dimensions.y);
+        });
+
     py::class_<Dataset>(m, "Dataset")
         .def(py::init<const std::string&>(), py::arg("file_path"))
         .def(py::init([](const BoundingBox& bbox, const std::vector<Camera>& cams,
@@ -116,6 +124,8 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
              py::arg("bounding_box"), py::arg("cameras"), py::arg("images"))
         .def("load_transforms", &Dataset::load_transforms)
         .def("copy", &Dataset::copy)
+        .def("to_json", [](const Dataset& d) { return py::module_::import("json").attr("loads")(d.to_json()); })
+        .def_property_readonly("images", &Dataset::images)
         .def("set_camera_at", [](Dataset& ds, int i, const Camera& c) { ds.cameras.at(i) = c; })
         .def_readwrite("file_path", &Dataset::file_path)
         .def_readwrite("cameras", &Dataset::cameras)

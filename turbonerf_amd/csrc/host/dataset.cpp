@@ -1,6 +1,7 @@
 // File-backed datasets: transforms.json (the instant-ngp / nerfstudio layout)
 // and PNG training images (reference src/models/dataset.cu:24-225).
 #include <algorithm>
+#include <cstdio>
 #include <atomic>
 #include <cmath>
 #include <filesystem>
@@ -130,6 +131,56 @@ void Dataset::load_transforms() {
     image_dimensions = cameras[0].resolution;
     n_pixels_per_image = (uint32_t)image_dimensions.x * (uint32_t)image_dimensions.y;
     unload_images();
+}
+
+namespace {
+std::string num(float v) {
+    if (!std::isfinite(v)) throw std::runtime_error("Dataset::to_json: a camera holds a non-finite value");
+    char b[32];
+    std::snprintf(b, sizeof(b), "%.9g", (double)v);  // 9 significant digits: reads back as the same float
+    return b;
+}
+std::string quoted(const std::string& s) {
+    std::string o = "\"";
+    for (const char c : s) {
+        if (c == '"' || c == '\\') o += '\\';
+        if ((unsigned char)c < 0x20) {
+            char b[8];
+            std::snprintf(b, sizeof(b), "\\u%04x", (unsigned)(unsigned char)c);
+            o += b;
+            continue;
+        }
+        o += c;
+    }
+    return o + "\"";
+}
+}  // namespace
+
+std::string Dataset::to_json() const {
+    if (!file_path.has_value() || image_paths.size() != cameras.size())
+        throw std::runtime_error("Dataset::to_json: only datasets read from a transforms.json can be written");
+    const fs::path base = fs::path(*file_path).parent_path();
+    std::string o = "{\n  \"w\": " + std::to_string(image_dimensions.x) + ",\n  \"h\": " +
+                    std::to_string(image_dimensions.y) + ",\n  \"aabb_scale\": " + num(bounding_box.size()) +
+                    ",\n  \"scene_scale\": 1.0,\n  \"frames\": [";
+    for (size_t i = 0; i < cameras.size(); ++i) {
+        const Camera& c = cameras[i];
+        const std::array<float, 12> m = c.transform.to_nerf().to_array();
+        std::string mat = "[";
+        for (int r = 0; r < 3; ++r)
+            mat += "[" + num(m[4 * r]) + ", " + num(m[4 * r + 1]) + ", " + num(m[4 * r + 2]) + ", " +
+                   num(m[4 * r + 3]) + "], ";
+        mat += "[0.0, 0.0, 0.0, 1.0]]";
+        const std::string rel = fs::path(image_paths[i]).lexically_relative(base).generic_string();
+        o += std::string(i ? "," : "") + "\n    {\"near\": " + num(c.near) + ", \"far\": " + num(c.far) +
+             ", \"cx\": " + num(c.principal_point.x) + ", \"cy\": " + num(c.principal_point.y) +
+             ", \"fl_x\": " + num(c.focal_length.x) + ", \"fl_y\": " + num(c.focal_length.y) +
+             ", \"k1\": " + num(c.dist_params.k1) + ", \"k2\": " + num(c.dist_params.k2) +
+             ", \"k3\": " + num(c.dist_params.k3) + ", \"p1\": " + num(c.dist_params.p1) +
+             ", \"p2\": " + num(c.dist_params.p2) + ", \"transform_matrix\": " + mat +
+             ", \"file_path\": " + quoted(rel) + "}";
+    }
+    return o + "\n  ]\n}\n";
 }
 
 void Dataset::load_images() {

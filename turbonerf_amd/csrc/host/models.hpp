@@ -151,6 +151,12 @@ struct Camera {
 // Training images and cameras (reference src/models/dataset.h). Images are
 // RGBA8, all of image_dimensions, concatenated. Loading transforms.json and
 // image files is outside this build; datasets are constructed in memory.
+// An image file of a dataset and its size (reference src/models/training-image.cuh:13-23)
+struct TrainingImage {
+    std::string file_path;
+    int2v dimensions;
+};
+
 // A training set: cameras and their RGBA8 images, either given in memory or
 // described by a transforms.json file (reference src/models/dataset.h:19-43).
 struct Dataset {
@@ -179,6 +185,15 @@ struct Dataset {
     void load_images();
     void unload_images() { std::vector<uint8_t>().swap(image_data); }
     bool is_loaded_cpu() const { return image_data.size() == (size_t)n_images() * n_pixels_per_image * 4; }
+    std::vector<TrainingImage> images() const {
+        std::vector<TrainingImage> out;
+        for (const std::string& p : image_paths) out.push_back({p, image_dimensions});
+        return out;
+    }
+    // The dataset as transforms.json text: NeRF-convention matrices, image
+    // paths relative to file_path's folder, scene scale 1 (reference
+    // dataset.cu:228-288).
+    std::string to_json() const;
     Dataset copy() const { return *this; }
 };
 
