@@ -156,6 +156,20 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
             "save", [](NeRFProxy* nerf, const std::string& path) { FileManager::save(nerf, path); },
             py::arg("nerf"), py::arg("path"));
 
+    // UpdatableProperty<T> wrappers (reference python_bindings.cu:43-53,257-258)
+    py::class_<UpdatableProperty<BoundingBox>>(m, "UpdatableBoundingBox")
+        .def(py::init<>())
+        .def("get", &UpdatableProperty<BoundingBox>::get)
+        .def("set", &UpdatableProperty<BoundingBox>::set)
+        .def("set_dirty", &UpdatableProperty<BoundingBox>::set_dirty)
+        .def("is_dirty", &UpdatableProperty<BoundingBox>::is_dirty);
+    py::class_<UpdatableProperty<Transform4f>>(m, "UpdatableTransform4f")
+        .def(py::init<>())
+        .def("get", &UpdatableProperty<Transform4f>::get)
+        .def("set", &UpdatableProperty<Transform4f>::set)
+        .def("set_dirty", &UpdatableProperty<Transform4f>::set_dirty)
+        .def("is_dirty", &UpdatableProperty<Transform4f>::is_dirty);
+
     py::class_<NeRFProxy>(m, "NeRF")
         .def("attach_dataset", &NeRFProxy::attach_dataset)
         .def("detach_dataset", &NeRFProxy::detach_dataset)
