@@ -39,13 +39,15 @@ SURFACE = {
     "NeRF": ["attach_dataset", "detach_dataset", "is_dirty", "can_train", "is_image_data_loaded", "render_bbox",
              "training_bbox", "transform", "is_visible", "is_dataset_dirty", "dataset", "can_render",
              "training_step", "id"],
-    "NeRFManager": ["create"],
+    "NeRFManager": ["create", "destroy"],
     "Trainer": ["setup_data", "teardown", "reset_training", "load_images", "train_step", "update_occupancy_grid",
                 "alpha_selection_threshold", "alpha_selection_probability", "min_step_size", "use_distortion_loss"],
     "Renderer": ["submit", "cancel"],
     "RenderRequest": ["cancel"],
     "Renderable": ["nerf", "bounding_box"],
     "CPURenderBuffer": ["get_rgba"],
+    "RenderTarget": ["save_image", "set_size", "width", "height", "free"],
+    "RenderModifiers": [], "RenderFlags": [], "RenderPattern": [],
     "FileManager": ["load", "save"],
     "UpdatableBoundingBox": ["get", "set", "set_dirty", "is_dirty"],
     "UpdatableTransform4f": ["get", "set", "set_dirty", "is_dirty"],
@@ -55,7 +57,7 @@ SURFACE = {
 def test_reference_surface_is_exposed():
     from turbonerf_amd import PyTurboNeRF as tn
 
-    assert tn.__version__ == "0.0.19"
+    assert tn.__version__ == "0.0.19" and callable(tn.teardown)
     missing = [f"{c}.{a}" for c, attrs in SURFACE.items() for a in attrs
                if not hasattr(tn, c) or not hasattr(getattr(tn, c), a)]
     assert not missing, missing

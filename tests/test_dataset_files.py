@@ -311,3 +311,24 @@ def test_to_json_round_trip_and_images(tmp_path):
         assert (da.k1, da.k2, da.k3, da.p1, da.p2) == (db.k1, db.k2, db.k3, db.p1, db.p2)
     with pytest.raises(RuntimeError, match="to_json"):
         t.Dataset(t.BoundingBox(1.0), list(ds.cameras), np.zeros((2, 9, 14, 4), np.uint8)).to_json()
+
+
+def test_render_buffer_save_image(tmp_path):
+    """RenderTarget.save_image writes the float RGBA buffer as an 8-bit RGBA
+    PNG with (uint8)(v * 255), truncated (reference gpu-image.cu:13-24), values
+    outside [0, 1] clamped; PIL reads back exactly those bytes."""
+    t = tn()
+    buf = t.CPURenderBuffer()
+    buf.set_size(13, 7)
+    rgba = np.asarray(buf.get_rgba())
+    rng = np.random.default_rng(12)
+    vals = rng.uniform(-0.2, 1.2, size=rgba.shape).astype(np.float32)
+    vals[0, 0] = [0.999, 1.0 / 255.0, 0.5, 1.0]
+    rgba[...] = vals
+    buf.save_image(str(tmp_path / "out.png"))
+    got = np.array(PIL.open(tmp_path / "out.png"))
+    want = np.clip(np.trunc(vals * np.float32(255)), 0, 255).astype(np.uint8)
+    assert got.shape == (7, 13, 4) and np.array_equal(got, want)
+    buf.free()
+    with pytest.raises(RuntimeError, match="no image"):
+        buf.save_image(str(tmp_path / "none.png"))

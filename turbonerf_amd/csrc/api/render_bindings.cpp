@@ -43,7 +43,9 @@ void register_renderer_bindings(py::module_& m) {
         .def("set_size", [](RenderTarget& rt, int w, int h) { rt.set_size(w, h); }, py::arg("width"), py::arg("height"))
         .def_readonly("width", &RenderTarget::width)
         .def_readonly("height", &RenderTarget::height)
-        .def("free", [](RenderTarget& rt) { rt.free(); });
+        .def("free", [](RenderTarget& rt) { rt.free(); })
+        .def("save_image", [](RenderTarget& rt, const std::string& path) { rt.save_image(path); },
+             py::arg("file_path"));
 
     py::class_<CPURenderBuffer, RenderTarget>(m, "CPURenderBuffer")
         .def(py::init<>())

@@ -230,6 +230,40 @@ Rgba8Image decode_png_rgba8(const uint8_t* f, size_t n, const std::string& name)
     return img;
 }
 
+void write_png_rgba8(const std::string& path, int width, int height, const uint8_t* rgba) {
+    if (width <= 0 || height <= 0 || !rgba) throw std::runtime_error("write_png: empty image " + path);
+    const size_t row = (size_t)width * 4;
+    std::vector<uint8_t> raw((row + 1) * (size_t)height);
+    for (int y = 0; y < height; ++y) {
+        raw[(size_t)y * (row + 1)] = 0;  // filter type None
+        std::memcpy(raw.data() + (size_t)y * (row + 1) + 1, rgba + (size_t)y * row, row);
+    }
+    uLongf zsize = compressBound((uLong)raw.size());
+    std::vector<uint8_t> z(zsize);
+    if (compress2(z.data(), &zsize, raw.data(), (uLong)raw.size(), 6) != Z_OK)
+        throw std::runtime_error("write_png: compression failed for " + path);
+    std::vector<uint8_t> out(kSignature, kSignature + 8);
+    auto chunk = [&out](const char* type, const uint8_t* data, size_t len) {
+        const uint8_t l[4] = {(uint8_t)(len >> 24), (uint8_t)(len >> 16), (uint8_t)(len >> 8), (uint8_t)len};
+        out.insert(out.end(), l, l + 4);
+        const size_t t0 = out.size();
+        out.insert(out.end(), type, type + 4);
+        out.insert(out.end(), data, data + len);
+        const uint32_t crc = (uint32_t)crc32(crc32(0L, Z_NULL, 0), out.data() + t0, (uInt)(len + 4));
+        const uint8_t c[4] = {(uint8_t)(crc >> 24), (uint8_t)(crc >> 16), (uint8_t)(crc >> 8), (uint8_t)crc};
+        out.insert(out.end(), c, c + 4);
+    };
+    const uint8_t ihdr[13] = {(uint8_t)(width >> 24), (uint8_t)(width >> 16), (uint8_t)(width >> 8), (uint8_t)width,
+                              (uint8_t)(height >> 24), (uint8_t)(height >> 16), (uint8_t)(height >> 8),
+                              (uint8_t)height, 8, 6, 0, 0, 0};
+    chunk("IHDR", ihdr, 13);
+    chunk("IDAT", z.data(), zsize);
+    chunk("IEND", nullptr, 0);
+    std::ofstream f(path, std::ios::binary);
+    if (!f || !f.write(reinterpret_cast<const char*>(out.data()), (std::streamsize)out.size()))
+        throw std::runtime_error("write_png: cannot write " + path);
+}
+
 Rgba8Image load_png_rgba8(const std::string& path) {
     const std::vector<uint8_t> bytes = read_file(path);
     return decode_png_rgba8(bytes.data(), bytes.size(), path);

@@ -1,5 +1,6 @@
-// Training-image decoding to RGBA8 (the reference decodes with stb_image's
-// stbi_load(..., 4), src/models/training-image.cu:11-13). PNG only: every
+// Training-image decoding to RGBA8 and render output as PNG (the reference
+// uses stb_image's stbi_load(..., 4), src/models/training-image.cu:11-13, and
+// stbi_write_png, src/utils/gpu-image.cu:47-83). Decoding: every
 // colour type and bit depth of non-interlaced PNG, converted as stb_image
 // converts them -- grey scaled to 8 bits (x255, x85, x17 for 1-, 2-, 4-bit),
 // 16-bit samples reduced to their high byte, palette alpha and grey/RGB colour
@@ -27,5 +28,8 @@ Rgba8Image load_png_rgba8(const std::string& path);
 
 // Same, from bytes in memory; `name` is used in error messages.
 Rgba8Image decode_png_rgba8(const uint8_t* data, size_t size, const std::string& name);
+
+// Writes an 8-bit RGBA PNG (unfiltered rows, zlib level 6); throws on I/O errors.
+void write_png_rgba8(const std::string& path, int width, int height, const uint8_t* rgba);
 
 }  // namespace turbo

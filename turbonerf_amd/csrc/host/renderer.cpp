@@ -9,6 +9,8 @@
 // pass needs one host read, to learn how many rays fitted the sample budget.
 #include "renderer.hpp"
 
+#include "image_io.hpp"
+
 #include <algorithm>
 #include <cstring>
 
@@ -101,6 +103,17 @@ uint64_t Renderer::render_pixels(hipStream_t s, const RenderRequest& req, const 
         samples_per_ray = std::max(1.0, (double)info_h[1] / (double)info_h[0]);
     }
     return n_evals;
+}
+
+void CPURenderBuffer::save_image(const std::string& file_path) {
+    if (width <= 0 || height <= 0 || rgba.size() != (size_t)width * height * 4)
+        throw std::runtime_error("save_image: the render buffer holds no image");
+    std::vector<uint8_t> px(rgba.size());
+    for (size_t i = 0; i < rgba.size(); ++i) {
+        const float v = rgba[i] * 255.0f;
+        px[i] = v >= 255.0f ? (uint8_t)255 : v > 0.0f ? (uint8_t)v : (uint8_t)0;  // truncation, as a cast does
+    }
+    write_png_rgba8(file_path, width, height, px.data());
 }
 
 NeRFRenderingController::NeRFRenderingController(RenderPattern pattern, uint32_t batch_size)

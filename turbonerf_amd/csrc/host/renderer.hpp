@@ -35,6 +35,10 @@ struct RenderTarget {
     virtual ~RenderTarget() = default;
     virtual void set_size(int w, int h) = 0;
     virtual void free() = 0;
+    // The image as an 8-bit RGBA PNG: each channel v becomes (uint8)(v * 255),
+    // truncated, values outside [0, 1] clamped (reference render-target.cuh:51-59,
+    // gpu-image.cu:13-24,47-83)
+    virtual void save_image(const std::string& file_path) = 0;
     int n_pixels() const { return width * height; }
 };
 
@@ -47,6 +51,7 @@ struct CPURenderBuffer : RenderTarget {
     }
     void free() override { rgba.clear(); rgba.shrink_to_fit(); width = height = 0; }
     float* get_rgba() { return rgba.data(); }
+    void save_image(const std::string& file_path) override;
 };
 
 struct NeRFRenderable {
