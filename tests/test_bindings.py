@@ -58,8 +58,9 @@ def test_reference_surface_is_exposed():
     from turbonerf_amd import PyTurboNeRF as tn
 
     assert tn.__version__ == "0.0.19" and callable(tn.teardown)
-    missing = [f"{c}.{a}" for c, attrs in SURFACE.items() for a in attrs
-               if not hasattr(tn, c) or not hasattr(getattr(tn, c), a)]
+    missing = [c for c in SURFACE if not hasattr(tn, c)]
+    missing += [f"{c}.{a}" for c, attrs in SURFACE.items() for a in attrs
+                if hasattr(tn, c) and not hasattr(getattr(tn, c), a)]
     assert not missing, missing
     u = tn.UpdatableTransform4f()
     assert u.is_dirty()
