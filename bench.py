@@ -97,6 +97,14 @@ def parse():
                     help="summarise rocprofv3 CSVs of a --profile-steps run: kernel_trace.csv [FETCH_SIZE "
                          "counter_collection.csv WRITE_SIZE counter_collection.csv copy-probe FETCH_SIZE csv "
                          "copy-probe WRITE_SIZE csv]; prints markdown (no GPU needed)")
+    ap.add_argument("--profile-render", type=int, default=0, metavar="FRAMES",
+                    help="profiling run: --warmup mixed-precision training steps (occupancy update every 16), "
+                         "then FRAMES renders of an 800x800 frame of the trained NeRF, and exit")
+    ap.add_argument("--summarize-after", default="write_rgba_kernel", metavar="KERNEL",
+                    help="summarise the dispatches after the last dispatch of KERNEL (adam_kernel for a "
+                         "--profile-render run)")
+    ap.add_argument("--summarize-units", type=int, default=0, metavar="N",
+                    help="divide totals by N (frames of a --profile-render run) instead of the Adam steps")
     ap.add_argument("--summarize-last", type=int, default=0,
                     help="with --summarize: only the last N training steps (after the warm-up)")
     ap.add_argument("--samples-per-step", type=float, default=0.0,
@@ -517,6 +525,27 @@ def profile_steps(tn, synthetic, scene, args):
     tr.teardown()
 
 
+def profile_render(tn, synthetic, scene, args):
+    """Renders only (after training), for rocprofv3: every dispatch after the
+    last adam_kernel belongs to one of the --profile-render frames."""
+    import torch
+
+    mgr, nerf, tr = make_trainer(tn, synthetic, scene, 1)
+    step = 0
+    while step < args.warmup:
+        k = min(16, args.warmup - step)
+        tr.train_steps(k, read_metrics=False)
+        step += k
+        if step % 16 == 0:
+            tr.update_occupancy_grid(step)
+    tr.train_step()
+    tr.synchronize()
+    torch.cuda.synchronize()
+    mr, ms = render_rate(tn, synthetic, scene, (mgr, nerf, tr), n_frames=args.profile_render - 1)
+    log(f"[bench profile] {args.profile_render} renders of {RES}x{RES}: {mr:.1f} Mrays/s ({ms:.2f} ms per frame)")
+    tr.teardown()
+
+
 def _short(name):
     base = name.split("(")[0]
     keys = {k for ks in KERNEL.values() for k in ks} | {
@@ -529,16 +558,18 @@ def _short(name):
     return base[:48]
 
 
-def summarize(paths, last=0, sps=0.0):
-    """Markdown summary of a --profile-steps run from rocprofv3 CSVs: the
-    dispatches after the last target-render kernel (write_rgba_kernel), or,
-    with `last`, after the start of the last `last` Adam steps' window."""
+def summarize(paths, last=0, sps=0.0, after="write_rgba_kernel", units=0):
+    """Markdown summary of a profiling run from rocprofv3 CSVs: the
+    dispatches after the last dispatch of `after` (the target renders'
+    write_rgba_kernel for --profile-steps, adam_kernel for --profile-render),
+    or, with `last`, after the start of the last `last` Adam steps' window.
+    Totals are per Adam step, or per `units` (frames) when given."""
     import csv
     import statistics as st
 
     def training_rows(path, ts_key):
         rows = list(csv.DictReader(open(path)))
-        marks = [int(r[ts_key]) for r in rows if "write_rgba_kernel" in r["Kernel_Name"]]
+        marks = [int(r[ts_key]) for r in rows if after in r["Kernel_Name"]]
         start = max(marks) if marks else 0
         if last:
             adam = sorted(int(r["End_Timestamp"]) for r in rows if "adam_kernel" in r["Kernel_Name"])
@@ -550,9 +581,10 @@ def summarize(paths, last=0, sps=0.0):
     per = {}
     for r in trace:
         per.setdefault(_short(r["Kernel_Name"]), []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
-    steps = len(per.get("adam_kernel", [])) or 1
+    steps = units or len(per.get("adam_kernel", [])) or 1
     total = sum(sum(v) for v in per.values())
-    out = ["| kernel | calls | avg us | total us per step | share |", "|---|---:|---:|---:|---:|"]
+    unit = "frame" if units else "step"
+    out = [f"| kernel | calls | avg us | total us per {unit} | share |", "|---|---:|---:|---:|---:|"]
     for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
         out.append(f"| {k} | {len(v)} | {st.mean(v) / 1e3:.2f} | {sum(v) / 1e3 / steps:.2f} | "
                    f"{100 * sum(v) / total:.1f}% |")
@@ -587,7 +619,8 @@ def summarize(paths, last=0, sps=0.0):
 def main():
     args = parse()
     if args.summarize:
-        summarize(args.summarize, args.summarize_last, args.samples_per_step)
+        summarize(args.summarize, args.summarize_last, args.samples_per_step, args.summarize_after,
+                  args.summarize_units)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build has no multi-GPU "
@@ -608,6 +641,9 @@ def main():
     scene = make_scene()
     if args.profile_steps:
         profile_steps(tn, synthetic, scene, args)
+        return
+    if args.profile_render:
+        profile_render(tn, synthetic, scene, args)
         return
     precs = {"mixed": [1], "fp32": [0], "both": [1, 0]}[args.precision]
     results = {}
