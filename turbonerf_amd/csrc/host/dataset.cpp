@@ -1,6 +1,7 @@
 // File-backed datasets: transforms.json (the instant-ngp / nerfstudio layout)
 // and PNG training images (reference src/models/dataset.cu:24-225).
 #include <algorithm>
+#include <charconv>
 #include <cstdio>
 #include <atomic>
 #include <cmath>
@@ -137,8 +138,9 @@ namespace {
 std::string num(float v) {
     if (!std::isfinite(v)) throw std::runtime_error("Dataset::to_json: a camera holds a non-finite value");
     char b[32];
-    std::snprintf(b, sizeof(b), "%.9g", (double)v);  // 9 significant digits: reads back as the same float
-    return b;
+    // 9 significant digits read back as the same float; to_chars ignores the locale
+    const auto r = std::to_chars(b, b + sizeof(b), v, std::chars_format::general, 9);
+    return std::string(b, r.ptr);
 }
 std::string quoted(const std::string& s) {
     std::string o = "\"";

@@ -1,6 +1,7 @@
-// Recursive-descent JSON reader (RFC 8259 grammar; numbers through strtod).
+// Recursive-descent JSON reader (RFC 8259 grammar; numbers through from_chars).
 #include "json.hpp"
 
+#include <charconv>
 #include <cstdlib>
 #include <cstring>
 
@@ -109,7 +110,7 @@ struct Reader {
         }
     }
     double num() {
-        // validate the RFC 8259 number grammar, then convert with strtod
+        // validate the RFC 8259 number grammar, then convert with from_chars
         const size_t b = i;
         if (i < s.size() && s[i] == '-') ++i;
         if (i < s.size() && s[i] == '0') {
@@ -130,8 +131,15 @@ struct Reader {
             if (!(i < s.size() && s[i] >= '0' && s[i] <= '9')) fail("bad exponent");
             while (i < s.size() && s[i] >= '0' && s[i] <= '9') ++i;
         }
-        const std::string tok = s.substr(b, i - b);
-        return std::strtod(tok.c_str(), nullptr);
+        double v = 0.0;
+        const auto r = std::from_chars(s.data() + b, s.data() + i, v);  // locale-independent
+        if (r.ec == std::errc::result_out_of_range) {
+            // out of double's range: the nearest representable value, as strtod gives
+            v = std::strtod(s.substr(b, i - b).c_str(), nullptr);
+        } else if (r.ec != std::errc()) {
+            fail("bad number");
+        }
+        return v;
     }
     JsonValue value() {
         if (++depth > 256) fail("nesting too deep");
