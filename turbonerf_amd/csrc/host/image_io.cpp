@@ -99,6 +99,7 @@ Rgba8Image decode_png_rgba8(const uint8_t* f, size_t n, const std::string& name)
             depth = data[8];
             ctype = data[9];
             if (w == 0 || h == 0 || w > (1u << 24) || h > (1u << 24)) fail("bad image size");
+            if ((uint64_t)w * h > (1ull << 30)) fail("image larger than 2^30 pixels");
             if (!depth_allowed(ctype, depth)) fail("unsupported colour type / bit depth");
             if (data[10] != 0 || data[11] != 0) fail("unknown compression or filter method");
             if (data[12] != 0) fail("interlaced PNG is not supported");
