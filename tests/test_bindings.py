@@ -67,3 +67,16 @@ def test_reference_surface_is_exposed():
     u.set(tn.Transform4f.Identity())
     u.set_dirty(False)
     assert not u.is_dirty() and u.get() == tn.Transform4f.Identity()
+
+
+def test_destroyed_slot_is_reused_in_place():
+    """NeRFManager.destroy then create reuses the slot: the old handle stays
+    valid and refers to the new NeRF (no dangling pointer)."""
+    from turbonerf_amd import PyTurboNeRF as tn
+
+    mgr = tn.NeRFManager()
+    a = mgr.create()
+    slot = a.id
+    mgr.destroy(a)
+    b = mgr.create()
+    assert b.id == slot and a.id == slot and a.training_step == 0 and not a.can_render

@@ -2,13 +2,22 @@
 #include "controllers.hpp"
 
 #include <algorithm>
+#include <new>
 
 namespace turbo {
 
 NeRFProxy* NeRFManager::create() {
     for (size_t i = 0; i < proxies_.size(); ++i) {
         if (!proxies_[i] || !proxies_[i]->is_valid) {
-            proxies_[i] = std::make_unique<NeRFProxy>();
+            // a destroyed slot is re-initialised in place: handles to it stay
+            // valid (they now refer to the new NeRF) instead of dangling
+            if (!proxies_[i]) {
+                proxies_[i] = std::make_unique<NeRFProxy>();
+            } else {
+                NeRFProxy* q = proxies_[i].get();
+                q->~NeRFProxy();
+                new (q) NeRFProxy();
+            }
             proxies_[i]->id = (int)i;
             proxies_[i]->is_valid = true;
             return proxies_[i].get();
