@@ -332,3 +332,27 @@ def test_render_buffer_save_image(tmp_path):
     buf.free()
     with pytest.raises(RuntimeError, match="no image"):
         buf.save_image(str(tmp_path / "none.png"))
+
+
+def test_transforms_json_escapes_and_number_forms(tmp_path):
+    """File names written with \\u escapes (a surrogate pair among them) and
+    matrices written with exponents, signs and integers read back as the
+    names and values they denote."""
+    t = tn()
+    name = "béb\U0001F600.png"
+    PIL.fromarray(np.zeros((4, 6, 4), np.uint8), "RGBA").save(tmp_path / name)
+    m = [[1e0, -0.0, 0, 2.5E-1], [0, 1, 0, -3e+2], [0, 0, 1.0, 1e-3], [0, 0, 0, 1]]
+    text = ('{"w": 6, "h": 4, "frames": [ {"file_path": "b\\u00e9b\\ud83d\\ude00.png", '
+            '"transform_matrix": [[1e0, -0.0, 0, 2.5E-1], [0, 1, 0, -3e+2], [0, 0, 1.0, 1e-3], [0, 0, 0, 1]]} ]}')
+    (tmp_path / "t.json").write_text(text, encoding="utf-8")
+    ds = t.Dataset(str(tmp_path / "t.json"))
+    ds.load_transforms()
+    assert ds.image_paths[0].endswith(name)
+    P = np.array([[0, 1, 0], [0, 0, 1], [1, 0, 0]], np.float32)
+    F = np.diag([1, -1, -1, 1]).astype(np.float32)
+    want = P @ (np.array(m, np.float32)[:3] @ F)
+    np.testing.assert_array_equal(ds.cameras[0].transform.to_matrix(), want)
+    for bad in ('{"frames": [}', '{"frames": []} x', '{"a": "\\ud83d"}', '{"a": 01}', '{"a": 1.}'):
+        (tmp_path / "bad.json").write_text(bad)
+        with pytest.raises(RuntimeError, match="JSON"):
+            t.Dataset(str(tmp_path / "bad.json")).load_transforms()
