@@ -165,3 +165,31 @@ def test_render_of_a_moved_nerf_equals_moving_the_camera(scene):
           f"alpha mean {ref[..., 3].mean():.3f}", flush=True)
     assert ref[..., 3].max() > 0.3
     assert d.mean() < 2e-5 and np.percentile(d, 99) < 5e-4
+
+
+def test_trainer_survives_slot_reuse(scene):
+    """A Trainer built on a NeRF whose slot is destroyed and handed out again:
+    its next step refuses to run until setup_data() is called again, and then
+    trains the new NeRF (the slot's NeRF storage does not move, so nothing
+    the trainer holds dangles)."""
+    from turbonerf_amd import PyTurboNeRF as tn
+    from turbonerf_amd import synthetic
+
+    blobs, cams, images = scene
+    mgr, nerf, tr = make_trainer(scene, 1)
+    assert np.isfinite(tr.train_step().loss)
+    slot = nerf.id
+    mgr.destroy(nerf)
+    with pytest.raises(RuntimeError, match="destroyed"):
+        tr.train_step()
+    again = mgr.create()
+    assert again.id == slot and nerf.training_step == 0
+    again.attach_dataset(tn.Dataset(tn.BoundingBox(1.0), synthetic.to_pyturbonerf_cameras(tn, cams), images))
+    with pytest.raises(RuntimeError, match="destroyed"):
+        tr.train_step()
+    tr.setup_data(batch_size=1 << 16)
+    tr.load_images()
+    for _ in range(3):
+        m = tr.train_step()
+        assert np.isfinite(m.loss) and m.n_samples > 0
+    assert again.training_step == 3

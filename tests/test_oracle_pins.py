@@ -657,6 +657,81 @@ def _layout():
     return g
 
 
+def _paper_layout(L, log2_T, n_min, b, vertex_rule="paper"):
+    """Level table from the Instant-NGP paper (Mueller et al. 2022, sec. 3):
+    N_l = floor(N_min * b^l) cells per axis (eq. 2), hence N_l + 1 vertices;
+    a level whose dense grid fits the table, (N_l + 1)^3 <= T, is indexed 1:1,
+    otherwise through the spatial hash. Levels hold min(vertices^3 rounded up
+    to 8, T) entries. vertex_rule="paper-1" and "strict" are planted mistakes
+    (one vertex too few; a strict dense threshold) that the test shows the
+    known-answer table rejects."""
+    T = 1 << log2_T
+    rows = []
+    off = 0
+    for l in range(L):
+        x = n_min * b ** l
+        v = math.floor(x) + 1
+        if vertex_rule == "paper-1":
+            v -= 1
+        vol = v ** 3
+        dense = vol < T if vertex_rule == "strict" else vol <= T
+        size = min((vol + 7) // 8 * 8, T)
+        rows.append((v, dense, size, off, x))
+        off += size
+    return rows, 2 * off
+
+
+def test_hashgrid_layout_matches_instant_ngp_equations():
+    """The level table at the training configuration (16 levels, T = 2^19,
+    N 16 -> 2048) against the paper's equations. tiny-cuda-nn, which the
+    reference configures (src/core/nerf-network.cu:86-104), counts
+    ceil(N_min b^l - 1) + 1 vertices per axis; that equals the paper's
+    floor(N_min b^l) + 1 wherever N_min b^l is not an integer and is one
+    fewer where it is (level 0 here: 16 vertices where the paper has 17).
+    The build keeps tiny-cuda-nn's count; this test fixes both the agreement
+    and that single departure, the dense rule (N_l + 1)^3 <= T, the level
+    sizes and offsets, and the known-answer parameter count 12,196,240."""
+    from turbonerf_amd import _capi, ops
+
+    b = math.exp((math.log(2048) - math.log(16)) / 15)
+    rows, n_params = _paper_layout(16, 19, 16, b)
+    # known answers of the paper's equations, written out
+    assert [r[0] for r in rows[:6]] == [17, 23, 31, 43, 59, 81]
+    assert [r[1] for r in rows] == [True] * 5 + [False] * 11
+    for name, g in (("oracle", _layout()), ("product", ops.hashgrid_layout())):
+        for l, (v, dense, size, off, x) in enumerate(rows):
+            integral = abs(x - round(x)) < 1e-9
+            want_v = v - 1 if integral else v
+            assert g.resolution[l] == want_v, (name, l, g.resolution[l], want_v)
+            assert bool(g.level_dense[l]) == (want_v ** 3 <= (1 << 19)), (name, l)
+            if l > 0:
+                assert g.level_size[l] == size and g.level_dense[l] == dense, (name, l)
+            # position scale: x * (vertices - 1) spans the level's vertices
+            assert abs(g.scale[l] - np.float32(x - 1)) <= 1e-6 * x, (name, l)
+        assert g.level_size[0] == 16 ** 3 and g.level_offset[1] == 16 ** 3
+        assert [g.level_offset[l] for l in range(16)] == list(
+            np.cumsum([0] + [g.level_size[l] for l in range(15)])), name
+        assert g.n_params == 12_196_240, (name, g.n_params)
+    # the paper's table differs from the build's only by level 0's one vertex
+    assert n_params == 12_196_240 + 2 * (4920 - 4096) == 12_197_888
+    # planted mistakes in the paper restatement are caught by the known answers
+    bad, _ = _paper_layout(16, 19, 16, b, "paper-1")
+    assert [r[0] for r in bad[:6]] != [17, 23, 31, 43, 59, 81]
+    # dense-threshold boundary: a level with exactly T vertices is dense under
+    # "<=" and hashed under "<"; the build follows "<="
+    b2 = 63.5 / 16.0
+    rows2, _ = _paper_layout(2, 18, 16, b2)
+    strict2, _ = _paper_layout(2, 18, 16, b2, "strict")
+    assert rows2[1][0] == 64 and rows2[1][1] and not strict2[1][1]
+    g2 = _capi.HashGrid()
+    oracle.lib().oc_hashgrid_layout(2, 2, 18, 16, b2, ctypes.addressof(g2))
+    g3 = ops.hashgrid_layout(2, 2, 18, 16, per_level_scale=b2)
+    for g in (g2, g3):
+        assert g.resolution[1] == 64 and g.level_dense[1] == 1 and g.level_size[1] == 1 << 18
+    oracle.lib().oc_hashgrid_layout(2, 2, 17, 16, b2, ctypes.addressof(g2))
+    assert g2.level_dense[1] == 0 and g2.level_size[1] == 1 << 17
+
+
 def test_hashgrid_partition_of_unity_and_adjoint():
     g = _layout()
     rng = np.random.default_rng(9)

@@ -10,13 +10,12 @@ NeRFProxy* NeRFManager::create() {
     for (size_t i = 0; i < proxies_.size(); ++i) {
         if (!proxies_[i] || !proxies_[i]->is_valid) {
             // a destroyed slot is re-initialised in place: handles to it stay
-            // valid (they now refer to the new NeRF) instead of dangling
+            // valid (they now refer to the new NeRF) instead of dangling, and
+            // the NeRF storage does not move, so no pointer into it dangles
             if (!proxies_[i]) {
                 proxies_[i] = std::make_unique<NeRFProxy>();
             } else {
-                NeRFProxy* q = proxies_[i].get();
-                q->~NeRFProxy();
-                new (q) NeRFProxy();
+                proxies_[i]->reset();
             }
             proxies_[i]->id = (int)i;
             proxies_[i]->is_valid = true;
@@ -61,8 +60,13 @@ NeRFProxy* NeRFManager::clone(NeRFProxy* src) {
 void NeRFManager::destroy(NeRFProxy* p) {
     for (auto& q : proxies_) {
         if (q.get() == p) {
+            bool held = false;
+            for (auto& n : q->nerfs) held = held || n.holds_device_memory();
+            // no queued work on any stream may still use the buffers
+            if (held) TN_HIP(hipDeviceSynchronize());
             q->free_device_memory();
             q->is_valid = false;
+            ++q->generation;
             return;
         }
     }
@@ -87,8 +91,11 @@ void NeRFTrainingController::setup_data(uint32_t batch_size) {
     const hipStream_t s = DeviceManager::stream();
     NeRF& nerf = proxy->nerfs[0];
     const Dataset& ds = *proxy->dataset;
+    if (ctx_ && generation_ != proxy->generation) ctx_.reset();  // the slot was destroyed and reused
     if (!ctx_) ctx_ = std::make_unique<Trainer::Context>(s, &ds, &nerf, batch_size);
+    generation_ = proxy->generation;
     ctx_->dataset = &ds;
+    ctx_->nerf = &nerf;
     ctx_->batch_size = batch_size;
     ctx_->rng_seed = config.rng_seed;
     ctx_->distortion_lambda = config.distortion_lambda;
@@ -151,6 +158,8 @@ void NeRFTrainingController::load_images(std::function<void(int, int)> on_image_
 
 void NeRFTrainingController::sync_settings() {
     if (!ctx_) throw std::runtime_error("train_step: call setup_data() first");
+    if (generation_ != proxy->generation || !proxy->is_valid)
+        throw std::runtime_error("train_step: the NeRF was destroyed since setup_data(); call setup_data() again");
     if (!proxy->nerfs[0].is_image_data_loaded) throw std::runtime_error("train_step: call load_images() first");
     ctx_->alpha_selection_probability = alpha_selection_probability;
     ctx_->alpha_selection_threshold = alpha_selection_threshold;

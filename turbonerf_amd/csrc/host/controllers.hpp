@@ -29,10 +29,29 @@ struct NeRFProxy {
     int id = -1;
     bool is_valid = false;
     NeRFProxy* clone_source = nullptr;
+    // Bumped whenever the slot is destroyed or handed out again; a controller
+    // remembers the value it was set up with and refuses to run on a newer one.
+    uint64_t generation = 0;
 
     NeRFProxy() : nerfs(1) {}
     NeRFProxy(const NeRFProxy&) = delete;
     NeRFProxy& operator=(const NeRFProxy&) = delete;
+    // Return the slot to a fresh state in place: the NeRF objects are reset
+    // element by element, so pointers to nerfs[i] held elsewhere keep
+    // pointing at live (now empty) objects.
+    void reset() {
+        for (auto& n : nerfs) n = NeRF();
+        dataset.reset();
+        render_bbox = {};
+        training_bbox = {};
+        transform = {};
+        is_visible = true;
+        is_dataset_dirty = true;
+        can_render = false;
+        training_step = 0;
+        clone_source = nullptr;
+        ++generation;
+    }
     void attach_dataset(const Dataset& ds) {
         dataset = ds;
         training_bbox.set(ds.bounding_box);
@@ -118,6 +137,7 @@ struct NeRFTrainingController {
 
 private:
     std::unique_ptr<Trainer::Context> ctx_;
+    uint64_t generation_ = 0;  // proxy->generation when setup_data() ran
     Trainer trainer_;
     void sync_settings();
 };

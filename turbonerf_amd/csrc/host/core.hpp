@@ -217,6 +217,10 @@ struct NeRF {
     OccupancyGrid occupancy_grid;
     DatasetWorkspace dataset_ws;
     bool is_image_data_loaded = false;
+    bool holds_device_memory() const {
+        return params.params_fp.size() || occupancy_grid.bitfield.size() || dataset_ws.images.size() ||
+               network.workspace.bytes();
+    }
     void free_device_memory() {
         params.free_allocations();
         occupancy_grid.free_device_memory();
