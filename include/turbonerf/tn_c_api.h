@@ -10,7 +10,8 @@
  * ori[i+2B].
  *
  * Device-side batch descriptor `info` (uint32[4]): info[0] = rays kept in the
- * batch, info[1] = samples in the batch, info[2] = status of the selection.
+ * batch, info[1] = samples in the batch, info[2] = status of the selection,
+ * info[3] = rays scanned (generated for the batch).
  * It is produced by tn_scan_select() and consumed by later stages, so a
  * training step needs no host round trip.
  *
@@ -66,7 +67,7 @@ int tn_march_count(uint32_t n_rays_max, const uint32_t* d_n_rays, uint32_t strid
     const float* ori, const float* ray_t, uint32_t* n_steps, tn_stream_t stream);
 
 /* Exclusive scan of n_steps into ray_offset and selection of the rays that
- * form the batch; writes info[0..2] on the device. mode 0 (training): keep the
+ * form the batch; writes info[0..3] on the device. mode 0 (training): keep the
  * rays before the last one whose offset is below max_samples, the reference's
  * rule. mode 1 (rendering): keep every ray whose samples end within
  * max_samples.
@@ -110,12 +111,14 @@ int tn_estimate_rays(const uint32_t* info, uint32_t max_samples, uint32_t cap, u
  * accumulation (oracle mode). density = raw network output s. Per-ray
  * buffers (bg_rgb[3], target_rgba[4], loss[4]) have stride `stride`; per-sample
  * buffers (density, color[3], dt, dL_ddensity, dL_dcolor[3]) have stride
- * `sample_stride`. dL_ddensity = dL/ds, dL_dcolor scaled by grad_scale.
+ * `sample_stride`. dL_ddensity = dL/ds, dL_dcolor scaled by grad_scale (and by
+ * 2^ceil(log2(info[0])) when scale_by_rays: the per-ray mean's 1/n_rays then
+ * leaves fp16 colour gradients in range; tn_adam_step takes it back out).
  * ray_scratch (256-byte aligned, tn_composite_scratch_bytes for the ray and
  * sample capacities) holds per-ray totals and per-64-sample-chunk sums. */
 size_t tn_composite_scratch_bytes(uint32_t n_rays_max, uint32_t sample_stride);
 int tn_composite_train(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t stride,
-    uint32_t sample_stride, float grad_scale, float min_transmittance, const uint32_t* ray_steps,
+    uint32_t sample_stride, float grad_scale, int scale_by_rays, float min_transmittance, const uint32_t* ray_steps,
     const uint32_t* ray_offset, const float* bg_rgb, const float* target_rgba,
     const void* density, const void* color, const float* dt, float* loss,
     float* dL_ddensity, void* dL_dcolor, void* ray_scratch, size_t scratch_bytes,
@@ -225,10 +228,13 @@ int tn_hashgrid_backward_binned(int dfeat_dtype, uint32_t n_max, const uint32_t*
 /* ------------------------------------------------------------------- MLPs */
 
 /* Density MLP (+ SH-4 direction encoding + colour MLP when color != 0) on
- * MFMA. precision 0 = fp32, 1 = fp16 with fp32 accumulation. params: network
+ * MFMA. precision 0 = fp32, 1 = fp16 with fp32 accumulation (the first layer
+ * of each MLP takes its inputs as two fp16 operands, hi + lo). params: network
  * parameters in that precision starting at element net_offset (see
- * tn_types.h). Replaces the FullyFusedMLP/SphericalHarmonics forward and
- * inference (reference src/core/nerf-network.cu:319-545). */
+ * tn_types.h); features fp32 [32][feat_stride] in both precisions; density
+ * and rgb out in the network precision. Replaces the
+ * FullyFusedMLP/SphericalHarmonics forward and inference (reference
+ * src/core/nerf-network.cu:319-545). */
 int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* info, uint32_t stride,
     uint32_t feat_stride, uint32_t out_stride, const void* params, uint32_t net_offset,
     const void* features, const float* dir, int color, void* density_out, void* rgb_out,
@@ -237,14 +243,16 @@ int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* info, uint32_t
 /* Backward of both MLPs with the reference's gradient plumbing (colour-MLP
  * backward, density row 0 += loss_scale*(dL_ds + dL_ds_extra), t^2 scaling for
  * t < 1, density-MLP backward; reference src/core/nerf-network.cu:668-768,
- * src/utils/training-kernels-fused.cuh:500-515). Writes dfeat and stores
+ * src/utils/training-kernels-fused.cuh:500-515). scale_by_rays: loss_scale is
+ * multiplied by 2^ceil(log2(info[0])) (as tn_composite_train's grad_scale).
+ * Features and the dfeat written are fp32 in both precisions; stores
  * (accumulate=0) or adds the 9408 weight gradients into grad_net (fp32). */
 size_t tn_mlp_backward_workspace_bytes(uint32_t n_max);
 int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* info, uint32_t stride,
     uint32_t feat_stride, const void* params, uint32_t net_offset, const void* features,
     const float* dir, const float* sample_t, const void* dL_drgb, uint32_t grad_stride,
-    const float* dL_ds, const float* dL_ds_extra, float loss_scale, void* dfeat, float* grad_net,
-    int accumulate, void* workspace, size_t workspace_bytes, tn_stream_t stream);
+    const float* dL_ds, const float* dL_ds_extra, float loss_scale, int scale_by_rays, void* dfeat,
+    float* grad_net, int accumulate, void* workspace, size_t workspace_bytes, tn_stream_t stream);
 
 /* Dynamic LDS bytes the MLP kernels launch with (for occupancy planning). */
 size_t tn_mlp_lds_bytes(int precision, int backward);
@@ -253,11 +261,17 @@ size_t tn_mlp_lds_bytes(int precision, int backward);
 
 /* NGP Adam over all parameters (reference ngp_adam_step,
  * src/core/adam-optimizer.cuh:48-96). Gradients are fp32 and are zeroed as
- * they are consumed. lp_dtype 0: no low-precision copy; 1: fp16 copy.
- * weights, gradients, moments and steps must be 16-byte aligned, weights_lp
- * 8-byte aligned. */
+ * they are consumed; they are divided by loss_scale, and by
+ * 2^ceil(log2(scale_by_rays_info[0])) when that batch descriptor is given.
+ * The learning rate is learning_rate, or learning_rate / (1 + lr_decay *
+ * *d_step) with the training step read on the device when d_step is given
+ * (the reference's schedule, src/core/nerf-network.cu:256; a captured step
+ * replays with the step counter advancing on the device). lp_dtype 0: no
+ * low-precision copy; 1: fp16 copy. weights, gradients, moments and steps
+ * must be 16-byte aligned, weights_lp 8-byte aligned. */
 int tn_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float loss_scale,
-    float learning_rate, float beta1, float beta2, float epsilon, float l2_reg, float* weights,
+    const uint32_t* scale_by_rays_info, float learning_rate, const uint32_t* d_step, float lr_decay,
+    float beta1, float beta2, float epsilon, float l2_reg, float* weights,
     int lp_dtype, void* weights_lp, float* gradients, float* first_moments, float* second_moments,
     uint32_t* param_steps, tn_stream_t stream);
 

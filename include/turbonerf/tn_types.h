@@ -53,12 +53,18 @@ typedef struct tn_camera {
 
 /* Multi-cascade occupancy grid (reference src/core/occupancy-grid.cuh:22-45).
  * bitfield: resolution^3 bytes in Morton order; bit k of a byte = cascade k.
- * density: n_levels * resolution^3 floats, cascade-major. */
+ * density: n_levels * resolution^3 floats, cascade-major.
+ * level_dt_scale: the marching kernels choose a sample's cascade from
+ * dt * level_dt_scale. Training marches with the world step (1, reference
+ * training-batch-kernels.cuh:273,441); the reference's renderer passes the
+ * step normalised by the scene size (1/aabb, rendering-kernels.cu:445,447).
+ * 0 reads as 1. */
 typedef struct tn_occupancy_grid {
     uint8_t* bitfield;
     float* density;
     int32_t n_levels;
     int32_t resolution;
+    float level_dt_scale;
 } tn_occupancy_grid_t;
 
 /* Multiresolution hash-grid geometry (Instant-NGP encoding configured at

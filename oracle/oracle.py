@@ -46,6 +46,7 @@ def lib():
             "oc_voxel_morton_index": (u32, [i32, i32, f32, f32, f32]),
             "oc_init_training_rays": (i32, [u32, u32, u32, u32, i32, i32, f32, C.c_void_p, C.c_void_p, u8p, f32p,
                                             f32p, u32p, f32p, f32p, f32p, f32p, u8p]),
+            "oc_deactivate_rays": (None, [u32, u32, f32, f32, f32p, f32p, u8p]),
             "oc_march_count": (None, [u32, u32, C.c_void_p, u8p, i32, i32, f32, f32, f32, f32p, f32p, u8p, f32p,
                                       f32p, u32p, u32]),
             "oc_scan_select": (i32, [u32, u32p, u32, u32p, C.POINTER(u32), C.POINTER(u32)]),
@@ -81,7 +82,7 @@ def lib():
             "oc_lens_residual_jacobian": (None, [f32, f32, f32, f32, f32p, f32p]),
             "oc_lens_undistort": (None, [f32, f32, f32p, C.POINTER(f32), C.POINTER(f32)]),
             "oc_render_rays": (None, [u32, u32, u32, C.c_void_p, C.c_void_p, f32p, f32p, f32p, f32p, u8p]),
-            "oc_render_fixed_step_analytic_f64": (None, [u32, u32, f32p, f32p, f32p, f32p, C.c_void_p, u8p, i32, i32,
+            "oc_render_fixed_step_analytic_f64": (None, [u32, u32, f32p, f32p, f32p, f32p, C.c_void_p, f32, u8p, i32, i32,
                                                          f32, f32, f32, u32, u32, f32p, f64, f64p]),
             "oc_num_threads": (i32, []),
         }
@@ -132,6 +133,13 @@ def init_training_rays(n_rays, stride, n_images, img_w, img_h, n_rays_per_image,
     if rc != 0:
         raise RuntimeError(f"oc_init_training_rays: {rc}")
     return dict(pix_rgba=pix, img_id=img_id, ori=ori, dir=d, t=t, tmax=tmax, alive=alive)
+
+
+def deactivate_rays(n_rays, stride, alpha_threshold, selection_probability, pix_rgba, random, alive):
+    a = c(alive, np.uint8).copy()
+    lib().oc_deactivate_rays(n_rays, stride, alpha_threshold, selection_probability, c(pix_rgba, np.float32),
+                             c(random, np.float32), a)
+    return a
 
 
 def march_count(n_rays, stride, bbox, bitfield, n_levels, res, cone, dt_min, dt_max, rays, max_steps=0xFFFFFFFF):
@@ -296,11 +304,12 @@ def render_rays(n, start, stride, camera, bbox):
 
 
 def render_fixed_step_analytic(n_rays, stride, ori, dir, t, tmax, bbox, bitfield, n_levels, res, cone, dt_min, dt_max,
-                               max_steps, blobs, min_T=1e-4):
-    """(4, n_rays) rgba of the reference's fixed-step render loop over the blob field."""
+                               max_steps, blobs, min_T=1e-4, inv_aabb=1.0):
+    """(4, n_rays) rgba of the reference's fixed-step render loop over the blob
+    field; inv_aabb = 1 / the scene size (the network step's normalisation)."""
     out = np.zeros(4 * n_rays, np.float64)
     lib().oc_render_fixed_step_analytic_f64(n_rays, stride, c(ori, np.float32), c(dir, np.float32), c(t, np.float32),
-                                            c(tmax, np.float32), C.addressof(bbox), c(bitfield, np.uint8), n_levels,
+                                            c(tmax, np.float32), C.addressof(bbox), inv_aabb, c(bitfield, np.uint8), n_levels,
                                             res, cone, dt_min, dt_max, max_steps, len(blobs),
                                             c(blobs, np.float32).ravel(), min_T, out)
     return out.reshape(4, n_rays)

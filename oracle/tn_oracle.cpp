@@ -232,6 +232,19 @@ extern "C" int oc_init_training_rays(uint32_t n_rays, uint32_t B, uint32_t n_ima
     return TN_OK;
 }
 
+/* deactivate_rays_with_alpha_threshold_kernel (reference
+ * src/utils/training-batch-kernels.cuh:189-208), launched by
+ * trainer.cu:85-99 with the random slice at random_float + batch_size: a ray
+ * whose target alpha is below the threshold is dropped when its uniform
+ * exceeds the selection probability. */
+extern "C" void oc_deactivate_rays(uint32_t n_rays, uint32_t B, float alpha_threshold, float selection_probability,
+                                   const float* pix_rgba, const float* random, uint8_t* ray_alive) {
+    for (uint32_t i = 0; i < n_rays; ++i) {
+        const float alpha = pix_rgba[3 * B + i];
+        if (alpha < alpha_threshold && random[i] > selection_probability) ray_alive[i] = 0;
+    }
+}
+
 /* ===================================================================== */
 /* OccupancyGrid methods (reference src/core/occupancy-grid.cuh:139-282)   */
 /* ===================================================================== */
@@ -1038,9 +1051,9 @@ extern "C" void oc_render_rays(uint32_t n, uint32_t start, uint32_t B, const tn_
  * divides the accumulated colour by alpha with that guard); stop once T <
  * min_T, past t_max or after max_steps steps. fp64 accumulation. */
 extern "C" void oc_render_fixed_step_analytic_f64(uint32_t n_rays, uint32_t B, const float* ori, const float* dir,
-    const float* ray_t, const float* ray_tmax, const tn_bbox_t* bbox, const uint8_t* bits, int n_levels, int res,
-    float cone, float dt_min, float dt_max, uint32_t max_steps, uint32_t nb, const float* bl, double min_T,
-    double* rgba) {
+    const float* ray_t, const float* ray_tmax, const tn_bbox_t* bbox, float inv_aabb, const uint8_t* bits,
+    int n_levels, int res, float cone, float dt_min, float dt_max, uint32_t max_steps, uint32_t nb, const float* bl,
+    double min_T, double* rgba) {
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t ii = 0; ii < (int64_t)n_rays; ++ii) {
         const uint32_t i = (uint32_t)ii;
@@ -1052,12 +1065,16 @@ extern "C" void oc_render_fixed_step_analytic_f64(uint32_t n_rays, uint32_t B, c
             const float x = ox + t * dx, y = oy + t * dy, z = oz + t * dz;
             const float dt = get_dt(t, cone, dt_min, dt_max);
             if (bbox_contains(*bbox, x, y, z)) {
-                const int lvl = oc_grid_level_at(n_levels, res, x, y, z, dt);
+                // the network step is normalised by the scene size
+                // (rendering-kernels.cu:445); it picks the cascade (:447) and
+                // turns density into alpha (:517-520)
+                const float ndt = inv_aabb * dt;
+                const int lvl = oc_grid_level_at(n_levels, res, x, y, z, ndt);
                 if (grid_occupied(bits, res, lvl, x, y, z)) {
                     const float xw[3] = {x, y, z};
                     double sig, c[3];
                     blob_eval_d(xw, nb, bl, sig, c);
-                    const double alpha = 1.0 - std::exp(-sig * (double)dt);
+                    const double alpha = 1.0 - std::exp(-sig * (double)ndt);
                     const double w = T * alpha;
                     if (alpha > 1e-4) { r += w * c[0]; g += w * c[1]; b += w * c[2]; }
                     a += w;

@@ -79,6 +79,8 @@ double oc_distortion_bruteforce_f64(uint32_t n, double lambda, const double* den
     const double* m_norm, const double* dt_norm);
 
 /* --- NGP Adam --------------------------------------------------------- */
+void oc_deactivate_rays(uint32_t n_rays, uint32_t stride, float alpha_threshold, float selection_probability,
+                        const float* pix_rgba, const float* random, uint8_t* ray_alive);
 void oc_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float loss_scale, float lr,
     float beta1, float beta2, float eps, float l2_reg, float* w_fp, float* w_lp,
     const float* grad, float* m, float* v, uint32_t* steps);
@@ -134,8 +136,8 @@ void oc_render_rays(uint32_t n, uint32_t start, uint32_t stride, const tn_camera
     const tn_bbox_t* render_bbox, float* ori, float* dir, float* ray_t, float* ray_tmax, uint8_t* alive);
 /* the reference's fixed-step render loop over the analytic field (fp64) */
 void oc_render_fixed_step_analytic_f64(uint32_t n_rays, uint32_t stride, const float* ori, const float* dir,
-    const float* ray_t, const float* ray_tmax, const tn_bbox_t* bbox, const uint8_t* bitfield, int n_levels,
-    int resolution, float cone, float dt_min, float dt_max, uint32_t max_steps, uint32_t n_blobs,
+    const float* ray_t, const float* ray_tmax, const tn_bbox_t* bbox, float inv_aabb, const uint8_t* bitfield,
+    int n_levels, int resolution, float cone, float dt_min, float dt_max, uint32_t max_steps, uint32_t n_blobs,
     const float* blobs, double min_transmittance, double* rgba);
 
 /* --- timing of the CPU baseline (OpenMP over rays) -------------------- */

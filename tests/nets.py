@@ -1,7 +1,14 @@
 """fp64 numpy restatement of the density/colour MLPs and their backward pass
 with the reference's density-gradient injection (src/core/nerf-network.cu:
 668-768). Used as the checker of the MFMA kernels; its backward is itself
-checked against torch autograd in tests/test_oracle_network.py."""
+checked against torch autograd (tests/test_oracle_pins.py,
+test_numpy_mlp_backward_matches_autograd).
+
+round16=True restates the fp16 MFMA kernels' arithmetic: the first layer of
+each MLP takes its inputs (features, h1, density outputs, SH) as fp16 hi + lo
+pairs, exact to about 2^-22 and treated as exact here; h2 and h3 enter their
+layers rounded to fp16; in the backward pass every delta and every
+activation operand of a weight gradient is rounded to fp16."""
 import numpy as np
 
 SH_C = [0.28209479177387814, 0.48860251190291987, 1.0925484305920792, 0.94617469575755997, 0.31539156525251999,
@@ -38,14 +45,14 @@ def _q(a, on):
 
 
 def forward(net, feat, dir01, round16=False, acc=np.float64):
-    """round16: round every matrix operand to fp16 as the fp16 MFMA kernels do;
+    """round16: the fp16 MFMA kernels' operand rounding (module docstring);
     acc: the accumulation type (np.float32 emulates the matrix cores' fp32
     accumulators, np.float64 is exact to working precision)."""
     W1, W2, W3, W4, W5 = (w.astype(acc) for w in split(net))
-    feat = _q(np.asarray(feat, acc), round16)
+    feat = np.asarray(feat, acc)
     h1 = np.maximum(feat @ W1.T, 0)
-    s16 = _q(h1, round16) @ W2.T
-    xc = np.concatenate([_q(s16, round16), _q(sh4(dir01).astype(acc), round16)], axis=1)
+    s16 = h1 @ W2.T
+    xc = np.concatenate([s16, sh4(dir01).astype(acc)], axis=1)
     h2 = np.maximum(xc @ W3.T, 0)
     h3 = np.maximum(_q(h2, round16) @ W4.T, 0)
     o = _q(h3, round16) @ W5.T
@@ -65,7 +72,7 @@ def backward(net, fw, G_rgb, dL_ds, t, loss_scale, extra=None):
     dH3 = _q((dO @ W5) * (fw["h3"] > 0), r)
     dW4 = dH3.T @ h2
     dH2 = _q((dH3 @ W4) * (fw["h2"] > 0), r)
-    dW3 = dH2.T @ fw["xc"]
+    dW3 = dH2.T @ _q(fw["xc"], r)
     dXc = dH2 @ W3
     dS = dXc[:, :16].copy()
     g0 = dS[:, 0] + acc(loss_scale) * np.asarray(dL_ds, acc)
@@ -76,7 +83,7 @@ def backward(net, fw, G_rgb, dL_ds, t, loss_scale, extra=None):
     dS = _q(dS, r)
     dW2 = dS.T @ h1
     dH1 = _q((dS @ W2) * (fw["h1"] > 0), r)
-    dW1 = dH1.T @ fw["feat"]
+    dW1 = dH1.T @ _q(fw["feat"], r)
     dX = dH1 @ W1
     grad = np.concatenate([dW1.ravel(), dW2.ravel(), dW3.ravel(), dW4.ravel(), dW5.ravel()])
     return dX, grad

@@ -92,8 +92,8 @@ def _mlp_case(seed, n, dtype):
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     dir01 = (0.5 * d + 0.5).astype(np.float32)
     if dtype == torch.float16:
-        net = net.astype(np.float16).astype(np.float64)
-        feat = feat.astype(np.float16).astype(np.float64)
+        net = net.astype(np.float16).astype(np.float64)  # the fp16 weights are the network
+    feat = feat.astype(np.float32).astype(np.float64)  # features are fp32 in both precisions
     return net, feat, dir01
 
 
@@ -107,7 +107,7 @@ def test_mlp_forward(cuda, prec):
     n_grid = 64  # a fake hash-table prefix exercises net_offset
     params = torch.zeros(n_grid + net.size, dtype=dtype, device="cuda")
     params[n_grid:] = torch.from_numpy(net).to(dtype).cuda()
-    feat_d = torch.from_numpy(np.ascontiguousarray(feat.T.ravel())).to(dtype).cuda()
+    feat_d = torch.from_numpy(np.ascontiguousarray(feat.T.ravel())).float().cuda()  # fp32 in both precisions
     dir_d = torch.from_numpy(np.ascontiguousarray(dir01.T.ravel())).cuda()
     from turbonerf_amd import ops
 
@@ -115,7 +115,9 @@ def test_mlp_forward(cuda, prec):
     fw = np_forward(net, feat, dir01)
     dens = dens.float().cpu().numpy()
     rgb = rgb.float().cpu().numpy().reshape(3, n).T
-    tol = 1e-5 if prec == "fp32" else 1e-2
+    # fp16: the first layers take their inputs as fp16 hi + lo pairs, so the
+    # error is that of rounding h2 and h3 (and the weights, fp16 here)
+    tol = 1e-5 if prec == "fp32" else 2e-3
     e1, e2 = norm_rel(dens, fw["s16"][:, 0]), norm_rel(rgb, fw["rgb"])
     print(f"[mlp fwd {prec}] density {e1:.2e} rgb {e2:.2e}", flush=True)
     assert e1 < tol and e2 < tol
@@ -144,7 +146,7 @@ def test_mlp_backward(cuda, prec):
     n_grid = 32
     params = torch.zeros(n_grid + net.size, dtype=dtype, device="cuda")
     params[n_grid:] = torch.from_numpy(net).to(dtype).cuda()
-    feat_d = torch.from_numpy(np.ascontiguousarray(feat.T.ravel())).to(dtype).cuda()
+    feat_d = torch.from_numpy(np.ascontiguousarray(feat.T.ravel())).float().cuda()  # fp32 in both precisions
     dir_d = torch.from_numpy(np.ascontiguousarray(dir01.T.ravel())).cuda()
     grad = torch.zeros(net.size, dtype=torch.float32, device="cuda")
     dfeat = ops.mlp_backward(params, n_grid, n, feat_d, n, dir_d, n, torch.from_numpy(t).cuda(),

@@ -101,6 +101,21 @@ def _rays(n, seed):
     return oracle.init_training_rays(n, n, 4, 64, 64, n / 4, scenes.bbox(1.0), cams, images, rnd)
 
 
+def test_deactivate_rays_known_answers():
+    """The reference's alpha-threshold rule (training-batch-kernels.cuh:
+    189-208): a ray is dropped iff alpha < threshold and its uniform >
+    probability; equality keeps it; rays past n_rays are untouched."""
+    B = 8
+    alpha = np.array([0.1, 0.1, 0.5, 0.5, 0.49, 0.9, 0.0, 0.1], np.float32)
+    rnd = np.array([0.9, 0.5, 0.9, 0.1, 0.51, 0.99, 0.5000001, 0.9], np.float32)
+    pix = np.zeros(4 * B, np.float32)
+    pix[3 * B:] = alpha
+    alive = np.ones(B, np.uint8)
+    alive[5] = 0
+    got = oracle.deactivate_rays(7, B, 0.5, 0.5, pix, rnd, alive)
+    assert got.tolist() == [0, 1, 1, 1, 0, 0, 0, 1]
+
+
 def test_march_count_all_occupied_bruteforce():
     """All cells occupied, aabb = 1 (constant dt): the count is the number of
     float32 steps t += dt taken while t < tmax and the point stays in the box."""

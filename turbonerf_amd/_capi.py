@@ -60,6 +60,7 @@ class OccupancyGrid(C.Structure):
         ("density", C.c_void_p),
         ("n_levels", C.c_int32),
         ("resolution", C.c_int32),
+        ("level_dt_scale", C.c_float),
     ]
 
 
@@ -106,8 +107,8 @@ _SIGS: dict[str, tuple] = {
     "tn_estimate_rays": (C.c_int, [vp, u32, u32, u32, vp, vp]),
     "tn_accumulate_samples": (C.c_int, [vp, vp, vp]),
     "tn_composite_scratch_bytes": (sz, [u32, u32]),
-    "tn_composite_train": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, f32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
-                                     vp, sz, vp]),
+    "tn_composite_train": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, C.c_int, f32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                     vp, vp, vp, sz, vp]),
     "tn_composite_forward": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, C.c_int, vp, vp, vp, vp, vp, vp, vp]),
     "tn_reduce_loss": (C.c_int, [vp, u32, u32, vp, vp, vp]),
     "tn_step_metrics": (C.c_int, [vp, u32, vp, vp, vp, vp, vp]),
@@ -126,10 +127,11 @@ _SIGS: dict[str, tuple] = {
     "tn_hashgrid_backward_binned": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp, C.c_size_t, vp]),
     "tn_mlp_forward": (C.c_int, [C.c_int, u32, vp, u32, u32, u32, vp, u32, vp, vp, C.c_int, vp, vp, vp]),
     "tn_mlp_backward_workspace_bytes": (sz, [u32]),
-    "tn_mlp_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, vp, u32, vp, vp, vp, vp, u32, vp, vp, f32, vp, vp,
-                                  C.c_int, vp, sz, vp]),
+    "tn_mlp_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, vp, u32, vp, vp, vp, vp, u32, vp, vp, f32, C.c_int, vp,
+                                  vp, C.c_int, vp, sz, vp]),
     "tn_mlp_lds_bytes": (sz, [C.c_int, C.c_int]),
-    "tn_adam_step": (C.c_int, [u32, u32, u32, f32, f32, f32, f32, f32, f32, vp, C.c_int, vp, vp, vp, vp, vp, vp]),
+    "tn_adam_step": (C.c_int, [u32, u32, u32, f32, vp, f32, vp, f32, f32, f32, f32, f32, vp, C.c_int, vp, vp, vp, vp,
+                               vp, vp]),
     "tn_cast_params": (C.c_int, [u64, vp, C.c_int, vp, vp]),
     "tn_render_rays": (C.c_int, [u32, u32, u32, Camera, BBox, vp, vp, vp, vp, vp, vp, vp, vp]),
     "tn_render_write_rgba": (C.c_int, [u32, vp, u32, vp, vp, vp, vp, vp]),

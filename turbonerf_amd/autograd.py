@@ -66,7 +66,7 @@ class NerfMLP(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, net: torch.Tensor, feat: torch.Tensor, dir01: torch.Tensor):
-        net, feat, dir01 = _contig(net), _contig(feat.to(net.dtype)), _contig(dir01.float())
+        net, feat, dir01 = _contig(net), _contig(feat.float()), _contig(dir01.float())
         n = feat.shape[1]
         dens, rgb = ops.mlp_forward(net, 0, n, feat, n, dir01, n, color=True)
         ctx.save_for_backward(net, feat, dir01)

@@ -1,6 +1,7 @@
 // PyTurboNeRF: pybind11 surface mirroring the reference module
 // (src/api/python_bindings.cu:56-694) for the training/rendering path.
 #include <pybind11/functional.h>
+#include <cmath>
 #include <pybind11/numpy.h>
 #include <pybind11/operators.h>
 #include <pybind11/pybind11.h>
@@ -25,6 +26,42 @@ float2v to_float2(py::handle h) {
     auto s = h.cast<py::sequence>();
     if (py::len(s) != 2) throw std::runtime_error("expected a pair");
     return float2v{s[0].cast<float>(), s[1].cast<float>()};
+}
+
+// host copy of the first n elements of a device buffer (n = 0: all)
+template <typename T>
+py::array_t<T> to_numpy(const DeviceBuffer<T>& b, size_t n = 0) {
+    if (n == 0 || n > b.size()) n = b.size();
+    py::array_t<T> a(n);
+    if (n) TN_HIP(hipMemcpy(a.mutable_data(), b.data(), n * sizeof(T), hipMemcpyDeviceToHost));
+    return a;
+}
+
+// device buffer <- host array of exactly its size
+template <typename T>
+void from_numpy(DeviceBuffer<T>& b, py::array_t<T, py::array::c_style | py::array::forcecast> a) {
+    if ((size_t)a.size() != b.size()) throw std::runtime_error("size mismatch");
+    TN_HIP(hipMemcpy(b.data(), a.data(), b.bytes(), hipMemcpyHostToDevice));
+}
+
+// fp32 host copy of a precision-typed (fp16 or fp32) byte buffer
+py::array_t<float> typed_to_float(const DeviceBuffer<uint8_t>& b, int precision) {
+    const size_t esz = precision == 1 ? 2 : 4, n = b.size() / esz;
+    py::array_t<float> a(n);
+    if (precision == 1) {
+        std::vector<uint16_t> h(n);
+        TN_HIP(hipMemcpy(h.data(), b.data(), n * 2, hipMemcpyDeviceToHost));
+        float* o = a.mutable_data();
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t x = h[i], sign = (x >> 15) & 1, e = (x >> 10) & 31, m = x & 1023;
+            float v = e == 0 ? std::ldexp((float)m, -24) : e == 31 ? (m ? NAN : INFINITY)
+                                                                 : std::ldexp((float)(m | 1024), (int)e - 25);
+            o[i] = sign ? -v : v;
+        }
+    } else if (n) {
+        TN_HIP(hipMemcpy(a.mutable_data(), b.data(), n * 4, hipMemcpyDeviceToHost));
+    }
+    return a;
 }
 
 Transform4f to_transform(py::array_t<float, py::array::c_style | py::array::forcecast> a) {
@@ -150,11 +187,13 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
     // snapshots (reference python_bindings.cu:647-664)
     py::class_<FileManager>(m, "FileManager")
         .def_static(
-            "load", [](NeRFProxy* nerf, const std::string& path) { FileManager::load(nerf, path); },
-            py::arg("nerf"), py::arg("path"))
+            "load", [](NeRFProxy* nerf, const std::string& path) { return FileManager::load(nerf, path); },
+            py::arg("nerf"), py::arg("path"),
+            "Load a snapshot; returns True when it held optimiser state (restored with the training step).")
         .def_static(
-            "save", [](NeRFProxy* nerf, const std::string& path) { FileManager::save(nerf, path); },
-            py::arg("nerf"), py::arg("path"));
+            "save",
+            [](NeRFProxy* nerf, const std::string& path, bool opt) { FileManager::save(nerf, path, opt); },
+            py::arg("nerf"), py::arg("path"), py::arg("include_optimizer_state") = false);
 
     // UpdatableProperty<T> wrappers (reference python_bindings.cu:43-53,257-258)
     py::class_<UpdatableProperty<BoundingBox>>(m, "UpdatableBoundingBox")
@@ -187,7 +226,23 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
         .def_property_readonly("dataset", [](const NeRFProxy& p) { return p.dataset; })
         .def_readonly("can_render", &NeRFProxy::can_render)
         .def_readonly("training_step", &NeRFProxy::training_step)
-        .def_readonly("id", &NeRFProxy::id);
+        .def_readonly("id", &NeRFProxy::id)
+        .def(
+            "set_analytic_field",
+            [](NeRFProxy& p, py::array_t<float, py::array::c_style | py::array::forcecast> blobs, float aabb,
+               float threshold) {
+                if (blobs.ndim() != 2 || blobs.shape(1) != 8) throw std::runtime_error("blobs must be (n, 8)");
+                set_analytic_field(&p, blobs.data(), (uint32_t)blobs.shape(0), aabb, threshold);
+            },
+            py::arg("blobs"), py::arg("aabb"), py::arg("threshold") = 0.01f,
+            "Synthetic scene: render a Gaussian-blob field (rows cx, cy, cz, s, p, r, g, b) instead of a network.")
+        .def("get_occupancy_bitfield", [](NeRFProxy& p) {
+            DeviceManager::synchronize();
+            const auto& og = p.nerfs[0].occupancy_grid;
+            py::array_t<uint8_t> a(og.bitfield.size());
+            TN_HIP(hipMemcpy(a.mutable_data(), og.bitfield.data(), og.bitfield.size(), hipMemcpyDeviceToHost));
+            return a;
+        });
 
     py::class_<NeRFManager>(m, "NeRFManager")
         .def(py::init<>())
@@ -295,6 +350,74 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
         .def("reset_timings", &NeRFTrainingController::reset_timings)
         .def("samples_trained", &NeRFTrainingController::samples_trained)
         .def("synchronize", &NeRFTrainingController::synchronize, py::call_guard<py::gil_scoped_release>())
+        // test support: the batch the last step trained on, the network
+        // workspace of that step and the optimiser state (host copies)
+        // which = 0: the batch the last step trained on; 1: the batch
+        // generated ahead for the next step
+        .def("_get_batch",
+             [](NeRFTrainingController& tc, int which) {
+                 tc.synchronize();
+                 Trainer::Context* c = tc.context();
+                 if (!c) throw std::runtime_error("call setup_data() first");
+                 const TrainingWorkspace& w = c->batches[which ? c->cur : c->last];
+                 py::dict d;
+                 d["info"] = to_numpy(w.info);
+                 d["random"] = to_numpy(w.random_float);
+                 d["pix_rgba"] = to_numpy(w.pix_rgba);
+                 d["ori"] = to_numpy(w.ray_origin);
+                 d["dir"] = to_numpy(w.ray_dir);
+                 d["t"] = to_numpy(w.ray_t);
+                 d["tmax"] = to_numpy(w.ray_t_max);
+                 d["img_id"] = to_numpy(w.ray_img_id);
+                 d["alive"] = to_numpy(w.ray_alive);
+                 d["n_steps"] = to_numpy(w.ray_step);
+                 d["offset"] = to_numpy(w.ray_offset);
+                 d["sample_t"] = to_numpy(w.sample_t);
+                 d["sample_pos"] = to_numpy(w.sample_pos);
+                 d["sample_dir"] = to_numpy(w.sample_dir);
+                 d["sample_dt"] = to_numpy(w.sample_dt);
+                 d["sample_m_norm"] = to_numpy(w.sample_m_norm);
+                 d["sample_dt_norm"] = to_numpy(w.sample_dt_norm);
+                 d["n_rays_next"] = to_numpy(c->n_rays_next);
+                 d["ray_capacity"] = w.ray_capacity;
+                 d["batch_size"] = w.batch_size;
+                 return d;
+             },
+             py::arg("which") = 0)
+        .def("_get_network_outputs",
+             [](NeRFTrainingController& tc) {
+                 tc.synchronize();
+                 const NerfNetwork& n = tc.proxy->nerfs[0].network;
+                 const int p = n.config.precision;
+                 py::dict d;
+                 d["density"] = typed_to_float(n.workspace.density, p);
+                 d["rgb"] = typed_to_float(n.workspace.rgb, p);
+                 d["dL_drgb"] = typed_to_float(n.workspace.dL_drgb, p);
+                 d["dL_ds"] = to_numpy(n.workspace.dL_ds);
+                 d["loss"] = to_numpy(n.workspace.loss_out);
+                 return d;
+             })
+        .def("_get_optimizer_state",
+             [](NeRFTrainingController& tc) {
+                 tc.synchronize();
+                 const NetworkParamsWorkspace& p = tc.proxy->nerfs[0].params;
+                 py::dict d;
+                 d["m1"] = to_numpy(p.m1);
+                 d["m2"] = to_numpy(p.m2);
+                 d["steps"] = to_numpy(p.steps);
+                 d["n_grid"] = p.n_grid;
+                 return d;
+             })
+        .def("_set_optimizer_state",
+             [](NeRFTrainingController& tc, py::array_t<float, py::array::c_style | py::array::forcecast> m1,
+                py::array_t<float, py::array::c_style | py::array::forcecast> m2,
+                py::array_t<uint32_t, py::array::c_style | py::array::forcecast> st) {
+                 tc.synchronize();
+                 NetworkParamsWorkspace& p = tc.proxy->nerfs[0].params;
+                 from_numpy(p.m1, m1);
+                 from_numpy(p.m2, m2);
+                 from_numpy(p.steps, st);
+             })
         .def_readwrite("alpha_selection_threshold", &NeRFTrainingController::alpha_selection_threshold)
         .def_readwrite("alpha_selection_probability", &NeRFTrainingController::alpha_selection_probability)
         .def_readwrite("min_step_size", &NeRFTrainingController::min_step_size)

@@ -79,6 +79,24 @@ std::vector<NeRFProxy*> NeRFManager::get_proxies() {
     return v;
 }
 
+void set_analytic_field(NeRFProxy* proxy, const float* blobs_host, uint32_t n_blobs, float aabb, float threshold) {
+    if (!proxy || !blobs_host || n_blobs == 0) throw std::runtime_error("set_analytic_field: empty field");
+    const hipStream_t s = DeviceManager::stream();
+    NeRF& nerf = proxy->nerfs[0];
+    nerf.analytic_blobs.resize(8 * (size_t)n_blobs);
+    TN_HIP(hipMemcpyAsync(nerf.analytic_blobs.data(), blobs_host, nerf.analytic_blobs.bytes(), hipMemcpyHostToDevice, s));
+    OccupancyGrid& og = nerf.occupancy_grid;
+    og.set_aabb_scale(aabb);
+    og.initialize_if_needed(s);
+    TN_TRY(tn_blob_density_grid(og.view(), nerf.analytic_blobs.data(), n_blobs, as_tn(s)));
+    TN_TRY(tn_occ_update_bits(og.view(), threshold, as_tn(s)));
+    const BoundingBox box(aabb);
+    proxy->training_bbox.set(box);
+    proxy->render_bbox.set(box);
+    proxy->can_render = true;
+    TN_HIP(hipStreamSynchronize(s));
+}
+
 // ------------------------------------------------------------------ training
 NeRFTrainingController::NeRFTrainingController(NeRFProxy* p, TrainingConfig cfg) : proxy(p), config(cfg) {
     if (!proxy) throw std::runtime_error("NeRFTrainingController: null NeRF");

@@ -65,6 +65,13 @@ struct NeRFProxy {
     void free_device_memory() { for (auto& n : nerfs) n.free_device_memory(); }
 };
 
+// Make `proxy` a synthetic scene: the renderer evaluates the blob field
+// (n_blobs rows of {cx, cy, cz, s, p, r, g, b}, world units) instead of a
+// network; the box is a cube of side `aabb`, the occupancy grid (log2(aabb)+1
+// cascades of 128^3) is the field's density at cell centres thresholded at
+// `threshold`.
+void set_analytic_field(NeRFProxy* proxy, const float* blobs_host, uint32_t n_blobs, float aabb, float threshold);
+
 // Fixed slot pool handing out stable raw pointers (reference nerf-manager.cuh:49-51)
 class NeRFManager {
 public:
@@ -127,6 +134,10 @@ struct NeRFTrainingController {
     // samples trained on since setup_data (device running total; synchronises)
     uint64_t samples_trained();
     void synchronize();
+    // Test support: the training context (batches, streams), or null before
+    // setup_data(). The batch the last step trained on is
+    // context()->batches[context()->last].
+    Trainer::Context* context() { return ctx_.get(); }
 
     float alpha_selection_threshold = 1.0f;
     float alpha_selection_probability = 1.0f;

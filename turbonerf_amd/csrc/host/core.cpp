@@ -67,11 +67,12 @@ void NetworkWorkspace::enlarge(uint32_t B, uint32_t R, int prec) {
     ray_capacity = std::max(R, ray_capacity);
     precision = prec;
     const size_t e = prec == 1 ? 2 : 4;
-    features.resize(32 * (size_t)batch_size * e);
+    // features and their gradients are fp32 in both precisions
+    features.resize(32 * (size_t)batch_size * 4);
     density.resize((size_t)batch_size * e);
     rgb.resize(3 * (size_t)batch_size * e);
     dL_drgb.resize(3 * (size_t)batch_size * e);
-    dfeat.resize(32 * (size_t)batch_size * e);
+    dfeat.resize(32 * (size_t)batch_size * 4);
     dL_ds.resize(batch_size);
     dL_dist.resize(batch_size);
     ray_loss.resize(4 * (size_t)ray_capacity);
@@ -151,7 +152,7 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
     // forward: encoding, density + colour MLPs (reference :431-545)
     {
         ScopedPhase ph(tm, "hashgrid_forward", s);
-        TN_TRY(tn_hashgrid_forward(prec, prec, B, info, B, B, grid, pos_batch, params, w.features.data(), ts));
+        TN_TRY(tn_hashgrid_forward(prec, 0, B, info, B, B, grid, pos_batch, params, w.features.data(), ts));
     }
     {
         ScopedPhase ph(tm, "mlp_forward", s);
@@ -168,7 +169,8 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
     // compositing + loss + its backward (reference :547-603)
     {
         ScopedPhase ph(tm, "composite", s);
-        TN_TRY(tn_composite_train(prec, info, R, R, B, config.loss_scale(), st.min_transmittance, ray_steps,
+        // gradients scaled by loss_scale * 2^ceil(log2(rays)), unscaled by Adam
+        TN_TRY(tn_composite_train(prec, info, R, R, B, config.loss_scale(), 1, st.min_transmittance, ray_steps,
                                   ray_offset, random_rgb, target_rgba, w.density.data(), w.rgb.data(), dt_batch,
                                   w.ray_loss.data(), w.dL_ds.data(), w.dL_drgb.data(), w.composite_scratch.data(),
                                   w.composite_scratch.size(), ts));
@@ -178,23 +180,23 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
         ScopedPhase ph(tm, "mlp_backward", s);
         TN_TRY(tn_mlp_backward(prec, B, info, B, B, params, p.n_grid, w.features.data(), dir_batch, t_batch,
                                w.dL_drgb.data(), B, w.dL_ds.data(), st.use_distortion_loss ? w.dL_dist.data() : nullptr,
-                               config.loss_scale(), w.dfeat.data(), p.gradients.data() + p.n_grid, 0, w.mlp_ws.data(),
-                               w.mlp_ws.size(), ts));
+                               config.loss_scale(), 1, w.dfeat.data(), p.gradients.data() + p.n_grid, 0,
+                               w.mlp_ws.data(), w.mlp_ws.size(), ts));
     }
     {
         ScopedPhase ph(tm, "hashgrid_backward", s);
         // the bin-sort scatter (its workspace follows the batch size and the layout)
         const size_t hb = tn_hashgrid_backward_workspace_bytes(B, grid);
         w.hash_ws.reserve(hb);
-        TN_TRY(tn_hashgrid_backward_binned(prec, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(),
+        TN_TRY(tn_hashgrid_backward_binned(0, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(),
                                            w.hash_ws.data(), w.hash_ws.size(), ts));
     }
     // optimiser (reference :770-781, lr schedule :256)
     const float lr = config.learning_rate / (1.0f + config.learning_rate_decay * (float)step);
     {
         ScopedPhase ph(tm, "adam", s);
-        TN_TRY(tn_adam_step(p.n_total, p.n_grid, p.n_total, config.loss_scale(), lr, config.beta1, config.beta2,
-                            config.epsilon, config.l2_reg, p.params_fp.data(), prec,
+        TN_TRY(tn_adam_step(p.n_total, p.n_grid, p.n_total, config.loss_scale(), info, lr, nullptr, 0.0f, config.beta1,
+                            config.beta2, config.epsilon, config.l2_reg, p.params_fp.data(), prec,
                             prec == 1 ? p.params_lp.data() : nullptr, p.gradients.data(), p.m1.data(), p.m2.data(),
                             p.steps.data(), ts));
     }
@@ -212,7 +214,7 @@ void NerfNetwork::inference(hipStream_t s, const NetworkParamsWorkspace& p, uint
                             void* density_out, void* rgb_out, bool use_color_network) {
     const int prec = config.precision;
     const void* params = params_for_kernels(p);
-    TN_TRY(tn_hashgrid_forward(prec, prec, n, info, stride, stride, grid, pos_batch, params, features, as_tn(s)));
+    TN_TRY(tn_hashgrid_forward(prec, 0, n, info, stride, stride, grid, pos_batch, params, features, as_tn(s)));
     TN_TRY(tn_mlp_forward(prec, n, info, stride, stride, stride, params, p.n_grid, features, dir_batch,
                           use_color_network ? 1 : 0, density_out, rgb_out, as_tn(s)));
 }

@@ -135,7 +135,8 @@ struct NetworkParamsWorkspace {
 struct NetworkWorkspace {
     uint32_t batch_size = 0, ray_capacity = 0;
     int precision = -1;
-    DeviceBuffer<uint8_t> features, density, rgb, dL_drgb, dfeat;  // precision-typed
+    DeviceBuffer<uint8_t> features, dfeat;         // fp32 [32][B]
+    DeviceBuffer<uint8_t> density, rgb, dL_drgb;   // network precision
     DeviceBuffer<float> dL_ds, dL_dist, ray_loss, ray_dist_loss, loss_out;
     DeviceBuffer<uint8_t> composite_scratch, distortion_scratch, mlp_ws, hash_ws;
     void enlarge(uint32_t B, uint32_t R, int precision);
@@ -217,6 +218,9 @@ struct NeRF {
     OccupancyGrid occupancy_grid;
     DatasetWorkspace dataset_ws;
     bool is_image_data_loaded = false;
+    // Synthetic scene: a Gaussian-blob field (rows {cx, cy, cz, s, p, r, g, b}
+    // in world units) that the renderer evaluates in place of the network.
+    DeviceBuffer<float> analytic_blobs;
     bool holds_device_memory() const {
         return params.params_fp.size() || occupancy_grid.bitfield.size() || dataset_ws.images.size() ||
                network.workspace.bytes();
@@ -224,6 +228,7 @@ struct NeRF {
     void free_device_memory() {
         params.free_allocations();
         occupancy_grid.free_device_memory();
+        analytic_blobs.release();
     }
 };
 

@@ -17,7 +17,7 @@
 namespace turbo {
 
 void RenderingWorkspace::enlarge(uint32_t rays, uint32_t samples, size_t e) {
-    if (rays <= ray_capacity && samples <= sample_capacity && features.size() >= 32 * (size_t)samples * e) return;
+    if (rays <= ray_capacity && samples <= sample_capacity && density.size() >= (size_t)samples * e) return;
     ray_capacity = rays;
     sample_capacity = samples;
     ray_ori.resize(3 * (size_t)rays);
@@ -39,7 +39,7 @@ void RenderingWorkspace::enlarge(uint32_t rays, uint32_t samples, size_t e) {
     s_dt.resize(samples);
     s_m.resize(samples);
     s_dtn.resize(samples);
-    features.resize(32 * (size_t)samples * e);
+    features.resize(32 * (size_t)samples * 4);  // fp32 in both precisions
     density.resize((size_t)samples * e);
     color.resize(3 * (size_t)samples * e);
 }
@@ -53,7 +53,10 @@ uint64_t Renderer::render_pixels(hipStream_t s, const RenderRequest& req, const 
     const tn_camera_t cam = req.camera.c();
     const tn_bbox_t bbox = r.bounding_box.c();
     const float aabb = r.proxy->training_bbox.get().size();
-    const tn_occupancy_grid_t grid = nerf.occupancy_grid.view();
+    // the reference's renderer picks a sample's cascade from the step
+    // normalised by the scene size (rendering-kernels.cu:445,447)
+    tn_occupancy_grid_t grid = nerf.occupancy_grid.view();
+    grid.level_dt_scale = 1.0f / aabb;
     const float dt_min = st.min_step_size;
     const float dt_max = 8.0f * dt_min;  // reference renderer.cu:305
     const Transform4f& tr = r.proxy->transform.get();
@@ -86,11 +89,22 @@ uint64_t Renderer::render_pixels(hipStream_t s, const RenderRequest& req, const 
                                  ws.ray_t.data(), ws.offset.data(), ws.alive.data(), ws.n_steps.data(), ws.s_img.data(),
                                  ws.s_t.data(), ws.s_pos.data(), ws.s_dir.data(), ws.s_dt.data(), ws.s_m.data(),
                                  ws.s_dtn.data(), ts));
-        net.inference(s, nerf.params, S, ws.info.data(), S, ws.s_pos.data(), ws.s_dir.data(), ws.features.data(),
-                      ws.density.data(), ws.color.data(), true);
-        TN_TRY(tn_composite_forward(net.config.precision, ws.info.data(), S, S, R, st.min_transmittance, 1,
-                                    ws.n_steps.data(), ws.offset.data(), ws.density.data(), ws.color.data(),
-                                    ws.s_dt.data(), ws.rgba.data(), ts));
+        if (nerf.analytic_blobs.size()) {
+            // synthetic scene: sigma and colour of the blob field
+            TN_TRY(tn_blob_eval(ws.info.data(), S, S, aabb, nerf.analytic_blobs.data(),
+                                (uint32_t)(nerf.analytic_blobs.size() / 8), ws.s_pos.data(),
+                                reinterpret_cast<float*>(ws.density.data()), reinterpret_cast<float*>(ws.color.data()),
+                                ts));
+            TN_TRY(tn_composite_forward(0, ws.info.data(), S, S, R, st.min_transmittance, 0, ws.n_steps.data(),
+                                        ws.offset.data(), ws.density.data(), ws.color.data(), ws.s_dt.data(),
+                                        ws.rgba.data(), ts));
+        } else {
+            net.inference(s, nerf.params, S, ws.info.data(), S, ws.s_pos.data(), ws.s_dir.data(), ws.features.data(),
+                          ws.density.data(), ws.color.data(), true);
+            TN_TRY(tn_composite_forward(net.config.precision, ws.info.data(), S, S, R, st.min_transmittance, 1,
+                                        ws.n_steps.data(), ws.offset.data(), ws.density.data(), ws.color.data(),
+                                        ws.s_dt.data(), ws.rgba.data(), ts));
+        }
         TN_TRY(tn_render_write_rgba(m, ws.info.data(), R, ws.pixel.data(), ws.alive.data(), ws.rgba.data(), image, ts));
         TN_HIP(hipMemcpyAsync(info_h, ws.info.data(), sizeof(info_h), hipMemcpyDeviceToHost, s));
         TN_HIP(hipStreamSynchronize(s));
@@ -137,13 +151,19 @@ void NeRFRenderingController::submit(std::shared_ptr<RenderRequest> req) {
     // the first visible NeRF that can render (scenes of several NeRFs are not
     // blended in this build)
     const NeRFRenderable* r = nullptr;
+    int n_visible = 0;
     for (auto& x : req->renderables)
-        if (x.proxy && x.proxy->can_render && x.proxy->is_visible) { r = &x; break; }
+        if (x.proxy && x.proxy->can_render && x.proxy->is_visible) {
+            if (!r) r = &x;
+            ++n_visible;
+        }
+    if (n_visible > 1) throw std::runtime_error("submit: a request may hold one visible NeRF that can render");
     DeviceBuffer<float> image(4 * (size_t)n_pixels);
     image.zero(s);
     last_sample_count = 0;
     if (r) {
-        const size_t e = r->proxy->nerfs[0].network.elem_bytes();
+        const NeRF& nerf = r->proxy->nerfs[0];
+        const size_t e = nerf.analytic_blobs.size() ? 4 : nerf.network.elem_bytes();
         // LinearChunks tasks of batch_size/8 rays (reference nerf-rendering-controller.cu:84)
         const uint32_t rays_per_task = std::max(1u, batch_size_ / 8);
         renderer_.ws.enlarge(rays_per_task, std::max(batch_size_ * 4u, 1u << 22), e);

@@ -75,13 +75,25 @@ TN_DEVICE void st_nt(uint32_t* p, const uint32_t (&a)[4]) {
 // gradient read only; otherwise all 32 parameters of the line are read and
 // written back whole (skipped entries keep their values bit for bit), since
 // partially written lines cost more than the bytes they save.
+// Learning rate of the step: lr0 / (1 + decay * step) with the step read on
+// the device when d_step is given (reference schedule, nerf-network.cu:256),
+// else lr0 as passed. Contraction off: the same float expression as the host's.
+TN_DEVICE float step_lr(float lr0, const uint32_t* d_step, float decay) {
+#pragma clang fp contract(off)
+    return d_step ? lr0 / (1.0f + decay * (float)(*d_step)) : lr0;
+}
+
 template <typename TL>
-__global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float inv_loss_scale, float lr, float beta1,
-                            float beta2, float eps, float l2, float* __restrict__ w, TL* __restrict__ w_lp,
-                            float* __restrict__ grad, float* __restrict__ m, float* __restrict__ v,
-                            uint32_t* __restrict__ steps) {
+__global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float inv_scale0,
+                            const uint32_t* __restrict__ info, float lr0, const uint32_t* __restrict__ d_step,
+                            float lr_decay, float beta1, float beta2, float eps, float l2, float* __restrict__ w,
+                            TL* __restrict__ w_lp, float* __restrict__ grad, float* __restrict__ m,
+                            float* __restrict__ v, uint32_t* __restrict__ steps) {
     const uint32_t i4 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (i4 >= n) return;
+    // gradients scaled by 2^ceil(log2(rays)) as well come back exactly
+    const float inv_loss_scale = info ? inv_scale0 / ray_pow2(info[0]) : inv_scale0;
+    const float lr = step_lr(lr0, d_step, lr_decay);
     if (i4 + 3 < n) {
         const float4 g4 = *reinterpret_cast<const float4*>(grad + i4);
         const bool any = g4.x != 0.0f || g4.y != 0.0f || g4.z != 0.0f || g4.w != 0.0f;
@@ -141,10 +153,11 @@ using namespace tn;
 // lp_dtype: 0 = no low-precision copy (plain fp32 training), 1 = fp16 copy.
 // Parameters [0, n_grid) are hash-table entries, [n_grid, n_net_end) network
 // weights (L2-regularised), the rest (if any) neither.
-extern "C" int tn_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float loss_scale, float learning_rate,
-                            float beta1, float beta2, float epsilon, float l2_reg, float* weights, int lp_dtype,
-                            void* weights_lp, float* gradients, float* first_moments, float* second_moments,
-                            uint32_t* param_steps, hipStream_t stream) {
+extern "C" int tn_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float loss_scale,
+                            const uint32_t* scale_by_rays_info, float learning_rate, const uint32_t* d_step,
+                            float lr_decay, float beta1, float beta2, float epsilon, float l2_reg, float* weights,
+                            int lp_dtype, void* weights_lp, float* gradients, float* first_moments,
+                            float* second_moments, uint32_t* param_steps, hipStream_t stream) {
     TN_CHECK_ARG(weights && gradients && first_moments && second_moments && param_steps && loss_scale > 0.0f);
     TN_CHECK_ARG(n_grid <= n_net_end && n_net_end <= n && (lp_dtype == 0 || weights_lp));
     // 16-byte vector access (8 bytes for the fp16 copy)
@@ -155,12 +168,13 @@ extern "C" int tn_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, flo
     const unsigned blocks = blocks_for((n + 3) / 4, 256);
     const float inv = 1.0f / loss_scale;
     if (lp_dtype == 0)
-        adam_kernel<float><<<blocks, 256, 0, stream>>>(n, n_grid, n_net_end, inv, learning_rate, beta1, beta2, epsilon,
-                                                       l2_reg, weights, nullptr, gradients, first_moments,
-                                                       second_moments, param_steps);
+        adam_kernel<float><<<blocks, 256, 0, stream>>>(n, n_grid, n_net_end, inv, scale_by_rays_info, learning_rate,
+                                                       d_step, lr_decay, beta1, beta2, epsilon, l2_reg, weights,
+                                                       nullptr, gradients, first_moments, second_moments, param_steps);
     else if (lp_dtype == 1)
-        adam_kernel<_Float16><<<blocks, 256, 0, stream>>>(n, n_grid, n_net_end, inv, learning_rate, beta1, beta2,
-                                                          epsilon, l2_reg, weights, static_cast<_Float16*>(weights_lp),
+        adam_kernel<_Float16><<<blocks, 256, 0, stream>>>(n, n_grid, n_net_end, inv, scale_by_rays_info,
+                                                          learning_rate, d_step, lr_decay, beta1, beta2, epsilon,
+                                                          l2_reg, weights, static_cast<_Float16*>(weights_lp),
                                                           gradients, first_moments, second_moments, param_steps);
     else
         return TN_ERR_INVALID_ARGUMENT;

@@ -127,6 +127,11 @@ TN_DEVICE bool bbox_intersect(const tn_bbox_t& b, float ox, float oy, float oz, 
     return tmax >= tmin;
 }
 
+// 2^ceil(log2(n)) for n >= 1 (1 for n <= 1). Training gradients are scaled
+// by it (and unscaled by its reciprocal, both exact) so that the per-ray mean
+// of the loss does not push fp16 gradients towards the subnormal range.
+TN_DEVICE float ray_pow2(uint32_t n) { return n <= 1u ? 1.0f : ldexpf(1.0f, 32 - __clz(n - 1u)); }
+
 // ---------------------------------------------------------------------------
 // occupancy grid queries
 // ---------------------------------------------------------------------------

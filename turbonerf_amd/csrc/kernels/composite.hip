@@ -335,7 +335,7 @@ composite_ray_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB,
 template <typename TIn, typename TGrad, typename R>
 __global__ void __launch_bounds__(64 * kCompositeWaves)
 composite_grad_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB, uint32_t SB, float grad_scale,
-                      float min_T, const uint32_t* __restrict__ steps, const uint32_t* __restrict__ offs,
+                      int scale_by_rays, float min_T, const uint32_t* __restrict__ steps, const uint32_t* __restrict__ offs,
                       const float* __restrict__ bg, const TIn* __restrict__ density, const TIn* __restrict__ color,
                       const float* __restrict__ dtb, float* __restrict__ dL_ddensity, TGrad* __restrict__ dL_dcolor,
                       const R* __restrict__ scratch, const Acc<R>* __restrict__ carry) {
@@ -385,7 +385,8 @@ composite_grad_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB
     if (st.inc) {
         const R suffix = Wg_tot - Q.val();
         dsig = (float)(st.dt * (st.T_next * g - suffix) * st.sigma);
-        const R k = (R)grad_scale * st.w * A;
+        const float gs = scale_by_rays ? grad_scale * ray_pow2(n_rays) : grad_scale;
+        const R k = (R)gs * st.w * A;
         gc_r = (float)(k * GR);
         gc_g = (float)(k * GG);
         gc_b = (float)(k * GB);
@@ -540,7 +541,7 @@ __global__ void step_metrics_kernel(const uint32_t* __restrict__ info, uint32_t 
 
 template <typename TIn, typename TGrad, typename R>
 int launch_composite(const uint32_t* info, uint32_t n_rays_max, uint32_t B, uint32_t PB, float grad_scale,
-                     float min_T, const uint32_t* steps, const uint32_t* offs, const float* bg, const float* target,
+                     int scale_by_rays, float min_T, const uint32_t* steps, const uint32_t* offs, const float* bg, const float* target,
                      const void* density, const void* color, const float* dt, float* loss, float* dL_ddensity,
                      void* dL_dcolor, void* scratch, hipStream_t stream) {
     unsigned char* base = static_cast<unsigned char*>(scratch);
@@ -555,7 +556,7 @@ int launch_composite(const uint32_t* info, uint32_t n_rays_max, uint32_t B, uint
     composite_ray_kernel<TIn, R><<<blocks_for(n_rays_max, kCompositeWaves), 64 * kCompositeWaves, 0, stream>>>(
         info, B, PB, n_rays_max, min_T, steps, offs, bg, target, d, c, dt, loss, rays, rec, carry);
     composite_grad_kernel<TIn, TGrad, R><<<chunk_blocks, 64 * kCompositeWaves, 0, stream>>>(
-        info, B, PB, n_rays_max, grad_scale, min_T, steps, offs, bg, d, c, dt, dL_ddensity,
+        info, B, PB, n_rays_max, grad_scale, scale_by_rays, min_T, steps, offs, bg, d, c, dt, dL_ddensity,
         static_cast<TGrad*>(dL_dcolor), rays, carry);
     TN_CHECK_LAUNCH();
     return TN_OK;
@@ -573,7 +574,8 @@ extern "C" size_t tn_composite_scratch_bytes(uint32_t n_rays_max, uint32_t sampl
 //        1 = fp16 network outputs and fp16 colour gradients,
 //        2 = fp32 in/out with fp64 accumulation (oracle mode)
 extern "C" int tn_composite_train(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t stride,
-                                  uint32_t sample_stride, float grad_scale, float min_transmittance, const uint32_t* ray_steps,
+                                  uint32_t sample_stride, float grad_scale, int scale_by_rays, float min_transmittance,
+                                  const uint32_t* ray_steps,
                                   const uint32_t* ray_offset, const float* bg_rgb, const float* target_rgba,
                                   const void* density, const void* color, const float* dt, float* loss,
                                   float* dL_ddensity, void* dL_dcolor, void* ray_scratch, size_t scratch_bytes,
@@ -585,17 +587,18 @@ extern "C" int tn_composite_train(int dtype, const uint32_t* info, uint32_t n_ra
     TN_CHECK_ARG((reinterpret_cast<uintptr_t>(ray_scratch) & 255) == 0);
     switch (dtype) {
         case 0:
-            return launch_composite<float, float, float>(info, n_rays_max, stride, sample_stride, grad_scale, min_transmittance,
+            return launch_composite<float, float, float>(info, n_rays_max, stride, sample_stride, grad_scale, scale_by_rays,
+                                                         min_transmittance,
                                                          ray_steps, ray_offset, bg_rgb, target_rgba, density, color,
                                                          dt, loss, dL_ddensity, dL_dcolor, ray_scratch, stream);
         case 1:
             return launch_composite<_Float16, _Float16, float>(info, n_rays_max, stride, sample_stride, grad_scale,
-                                                               min_transmittance, ray_steps, ray_offset, bg_rgb,
+                                                               scale_by_rays, min_transmittance, ray_steps, ray_offset, bg_rgb,
                                                                target_rgba, density, color, dt, loss, dL_ddensity,
                                                                dL_dcolor, ray_scratch, stream);
         case 2:
             return launch_composite<float, float, double>(info, n_rays_max, stride, sample_stride, grad_scale,
-                                                          min_transmittance, ray_steps, ray_offset, bg_rgb,
+                                                          scale_by_rays, min_transmittance, ray_steps, ray_offset, bg_rgb,
                                                           target_rgba, density, color, dt, loss, dL_ddensity,
                                                           dL_dcolor, ray_scratch, stream);
         default:

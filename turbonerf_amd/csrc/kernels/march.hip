@@ -19,6 +19,7 @@ struct MarchGrid {
     float res_f;
     float inv_res;
     uint32_t lim;
+    float level_scale;  // the cascade is chosen from dt * level_scale
 };
 
 TN_DEVICE MarchGrid make_grid(const tn_occupancy_grid_t& g) {
@@ -28,6 +29,7 @@ TN_DEVICE MarchGrid make_grid(const tn_occupancy_grid_t& g) {
     m.res_f = (float)g.resolution;
     m.inv_res = 1.0f / m.res_f;
     m.lim = (uint32_t)g.resolution - 1u;
+    m.level_scale = g.level_dt_scale > 0.0f ? g.level_dt_scale : 1.0f;
     return m;
 }
 
@@ -107,7 +109,7 @@ TN_DEVICE uint32_t occupied_run(const MarchGrid& g, const tn_bbox_t& bbox, const
     bool good = within;
     if (within) {
         const float dtj = get_dt(tj, cone, dt_min, dt_max);
-        const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dtj);
+        const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, dtj * g.level_scale);
         const uint32_t m = voxel_morton_index(g.res_f, g.lim, lvl, x, y, z);
         good = ((g.bits[m] >> lvl) & 1u) != 0u;
     }
@@ -124,7 +126,7 @@ TN_DEVICE uint32_t occupied_run(const MarchGrid& g, const tn_bbox_t& bbox, const
 TN_DEVICE float skip_empty(const MarchGrid& g, const tn_bbox_t& bbox, const MarchRay& r, float t, float t_max,
                            float cone, float dt_min, float dt_max, float idx, float idy, float idz, WaveCache& wc) {
     float x = r.ox + t * r.dx, y = r.oy + t * r.dy, z = r.oz + t * r.dz;
-    int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(t, cone, dt_min, dt_max));
+    int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(t, cone, dt_min, dt_max) * g.level_scale);
     for (;;) {
         t += dt_to_next_voxel(g.res_f, g.inv_res, x, y, z, r.dx, r.dy, r.dz, idx, idy, idz, dt_min, lvl);
         if (!(t < t_max)) return t;
@@ -132,7 +134,7 @@ TN_DEVICE float skip_empty(const MarchGrid& g, const tn_bbox_t& bbox, const Marc
         y = r.oy + t * r.dy;
         z = r.oz + t * r.dz;
         if (!bbox_contains(bbox, x, y, z)) return t;
-        lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(t, cone, dt_min, dt_max));
+        lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(t, cone, dt_min, dt_max) * g.level_scale);
         if (wave_occupied(g, lvl, t, x, y, z, r.dx, r.dy, r.dz, wc)) return t;
     }
 }
@@ -219,7 +221,7 @@ TN_DEVICE void lane_skip_step(const MarchGrid& g, const tn_bbox_t& bbox, float c
         q.done = true;
         return;
     }
-    const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(q.t, cone, dt_min, dt_max));
+    const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(q.t, cone, dt_min, dt_max) * g.level_scale);
     if ((g.bits[voxel_morton_index(g.res_f, g.lim, lvl, x, y, z)] >> lvl) & 1u)
         q.ready = true;
     else
@@ -298,7 +300,7 @@ TN_DEVICE void march_count_lanes(uint32_t first, uint32_t K, uint32_t n_rays, ui
 // its wave cache, which is faster per ray.
 constexpr uint32_t kLaneRaysMin = 4;
 
-inline TN_DEVICE uint32_t rays_per_wave(uint32_t n_rays, uint32_t n_waves) {
+TN_DEVICE uint32_t rays_per_wave(uint32_t n_rays, uint32_t n_waves) {
     const uint32_t k = (n_rays + n_waves - 1) / n_waves;
     return k < kLaneRaysMin ? 1u : min(k, 64u);
 }
@@ -386,7 +388,8 @@ __global__ void scan_tile_prefix_kernel(uint32_t n_tiles, uint32_t* __restrict__
     }
 }
 
-// info[0] = n_rays kept, info[1] = n_samples, info[2] = status.
+// info[0] = n_rays kept, info[1] = n_samples, info[2] = status, info[3] =
+// rays scanned (the batch's generated ray count).
 // mode 0 (training, the reference's rule): j = last ray whose offset is below
 //   max_samples; rays [0, j) are kept and n_samples = offset[j].
 // mode 1 (rendering): every ray whose samples end within max_samples is kept.
@@ -396,6 +399,7 @@ __global__ void scan_write_select_kernel(uint32_t n, const uint32_t* __restrict_
                                          uint32_t* __restrict__ offsets, uint32_t* __restrict__ info) {
     __shared__ uint32_t smem[kScanThreads / 64];
     if (d_n) n = min(*d_n, n);
+    if (blockIdx.x == 0 && threadIdx.x == 0) info[3] = n;
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
     uint32_t s = 0;
@@ -510,7 +514,7 @@ TN_DEVICE void march_generate_lanes(uint32_t first, uint32_t K, uint32_t n_rays,
                     out_ray[off + q.n] = i;
                     ++q.n;
                 } else {
-                    const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(q.t, cone, dt_min, dt_max));
+                    const int lvl = grid_level_at(g.n_levels, g.inv_res, x, y, z, get_dt(q.t, cone, dt_min, dt_max) * g.level_scale);
                     if ((g.bits[voxel_morton_index(g.res_f, g.lim, lvl, x, y, z)] >> lvl) & 1u)
                         q.ready = true;
                     else

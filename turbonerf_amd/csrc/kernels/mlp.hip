@@ -16,6 +16,16 @@
 // tiles it processes and the workgroup sums them before one partial per
 // workgroup goes to memory.
 //
+// Mixed precision keeps the fp16 operands of the first layer of each MLP
+// exact to about 2^-22: the hash features, the hidden layer h1, the density
+// outputs and the SH coefficients enter their MFMAs as two fp16 operands,
+// hi = fp16(v) and lo = fp16(v - hi), against the same weight operand. Plain
+// fp16 rounding of these inputs (2^-11) moves the density and colour enough
+// to flip ReLU masks and compositing terms downstream; it was the largest
+// part of the gradient error against an FP64 evaluation (about 6e-3 on the
+// hash table at 2^18 samples, under 7e-4 with the split). The wider layers
+// (h2 -> W4, h3 -> W5) and every backward operand stay single fp16.
+//
 // Weight layout in the flat parameter buffer (after the hash table):
 // W1[64][32] W2[16][64] | W3[64][32] W4[64][64] W5[3][64], row-major [out][in].
 #include "common.hpp"
@@ -37,6 +47,16 @@ struct PrecF16 {
 #pragma unroll
         for (int j = 0; j < 8; ++j) o[j] = (_Float16)v[j];
         return o;
+    }
+    // v = hi + lo to about 2^-22 relative: hi = fp16(v), lo = fp16(v - hi)
+    // (v - hi is exact in fp32)
+    static TN_DEVICE void split(const float* v, Op& hi, Op& lo) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const _Float16 a = (_Float16)v[j];
+            hi[j] = a;
+            lo[j] = (_Float16)(v[j] - (float)a);
+        }
     }
 };
 
@@ -216,6 +236,19 @@ TN_DEVICE OpTile<P> to_ops(const f16v& D) {
     return t;
 }
 
+// lo part of a tile whose hi part is `hi` (mixed precision only)
+template <class P>
+TN_DEVICE OpTile<P> lo_ops(const f16v& D, const OpTile<P>& hi) {
+    OpTile<P> t;
+    if constexpr (P::SL == 8) {
+#pragma unroll
+        for (int ks = 0; ks < kKsPerTile<P>; ++ks)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) t.k[ks][j] = (_Float16)(D[8 * ks + j] - (float)hi.k[ks][j]);
+    }
+    return t;
+}
+
 // accumulator register r of a kept tile
 template <class P>
 TN_DEVICE float op_get(const OpTile<P>& t, int r) {
@@ -231,6 +264,20 @@ TN_DEVICE f16v gemm_op(const typename P::T* lds, const LayerDesc& L, int mt, con
 #pragma unroll
     for (int ks = 0; ks < n_ks; ++ks)
         acc = P::mma(load_a<P>(lds, L, mt, ks, lane), x[ks / kKsPerTile<P>].k[ks % kKsPerTile<P>], acc);
+    return acc;
+}
+
+// y = A . (x + x_lo) over K-steps from kept tiles (mixed precision)
+template <class P, int NT_IN>
+TN_DEVICE f16v gemm_op2(const typename P::T* lds, const LayerDesc& L, int mt, const OpTile<P> (&x)[NT_IN],
+                        const OpTile<P> (&xlo)[NT_IN], int lane, int n_ks) {
+    f16v acc = zero16();
+#pragma unroll
+    for (int ks = 0; ks < n_ks; ++ks) {
+        const typename P::Op a = load_a<P>(lds, L, mt, ks, lane);
+        acc = P::mma(a, x[ks / kKsPerTile<P>].k[ks % kKsPerTile<P>], acc);
+        acc = P::mma(a, xlo[ks / kKsPerTile<P>].k[ks % kKsPerTile<P>], acc);
+    }
     return acc;
 }
 
@@ -268,12 +315,21 @@ struct Tile {
     f16v o;    // colour logits (rows 0..2)
 };
 
+// lo parts of the split inputs (mixed precision; used by the first layers'
+// MFMAs only, so they live just through the forward pass)
+template <class P>
+struct TileLo {
+    typename P::Op x[32 / P::KC];
+    typename P::Op sh[16 / P::KC];
+};
+
 // Inputs of one tile. Lanes past the batch load sample 0 and zero what they
 // loaded (no divergent branches around the loads); element offsets are
 // 32-bit (the host checks 32 * stride < 2^32).
 template <class P, typename TF>
-TN_DEVICE void load_inputs(Tile<P>& tl, bool valid, uint32_t sample, int h, const TF* __restrict__ feat,
-                           uint32_t FS, const float* __restrict__ dir, uint32_t B, bool need_sh) {
+TN_DEVICE void load_inputs(Tile<P>& tl, TileLo<P>& lo, bool valid, uint32_t sample, int h,
+                           const TF* __restrict__ feat, uint32_t FS, const float* __restrict__ dir, uint32_t B,
+                           bool need_sh) {
     const uint32_t sl = valid ? sample : 0u;
 #pragma unroll
     for (int ks = 0; ks < 32 / P::KC; ++ks) {
@@ -284,7 +340,8 @@ TN_DEVICE void load_inputs(Tile<P>& tl, bool valid, uint32_t sample, int h, cons
             const float x = (float)feat[row * FS + sl];
             v[j] = valid ? x : 0.0f;
         }
-        tl.x[ks] = P::from(v);
+        if constexpr (P::SL == 8) P::split(v, tl.x[ks], lo.x[ks]);
+        else tl.x[ks] = P::from(v);
     }
     if (need_sh) {
         float shv[16];
@@ -296,34 +353,53 @@ TN_DEVICE void load_inputs(Tile<P>& tl, bool valid, uint32_t sample, int h, cons
             float v[P::SL];
 #pragma unroll
             for (int j = 0; j < P::SL; ++j) v[j] = shv[nat_in<P>(ks, h, j)];
-            tl.sh[ks] = P::from(v);
+            if constexpr (P::SL == 8) P::split(v, tl.sh[ks], lo.sh[ks]);
+            else tl.sh[ks] = P::from(v);
         }
     }
 }
 
 template <class P>
-TN_DEVICE void forward_tile(Tile<P>& tl, const FwdLds<P>& W, int lane, bool color) {
+TN_DEVICE void forward_tile(Tile<P>& tl, const TileLo<P>& lo, const FwdLds<P>& W, int lane, bool color) {
     using N = Net<P>;
+    constexpr bool kSplit = P::SL == 8;
+    OpTile<P> h1lo[2];
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
         f16v acc = zero16();
 #pragma unroll
-        for (int ks = 0; ks < N::L1.n_ks; ++ks) acc = P::mma(load_a<P>(W.l1, N::L1, mt, ks, lane), tl.x[ks], acc);
-        tl.h1[mt] = to_ops<P>(relu(acc));
+        for (int ks = 0; ks < N::L1.n_ks; ++ks) {
+            const typename P::Op a = load_a<P>(W.l1, N::L1, mt, ks, lane);
+            acc = P::mma(a, tl.x[ks], acc);
+            if constexpr (kSplit) acc = P::mma(a, lo.x[ks], acc);
+        }
+        const f16v r = relu(acc);
+        tl.h1[mt] = to_ops<P>(r);
+        if constexpr (kSplit) h1lo[mt] = lo_ops<P>(r, tl.h1[mt]);
     }
-    const f16v s = gemm_op<P, 2>(W.l2, N::L2, 0, tl.h1, lane, N::L2.n_ks);
+    f16v s;
+    if constexpr (kSplit) s = gemm_op2<P, 2>(W.l2, N::L2, 0, tl.h1, h1lo, lane, N::L2.n_ks);
+    else s = gemm_op<P, 2>(W.l2, N::L2, 0, tl.h1, lane, N::L2.n_ks);
     tl.s0 = s[0];
     tl.s = to_ops<P>(s);
     if (!color) return;
+    const OpTile<P> slo = lo_ops<P>(s, tl.s);
     constexpr int half = kKsPerTile<P> / 2;
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
         f16v acc = zero16();
 #pragma unroll
-        for (int ks = 0; ks < half; ++ks) acc = P::mma(load_a<P>(W.l3, N::L3, mt, ks, lane), tl.s.k[ks], acc);
+        for (int ks = 0; ks < half; ++ks) {
+            const typename P::Op a = load_a<P>(W.l3, N::L3, mt, ks, lane);
+            acc = P::mma(a, tl.s.k[ks], acc);
+            if constexpr (kSplit) acc = P::mma(a, slo.k[ks], acc);
+        }
 #pragma unroll
-        for (int ks = 0; ks < 16 / P::KC; ++ks)
-            acc = P::mma(load_a<P>(W.l3, N::L3, mt, half + ks, lane), tl.sh[ks], acc);
+        for (int ks = 0; ks < 16 / P::KC; ++ks) {
+            const typename P::Op a = load_a<P>(W.l3, N::L3, mt, half + ks, lane);
+            acc = P::mma(a, tl.sh[ks], acc);
+            if constexpr (kSplit) acc = P::mma(a, lo.sh[ks], acc);
+        }
         tl.h2[mt] = to_ops<P>(relu(acc));
     }
 #pragma unroll
@@ -355,8 +431,9 @@ mlp_forward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t B
         const uint32_t sample = t * 32 + (lane & 31);
         const bool valid = sample < n;
         Tile<P> tl;
-        load_inputs<P>(tl, valid, sample, h, feat, FS, dir, B, color != 0);
-        forward_tile<P>(tl, W, lane, color != 0);
+        TileLo<P> lo;
+        load_inputs<P>(tl, lo, valid, sample, h, feat, FS, dir, B, color != 0);
+        forward_tile<P>(tl, lo, W, lane, color != 0);
         if (valid && h == 0) {
             density_out[sample] = (typename P::T)tl.s0;
             if (color) {
@@ -479,8 +556,8 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
                     const typename P::T* __restrict__ params, const TF* __restrict__ feat,
                     const float* __restrict__ dir, const float* __restrict__ sample_t,
                     const TG* __restrict__ dL_drgb, uint32_t GS, const float* __restrict__ dL_ds,
-                    const float* __restrict__ dL_ds_extra, float loss_scale, TF* __restrict__ dfeat,
-                    float* __restrict__ dw_partial) {
+                    const float* __restrict__ dL_ds_extra, float loss_scale, int scale_by_rays,
+                    TF* __restrict__ dfeat, float* __restrict__ dw_partial) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     using T = typename P::T;
     using N = Net<P>;
@@ -494,6 +571,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
 
     const uint32_t n = info ? min(info[1], n_max) : n_max;
     const uint32_t n_grp = (n + kGrp - 1) / kGrp;
+    const float ls = scale_by_rays ? loss_scale * ray_pow2(info[0]) : loss_scale;
     const int lane = (int)__lane_id();
     const int h = lane >> 5;
     const int wid = threadIdx.x >> 6;
@@ -507,15 +585,16 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         const bool valid = sample < n;
         const uint32_t sl = valid ? sample : 0u;
         Tile<P> tl;
-        load_inputs<P>(tl, valid, sample, h, feat, FS, dir, B, true);
+        TileLo<P> lo;
+        load_inputs<P>(tl, lo, valid, sample, h, feat, FS, dir, B, true);
         // per-sample gradients from the compositing, loaded up front
         float g_rgb[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) g_rgb[c] = (float)dL_drgb[sl + c * GS];
-        const float g_s = loss_scale * dL_ds[sl];
-        const float g_x = dL_ds_extra ? loss_scale * dL_ds_extra[sl] : 0.0f;
+        const float g_s = ls * dL_ds[sl];
+        const float g_x = dL_ds_extra ? ls * dL_ds_extra[sl] : 0.0f;
         const float t_s = sample_t[sl];
-        forward_tile<P>(tl, W, lane, true);
+        forward_tile<P>(tl, lo, W, lane, true);
 
         // dL/do through the sigmoid (rows 0..2 live in lanes 0..31, registers 0..2)
         f16v dO = zero16();
@@ -700,9 +779,9 @@ using namespace tn;
 
 // precision: 0 = fp32 (MFMA fp32), 1 = fp16 (MFMA f16, fp32 accumulate).
 // params: the network's parameter buffer in that precision (hash table first,
-// `net_offset` = its length). features [32][feat_stride] in that precision,
-// dir [3][stride] fp32 (0.5*d + 0.5). Outputs: density (raw s) [stride] and
-// rgb [3][out_stride] in that precision.
+// `net_offset` = its length). features [32][feat_stride] fp32 (both
+// precisions), dir [3][stride] fp32 (0.5*d + 0.5). Outputs: density (raw s)
+// [stride] and rgb [3][out_stride] in the network precision.
 extern "C" int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* info, uint32_t stride,
                               uint32_t feat_stride, uint32_t out_stride, const void* params, uint32_t net_offset,
                               const void* features, const float* dir, int color, void* density_out, void* rgb_out,
@@ -715,8 +794,8 @@ extern "C" int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* inf
     if (precision == 1) {
         using P = PrecF16;
         const auto* w = static_cast<const P::T*>(params) + net_offset;
-        mlp_forward_kernel<P, P::T><<<blocks, 64 * kMlpWaves, fwd_lds_bytes<P>(), stream>>>(
-            n_max, info, stride, feat_stride, w, static_cast<const P::T*>(features), dir, color,
+        mlp_forward_kernel<P, float><<<blocks, 64 * kMlpWaves, fwd_lds_bytes<P>(), stream>>>(
+            n_max, info, stride, feat_stride, w, static_cast<const float*>(features), dir, color,
             static_cast<P::T*>(density_out), static_cast<P::T*>(rgb_out), out_stride);
     } else if (precision == 0) {
         using P = PrecF32;
@@ -738,16 +817,18 @@ extern "C" size_t tn_mlp_backward_workspace_bytes(uint32_t n_max) {
 // Backward of both MLPs for one batch. dL_drgb [3][grad_stride] (loss-scaled,
 // in the network precision), dL_ds [stride] fp32 compositing gradient,
 // dL_ds_extra (may be null) e.g. the distortion gradient; both are multiplied
-// by loss_scale and added to the density output's row 0, which is then scaled
-// by t^2 where t < 1. Writes dfeat [32][feat_stride] and adds (accumulate=1)
-// or stores the weight gradient into grad_net[9408] (fp32).
+// by loss_scale (times 2^ceil(log2(info[0])) when scale_by_rays) and added to
+// the density output's row 0, which is then scaled by t^2 where t < 1. Writes
+// dfeat [32][feat_stride] fp32 and adds (accumulate=1) or stores the weight
+// gradient into grad_net[9408] (fp32).
 extern "C" int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* info, uint32_t stride,
                                uint32_t feat_stride, const void* params, uint32_t net_offset, const void* features,
                                const float* dir, const float* sample_t, const void* dL_drgb, uint32_t grad_stride,
-                               const float* dL_ds, const float* dL_ds_extra, float loss_scale, void* dfeat,
-                               float* grad_net, int accumulate, void* workspace, size_t workspace_bytes,
+                               const float* dL_ds, const float* dL_ds_extra, float loss_scale, int scale_by_rays,
+                               void* dfeat, float* grad_net, int accumulate, void* workspace, size_t workspace_bytes,
                                hipStream_t stream) {
     TN_CHECK_ARG(params && features && dir && sample_t && dL_drgb && dL_ds && dfeat && grad_net && workspace);
+    TN_CHECK_ARG(!scale_by_rays || info);
     TN_CHECK_ARG(n_max <= stride && n_max <= feat_stride && n_max <= grad_stride);
     TN_CHECK_ARG(32ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * grad_stride < (1ull << 32));
     if (workspace_bytes < tn_mlp_backward_workspace_bytes(n_max)) return TN_ERR_OUT_OF_MEMORY;
@@ -757,18 +838,18 @@ extern "C" int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* in
     if (precision == 1) {
         using P = PrecF16;
         const auto* w = static_cast<const P::T*>(params) + net_offset;
-        if (!allow_lds(mlp_backward_kernel<P, P::T, P::T>, bwd_lds_bytes<P>())) return TN_ERR_LAUNCH;
-        mlp_backward_kernel<P, P::T, P::T><<<blocks, 64 * kMlpWaves, bwd_lds_bytes<P>(), stream>>>(
-            n_max, info, stride, feat_stride, w, static_cast<const P::T*>(features), dir, sample_t,
-            static_cast<const P::T*>(dL_drgb), grad_stride, dL_ds, dL_ds_extra, loss_scale,
-            static_cast<P::T*>(dfeat), partial);
+        if (!allow_lds(mlp_backward_kernel<P, float, P::T>, bwd_lds_bytes<P>())) return TN_ERR_LAUNCH;
+        mlp_backward_kernel<P, float, P::T><<<blocks, 64 * kMlpWaves, bwd_lds_bytes<P>(), stream>>>(
+            n_max, info, stride, feat_stride, w, static_cast<const float*>(features), dir, sample_t,
+            static_cast<const P::T*>(dL_drgb), grad_stride, dL_ds, dL_ds_extra, loss_scale, scale_by_rays,
+            static_cast<float*>(dfeat), partial);
     } else if (precision == 0) {
         using P = PrecF32;
         const auto* w = static_cast<const P::T*>(params) + net_offset;
         if (!allow_lds(mlp_backward_kernel<P, P::T, P::T>, bwd_lds_bytes<P>())) return TN_ERR_LAUNCH;
         mlp_backward_kernel<P, P::T, P::T><<<blocks, 64 * kMlpWaves, bwd_lds_bytes<P>(), stream>>>(
             n_max, info, stride, feat_stride, w, static_cast<const P::T*>(features), dir, sample_t,
-            static_cast<const P::T*>(dL_drgb), grad_stride, dL_ds, dL_ds_extra, loss_scale,
+            static_cast<const P::T*>(dL_drgb), grad_stride, dL_ds, dL_ds_extra, loss_scale, scale_by_rays,
             static_cast<P::T*>(dfeat), partial);
     } else {
         return TN_ERR_INVALID_ARGUMENT;

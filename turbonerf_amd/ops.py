@@ -128,7 +128,7 @@ _DTYPE_CODE = {torch.float32: 0, torch.float16: 1}
 def composite_train(info: torch.Tensor, n_rays_max: int, stride: int, sample_stride: int, steps: torch.Tensor,
                     offs: torch.Tensor, bg: torch.Tensor, target: torch.Tensor, density: torch.Tensor,
                     color: torch.Tensor, dt: torch.Tensor, grad_scale: float = 1.0, min_T: float = 1e-4,
-                    fp64_accumulate: bool = False):
+                    fp64_accumulate: bool = False, scale_by_rays: bool = False):
     """Returns (loss[4*stride], dL_ddensity[sample_stride], dL_dcolor[3*sample_stride])."""
     code = 2 if fp64_accumulate else _DTYPE_CODE[density.dtype]
     if fp64_accumulate and density.dtype != torch.float32:
@@ -138,7 +138,8 @@ def composite_train(info: torch.Tensor, n_rays_max: int, stride: int, sample_str
     dLc = torch.zeros(3 * sample_stride, dtype=density.dtype, device="cuda")
     nb = query("tn_composite_scratch_bytes", n_rays_max, sample_stride)
     scratch = torch.empty(nb // 4, dtype=torch.float32, device="cuda")
-    call("tn_composite_train", code, ptr(info), n_rays_max, stride, sample_stride, grad_scale, min_T, ptr(steps), ptr(offs),
+    call("tn_composite_train", code, ptr(info), n_rays_max, stride, sample_stride, grad_scale, int(scale_by_rays), min_T,
+         ptr(steps), ptr(offs),
          ptr(bg), ptr(target), ptr(density), ptr(color), ptr(dt), ptr(loss), ptr(dLd), ptr(dLc), ptr(scratch), nb,
          stream_ptr())
     return loss, dLd, dLc
@@ -221,7 +222,9 @@ _PREC = {torch.float32: 0, torch.float16: 1}
 
 def hashgrid_forward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: int, table: torch.Tensor,
                      feat_dtype=None, info: torch.Tensor | None = None, out_stride: int | None = None):
-    feat_dtype = table.dtype if feat_dtype is None else feat_dtype
+    """Features [32][out_stride]; fp32 unless feat_dtype says otherwise (the
+    MLP kernels take fp32 features in both precisions)."""
+    feat_dtype = torch.float32 if feat_dtype is None else feat_dtype
     out_stride = stride if out_stride is None else out_stride
     out = torch.zeros(2 * grid.n_levels * out_stride, dtype=feat_dtype, device="cuda")
     call("tn_hashgrid_forward", _PREC[table.dtype], _PREC[feat_dtype], n, ptr(info), stride, out_stride, grid,
@@ -250,6 +253,8 @@ def hashgrid_backward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: i
 def mlp_forward(params: torch.Tensor, net_offset: int, n: int, feat: torch.Tensor, feat_stride: int,
                 dir: torch.Tensor | None, stride: int, color: bool = True, info: torch.Tensor | None = None):
     prec = _PREC[params.dtype]
+    if feat.dtype != torch.float32 or feat.numel() < 32 * feat_stride:
+        raise ValueError("the MLP kernels take fp32 features [32][feat_stride]")
     dens = torch.zeros(stride, dtype=params.dtype, device="cuda")
     rgb = torch.zeros(3 * stride, dtype=params.dtype, device="cuda") if color else None
     call("tn_mlp_forward", prec, n, ptr(info), stride, feat_stride, stride, ptr(params), net_offset, ptr(feat),
@@ -260,21 +265,27 @@ def mlp_forward(params: torch.Tensor, net_offset: int, n: int, feat: torch.Tenso
 def mlp_backward(params: torch.Tensor, net_offset: int, n: int, feat: torch.Tensor, feat_stride: int,
                  dir: torch.Tensor, stride: int, sample_t: torch.Tensor, dL_drgb: torch.Tensor, dL_ds: torch.Tensor,
                  loss_scale: float, grad_net: torch.Tensor, dL_ds_extra: torch.Tensor | None = None,
-                 accumulate: bool = False, info: torch.Tensor | None = None):
-    """Returns the feature gradients [32][feat_stride]; weight gradients go to grad_net."""
+                 accumulate: bool = False, info: torch.Tensor | None = None, scale_by_rays: bool = False):
+    """Returns the fp32 feature gradients [32][feat_stride]; weight gradients go to grad_net."""
     prec = _PREC[params.dtype]
-    dfeat = torch.zeros(32 * feat_stride, dtype=params.dtype, device="cuda")
+    if feat.dtype != torch.float32 or feat.numel() < 32 * feat_stride:
+        raise ValueError("the MLP kernels take fp32 features [32][feat_stride]")
+    if dL_drgb.dtype != params.dtype:
+        raise ValueError("dL_drgb is in the network precision")
+    dfeat = torch.zeros(32 * feat_stride, dtype=torch.float32, device="cuda")
     nb = query("tn_mlp_backward_workspace_bytes", n)
     ws = torch.empty(max(nb, 4), dtype=torch.uint8, device="cuda")
     call("tn_mlp_backward", prec, n, ptr(info), stride, feat_stride, ptr(params), net_offset, ptr(feat), ptr(dir),
-         ptr(sample_t), ptr(dL_drgb), stride, ptr(dL_ds), ptr(dL_ds_extra), loss_scale, ptr(dfeat), ptr(grad_net),
+         ptr(sample_t), ptr(dL_drgb), stride, ptr(dL_ds), ptr(dL_ds_extra), loss_scale, int(scale_by_rays), ptr(dfeat),
+         ptr(grad_net),
          int(accumulate), ptr(ws), nb, stream_ptr())
     return dfeat
 
 
 def adam_step(n_grid: int, n_net_end: int, loss_scale: float, lr: float, beta1: float, beta2: float, eps: float,
               l2: float, w: torch.Tensor, w_lp: torch.Tensor | None, grad: torch.Tensor, m: torch.Tensor,
-              v: torch.Tensor, steps: torch.Tensor) -> None:
+              v: torch.Tensor, steps: torch.Tensor, scale_by_rays_info: torch.Tensor | None = None,
+              d_step: torch.Tensor | None = None, lr_decay: float = 0.0) -> None:
     lp = 0 if w_lp is None else _PREC[w_lp.dtype]
-    call("tn_adam_step", w.numel(), n_grid, n_net_end, loss_scale, lr, beta1, beta2, eps, l2, ptr(w), lp, ptr(w_lp),
-         ptr(grad), ptr(m), ptr(v), ptr(steps), stream_ptr())
+    call("tn_adam_step", w.numel(), n_grid, n_net_end, loss_scale, ptr(scale_by_rays_info), lr, ptr(d_step), lr_decay,
+         beta1, beta2, eps, l2, ptr(w), lp, ptr(w_lp), ptr(grad), ptr(m), ptr(v), ptr(steps), stream_ptr())
