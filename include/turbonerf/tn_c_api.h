@@ -124,6 +124,21 @@ int tn_composite_train(int dtype, const uint32_t* info, uint32_t n_rays_max, uin
     float* dL_ddensity, void* dL_dcolor, void* ray_scratch, size_t scratch_bytes,
     tn_stream_t stream);
 
+/* tn_composite_train without a workspace argument, allocating nothing: its
+ * scratch is a static 32 MiB device array (one per GPU, held until the
+ * process ends). No counterpart in the reference: for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). Returns TN_ERR_OUT_OF_MEMORY when
+ * tn_composite_scratch_bytes(n_rays_max, sample_stride) exceeds
+ * tn_composite_train_noalloc_max_bytes(). Calls that overlap on one GPU share
+ * the array and race. */
+size_t tn_composite_train_noalloc_max_bytes(void);
+int tn_composite_train_noalloc(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t stride,
+    uint32_t sample_stride, float grad_scale, int scale_by_rays, float min_transmittance,
+    const uint32_t* ray_steps, const uint32_t* ray_offset, const float* bg_rgb, const float* target_rgba,
+    const void* density, const void* color, const float* dt, float* loss, float* dL_ddensity,
+    void* dL_dcolor, tn_stream_t stream);
+
 /* Forward-only compositing to premultiplied rgba[4][out_stride] per ray
  * (reference density_to_ray_rgba_forward, training-kernels-fused.cuh:227-264).
  * raw_density != 0: density holds network outputs, else sigma itself. */

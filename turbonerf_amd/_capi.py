@@ -109,6 +109,9 @@ _SIGS: dict[str, tuple] = {
     "tn_composite_scratch_bytes": (sz, [u32, u32]),
     "tn_composite_train": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, C.c_int, f32, vp, vp, vp, vp, vp, vp, vp, vp,
                                      vp, vp, vp, sz, vp]),
+    "tn_composite_train_noalloc": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, C.c_int, f32, vp, vp, vp, vp, vp, vp,
+                                             vp, vp, vp, vp, vp]),
+    "tn_composite_train_noalloc_max_bytes": (sz, []),
     "tn_composite_forward": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, C.c_int, vp, vp, vp, vp, vp, vp, vp]),
     "tn_reduce_loss": (C.c_int, [vp, u32, u32, vp, vp, vp]),
     "tn_step_metrics": (C.c_int, [vp, u32, vp, vp, vp, vp, vp]),

@@ -29,6 +29,8 @@
 //      (T < min_T) get zero gradient, as in the reference.
 // Forward-only compositing (rendering) walks each wave's window of rays.
 
+#include <mutex>
+
 #include "segscan.hpp"
 
 namespace tn {
@@ -655,4 +657,55 @@ extern "C" int tn_composite_forward(int dtype, const uint32_t* info, uint32_t st
     }
     TN_CHECK_LAUNCH();
     return TN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant for callers that cannot allocate (code under graph
+// capture, serving loops): the same kernels with their scratch in a static
+// device array instead of a caller-passed workspace. Each GPU the library
+// runs on holds its own copy until the process ends; calls that overlap on
+// one GPU share it and race.
+// ---------------------------------------------------------------------------
+namespace tn {
+constexpr size_t kStaticScratchBytes = (size_t)32 << 20;
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+}  // namespace tn
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's copy of `scratch` (in host code the
+// array's name is only a stand-in): looked up once per device, with that
+// device current, and kept.
+int static_scratch(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return TN_ERR_LAUNCH;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(tn::scratch)) != hipSuccess || p == nullptr) return TN_ERR_LAUNCH;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TN_OK;
+}
+}  // namespace
+
+extern "C" size_t tn_composite_train_noalloc_max_bytes(void) { return kStaticScratchBytes; }
+
+extern "C" int tn_composite_train_noalloc(int dtype, const uint32_t* info, uint32_t n_rays_max, uint32_t stride,
+                                          uint32_t sample_stride, float grad_scale, int scale_by_rays,
+                                          float min_transmittance, const uint32_t* ray_steps,
+                                          const uint32_t* ray_offset, const float* bg_rgb, const float* target_rgba,
+                                          const void* density, const void* color, const float* dt, float* loss,
+                                          float* dL_ddensity, void* dL_dcolor, hipStream_t stream) {
+    if (tn_composite_scratch_bytes(n_rays_max, sample_stride) > kStaticScratchBytes) return TN_ERR_OUT_OF_MEMORY;
+    void* p = nullptr;
+    const int rc = static_scratch(&p);
+    if (rc != TN_OK) return rc;
+    return tn_composite_train(dtype, info, n_rays_max, stride, sample_stride, grad_scale, scale_by_rays,
+                              min_transmittance, ray_steps, ray_offset, bg_rgb, target_rgba, density, color, dt, loss,
+                              dL_ddensity, dL_dcolor, p, kStaticScratchBytes, stream);
 }
