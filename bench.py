@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Training-step throughput of the TurboNeRF hot path on one MI355X.
 
-Workload (the reference's full training step, src/core/trainer.cu:281-406 and
+Workload (the reference's full training step, src/core/trainer.cu:313-349 and
 src/core/nerf-network.cu:547-781): hash grid 16 levels x 2 features, 2^19
 entries per level, resolution 16 -> 2048, table init U(+-1e-4); density MLP
 32 -> 64 -> 16, colour MLP (16 + 16 SH) -> 64 -> 64 -> 3; 2^18 compacted
