@@ -11,10 +11,12 @@ kernel, on the default 16-level layout (T = 2^19):
   fixed-point scale follows each bin's largest value), and a non-finite
   gradient (the bin falls back to float sums and the inf reaches the table).
 
-Bar: norm-wise 1e-6 against fp64; for the binned kernel also per entry
-|err| <= 2e-6 sum|w g| (fp32 products and run sums, summed exactly) plus
-6.4e-8 max|g| of the level (the fixed-point rounding: 2^-S per record, at
-most 2^-31 of a bin's largest record, which is a run of at most 64 samples)."""
+Bar: norm-wise 1e-6 against fp64 (atomic kernel), 2e-6 (binned); for the
+binned kernel also per entry |err| <= 6e-6 sum|w g| (fp32 products and run
+sums, 2e-6, and each record's value rounded to 17 significant bits, 2^-18)
+plus 6.4e-8 max|g| of the level (the fixed-point rounding: 2^-S per record,
+at most 2^-31 of a bin's largest record, which is a run of at most 64
+samples)."""
 import ctypes
 
 import numpy as np
@@ -52,13 +54,13 @@ def _level_of_entry(g):
     return np.repeat(lv, 2)
 
 
-def _check(got, ref, tag, g=None, ref_abs=None, dfeat=None):
+def _check(got, ref, tag, g=None, ref_abs=None, dfeat=None, binned=True):
     rel = np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300)
     msg = f"[hash bwd {tag}] rel {rel:.2e}"
-    assert rel < 1e-6, msg
+    assert rel < (2e-6 if binned else 1e-6), msg
     if ref_abs is not None:
         gmax = np.array([np.abs(dfeat[2 * l:2 * l + 2]).max() for l in range(g.n_levels)])
-        bar = 2e-6 * ref_abs + 6.4e-8 * gmax[_level_of_entry(g)]
+        bar = 6e-6 * ref_abs + 6.4e-8 * gmax[_level_of_entry(g)]
         worst = float(np.max(np.abs(got - ref) / np.maximum(bar, 1e-300)))
         msg += f" worst/bar {worst:.3f}"
         assert worst <= 1.0, msg
@@ -94,7 +96,7 @@ def test_binned_rays_vs_oracle(cuda, half):
     ops.hashgrid_backward(g, B, pos_d.view(-1), B, df_d.view(-1), atom, info=info)
     got_h, atom_h = got.cpu().numpy().astype(np.float64), atom.cpu().numpy().astype(np.float64)
     _check(got_h, ref, f"rays {'fp16' if half else 'fp32'}", g, ref_abs, dfeat_used)
-    _check(atom_h, ref, "rays atomic")
+    _check(atom_h, ref, "rays atomic", binned=False)
     # untouched entries stay exactly zero in both
     assert np.all(got_h[ref == 0] == 0)
 

@@ -178,8 +178,9 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_kernel(uint32_t n_max, const
 // none. Sums in LDS are 64-bit fixed point (integer LDS atomics run several
 // times faster than float ones here): with every |v| < 2^E in the bin and m
 // records, the scale 2^S with S = 62 - E - ceil(log2(m + 1)) cannot overflow,
-// and the sum is exact up to the rounding of each value to a multiple of 2^-S
-// (relative 2^-38 of the bin's largest value or better). Blocks of count and
+// and the sum of the records is exact up to the rounding of each value to a
+// multiple of 2^-S (relative 2^-38 of the bin's largest value or better);
+// each record carries its value to 17 significant bits (HbRecord). Blocks of count and
 // scatter work on one level each: block b serves level b % L and chunks
 // b / L + k * (blocks / L) of 256 samples.
 // ---------------------------------------------------------------------------
@@ -202,10 +203,28 @@ struct HbPlan {
     uint64_t cap;         // record capacity: n_max * L * 8
 };
 
+// A record is one contribution to both features of one entry of a bin, in 8
+// bytes: each value rounded to 17 significant bits (relative error at most
+// 2^-18, far below the fp16 gradient accumulation of the reference) and the
+// bin-relative 12-bit entry index in the freed low mantissa bits, its bits
+// 0..5 in the first value's and 6..11 in the second's. (The scatter's write
+// and the accumulation's read of the records bound both passes; 12-byte
+// records took 1.5 times as long.)
 struct HbRecord {
-    uint32_t e;  // entry within its bin
-    float v0, v1;
+    uint32_t a, b;
 };
+
+// v rounded to 17 significant bits (half away from zero; a carry may raise the
+// exponent, up to inf), low 6 bits cleared; inf and NaN keep their pattern
+TN_DEVICE uint32_t hb_round17(float v) {
+    const uint32_t u = __float_as_uint(v);
+    if ((u & 0x7F800000u) == 0x7F800000u) return u & ~63u;
+    return (u + 32u) & ~63u;
+}
+TN_DEVICE HbRecord hb_pack(uint32_t e, uint32_t r0, uint32_t r1) { return HbRecord{r0 | (e & 63u), r1 | (e >> 6)}; }
+TN_DEVICE uint32_t hb_entry(const HbRecord& r) { return (r.a & 63u) | ((r.b & 63u) << 6); }
+TN_DEVICE float hb_v0(const HbRecord& r) { return __uint_as_float(r.a & ~63u); }
+TN_DEVICE float hb_v1(const HbRecord& r) { return __uint_as_float(r.b & ~63u); }
 
 // One sample per lane, level l: the eight corner contributions of both
 // features, with runs of consecutive lanes on one entry summed into the run's
@@ -251,11 +270,11 @@ TN_DEVICE void seg_sum16(float (&a0)[8], float (&a1)[8], int key) {
 // corner entries, and a run of them (keyed by its ordinal, since a cell may
 // recur further on) emits one record per corner from its last lane. Inactive
 // lanes carry a cell no sample has, so they stay runs of their own.
-TN_DEVICE void hb_cells(const tn_hashgrid_t& g, int l, uint32_t i, bool active, uint32_t B,
-                        const float* __restrict__ pos, HashCorners& c8, uint64_t& heads) {
+TN_DEVICE void hb_cells(const tn_hashgrid_t& g, int l, float x, float y, float z, bool active, HashCorners& c8,
+                        uint64_t& heads) {
     const uint32_t lane = __lane_id();
     if (active) {
-        hash_corners(g, l, pos[i], pos[i + B], pos[i + 2 * (uint64_t)B], c8);
+        hash_corners(g, l, x, y, z, c8);
     } else {
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
@@ -270,16 +289,29 @@ TN_DEVICE void hb_cells(const tn_hashgrid_t& g, int l, uint32_t i, bool active, 
     heads = __ballot(lane == 0 || px != cx || py != cy || pz != cz);
 }
 
-// the feature gradients of sample i at level l (zero past the batch)
+// Inputs of sample i at level l: its position and both feature gradients
+// (zero past the batch). The five loads are independent, so a block loads
+// them for its next chunk while it works on the current one.
+struct HbIn {
+    float x, y, z, g0, g1;
+};
+
 template <typename TG>
-TN_DEVICE void hb_grads(int l, uint32_t i, uint32_t n, uint32_t GB, const TG* __restrict__ dfeat, float& g0,
-                        float& g1) {
-    g0 = g1 = 0.0f;
+TN_DEVICE HbIn hb_load(int l, uint32_t i, uint32_t n, uint32_t B, uint32_t GB, const float* __restrict__ pos,
+                       const TG* __restrict__ dfeat) {
+    HbIn v = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (i < n) {
-        g0 = (float)dfeat[(uint64_t)(2 * l) * GB + i];
-        g1 = (float)dfeat[(uint64_t)(2 * l + 1) * GB + i];
+        v.x = pos[i];
+        v.y = pos[i + B];
+        v.z = pos[i + 2 * (uint64_t)B];
+        v.g0 = (float)dfeat[(uint64_t)(2 * l) * GB + i];
+        v.g1 = (float)dfeat[(uint64_t)(2 * l + 1) * GB + i];
     }
+    return v;
 }
+
+// a sample takes part when it is in the batch and has a gradient
+TN_DEVICE bool hb_active(uint32_t i, uint32_t n, const HbIn& v) { return i < n && (v.g0 != 0.0f || v.g1 != 0.0f); }
 
 // whether this lane ends its run (emits the run's records)
 TN_DEVICE bool hb_emits(bool active, uint64_t heads) {
@@ -287,23 +319,18 @@ TN_DEVICE bool hb_emits(bool active, uint64_t heads) {
     return active && (lane == 63 || ((heads >> (lane + 1)) & 1ull) != 0);
 }
 
-template <typename TG>
-TN_DEVICE void hb_corners(const tn_hashgrid_t& g, int l, uint32_t i, uint32_t n, uint32_t B, uint32_t GB,
-                          const float* __restrict__ pos, const TG* __restrict__ dfeat, HbCorners& hc) {
+TN_DEVICE void hb_corners(const tn_hashgrid_t& g, int l, bool active, const HbIn& v, HbCorners& hc) {
     const uint32_t lane = __lane_id();
-    float g0, g1;
-    hb_grads<TG>(l, i, n, GB, dfeat, g0, g1);
-    const bool active = i < n && (g0 != 0.0f || g1 != 0.0f);
     hc.emit = false;
     if (__ballot(active) == 0) return;  // a wave of samples without gradient emits nothing
     HashCorners c8;
     uint64_t hm;
-    hb_cells(g, l, i, active, B, pos, c8, hm);
+    hb_cells(g, l, v.x, v.y, v.z, active, c8, hm);
     float a0[8], a1[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        a0[c] = c8.w[c] * g0;
-        a1[c] = c8.w[c] * g1;
+        a0[c] = c8.w[c] * v.g0;
+        a1[c] = c8.w[c] * v.g1;
     }
     if (hm != ~0ull) {
         const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
@@ -338,15 +365,17 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_count_kernel(uint32_t n_max,
     __syncthreads();
     const uint32_t n = info ? min(info[1], n_max) : n_max;
     const uint32_t chunks = (n + 255) / 256, step = p.blocks / L;
-    for (uint32_t c = blockIdx.x / L; c < chunks; c += step) {
+    uint32_t c = blockIdx.x / L;
+    HbIn next = hb_load<TG>(l, c * 256 + threadIdx.x, n, B, GB, pos, dfeat);
+    for (; c < chunks; c += step) {
         const uint32_t i = c * 256 + threadIdx.x;
-        float g0, g1;
-        hb_grads<TG>(l, i, n, GB, dfeat, g0, g1);
-        const bool active = i < n && (g0 != 0.0f || g1 != 0.0f);
+        const HbIn v = next;
+        next = hb_load<TG>(l, (c + step) * 256 + threadIdx.x, n, B, GB, pos, dfeat);
+        const bool active = hb_active(i, n, v);
         if (__ballot(active) == 0) continue;
         HashCorners c8;
         uint64_t hm;
-        hb_cells(g, l, i, active, B, pos, c8, hm);
+        hb_cells(g, l, v.x, v.y, v.z, active, c8, hm);
         if (hb_emits(active, hm)) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) atomicAdd(&hist[(c8.idx[k] - g.level_offset[l]) >> kHbBinLog], 1u);
@@ -487,28 +516,29 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_scatter_kernel(uint32_t n_ma
     __syncthreads();
     const uint32_t n = info ? min(info[1], n_max) : n_max;
     const uint32_t chunks = (n + 255) / 256, step = p.blocks / L;
-    for (uint32_t c = blockIdx.x / L; c < chunks; c += step) {
+    uint32_t c = blockIdx.x / L;
+    HbIn next = hb_load<TG>(l, c * 256 + threadIdx.x, n, B, GB, pos, dfeat);
+    for (; c < chunks; c += step) {
+        const uint32_t i = c * 256 + threadIdx.x;
+        const HbIn v = next;
+        next = hb_load<TG>(l, (c + step) * 256 + threadIdx.x, n, B, GB, pos, dfeat);
         HbCorners hc;
-        hb_corners<TG>(g, l, c * 256 + threadIdx.x, n, B, GB, pos, dfeat, hc);
+        hb_corners(g, l, hb_active(i, n, v), v, hc);
         if (!hc.emit) continue;
-        uint32_t slot[8];
+        uint32_t slot[8], r0[8], r1[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const uint32_t bin = (hc.idx[k] - g.level_offset[l]) >> kHbBinLog;
             slot[k] = atomicAdd(&cur[bin], 1u);
-            const uint32_t m = abs_bits(hc.a0[k], hc.a1[k]);
+            r0[k] = hb_round17(hc.a0[k]);
+            r1[k] = hb_round17(hc.a1[k]);
+            const uint32_t m = max(r0[k] & 0x7FFFFFFFu, r1[k] & 0x7FFFFFFFu);  // of the rounded values
             if (m != 0u) atomicMax(&bmax[bin], m);
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (slot[k] < p.cap) {
-                HbRecord r;
-                r.e = (hc.idx[k] - g.level_offset[l]) & (kHbBinEntries - 1u);
-                r.v0 = hc.a0[k];
-                r.v1 = hc.a1[k];
-                rec[slot[k]] = r;
-            }
-        }
+        for (int k = 0; k < 8; ++k)
+            if (slot[k] < p.cap)
+                rec[slot[k]] = hb_pack((hc.idx[k] - g.level_offset[l]) & (kHbBinEntries - 1u), r0[k], r1[k]);
     }
     __syncthreads();
     const uint32_t j = blockIdx.x / L;
@@ -596,20 +626,23 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgri
     float* accf = reinterpret_cast<float*>(acc);
     for (uint64_t r = r0 + threadIdx.x; r < r1; r += 4 * blockDim.x) {
         HbRecord q[4];
+        bool in[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const uint64_t rr = r + (uint64_t)u * blockDim.x;
-            q[u] = rr < r1 ? rec[rr] : HbRecord{~0u, 0.0f, 0.0f};
+            in[u] = rr < r1;
+            q[u] = in[u] ? rec[rr] : HbRecord{0u, 0u};
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            if (q[u].e >= ne) continue;
+            const uint32_t e = hb_entry(q[u]);
+            if (!in[u] || e >= ne) continue;
             if (finite) {
-                atomicAdd(&acc[2 * q[u].e], (unsigned long long)hb_fixed(q[u].v0, S));
-                atomicAdd(&acc[2 * q[u].e + 1], (unsigned long long)hb_fixed(q[u].v1, S));
+                atomicAdd(&acc[2 * e], (unsigned long long)hb_fixed(hb_v0(q[u]), S));
+                atomicAdd(&acc[2 * e + 1], (unsigned long long)hb_fixed(hb_v1(q[u]), S));
             } else {
-                atomicAdd(&accf[2 * q[u].e], q[u].v0);
-                atomicAdd(&accf[2 * q[u].e + 1], q[u].v1);
+                atomicAdd(&accf[2 * e], hb_v0(q[u]));
+                atomicAdd(&accf[2 * e + 1], hb_v1(q[u]));
             }
         }
     }
