@@ -38,6 +38,12 @@ typedef struct ihipStream_t* tn_stream_t; /* == hipStream_t */
  * Replaces curandGenerateUniform (reference src/core/trainer.cu:45-50). */
 int tn_fill_uniform(uint64_t n, uint64_t seed, uint64_t offset, float* out, tn_stream_t stream);
 
+/* The next n uniforms of the same stream at a position kept on the device:
+ * element j is element (*d_offset + j) of tn_fill_uniform's stream, and
+ * *d_offset advances by n on the stream. A captured training step draws its
+ * batch's uniforms this way, so each replay draws new ones. */
+int tn_fill_uniform_dev(uint64_t n, uint64_t seed, unsigned long long* d_offset, float* out, tn_stream_t stream);
+
 /* Training rays through random pixels, clipped to the training box.
  * Replaces initialize_training_rays_and_pixels_kernel
  * (reference src/utils/training-batch-kernels.cuh:82-187). d_n_rays: optional
@@ -170,12 +176,13 @@ int tn_reduce_loss(const uint32_t* info, uint32_t stride, uint32_t n_channels, c
 
 /* A training step's metrics in one launch: out[0] = mean over the info[0]
  * rays of the four loss channels (stride `stride`), out[1] = mean of
- * dist_loss (0 when dist_loss is null), and samples_total[0] += info[1] when
- * samples_total is given. Replaces the loss reductions at the end of the
- * reference's step (src/core/nerf-network.cu:594-602, :649-666) together with
- * tn_accumulate_samples. */
+ * dist_loss (0 when dist_loss is null), samples_total[0] += info[1] when
+ * samples_total is given, and step_counter[0] += 1 when given (the device
+ * training step tn_adam_step reads for its schedule). Replaces the loss
+ * reductions at the end of the reference's step (src/core/nerf-network.cu:
+ * 594-602, :649-666) together with tn_accumulate_samples. */
 int tn_step_metrics(const uint32_t* info, uint32_t stride, const float* loss, const float* dist_loss,
-    float* out, unsigned long long* samples_total, tn_stream_t stream);
+    float* out, unsigned long long* samples_total, uint32_t* step_counter, tn_stream_t stream);
 
 /* ------------------------------------------------------- occupancy grid */
 

@@ -161,11 +161,11 @@ def test_step_metrics(cuda):
     total = torch.tensor([1000], dtype=torch.int64, device="cuda")
     out = torch.full((2,), 7.0, dtype=torch.float32, device="cuda")
     ld, dd = torch.from_numpy(loss).cuda(), torch.from_numpy(dist).cuda()
-    call("tn_step_metrics", ptr(info), stride, ptr(ld), ptr(dd), ptr(out), ptr(total), stream_ptr())
+    call("tn_step_metrics", ptr(info), stride, ptr(ld), ptr(dd), ptr(out), ptr(total), None, stream_ptr())
     o = out.cpu().numpy()
     ref = loss.astype(np.float64).reshape(4, stride)[:, :n].sum() / n
     assert abs(o[0] - ref) <= 1e-6 * ref and abs(o[1] - dist[:n].astype(np.float64).mean()) <= 1e-6
     assert int(total.cpu()[0]) == 1000 + 123456
-    call("tn_step_metrics", ptr(info), stride, ptr(ld), None, ptr(out), None, stream_ptr())
+    call("tn_step_metrics", ptr(info), stride, ptr(ld), None, ptr(out), None, None, stream_ptr())
     o = out.cpu().numpy()
     assert o[1] == 0.0 and abs(o[0] - ref) <= 1e-6 * ref and int(total.cpu()[0]) == 1000 + 123456

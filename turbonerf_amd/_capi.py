@@ -95,6 +95,7 @@ sz = C.c_size_t
 
 _SIGS: dict[str, tuple] = {
     "tn_fill_uniform": (C.c_int, [u64, u64, u64, vp, vp]),
+    "tn_fill_uniform_dev": (C.c_int, [u64, u64, vp, vp, vp]),
     "tn_init_training_rays": (C.c_int, [u32, vp, u32, u32, u32, i32, i32, f32, vp, vp, vp, vp, vp, vp, vp,
                                         vp, vp, vp, vp, vp]),
     "tn_deactivate_rays": (C.c_int, [u32, vp, u32, f32, f32, vp, vp, vp, vp]),
@@ -114,7 +115,7 @@ _SIGS: dict[str, tuple] = {
     "tn_composite_train_noalloc_max_bytes": (sz, []),
     "tn_composite_forward": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, C.c_int, vp, vp, vp, vp, vp, vp, vp]),
     "tn_reduce_loss": (C.c_int, [vp, u32, u32, vp, vp, vp]),
-    "tn_step_metrics": (C.c_int, [vp, u32, vp, vp, vp, vp, vp]),
+    "tn_step_metrics": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp]),
     "tn_occ_decay": (C.c_int, [OccupancyGrid, f32, vp]),
     "tn_occ_sample_points": (C.c_int, [u32, u32, u32, OccupancyGrid, C.c_int, f32, vp, vp, vp]),
     "tn_occ_update_density": (C.c_int, [C.c_int, u32, u32, OccupancyGrid, C.c_int, C.c_int, vp, vp, vp]),

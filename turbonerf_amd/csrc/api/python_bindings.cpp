@@ -312,7 +312,7 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
         .def("update_occupancy_grid", &NeRFTrainingController::update_occupancy_grid, py::arg("training_step"))
         .def("train_step", &NeRFTrainingController::train_step)
         .def("train_steps", &NeRFTrainingController::train_steps, py::arg("n_steps"), py::arg("read_metrics") = true,
-             py::call_guard<py::gil_scoped_release>())
+             py::arg("graph") = false, py::call_guard<py::gil_scoped_release>())
         .def("set_occupancy_from_density",
              [](NeRFTrainingController& tc, py::array_t<float, py::array::c_style | py::array::forcecast> d,
                 float thr) { tc.set_occupancy_from_density(d.data(), thr); },
@@ -397,6 +397,7 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
                  d["loss"] = to_numpy(n.workspace.loss_out);
                  return d;
              })
+        .def("_graph_captures", [](NeRFTrainingController& tc) { return tc.context() ? tc.context()->n_captures : 0u; })
         .def("_get_optimizer_state",
              [](NeRFTrainingController& tc) {
                  tc.synchronize();

@@ -132,7 +132,13 @@ void NeRFTrainingController::setup_data(uint32_t batch_size) {
     TN_HIP(hipMemcpyAsync(ctx_->n_rays_next.data(), &init, sizeof(init), hipMemcpyHostToDevice, s));
     ctx_->samples_total.resize(1);
     ctx_->samples_total.zero(s);
-    nerf.network.workspace.enlarge(batch_size, R, config.network.precision);
+    ctx_->d_rng.resize(1);
+    ctx_->d_rng.zero(s);
+    ctx_->d_step.resize(1);
+    ctx_->d_step.zero(s);
+    ctx_->occ_rng_offset = 0;
+    ctx_->destroy_graphs();
+    nerf.network.prepare_workspace(batch_size, R);
     nerf.network.update_params_if_needed(s, nerf.params);
     proxy->training_step = 0;
     TN_HIP(hipStreamSynchronize(s));
@@ -198,11 +204,16 @@ NeRFTrainingController::TrainingMetrics NeRFTrainingController::train_step() {
     return m;
 }
 
-NeRFTrainingController::TrainingMetrics NeRFTrainingController::train_steps(uint32_t n, bool read) {
+NeRFTrainingController::TrainingMetrics NeRFTrainingController::train_steps(uint32_t n, bool read, bool graph) {
     sync_settings();
-    for (uint32_t k = 0; k < n; ++k) {
-        trainer_.enqueue_train_step(*ctx_, proxy->training_step);
-        proxy->training_step++;
+    if (graph) {
+        trainer_.enqueue_train_steps_graph(*ctx_, proxy->training_step, n);
+        proxy->training_step += n;
+    } else {
+        for (uint32_t k = 0; k < n; ++k) {
+            trainer_.enqueue_train_step(*ctx_, proxy->training_step);
+            proxy->training_step++;
+        }
     }
     TrainingMetrics m;
     m.step = proxy->training_step;

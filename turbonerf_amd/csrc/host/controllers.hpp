@@ -117,7 +117,10 @@ struct NeRFTrainingController {
     void load_images(std::function<void(int, int)> on_image_loaded = {});
     TrainingMetrics train_step();
     // Enqueue n steps without host synchronisation; metrics of the last one.
-    TrainingMetrics train_steps(uint32_t n_steps, bool read_metrics = true);
+    // graph: each step is the replay of a captured HIP graph (captured on
+    // first use and again whenever a setting or buffer it was built from
+    // changes); results are the same as without.
+    TrainingMetrics train_steps(uint32_t n_steps, bool read_metrics = true, bool graph = false);
     OccupancyGridMetrics update_occupancy_grid(uint32_t training_step);
     std::vector<size_t> get_cuda_memory_allocated() const;
 

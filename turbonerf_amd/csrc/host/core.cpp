@@ -2,6 +2,7 @@
 #include "core.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <random>
 
 namespace turbo {
@@ -191,11 +192,15 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
         TN_TRY(tn_hashgrid_backward_binned(0, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(),
                                            w.hash_ws.data(), w.hash_ws.size(), ts));
     }
-    // optimiser (reference :770-781, lr schedule :256)
-    const float lr = config.learning_rate / (1.0f + config.learning_rate_decay * (float)step);
+    // optimiser (reference :770-781, lr schedule :256; from the device step
+    // counter in a captured step)
+    const bool dev_lr = st.step_for_lr && st.step_counter;
+    const float lr = dev_lr ? config.learning_rate
+                            : config.learning_rate / (1.0f + config.learning_rate_decay * (float)step);
     {
         ScopedPhase ph(tm, "adam", s);
-        TN_TRY(tn_adam_step(p.n_total, p.n_grid, p.n_total, config.loss_scale(), info, lr, nullptr, 0.0f, config.beta1,
+        TN_TRY(tn_adam_step(p.n_total, p.n_grid, p.n_total, config.loss_scale(), info, lr,
+                            dev_lr ? st.step_counter : nullptr, config.learning_rate_decay, config.beta1,
                             config.beta2, config.epsilon, config.l2_reg, p.params_fp.data(), prec,
                             prec == 1 ? p.params_lp.data() : nullptr, p.gradients.data(), p.m1.data(), p.m2.data(),
                             p.steps.data(), ts));
@@ -205,8 +210,13 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
     {
         ScopedPhase ph(tm, "loss_reduce", s);
         TN_TRY(tn_step_metrics(info, R, w.ray_loss.data(), st.use_distortion_loss ? w.ray_dist_loss.data() : nullptr,
-                               w.loss_out.data(), st.samples_total, ts));
+                               w.loss_out.data(), st.samples_total, st.step_counter, ts));
     }
+}
+
+void NerfNetwork::prepare_workspace(uint32_t B, uint32_t R) {
+    workspace.enlarge(B, R, config.precision);
+    workspace.hash_ws.reserve(tn_hashgrid_backward_workspace_bytes(B, grid));
 }
 
 void NerfNetwork::inference(hipStream_t s, const NetworkParamsWorkspace& p, uint32_t n, const uint32_t* info,
@@ -280,6 +290,16 @@ Trainer::Context::Context(hipStream_t s, const Dataset* ds, NeRF* n, uint32_t B)
     }
     TN_HIP(hipEventCreateWithFlags(&ev_grid, hipEventDisableTiming));
     TN_HIP(hipEventCreateWithFlags(&ev_gen_idle, hipEventDisableTiming));
+    TN_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    TN_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+}
+
+void Trainer::Context::destroy_graphs() {
+    for (auto& g : graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    graph_key.clear();
 }
 
 Trainer::Context::~Context() {
@@ -289,17 +309,19 @@ Trainer::Context::~Context() {
         (void)hipEventDestroy(ev_ready[k]);
         (void)hipEventDestroy(ev_consumed[k]);
     }
+    destroy_graphs();
     (void)hipEventDestroy(ev_grid);
     (void)hipEventDestroy(ev_gen_idle);
+    (void)hipEventDestroy(ev_fork);
+    (void)hipEventDestroy(ev_join);
     (void)hipStreamDestroy(gen_stream);
 }
 
-void Trainer::generate_next_training_batch(Context& ctx, TrainingWorkspace& ws, hipStream_t s) {
+void Trainer::generate_next_training_batch(Context& ctx, TrainingWorkspace& ws, hipStream_t s, PhaseTimer* tm) {
     NeRF& nerf = *ctx.nerf;
     const tn_stream_t ts = as_tn(s);
     const uint32_t R = ws.ray_capacity, B = ws.batch_size;
     const Dataset& ds = *ctx.dataset;
-    PhaseTimer* tm = &ctx.timer;
     const DatasetWorkspace& dw = nerf.dataset_ws;
     {
         // seeded uniforms: slice 0 picks pixels, slices 1-3 are the background
@@ -307,8 +329,7 @@ void Trainer::generate_next_training_batch(Context& ctx, TrainingWorkspace& ws, 
         // device-side estimate of how many fill the batch and split evenly
         // over the images by that count (-1 below)
         ScopedPhase ph(tm, "rays", s);
-        TN_TRY(tn_fill_uniform(4 * (uint64_t)R, ctx.rng_seed, ctx.rng_offset, ws.random_float.data(), ts));
-        ctx.rng_offset += 4 * (uint64_t)R;
+        TN_TRY(tn_fill_uniform_dev(4 * (uint64_t)R, ctx.rng_seed, ctx.d_rng.data(), ws.random_float.data(), ts));
         TN_TRY(tn_init_training_rays(R, ctx.n_rays_next.data(), R, dw.n_images, dw.n_pixels_per_image,
                                      dw.image_dims.x, dw.image_dims.y, -1.0f,
                                      reinterpret_cast<const tn_bbox_t*>(dw.bbox.data()),
@@ -348,37 +369,133 @@ void Trainer::generate_next_training_batch(Context& ctx, TrainingWorkspace& ws, 
     }
 }
 
-void Trainer::enqueue_train_step(Context& ctx, uint32_t training_step) {
-    const int c = ctx.cur, nx = c ^ 1;
+// Train on batches[c] (network stream) while batches[c ^ 1] is generated
+// (generation stream); with `graph` the work is being captured.
+void Trainer::train_batch(Context& ctx, int c, uint32_t training_step, bool graph) {
+    const int nx = c ^ 1;
     const hipStream_t gen = ctx.gen_stream, net = ctx.stream;
-    if (!ctx.prefetched) {
-        // first step, or the grid changed since the batch generated ahead
-        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_grid, 0));
-        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[c], 0));
-        generate_next_training_batch(ctx, ctx.batches[c], gen);
-        TN_HIP(hipEventRecord(ctx.ev_ready[c], gen));
+    PhaseTimer* tm = graph ? nullptr : &ctx.timer;
+    if (graph) {
+        // fork the generation stream off the captured network stream
+        TN_HIP(hipEventRecord(ctx.ev_fork, net));
+        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_fork, 0));
+    } else {
+        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[nx], 0));
     }
-    // generate the next step's batch while this one trains
-    TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[nx], 0));
-    generate_next_training_batch(ctx, ctx.batches[nx], gen);
-    TN_HIP(hipEventRecord(ctx.ev_ready[nx], gen));
-
+    generate_next_training_batch(ctx, ctx.batches[nx], gen, tm);
+    if (graph) {
+        TN_HIP(hipEventRecord(ctx.ev_join, gen));
+    } else {
+        TN_HIP(hipEventRecord(ctx.ev_ready[nx], gen));
+        TN_HIP(hipStreamWaitEvent(net, ctx.ev_ready[c], 0));
+    }
     TrainingWorkspace& ws = ctx.batches[c];
-    TN_HIP(hipStreamWaitEvent(net, ctx.ev_ready[c], 0));
     NerfNetwork::Settings st;
     st.use_distortion_loss = ctx.use_distortion_loss;
     st.distortion_lambda = ctx.distortion_lambda;
-    st.timer = &ctx.timer;
+    st.timer = tm;
     st.samples_total = ctx.samples_total.size() ? ctx.samples_total.data() : nullptr;
-    ctx.nerf->network.train(net, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity,
-                            ws.info.data(), ws.random_float.data() + ws.ray_capacity, ws.ray_step.data(),
-                            ws.ray_offset.data(), ws.sample_t.data(), ws.sample_pos.data(), ws.sample_dir.data(),
-                            ws.sample_dt.data(), ws.sample_m_norm.data(), ws.sample_dt_norm.data(), ws.pix_rgba.data(),
-                            st);
-    TN_HIP(hipEventRecord(ctx.ev_consumed[c], net));
+    st.step_counter = ctx.d_step.data();
+    st.step_for_lr = graph;
+    ctx.nerf->network.train(net, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity, ws.info.data(),
+                            ws.random_float.data() + ws.ray_capacity, ws.ray_step.data(), ws.ray_offset.data(),
+                            ws.sample_t.data(), ws.sample_pos.data(), ws.sample_dir.data(), ws.sample_dt.data(),
+                            ws.sample_m_norm.data(), ws.sample_dt_norm.data(), ws.pix_rgba.data(), st);
+    if (graph) TN_HIP(hipStreamWaitEvent(net, ctx.ev_join, 0));  // join before the capture ends
+}
+
+void Trainer::enqueue_train_step(Context& ctx, uint32_t training_step) {
+    const int c = ctx.cur;
+    if (!ctx.prefetched) {
+        // first step, or the grid changed since the batch generated ahead
+        TN_HIP(hipStreamWaitEvent(ctx.gen_stream, ctx.ev_grid, 0));
+        TN_HIP(hipStreamWaitEvent(ctx.gen_stream, ctx.ev_consumed[c], 0));
+        generate_next_training_batch(ctx, ctx.batches[c], ctx.gen_stream, &ctx.timer);
+        TN_HIP(hipEventRecord(ctx.ev_ready[c], ctx.gen_stream));
+    }
+    train_batch(ctx, c, training_step, false);
+    TN_HIP(hipEventRecord(ctx.ev_consumed[c], ctx.stream));
     ctx.last = c;
-    ctx.cur = nx;
+    ctx.cur = c ^ 1;
     ctx.prefetched = true;
+}
+
+// What a captured step has baked into its kernel arguments: buffer addresses,
+// sizes and settings. A change of any of them recaptures.
+std::vector<uint64_t> Trainer::graph_key(const Context& ctx) const {
+    const NeRF& n = *ctx.nerf;
+    const NetworkConfig& c = n.network.config;
+    auto bits = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return (uint64_t)u; };
+    auto ptr = [](const void* p) { return (uint64_t)reinterpret_cast<uintptr_t>(p); };
+    std::vector<uint64_t> k = {
+        ptr(ctx.nerf), ptr(ctx.dataset), ctx.batch_size, ctx.batches[0].ray_capacity, ctx.batches[0].batch_size,
+        bits(ctx.alpha_selection_threshold), bits(ctx.alpha_selection_probability), bits(ctx.min_step_size),
+        (uint64_t)ctx.use_distortion_loss, bits(ctx.distortion_lambda), ctx.rng_seed,
+        ptr(n.params.params_fp.data()), ptr(n.params.params_lp.data()), ptr(n.params.gradients.data()),
+        ptr(n.params.m1.data()), ptr(n.params.m2.data()), ptr(n.params.steps.data()), n.params.n_total, n.params.n_grid,
+        ptr(n.occupancy_grid.bitfield.data()), (uint64_t)n.occupancy_grid.n_levels, ptr(n.dataset_ws.images.data()),
+        ptr(n.dataset_ws.cameras.data()), n.dataset_ws.n_images, (uint64_t)c.precision, bits(c.learning_rate),
+        bits(c.learning_rate_decay), bits(c.beta1), bits(c.beta2), bits(c.epsilon), bits(c.l2_reg),
+        (uint64_t)n.network.aabb_scale, n.network.grid.n_params, bits(ctx.dataset->bounding_box.size()),
+        ptr(n.network.workspace.features.data()), ptr(n.network.workspace.hash_ws.data()),
+        ptr(n.network.workspace.mlp_ws.data()), ptr(ctx.samples_total.data())};
+    for (const auto& b : ctx.batches) k.push_back(ptr(b.random_float.data()));
+    return k;
+}
+
+void Trainer::capture_step(Context& ctx, int c) {
+    hipGraph_t g = nullptr;
+    TN_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        train_batch(ctx, c, 0, true);
+    } catch (...) {
+        (void)hipStreamEndCapture(ctx.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    TN_HIP(hipStreamEndCapture(ctx.stream, &g));
+    const hipError_t e = hipGraphInstantiate(&ctx.graph[c], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    TN_HIP(e);
+    ++ctx.n_captures;
+}
+
+void Trainer::enqueue_train_steps_graph(Context& ctx, uint32_t first_step, uint32_t n) {
+    if (n == 0) return;
+    const hipStream_t net = ctx.stream, gen = ctx.gen_stream;
+    // a captured step may not allocate: size every buffer first
+    ctx.nerf->network.prepare_workspace(ctx.batches[0].batch_size, ctx.batches[0].ray_capacity);
+    const std::vector<uint64_t> key = graph_key(ctx);
+    if (key != ctx.graph_key) {
+        ctx.destroy_graphs();
+        ctx.graph_key = key;
+    }
+    // the device step counter takes the host's training step
+    TN_HIP(hipStreamSynchronize(net));  // the pinned staging value is free
+    ctx.step_host[0] = first_step;
+    TN_HIP(hipMemcpyAsync(ctx.d_step.data(), ctx.step_host.data(), sizeof(uint32_t), hipMemcpyHostToDevice, net));
+    for (uint32_t k = 0; k < n; ++k) {
+        const int c = ctx.cur, nx = c ^ 1;
+        if (!ctx.prefetched) {
+            TN_HIP(hipStreamWaitEvent(gen, ctx.ev_grid, 0));
+            TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[c], 0));
+            generate_next_training_batch(ctx, ctx.batches[c], gen, &ctx.timer);
+            TN_HIP(hipEventRecord(ctx.ev_ready[c], gen));
+        }
+        // the batch and the generation stream's last work come before the graph
+        TN_HIP(hipStreamWaitEvent(net, ctx.ev_ready[c], 0));
+        TN_HIP(hipEventRecord(ctx.ev_gen_idle, gen));
+        TN_HIP(hipStreamWaitEvent(net, ctx.ev_gen_idle, 0));
+        if (!ctx.graph[c]) capture_step(ctx, c);
+        TN_HIP(hipGraphLaunch(ctx.graph[c], net));
+        // the graph generated batches[nx] and consumed batches[c]
+        TN_HIP(hipEventRecord(ctx.ev_ready[nx], net));
+        TN_HIP(hipEventRecord(ctx.ev_consumed[c], net));
+        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[c], 0));  // later eager generation follows the graph
+        ctx.last = c;
+        ctx.cur = nx;
+        ctx.prefetched = true;
+    }
 }
 
 void Trainer::read_metrics(Context& ctx, float& loss) {
@@ -425,8 +542,10 @@ uint32_t Trainer::update_occupancy_grid(Context& ctx, uint32_t training_step) {
     for (int level = 0; level < og.n_levels; ++level) {
         for (uint32_t start = 0; start < vol; start += chunk) {
             const uint32_t n = std::min(chunk, vol - start);
-            TN_TRY(tn_fill_uniform(4 * (uint64_t)chunk, ctx.rng_seed, ctx.rng_offset, ws.random_float.data(), ts));
-            ctx.rng_offset += 4 * (uint64_t)chunk;
+            // the occupancy update draws from its own stream (key seed ^ k)
+            TN_TRY(tn_fill_uniform(4 * (uint64_t)chunk, ctx.rng_seed ^ 0x0cc0cc0cc0cc0cc0ull, ctx.occ_rng_offset,
+                                   ws.random_float.data(), ts));
+            ctx.occ_rng_offset += 4 * (uint64_t)chunk;
             TN_TRY(tn_occ_sample_points(n, chunk, start, og.view(), level, 1.0f / aabb, ws.random_float.data(),
                                         ws.sample_pos.data(), ts));
             nerf.network.inference(s, nerf.params, n, nullptr, chunk, ws.sample_pos.data(), nullptr,

@@ -150,6 +150,11 @@ struct NerfNetwork {
         float min_transmittance = NeRFConstants::min_transmittance;
         PhaseTimer* timer = nullptr;
         unsigned long long* samples_total = nullptr;  // device counter of trained samples, if kept
+        // device training step: incremented at the end of the step when
+        // given; with step_for_lr also the step the learning-rate schedule
+        // reads (a captured step replays with it advancing on the device)
+        uint32_t* step_counter = nullptr;
+        bool step_for_lr = false;
     };
 
     NetworkConfig config;
@@ -162,6 +167,9 @@ struct NerfNetwork {
     void update_aabb_scale_if_needed(int aabb);
     // allocate and seed parameters, Adam state (reference :127-190)
     void update_params_if_needed(hipStream_t s, NetworkParamsWorkspace& p);
+    // every buffer a training step of this batch shape uses, allocated up
+    // front (a captured step may not allocate)
+    void prepare_workspace(uint32_t batch_size, uint32_t ray_capacity);
     const void* params_for_kernels(const NetworkParamsWorkspace& p) const {
         return config.precision == 1 ? (const void*)p.params_lp.data() : (const void*)p.params_fp.data();
     }
@@ -246,8 +254,20 @@ struct Trainer {
         int cur = 0;              // batch the next step trains on
         int last = 0;             // batch the last step trained on
         bool prefetched = false;  // batches[cur] already holds a generated batch
-        hipEvent_t ev_ready[2]{}, ev_consumed[2]{}, ev_grid{}, ev_gen_idle{};
+        hipEvent_t ev_ready[2]{}, ev_consumed[2]{}, ev_grid{}, ev_gen_idle{}, ev_fork{}, ev_join{};
         DeviceBuffer<uint32_t> n_rays_next;  // device-side ray count for the next batch
+        // position of the batch-generation uniform stream and the training
+        // step, both on the device, so a captured step needs no host values
+        DeviceBuffer<unsigned long long> d_rng;
+        DeviceBuffer<uint32_t> d_step;
+        PinnedBuffer<uint32_t> step_host{1};
+        uint64_t occ_rng_offset = 0;  // occupancy-update uniforms (their own key)
+        // captured steps: graph[c] trains on batches[c] while it generates
+        // batches[c ^ 1]; recaptured when anything they were built from changes
+        hipGraphExec_t graph[2] = {nullptr, nullptr};
+        std::vector<uint64_t> graph_key;
+        uint32_t n_captures = 0;  // graphs captured so far
+        void destroy_graphs();
         const Dataset* dataset;
         NeRF* nerf;
         uint32_t batch_size;
@@ -261,7 +281,6 @@ struct Trainer {
         float distortion_lambda = NeRFConstants::mipNeRF360_distortion_loss_lambda;
         float occupancy_threshold = NeRFConstants::occupancy_threshold;
         uint64_t rng_seed = 0x5eed;
-        uint64_t rng_offset = 0;
         uint32_t initial_rays = 16384;
         PinnedBuffer<uint32_t> info_host{4};
         PinnedBuffer<float> loss_host{2};
@@ -283,15 +302,24 @@ struct Trainer {
 
     // Enqueue one training step; no host synchronisation.
     void enqueue_train_step(Context& ctx, uint32_t training_step);
+    // Enqueue n training steps as replays of captured HIP graphs (one graph
+    // launch per step, the next batch generated inside it); training steps
+    // first_step .. first_step + n - 1. Same results as enqueue_train_step.
+    void enqueue_train_steps_graph(Context& ctx, uint32_t first_step, uint32_t n);
     // One training step, synchronising like the reference: throws when the
     // batch has no rays/samples, returns the loss (reference trainer.cu:313-349).
     float train_step(Context& ctx, uint32_t training_step);
     // Batch generation into `ws` on `s` (reference trainer.cu:39-217).
-    void generate_next_training_batch(Context& ctx, TrainingWorkspace& ws, hipStream_t s);
+    void generate_next_training_batch(Context& ctx, TrainingWorkspace& ws, hipStream_t s, PhaseTimer* tm);
     // Occupancy update; returns the number of occupied bits (reference trainer.cu:221-311).
     uint32_t update_occupancy_grid(Context& ctx, uint32_t training_step);
     // Read back batch counts and losses of the last enqueued step.
     void read_metrics(Context& ctx, float& loss);
+
+private:
+    void train_batch(Context& ctx, int c, uint32_t training_step, bool graph);
+    std::vector<uint64_t> graph_key(const Context& ctx) const;
+    void capture_step(Context& ctx, int c);
 };
 
 }  // namespace turbo

@@ -505,11 +505,12 @@ inline size_t comp_carry_bytes(uint32_t sample_stride) {
 static_assert(sizeof(Acc<float>) == 8 && sizeof(Acc<double>) == 8, "8-byte accumulators");
 
 // A training step's metrics in one launch: out[0] = mean over rays of the
-// four loss channels, out[1] = mean distortion loss (0 without one), and
-// total[0] += info[1] when a counter is given.
+// four loss channels, out[1] = mean distortion loss (0 without one),
+// total[0] += info[1] when a counter is given, and ++step[0] when a step
+// counter is given (the last kernel of a step).
 __global__ void step_metrics_kernel(const uint32_t* __restrict__ info, uint32_t B, const float* __restrict__ loss,
                                     const float* __restrict__ dist_loss, float* __restrict__ out,
-                                    unsigned long long* __restrict__ total) {
+                                    unsigned long long* __restrict__ total, uint32_t* __restrict__ step) {
     __shared__ double part[2][16];
     const uint32_t n = info[0];
     double acc = 0.0, accd = 0.0;
@@ -538,6 +539,7 @@ __global__ void step_metrics_kernel(const uint32_t* __restrict__ info, uint32_t 
         out[0] = n > 0 ? (float)(t / (double)n) : 0.0f;
         out[1] = n > 0 && dist_loss ? (float)(td / (double)n) : 0.0f;
         if (total) *total += info[1];
+        if (step) *step += 1u;
     }
 }
 
@@ -617,9 +619,10 @@ extern "C" int tn_reduce_loss(const uint32_t* info, uint32_t stride, uint32_t n_
 }
 
 extern "C" int tn_step_metrics(const uint32_t* info, uint32_t stride, const float* loss, const float* dist_loss,
-                               float* out, unsigned long long* samples_total, hipStream_t stream) {
+                               float* out, unsigned long long* samples_total, uint32_t* step_counter,
+                               hipStream_t stream) {
     TN_CHECK_ARG(info && loss && out);
-    step_metrics_kernel<<<1, 1024, 0, stream>>>(info, stride, loss, dist_loss, out, samples_total);
+    step_metrics_kernel<<<1, 1024, 0, stream>>>(info, stride, loss, dist_loss, out, samples_total, step_counter);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }

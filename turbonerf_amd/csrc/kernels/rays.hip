@@ -25,6 +25,27 @@ __global__ void uniform_kernel(uint64_t n, uint32_t k0, uint32_t k1, uint64_t of
     }
 }
 
+// The same stream at an offset read on the device (a captured step replays
+// with the offset advancing on the device); the grid covers any alignment.
+__global__ void uniform_dev_kernel(uint64_t n, uint32_t k0, uint32_t k1, const unsigned long long* __restrict__ d_off,
+                                   float* __restrict__ out) {
+    const uint64_t offset = *d_off;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t block = (offset >> 2) + tid;
+    if (block > (offset + n - 1) >> 2) return;
+    uint32_t c[4] = {(uint32_t)block, (uint32_t)(block >> 32), 0u, 0u};
+    philox4x32_10(c, k0, k1);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const uint64_t g = block * 4 + w;
+        if (g < offset) continue;
+        const uint64_t j = g - offset;
+        if (j < n) out[j] = u32_to_unit(c[w]);
+    }
+}
+
+__global__ void counter_add_kernel(unsigned long long* __restrict__ c, unsigned long long inc) { *c += inc; }
+
 __global__ void init_training_rays_kernel(uint32_t n_rays_max, const uint32_t* __restrict__ d_n_rays,
                                           uint32_t B, uint32_t n_images, uint32_t n_pix, int32_t img_w,
                                           float n_rays_per_image, const tn_bbox_t* __restrict__ bbox,
@@ -111,6 +132,17 @@ extern "C" int tn_fill_uniform(uint64_t n, uint64_t seed, uint64_t offset, float
     const uint64_t n_threads = last_block - first_block + 1;
     uniform_kernel<<<blocks_for(n_threads, 256), 256, 0, stream>>>(n, (uint32_t)seed, (uint32_t)(seed >> 32),
                                                                    offset, out);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+extern "C" int tn_fill_uniform_dev(uint64_t n, uint64_t seed, unsigned long long* d_offset, float* out,
+                                   hipStream_t stream) {
+    TN_CHECK_ARG(out != nullptr && d_offset != nullptr);
+    if (n == 0) return TN_OK;
+    uniform_dev_kernel<<<blocks_for(n / 4 + 2, 256), 256, 0, stream>>>(n, (uint32_t)seed, (uint32_t)(seed >> 32),
+                                                                      d_offset, out);
+    counter_add_kernel<<<1, 1, 0, stream>>>(d_offset, (unsigned long long)n);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }
