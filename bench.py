@@ -167,7 +167,7 @@ def run_precision(tn, synthetic, scene, precision, args):
     step = 0
     while step < args.warmup:
         k = min(16, args.warmup - step)
-        tr.train_steps(k, read_metrics=False)
+        tr.train_steps(k, read_metrics=False, graph=True)
         step += k
         if step % 16 == 0:
             om = tr.update_occupancy_grid(step)
@@ -177,22 +177,32 @@ def run_precision(tn, synthetic, scene, precision, args):
     step += 1
     log(f"[bench {name}] warm-up done: loss {m.loss:.4f} rays {m.n_rays} samples {m.n_samples}")
 
-    # timed region: K steps, no host synchronisation inside
+    # timed region: K steps, each the replay of the captured training-step
+    # graph (captured during the warm-up), no host synchronisation inside
     tr.set_timing(False)
-    s0 = tr.samples_trained()
-    tr.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    tr.train_steps(args.steps, read_metrics=False)
-    tr.synchronize()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    s1 = tr.samples_trained()
+    tr.train_steps(1, read_metrics=False, graph=True)  # graphs captured before the clock starts
+    step += 1
+
+    def timed(graph):
+        s0 = tr.samples_trained()
+        tr.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr.train_steps(args.steps, read_metrics=False, graph=graph)
+        tr.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return tr.samples_trained() - s0, wall
+
+    samples, wall = timed(True)
     step += args.steps
-    wall = t1 - t0
-    samples = s1 - s0
     res = dict(samples_per_s=samples / wall, ms_per_step=1e3 * wall / args.steps,
                samples_per_step=samples / args.steps)
+    # the same K steps enqueued kernel by kernel, for comparison
+    samples_e, wall_e = timed(False)
+    step += args.steps
+    res["eager_samples_per_s"] = samples_e / wall_e
+    res["eager_ms_per_step"] = 1e3 * wall_e / args.steps
 
     # the same K steps again with HIP events around every phase on the
     # trainer's stream (per-kernel durations for the roofline)
@@ -223,7 +233,8 @@ def run_precision(tn, synthetic, scene, precision, args):
     res["occupied_fraction"] = om.n_occupied / om.n_total
     res["bitfield"] = tr.get_occupancy_bitfield()
     res["trainer"] = (mgr, nerf, tr)
-    log(f"[bench {name}] {res['samples_per_s'] / 1e6:.2f} M samples/s, {res['ms_per_step']:.3f} ms/step, "
+    log(f"[bench {name}] {res['samples_per_s'] / 1e6:.2f} M samples/s, {res['ms_per_step']:.3f} ms/step "
+        f"(captured step; eager {res['eager_samples_per_s'] / 1e6:.2f} M samples/s), "
         f"{res['samples_per_step']:.0f} samples/step, {res['rays_per_step']:.0f} rays/step")
     return res
 
@@ -295,76 +306,98 @@ def render_rate(tn, synthetic, scene, trainer_tuple, n_frames=5):
     return RES * RES / dt / 1e6, 1e3 * dt
 
 
-def timed_steps(tr, steps):
+def timed_steps(tr, steps, graph=True):
     """(samples/s, ms/step) of `steps` training steps without host
-    synchronisation inside."""
+    synchronisation inside, each the replay of a captured step by default."""
+    tr.train_steps(1, read_metrics=False, graph=graph)  # captured before the clock starts
     tr.synchronize()
     s0 = tr.samples_trained()
     t0 = time.perf_counter()
-    tr.train_steps(steps, read_metrics=False)
+    tr.train_steps(steps, read_metrics=False, graph=graph)
     tr.synchronize()
     wall = time.perf_counter() - t0
     return (tr.samples_trained() - s0) / wall, 1e3 * wall / steps
 
 
-def distortion_rate(tr, steps):
+def distortion_rate(tr, steps, lam=1e-3):
     """The trained mixed-precision NeRF with the mip-NeRF 360 distortion loss
-    switched on (the reference's lambda 0.01; its forward and backward run
-    every step)."""
+    switched on at weight `lam` (its forward and backward run every step)."""
+    lam0 = tr.distortion_lambda
     tr.use_distortion_loss = True
-    tr.train_steps(16, read_metrics=False)
+    tr.distortion_lambda = lam
+    tr.train_steps(16, read_metrics=False, graph=True)
     sps, ms = timed_steps(tr, steps)
     m = tr.train_step()
     tr.use_distortion_loss = False
-    log(f"[bench distortion] {sps / 1e6:.2f} M samples/s, {ms:.3f} ms/step (loss with distortion {m.loss:.5f})")
-    return dict(samples_per_s=round(sps, 1), ms_per_step=round(ms, 4))
+    tr.distortion_lambda = lam0
+    log(f"[bench distortion] lambda {lam}: {sps / 1e6:.2f} M samples/s, {ms:.3f} ms/step (loss with distortion "
+        f"{m.loss:.5f})")
+    return dict(samples_per_s=round(sps, 1), ms_per_step=round(ms, 4), distortion_lambda=lam)
+
+
+# eight scenes: hash tables of 2^15 ... 2^21, 2^21 entries per level, blob
+# fields of 4 to 32 blobs, uneven sample budgets summing to 2^19
+MULTI_LOG2_T = [15, 16, 17, 18, 19, 20, 21, 21]
+MULTI_BLOBS = [4, 8, 12, 16, 20, 24, 28, 32]
+MULTI_BATCH = [8192, 16384, 32768, 65536, 131072, 131072, 65536, 73728]
 
 
 def multi_scene(tn, synthetic, steps):
-    """Eight NeRFs in one manager, hash tables of 2^15 ... 2^21 entries per
-    level, blob fields of 4 to 32 blobs, batches summing to 2^19 samples,
-    trained in interleaved steps: aggregate samples/s after 128 warm-up
-    steps each."""
-    log2_t = [15, 16, 17, 18, 19, 20, 21, 21]
-    n_blobs = [4, 8, 12, 16, 20, 24, 28, 32]
-    batch = [16384, 32768, 65536, 131072, 131072, 65536, 49152, 32768]
+    """Eight NeRFs trained side by side: every round is one training step of
+    each, launched as one captured HIP graph (MultiTrainer). Aggregate
+    samples/s after 128 warm-up steps each; the same rounds enqueued as eight
+    eager steps one after another are timed for comparison."""
+    assert sum(MULTI_BATCH) == 1 << 19
     mgr = tn.NeRFManager()
     trainers = []
     for k in range(8):
         blobs, cams, images = synthetic.blob_dataset(seed=100 + k, n_cams=16, width=128, height=128,
-                                                     n_blobs=n_blobs[k])
+                                                     n_blobs=MULTI_BLOBS[k])
         nerf = mgr.create()
         nerf.attach_dataset(tn.Dataset(tn.BoundingBox(1.0), synthetic.to_pyturbonerf_cameras(tn, cams), images))
         cfg = tn.TrainingConfig()
         cfg.network.precision = 1
-        cfg.network.log2_hashmap_size = log2_t[k]
+        cfg.network.log2_hashmap_size = MULTI_LOG2_T[k]
         cfg.network.beta2 = 0.99
         cfg.rng_seed = 7 + k
         tr = tn.Trainer(nerf, cfg)
-        tr.setup_data(batch_size=batch[k])
+        tr.setup_data(batch_size=MULTI_BATCH[k])
         tr.load_images()
         trainers.append(tr)
+    mt = tn.MultiTrainer(trainers)
     for i in range(8):
+        mt.train_steps(16)
         for tr in trainers:
-            tr.train_steps(16, read_metrics=False)
             tr.update_occupancy_grid(16 * (i + 1))
-    for tr in trainers:
-        tr.synchronize()
-    s0 = sum(tr.samples_trained() for tr in trainers)
-    t0 = time.perf_counter()
-    for i in range(steps):
-        for tr in trainers:
-            tr.train_steps(1, read_metrics=False)
-    for tr in trainers:
-        tr.synchronize()
-    wall = time.perf_counter() - t0
-    samples = sum(tr.samples_trained() for tr in trainers) - s0
+    mt.train_steps(2)  # both graphs captured before the clock starts
+    rays = [tr.train_steps(0, read_metrics=True).n_rays for tr in trainers]
+
+    def timed(graph):
+        mt.synchronize()
+        s0 = sum(tr.samples_trained() for tr in trainers)
+        t0 = time.perf_counter()
+        if graph:
+            mt.train_steps(steps)
+        else:
+            for _ in range(steps):
+                for tr in trainers:
+                    tr.train_steps(1, read_metrics=False)
+        mt.synchronize()
+        wall = time.perf_counter() - t0
+        return sum(tr.samples_trained() for tr in trainers) - s0, wall
+
+    samples, wall = timed(True)
+    samples_e, wall_e = timed(False)
+    captures = mt._graph_captures
+    del mt
     for tr in trainers:
         tr.teardown()
     log(f"[bench multi-scene] {samples / wall / 1e6:.2f} M samples/s over 8 scenes ({1e3 * wall / steps:.3f} ms per "
-        f"round of 8 steps, {samples / steps:.0f} samples per round)")
+        f"round, one graph launch; eight eager steps: {samples_e / wall_e / 1e6:.2f} M samples/s), "
+        f"{samples / steps:.0f} samples per round, rays per scene {rays}, {captures} captures")
     return dict(samples_per_s=round(samples / wall, 1), ms_per_round=round(1e3 * wall / steps, 4),
-                samples_per_round=round(samples / steps, 1), log2_hashmap_size=log2_t, batch=batch)
+                samples_per_round=round(samples / steps, 1), eager_samples_per_s=round(samples_e / wall_e, 1),
+                rays_per_scene=rays, log2_hashmap_size=MULTI_LOG2_T, batch=MULTI_BATCH)
 
 
 def large_scene(tn, synthetic):
@@ -656,7 +689,9 @@ def main():
     roof, table = roofline(main, n_params, dtype)
     extra = dict(phases_ms_per_step=table, occupancy_update_ms=round(main["occupancy_update_ms"], 3),
                  occupied_fraction=round(main["occupied_fraction"], 4), loss=round(main["loss"], 5),
-                 samples_per_step=round(main["samples_per_step"], 1), rays_per_step=round(main["rays_per_step"], 1))
+                 samples_per_step=round(main["samples_per_step"], 1), rays_per_step=round(main["rays_per_step"], 1),
+                 eager_samples_per_s=round(main["eager_samples_per_s"], 1),
+                 eager_ms_per_step=round(main["eager_ms_per_step"], 4))
     if 0 in results and main_p != 0:
         f = results[0]
         extra["fp32_samples_per_s"] = round(f["samples_per_s"], 1)

@@ -423,7 +423,24 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
         .def_readwrite("alpha_selection_probability", &NeRFTrainingController::alpha_selection_probability)
         .def_readwrite("min_step_size", &NeRFTrainingController::min_step_size)
         .def_readwrite("use_distortion_loss", &NeRFTrainingController::use_distortion_loss)
+        .def_property(
+            "distortion_lambda", [](const NeRFTrainingController& tc) { return tc.config.distortion_lambda; },
+            [](NeRFTrainingController& tc, float v) { tc.config.distortion_lambda = v; })
         .def_readwrite("nerf", &NeRFTrainingController::proxy);
+
+    // several NeRFs trained side by side, one captured launch per round
+    py::class_<NeRFMultiTrainingController>(m, "MultiTrainer")
+        .def(py::init([](const std::vector<std::shared_ptr<NeRFTrainingController>>& ts) {
+                 std::vector<NeRFTrainingController*> raw;
+                 for (const auto& t : ts) raw.push_back(t.get());
+                 return std::make_unique<NeRFMultiTrainingController>(raw);
+             }),
+             py::arg("trainers"), py::keep_alive<1, 2>())
+        .def("train_steps", &NeRFMultiTrainingController::train_steps, py::arg("n_rounds"),
+             "One training step of every NeRF per round, each round one graph launch.",
+             py::call_guard<py::gil_scoped_release>())
+        .def("synchronize", &NeRFMultiTrainingController::synchronize, py::call_guard<py::gil_scoped_release>())
+        .def_readonly("_graph_captures", &NeRFMultiTrainingController::n_captures);
 
     register_renderer_bindings(m);
 }

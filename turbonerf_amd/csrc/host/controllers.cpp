@@ -102,7 +102,23 @@ NeRFTrainingController::NeRFTrainingController(NeRFProxy* p, TrainingConfig cfg)
     if (!proxy) throw std::runtime_error("NeRFTrainingController: null NeRF");
 }
 
-NeRFTrainingController::~NeRFTrainingController() = default;
+NeRFTrainingController::~NeRFTrainingController() {
+    ctx_.reset();
+    if (ev_in_) (void)hipEventDestroy(ev_in_);
+    if (ev_out_) (void)hipEventDestroy(ev_out_);
+}
+
+void NeRFTrainingController::begin_work() {
+    if (!ev_in_) TN_HIP(hipEventCreateWithFlags(&ev_in_, hipEventDisableTiming));
+    TN_HIP(hipEventRecord(ev_in_, DeviceManager::stream()));
+    TN_HIP(hipStreamWaitEvent(ctx_->stream, ev_in_, 0));
+}
+
+void NeRFTrainingController::end_work() {
+    if (!ev_out_) TN_HIP(hipEventCreateWithFlags(&ev_out_, hipEventDisableTiming));
+    TN_HIP(hipEventRecord(ev_out_, ctx_->stream));
+    TN_HIP(hipStreamWaitEvent(DeviceManager::stream(), ev_out_, 0));
+}
 
 void NeRFTrainingController::setup_data(uint32_t batch_size) {
     if (!proxy->dataset) throw std::runtime_error("setup_data: the NeRF has no dataset attached");
@@ -110,7 +126,7 @@ void NeRFTrainingController::setup_data(uint32_t batch_size) {
     NeRF& nerf = proxy->nerfs[0];
     const Dataset& ds = *proxy->dataset;
     if (ctx_ && generation_ != proxy->generation) ctx_.reset();  // the slot was destroyed and reused
-    if (!ctx_) ctx_ = std::make_unique<Trainer::Context>(s, &ds, &nerf, batch_size);
+    if (!ctx_) ctx_ = std::make_unique<Trainer::Context>(&ds, &nerf, batch_size);
     generation_ = proxy->generation;
     ctx_->dataset = &ds;
     ctx_->nerf = &nerf;
@@ -189,11 +205,14 @@ void NeRFTrainingController::sync_settings() {
     ctx_->alpha_selection_threshold = alpha_selection_threshold;
     ctx_->min_step_size = min_step_size;
     ctx_->use_distortion_loss = use_distortion_loss;
+    ctx_->distortion_lambda = config.distortion_lambda;
 }
 
 NeRFTrainingController::TrainingMetrics NeRFTrainingController::train_step() {
     sync_settings();
+    begin_work();
     const float loss = trainer_.train_step(*ctx_, proxy->training_step);
+    end_work();
     proxy->training_step++;
     TrainingMetrics m;
     m.loss = loss;
@@ -206,6 +225,7 @@ NeRFTrainingController::TrainingMetrics NeRFTrainingController::train_step() {
 
 NeRFTrainingController::TrainingMetrics NeRFTrainingController::train_steps(uint32_t n, bool read, bool graph) {
     sync_settings();
+    begin_work();
     if (graph) {
         trainer_.enqueue_train_steps_graph(*ctx_, proxy->training_step, n);
         proxy->training_step += n;
@@ -215,6 +235,7 @@ NeRFTrainingController::TrainingMetrics NeRFTrainingController::train_steps(uint
             proxy->training_step++;
         }
     }
+    end_work();
     TrainingMetrics m;
     m.step = proxy->training_step;
     if (read) {
@@ -229,7 +250,9 @@ NeRFTrainingController::TrainingMetrics NeRFTrainingController::train_steps(uint
 NeRFTrainingController::OccupancyGridMetrics NeRFTrainingController::update_occupancy_grid(uint32_t training_step) {
     sync_settings();
     OccupancyGridMetrics m;
+    begin_work();
     m.n_occupied = trainer_.update_occupancy_grid(*ctx_, training_step);
+    end_work();
     m.n_total = proxy->nerfs[0].occupancy_grid.get_n_total_elements();
     return m;
 }
@@ -247,6 +270,8 @@ void NeRFTrainingController::set_occupancy_from_density(const float* density_hos
 std::vector<float> NeRFTrainingController::get_params() const {
     const NeRF& nerf = proxy->nerfs[0];
     std::vector<float> h(nerf.params.params_fp.size());
+    // training work ends with the device stream waiting for it
+    TN_HIP(hipStreamSynchronize(DeviceManager::stream()));
     TN_HIP(hipMemcpy(h.data(), nerf.params.params_fp.data(), h.size() * sizeof(float), hipMemcpyDeviceToHost));
     return h;
 }
@@ -286,8 +311,128 @@ uint64_t NeRFTrainingController::samples_trained() {
 }
 
 void NeRFTrainingController::synchronize() {
-    if (ctx_) TN_HIP(hipStreamSynchronize(ctx_->gen_stream));
+    if (ctx_) {
+        TN_HIP(hipStreamSynchronize(ctx_->gen_stream));
+        TN_HIP(hipStreamSynchronize(ctx_->stream));
+    }
     TN_HIP(hipStreamSynchronize(DeviceManager::stream()));
+}
+
+// ------------------------------------------------------------------ several NeRFs
+NeRFMultiTrainingController::NeRFMultiTrainingController(std::vector<NeRFTrainingController*> trainers)
+    : trainers_(std::move(trainers)) {
+    if (trainers_.empty()) throw std::runtime_error("NeRFMultiTrainingController: no trainers");
+    for (size_t i = 0; i < trainers_.size(); ++i) {
+        if (!trainers_[i]) throw std::runtime_error("NeRFMultiTrainingController: null trainer");
+        for (size_t j = 0; j < i; ++j)
+            if (trainers_[j] == trainers_[i] || trainers_[j]->proxy == trainers_[i]->proxy)
+                throw std::runtime_error("NeRFMultiTrainingController: a NeRF appears twice");
+    }
+    TN_HIP(hipStreamCreateWithFlags(&root_, hipStreamNonBlocking));
+    ev_pre_.resize(3 * trainers_.size());
+    for (auto& e : ev_pre_) TN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+NeRFMultiTrainingController::~NeRFMultiTrainingController() {
+    (void)hipStreamSynchronize(root_);
+    for (auto& g : graph_)
+        if (g) (void)hipGraphExecDestroy(g);
+    for (auto e : ev_pre_) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(root_);
+}
+
+void NeRFMultiTrainingController::synchronize() {
+    TN_HIP(hipStreamSynchronize(root_));
+    for (auto* t : trainers_) t->synchronize();
+}
+
+// The round's graph: every NeRF's step captured on its own network stream
+// (the generation stream forked off it and joined back), each added to one
+// parent graph as an independent child-graph node.
+void NeRFMultiTrainingController::capture(int slot, const std::vector<uint64_t>& key) {
+    if (graph_[slot]) (void)hipGraphExecDestroy(graph_[slot]);
+    graph_[slot] = nullptr;
+    hipGraph_t parent = nullptr;
+    TN_HIP(hipGraphCreate(&parent, 0));
+    try {
+        for (auto* t : trainers_) {
+            Trainer::Context& ctx = *t->ctx_;
+            hipGraph_t child = t->trainer_.capture_graph(ctx, ctx.cur);
+            hipGraphNode_t node = nullptr;
+            const hipError_t e = hipGraphAddChildGraphNode(&node, parent, nullptr, 0, child);  // copies `child`
+            (void)hipGraphDestroy(child);
+            TN_HIP(e);
+        }
+    } catch (...) {
+        (void)hipGraphDestroy(parent);
+        throw;
+    }
+    const hipError_t e = hipGraphInstantiate(&graph_[slot], parent, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(parent);
+    TN_HIP(e);
+    key_[slot] = key;
+    ++n_captures;
+}
+
+void NeRFMultiTrainingController::train_steps(uint32_t n_rounds) {
+    if (n_rounds == 0) return;
+    for (size_t k = 0; k < trainers_.size(); ++k) {
+        NeRFTrainingController& t = *trainers_[k];
+        t.sync_settings();
+        t.begin_work();
+        Trainer::Context& ctx = *t.ctx_;
+        TN_HIP(hipStreamSynchronize(ctx.stream));  // the pinned step value below is free
+        TN_HIP(hipStreamSynchronize(root_));
+        ctx.nerf->network.prepare_workspace(ctx.batches[0].batch_size, ctx.batches[0].ray_capacity);
+        const int c = ctx.cur;
+        if (!ctx.prefetched) {
+            TN_HIP(hipStreamWaitEvent(ctx.gen_stream, ctx.ev_grid, 0));
+            TN_HIP(hipStreamWaitEvent(ctx.gen_stream, ctx.ev_consumed[c], 0));
+            t.trainer_.generate_next_training_batch(ctx, ctx.batches[c], ctx.gen_stream, &ctx.timer);
+            TN_HIP(hipEventRecord(ctx.ev_ready[c], ctx.gen_stream));
+            ctx.prefetched = true;
+        }
+        // the graph follows everything already on this NeRF's streams
+        TN_HIP(hipEventRecord(ev_pre_[3 * k], ctx.stream));
+        TN_HIP(hipEventRecord(ev_pre_[3 * k + 1], ctx.gen_stream));
+        TN_HIP(hipStreamWaitEvent(root_, ev_pre_[3 * k], 0));
+        TN_HIP(hipStreamWaitEvent(root_, ev_pre_[3 * k + 1], 0));
+        TN_HIP(hipStreamWaitEvent(root_, ctx.ev_ready[c], 0));
+        ctx.step_host[0] = t.proxy->training_step;
+        TN_HIP(hipMemcpyAsync(ctx.d_step.data(), ctx.step_host.data(), sizeof(uint32_t), hipMemcpyHostToDevice,
+                              root_));
+    }
+    for (uint32_t r = 0; r < n_rounds; ++r) {
+        // what every NeRF's captured step has baked in, with the batch each trains on
+        std::vector<uint64_t> key;
+        for (auto* t : trainers_) {
+            const std::vector<uint64_t> kt = t->trainer_.graph_key(*t->ctx_);
+            key.insert(key.end(), kt.begin(), kt.end());
+            key.push_back((uint64_t)t->ctx_->cur);
+        }
+        const int slot = trainers_[0]->ctx_->cur;
+        if (!graph_[slot] || key_[slot] != key) capture(slot, key);
+        TN_HIP(hipGraphLaunch(graph_[slot], root_));
+        for (auto* t : trainers_) {
+            Trainer::Context& ctx = *t->ctx_;
+            const int c = ctx.cur, nx = c ^ 1;
+            TN_HIP(hipEventRecord(ctx.ev_ready[nx], root_));
+            TN_HIP(hipEventRecord(ctx.ev_consumed[c], root_));
+            ctx.last = c;
+            ctx.cur = nx;
+            ctx.prefetched = true;
+            t->proxy->training_step++;
+            if (!t->proxy->can_render) t->proxy->can_render = true;
+        }
+    }
+    // later work on each NeRF's own streams follows the rounds
+    for (size_t k = 0; k < trainers_.size(); ++k) {
+        Trainer::Context& ctx = *trainers_[k]->ctx_;
+        TN_HIP(hipEventRecord(ev_pre_[3 * k + 2], root_));
+        TN_HIP(hipStreamWaitEvent(ctx.stream, ev_pre_[3 * k + 2], 0));
+        TN_HIP(hipStreamWaitEvent(ctx.gen_stream, ev_pre_[3 * k + 2], 0));
+        trainers_[k]->end_work();
+    }
 }
 
 std::vector<size_t> NeRFTrainingController::get_cuda_memory_allocated() const {

@@ -150,10 +150,42 @@ struct NeRFTrainingController {
     TrainingConfig config;
 
 private:
+    friend class NeRFMultiTrainingController;
     std::unique_ptr<Trainer::Context> ctx_;
     uint64_t generation_ = 0;  // proxy->generation when setup_data() ran
     Trainer trainer_;
+    hipEvent_t ev_in_ = nullptr, ev_out_ = nullptr;
     void sync_settings();
+    // Training runs on the trainer's own streams; begin_work orders it after
+    // the work already on the device stream (uploads, parameter writes),
+    // end_work orders later device-stream work (renders, snapshots) after it.
+    void begin_work();
+    void end_work();
+};
+
+// Several NeRFs trained side by side (reference: one training controller per
+// NeRF, stepped in turn). Each round enqueues one training step of every NeRF
+// as one launch of a HIP graph that holds every NeRF's captured step as an
+// independent child graph, so the steps may run concurrently; results are
+// those of stepping the controllers one after another. A controller may
+// still be stepped on its own between rounds.
+class NeRFMultiTrainingController {
+public:
+    explicit NeRFMultiTrainingController(std::vector<NeRFTrainingController*> trainers);
+    ~NeRFMultiTrainingController();
+    NeRFMultiTrainingController(const NeRFMultiTrainingController&) = delete;
+    NeRFMultiTrainingController& operator=(const NeRFMultiTrainingController&) = delete;
+    void train_steps(uint32_t n_rounds);
+    void synchronize();
+    uint32_t n_captures = 0;
+
+private:
+    std::vector<NeRFTrainingController*> trainers_;
+    hipStream_t root_ = nullptr;
+    std::vector<hipEvent_t> ev_pre_;
+    hipGraphExec_t graph_[2] = {nullptr, nullptr};
+    std::vector<uint64_t> key_[2];
+    void capture(int slot, const std::vector<uint64_t>& key);
 };
 
 }  // namespace turbo

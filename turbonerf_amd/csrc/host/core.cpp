@@ -281,8 +281,8 @@ void DatasetWorkspace::upload(const Dataset& ds, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ trainer
-Trainer::Context::Context(hipStream_t s, const Dataset* ds, NeRF* n, uint32_t B)
-    : stream(s), dataset(ds), nerf(n), batch_size(B) {
+Trainer::Context::Context(const Dataset* ds, NeRF* n, uint32_t B) : dataset(ds), nerf(n), batch_size(B) {
+    TN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     TN_HIP(hipStreamCreateWithFlags(&gen_stream, hipStreamNonBlocking));
     for (int k = 0; k < 2; ++k) {
         TN_HIP(hipEventCreateWithFlags(&ev_ready[k], hipEventDisableTiming));
@@ -315,6 +315,7 @@ Trainer::Context::~Context() {
     (void)hipEventDestroy(ev_fork);
     (void)hipEventDestroy(ev_join);
     (void)hipStreamDestroy(gen_stream);
+    (void)hipStreamDestroy(stream);
 }
 
 void Trainer::generate_next_training_batch(Context& ctx, TrainingWorkspace& ws, hipStream_t s, PhaseTimer* tm) {
@@ -443,7 +444,7 @@ std::vector<uint64_t> Trainer::graph_key(const Context& ctx) const {
     return k;
 }
 
-void Trainer::capture_step(Context& ctx, int c) {
+hipGraph_t Trainer::capture_graph(Context& ctx, int c) {
     hipGraph_t g = nullptr;
     TN_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
     try {
@@ -454,6 +455,11 @@ void Trainer::capture_step(Context& ctx, int c) {
         throw;
     }
     TN_HIP(hipStreamEndCapture(ctx.stream, &g));
+    return g;
+}
+
+void Trainer::capture_step(Context& ctx, int c) {
+    hipGraph_t g = capture_graph(ctx, c);
     const hipError_t e = hipGraphInstantiate(&ctx.graph[c], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     TN_HIP(e);

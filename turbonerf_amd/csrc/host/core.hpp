@@ -248,7 +248,7 @@ struct Trainer {
     // discards the batch generated ahead, so every batch is marched through
     // the grid the reference would use.
     struct Context {
-        hipStream_t stream;      // network, optimiser and occupancy update
+        hipStream_t stream;      // network, optimiser and occupancy update (the trainer's own)
         hipStream_t gen_stream;  // batch generation
         TrainingWorkspace batches[2];
         int cur = 0;              // batch the next step trains on
@@ -288,7 +288,7 @@ struct Trainer {
         PhaseTimer timer;
         DeviceBuffer<unsigned long long> samples_total;  // running sum of samples trained on
 
-        Context(hipStream_t s, const Dataset* ds, NeRF* n, uint32_t B);
+        Context(const Dataset* ds, NeRF* n, uint32_t B);
         ~Context();
         Context(const Context&) = delete;
         Context& operator=(const Context&) = delete;
@@ -316,9 +316,17 @@ struct Trainer {
     // Read back batch counts and losses of the last enqueued step.
     void read_metrics(Context& ctx, float& loss);
 
-private:
+    // Building blocks of captured steps. train_batch trains on batches[c]
+    // (network stream) while it generates batches[c ^ 1] (generation
+    // stream); with `graph` the work is being captured and joins back into
+    // the network stream. graph_key lists what a captured step has baked in.
     void train_batch(Context& ctx, int c, uint32_t training_step, bool graph);
     std::vector<uint64_t> graph_key(const Context& ctx) const;
+    // train_batch(ctx, c, 0, true) captured on the network stream (the
+    // caller owns the returned graph)
+    hipGraph_t capture_graph(Context& ctx, int c);
+
+private:
     void capture_step(Context& ctx, int c);
 };
 
