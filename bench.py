@@ -173,6 +173,12 @@ def run_precision(tn, synthetic, scene, precision, args):
             om = tr.update_occupancy_grid(step)
             if step % 512 == 0:
                 log(f"[bench {name}] warm-up step {step}: occupied {om.n_occupied / om.n_total:.3f}")
+    # launch shapes from the autotuner, timed on this workload (its trial
+    # rounds are training steps too, part of the warm-up)
+    tune = tr.autotune(rounds_per_trial=16, repetitions=3)
+    step = nerf.training_step
+    log(f"[bench {name}] autotuned launch shapes {tune['picks']}: {tune['ns_per_sample_before']:.3f} -> "
+        f"{tune['ns_per_sample_after']:.3f} ns/sample")
     m = tr.train_step()
     step += 1
     log(f"[bench {name}] warm-up done: loss {m.loss:.4f} rays {m.n_rays} samples {m.n_samples}")
@@ -233,6 +239,8 @@ def run_precision(tn, synthetic, scene, precision, args):
     res["occupied_fraction"] = om.n_occupied / om.n_total
     res["bitfield"] = tr.get_occupancy_bitfield()
     res["trainer"] = (mgr, nerf, tr)
+    res["autotune"] = dict(picks=tune["picks"], ns_per_sample_before=round(tune["ns_per_sample_before"], 4),
+                           ns_per_sample_after=round(tune["ns_per_sample_after"], 4))
     log(f"[bench {name}] {res['samples_per_s'] / 1e6:.2f} M samples/s, {res['ms_per_step']:.3f} ms/step "
         f"(captured step; eager {res['eager_samples_per_s'] / 1e6:.2f} M samples/s), "
         f"{res['samples_per_step']:.0f} samples/step, {res['rays_per_step']:.0f} rays/step")
@@ -369,6 +377,7 @@ def multi_scene(tn, synthetic, steps):
         mt.train_steps(16)
         for tr in trainers:
             tr.update_occupancy_grid(16 * (i + 1))
+    tune = mt.autotune(rounds_per_trial=8, repetitions=3)
     mt.train_steps(2)  # both graphs captured before the clock starts
     rays = [tr.train_steps(0, read_metrics=True).n_rays for tr in trainers]
 
@@ -394,10 +403,11 @@ def multi_scene(tn, synthetic, steps):
         tr.teardown()
     log(f"[bench multi-scene] {samples / wall / 1e6:.2f} M samples/s over 8 scenes ({1e3 * wall / steps:.3f} ms per "
         f"round, one graph launch; eight eager steps: {samples_e / wall_e / 1e6:.2f} M samples/s), "
-        f"{samples / steps:.0f} samples per round, rays per scene {rays}, {captures} captures")
+        f"{samples / steps:.0f} samples per round, rays per scene {rays}, {captures} captures, autotuned "
+        f"{tune['picks']}")
     return dict(samples_per_s=round(samples / wall, 1), ms_per_round=round(1e3 * wall / steps, 4),
                 samples_per_round=round(samples / steps, 1), eager_samples_per_s=round(samples_e / wall_e, 1),
-                rays_per_scene=rays, log2_hashmap_size=MULTI_LOG2_T, batch=MULTI_BATCH)
+                rays_per_scene=rays, log2_hashmap_size=MULTI_LOG2_T, batch=MULTI_BATCH, autotune=tune["picks"])
 
 
 def large_scene(tn, synthetic):
@@ -684,6 +694,8 @@ def main():
         results[p] = run_precision(tn, synthetic, scene, p, args)
     main_p = precs[0]
     main = results[main_p]
+    for k, v in main["autotune"]["picks"].items():  # the later legs train and render the main NeRF
+        tn.set_tuning(k, v)
     dtype = "fp16" if main_p == 1 else "fp32"
     n_params = ops.hashgrid_layout().n_params + 9408
     roof, table = roofline(main, n_params, dtype)
@@ -691,13 +703,14 @@ def main():
                  occupied_fraction=round(main["occupied_fraction"], 4), loss=round(main["loss"], 5),
                  samples_per_step=round(main["samples_per_step"], 1), rays_per_step=round(main["rays_per_step"], 1),
                  eager_samples_per_s=round(main["eager_samples_per_s"], 1),
-                 eager_ms_per_step=round(main["eager_ms_per_step"], 4))
+                 eager_ms_per_step=round(main["eager_ms_per_step"], 4), autotune=main["autotune"])
     if 0 in results and main_p != 0:
         f = results[0]
         extra["fp32_samples_per_s"] = round(f["samples_per_s"], 1)
         extra["fp32_ms_per_step"] = round(f["ms_per_step"], 4)
         extra["mixed_over_fp32"] = round(main["samples_per_s"] / f["samples_per_s"], 3)
         extra["fp32_roofline"] = roofline(f, n_params, "fp32")[0]
+        extra["fp32_autotune"] = f["autotune"]
     if not args.no_render:
         mr, ms = render_rate(tn, synthetic, scene, main["trainer"])
         extra["render_mrays_per_s"] = round(mr, 2)

@@ -301,6 +301,17 @@ int tn_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float loss_sca
  * src/utils/parallel-utils.cuh:60-82). */
 int tn_cast_params(uint64_t n, const float* src, int lp_dtype, void* dst, tn_stream_t stream);
 
+/* -------------------------------------------------------------- tuning */
+
+/* Set / read a launch-shape knob (enum tn_tune_knob); a value outside the
+ * knob's range is refused with TN_ERR_INVALID_ARGUMENT and changes nothing.
+ * No reference counterpart: the reference's launch shapes are compile-time
+ * constants. tn_tune_generation() counts accepted changes, so a captured HIP
+ * graph (whose launch shapes are baked in) can tell that it is stale. */
+int tn_tune_set(int knob, int value);
+int tn_tune_get(int knob, int* value);
+uint32_t tn_tune_generation(void);
+
 /* -------------------------------------------------------------- render */
 
 /* Row-major pixel rays for pixels [start_pixel, start_pixel + n_rays) of

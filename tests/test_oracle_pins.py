@@ -922,10 +922,34 @@ def test_c_abi_exports_every_declared_symbol():
     from turbonerf_amd import _capi
 
     hdr = (Path(__file__).resolve().parent.parent / "include" / "turbonerf" / "tn_c_api.h").read_text()
-    declared = set(re.findall(r"^(?:int|size_t)\s+(tn_\w+)\(", hdr, re.M))
+    declared = set(re.findall(r"^(?:int|size_t|uint32_t)\s+(tn_\w+)\(", hdr, re.M))
     assert len(declared) >= 20
     lib = ctypes.CDLL(str(_capi.LIB_PATH))
     missing = [s for s in declared if not hasattr(lib, s)]
     assert not missing, missing
     # and the Python binding covers all of them
     assert declared <= set(_capi.declared_symbols()), declared - set(_capi.declared_symbols())
+
+
+def test_tuning_knobs_validate_and_count_changes():
+    """Launch-shape knobs: defaults, refused out-of-range values (nothing
+    changes), and the change counter a captured graph is keyed on (host-only
+    entry points: no GPU needed)."""
+    from turbonerf_amd import _capi
+
+    lib = _capi.lib()
+    v = ctypes.c_int()
+    defaults = {0: 256, 1: 128, 2: 2, 3: 1, 4: 2048, 5: 256}
+    for k, d in defaults.items():
+        assert lib.tn_tune_get(k, ctypes.byref(v)) == 0 and v.value == d
+    assert lib.tn_tune_get(6, ctypes.byref(v)) == 1 and lib.tn_tune_get(-1, ctypes.byref(v)) == 1
+    g0 = lib.tn_tune_generation()
+    bad = {0: [32, 192, 512], 1: [7, 129], 2: [0, 5], 3: [0, 3], 4: [255, 8193], 5: [0, 96, 1088], 6: [1]}
+    for k, vals in bad.items():
+        for x in vals:
+            assert lib.tn_tune_set(k, x) == 1, (k, x)
+    assert lib.tn_tune_generation() == g0
+    assert lib.tn_tune_set(4, 2048) == 0 and lib.tn_tune_generation() == g0  # unchanged value
+    assert lib.tn_tune_set(4, 1024) == 0 and lib.tn_tune_generation() == g0 + 1
+    assert lib.tn_tune_get(4, ctypes.byref(v)) == 0 and v.value == 1024
+    assert lib.tn_tune_set(4, 2048) == 0 and lib.tn_tune_generation() == g0 + 2

@@ -141,6 +141,9 @@ _SIGS: dict[str, tuple] = {
     "tn_render_write_rgba": (C.c_int, [u32, vp, u32, vp, vp, vp, vp, vp]),
     "tn_distortion_scratch_bytes": (sz, [u32, u32]),
     "tn_distortion": (C.c_int, [C.c_int, vp, u32, u32, f32, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]),
+    "tn_tune_set": (C.c_int, [C.c_int, C.c_int]),
+    "tn_tune_get": (C.c_int, [C.c_int, vp]),
+    "tn_tune_generation": (u32, []),
 }
 
 # entry points added by later kernel files register their signatures here

@@ -7,6 +7,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include "autotune.hpp"
 #include "controllers.hpp"
 #include "file_manager.hpp"
 #include "renderer.hpp"
@@ -29,6 +30,15 @@ float2v to_float2(py::handle h) {
 }
 
 // host copy of the first n elements of a device buffer (n = 0: all)
+py::dict autotune_dict(const AutotuneResult& r) {
+    py::dict d;
+    d["picks"] = r.picks;
+    d["trials"] = r.trials;
+    d["ns_per_sample_before"] = r.ns_per_sample_before;
+    d["ns_per_sample_after"] = r.ns_per_sample_after;
+    return d;
+}
+
 template <typename T>
 py::array_t<T> to_numpy(const DeviceBuffer<T>& b, size_t n = 0) {
     if (n == 0 || n > b.size()) n = b.size();
@@ -398,6 +408,19 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
                  return d;
              })
         .def("_graph_captures", [](NeRFTrainingController& tc) { return tc.context() ? tc.context()->n_captures : 0u; })
+        .def(
+            "autotune",
+            [](NeRFTrainingController& tc, uint32_t rounds, uint32_t reps) {
+                AutotuneResult r;
+                {
+                    py::gil_scoped_release release;
+                    r = Autotuner::tune([&](uint32_t n) { tc.train_steps(n, false, true); },
+                                        [&]() { return tc.samples_trained(); }, rounds, reps);
+                }
+                return autotune_dict(r);
+            },
+            py::arg("rounds_per_trial") = 16, py::arg("repetitions") = 3,
+            "Pick the kernels' launch shapes by timing captured training steps (the steps train the NeRF).")
         .def("_get_optimizer_state",
              [](NeRFTrainingController& tc) {
                  tc.synchronize();
@@ -440,7 +463,24 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
              "One training step of every NeRF per round, each round one graph launch.",
              py::call_guard<py::gil_scoped_release>())
         .def("synchronize", &NeRFMultiTrainingController::synchronize, py::call_guard<py::gil_scoped_release>())
+        .def("samples_trained", &NeRFMultiTrainingController::samples_trained)
+        .def(
+            "autotune",
+            [](NeRFMultiTrainingController& mt, uint32_t rounds, uint32_t reps) {
+                AutotuneResult r;
+                {
+                    py::gil_scoped_release release;
+                    r = Autotuner::tune([&](uint32_t n) { mt.train_steps(n); }, [&]() { return mt.samples_trained(); },
+                                        rounds, reps);
+                }
+                return autotune_dict(r);
+            },
+            py::arg("rounds_per_trial") = 16, py::arg("repetitions") = 3,
+            "Pick the kernels' launch shapes by timing rounds of these NeRFs (the rounds train them).")
         .def_readonly("_graph_captures", &NeRFMultiTrainingController::n_captures);
+
+    m.def("get_tuning", &Autotuner::current, "Current launch-shape knobs (name -> value).");
+    m.def("set_tuning", &Autotuner::set, py::arg("name"), py::arg("value"), "Set one launch-shape knob.");
 
     register_renderer_bindings(m);
 }

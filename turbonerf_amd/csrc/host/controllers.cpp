@@ -341,14 +341,20 @@ NeRFMultiTrainingController::~NeRFMultiTrainingController() {
     (void)hipStreamDestroy(root_);
 }
 
+uint64_t NeRFMultiTrainingController::samples_trained() {
+    uint64_t s = 0;
+    for (auto* t : trainers_) s += t->samples_trained();
+    return s;
+}
+
 void NeRFMultiTrainingController::synchronize() {
     TN_HIP(hipStreamSynchronize(root_));
     for (auto* t : trainers_) t->synchronize();
 }
 
-// The round's graph: every NeRF's step captured on its own network stream
-// (the generation stream forked off it and joined back), each added to one
-// parent graph as an independent child-graph node.
+// The round's graph: every NeRF's training of its current batch and
+// generation of its next batch, each captured on its own stream and added to
+// one parent graph as an independent child-graph node.
 void NeRFMultiTrainingController::capture(int slot, const std::vector<uint64_t>& key) {
     if (graph_[slot]) (void)hipGraphExecDestroy(graph_[slot]);
     graph_[slot] = nullptr;
@@ -357,11 +363,13 @@ void NeRFMultiTrainingController::capture(int slot, const std::vector<uint64_t>&
     try {
         for (auto* t : trainers_) {
             Trainer::Context& ctx = *t->ctx_;
-            hipGraph_t child = t->trainer_.capture_graph(ctx, ctx.cur);
-            hipGraphNode_t node = nullptr;
-            const hipError_t e = hipGraphAddChildGraphNode(&node, parent, nullptr, 0, child);  // copies `child`
-            (void)hipGraphDestroy(child);
-            TN_HIP(e);
+            const int c = ctx.cur;
+            for (hipGraph_t child : {t->trainer_.capture_train(ctx, c), t->trainer_.capture_gen(ctx, c ^ 1)}) {
+                hipGraphNode_t node = nullptr;
+                const hipError_t e = hipGraphAddChildGraphNode(&node, parent, nullptr, 0, child);  // copies `child`
+                (void)hipGraphDestroy(child);
+                TN_HIP(e);
+            }
         }
     } catch (...) {
         (void)hipGraphDestroy(parent);

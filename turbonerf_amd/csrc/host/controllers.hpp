@@ -177,6 +177,7 @@ public:
     NeRFMultiTrainingController& operator=(const NeRFMultiTrainingController&) = delete;
     void train_steps(uint32_t n_rounds);
     void synchronize();
+    uint64_t samples_trained();  // summed over the NeRFs (synchronises)
     uint32_t n_captures = 0;
 
 private:

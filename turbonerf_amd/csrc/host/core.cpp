@@ -290,14 +290,13 @@ Trainer::Context::Context(const Dataset* ds, NeRF* n, uint32_t B) : dataset(ds),
     }
     TN_HIP(hipEventCreateWithFlags(&ev_grid, hipEventDisableTiming));
     TN_HIP(hipEventCreateWithFlags(&ev_gen_idle, hipEventDisableTiming));
-    TN_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    TN_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
 }
 
 void Trainer::Context::destroy_graphs() {
-    for (auto& g : graph) {
-        if (g) (void)hipGraphExecDestroy(g);
-        g = nullptr;
+    for (int c = 0; c < 2; ++c) {
+        if (train_graph[c]) (void)hipGraphExecDestroy(train_graph[c]);
+        if (gen_graph[c]) (void)hipGraphExecDestroy(gen_graph[c]);
+        train_graph[c] = gen_graph[c] = nullptr;
     }
     graph_key.clear();
 }
@@ -312,8 +311,6 @@ Trainer::Context::~Context() {
     destroy_graphs();
     (void)hipEventDestroy(ev_grid);
     (void)hipEventDestroy(ev_gen_idle);
-    (void)hipEventDestroy(ev_fork);
-    (void)hipEventDestroy(ev_join);
     (void)hipStreamDestroy(gen_stream);
     (void)hipStreamDestroy(stream);
 }
@@ -370,39 +367,34 @@ void Trainer::generate_next_training_batch(Context& ctx, TrainingWorkspace& ws, 
     }
 }
 
-// Train on batches[c] (network stream) while batches[c ^ 1] is generated
-// (generation stream); with `graph` the work is being captured.
-void Trainer::train_batch(Context& ctx, int c, uint32_t training_step, bool graph) {
-    const int nx = c ^ 1;
-    const hipStream_t gen = ctx.gen_stream, net = ctx.stream;
-    PhaseTimer* tm = graph ? nullptr : &ctx.timer;
-    if (graph) {
-        // fork the generation stream off the captured network stream
-        TN_HIP(hipEventRecord(ctx.ev_fork, net));
-        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_fork, 0));
-    } else {
-        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[nx], 0));
-    }
-    generate_next_training_batch(ctx, ctx.batches[nx], gen, tm);
-    if (graph) {
-        TN_HIP(hipEventRecord(ctx.ev_join, gen));
-    } else {
-        TN_HIP(hipEventRecord(ctx.ev_ready[nx], gen));
-        TN_HIP(hipStreamWaitEvent(net, ctx.ev_ready[c], 0));
-    }
+// Train on batches[c] on the network stream; with `graph` the work is being
+// captured.
+void Trainer::train_network(Context& ctx, int c, uint32_t training_step, bool graph) {
     TrainingWorkspace& ws = ctx.batches[c];
     NerfNetwork::Settings st;
     st.use_distortion_loss = ctx.use_distortion_loss;
     st.distortion_lambda = ctx.distortion_lambda;
-    st.timer = tm;
+    st.timer = graph ? nullptr : &ctx.timer;
     st.samples_total = ctx.samples_total.size() ? ctx.samples_total.data() : nullptr;
     st.step_counter = ctx.d_step.data();
     st.step_for_lr = graph;
-    ctx.nerf->network.train(net, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity, ws.info.data(),
-                            ws.random_float.data() + ws.ray_capacity, ws.ray_step.data(), ws.ray_offset.data(),
-                            ws.sample_t.data(), ws.sample_pos.data(), ws.sample_dir.data(), ws.sample_dt.data(),
-                            ws.sample_m_norm.data(), ws.sample_dt_norm.data(), ws.pix_rgba.data(), st);
-    if (graph) TN_HIP(hipStreamWaitEvent(net, ctx.ev_join, 0));  // join before the capture ends
+    ctx.nerf->network.train(ctx.stream, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity,
+                            ws.info.data(), ws.random_float.data() + ws.ray_capacity, ws.ray_step.data(),
+                            ws.ray_offset.data(), ws.sample_t.data(), ws.sample_pos.data(), ws.sample_dir.data(),
+                            ws.sample_dt.data(), ws.sample_m_norm.data(), ws.sample_dt_norm.data(),
+                            ws.pix_rgba.data(), st);
+}
+
+// Eager step: batches[c ^ 1] is generated on the generation stream once its
+// last training has consumed it, while batches[c] trains on the network
+// stream once it is ready.
+void Trainer::train_batch(Context& ctx, int c, uint32_t training_step) {
+    const int nx = c ^ 1;
+    TN_HIP(hipStreamWaitEvent(ctx.gen_stream, ctx.ev_consumed[nx], 0));
+    generate_next_training_batch(ctx, ctx.batches[nx], ctx.gen_stream, &ctx.timer);
+    TN_HIP(hipEventRecord(ctx.ev_ready[nx], ctx.gen_stream));
+    TN_HIP(hipStreamWaitEvent(ctx.stream, ctx.ev_ready[c], 0));
+    train_network(ctx, c, training_step, false);
 }
 
 void Trainer::enqueue_train_step(Context& ctx, uint32_t training_step) {
@@ -414,7 +406,7 @@ void Trainer::enqueue_train_step(Context& ctx, uint32_t training_step) {
         generate_next_training_batch(ctx, ctx.batches[c], ctx.gen_stream, &ctx.timer);
         TN_HIP(hipEventRecord(ctx.ev_ready[c], ctx.gen_stream));
     }
-    train_batch(ctx, c, training_step, false);
+    train_batch(ctx, c, training_step);
     TN_HIP(hipEventRecord(ctx.ev_consumed[c], ctx.stream));
     ctx.last = c;
     ctx.cur = c ^ 1;
@@ -439,30 +431,50 @@ std::vector<uint64_t> Trainer::graph_key(const Context& ctx) const {
         bits(c.learning_rate_decay), bits(c.beta1), bits(c.beta2), bits(c.epsilon), bits(c.l2_reg),
         (uint64_t)n.network.aabb_scale, n.network.grid.n_params, bits(ctx.dataset->bounding_box.size()),
         ptr(n.network.workspace.features.data()), ptr(n.network.workspace.hash_ws.data()),
-        ptr(n.network.workspace.mlp_ws.data()), ptr(ctx.samples_total.data())};
+        ptr(n.network.workspace.mlp_ws.data()), ptr(ctx.samples_total.data()), tn_tune_generation()};
     for (const auto& b : ctx.batches) k.push_back(ptr(b.random_float.data()));
     return k;
 }
 
-hipGraph_t Trainer::capture_graph(Context& ctx, int c) {
+namespace {
+
+// `enqueue` captured on `s` (the caller owns the returned graph)
+template <typename F>
+hipGraph_t capture_on(hipStream_t s, F&& enqueue) {
     hipGraph_t g = nullptr;
-    TN_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+    TN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
-        train_batch(ctx, c, 0, true);
+        enqueue();
     } catch (...) {
-        (void)hipStreamEndCapture(ctx.stream, &g);
+        (void)hipStreamEndCapture(s, &g);
         if (g) (void)hipGraphDestroy(g);
         throw;
     }
-    TN_HIP(hipStreamEndCapture(ctx.stream, &g));
+    TN_HIP(hipStreamEndCapture(s, &g));
     return g;
 }
 
-void Trainer::capture_step(Context& ctx, int c) {
-    hipGraph_t g = capture_graph(ctx, c);
-    const hipError_t e = hipGraphInstantiate(&ctx.graph[c], g, nullptr, nullptr, 0);
+hipGraphExec_t instantiate(hipGraph_t g) {
+    hipGraphExec_t x = nullptr;
+    const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     TN_HIP(e);
+    return x;
+}
+
+}  // namespace
+
+hipGraph_t Trainer::capture_train(Context& ctx, int c) {
+    return capture_on(ctx.stream, [&] { train_network(ctx, c, 0, true); });
+}
+
+hipGraph_t Trainer::capture_gen(Context& ctx, int c) {
+    return capture_on(ctx.gen_stream, [&] { generate_next_training_batch(ctx, ctx.batches[c], ctx.gen_stream, nullptr); });
+}
+
+void Trainer::instantiate_graphs(Context& ctx, int c) {
+    ctx.train_graph[c] = instantiate(capture_train(ctx, c));
+    ctx.gen_graph[c ^ 1] = instantiate(capture_gen(ctx, c ^ 1));
     ++ctx.n_captures;
 }
 
@@ -488,16 +500,15 @@ void Trainer::enqueue_train_steps_graph(Context& ctx, uint32_t first_step, uint3
             generate_next_training_batch(ctx, ctx.batches[c], gen, &ctx.timer);
             TN_HIP(hipEventRecord(ctx.ev_ready[c], gen));
         }
-        // the batch and the generation stream's last work come before the graph
+        if (!ctx.train_graph[c]) instantiate_graphs(ctx, c);
+        // the next batch, once its last training has consumed it
+        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[nx], 0));
+        TN_HIP(hipGraphLaunch(ctx.gen_graph[nx], gen));
+        TN_HIP(hipEventRecord(ctx.ev_ready[nx], gen));
+        // this batch, once it is ready
         TN_HIP(hipStreamWaitEvent(net, ctx.ev_ready[c], 0));
-        TN_HIP(hipEventRecord(ctx.ev_gen_idle, gen));
-        TN_HIP(hipStreamWaitEvent(net, ctx.ev_gen_idle, 0));
-        if (!ctx.graph[c]) capture_step(ctx, c);
-        TN_HIP(hipGraphLaunch(ctx.graph[c], net));
-        // the graph generated batches[nx] and consumed batches[c]
-        TN_HIP(hipEventRecord(ctx.ev_ready[nx], net));
+        TN_HIP(hipGraphLaunch(ctx.train_graph[c], net));
         TN_HIP(hipEventRecord(ctx.ev_consumed[c], net));
-        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[c], 0));  // later eager generation follows the graph
         ctx.last = c;
         ctx.cur = nx;
         ctx.prefetched = true;

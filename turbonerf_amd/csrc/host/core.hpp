@@ -254,7 +254,7 @@ struct Trainer {
         int cur = 0;              // batch the next step trains on
         int last = 0;             // batch the last step trained on
         bool prefetched = false;  // batches[cur] already holds a generated batch
-        hipEvent_t ev_ready[2]{}, ev_consumed[2]{}, ev_grid{}, ev_gen_idle{}, ev_fork{}, ev_join{};
+        hipEvent_t ev_ready[2]{}, ev_consumed[2]{}, ev_grid{}, ev_gen_idle{};
         DeviceBuffer<uint32_t> n_rays_next;  // device-side ray count for the next batch
         // position of the batch-generation uniform stream and the training
         // step, both on the device, so a captured step needs no host values
@@ -262,9 +262,13 @@ struct Trainer {
         DeviceBuffer<uint32_t> d_step;
         PinnedBuffer<uint32_t> step_host{1};
         uint64_t occ_rng_offset = 0;  // occupancy-update uniforms (their own key)
-        // captured steps: graph[c] trains on batches[c] while it generates
-        // batches[c ^ 1]; recaptured when anything they were built from changes
-        hipGraphExec_t graph[2] = {nullptr, nullptr};
+        // captured steps: train_graph[c] trains on batches[c] (network
+        // stream), gen_graph[c] generates batches[c] (generation stream); the
+        // two streams order them with the events of eager steps, so a step's
+        // training overlaps the next batch's generation as in eager steps.
+        // Recaptured when anything they were built from changes.
+        hipGraphExec_t train_graph[2] = {nullptr, nullptr};
+        hipGraphExec_t gen_graph[2] = {nullptr, nullptr};
         std::vector<uint64_t> graph_key;
         uint32_t n_captures = 0;  // graphs captured so far
         void destroy_graphs();
@@ -316,18 +320,21 @@ struct Trainer {
     // Read back batch counts and losses of the last enqueued step.
     void read_metrics(Context& ctx, float& loss);
 
-    // Building blocks of captured steps. train_batch trains on batches[c]
-    // (network stream) while it generates batches[c ^ 1] (generation
-    // stream); with `graph` the work is being captured and joins back into
-    // the network stream. graph_key lists what a captured step has baked in.
-    void train_batch(Context& ctx, int c, uint32_t training_step, bool graph);
+    // Building blocks of steps. train_network trains on batches[c] on the
+    // network stream (`graph`: it is being captured, so the learning-rate
+    // step is read on the device and no phase is timed). capture_train and
+    // capture_gen return the captured training of batches[c] and generation
+    // into batches[c] (the caller owns the graphs). graph_key lists what a
+    // captured step has baked in.
+    void train_network(Context& ctx, int c, uint32_t training_step, bool graph);
+    hipGraph_t capture_train(Context& ctx, int c);
+    hipGraph_t capture_gen(Context& ctx, int c);
     std::vector<uint64_t> graph_key(const Context& ctx) const;
-    // train_batch(ctx, c, 0, true) captured on the network stream (the
-    // caller owns the returned graph)
-    hipGraph_t capture_graph(Context& ctx, int c);
+    // Eager step on batches[c] while batches[c ^ 1] is generated.
+    void train_batch(Context& ctx, int c, uint32_t training_step);
 
 private:
-    void capture_step(Context& ctx, int c);
+    void instantiate_graphs(Context& ctx, int c);
 };
 
 }  // namespace turbo

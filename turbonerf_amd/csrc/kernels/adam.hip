@@ -165,14 +165,15 @@ extern "C" int tn_adam_step(uint32_t n, uint32_t n_grid, uint32_t n_net_end, flo
     TN_CHECK_ARG(aligned(weights, 16) && aligned(gradients, 16) && aligned(first_moments, 16) &&
                  aligned(second_moments, 16) && aligned(param_steps, 16) && (lp_dtype == 0 || aligned(weights_lp, 8)));
     if (n == 0) return TN_OK;
-    const unsigned blocks = blocks_for((n + 3) / 4, 256);
+    const unsigned tb = (unsigned)tune_value(TN_TUNE_ADAM_BLOCK);
+    const unsigned blocks = blocks_for((n + 3) / 4, tb);
     const float inv = 1.0f / loss_scale;
     if (lp_dtype == 0)
-        adam_kernel<float><<<blocks, 256, 0, stream>>>(n, n_grid, n_net_end, inv, scale_by_rays_info, learning_rate,
+        adam_kernel<float><<<blocks, tb, 0, stream>>>(n, n_grid, n_net_end, inv, scale_by_rays_info, learning_rate,
                                                        d_step, lr_decay, beta1, beta2, epsilon, l2_reg, weights,
                                                        nullptr, gradients, first_moments, second_moments, param_steps);
     else if (lp_dtype == 1)
-        adam_kernel<_Float16><<<blocks, 256, 0, stream>>>(n, n_grid, n_net_end, inv, scale_by_rays_info,
+        adam_kernel<_Float16><<<blocks, tb, 0, stream>>>(n, n_grid, n_net_end, inv, scale_by_rays_info,
                                                           learning_rate, d_step, lr_decay, beta1, beta2, epsilon,
                                                           l2_reg, weights, static_cast<_Float16*>(weights_lp),
                                                           gradients, first_moments, second_moments, param_steps);

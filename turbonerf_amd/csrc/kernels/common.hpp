@@ -37,6 +37,10 @@ inline unsigned blocks_for(uint64_t n, unsigned threads) {
     return (unsigned)(b == 0 ? 1 : b);
 }
 
+// Launch-shape knobs (enum tn_tune_knob), defined in tuning.hip; read by the
+// launchers on the host when they enqueue a kernel.
+int tune_value(int knob);
+
 // ---------------------------------------------------------------------------
 // Philox-4x32-10 counter-based RNG (Salmon et al., SC'11). Element g of a
 // stream keyed by `seed` is word (g & 3) of philox(counter = g >> 2).

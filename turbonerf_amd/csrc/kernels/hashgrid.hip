@@ -725,9 +725,10 @@ __global__ void __launch_bounds__(kHbFinishThreads) hashgrid_bwd_finish_kernel(
     }
 }
 
-// Bin plan for a layout and batch capacity; false when the table does not
-// fit the bin limits (the caller then takes the atomic kernel).
-inline bool hb_plan(const tn_hashgrid_t& g, uint32_t n_max, HbPlan& p) {
+// Bin plan for a layout and batch capacity with at most `level_blocks_max`
+// count/scatter workgroups per level; false when the table does not fit the
+// bin limits (the caller then takes the atomic kernel).
+inline bool hb_plan(const tn_hashgrid_t& g, uint32_t n_max, uint32_t level_blocks_max, HbPlan& p) {
     p = HbPlan{};
     uint32_t n_bins = 0, row = 0;
     for (int l = 0; l < g.n_levels; ++l) {
@@ -739,7 +740,7 @@ inline bool hb_plan(const tn_hashgrid_t& g, uint32_t n_max, HbPlan& p) {
     if (row > kHbMaxLevelBins || n_bins > kHbMaxBins || n_bins == 0) return false;
     const uint64_t L = (uint64_t)g.n_levels;
     const uint64_t chunks = ((uint64_t)n_max + 255) / 256;
-    const uint64_t per_level = chunks < kHbBlocksPerLevel ? (chunks > 0 ? chunks : 1) : kHbBlocksPerLevel;
+    const uint64_t per_level = chunks < level_blocks_max ? (chunks > 0 ? chunks : 1) : level_blocks_max;
     p.n_bins = n_bins;
     p.row = row;
     p.blocks = (uint32_t)(per_level * L);
@@ -855,17 +856,18 @@ extern "C" int tn_hashgrid_forward(int table_dtype, int feat_dtype, uint32_t n_m
     TN_CHECK_ARG(pos && table && features && n_max <= stride && n_max <= out_stride && grid.n_features == 2);
     TN_CHECK_ARG(grid.n_levels >= 1 && grid.n_levels <= TN_HASHGRID_MAX_LEVELS);
     if (n_max == 0) return TN_OK;
-    const dim3 blocks(blocks_for(n_max, 256), grid.n_levels);
+    const unsigned tb = (unsigned)tune_value(TN_TUNE_HASH_FWD_BLOCK);  // <= 256, the kernel's bound
+    const dim3 blocks(blocks_for(n_max, tb), grid.n_levels);
     if (table_dtype == 0 && feat_dtype == 0)
-        hashgrid_fwd_kernel<float, float><<<blocks, 256, 0, stream>>>(n_max, info, stride, out_stride, grid, pos,
+        hashgrid_fwd_kernel<float, float><<<blocks, tb, 0, stream>>>(n_max, info, stride, out_stride, grid, pos,
                                                                       static_cast<const float*>(table),
                                                                       static_cast<float*>(features));
     else if (table_dtype == 1 && feat_dtype == 1)
-        hashgrid_fwd_kernel<_Float16, _Float16><<<blocks, 256, 0, stream>>>(
+        hashgrid_fwd_kernel<_Float16, _Float16><<<blocks, tb, 0, stream>>>(
             n_max, info, stride, out_stride, grid, pos, static_cast<const _Float16*>(table),
             static_cast<_Float16*>(features));
     else if (table_dtype == 1 && feat_dtype == 0)
-        hashgrid_fwd_kernel<_Float16, float><<<blocks, 256, 0, stream>>>(n_max, info, stride, out_stride, grid, pos,
+        hashgrid_fwd_kernel<_Float16, float><<<blocks, tb, 0, stream>>>(n_max, info, stride, out_stride, grid, pos,
                                                                          static_cast<const _Float16*>(table),
                                                                          static_cast<float*>(features));
     else
@@ -896,10 +898,11 @@ extern "C" int tn_hashgrid_backward(int dfeat_dtype, uint32_t n_max, const uint3
 }
 
 // Workspace of tn_hashgrid_backward_binned for batches of up to n_max
-// samples; 0 when the layout exceeds the bin limits.
+// samples (for any TN_TUNE_HASH_BWD_LEVEL_BLOCKS); 0 when the layout exceeds
+// the bin limits.
 extern "C" size_t tn_hashgrid_backward_workspace_bytes(uint32_t n_max, tn_hashgrid_t grid) {
     HbPlan p;
-    if (!hb_layout_ok(grid) || n_max == 0 || !hb_plan(grid, n_max, p)) return 0;
+    if (!hb_layout_ok(grid) || n_max == 0 || !hb_plan(grid, n_max, kHbBlocksPerLevel, p)) return 0;
     return hb_workspace(p, nullptr, nullptr);
 }
 
@@ -914,7 +917,8 @@ extern "C" int tn_hashgrid_backward_binned(int dfeat_dtype, uint32_t n_max, cons
     TN_CHECK_ARG(dfeat_dtype == 0 || dfeat_dtype == 1);
     if (n_max == 0) return TN_OK;
     HbPlan p;
-    if (!workspace || !hb_plan(grid, n_max, p) || workspace_bytes < hb_workspace(p, nullptr, nullptr))
+    const uint32_t level_blocks = (uint32_t)tune_value(TN_TUNE_HASH_BWD_LEVEL_BLOCKS);
+    if (!workspace || !hb_plan(grid, n_max, level_blocks, p) || workspace_bytes < hb_workspace(p, nullptr, nullptr))
         return tn_hashgrid_backward(dfeat_dtype, n_max, info, stride, dfeat_stride, grid, pos, dfeat, dtable, stream);
     TN_CHECK_ARG((reinterpret_cast<uintptr_t>(workspace) & 255) == 0 && (reinterpret_cast<uintptr_t>(dtable) & 7) == 0);
     if (dfeat_dtype == 0)

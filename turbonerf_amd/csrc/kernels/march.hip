@@ -147,15 +147,14 @@ TN_DEVICE float after_run(uint32_t f, float t_lane, float cone, float dt_min, fl
 }
 
 constexpr int kMarchWaves = 4;  // waves per workgroup
-// The marching kernels are launched on at most this many workgroups (eight
-// waves per SIMD over 256 CUs) and stride over the live rays, one ray per wave
-// at a time: the ray count lives on the device, and a grid sized for the
-// largest batch would launch a wave per possible ray, nearly all of which exit
-// at once yet occupy the dispatcher beside the network kernels.
-constexpr uint32_t kMarchMaxBlocks = 2048;
-
+// The marching kernels are launched on at most TN_TUNE_MARCH_MAX_BLOCKS
+// workgroups (2048 by default: eight waves per SIMD over 256 CUs) and stride
+// over the live rays, one ray per wave at a time: the ray count lives on the
+// device, and a grid sized for the largest batch would launch a wave per
+// possible ray, nearly all of which exit at once yet occupy the dispatcher
+// beside the network kernels.
 inline unsigned march_blocks(uint32_t n_rays_max) {
-    return std::min(blocks_for(n_rays_max, kMarchWaves), (unsigned)kMarchMaxBlocks);
+    return std::min(blocks_for(n_rays_max, kMarchWaves), (unsigned)tune_value(TN_TUNE_MARCH_MAX_BLOCKS));
 }
 
 TN_DEVICE void march_count_ray(uint32_t i, uint32_t B, const tn_bbox_t& bbox, const MarchGrid& g, float cone,

@@ -771,7 +771,9 @@ inline unsigned mlp_blocks(uint32_t n_max, unsigned per_cu) {
     const unsigned cap = per_cu * compute_units();
     return b > cap ? cap : (b == 0 ? 1 : b);
 }
-constexpr unsigned kFwdBlocksPerCU = 2, kBwdBlocksPerCU = 1;
+// the backward pass's partial-sum workspace is sized for the largest
+// TN_TUNE_MLP_BWD_BLOCKS_PER_CU
+constexpr unsigned kBwdBlocksPerCUMax = 2;
 
 }  // namespace tn
 
@@ -790,7 +792,7 @@ extern "C" int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* inf
     TN_CHECK_ARG(n_max <= stride && n_max <= feat_stride && n_max <= out_stride);
     TN_CHECK_ARG(32ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * out_stride < (1ull << 32));
     if (n_max == 0) return TN_OK;
-    const unsigned blocks = mlp_blocks(n_max, kFwdBlocksPerCU);
+    const unsigned blocks = mlp_blocks(n_max, (unsigned)tune_value(TN_TUNE_MLP_FWD_BLOCKS_PER_CU));
     if (precision == 1) {
         using P = PrecF16;
         const auto* w = static_cast<const P::T*>(params) + net_offset;
@@ -811,7 +813,7 @@ extern "C" int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* inf
 }
 
 extern "C" size_t tn_mlp_backward_workspace_bytes(uint32_t n_max) {
-    return (size_t)mlp_blocks(n_max, kBwdBlocksPerCU) * kNetParams * sizeof(float);
+    return (size_t)mlp_blocks(n_max, kBwdBlocksPerCUMax) * kNetParams * sizeof(float);
 }
 
 // Backward of both MLPs for one batch. dL_drgb [3][grad_stride] (loss-scaled,
@@ -833,7 +835,7 @@ extern "C" int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* in
     TN_CHECK_ARG(32ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * grad_stride < (1ull << 32));
     if (workspace_bytes < tn_mlp_backward_workspace_bytes(n_max)) return TN_ERR_OUT_OF_MEMORY;
     if (n_max == 0) return TN_OK;
-    const unsigned blocks = mlp_blocks(n_max, kBwdBlocksPerCU);
+    const unsigned blocks = mlp_blocks(n_max, (unsigned)tune_value(TN_TUNE_MLP_BWD_BLOCKS_PER_CU));
     float* partial = static_cast<float*>(workspace);
     if (precision == 1) {
         using P = PrecF16;
