@@ -178,7 +178,8 @@ def run_precision(tn, synthetic, scene, precision, args):
     tune = tr.autotune(rounds_per_trial=16, repetitions=3)
     step = nerf.training_step
     log(f"[bench {name}] autotuned launch shapes {tune['picks']}: {tune['ns_per_sample_before']:.3f} -> "
-        f"{tune['ns_per_sample_after']:.3f} ns/sample")
+        f"{tune['ns_per_sample_after']:.3f} ns/sample; trials (value, ns/sample): "
+        + "; ".join(f"{k} " + " ".join(f"{v}:{t:.3f}" for v, t in rows) for k, rows in tune["trials"].items()))
     m = tr.train_step()
     step += 1
     log(f"[bench {name}] warm-up done: loss {m.loss:.4f} rays {m.n_rays} samples {m.n_samples}")

@@ -195,3 +195,33 @@ def test_binned_is_reproducible(cuda):
         ops.hashgrid_backward(g, n, pos, n, df, got, binned=True)
         runs.append(got.cpu().numpy())
     assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("groups,level_blocks", [(2, 128), (3, 40), (16, 8), (1, 256)])
+def test_binned_level_groups_and_blocks_give_the_same_bits(cuda, groups, level_blocks):
+    """Scattering and summing the levels in groups, and fewer count/scatter
+    blocks per level, give the default launch's gradient bit for bit (rays
+    with merged runs, plus random points whose level-0 bin is split)."""
+    import torch
+
+    from turbonerf_amd import _capi, ops
+
+    lib = _capi.lib()
+    rng = np.random.default_rng(17)
+    g = ops.hashgrid_layout()
+    pos = np.concatenate([_rays(rng, 512, 64, 0.004), rng.uniform(0, 1, size=(3, 32768)).astype(np.float32)], axis=1)
+    n = pos.shape[1]
+    pos_d = torch.from_numpy(np.ascontiguousarray(pos)).cuda().view(-1)
+    df = torch.from_numpy(rng.normal(0.0, 1.0, size=(32, n)).astype(np.float32)).cuda().view(-1)
+    runs = []
+    try:
+        for knobs in ({6: 1, 1: 128}, {6: groups, 1: level_blocks}):
+            for k, v in knobs.items():
+                assert lib.tn_tune_set(k, v) == 0
+            got = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
+            ops.hashgrid_backward(g, n, pos_d, n, df, got, binned=True)
+            runs.append(got.cpu().numpy())
+    finally:
+        assert lib.tn_tune_set(6, 1) == 0 and lib.tn_tune_set(1, 128) == 0
+    assert np.abs(runs[0]).max() > 0
+    assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))

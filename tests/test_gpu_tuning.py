@@ -18,9 +18,9 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"hash_fwd_block": 256, "hash_bwd_level_blocks": 128, "mlp_fwd_blocks_per_cu": 2,
-            "mlp_bwd_blocks_per_cu": 1, "march_max_blocks": 2048, "adam_block": 256}
+            "mlp_bwd_blocks_per_cu": 1, "march_max_blocks": 2048, "adam_block": 256, "hash_bwd_level_groups": 1}
 OTHER = {"hash_fwd_block": 64, "hash_bwd_level_blocks": 16, "mlp_fwd_blocks_per_cu": 1, "march_max_blocks": 512,
-         "adam_block": 1024}
+         "adam_block": 1024, "hash_bwd_level_groups": 3}
 
 
 def _set(tn, knobs):
@@ -73,7 +73,7 @@ def test_launch_shapes_leave_results_unchanged(tuning, precision):
         assert np.float32(mr.loss) == np.float32(ma.loss)
         _same(ref[2], alt[2], f"eager round {r}")
     caps = alt[2]._graph_captures()
-    for r, knobs in enumerate(({"march_max_blocks": 1024}, {"adam_block": 128}, {"hash_fwd_block": 128})):
+    for r, knobs in enumerate(({"march_max_blocks": 1024}, {"adam_block": 128}, {"hash_bwd_level_groups": 16})):
         _set(tn, DEFAULTS)
         ref[2].train_steps(4, read_metrics=False, graph=True)
         _set(tn, {**OTHER, **knobs})
@@ -114,9 +114,10 @@ def test_autotune_picks_from_the_candidates_and_training_goes_on(tuning):
     res = single[2].autotune(rounds_per_trial=4, repetitions=1)
     print(f"[tuning] single: {res['picks']} {res['ns_per_sample_before']:.3f} -> "
           f"{res['ns_per_sample_after']:.3f} ns/sample", flush=True)
-    cands = {"hash_fwd_block": [64, 128, 256], "hash_bwd_level_blocks": [16, 32, 64, 128],
+    cands = {"hash_fwd_block": [64, 128, 256], "hash_bwd_level_blocks": [64, 128, 256],
              "mlp_fwd_blocks_per_cu": [1, 2, 3, 4], "mlp_bwd_blocks_per_cu": [1, 2],
-             "march_max_blocks": [512, 1024, 2048, 4096], "adam_block": [128, 256, 512, 1024]}
+             "march_max_blocks": [512, 1024, 2048, 4096], "adam_block": [128, 256, 512, 1024],
+             "hash_bwd_level_groups": [1, 2, 4, 8]}
     assert set(res["picks"]) == set(cands) and tn.get_tuning() == res["picks"]
     for k, c in cands.items():
         assert res["picks"][k] in c and [v for v, _ in res["trials"][k]] == c

@@ -12,9 +12,10 @@ namespace turbo {
 namespace {
 
 // order of the coordinate search: the knobs of the longest kernels first
-constexpr int kOrder[] = {TN_TUNE_MLP_BWD_BLOCKS_PER_CU, TN_TUNE_MLP_FWD_BLOCKS_PER_CU,
-                          TN_TUNE_HASH_BWD_LEVEL_BLOCKS, TN_TUNE_HASH_FWD_BLOCK,
-                          TN_TUNE_MARCH_MAX_BLOCKS,      TN_TUNE_ADAM_BLOCK};
+constexpr int kOrder[] = {TN_TUNE_HASH_BWD_LEVEL_GROUPS, TN_TUNE_HASH_BWD_LEVEL_BLOCKS,
+                          TN_TUNE_MLP_BWD_BLOCKS_PER_CU, TN_TUNE_MLP_FWD_BLOCKS_PER_CU,
+                          TN_TUNE_HASH_FWD_BLOCK,        TN_TUNE_MARCH_MAX_BLOCKS,
+                          TN_TUNE_ADAM_BLOCK};
 
 int get(int knob) {
     int v = 0;
@@ -32,6 +33,7 @@ const char* Autotuner::knob_name(int knob) {
     case TN_TUNE_MLP_BWD_BLOCKS_PER_CU: return "mlp_bwd_blocks_per_cu";
     case TN_TUNE_MARCH_MAX_BLOCKS: return "march_max_blocks";
     case TN_TUNE_ADAM_BLOCK: return "adam_block";
+    case TN_TUNE_HASH_BWD_LEVEL_GROUPS: return "hash_bwd_level_groups";
     default: throw std::runtime_error("autotune: unknown knob");
     }
 }
@@ -39,11 +41,12 @@ const char* Autotuner::knob_name(int knob) {
 std::vector<int> Autotuner::candidates(int knob) {
     switch (knob) {
     case TN_TUNE_HASH_FWD_BLOCK: return {64, 128, 256};
-    case TN_TUNE_HASH_BWD_LEVEL_BLOCKS: return {16, 32, 64, 128};
+    case TN_TUNE_HASH_BWD_LEVEL_BLOCKS: return {64, 128, 256};
     case TN_TUNE_MLP_FWD_BLOCKS_PER_CU: return {1, 2, 3, 4};
     case TN_TUNE_MLP_BWD_BLOCKS_PER_CU: return {1, 2};
     case TN_TUNE_MARCH_MAX_BLOCKS: return {512, 1024, 2048, 4096};
     case TN_TUNE_ADAM_BLOCK: return {128, 256, 512, 1024};
+    case TN_TUNE_HASH_BWD_LEVEL_GROUPS: return {1, 2, 4, 8};
     default: throw std::runtime_error("autotune: unknown knob");
     }
 }

@@ -189,7 +189,7 @@ constexpr uint32_t kHbBinEntries = 1u << kHbBinLog;
 constexpr uint32_t kHbMaxLevelBins = 512;   // bins per level (level size <= 2^21)
 constexpr uint32_t kHbMaxBins = 8192;       // bins in all (16 levels of 2^21 entries)
 constexpr uint32_t kHbChunk = 32768;        // records per accumulating block at most
-constexpr uint32_t kHbBlocksPerLevel = 128; // count/scatter blocks per level at most
+constexpr uint32_t kHbBlocksPerLevel = 256; // count/scatter blocks per level at most
 
 struct HbPlan {
     uint32_t base[TN_HASHGRID_MAX_LEVELS];   // first bin of level l
@@ -386,11 +386,12 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_count_kernel(uint32_t n_max,
 }
 
 // (2a) one block per (level, slice of 64 bins): bin totals and each block's
-// record offset within its bins (blocks in order). Thread (q, c) sums rows
-// [8q, 8q + 8) of column c (a bin) of the level's count matrix, the 16 row
-// groups of a column are scanned through LDS, and each thread writes the
+// record offset within its bins (blocks in order). Thread (q, c) sums its
+// rows (at most 16) of column c (a bin) of the level's count matrix, the 16
+// row groups of a column are scanned through LDS, and each thread writes the
 // offsets of its rows.
-static_assert(kHbBlocksPerLevel <= 16 * 8, "eight rows per thread");
+constexpr uint32_t kHbReduceRows = 16;
+static_assert(kHbBlocksPerLevel <= 16 * kHbReduceRows, "rows per thread");
 __global__ void __launch_bounds__(1024) hashgrid_bwd_reduce_kernel(tn_hashgrid_t g, HbPlan p,
                                                                    const uint32_t* __restrict__ counts,
                                                                    uint32_t* __restrict__ offsets,
@@ -401,9 +402,9 @@ __global__ void __launch_bounds__(1024) hashgrid_bwd_reduce_kernel(tn_hashgrid_t
     const uint32_t c = threadIdx.x & 63u, q = threadIdx.x >> 6;
     const uint32_t rows = (p.level_blocks + 15) / 16, j0 = q * rows, j1 = min(p.level_blocks, j0 + rows);
     const bool col = k0 + c < nb;
-    uint32_t v[8], sum = 0;
+    uint32_t v[kHbReduceRows], sum = 0;
 #pragma unroll
-    for (uint32_t r = 0; r < 8; ++r) {
+    for (uint32_t r = 0; r < kHbReduceRows; ++r) {
         const uint32_t j = j0 + r;
         v[r] = col && j < j1 ? counts[(uint64_t)(l + L * j) * p.row + k0 + c] : 0u;
         sum += v[r];
@@ -414,7 +415,7 @@ __global__ void __launch_bounds__(1024) hashgrid_bwd_reduce_kernel(tn_hashgrid_t
     for (uint32_t t = 0; t < q; ++t) run += part[t][c];
     if (!col) return;
 #pragma unroll
-    for (uint32_t r = 0; r < 8; ++r) {
+    for (uint32_t r = 0; r < kHbReduceRows; ++r) {
         const uint32_t j = j0 + r;
         if (j < j1) offsets[(uint64_t)(l + L * j) * p.row + k0 + c] = run;
         run += v[r];
@@ -504,19 +505,23 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_scatter_kernel(uint32_t n_ma
                                                                    const uint32_t* __restrict__ offsets,
                                                                    const uint32_t* __restrict__ start,
                                                                    uint32_t* __restrict__ binmax,
-                                                                   HbRecord* __restrict__ rec) {
+                                                                   HbRecord* __restrict__ rec, uint32_t l0,
+                                                                   uint32_t Lg) {
     __shared__ uint32_t cur[kHbMaxLevelBins], bmax[kHbMaxLevelBins];
     const uint32_t L = (uint32_t)g.n_levels;
-    const int l = (int)(blockIdx.x % L);
+    // this launch covers levels [l0, l0 + Lg); block (l, j) is block l + L j
+    // of the count pass
+    const int l = (int)(l0 + blockIdx.x % Lg);
+    const uint32_t j = blockIdx.x / Lg, b = (uint32_t)l + L * j;
     const uint32_t nb = p.count[l];
     for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
-        cur[k] = start[p.base[l] + k] + offsets[(uint64_t)blockIdx.x * p.row + k];
+        cur[k] = start[p.base[l] + k] + offsets[(uint64_t)b * p.row + k];
         bmax[k] = 0u;
     }
     __syncthreads();
     const uint32_t n = info ? min(info[1], n_max) : n_max;
-    const uint32_t chunks = (n + 255) / 256, step = p.blocks / L;
-    uint32_t c = blockIdx.x / L;
+    const uint32_t chunks = (n + 255) / 256, step = p.level_blocks;
+    uint32_t c = j;
     HbIn next = hb_load<TG>(l, c * 256 + threadIdx.x, n, B, GB, pos, dfeat);
     for (; c < chunks; c += step) {
         const uint32_t i = c * 256 + threadIdx.x;
@@ -541,7 +546,6 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_scatter_kernel(uint32_t n_ma
                 rec[slot[k]] = hb_pack((hc.idx[k] - g.level_offset[l]) & (kHbBinEntries - 1u), r0[k], r1[k]);
     }
     __syncthreads();
-    const uint32_t j = blockIdx.x / L;
     for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x)
         binmax[(uint64_t)(p.base[l] + k) * p.level_blocks + j] = bmax[k];
 }
@@ -601,12 +605,14 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgri
                                                                       const uint32_t* __restrict__ slot_base,
                                                                       const HbRecord* __restrict__ rec,
                                                                       unsigned long long* __restrict__ slots,
-                                                                      float* __restrict__ dtable) {
+                                                                      float* __restrict__ dtable, uint32_t k_lo,
+                                                                      uint32_t k_hi) {
     __shared__ unsigned long long acc[2 * kHbBinEntries];
     __shared__ uint32_t smax;
-    const uint32_t j = blockIdx.x;
-    if (j >= chunk_start[p.n_bins]) return;
-    uint32_t lo = 0, hi = p.n_bins;  // the bin k with chunk_start[k] <= j < chunk_start[k + 1]
+    // this launch covers the chunks of bins [k_lo, k_hi)
+    const uint32_t j = chunk_start[k_lo] + blockIdx.x;
+    if (j >= chunk_start[k_hi]) return;
+    uint32_t lo = k_lo, hi = k_hi;  // the bin k with chunk_start[k] <= j < chunk_start[k + 1]
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) / 2;
         if (chunk_start[mid] <= j) lo = mid;
@@ -799,10 +805,23 @@ void hashgrid_backward_binned(uint32_t n_max, const uint32_t* info, uint32_t B, 
                                                                                      ws.totals);
     hashgrid_bwd_scan_kernel<<<1, 1024, 0, stream>>>(p, ws.totals, ws.start, ws.chunk_start, ws.slot_base,
                                                      ws.split_list);
-    hashgrid_bwd_scatter_kernel<TG><<<p.blocks, 256, 0, stream>>>(n_max, info, B, GB, g, p, pos, dfeat, ws.offsets,
-                                                                  ws.start, ws.binmax, ws.rec);
-    hashgrid_bwd_accumulate_kernel<<<p.acc_blocks, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax,
-                                                                     ws.slot_base, ws.rec, ws.slots, dtable);
+    // scatter and accumulate by groups of levels, so a group's records are
+    // read back while they are still in the device's last-level cache
+    const uint32_t L = (uint32_t)g.n_levels;
+    const uint32_t G = std::min<uint32_t>(L, (uint32_t)tune_value(TN_TUNE_HASH_BWD_LEVEL_GROUPS));
+    const uint32_t per = (L + G - 1) / G;
+    for (uint32_t l0 = 0; l0 < L; l0 += per) {
+        const uint32_t Lg = std::min(per, L - l0);
+        const uint32_t k_lo = p.base[l0], k_hi = l0 + Lg < L ? p.base[l0 + Lg] : p.n_bins;
+        // chunks of the group's bins: at most one per bin plus one per kHbChunk records
+        const uint64_t cap_g = (uint64_t)n_max * Lg * 8;
+        const uint32_t acc_blocks = (uint32_t)((k_hi - k_lo) + (cap_g + kHbChunk - 1) / kHbChunk);
+        hashgrid_bwd_scatter_kernel<TG><<<p.level_blocks * Lg, 256, 0, stream>>>(
+            n_max, info, B, GB, g, p, pos, dfeat, ws.offsets, ws.start, ws.binmax, ws.rec, l0, Lg);
+        hashgrid_bwd_accumulate_kernel<<<acc_blocks, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax,
+                                                                       ws.slot_base, ws.rec, ws.slots, dtable, k_lo,
+                                                                       k_hi);
+    }
     hashgrid_bwd_finish_kernel<<<kHbFinishBlocks, kHbFinishThreads, 0, stream>>>(
         g, p, ws.start, ws.chunk_start, ws.binmax, ws.slot_base, ws.split_list, ws.slots, dtable);
 }

@@ -2,7 +2,7 @@
 // on the host when they enqueue work, set by tn_tune_set (an autotuner, or a
 // caller that knows its workload). Ranges are those every launcher accepts:
 // the hash-forward kernel is bounded to 256 threads per workgroup, the binned
-// backward's row reduction to 128 count/scatter workgroups per level, and the
+// backward's row reduction to 256 count/scatter workgroups per level, and the
 // MLP backward's partial-sum workspace is sized for 2 workgroups per CU.
 #include <atomic>
 
@@ -12,17 +12,18 @@ namespace tn {
 
 namespace {
 
-std::atomic<int> g_tune[TN_TUNE_COUNT] = {256, 128, 2, 1, 2048, 256};
+std::atomic<int> g_tune[TN_TUNE_COUNT] = {256, 128, 2, 1, 2048, 256, 1};
 std::atomic<uint32_t> g_generation{0};
 
 bool tune_valid(int knob, int v) {
     switch (knob) {
     case TN_TUNE_HASH_FWD_BLOCK: return v == 64 || v == 128 || v == 256;
-    case TN_TUNE_HASH_BWD_LEVEL_BLOCKS: return v >= 8 && v <= 128;
+    case TN_TUNE_HASH_BWD_LEVEL_BLOCKS: return v >= 8 && v <= 256;
     case TN_TUNE_MLP_FWD_BLOCKS_PER_CU: return v >= 1 && v <= 4;
     case TN_TUNE_MLP_BWD_BLOCKS_PER_CU: return v >= 1 && v <= 2;
     case TN_TUNE_MARCH_MAX_BLOCKS: return v >= 256 && v <= 8192;
     case TN_TUNE_ADAM_BLOCK: return v >= 64 && v <= 1024 && v % 64 == 0;
+    case TN_TUNE_HASH_BWD_LEVEL_GROUPS: return v >= 1 && v <= 16;
     default: return false;
     }
 }
