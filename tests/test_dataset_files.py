@@ -1,5 +1,7 @@
 """File-backed datasets (host code, no GPU): the PNG decoder against PIL's
-decoder and against scanlines filtered here with each PNG filter type, and
+decoder and against scanlines filtered here with each PNG filter type, the
+JPEG decoder against PIL's (libjpeg) within a few levels and against itself
+across progressive and restart-interval codings of the same image, and
 Dataset.load_transforms against the transforms.json rules of the reference
 loader (src/models/dataset.cu:24-188) stated independently: the NeRF-to-this-
 frame change of basis as a matrix product P M F, focal lengths from the field
@@ -164,7 +166,6 @@ def test_png_every_filter_and_bit_depth(tmp_path):
 def test_png_errors(tmp_path):
     rng = np.random.default_rng(7)
     w, h = 8, 4
-    PIL.fromarray(rng.integers(0, 256, (h, w, 3), dtype=np.uint8), "RGB").save(tmp_path / "x.jpg", format="JPEG")
     good = _png_bytes(w, h, 6, 8, [bytes(w * 4)] * h, [0])
     (tmp_path / "i.png").write_bytes(_png_bytes(w, h, 6, 8, [bytes(w * 4)] * h, [0], interlace=1))
     bad_crc = bytearray(good)
@@ -172,8 +173,104 @@ def test_png_errors(tmp_path):
     (tmp_path / "crc.png").write_bytes(bytes(bad_crc))
     (tmp_path / "trunc.png").write_bytes(good[:-20])
     PIL.fromarray(np.zeros((h + 1, w, 4), np.uint8), "RGBA").save(tmp_path / "size.png")
-    for fn, msg in (("x.jpg", "JPEG"), ("i.png", "interlaced"), ("crc.png", "CRC"), ("trunc.png", "truncated"),
-                    ("size.png", "declares")):
+    for fn, msg in (("i.png", "interlaced"), ("crc.png", "CRC"), ("trunc.png", "truncated"), ("size.png", "declares")):
+        with pytest.raises(RuntimeError, match=msg):
+            _decode(tmp_path, [fn], w, h)
+
+
+# ------------------------------------------------------------------ JPEG
+def _photo(rng, h, w, c):
+    """Smooth content with an edge and sensor-like noise (what JPEG is for)."""
+    y, x = np.mgrid[0:h, 0:w]
+    img = np.zeros((h, w, c))
+    for k in range(c):
+        img[..., k] = 128 + 80 * np.sin(x / (5 + 3 * k) + y / 11.0) + 30 * np.cos(y / (7.0 + k))
+    img[h // 3:, w // 2:, :] += 40
+    img += rng.normal(0, 6, img.shape)
+    return np.clip(img, 0, 255).astype(np.uint8)
+
+
+def _jpeg_cases(rng, w, h):
+    rgb = _photo(rng, h, w, 3)
+    grey = _photo(rng, h, w, 1)[..., 0]
+    cases = {}
+    for sub, tag in ((0, "444"), (1, "422"), (2, "420")):
+        for q in (75, 95):
+            cases[f"{tag}_q{q}"] = (PIL.fromarray(rgb), dict(quality=q, subsampling=sub))
+    cases["grey"] = (PIL.fromarray(grey, "L"), dict(quality=90))
+    return cases
+
+
+@pytest.mark.parametrize("w,h", [(64, 48), (37, 23), (129, 67)])
+def test_jpeg_decoder_matches_pil(tmp_path, w, h):
+    """Baseline JPEGs written by PIL in 4:4:4, 4:2:2 and 4:2:0 at two
+    qualities and in grey, sizes that are not whole MCUs: this decoder's RGBA
+    against PIL's (libjpeg's integer IDCT, fixed-point colour conversion and
+    triangle upsampling) within 3 levels per channel, 0.2 on average; grey
+    within 1 level (the IDCT alone)."""
+    rng = np.random.default_rng(w * h)
+    for name, (im, kw) in _jpeg_cases(rng, w, h).items():
+        fn = f"{name}.jpg"
+        im.save(tmp_path / fn, format="JPEG", **kw)
+        ref = np.array(PIL.open(tmp_path / fn).convert("RGBA")).astype(int)
+        got = _decode(tmp_path, [fn], w, h).reshape(h, w, 4).astype(int)
+        d = np.abs(got - ref)
+        assert (got[..., 3] == 255).all()
+        bar = 1 if name == "grey" else 3
+        assert d.max() <= bar and d.mean() < 0.2, (name, d.max(), d.mean())
+
+
+def test_jpeg_progressive_and_restart_decode_to_the_baseline_pixels(tmp_path):
+    """Progressive files (spectral selection and successive approximation)
+    and files with restart intervals hold the baseline file's quantised
+    coefficients, so they decode to the same pixels bit for bit."""
+    rng = np.random.default_rng(5)
+    w, h = 77, 45
+    im = PIL.fromarray(_photo(rng, h, w, 3))
+    variants = {"base": dict(), "prog": dict(progressive=True), "rst": dict(restart_marker_blocks=3),
+                "prog_rst": dict(progressive=True, restart_marker_rows=1)}
+    out = {}
+    for sub in (0, 2):
+        for name, kw in variants.items():
+            fn = f"{name}_{sub}.jpg"
+            im.save(tmp_path / fn, format="JPEG", quality=90, subsampling=sub, **kw)
+            out[name] = _decode(tmp_path, [fn], w, h)
+        assert (tmp_path / f"prog_{sub}.jpg").read_bytes()[:4] == b"\xff\xd8\xff\xe0"
+        for name in ("prog", "rst", "prog_rst"):
+            assert np.array_equal(out[name], out["base"]), (sub, name)
+
+
+def test_jpeg_size_from_the_frame_header(tmp_path):
+    """transforms.json without w/h takes a JPEG's size from its frame
+    header (after an EXIF-sized APP1 segment)."""
+    rng = np.random.default_rng(6)
+    im = PIL.fromarray(_photo(rng, 31, 53, 3))
+    im.save(tmp_path / "a.jpg", format="JPEG", quality=90, exif=b"Exif\x00\x00" + bytes(3000))
+    p = _write_transforms(tmp_path / "transforms.json",
+                          {"camera_angle_x": 1.0, "frames": [{"file_path": "a.jpg", "transform_matrix": np.eye(4).tolist()}]})
+    ds = tn().Dataset(p)
+    ds.load_transforms()
+    assert tuple(ds.image_dimensions) == (53, 31)
+    ds.load_images()
+    assert np.array(ds.image_data).size == 53 * 31 * 4
+
+
+def test_jpeg_errors(tmp_path):
+    rng = np.random.default_rng(8)
+    w, h = 16, 16
+    PIL.fromarray(_photo(rng, h, w, 3)).save(tmp_path / "ok.jpg", format="JPEG", quality=90)
+    good = (tmp_path / "ok.jpg").read_bytes()
+    sof = good.index(b"\xff\xc0")
+    arith = bytearray(good)
+    arith[sof + 1] = 0xC9  # arithmetic-coded frame
+    (tmp_path / "arith.jpg").write_bytes(bytes(arith))
+    (tmp_path / "trunc.jpg").write_bytes(good[:sof + 6])
+    twelve = bytearray(good)
+    twelve[sof + 4] = 12  # sample precision
+    (tmp_path / "p12.jpg").write_bytes(bytes(twelve))
+    PIL.fromarray(_photo(rng, h, w, 4), "CMYK").save(tmp_path / "cmyk.jpg", format="JPEG")
+    for fn, msg in (("arith.jpg", "arithmetic"), ("trunc.jpg", "truncated"), ("p12.jpg", "12-bit"),
+                    ("cmyk.jpg", "4 components")):
         with pytest.raises(RuntimeError, match=msg):
             _decode(tmp_path, [fn], w, h)
 

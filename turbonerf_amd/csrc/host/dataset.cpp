@@ -83,9 +83,9 @@ void Dataset::load_transforms() {
         int2v dims{frame.value("w", global_dims.x), frame.value("h", global_dims.y)};
         if (dims.x <= 0 || dims.y <= 0) {
             // no size in the file: take it from the image's header
-            if (!read_png_size(image_path.string(), dims.x, dims.y))
+            if (!read_image_size(image_path.string(), dims.x, dims.y))
                 throw std::runtime_error("transforms: no image size given and " + image_path.string() +
-                                         " is not a PNG");
+                                         " is not a PNG or JPEG");
         }
         const float near = scene_scale * frame.value("near", global_near);
         const float far = scene_scale * frame.value("far", global_far);
@@ -199,7 +199,7 @@ void Dataset::load_images() {
         pool.emplace_back([&] {
             for (size_t i; (i = next.fetch_add(1)) < n;) {
                 try {
-                    const Rgba8Image img = load_png_rgba8(image_paths[i]);
+                    const Rgba8Image img = load_image_rgba8(image_paths[i]);
                     if (img.width != image_dimensions.x || img.height != image_dimensions.y)
                         throw std::runtime_error(image_paths[i] + " is " + std::to_string(img.width) + "x" +
                                                  std::to_string(img.height) + ", the dataset declares " +

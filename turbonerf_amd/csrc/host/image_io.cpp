@@ -70,7 +70,7 @@ bool read_png_size(const std::string& path, int& width, int& height) {
 
 Rgba8Image decode_png_rgba8(const uint8_t* f, size_t n, const std::string& name) {
     auto fail = [&](const std::string& msg) -> void { throw std::runtime_error("PNG " + name + ": " + msg); };
-    if (n >= 2 && f[0] == 0xFF && f[1] == 0xD8) fail("JPEG images are not supported; convert the dataset to PNG");
+    if (n >= 2 && f[0] == 0xFF && f[1] == 0xD8) fail("this is a JPEG file (decode it with decode_jpeg_rgba8)");
     if (n < 8 || std::memcmp(f, kSignature, 8) != 0) fail("not a PNG file");
 
     uint32_t w = 0, h = 0;
@@ -268,6 +268,23 @@ void write_png_rgba8(const std::string& path, int width, int height, const uint8
 Rgba8Image load_png_rgba8(const std::string& path) {
     const std::vector<uint8_t> bytes = read_file(path);
     return decode_png_rgba8(bytes.data(), bytes.size(), path);
+}
+
+Rgba8Image load_image_rgba8(const std::string& path) {
+    const std::vector<uint8_t> bytes = read_file(path);
+    if (bytes.size() >= 2 && bytes[0] == 0xFF && bytes[1] == 0xD8)
+        return decode_jpeg_rgba8(bytes.data(), bytes.size(), "JPEG " + path);
+    return decode_png_rgba8(bytes.data(), bytes.size(), path);
+}
+
+bool read_image_size(const std::string& path, int& width, int& height) {
+    if (read_png_size(path, width, height)) return true;
+    // a JPEG's frame header may follow large APPn segments: read the file
+    std::ifstream in(path, std::ios::binary);
+    uint8_t head[2];
+    if (!in || !in.read(reinterpret_cast<char*>(head), 2) || head[0] != 0xFF || head[1] != 0xD8) return false;
+    const std::vector<uint8_t> bytes = read_file(path);
+    return read_jpeg_size(bytes.data(), bytes.size(), width, height);
 }
 
 }  // namespace turbo
