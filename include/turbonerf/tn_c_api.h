@@ -260,7 +260,7 @@ int tn_hashgrid_backward_binned(int dfeat_dtype, uint32_t n_max, const uint32_t*
 int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* info, uint32_t stride,
     uint32_t feat_stride, uint32_t out_stride, const void* params, uint32_t net_offset,
     const void* features, const float* dir, int color, void* density_out, void* rgb_out,
-    tn_stream_t stream);
+    const tn_appearance_t* appearance, tn_stream_t stream);
 
 /* Backward of both MLPs with the reference's gradient plumbing (colour-MLP
  * backward, density row 0 += loss_scale*(dL_ds + dL_ds_extra), t^2 scaling for
@@ -274,10 +274,31 @@ int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* info, uint32_
     uint32_t feat_stride, const void* params, uint32_t net_offset, const void* features,
     const float* dir, const float* sample_t, const void* dL_drgb, uint32_t grad_stride,
     const float* dL_ds, const float* dL_ds_extra, float loss_scale, int scale_by_rays, void* dfeat,
-    float* grad_net, int accumulate, void* workspace, size_t workspace_bytes, tn_stream_t stream);
+    float* grad_net, int accumulate, void* workspace, size_t workspace_bytes,
+    const tn_appearance_t* appearance, tn_stream_t stream);
 
 /* Dynamic LDS bytes the MLP kernels launch with (for occupancy planning). */
 size_t tn_mlp_lds_bytes(int precision, int backward);
+
+/* Appearance embedding (see tn_appearance_t; `appearance` of the two calls
+ * above may be null: no embedding). tn_appearance_bias forms the per-image
+ * bias rows W3e emb[k] (fp32; W3e [64][16], emb [n][16]). Replaces the
+ * reference's embedding forward (src/core/embedding.cuh:45-66) and its
+ * concatenation into the colour input (src/core/nerf-network.cu:494-520). */
+int tn_appearance_bias(uint32_t n_appearances, const float* w3e, const float* emb, float* bias,
+    tn_stream_t stream);
+
+/* Gradients of W3e and of the embedding from the colour layer's dz written
+ * by tn_mlp_backward (both stored, fp32, scaled as dz is). image_id must be
+ * non-decreasing over the batch's samples (a training batch's are). Replaces
+ * the reference's embedding backward (src/core/embedding.cuh:68-99 and
+ * nerf-network.cu:698-715), whose kernel indexes the output gradient with
+ * the embedding width as its row stride and writes into the parameters. */
+size_t tn_appearance_workspace_bytes(uint32_t n_appearances);
+int tn_appearance_backward(uint32_t n_max, const uint32_t* info, uint32_t stride,
+    const uint32_t* image_id, const float* dz, uint32_t n_appearances, const float* w3e,
+    const float* emb, float* grad_w3e, float* grad_emb, void* workspace, size_t workspace_bytes,
+    tn_stream_t stream);
 
 /* ------------------------------------------------------------------ Adam */
 

@@ -108,6 +108,24 @@ typedef struct tn_hashgrid {
 #define TN_DENSITY_MLP_PARAMS (64 * 32 + 16 * 64)
 #define TN_COLOR_MLP_PARAMS (64 * 32 + 64 * 64 + 3 * 64)
 
+/* Appearance embedding (reference src/core/embedding.cuh:45-99,
+ * src/core/nerf-network.cu:27-72): with it the colour MLP's first layer takes
+ * 16 more inputs, the embedding emb[k][16] of the sample's image k, through
+ * 16 more weight columns W3e[64][16]. Their contribution W3e emb[k] is the
+ * same for every sample of image k, so it enters the MLP kernels as a
+ * per-image bias of that layer (tn_appearance_bias), and the gradients of
+ * W3e and emb follow from the per-image sums of that layer's pre-activation
+ * gradient dz (tn_appearance_backward). */
+#define TN_APPEARANCE_DIMS 16
+#define TN_APPEARANCE_WEIGHTS (64 * TN_APPEARANCE_DIMS)
+typedef struct tn_appearance {
+    const float* bias;         /* [n_appearances][64] fp32 (16-byte aligned); null: no embedding */
+    const uint32_t* image_id;  /* per-sample image [stride]; null: every sample takes row 0 */
+    uint32_t n_appearances;    /* rows of bias; ids beyond it take row 0 */
+    float* dz;                 /* backward only (may be null): dL/dz of the colour MLP's first layer,
+                                  fp32 [64][stride], written for the batch's samples */
+} tn_appearance_t;
+
 #ifdef __cplusplus
 }
 #endif

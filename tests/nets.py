@@ -44,24 +44,33 @@ def _q(a, on):
     return np.asarray(a, np.float16).astype(np.asarray(a).dtype) if on else a
 
 
-def forward(net, feat, dir01, round16=False, acc=np.float64):
+def forward(net, feat, dir01, round16=False, acc=np.float64, app=None):
     """round16: the fp16 MFMA kernels' operand rounding (module docstring);
     acc: the accumulation type (np.float32 emulates the matrix cores' fp32
-    accumulators, np.float64 is exact to working precision)."""
+    accumulators, np.float64 is exact to working precision).
+    app: appearance embedding (w3e [64][16], emb [n_img][16], img [n]): the
+    colour MLP's first layer takes [s16 | sh16 | emb[img]] through
+    [W3 | w3e], as the reference concatenates them
+    (src/core/nerf-network.cu:494-520)."""
     W1, W2, W3, W4, W5 = (w.astype(acc) for w in split(net))
     feat = np.asarray(feat, acc)
     h1 = np.maximum(feat @ W1.T, 0)
     s16 = h1 @ W2.T
     xc = np.concatenate([s16, sh4(dir01).astype(acc)], axis=1)
-    h2 = np.maximum(xc @ W3.T, 0)
+    z3 = xc @ W3.T
+    if app is not None:
+        w3e, emb, img = app
+        z3 = z3 + np.asarray(emb, acc)[np.asarray(img)] @ np.asarray(w3e, acc).T
+    h2 = np.maximum(z3, 0)
     h3 = np.maximum(_q(h2, round16) @ W4.T, 0)
     o = _q(h3, round16) @ W5.T
     rgb = 1.0 / (1.0 + np.exp(-o))
-    return dict(h1=h1, s16=s16, xc=xc, h2=h2, h3=h3, o=o, rgb=rgb, feat=feat, round16=round16, acc=acc)
+    return dict(h1=h1, s16=s16, xc=xc, h2=h2, h3=h3, o=o, rgb=rgb, feat=feat, round16=round16, acc=acc, app=app)
 
 
 def backward(net, fw, G_rgb, dL_ds, t, loss_scale, extra=None):
-    """Returns (dfeat (n,32), grad_net (9408,))."""
+    """Returns (dfeat (n,32), grad_net (9408,)); with an appearance
+    embedding in `fw` also (grad_w3e [64][16], grad_emb [n_img][16])."""
     acc = fw.get("acc", np.float64)
     W1, W2, W3, W4, W5 = (w.astype(acc) for w in split(net))
     r = fw.get("round16", False)
@@ -71,7 +80,8 @@ def backward(net, fw, G_rgb, dL_ds, t, loss_scale, extra=None):
     dW5 = dO.T @ h3
     dH3 = _q((dO @ W5) * (fw["h3"] > 0), r)
     dW4 = dH3.T @ h2
-    dH2 = _q((dH3 @ W4) * (fw["h2"] > 0), r)
+    dz3 = (dH3 @ W4) * (fw["h2"] > 0)
+    dH2 = _q(dz3, r)
     dW3 = dH2.T @ _q(fw["xc"], r)
     dXc = dH2 @ W3
     dS = dXc[:, :16].copy()
@@ -86,7 +96,14 @@ def backward(net, fw, G_rgb, dL_ds, t, loss_scale, extra=None):
     dW1 = dH1.T @ _q(fw["feat"], r)
     dX = dH1 @ W1
     grad = np.concatenate([dW1.ravel(), dW2.ravel(), dW3.ravel(), dW4.ravel(), dW5.ravel()])
-    return dX, grad
+    if fw.get("app") is None:
+        return dX, grad
+    w3e, emb, img = fw["app"]
+    img = np.asarray(img)
+    g_w3e = dz3.T @ np.asarray(emb, acc)[img]
+    g_emb = np.zeros(np.shape(emb), acc)
+    np.add.at(g_emb, img, dz3 @ np.asarray(w3e, acc))
+    return dX, grad, g_w3e, g_emb
 
 
 def xavier(rng, n_out, n_in):

@@ -852,6 +852,45 @@ def test_numpy_mlp_backward_matches_autograd():
     np.testing.assert_allclose(dX, X.grad.numpy(), rtol=1e-10, atol=1e-12)
 
 
+def test_numpy_appearance_embedding_matches_autograd():
+    """The embedding's concatenated input (reference nerf-network.cu:494-520)
+    and its gradients in the numpy restatement against torch autograd."""
+    import torch
+
+    from nets import backward, forward, random_net, sh4
+
+    rng = np.random.default_rng(13)
+    n, n_img = 48, 5
+    net = random_net(4)
+    w3e = rng.uniform(-0.2, 0.2, size=(64, 16))
+    emb = rng.uniform(0.0, 1.0, size=(n_img, 16))
+    img = np.sort(rng.integers(0, n_img - 1, n))  # image 4 unused: its gradient is zero
+    feat = rng.normal(size=(n, 32))
+    d = rng.normal(size=(n, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    dir01 = 0.5 * d + 0.5
+    G = rng.normal(size=(n, 3))
+    dls = rng.normal(size=n)
+    t = rng.uniform(1.0, 2.0, n)
+    fw = forward(net, feat, dir01, app=(w3e, emb, img))
+    dX, gr, g_w3e, g_emb = backward(net, fw, G, dls, t, 3.0)
+    W = torch.tensor(net, dtype=torch.float64, requires_grad=True)
+    We = torch.tensor(w3e, requires_grad=True)
+    E = torch.tensor(emb, requires_grad=True)
+    X = torch.tensor(feat, requires_grad=True)
+    W1, W2, W3, W4, W5 = W[:2048].view(64, 32), W[2048:3072].view(16, 64), W[3072:5120].view(64, 32), \
+        W[5120:9216].view(64, 64), W[9216:].view(3, 64)
+    s16 = torch.relu(X @ W1.T) @ W2.T
+    xc = torch.cat([s16, torch.tensor(sh4(dir01)), E[torch.tensor(img)]], 1)
+    rgb = torch.sigmoid(torch.relu(torch.relu(xc @ torch.cat([W3, We], 1).T) @ W4.T) @ W5.T)
+    total = (rgb * torch.tensor(G)).sum() + 3.0 * (s16[:, 0] * torch.tensor(dls)).sum()
+    total.backward()
+    np.testing.assert_allclose(gr, W.grad.numpy(), rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(g_w3e, We.grad.numpy(), rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(g_emb, E.grad.numpy(), rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(dX, X.grad.numpy(), rtol=1e-10, atol=1e-12)
+    assert not g_emb[4].any() and np.abs(g_emb[:4]).min() >= 0 and np.abs(g_emb).max() > 0
+
+
 def test_oracle_mlp_train_matches_numpy_reference():
     """The C MLP forward+backward (CPU-baseline training step) against the
     autograd-pinned numpy restatement: fp64 to 1e-12, fp32 to 1e-4."""

@@ -86,6 +86,15 @@ class HashGrid(C.Structure):
 # ----------------------------------------------------------------- loading
 _lib: C.CDLL | None = None
 
+class Appearance(C.Structure):
+    _fields_ = [
+        ("bias", C.c_void_p),
+        ("image_id", C.c_void_p),
+        ("n_appearances", C.c_uint32),
+        ("dz", C.c_void_p),
+    ]
+
+
 vp = C.c_void_p
 u32 = C.c_uint32
 u64 = C.c_uint64
@@ -129,10 +138,10 @@ _SIGS: dict[str, tuple] = {
     "tn_hashgrid_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp]),
     "tn_hashgrid_backward_workspace_bytes": (C.c_size_t, [u32, HashGrid]),
     "tn_hashgrid_backward_binned": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp, C.c_size_t, vp]),
-    "tn_mlp_forward": (C.c_int, [C.c_int, u32, vp, u32, u32, u32, vp, u32, vp, vp, C.c_int, vp, vp, vp]),
+    "tn_mlp_forward": (C.c_int, [C.c_int, u32, vp, u32, u32, u32, vp, u32, vp, vp, C.c_int, vp, vp, vp, vp]),
     "tn_mlp_backward_workspace_bytes": (sz, [u32]),
     "tn_mlp_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, vp, u32, vp, vp, vp, vp, u32, vp, vp, f32, C.c_int, vp,
-                                  vp, C.c_int, vp, sz, vp]),
+                                  vp, C.c_int, vp, sz, vp, vp]),
     "tn_mlp_lds_bytes": (sz, [C.c_int, C.c_int]),
     "tn_adam_step": (C.c_int, [u32, u32, u32, f32, vp, f32, vp, f32, f32, f32, f32, f32, vp, C.c_int, vp, vp, vp, vp,
                                vp, vp]),
@@ -141,6 +150,9 @@ _SIGS: dict[str, tuple] = {
     "tn_render_write_rgba": (C.c_int, [u32, vp, u32, vp, vp, vp, vp, vp]),
     "tn_distortion_scratch_bytes": (sz, [u32, u32]),
     "tn_distortion": (C.c_int, [C.c_int, vp, u32, u32, f32, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]),
+    "tn_appearance_bias": (C.c_int, [u32, vp, vp, vp, vp]),
+    "tn_appearance_workspace_bytes": (sz, [u32]),
+    "tn_appearance_backward": (C.c_int, [u32, vp, u32, vp, vp, u32, vp, vp, vp, vp, vp, sz, vp]),
     "tn_tune_set": (C.c_int, [C.c_int, C.c_int]),
     "tn_tune_get": (C.c_int, [C.c_int, vp]),
     "tn_tune_generation": (u32, []),

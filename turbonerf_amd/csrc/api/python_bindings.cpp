@@ -237,6 +237,8 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
         .def_readonly("can_render", &NeRFProxy::can_render)
         .def_readonly("training_step", &NeRFProxy::training_step)
         .def_readonly("id", &NeRFProxy::id)
+        .def_property_readonly("n_appearances",
+                               [](const NeRFProxy& p) { return p.nerfs.empty() ? 0u : p.nerfs[0].params.n_appearances; })
         .def(
             "set_analytic_field",
             [](NeRFProxy& p, py::array_t<float, py::array::c_style | py::array::forcecast> blobs, float aabb,
@@ -276,7 +278,16 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
         .def_readwrite("beta1", &NetworkConfig::beta1)
         .def_readwrite("beta2", &NetworkConfig::beta2)
         .def_readwrite("epsilon", &NetworkConfig::epsilon)
-        .def_readwrite("l2_reg", &NetworkConfig::l2_reg);
+        .def_readwrite("l2_reg", &NetworkConfig::l2_reg)
+        .def_property(
+            "appearance_embedding_dims", [](const NetworkConfig& c) { return c.appearance_dims; },
+            [](NetworkConfig& c, int v) {
+                if (v != 0 && v != TN_APPEARANCE_DIMS)
+                    throw std::invalid_argument("appearance_embedding_dims is 0 or " +
+                                                std::to_string(TN_APPEARANCE_DIMS));
+                c.appearance_dims = v;
+            },
+            "0: no appearance embedding; 16: one per training image (the reference's network)");
 
     py::class_<TrainingConfig>(m, "TrainingConfig")
         .def(py::init<>())

@@ -154,6 +154,9 @@ void NeRFTrainingController::setup_data(uint32_t batch_size) {
     ctx_->d_step.zero(s);
     ctx_->occ_rng_offset = 0;
     ctx_->destroy_graphs();
+    // one appearance embedding per training image when the network has them
+    // (reference nerf-training-controller.cu:55, nerf-proxy.cuh:106)
+    nerf.network.update_appearance_embedding_if_needed((uint32_t)ds.n_images());
     nerf.network.prepare_workspace(batch_size, R);
     nerf.network.update_params_if_needed(s, nerf.params);
     proxy->training_step = 0;

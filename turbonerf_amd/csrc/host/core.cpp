@@ -82,12 +82,13 @@ void NetworkWorkspace::enlarge(uint32_t B, uint32_t R, int prec) {
     composite_scratch.resize(tn_composite_scratch_bytes(ray_capacity, batch_size));
     distortion_scratch.resize(tn_distortion_scratch_bytes(ray_capacity, batch_size));
     mlp_ws.resize(std::max<size_t>(tn_mlp_backward_workspace_bytes(batch_size), 4));
+    if (app_dz.size()) app_dz.resize(64 * (size_t)batch_size);
 }
 
 size_t NetworkWorkspace::bytes() const {
     return features.bytes() + density.bytes() + rgb.bytes() + dL_drgb.bytes() + dfeat.bytes() + dL_ds.bytes() +
            dL_dist.bytes() + ray_loss.bytes() + ray_dist_loss.bytes() + composite_scratch.bytes() +
-           distortion_scratch.bytes() + mlp_ws.bytes() + hash_ws.bytes();
+           distortion_scratch.bytes() + mlp_ws.bytes() + hash_ws.bytes() + app_dz.bytes() + app_ws.bytes();
 }
 
 void NerfNetwork::update_aabb_scale_if_needed(int aabb) {
@@ -102,23 +103,38 @@ void NerfNetwork::update_aabb_scale_if_needed(int aabb) {
 
 void NerfNetwork::update_params_if_needed(hipStream_t s, NetworkParamsWorkspace& p) {
     const uint32_t n_net = TN_DENSITY_MLP_PARAMS + TN_COLOR_MLP_PARAMS;
-    const uint32_t total = grid.n_params + n_net;
+    const uint32_t n_app = n_appearances;
+    const uint32_t total = grid.n_params + n_net + (n_app ? TN_APPEARANCE_WEIGHTS + n_app * TN_APPEARANCE_DIMS : 0);
     const bool want_lp = config.precision == 1;
-    if (p.n_total == total && p.params_lp.size() == (want_lp ? (size_t)total * 2 : 0)) return;
+    if (p.n_total == total && p.n_appearances == n_app && p.params_lp.size() == (want_lp ? (size_t)total * 2 : 0))
+        return;
     p.n_grid = grid.n_params;
     p.n_total = total;
+    p.n_appearances = n_app;
     // seeded initialisation: hash entries uniform in +-hash_init_scale, MLP
-    // weights Xavier-uniform per matrix
+    // weights Xavier-uniform per matrix (the colour MLP's first layer over its
+    // 48 inputs with an embedding), embeddings uniform in [0, 1) as the
+    // reference initialises them (src/core/embedding.cuh:155-157)
     std::vector<float> h(total);
     std::mt19937_64 rng(config.seed);
     std::uniform_real_distribution<float> u(-1.0f, 1.0f);
     for (uint32_t i = 0; i < grid.n_params; ++i) h[i] = config.hash_init_scale * u(rng);
-    const int shapes[5][2] = {{64, 32}, {16, 64}, {64, 32}, {64, 64}, {3, 64}};
+    const int w3_in = n_app ? 32 + TN_APPEARANCE_DIMS : 32;
+    const int shapes[5][3] = {{64, 32, 32}, {16, 64, 64}, {64, 32, w3_in}, {64, 64, 64}, {3, 64, 64}};
     uint32_t off = grid.n_params;
     for (auto& sh : shapes) {
-        const float a = std::sqrt(6.0f / (float)(sh[0] + sh[1]));
+        const float a = std::sqrt(6.0f / (float)(sh[0] + sh[2]));
         for (int k = 0; k < sh[0] * sh[1]; ++k) h[off + k] = a * u(rng);
         off += sh[0] * sh[1];
+    }
+    if (n_app) {
+        const float a = std::sqrt(6.0f / (float)(64 + w3_in));
+        for (int k = 0; k < TN_APPEARANCE_WEIGHTS; ++k) h[off + k] = a * u(rng);
+        off += TN_APPEARANCE_WEIGHTS;
+        for (uint32_t k = 0; k < n_app * TN_APPEARANCE_DIMS; ++k) h[off + k] = 0.5f * (u(rng) + 1.0f);
+        app_bias.resize((size_t)n_app * 64);
+    } else {
+        app_bias.release();
     }
     p.params_fp.resize(total);
     TN_HIP(hipMemcpyAsync(p.params_fp.data(), h.data(), total * sizeof(float), hipMemcpyHostToDevice, s));
@@ -141,15 +157,27 @@ void NerfNetwork::update_params_if_needed(hipStream_t s, NetworkParamsWorkspace&
 
 void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step, uint32_t B, uint32_t R,
                         const uint32_t* info, const float* random_rgb, const uint32_t* ray_steps,
-                        const uint32_t* ray_offset, const float* t_batch, const float* pos_batch,
-                        const float* dir_batch, const float* dt_batch, const float* m_norm_batch,
-                        const float* dt_norm_batch, const float* target_rgba, const Settings& st) {
+                        const uint32_t* ray_offset, const uint32_t* appearance_id_batch, const float* t_batch,
+                        const float* pos_batch, const float* dir_batch, const float* dt_batch,
+                        const float* m_norm_batch, const float* dt_norm_batch, const float* target_rgba,
+                        const Settings& st) {
     const int prec = config.precision;
-    workspace.enlarge(B, R, prec);
+    prepare_workspace(B, R);
     NetworkWorkspace& w = workspace;
     const void* params = params_for_kernels(p);
     const tn_stream_t ts = as_tn(s);
     PhaseTimer* tm = st.timer;
+    // appearance embedding: this step's per-image bias rows of the colour MLP
+    const uint32_t n_app = p.n_appearances;
+    tn_appearance_t app_fwd{}, app_bwd{};
+    if (n_app) {
+        if (!appearance_id_batch) throw std::runtime_error("NerfNetwork::train: the embedding needs image ids");
+        ScopedPhase ph(tm, "appearance", s);
+        TN_TRY(tn_appearance_bias(n_app, p.params_fp.data() + p.w3e_offset(), p.params_fp.data() + p.emb_offset(),
+                                  app_bias.data(), ts));
+        app_fwd = tn_appearance_t{app_bias.data(), appearance_id_batch, n_app, nullptr};
+        app_bwd = tn_appearance_t{app_bias.data(), appearance_id_batch, n_app, w.app_dz.data()};
+    }
     // forward: encoding, density + colour MLPs (reference :431-545)
     {
         ScopedPhase ph(tm, "hashgrid_forward", s);
@@ -158,7 +186,7 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
     {
         ScopedPhase ph(tm, "mlp_forward", s);
         TN_TRY(tn_mlp_forward(prec, B, info, B, B, B, params, p.n_grid, w.features.data(), dir_batch, 1,
-                              w.density.data(), w.rgb.data(), ts));
+                              w.density.data(), w.rgb.data(), n_app ? &app_fwd : nullptr, ts));
     }
     // distortion loss (reference :273-283)
     if (st.use_distortion_loss) {
@@ -182,7 +210,14 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
         TN_TRY(tn_mlp_backward(prec, B, info, B, B, params, p.n_grid, w.features.data(), dir_batch, t_batch,
                                w.dL_drgb.data(), B, w.dL_ds.data(), st.use_distortion_loss ? w.dL_dist.data() : nullptr,
                                config.loss_scale(), 1, w.dfeat.data(), p.gradients.data() + p.n_grid, 0,
-                               w.mlp_ws.data(), w.mlp_ws.size(), ts));
+                               w.mlp_ws.data(), w.mlp_ws.size(), n_app ? &app_bwd : nullptr, ts));
+    }
+    if (n_app) {
+        ScopedPhase ph(tm, "appearance", s);
+        TN_TRY(tn_appearance_backward(B, info, B, appearance_id_batch, w.app_dz.data(), n_app,
+                                      p.params_fp.data() + p.w3e_offset(), p.params_fp.data() + p.emb_offset(),
+                                      p.gradients.data() + p.w3e_offset(), p.gradients.data() + p.emb_offset(),
+                                      w.app_ws.data(), w.app_ws.bytes(), ts));
     }
     {
         ScopedPhase ph(tm, "hashgrid_backward", s);
@@ -199,7 +234,7 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
                             : config.learning_rate / (1.0f + config.learning_rate_decay * (float)step);
     {
         ScopedPhase ph(tm, "adam", s);
-        TN_TRY(tn_adam_step(p.n_total, p.n_grid, p.n_total, config.loss_scale(), info, lr,
+        TN_TRY(tn_adam_step(p.n_total, p.n_grid, p.n_net_end(), config.loss_scale(), info, lr,
                             dev_lr ? st.step_counter : nullptr, config.learning_rate_decay, config.beta1,
                             config.beta2, config.epsilon, config.l2_reg, p.params_fp.data(), prec,
                             prec == 1 ? p.params_lp.data() : nullptr, p.gradients.data(), p.m1.data(), p.m2.data(),
@@ -217,6 +252,11 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
 void NerfNetwork::prepare_workspace(uint32_t B, uint32_t R) {
     workspace.enlarge(B, R, config.precision);
     workspace.hash_ws.reserve(tn_hashgrid_backward_workspace_bytes(B, grid));
+    if (n_appearances) {
+        if (workspace.app_dz.size() < 64 * (size_t)workspace.batch_size) workspace.app_dz.resize(64 * (size_t)workspace.batch_size);
+        const size_t ab = tn_appearance_workspace_bytes(n_appearances) / sizeof(float);
+        if (workspace.app_ws.size() < ab) workspace.app_ws.resize(ab);
+    }
 }
 
 void NerfNetwork::inference(hipStream_t s, const NetworkParamsWorkspace& p, uint32_t n, const uint32_t* info,
@@ -225,8 +265,14 @@ void NerfNetwork::inference(hipStream_t s, const NetworkParamsWorkspace& p, uint
     const int prec = config.precision;
     const void* params = params_for_kernels(p);
     TN_TRY(tn_hashgrid_forward(prec, 0, n, info, stride, stride, grid, pos_batch, params, features, as_tn(s)));
+    tn_appearance_t app{};
+    if (use_color_network && p.n_appearances) {
+        TN_TRY(tn_appearance_bias(1, p.params_fp.data() + p.w3e_offset(), p.params_fp.data() + p.emb_offset(),
+                                  app_bias.data(), as_tn(s)));
+        app = tn_appearance_t{app_bias.data(), nullptr, 1, nullptr};
+    }
     TN_TRY(tn_mlp_forward(prec, n, info, stride, stride, stride, params, p.n_grid, features, dir_batch,
-                          use_color_network ? 1 : 0, density_out, rgb_out, as_tn(s)));
+                          use_color_network ? 1 : 0, density_out, rgb_out, app.bias ? &app : nullptr, as_tn(s)));
 }
 
 // ------------------------------------------------------------------ workspaces
@@ -380,9 +426,9 @@ void Trainer::train_network(Context& ctx, int c, uint32_t training_step, bool gr
     st.step_for_lr = graph;
     ctx.nerf->network.train(ctx.stream, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity,
                             ws.info.data(), ws.random_float.data() + ws.ray_capacity, ws.ray_step.data(),
-                            ws.ray_offset.data(), ws.sample_t.data(), ws.sample_pos.data(), ws.sample_dir.data(),
-                            ws.sample_dt.data(), ws.sample_m_norm.data(), ws.sample_dt_norm.data(),
-                            ws.pix_rgba.data(), st);
+                            ws.ray_offset.data(), ws.sample_img_id.data(), ws.sample_t.data(), ws.sample_pos.data(),
+                            ws.sample_dir.data(), ws.sample_dt.data(), ws.sample_m_norm.data(),
+                            ws.sample_dt_norm.data(), ws.pix_rgba.data(), st);
 }
 
 // Eager step: batches[c ^ 1] is generated on the generation stream once its
@@ -431,7 +477,9 @@ std::vector<uint64_t> Trainer::graph_key(const Context& ctx) const {
         bits(c.learning_rate_decay), bits(c.beta1), bits(c.beta2), bits(c.epsilon), bits(c.l2_reg),
         (uint64_t)n.network.aabb_scale, n.network.grid.n_params, bits(ctx.dataset->bounding_box.size()),
         ptr(n.network.workspace.features.data()), ptr(n.network.workspace.hash_ws.data()),
-        ptr(n.network.workspace.mlp_ws.data()), ptr(ctx.samples_total.data()), tn_tune_generation()};
+        ptr(n.network.workspace.mlp_ws.data()), ptr(ctx.samples_total.data()), tn_tune_generation(),
+        n.params.n_appearances, ptr(n.network.app_bias.data()), ptr(n.network.workspace.app_dz.data()),
+        ptr(n.network.workspace.app_ws.data())};
     for (const auto& b : ctx.batches) k.push_back(ptr(b.random_float.data()));
     return k;
 }

@@ -110,15 +110,24 @@ struct NetworkConfig {
     float beta2 = 0.999f;
     float epsilon = 1e-15f;
     float l2_reg = 1e-6f;
+    // appearance embedding width: 0 (none; the benchmark network) or
+    // TN_APPEARANCE_DIMS per training image, as the reference always trains
+    // (src/core/nerf-network.cu:27-72, nerf-constants.cuh:31)
+    int appearance_dims = 0;
     float loss_scale() const { return precision == 1 ? 1024.0f : 1.0f; }
 };
 
-// Flat parameters (hash table, density MLP, colour MLP), fp32 master copy,
-// optional fp16 copy, fp32 gradients and Adam state
-// (reference src/workspaces/network-params-workspace.cuh,
-//  src/core/adam-optimizer.cuh).
+// Flat parameters (hash table, density MLP, colour MLP, then with an
+// appearance embedding the colour MLP's 16 embedding columns W3e [64][16] and
+// the embeddings [n_appearances][16]), fp32 master copy, optional fp16 copy,
+// fp32 gradients and Adam state (reference
+// src/workspaces/network-params-workspace.cuh, src/core/adam-optimizer.cuh).
 struct NetworkParamsWorkspace {
-    uint32_t n_grid = 0, n_total = 0;
+    uint32_t n_grid = 0, n_total = 0, n_appearances = 0;
+    uint32_t w3e_offset() const { return n_grid + TN_DENSITY_MLP_PARAMS + TN_COLOR_MLP_PARAMS; }
+    uint32_t emb_offset() const { return w3e_offset() + TN_APPEARANCE_WEIGHTS; }
+    // end of the network weights (L2-regularised by Adam); the embeddings follow
+    uint32_t n_net_end() const { return n_appearances ? emb_offset() : n_total; }
     DeviceBuffer<float> params_fp;
     DeviceBuffer<uint8_t> params_lp;  // fp16 copy when training in mixed precision
     DeviceBuffer<float> gradients;
@@ -139,6 +148,7 @@ struct NetworkWorkspace {
     DeviceBuffer<uint8_t> density, rgb, dL_drgb;   // network precision
     DeviceBuffer<float> dL_ds, dL_dist, ray_loss, ray_dist_loss, loss_out;
     DeviceBuffer<uint8_t> composite_scratch, distortion_scratch, mlp_ws, hash_ws;
+    DeviceBuffer<float> app_dz, app_ws;  // appearance embedding: colour-layer dz [64][B], per-image sums
     void enlarge(uint32_t B, uint32_t R, int precision);
     size_t bytes() const;
 };
@@ -161,6 +171,16 @@ struct NerfNetwork {
     tn_hashgrid_t grid{};
     int aabb_scale = 0;
     NetworkWorkspace workspace;
+    // appearance embedding: one per training image (0: none), and the
+    // per-image bias rows W3e emb[k] [n_appearances][64] of the current
+    // parameters
+    uint32_t n_appearances = 0;
+    DeviceBuffer<float> app_bias;
+
+    // an embedding for n appearances, or none for 0 (reference
+    // update_appearance_embedding_if_needed, src/core/nerf-network.cu:27-72);
+    // the parameters are laid out again on the next update_params_if_needed
+    void update_appearance_embedding_if_needed(uint32_t n) { n_appearances = config.appearance_dims ? n : 0; }
 
     // (re)build the encoding layout for a scene of side aabb (reference
     // update_aabb_scale_if_needed, src/core/nerf-network.cu:77-124)
@@ -177,14 +197,18 @@ struct NerfNetwork {
     // One training step over a batch already generated on the device. The
     // losses (reconstruction, distortion) are left in workspace.loss_out.
     // Reference NerfNetwork::train (src/core/nerf-network.cu:223-317).
+    // appearance_id_batch: per-sample image (non-decreasing over the batch;
+    // used with an appearance embedding only).
     void train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step, uint32_t batch_size, uint32_t ray_capacity,
                const uint32_t* info, const float* random_rgb, const uint32_t* ray_steps, const uint32_t* ray_offset,
-               const float* t_batch, const float* pos_batch, const float* dir_batch, const float* dt_batch,
-               const float* m_norm_batch, const float* dt_norm_batch, const float* target_rgba,
-               const Settings& settings);
+               const uint32_t* appearance_id_batch, const float* t_batch, const float* pos_batch,
+               const float* dir_batch, const float* dt_batch, const float* m_norm_batch, const float* dt_norm_batch,
+               const float* target_rgba, const Settings& settings);
 
     // density (and colour) of n points (reference NerfNetwork::inference,
-    // src/core/nerf-network.cu:319-429); outputs in the network precision
+    // src/core/nerf-network.cu:319-429); outputs in the network precision.
+    // With an appearance embedding every sample takes appearance 0, as the
+    // reference's renderer sets (src/core/renderer.cu:146-153).
     void inference(hipStream_t s, const NetworkParamsWorkspace& p, uint32_t n, const uint32_t* info, uint32_t stride,
                    const float* pos_batch, const float* dir_batch, void* features, void* density_out, void* rgb_out,
                    bool use_color_network);

@@ -10,14 +10,34 @@ namespace turbo {
 
 namespace {
 
-constexpr uint32_t kW5Rows = 3, kW5PaddedRows = 16, kHidden = 64;
-constexpr uint32_t kColorBeforeW5 = TN_COLOR_MLP_PARAMS - kW5Rows * kHidden;  // W3 and W4
+constexpr uint32_t kW5Rows = 3, kW5PaddedRows = 16, kHidden = 64, kColorIn = 32, kEmb = TN_APPEARANCE_DIMS;
+constexpr uint32_t kW3 = kHidden * kColorIn, kW4 = kHidden * kHidden;
 
-// colour parameters of this build -> the stored form (W5 padded to 16 rows)
-void pad_color(const float* in, float* out) {
-    std::memcpy(out, in, kColorBeforeW5 * sizeof(float));
-    std::memcpy(out + kColorBeforeW5, in + kColorBeforeW5, kW5Rows * kHidden * sizeof(float));
-    std::fill(out + kColorBeforeW5 + kW5Rows * kHidden, out + kColorBeforeW5 + kW5PaddedRows * kHidden, 0.0f);
+// colour parameters of this build (W3 [64][32], W4, W5 [3][64], and with an
+// embedding W3e [64][16] kept apart) -> tiny-cuda-nn's form: W3 [64][32 + 16]
+// with the embedding columns last, W4, W5 padded to 16 rows
+uint32_t stored_color_params(bool emb, bool padded) {
+    return kHidden * (kColorIn + (emb ? kEmb : 0)) + kW4 + (padded ? kW5PaddedRows : kW5Rows) * kHidden;
+}
+void pack_color(const float* in, const float* w3e, float* out) {
+    const uint32_t win = kColorIn + (w3e ? kEmb : 0);
+    for (uint32_t r = 0; r < kHidden; ++r) {
+        std::memcpy(out + r * win, in + r * kColorIn, kColorIn * sizeof(float));
+        if (w3e) std::memcpy(out + r * win + kColorIn, w3e + r * kEmb, kEmb * sizeof(float));
+    }
+    out += kHidden * win;
+    std::memcpy(out, in + kW3, (kW4 + kW5Rows * kHidden) * sizeof(float));
+    std::fill(out + kW4 + kW5Rows * kHidden, out + kW4 + kW5PaddedRows * kHidden, 0.0f);
+}
+// the stored form (padded or not) -> this build's
+void unpack_color(const float* in, bool emb, float* out, float* w3e) {
+    const uint32_t win = kColorIn + (emb ? kEmb : 0);
+    for (uint32_t r = 0; r < kHidden; ++r) {
+        std::memcpy(out + r * kColorIn, in + r * win, kColorIn * sizeof(float));
+        if (emb) std::memcpy(w3e + r * kEmb, in + r * win + kColorIn, kEmb * sizeof(float));
+    }
+    // W4 and the first 3 rows of W5 (a padded form's extra rows follow them)
+    std::memcpy(out + kW3, in + kHidden * win, (kW4 + kW5Rows * kHidden) * sizeof(float));
 }
 
 template <typename T>
@@ -53,18 +73,24 @@ void FileManager::save(const NeRFProxy* proxy, const std::string& file_path, boo
         std::memcpy(data.dataset_path, dp.data(), n);
     }
     const uint32_t n_density = p.n_grid + TN_DENSITY_MLP_PARAMS;
+    const uint32_t n_app = p.n_appearances;
     data.n_density_params = n_density;
-    data.n_color_params = kStoredColorParams;
-    data.n_appearance_embedding_params = 0;
+    data.n_color_params = stored_color_params(n_app > 0, true);
+    data.n_appearance_embedding_params = n_app * kEmb;
     data.n_occ_grid_bits = (uint32_t)nerf.occupancy_grid.bitfield.size();
     data.aabb_scale = (uint32_t)proxy->training_bbox.get().size();
-    data.n_appearances = 0;
+    data.n_appearances = n_app;
 
     std::vector<float> params(p.n_total);
     TN_HIP(hipMemcpy(params.data(), p.params_fp.data(), params.size() * sizeof(float), hipMemcpyDeviceToHost));
     std::vector<float> stored(data.n_params());
     std::memcpy(stored.data(), params.data(), n_density * sizeof(float));
-    pad_color(params.data() + n_density, stored.data() + n_density);
+    pack_color(params.data() + n_density, n_app ? params.data() + p.w3e_offset() : nullptr,
+               stored.data() + n_density);
+    // the embeddings follow the colour network, as the reference stores them
+    if (n_app)
+        std::memcpy(stored.data() + n_density + data.n_color_params, params.data() + p.emb_offset(),
+                    (size_t)n_app * kEmb * sizeof(float));
     std::vector<uint8_t> bits(data.n_occ_grid_bits);
     TN_HIP(hipMemcpy(bits.data(), nerf.occupancy_grid.bitfield.data(), bits.size(), hipMemcpyDeviceToHost));
 
@@ -98,13 +124,16 @@ bool FileManager::load(NeRFProxy* proxy, const std::string& file_path) {
     if (data.version != kFileVersion)
         throw std::runtime_error("File version " + std::to_string(data.version) +
                                  " is not supported. Supported version is " + std::to_string(kFileVersion));
-    if (data.n_appearance_embedding_params != 0 || data.n_appearances != 0)
-        throw std::runtime_error("FileManager::load: appearance embeddings are not part of this build");
+    const uint32_t n_app = data.n_appearances;
+    if (data.n_appearance_embedding_params != n_app * kEmb)
+        throw std::runtime_error("FileManager::load: " + std::to_string(data.n_appearance_embedding_params) +
+                                 " embedding parameters for " + std::to_string(n_app) +
+                                 " appearances; this build's embeddings have " + std::to_string(kEmb) + " each");
     // the padded form tiny-cuda-nn counts, or the unpadded one
-    const bool padded = data.n_color_params == kStoredColorParams;
-    if (!padded && data.n_color_params != TN_COLOR_MLP_PARAMS)
+    const bool emb = n_app > 0;
+    if (data.n_color_params != stored_color_params(emb, true) && data.n_color_params != stored_color_params(emb, false))
         throw std::runtime_error("FileManager::load: a colour network of " + std::to_string(data.n_color_params) +
-                                 " parameters, this build's has " + std::to_string(kStoredColorParams));
+                                 " parameters, this build's has " + std::to_string(stored_color_params(emb, true)));
     // every cascade shares one byte per cell of the 128^3 grid
     const uint32_t grid_bytes = 128u * 128u * 128u;
     if (data.n_occ_grid_bits != grid_bytes)
@@ -137,7 +166,8 @@ bool FileManager::load(NeRFProxy* proxy, const std::string& file_path) {
     if (log2_T < 0)
         throw std::runtime_error("FileManager::load: a density network of " + std::to_string(data.n_density_params) +
                                  " parameters matches no hash-grid layout of this build");
-    const uint32_t n_total = data.n_density_params + TN_COLOR_MLP_PARAMS;
+    const uint32_t n_total =
+        data.n_density_params + TN_COLOR_MLP_PARAMS + (emb ? TN_APPEARANCE_WEIGHTS + n_app * kEmb : 0);
 
     // the optional optimiser section, read in full before anything changes
     bool has_opt = false;
@@ -164,12 +194,16 @@ bool FileManager::load(NeRFProxy* proxy, const std::string& file_path) {
 
     std::vector<float> params(n_total);
     std::memcpy(params.data(), stored.data(), data.n_density_params * sizeof(float));
-    // the colour network: W3 and W4 as stored, then the first 3 rows of W5
-    // (the padded form's extra rows follow them and are dropped)
-    std::memcpy(params.data() + data.n_density_params, stored.data() + data.n_density_params,
-                TN_COLOR_MLP_PARAMS * sizeof(float));
+    const uint32_t w3e_at = data.n_density_params + TN_COLOR_MLP_PARAMS;
+    unpack_color(stored.data() + data.n_density_params, emb, params.data() + data.n_density_params,
+                 params.data() + w3e_at);
+    if (emb)
+        std::memcpy(params.data() + w3e_at + TN_APPEARANCE_WEIGHTS,
+                    stored.data() + data.n_density_params + data.n_color_params, (size_t)n_app * kEmb * sizeof(float));
 
     net.config.log2_hashmap_size = log2_T;
+    net.config.appearance_dims = emb ? (int)kEmb : 0;
+    net.n_appearances = n_app;
     net.aabb_scale = 0;  // rebuild the layout
     net.update_aabb_scale_if_needed(aabb);
     const hipStream_t s = DeviceManager::stream();

@@ -1,10 +1,12 @@
 // Snapshot files of a trained NeRF (reference src/services/file-manager.cuh:
 // 17-198): a 300-byte header, the fp32 parameters (the density network --
-// hash table, then its MLP -- followed by the colour network), then the
+// hash table, then its MLP -- followed by the colour network and, with an
+// appearance embedding, the embeddings [n_appearances][16]), then the
 // occupancy bitfield. Same names, layout and errors as the reference. The
-// colour network is stored as tiny-cuda-nn's fully fused MLP counts it, its
-// 3-row output layer padded with zero rows to 16 (n_color_params = 7168), so
-// the header's counts match a file of the reference's.
+// colour network is stored as tiny-cuda-nn's fully fused MLP counts it: its
+// first layer [64][32 + 16 embedding columns], and its 3-row output layer
+// padded with zero rows to 16 (n_color_params = 7168, or 8192 with an
+// embedding), so the header's counts match a file of the reference's.
 //
 // Optionally (save with include_optimizer_state) an optimiser section follows
 // the bitfield, for an exact resume of training (the reference serialises the

@@ -304,6 +304,14 @@ TN_DEVICE void sh4(float ix, float iy, float iz, float* o) {
     o[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
 }
 
+// this lane's appearance row: the bias row of its sample's image (row 0 for
+// every sample when no image ids are given), or null without an embedding
+TN_DEVICE const float* app_row(const tn_appearance_t& app, bool valid, uint32_t sample) {
+    if (!app.bias) return nullptr;
+    const uint32_t k = app.image_id && valid ? app.image_id[sample] : 0u;
+    return app.bias + 64u * (k < app.n_appearances ? k : 0u);
+}
+
 // Activations of one 32-sample tile (the forward pass; the backward pass
 // recomputes it instead of storing activations).
 template <class P>
@@ -359,8 +367,29 @@ TN_DEVICE void load_inputs(Tile<P>& tl, TileLo<P>& lo, bool valid, uint32_t samp
     }
 }
 
+// row of accumulator register r in lane half h of a 32-row tile t
+TN_DEVICE int acc_row(int t, int r, int h) { return 32 * t + 8 * (r >> 2) + 4 * h + (r & 3); }
+
+// accumulator tile t holding rows 32t.. of a 64-vector (4 float4 loads)
+TN_DEVICE f16v load_rows(const float* __restrict__ v, int t, int h) {
+    f16v a;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 f = *reinterpret_cast<const float4*>(v + 32 * t + 8 * q + 4 * h);
+        a[4 * q] = f.x;
+        a[4 * q + 1] = f.y;
+        a[4 * q + 2] = f.z;
+        a[4 * q + 3] = f.w;
+    }
+    return a;
+}
+
+// Forward of one tile. app_bias (may be null): this lane's sample's
+// appearance row, W3[:, 32:48] times its image's embedding, which starts the
+// colour MLP's first-layer accumulator.
 template <class P>
-TN_DEVICE void forward_tile(Tile<P>& tl, const TileLo<P>& lo, const FwdLds<P>& W, int lane, bool color) {
+TN_DEVICE void forward_tile(Tile<P>& tl, const TileLo<P>& lo, const FwdLds<P>& W, int lane, bool color,
+                            const float* __restrict__ app_bias = nullptr) {
     using N = Net<P>;
     constexpr bool kSplit = P::SL == 8;
     OpTile<P> h1lo[2];
@@ -387,7 +416,7 @@ TN_DEVICE void forward_tile(Tile<P>& tl, const TileLo<P>& lo, const FwdLds<P>& W
     constexpr int half = kKsPerTile<P> / 2;
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
-        f16v acc = zero16();
+        f16v acc = app_bias ? load_rows(app_bias, mt, lane >> 5) : zero16();
 #pragma unroll
         for (int ks = 0; ks < half; ++ks) {
             const typename P::Op a = load_a<P>(W.l3, N::L3, mt, ks, lane);
@@ -416,7 +445,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves)
 mlp_forward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t B, uint32_t FS,
                    const typename P::T* __restrict__ params, const TF* __restrict__ feat,
                    const float* __restrict__ dir, int color, typename P::T* __restrict__ density_out,
-                   typename P::T* __restrict__ rgb_out, uint32_t OS) {
+                   typename P::T* __restrict__ rgb_out, uint32_t OS, tn_appearance_t app) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     typename P::T* lds = reinterpret_cast<typename P::T*>(smem_raw);
     const FwdLds<P> W = stage_forward<P>(lds, params);
@@ -433,7 +462,7 @@ mlp_forward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t B
         Tile<P> tl;
         TileLo<P> lo;
         load_inputs<P>(tl, lo, valid, sample, h, feat, FS, dir, B, color != 0);
-        forward_tile<P>(tl, lo, W, lane, color != 0);
+        forward_tile<P>(tl, lo, W, lane, color != 0, app_row(app, valid, sample));
         if (valid && h == 0) {
             density_out[sample] = (typename P::T)tl.s0;
             if (color) {
@@ -483,9 +512,6 @@ static_assert(kMlpWaves == 4, "the dW split assumes four waves");
 // spreads banks)
 template <class P>
 constexpr int kGW = P::SL == 8 ? kGrp + 8 : kGrp + 1;
-
-// row of accumulator register r in lane half h of a 32-row tile t
-TN_DEVICE int acc_row(int t, int r, int h) { return 32 * t + 8 * (r >> 2) + 4 * h + (r & 3); }
 
 // tile t of a wave's samples (columns 32w..32w+31); HALF: rows 32t..32t+15
 // only (registers 0..7)
@@ -557,7 +583,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
                     const float* __restrict__ dir, const float* __restrict__ sample_t,
                     const TG* __restrict__ dL_drgb, uint32_t GS, const float* __restrict__ dL_ds,
                     const float* __restrict__ dL_ds_extra, float loss_scale, int scale_by_rays,
-                    TF* __restrict__ dfeat, float* __restrict__ dw_partial) {
+                    TF* __restrict__ dfeat, float* __restrict__ dw_partial, tn_appearance_t app) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     using T = typename P::T;
     using N = Net<P>;
@@ -594,7 +620,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         const float g_s = ls * dL_ds[sl];
         const float g_x = dL_ds_extra ? ls * dL_ds_extra[sl] : 0.0f;
         const float t_s = sample_t[sl];
-        forward_tile<P>(tl, lo, W, lane, true);
+        forward_tile<P>(tl, lo, W, lane, true, app_row(app, valid, sample));
 
         // dL/do through the sigmoid (rows 0..2 live in lanes 0..31, registers 0..2)
         f16v dO = zero16();
@@ -620,8 +646,15 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         // dH2 = relu'(h2) (W4^T dH3); dW4 += dH3 h2^T
         OpTile<P> dH2[2];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
-            dH2[mt] = to_ops<P>(relu_mask<P>(gemm_op<P, 2>(WB.b4, N::B4, mt, dH3, lane, N::B4.n_ks), tl.h2[mt]));
+        for (int mt = 0; mt < 2; ++mt) {
+            const f16v d = relu_mask<P>(gemm_op<P, 2>(WB.b4, N::B4, mt, dH3, lane, N::B4.n_ks), tl.h2[mt]);
+            dH2[mt] = to_ops<P>(d);
+            // the appearance bias's gradient per sample (summed per image later)
+            if (app.dz && valid) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) app.dz[(uint32_t)acc_row(mt, r, h) * B + sample] = d[r];
+            }
+        }
         put_grp<P>(dbuf, 0, wid, dH3[0], lane);
         put_grp<P>(dbuf, 1, wid, dH3[1], lane);
         put_grp<P>(abuf, 0, wid, tl.h2[0], lane);
@@ -787,8 +820,10 @@ using namespace tn;
 extern "C" int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* info, uint32_t stride,
                               uint32_t feat_stride, uint32_t out_stride, const void* params, uint32_t net_offset,
                               const void* features, const float* dir, int color, void* density_out, void* rgb_out,
-                              hipStream_t stream) {
+                              const tn_appearance_t* appearance, hipStream_t stream) {
     TN_CHECK_ARG(params && features && density_out && (!color || (dir && rgb_out)));
+    const tn_appearance_t app = appearance && color ? *appearance : tn_appearance_t{};
+    TN_CHECK_ARG(!app.bias || (app.n_appearances > 0 && (reinterpret_cast<uintptr_t>(app.bias) & 15) == 0));
     TN_CHECK_ARG(n_max <= stride && n_max <= feat_stride && n_max <= out_stride);
     TN_CHECK_ARG(32ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * out_stride < (1ull << 32));
     if (n_max == 0) return TN_OK;
@@ -798,13 +833,13 @@ extern "C" int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* inf
         const auto* w = static_cast<const P::T*>(params) + net_offset;
         mlp_forward_kernel<P, float><<<blocks, 64 * kMlpWaves, fwd_lds_bytes<P>(), stream>>>(
             n_max, info, stride, feat_stride, w, static_cast<const float*>(features), dir, color,
-            static_cast<P::T*>(density_out), static_cast<P::T*>(rgb_out), out_stride);
+            static_cast<P::T*>(density_out), static_cast<P::T*>(rgb_out), out_stride, app);
     } else if (precision == 0) {
         using P = PrecF32;
         const auto* w = static_cast<const P::T*>(params) + net_offset;
         mlp_forward_kernel<P, P::T><<<blocks, 64 * kMlpWaves, fwd_lds_bytes<P>(), stream>>>(
             n_max, info, stride, feat_stride, w, static_cast<const P::T*>(features), dir, color,
-            static_cast<P::T*>(density_out), static_cast<P::T*>(rgb_out), out_stride);
+            static_cast<P::T*>(density_out), static_cast<P::T*>(rgb_out), out_stride, app);
     } else {
         return TN_ERR_INVALID_ARGUMENT;
     }
@@ -828,8 +863,11 @@ extern "C" int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* in
                                const float* dir, const float* sample_t, const void* dL_drgb, uint32_t grad_stride,
                                const float* dL_ds, const float* dL_ds_extra, float loss_scale, int scale_by_rays,
                                void* dfeat, float* grad_net, int accumulate, void* workspace, size_t workspace_bytes,
-                               hipStream_t stream) {
+                               const tn_appearance_t* appearance, hipStream_t stream) {
     TN_CHECK_ARG(params && features && dir && sample_t && dL_drgb && dL_ds && dfeat && grad_net && workspace);
+    const tn_appearance_t app = appearance ? *appearance : tn_appearance_t{};
+    TN_CHECK_ARG(!app.bias || (app.n_appearances > 0 && (reinterpret_cast<uintptr_t>(app.bias) & 15) == 0));
+    TN_CHECK_ARG(!app.dz || app.bias);
     TN_CHECK_ARG(!scale_by_rays || info);
     TN_CHECK_ARG(n_max <= stride && n_max <= feat_stride && n_max <= grad_stride);
     TN_CHECK_ARG(32ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * grad_stride < (1ull << 32));
@@ -844,7 +882,7 @@ extern "C" int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* in
         mlp_backward_kernel<P, float, P::T><<<blocks, 64 * kMlpWaves, bwd_lds_bytes<P>(), stream>>>(
             n_max, info, stride, feat_stride, w, static_cast<const float*>(features), dir, sample_t,
             static_cast<const P::T*>(dL_drgb), grad_stride, dL_ds, dL_ds_extra, loss_scale, scale_by_rays,
-            static_cast<float*>(dfeat), partial);
+            static_cast<float*>(dfeat), partial, app);
     } else if (precision == 0) {
         using P = PrecF32;
         const auto* w = static_cast<const P::T*>(params) + net_offset;
@@ -852,7 +890,7 @@ extern "C" int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* in
         mlp_backward_kernel<P, P::T, P::T><<<blocks, 64 * kMlpWaves, bwd_lds_bytes<P>(), stream>>>(
             n_max, info, stride, feat_stride, w, static_cast<const P::T*>(features), dir, sample_t,
             static_cast<const P::T*>(dL_drgb), grad_stride, dL_ds, dL_ds_extra, loss_scale, scale_by_rays,
-            static_cast<P::T*>(dfeat), partial);
+            static_cast<P::T*>(dfeat), partial, app);
     } else {
         return TN_ERR_INVALID_ARGUMENT;
     }

@@ -250,22 +250,58 @@ def hashgrid_backward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: i
          ptr(dtable), stream_ptr())
 
 
+def appearance(bias: torch.Tensor | None, image_id: torch.Tensor | None = None, dz: torch.Tensor | None = None):
+    """tn_appearance_t for the MLP calls (None without an embedding); bias [n][64] fp32."""
+    if bias is None:
+        return None
+    if bias.dtype != torch.float32 or bias.numel() % 64:
+        raise ValueError("appearance bias rows are fp32 [n][64]")
+    return _capi.Appearance(ptr(bias), ptr(image_id), bias.numel() // 64, ptr(dz))
+
+
+def _app_ref(app):
+    return None if app is None else C.byref(app)
+
+
 def mlp_forward(params: torch.Tensor, net_offset: int, n: int, feat: torch.Tensor, feat_stride: int,
-                dir: torch.Tensor | None, stride: int, color: bool = True, info: torch.Tensor | None = None):
+                dir: torch.Tensor | None, stride: int, color: bool = True, info: torch.Tensor | None = None,
+                app=None):
     prec = _PREC[params.dtype]
     if feat.dtype != torch.float32 or feat.numel() < 32 * feat_stride:
         raise ValueError("the MLP kernels take fp32 features [32][feat_stride]")
     dens = torch.zeros(stride, dtype=params.dtype, device="cuda")
     rgb = torch.zeros(3 * stride, dtype=params.dtype, device="cuda") if color else None
     call("tn_mlp_forward", prec, n, ptr(info), stride, feat_stride, stride, ptr(params), net_offset, ptr(feat),
-         ptr(dir), int(color), ptr(dens), ptr(rgb), stream_ptr())
+         ptr(dir), int(color), ptr(dens), ptr(rgb), _app_ref(app), stream_ptr())
     return dens, rgb
+
+
+def appearance_bias(w3e: torch.Tensor, emb: torch.Tensor) -> torch.Tensor:
+    """Per-image bias rows W3e emb[k]: w3e fp32 [64*16], emb fp32 [n*16] -> [n*64]."""
+    n = emb.numel() // 16
+    bias = torch.empty(64 * max(n, 1), dtype=torch.float32, device="cuda")
+    call("tn_appearance_bias", n, ptr(w3e), ptr(emb), ptr(bias), stream_ptr())
+    return bias
+
+
+def appearance_backward(n: int, stride: int, image_id: torch.Tensor, dz: torch.Tensor, w3e: torch.Tensor,
+                        emb: torch.Tensor, info: torch.Tensor | None = None):
+    """(dL/dW3e [64*16], dL/demb [n_img*16]) from the colour layer's dz [64][stride]."""
+    n_app = emb.numel() // 16
+    g_w = torch.empty(64 * 16, dtype=torch.float32, device="cuda")
+    g_e = torch.empty(n_app * 16, dtype=torch.float32, device="cuda")
+    nb = query("tn_appearance_workspace_bytes", n_app)
+    ws = torch.empty(max(nb, 4), dtype=torch.uint8, device="cuda")
+    call("tn_appearance_backward", n, ptr(info), stride, ptr(image_id), ptr(dz), n_app, ptr(w3e), ptr(emb), ptr(g_w),
+         ptr(g_e), ptr(ws), nb, stream_ptr())
+    return g_w, g_e
 
 
 def mlp_backward(params: torch.Tensor, net_offset: int, n: int, feat: torch.Tensor, feat_stride: int,
                  dir: torch.Tensor, stride: int, sample_t: torch.Tensor, dL_drgb: torch.Tensor, dL_ds: torch.Tensor,
                  loss_scale: float, grad_net: torch.Tensor, dL_ds_extra: torch.Tensor | None = None,
-                 accumulate: bool = False, info: torch.Tensor | None = None, scale_by_rays: bool = False):
+                 accumulate: bool = False, info: torch.Tensor | None = None, scale_by_rays: bool = False,
+                 app=None):
     """Returns the fp32 feature gradients [32][feat_stride]; weight gradients go to grad_net."""
     prec = _PREC[params.dtype]
     if feat.dtype != torch.float32 or feat.numel() < 32 * feat_stride:
@@ -277,8 +313,7 @@ def mlp_backward(params: torch.Tensor, net_offset: int, n: int, feat: torch.Tens
     ws = torch.empty(max(nb, 4), dtype=torch.uint8, device="cuda")
     call("tn_mlp_backward", prec, n, ptr(info), stride, feat_stride, ptr(params), net_offset, ptr(feat), ptr(dir),
          ptr(sample_t), ptr(dL_drgb), stride, ptr(dL_ds), ptr(dL_ds_extra), loss_scale, int(scale_by_rays), ptr(dfeat),
-         ptr(grad_net),
-         int(accumulate), ptr(ws), nb, stream_ptr())
+         ptr(grad_net), int(accumulate), ptr(ws), nb, _app_ref(app), stream_ptr())
     return dfeat
 
 
