@@ -711,6 +711,7 @@ def main():
         extra["fp32_ms_per_step"] = round(f["ms_per_step"], 4)
         extra["mixed_over_fp32"] = round(main["samples_per_s"] / f["samples_per_s"], 3)
         extra["fp32_roofline"] = roofline(f, n_params, "fp32")[0]
+        extra["fp32_phases_ms_per_step"] = roofline(f, n_params, "fp32")[1]
         extra["fp32_autotune"] = f["autotune"]
     if not args.no_render:
         mr, ms = render_rate(tn, synthetic, scene, main["trainer"])
