@@ -354,6 +354,47 @@ int tn_render_rays(uint32_t n_rays, uint32_t start_pixel, uint32_t stride, tn_ca
 int tn_render_write_rgba(uint32_t n_rays, const uint32_t* info, uint32_t rgba_stride,
     const uint32_t* pixel, const uint8_t* alive, const float* rgba, float* out, tn_stream_t stream);
 
+/* Several NeRFs in one render (reference renderer.cu:344-521): world rays
+ * clipped to the union of the NeRFs' boxes (prepare_for_linear_raymarching_kernel,
+ * rendering-kernels.cu:175-260: origin moved to the first entry, t = 0 there,
+ * t_max the last exit capped by far - near, rebased); k fixed steps per live
+ * ray per pass, samples step-major (s * n_rays + i), absent steps dt < 0
+ * (march_rays_and_generate_global_sample_points_kernel, :264-332); per NeRF,
+ * the samples inside its box on occupied cells with their network inputs
+ * (filter_and_localize_samples_for_nerf_kernel + the spatial effects +
+ * assign_normalized_network_inputs_kernel, :336-478), packed to the front
+ * (tn_scan_select's offsets), evaluated and added into the samples
+ * (accumulate_nerf_samples_kernel, :480-528); then front-to-back compositing
+ * of the summed alpha (capped at 1) and mean colour (composite_samples_kernel,
+ * :531-606), ending rays at min_transmittance or their last step. */
+int tn_render_rays_multi(uint32_t n_rays, uint32_t start_pixel, uint32_t stride, tn_camera_t camera,
+    uint32_t n_nerfs, const tn_render_nerf_t* nerfs, float* ori, float* dir, float* ray_t,
+    float* ray_tmax, uint32_t* ray_steps, uint32_t* pixel, uint8_t* alive, float* trans,
+    tn_stream_t stream);
+int tn_render_march_fixed(uint32_t n_rays, uint32_t k, uint32_t sample_stride, float cone_angle,
+    float dt_min, float dt_max, uint32_t max_steps, const uint8_t* alive, const float* ori,
+    const float* dir, uint32_t ray_stride, float* ray_t, const float* ray_tmax, uint32_t* ray_steps,
+    float* s_pos, float* s_dt, tn_stream_t stream);
+int tn_render_localize(uint32_t n_rays, uint32_t k, uint32_t sample_stride, tn_render_nerf_t nerf,
+    tn_occupancy_grid_t grid, const float* dir, uint32_t ray_stride, const float* s_pos,
+    const float* s_dt, uint32_t* valid, float* net_pos, float* net_dir, float* net_dt,
+    tn_stream_t stream);
+int tn_render_compact(uint32_t n_samples, uint32_t sample_stride, const uint32_t* valid,
+    const uint32_t* offsets, const float* net_pos, const float* net_dir, float* c_pos, float* c_dir,
+    tn_stream_t stream);
+int tn_render_accumulate(int dtype, uint32_t n_samples, uint32_t sample_stride, const uint32_t* valid,
+    const uint32_t* offsets, const void* density, const void* rgb, int raw_density,
+    const float* net_dt, float* s_rgba, uint32_t* count, tn_stream_t stream);
+int tn_render_composite_fixed(uint32_t n_rays, uint32_t k, uint32_t sample_stride,
+    float min_transmittance, const float* s_dt, const float* s_rgba, const uint32_t* count,
+    const uint32_t* pixel, uint8_t* alive, float* trans, float* image, tn_stream_t stream);
+
+/* *out = the number of non-zero bytes in v[0, n) (reference
+ * count_valued_elements, src/utils/stream-compaction.cuh:58); p[0, n) = v (in place
+ * of cudaMemsetAsync, which a captured graph should not hold). */
+int tn_count_nonzero(uint32_t n, const uint8_t* v, uint32_t* out, tn_stream_t stream);
+int tn_fill_u32(uint32_t n, uint32_t* p, uint32_t v, tn_stream_t stream);
+
 /* ------------------------------------------- synthetic analytic field */
 
 /* Gaussian-blob field (blob k = {cx,cy,cz,s,p,r,g,b}): density at every cell

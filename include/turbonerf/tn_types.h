@@ -108,6 +108,23 @@ typedef struct tn_hashgrid {
 #define TN_DENSITY_MLP_PARAMS (64 * 32 + 16 * 64)
 #define TN_COLOR_MLP_PARAMS (64 * 32 + 64 * 64 + 3 * 64)
 
+/* One NeRF of a render of several (reference NeRFRenderable with its
+ * spatial effects, src/models/nerf-renderable.cuh, src/effects/spatial/
+ * repeater.cu:8-76): the world -> NeRF transform (3x4 row-major), the box a
+ * sample must fall in, in the NeRF frame (the render box grown by the
+ * effects), and an optional repeater that folds positions into its source
+ * box (p -> ((p - min) mod size + size) mod size + min) before the grid and
+ * the network see them. */
+typedef struct tn_render_nerf {
+    float to_local[12];
+    tn_bbox_t render_box;
+    tn_bbox_t repeat_source;
+    int32_t repeat;          /* 1: fold positions into repeat_source */
+    float inv_aabb;          /* 1 / the NeRF's training-box side (step normalisation) */
+    float training_min[3];   /* training box: network inputs are (p - min) / size */
+    float training_size;
+} tn_render_nerf_t;
+
 /* Appearance embedding (reference src/core/embedding.cuh:45-99,
  * src/core/nerf-network.cu:27-72): with it the colour MLP's first layer takes
  * 16 more inputs, the embedding emb[k][16] of the sample's image k, through

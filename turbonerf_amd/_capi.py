@@ -86,6 +86,18 @@ class HashGrid(C.Structure):
 # ----------------------------------------------------------------- loading
 _lib: C.CDLL | None = None
 
+class RenderNerf(C.Structure):
+    _fields_ = [
+        ("to_local", C.c_float * 12),
+        ("render_box", BBox),
+        ("repeat_source", BBox),
+        ("repeat", C.c_int32),
+        ("inv_aabb", C.c_float),
+        ("training_min", C.c_float * 3),
+        ("training_size", C.c_float),
+    ]
+
+
 class Appearance(C.Structure):
     _fields_ = [
         ("bias", C.c_void_p),
@@ -153,6 +165,14 @@ _SIGS: dict[str, tuple] = {
     "tn_appearance_bias": (C.c_int, [u32, vp, vp, vp, vp]),
     "tn_appearance_workspace_bytes": (sz, [u32]),
     "tn_appearance_backward": (C.c_int, [u32, vp, u32, vp, vp, u32, vp, vp, vp, vp, vp, sz, vp]),
+    "tn_render_rays_multi": (C.c_int, [u32, u32, u32, Camera, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "tn_render_march_fixed": (C.c_int, [u32, u32, u32, f32, f32, f32, u32, vp, vp, vp, u32, vp, vp, vp, vp, vp, vp]),
+    "tn_render_localize": (C.c_int, [u32, u32, u32, RenderNerf, OccupancyGrid, vp, u32, vp, vp, vp, vp, vp, vp, vp]),
+    "tn_render_compact": (C.c_int, [u32, u32, vp, vp, vp, vp, vp, vp, vp]),
+    "tn_render_accumulate": (C.c_int, [C.c_int, u32, u32, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]),
+    "tn_render_composite_fixed": (C.c_int, [u32, u32, u32, f32, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "tn_count_nonzero": (C.c_int, [u32, vp, vp, vp]),
+    "tn_fill_u32": (C.c_int, [u32, vp, u32, vp]),
     "tn_tune_set": (C.c_int, [C.c_int, C.c_int]),
     "tn_tune_get": (C.c_int, [C.c_int, vp]),
     "tn_tune_generation": (u32, []),

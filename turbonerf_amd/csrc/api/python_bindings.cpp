@@ -143,6 +143,14 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
     py::class_<BoundingBox>(m, "BoundingBox")
         .def(py::init<>())
         .def(py::init<float>(), py::arg("size"))
+        .def(py::init([](float x0, float y0, float z0, float x1, float y1, float z1) {
+                 BoundingBox b;
+                 b.min_x = x0; b.min_y = y0; b.min_z = z0;
+                 b.max_x = x1; b.max_y = y1; b.max_z = z1;
+                 return b;
+             }),
+             py::arg("min_x"), py::arg("min_y"), py::arg("min_z"), py::arg("max_x"), py::arg("max_y"),
+             py::arg("max_z"))
         .def("size", &BoundingBox::size)
         .def(py::self == py::self)
         .def_readwrite("min_x", &BoundingBox::min_x)

@@ -12,13 +12,22 @@ namespace py = pybind11;
 namespace turbo {
 
 void register_renderer_bindings(py::module_& m) {
+    py::class_<SpatialEffect, std::shared_ptr<SpatialEffect>>(m, "SpatialEffect");
+    py::class_<RepeaterEffect, SpatialEffect, std::shared_ptr<RepeaterEffect>>(m, "RepeaterEffect")
+        .def(py::init<BoundingBox, BoundingBox, Transform4f>(), py::arg("source_bbox"), py::arg("extend_bbox"),
+             py::arg("transform") = Transform4f::Identity())
+        .def_readonly("source_bbox", &RepeaterEffect::source_bbox)
+        .def_readonly("extend_bbox", &RepeaterEffect::extend_bbox);
+
     py::class_<NeRFRenderable>(m, "Renderable")
-        .def(py::init([](NeRFProxy* nerf, py::object bbox) {
-                 if (bbox.is_none()) return NeRFRenderable(nerf);
-                 return NeRFRenderable(nerf, bbox.cast<BoundingBox>());
+        .def(py::init([](NeRFProxy* nerf, std::vector<std::shared_ptr<SpatialEffect>> fx, py::object bbox) {
+                 if (bbox.is_none()) return NeRFRenderable(nerf, std::move(fx));
+                 return NeRFRenderable(nerf, bbox.cast<BoundingBox>(), std::move(fx));
              }),
-             py::arg("nerf"), py::arg("bounding_box") = py::none(), py::keep_alive<1, 2>())
+             py::arg("nerf"), py::arg("spatial_effects") = std::vector<std::shared_ptr<SpatialEffect>>(),
+             py::arg("bounding_box") = py::none(), py::keep_alive<1, 2>())
         .def_readonly("nerf", &NeRFRenderable::proxy)
+        .def_readonly("spatial_effects", &NeRFRenderable::spatial_effects)
         .def_readwrite("bounding_box", &NeRFRenderable::bounding_box);
 
     py::enum_<RenderPattern>(m, "RenderPattern")
@@ -77,7 +86,8 @@ void register_renderer_bindings(py::module_& m) {
         .def("cancel", &NeRFRenderingController::cancel)
         .def_readwrite("min_step_size", &NeRFRenderingController::min_step_size)
         .def_readwrite("max_samples_per_ray", &NeRFRenderingController::max_samples_per_ray)
-        .def_readonly("last_sample_count", &NeRFRenderingController::last_sample_count);
+        .def_readonly("last_sample_count", &NeRFRenderingController::last_sample_count)
+        .def_readwrite("_force_shared_steps", &NeRFRenderingController::force_shared_steps);
 }
 
 }  // namespace turbo

@@ -119,6 +119,132 @@ uint64_t Renderer::render_pixels(hipStream_t s, const RenderRequest& req, const 
     return n_evals;
 }
 
+BoundingBox RepeaterEffect::get_bbox(const BoundingBox& b) const {
+    BoundingBox o;
+    o.min_x = std::min(extend_bbox.min_x, b.min_x);
+    o.min_y = std::min(extend_bbox.min_y, b.min_y);
+    o.min_z = std::min(extend_bbox.min_z, b.min_z);
+    o.max_x = std::max(extend_bbox.max_x, b.max_x);
+    o.max_y = std::max(extend_bbox.max_y, b.max_y);
+    o.max_z = std::max(extend_bbox.max_z, b.max_z);
+    return o;
+}
+
+void RenderingWorkspace::enlarge_multi(uint32_t rays, uint32_t samples, uint32_t n_nerfs) {
+    if (m_trans.size() < rays) m_trans.resize(rays);
+    if (m_dt.size() < samples) {
+        m_pos.resize(3 * (size_t)samples);
+        m_dt.resize(samples);
+        m_net_pos.resize(3 * (size_t)samples);
+        m_net_dir.resize(3 * (size_t)samples);
+        m_net_dt.resize(samples);
+        m_c_pos.resize(3 * (size_t)samples);
+        m_c_dir.resize(3 * (size_t)samples);
+        m_rgba.resize(4 * (size_t)samples);
+        m_valid.resize(samples);
+        m_offs.resize(samples);
+        m_count.resize(samples);
+        m_scan_ws.resize(std::max<size_t>(tn_scan_workspace_bytes(samples), 4));
+    }
+    m_alive_n.resize(4);
+    if (m_nerfs.size() < n_nerfs * sizeof(tn_render_nerf_t)) m_nerfs.resize(n_nerfs * sizeof(tn_render_nerf_t));
+}
+
+uint64_t Renderer::render_pixels_multi(hipStream_t s, const RenderRequest& req,
+                                       const std::vector<const NeRFRenderable*>& rs, uint32_t start, uint32_t n,
+                                       float* image, const Settings& st) {
+    const tn_stream_t ts = as_tn(s);
+    const uint32_t R = ws.ray_capacity, S = ws.sample_capacity;
+    const uint32_t n_nerfs = (uint32_t)rs.size();
+    ws.enlarge_multi(R, S, n_nerfs);
+    // each NeRF's placement: world -> NeRF transform, its render box grown by
+    // its effects, an optional repeater and its training box
+    std::vector<tn_render_nerf_t> nf(n_nerfs);
+    for (uint32_t k = 0; k < n_nerfs; ++k) {
+        const NeRFRenderable& r = *rs[k];
+        tn_render_nerf_t& d = nf[k];
+        const auto a = r.proxy->transform.get().inverse().to_array();
+        std::copy(a.begin(), a.end(), d.to_local);
+        BoundingBox box = r.bounding_box;
+        d.repeat = 0;
+        for (const auto& fx : r.spatial_effects) {
+            box = fx->get_bbox(box);
+            if (auto* rep = dynamic_cast<const RepeaterEffect*>(fx.get())) {
+                d.repeat = 1;
+                d.repeat_source = rep->source_bbox.c();
+            }
+        }
+        d.render_box = box.c();
+        const BoundingBox tb = r.proxy->training_bbox.get();
+        d.inv_aabb = 1.0f / tb.size();
+        d.training_min[0] = tb.min_x;
+        d.training_min[1] = tb.min_y;
+        d.training_min[2] = tb.min_z;
+        d.training_size = tb.size();
+    }
+    TN_HIP(hipMemcpyAsync(ws.m_nerfs.data(), nf.data(), n_nerfs * sizeof(tn_render_nerf_t), hipMemcpyHostToDevice, s));
+    const auto* d_nerfs = reinterpret_cast<const tn_render_nerf_t*>(ws.m_nerfs.data());
+    const float dt_min = st.min_step_size, dt_max = 8.0f * dt_min;  // reference renderer.cu:305
+    const uint32_t max_steps = std::min<uint32_t>(st.max_samples_per_ray, 1u << 20);
+    uint64_t n_evals = 0;
+    for (uint32_t p = start; p < start + n; p += R) {
+        if (req.is_canceled()) break;
+        const uint32_t m = std::min(start + n - p, R);
+        const uint32_t k = std::max(1u, std::min(S / m, 1024u));  // fixed steps per ray per pass
+        TN_TRY(tn_render_rays_multi(m, p, R, req.camera.c(), n_nerfs, d_nerfs, ws.ray_ori.data(), ws.ray_dir.data(),
+                                    ws.ray_t.data(), ws.ray_tmax.data(), ws.n_steps.data(), ws.pixel.data(),
+                                    ws.alive.data(), ws.m_trans.data(), ts));
+        while (true) {
+            uint32_t alive_h = 0;
+            TN_TRY(tn_count_nonzero(m, ws.alive.data(), ws.m_alive_n.data(), ts));
+            TN_HIP(hipMemcpyAsync(&alive_h, ws.m_alive_n.data(), sizeof(alive_h), hipMemcpyDeviceToHost, s));
+            TN_HIP(hipStreamSynchronize(s));
+            if (alive_h == 0 || req.is_canceled()) break;
+            const uint32_t nq = m * k;
+            TN_TRY(tn_render_march_fixed(m, k, S, NeRFConstants::cone_angle, dt_min, dt_max, max_steps,
+                                         ws.alive.data(), ws.ray_ori.data(), ws.ray_dir.data(), R, ws.ray_t.data(),
+                                         ws.ray_tmax.data(), ws.n_steps.data(), ws.m_pos.data(), ws.m_dt.data(), ts));
+            for (int c = 0; c < 4; ++c)
+                TN_TRY(tn_fill_u32(nq, reinterpret_cast<uint32_t*>(ws.m_rgba.data()) + (size_t)c * S, 0u, ts));
+            TN_TRY(tn_fill_u32(nq, ws.m_count.data(), 0u, ts));
+            for (uint32_t j = 0; j < n_nerfs; ++j) {
+                NeRF& nerf = rs[j]->proxy->nerfs[0];
+                tn_occupancy_grid_t grid = nerf.occupancy_grid.view();
+                TN_TRY(tn_render_localize(m, k, S, nf[j], grid, ws.ray_dir.data(), R, ws.m_pos.data(), ws.m_dt.data(),
+                                          ws.m_valid.data(), ws.m_net_pos.data(), ws.m_net_dir.data(),
+                                          ws.m_net_dt.data(), ts));
+                // the valid samples packed to the front (info[1] = their number)
+                TN_TRY(tn_scan_select(1, nq, nullptr, ws.m_valid.data(), S, ws.m_offs.data(), ws.info.data(),
+                                      ws.m_scan_ws.data(), ws.m_scan_ws.size(), ts));
+                TN_TRY(tn_render_compact(nq, S, ws.m_valid.data(), ws.m_offs.data(), ws.m_net_pos.data(),
+                                         ws.m_net_dir.data(), ws.m_c_pos.data(), ws.m_c_dir.data(), ts));
+                int dtype, raw;
+                if (nerf.analytic_blobs.size()) {
+                    TN_TRY(tn_blob_eval(ws.info.data(), S, S, rs[j]->proxy->training_bbox.get().size(),
+                                        nerf.analytic_blobs.data(), (uint32_t)(nerf.analytic_blobs.size() / 8),
+                                        ws.m_c_pos.data(), reinterpret_cast<float*>(ws.density.data()),
+                                        reinterpret_cast<float*>(ws.color.data()), ts));
+                    dtype = 0;
+                    raw = 0;
+                } else {
+                    nerf.network.inference(s, nerf.params, S, ws.info.data(), S, ws.m_c_pos.data(), ws.m_c_dir.data(),
+                                           ws.features.data(), ws.density.data(), ws.color.data(), true);
+                    dtype = nerf.network.config.precision;
+                    raw = 1;
+                }
+                TN_TRY(tn_render_accumulate(dtype, nq, S, ws.m_valid.data(), ws.m_offs.data(), ws.density.data(),
+                                            ws.color.data(), raw, ws.m_net_dt.data(), ws.m_rgba.data(),
+                                            ws.m_count.data(), ts));
+            }
+            TN_TRY(tn_render_composite_fixed(m, k, S, st.min_transmittance, ws.m_dt.data(), ws.m_rgba.data(),
+                                             ws.m_count.data(), ws.pixel.data(), ws.alive.data(), ws.m_trans.data(),
+                                             image, ts));
+            n_evals += (uint64_t)alive_h * k * n_nerfs;
+        }
+    }
+    return n_evals;
+}
+
 void CPURenderBuffer::save_image(const std::string& file_path) {
     if (width <= 0 || height <= 0 || rgba.size() != (size_t)width * height * 4)
         throw std::runtime_error("save_image: the render buffer holds no image");
@@ -148,22 +274,22 @@ void NeRFRenderingController::submit(std::shared_ptr<RenderRequest> req) {
     auto* cpu = dynamic_cast<CPURenderBuffer*>(req->output);
     if (!cpu) throw std::runtime_error("submit: unsupported render target");
     const uint32_t n_pixels = (uint32_t)W * (uint32_t)H;
-    // the first visible NeRF that can render (scenes of several NeRFs are not
-    // blended in this build)
-    const NeRFRenderable* r = nullptr;
-    int n_visible = 0;
+    // the visible NeRFs that can render: one without spatial effects takes
+    // the occupancy-skipping renderer, several (or effects) the reference's
+    // shared fixed-step loop
+    std::vector<const NeRFRenderable*> rs;
     for (auto& x : req->renderables)
-        if (x.proxy && x.proxy->can_render && x.proxy->is_visible) {
-            if (!r) r = &x;
-            ++n_visible;
-        }
-    if (n_visible > 1) throw std::runtime_error("submit: a request may hold one visible NeRF that can render");
+        if (x.proxy && x.proxy->can_render && x.proxy->is_visible) rs.push_back(&x);
     DeviceBuffer<float> image(4 * (size_t)n_pixels);
     image.zero(s);
     last_sample_count = 0;
-    if (r) {
-        const NeRF& nerf = r->proxy->nerfs[0];
-        const size_t e = nerf.analytic_blobs.size() ? 4 : nerf.network.elem_bytes();
+    if (!rs.empty()) {
+        size_t e = 4;
+        for (const auto* r : rs) {
+            const NeRF& nerf = r->proxy->nerfs[0];
+            e = std::max(e, nerf.analytic_blobs.size() ? (size_t)4 : nerf.network.elem_bytes());
+        }
+        const bool multi = rs.size() > 1 || !rs[0]->spatial_effects.empty() || force_shared_steps;
         // LinearChunks tasks of batch_size/8 rays (reference nerf-rendering-controller.cu:84)
         const uint32_t rays_per_task = std::max(1u, batch_size_ / 8);
         renderer_.ws.enlarge(rays_per_task, std::max(batch_size_ * 4u, 1u << 22), e);
@@ -173,7 +299,8 @@ void NeRFRenderingController::submit(std::shared_ptr<RenderRequest> req) {
         for (uint32_t p = 0; p < n_pixels; p += rays_per_task) {
             if (req->is_canceled()) break;
             const uint32_t m = std::min(rays_per_task, n_pixels - p);
-            last_sample_count += renderer_.render_pixels(s, *req, *r, p, m, image.data(), st);
+            last_sample_count += multi ? renderer_.render_pixels_multi(s, *req, rs, p, m, image.data(), st)
+                                       : renderer_.render_pixels(s, *req, *rs[0], p, m, image.data(), st);
             if (req->on_progress) req->on_progress((float)(p + m) / (float)n_pixels);
         }
     }

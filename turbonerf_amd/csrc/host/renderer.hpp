@@ -54,11 +54,33 @@ struct CPURenderBuffer : RenderTarget {
     void save_image(const std::string& file_path) override;
 };
 
+// Spatial effects applied to a NeRF's samples when it renders (reference
+// src/effects/spatial-effect.cuh, spatial/repeater.cuh:14-40).
+struct SpatialEffect {
+    virtual ~SpatialEffect() = default;
+    // the box the effect lets the NeRF occupy, given the box it had
+    virtual BoundingBox get_bbox(const BoundingBox& bbox) const = 0;
+};
+
+// Repeats the contents of source_bbox to fill extend_bbox: a sample at p is
+// evaluated at ((p - min) mod size + size) mod size + min of the source box
+// (reference repeater.cu:8-76; its transform is stored but not applied).
+struct RepeaterEffect : SpatialEffect {
+    BoundingBox source_bbox, extend_bbox;
+    Transform4f transform;
+    RepeaterEffect(const BoundingBox& src, const BoundingBox& ext, const Transform4f& t)
+        : source_bbox(src), extend_bbox(ext), transform(t) {}
+    BoundingBox get_bbox(const BoundingBox& b) const override;
+};
+
 struct NeRFRenderable {
     NeRFProxy* proxy = nullptr;
     BoundingBox bounding_box;
-    explicit NeRFRenderable(NeRFProxy* p) : proxy(p), bounding_box(p ? p->render_bbox.get() : BoundingBox()) {}
-    NeRFRenderable(NeRFProxy* p, const BoundingBox& b) : proxy(p), bounding_box(b) {}
+    std::vector<std::shared_ptr<SpatialEffect>> spatial_effects;
+    explicit NeRFRenderable(NeRFProxy* p, std::vector<std::shared_ptr<SpatialEffect>> fx = {})
+        : proxy(p), bounding_box(p ? p->render_bbox.get() : BoundingBox()), spatial_effects(std::move(fx)) {}
+    NeRFRenderable(NeRFProxy* p, const BoundingBox& b, std::vector<std::shared_ptr<SpatialEffect>> fx = {})
+        : proxy(p), bounding_box(b), spatial_effects(std::move(fx)) {}
 };
 
 using OnCompleteCallback = std::function<void()>;
@@ -93,7 +115,12 @@ struct RenderingWorkspace {
     DeviceBuffer<uint32_t> s_img;
     DeviceBuffer<float> s_t, s_pos, s_dir, s_dt, s_m, s_dtn;
     DeviceBuffer<uint8_t> features, density, color;
+    // several NeRFs (fixed-step world samples shared by all NeRFs)
+    DeviceBuffer<float> m_trans, m_pos, m_dt, m_net_pos, m_net_dir, m_net_dt, m_c_pos, m_c_dir, m_rgba;
+    DeviceBuffer<uint32_t> m_valid, m_offs, m_count, m_alive_n;
+    DeviceBuffer<uint8_t> m_scan_ws, m_nerfs;
     void enlarge(uint32_t rays, uint32_t samples, size_t elem_bytes);
+    void enlarge_multi(uint32_t rays, uint32_t samples, uint32_t n_nerfs);
 };
 
 struct Renderer {
@@ -110,6 +137,13 @@ struct Renderer {
     // (device, interleaved RGBA). Returns the number of sample evaluations.
     uint64_t render_pixels(hipStream_t s, const RenderRequest& req, const NeRFRenderable& r, uint32_t start, uint32_t n,
                            float* image, const Settings& st);
+    // Several NeRFs (or one with spatial effects) the reference's way
+    // (renderer.cu:344-521): every ray steps through world space with the
+    // fixed step rule, each NeRF evaluates the samples inside its box on
+    // occupied cells, a sample's alphas add (capped at 1) and its colour is
+    // their alpha-weighted mean; front-to-back compositing into `image`.
+    uint64_t render_pixels_multi(hipStream_t s, const RenderRequest& req, const std::vector<const NeRFRenderable*>& rs,
+                                 uint32_t start, uint32_t n, float* image, const Settings& st);
 };
 
 struct NeRFRenderingController {
@@ -119,6 +153,8 @@ struct NeRFRenderingController {
     float min_step_size = NeRFConstants::min_step_size;
     uint32_t max_samples_per_ray = 0xFFFFFFFFu;
     uint64_t last_sample_count = 0;
+    // render a single NeRF through the shared fixed-step loop too (tests)
+    bool force_shared_steps = false;
 
 private:
     RenderPattern pattern_;

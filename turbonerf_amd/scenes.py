@@ -34,11 +34,13 @@ def blob_field(seed: int, n_blobs: int = 8, aabb: float = 1.0, sigma_range=(0.03
     return out
 
 
-def look_at_transform(eye: np.ndarray) -> np.ndarray:
-    """3x4 camera-to-world matrix looking from `eye` at the origin. Columns:
-    right, down, forward (the reference camera looks down its +z column,
-    src/models/camera.cuh:119-132)."""
-    fwd = -eye / np.linalg.norm(eye)
+def look_at_transform(eye: np.ndarray, target=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """3x4 camera-to-world matrix looking from `eye` at `target` (the
+    origin by default). Columns: right, down, forward (the reference camera
+    looks down its +z column, src/models/camera.cuh:119-132)."""
+    eye = np.asarray(eye, float)
+    fwd = np.asarray(target, float) - eye
+    fwd /= np.linalg.norm(fwd)
     up = np.array([0.0, 0.0, 1.0])
     if abs(float(np.dot(fwd, up))) > 0.999:
         up = np.array([0.0, 1.0, 0.0])
@@ -76,6 +78,12 @@ def make_camera(transform34: np.ndarray, width: int, height: int, fov_deg: float
         cam.dist_params[k] = float(dist_params[k])
     cam.show_image_planes = 1
     return cam
+
+
+def look_at_camera(eye, target, width: int, height: int, fov_deg: float, near: float = 0.05,
+                   far: float = 100.0) -> Camera:
+    """A pinhole camera at `eye` looking at `target`."""
+    return make_camera(look_at_transform(eye, target), width, height, fov_deg, near, far)
 
 
 def sphere_cameras(seed: int, n: int, radius: float, width: int, height: int, fov_deg: float,
