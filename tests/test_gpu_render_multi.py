@@ -274,3 +274,37 @@ def test_repeater_equals_moved_copies(cuda):
     assert d.mean() < 1e-6 and d.max() < 5e-4
     for x in [base] + copies:
         mgr.destroy(x)
+
+
+def test_device_render_target_equals_host_target(cuda, tmp_path):
+    """CUDARenderBuffer (reference cuda-render-buffer.cuh:11-42): the render
+    lands in device memory, equal bit for bit to the host target's image,
+    readable in place through its device address, and saved the same way."""
+    import torch
+
+    from turbonerf_amd import PyTurboNeRF as tn
+    from turbonerf_amd import scenes, synthetic
+
+    mgr = tn.NeRFManager()
+    na = _nerf(tn, mgr, scenes.blob_field(61, 8, 1.0), 1.0)
+    nb = _nerf(tn, mgr, scenes.blob_field(62, 8, 1.0), 1.0, _rigid(0.3, [0, 1, 0], [0.2, 0, 0]))
+    cam = synthetic.to_pyturbonerf_cameras(tn, scenes.sphere_cameras(63, 1, 2.2, 40, 30, 50.0))[0]
+    for renderables in ([tn.Renderable(na)], [tn.Renderable(na), tn.Renderable(nb)]):
+        host, dev = tn.CPURenderBuffer(), tn.CUDARenderBuffer()
+        r = tn.Renderer()
+        r.submit(tn.RenderRequest(cam, renderables, host))
+        r.submit(tn.RenderRequest(cam, renderables, dev))
+        h = np.array(host.get_rgba())
+        assert (dev.width, dev.height) == (40, 30) and h[..., 3].max() > 0.1
+        assert np.array_equal(dev.get_rgba(), h)
+        # the device image in place, through its address
+        view = torch.empty(30 * 40 * 4, dtype=torch.float32, device="cuda")
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        assert hip.hipMemcpy(view.data_ptr(), dev.data_ptr, view.numel() * 4, 3) == 0  # device to device
+        assert np.array_equal(view.cpu().numpy().reshape(30, 40, 4), h)
+    host.save_image(str(tmp_path / "h.png"))
+    dev.save_image(str(tmp_path / "d.png"))
+    assert (tmp_path / "h.png").read_bytes() == (tmp_path / "d.png").read_bytes()
+    for x in (na, nb):
+        mgr.destroy(x)

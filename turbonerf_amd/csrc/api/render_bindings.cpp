@@ -68,6 +68,18 @@ void register_renderer_bindings(py::module_& m) {
                                       rb.get_rgba(), self);
         });
 
+    py::class_<CUDARenderBuffer, RenderTarget>(m, "CUDARenderBuffer")
+        .def(py::init<>())
+        .def_property_readonly("data_ptr", [](CUDARenderBuffer& rb) { return reinterpret_cast<uintptr_t>(rb.data()); },
+                               "device address of the interleaved float RGBA image")
+        .def("get_rgba", [](CUDARenderBuffer& rb) {
+            // a host copy (the image itself stays on the device)
+            py::array_t<float> a({(py::ssize_t)rb.height, (py::ssize_t)rb.width, (py::ssize_t)4});
+            if (rb.rgba.size())
+                TN_HIP(hipMemcpy(a.mutable_data(), rb.rgba.data(), rb.rgba.bytes(), hipMemcpyDeviceToHost));
+            return a;
+        });
+
     py::class_<RenderRequest, std::shared_ptr<RenderRequest>>(m, "RenderRequest")
         .def(py::init([](const Camera& cam, std::vector<NeRFRenderable> nerfs, RenderTarget* output,
                          const RenderModifiers& mods, RenderFlags flags, OnCompleteCallback oc, OnProgressCallback op,

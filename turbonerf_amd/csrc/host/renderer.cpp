@@ -245,6 +245,15 @@ uint64_t Renderer::render_pixels_multi(hipStream_t s, const RenderRequest& req,
     return n_evals;
 }
 
+void CUDARenderBuffer::save_image(const std::string& file_path) {
+    if (width <= 0 || height <= 0 || rgba.size() != (size_t)width * height * 4)
+        throw std::runtime_error("save_image: the render buffer holds no image");
+    CPURenderBuffer host;
+    host.set_size(width, height);
+    TN_HIP(hipMemcpy(host.get_rgba(), rgba.data(), rgba.bytes(), hipMemcpyDeviceToHost));
+    host.save_image(file_path);
+}
+
 void CPURenderBuffer::save_image(const std::string& file_path) {
     if (width <= 0 || height <= 0 || rgba.size() != (size_t)width * height * 4)
         throw std::runtime_error("save_image: the render buffer holds no image");
@@ -272,7 +281,8 @@ void NeRFRenderingController::submit(std::shared_ptr<RenderRequest> req) {
     const int W = req->camera.resolution.x, H = req->camera.resolution.y;
     if (req->output->width != W || req->output->height != H) req->output->set_size(W, H);
     auto* cpu = dynamic_cast<CPURenderBuffer*>(req->output);
-    if (!cpu) throw std::runtime_error("submit: unsupported render target");
+    auto* dev = dynamic_cast<CUDARenderBuffer*>(req->output);
+    if (!cpu && !dev) throw std::runtime_error("submit: unsupported render target");
     const uint32_t n_pixels = (uint32_t)W * (uint32_t)H;
     // the visible NeRFs that can render: one without spatial effects takes
     // the occupancy-skipping renderer, several (or effects) the reference's
@@ -280,8 +290,11 @@ void NeRFRenderingController::submit(std::shared_ptr<RenderRequest> req) {
     std::vector<const NeRFRenderable*> rs;
     for (auto& x : req->renderables)
         if (x.proxy && x.proxy->can_render && x.proxy->is_visible) rs.push_back(&x);
-    DeviceBuffer<float> image(4 * (size_t)n_pixels);
-    image.zero(s);
+    // a device target is rendered into in place; a host one through a device image
+    DeviceBuffer<float> staging;
+    if (!dev) staging.resize(4 * (size_t)n_pixels);
+    float* image = dev ? dev->data() : staging.data();
+    TN_HIP(hipMemsetAsync(image, 0, 4 * (size_t)n_pixels * sizeof(float), s));
     last_sample_count = 0;
     if (!rs.empty()) {
         size_t e = 4;
@@ -299,12 +312,13 @@ void NeRFRenderingController::submit(std::shared_ptr<RenderRequest> req) {
         for (uint32_t p = 0; p < n_pixels; p += rays_per_task) {
             if (req->is_canceled()) break;
             const uint32_t m = std::min(rays_per_task, n_pixels - p);
-            last_sample_count += multi ? renderer_.render_pixels_multi(s, *req, rs, p, m, image.data(), st)
-                                       : renderer_.render_pixels(s, *req, *rs[0], p, m, image.data(), st);
+            last_sample_count += multi ? renderer_.render_pixels_multi(s, *req, rs, p, m, image, st)
+                                       : renderer_.render_pixels(s, *req, *rs[0], p, m, image, st);
             if (req->on_progress) req->on_progress((float)(p + m) / (float)n_pixels);
         }
     }
-    TN_HIP(hipMemcpyAsync(cpu->get_rgba(), image.data(), image.bytes(), hipMemcpyDeviceToHost, s));
+    if (cpu)
+        TN_HIP(hipMemcpyAsync(cpu->get_rgba(), image, 4 * (size_t)n_pixels * sizeof(float), hipMemcpyDeviceToHost, s));
     TN_HIP(hipStreamSynchronize(s));
     if (req->is_canceled()) {
         if (req->on_cancel) req->on_cancel();

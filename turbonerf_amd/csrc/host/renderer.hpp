@@ -54,6 +54,23 @@ struct CPURenderBuffer : RenderTarget {
     void save_image(const std::string& file_path) override;
 };
 
+// Output surface in device memory (reference CUDARenderBuffer,
+// src/render-targets/cuda-render-buffer.cuh:11-42): renders write straight
+// into it, with no copy to the host.
+struct CUDARenderBuffer : RenderTarget {
+    DeviceBuffer<float> rgba;
+    void set_size(int w, int h) override {
+        width = w;
+        height = h;
+        rgba.resize((size_t)w * h * 4);
+        rgba.zero(DeviceManager::stream());
+        TN_HIP(hipStreamSynchronize(DeviceManager::stream()));
+    }
+    void free() override { rgba.release(); width = height = 0; }
+    float* data() { return rgba.data(); }
+    void save_image(const std::string& file_path) override;
+};
+
 // Spatial effects applied to a NeRF's samples when it renders (reference
 // src/effects/spatial-effect.cuh, spatial/repeater.cuh:14-40).
 struct SpatialEffect {
