@@ -389,6 +389,13 @@ int tn_render_composite_fixed(uint32_t n_rays, uint32_t k, uint32_t sample_strid
     float min_transmittance, const float* s_dt, const float* s_rgba, const uint32_t* count,
     const uint32_t* pixel, uint8_t* alive, float* trans, float* image, tn_stream_t stream);
 
+/* A preview: the interleaved RGBA image of a grid_w x grid_h grid of
+ * pixels spread over the image, each grid pixel filling the block of image
+ * pixels nearest to it (reference copy_packed_rgba_rectangular_grid_kernel,
+ * src/rendering/ray-batch-coordinators/rectangular-grid-ray-batch-coordinator.cu:62-105). */
+int tn_render_expand_nearest(uint32_t grid_w, uint32_t grid_h, uint32_t width, uint32_t height,
+    const float* grid_rgba, float* rgba, tn_stream_t stream);
+
 /* *out = the number of non-zero bytes in v[0, n) (reference
  * count_valued_elements, src/utils/stream-compaction.cuh:58); p[0, n) = v (in place
  * of cudaMemsetAsync, which a captured graph should not hold). */

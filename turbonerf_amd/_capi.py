@@ -171,6 +171,7 @@ _SIGS: dict[str, tuple] = {
     "tn_render_compact": (C.c_int, [u32, u32, vp, vp, vp, vp, vp, vp, vp]),
     "tn_render_accumulate": (C.c_int, [C.c_int, u32, u32, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]),
     "tn_render_composite_fixed": (C.c_int, [u32, u32, u32, f32, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "tn_render_expand_nearest": (C.c_int, [u32, u32, u32, u32, vp, vp, vp]),
     "tn_count_nonzero": (C.c_int, [u32, vp, vp, vp]),
     "tn_fill_u32": (C.c_int, [u32, vp, u32, vp]),
     "tn_tune_set": (C.c_int, [C.c_int, C.c_int]),

@@ -37,7 +37,11 @@ void register_renderer_bindings(py::module_& m) {
 
     py::enum_<RenderFlags>(m, "RenderFlags", py::arithmetic())
         .value("Preview", RenderFlags::Preview)
-        .value("Final", RenderFlags::Final);
+        .value("Final", RenderFlags::Final)
+        .def(py::self | py::self)
+        .def(py::self & py::self)
+        .def(py::self |= py::self)
+        .def(py::self &= py::self);
 
     py::class_<RenderProperties>(m, "RenderProperties")
         .def(py::init<>())

@@ -274,8 +274,6 @@ void NeRFRenderingController::cancel() {
 
 void NeRFRenderingController::submit(std::shared_ptr<RenderRequest> req) {
     if (!req || !req->output) throw std::runtime_error("submit: request without output");
-    if (pattern_ != RenderPattern::LinearChunks)
-        throw std::runtime_error("submit: only the LinearChunks render pattern is part of this build");
     current_ = req;
     const hipStream_t s = DeviceManager::stream();
     const int W = req->camera.resolution.x, H = req->camera.resolution.y;
@@ -296,6 +294,12 @@ void NeRFRenderingController::submit(std::shared_ptr<RenderRequest> req) {
     float* image = dev ? dev->data() : staging.data();
     TN_HIP(hipMemsetAsync(image, 0, 4 * (size_t)n_pixels * sizeof(float), s));
     last_sample_count = 0;
+    auto publish = [&]() {
+        if (cpu)
+            TN_HIP(hipMemcpyAsync(cpu->get_rgba(), image, 4 * (size_t)n_pixels * sizeof(float), hipMemcpyDeviceToHost,
+                                  s));
+        TN_HIP(hipStreamSynchronize(s));
+    };
     if (!rs.empty()) {
         size_t e = 4;
         for (const auto* r : rs) {
@@ -303,23 +307,52 @@ void NeRFRenderingController::submit(std::shared_ptr<RenderRequest> req) {
             e = std::max(e, nerf.analytic_blobs.size() ? (size_t)4 : nerf.network.elem_bytes());
         }
         const bool multi = rs.size() > 1 || !rs[0]->spatial_effects.empty() || force_shared_steps;
-        // LinearChunks tasks of batch_size/8 rays (reference nerf-rendering-controller.cu:84)
+        // tasks of batch_size/8 rays (reference nerf-rendering-controller.cu:84)
         const uint32_t rays_per_task = std::max(1u, batch_size_ / 8);
         renderer_.ws.enlarge(rays_per_task, std::max(batch_size_ * 4u, 1u << 22), e);
         Renderer::Settings st;
         st.min_step_size = min_step_size;
         st.max_samples_per_ray = max_samples_per_ray;
-        for (uint32_t p = 0; p < n_pixels; p += rays_per_task) {
-            if (req->is_canceled()) break;
-            const uint32_t m = std::min(rays_per_task, n_pixels - p);
-            last_sample_count += multi ? renderer_.render_pixels_multi(s, *req, rs, p, m, image, st)
-                                       : renderer_.render_pixels(s, *req, *rs[0], p, m, image, st);
-            if (req->on_progress) req->on_progress((float)(p + m) / (float)n_pixels);
+        auto render_all = [&](const RenderRequest& rq, uint32_t npx, float* img, bool progress, bool cancelable) {
+            for (uint32_t p = 0; p < npx; p += rays_per_task) {
+                if (cancelable && req->is_canceled()) break;
+                const uint32_t m = std::min(rays_per_task, npx - p);
+                last_sample_count += multi ? renderer_.render_pixels_multi(s, rq, rs, p, m, img, st)
+                                           : renderer_.render_pixels(s, rq, *rs[0], p, m, img, st);
+                if (progress && req->on_progress) req->on_progress((float)(p + m) / (float)npx);
+            }
+        };
+        const bool grid = pattern_ != RenderPattern::LinearChunks;
+        const bool want_preview = (req->flags & RenderFlags::Preview) == RenderFlags::Preview;
+        const bool want_final = (req->flags & RenderFlags::Final) == RenderFlags::Final;
+        const uint32_t n_tasks = (n_pixels + rays_per_task - 1) / rays_per_task + (grid ? 1 : 0);
+        if (grid && want_preview) {
+            // a preview of about 2^14 rays spread over the image (reference
+            // nerf-rendering-controller.cu:85,111-116; the preview task is
+            // not cancelable), each filling its block of pixels; rendered
+            // with the camera scaled to the grid
+            const double aspect = (double)W / (double)H;
+            const uint32_t gh = std::max(1u, std::min((uint32_t)H, (uint32_t)std::lround(std::sqrt(16384.0 / aspect))));
+            const uint32_t gw = std::max(1u, std::min((uint32_t)W, 16384u / gh));
+            Camera pc = req->camera;
+            pc.resolution = int2v{(int)gw, (int)gh};
+            pc.focal_length = float2v{pc.focal_length.x * (float)gw / (float)W, pc.focal_length.y * (float)gh / (float)H};
+            pc.principal_point =
+                float2v{pc.principal_point.x * (float)gw / (float)W, pc.principal_point.y * (float)gh / (float)H};
+            RenderRequest prq(pc, req->renderables, nullptr, req->modifiers, RenderFlags::Preview);
+            DeviceBuffer<float> pimg(4 * (size_t)gw * gh);
+            pimg.zero(s);
+            render_all(prq, gw * gh, pimg.data(), false, false);
+            TN_TRY(tn_render_expand_nearest(gw, gh, (uint32_t)W, (uint32_t)H, pimg.data(), image, as_tn(s)));
+            publish();
+            if (req->on_progress) req->on_progress(1.0f / (float)n_tasks);
+        }
+        if (want_final) {
+            TN_HIP(hipMemsetAsync(image, 0, 4 * (size_t)n_pixels * sizeof(float), s));
+            render_all(*req, n_pixels, image, true, true);
         }
     }
-    if (cpu)
-        TN_HIP(hipMemcpyAsync(cpu->get_rgba(), image, 4 * (size_t)n_pixels * sizeof(float), hipMemcpyDeviceToHost, s));
-    TN_HIP(hipStreamSynchronize(s));
+    publish();
     if (req->is_canceled()) {
         if (req->on_cancel) req->on_cancel();
     } else if (req->on_complete) {

@@ -20,6 +20,15 @@ namespace turbo {
 
 enum class RenderPattern { LinearChunks = 0, HexagonalGrid = 1, RectangularGrid = 2 };
 enum class RenderFlags : int { Preview = 1 << 0, Final = 1 << 1 };
+// flag sets (reference src/utils/render-flags.cuh:12-29)
+inline RenderFlags operator|(RenderFlags a, RenderFlags b) {
+    return static_cast<RenderFlags>(static_cast<int>(a) | static_cast<int>(b));
+}
+inline RenderFlags operator&(RenderFlags a, RenderFlags b) {
+    return static_cast<RenderFlags>(static_cast<int>(a) & static_cast<int>(b));
+}
+inline RenderFlags& operator|=(RenderFlags& a, RenderFlags b) { return a = a | b; }
+inline RenderFlags& operator&=(RenderFlags& a, RenderFlags b) { return a = a & b; }
 
 struct RenderProperties {
     bool show_near_planes = false;

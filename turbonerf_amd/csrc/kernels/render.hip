@@ -410,6 +410,28 @@ extern "C" int tn_render_composite_fixed(uint32_t n_rays, uint32_t k, uint32_t s
     return TN_OK;
 }
 
+// Preview expansion (reference copy_packed_rgba_rectangular_grid_kernel,
+// rectangular-grid-ray-batch-coordinator.cu:62-105): output pixel (x, y)
+// takes grid pixel (x gw / W, y gh / H).
+__global__ void expand_nearest_kernel(uint32_t gw, uint32_t gh, uint32_t W, uint32_t H, const float* __restrict__ in,
+                                      float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * H) return;
+    const uint32_t y = i / W, x = i - y * W;
+    const uint32_t gx = (uint32_t)(((uint64_t)x * gw) / W), gy = (uint32_t)(((uint64_t)y * gh) / H);
+    reinterpret_cast<float4*>(out)[i] = reinterpret_cast<const float4*>(in)[gy * gw + gx];
+}
+
+extern "C" int tn_render_expand_nearest(uint32_t grid_w, uint32_t grid_h, uint32_t width, uint32_t height,
+                                        const float* grid_rgba, float* rgba, hipStream_t stream) {
+    TN_CHECK_ARG(grid_rgba && rgba && grid_w > 0 && grid_h > 0 && (uint64_t)width * height < (1ull << 32));
+    if (width == 0 || height == 0) return TN_OK;
+    expand_nearest_kernel<<<blocks_for((uint64_t)width * height, 256), 256, 0, stream>>>(grid_w, grid_h, width, height,
+                                                                                       grid_rgba, rgba);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
 // *out = number of non-zero bytes of v[0, n) (out is cleared first)
 extern "C" int tn_count_nonzero(uint32_t n, const uint8_t* v, uint32_t* out, hipStream_t stream) {
     TN_CHECK_ARG(v && out);
