@@ -50,6 +50,11 @@ HOST_FLAGS = [
     "-Wall",
 ]
 
+# per-source code generation options: the fused MLP kernels keep their MFMA
+# results in ordinary VGPRs, so the VALU work between layers reads them
+# without a v_accvgpr_read per value
+KERNEL_FLAGS = {"mlp": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
 KERNEL_SOURCES = sorted((CSRC / "kernels").glob("*.hip"))
 HOST_SOURCES = sorted((CSRC / "host").glob("*.cpp"))
 API_SOURCES = sorted((CSRC / "api").glob("*.cpp"))
@@ -78,7 +83,7 @@ def build_kernels(jobs: int = 8) -> Path:
     for src in KERNEL_SOURCES:
         obj = OBJ / (src.stem + ".o")
         objs.append(obj)
-        cmds.append([hipcc, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
+        cmds.append([hipcc, *HIP_FLAGS, *KERNEL_FLAGS.get(src.stem, []), "-c", str(src), "-o", str(obj)])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     out = LIB / "libturbonerf_kernels.so"

@@ -494,59 +494,120 @@ __global__ void __launch_bounds__(1024) hashgrid_bwd_scan_kernel(HbPlan p, const
     }
 }
 
-// (3) records to their bins, in the same order as the count pass, and the
-// block's largest |value| per bin into column j (the block's index within its
-// level) of that bin's row of binmax [n_bins][blocks / L]
+// (3) records to their bins and the block's largest |value| per bin into
+// column j (the block's index within its level) of that bin's row of binmax
+// [n_bins][blocks / L]. A chunk's records (at most 8 per sample) are first
+// sorted by bin in LDS -- per-bin ranks from LDS counters, an exclusive scan
+// of the counts, then one store each to its place -- and then leave in that
+// order, so consecutive lanes write consecutive slots of one bin: the global
+// stores go out as runs of whole lines instead of one 8-byte request per
+// record, whose rate bounded the unsorted version. The order of records
+// within a bin depends on the order of the LDS atomics; the accumulation sums
+// them exactly (fixed point), so the result does not.
+constexpr uint32_t kHbScatterThreads = 256;
+constexpr uint32_t kHbChunkRecords = 8 * kHbScatterThreads;
+static_assert(kHbMaxLevelBins <= 2 * kHbScatterThreads, "two bins per thread in the chunk scan");
 template <typename TG>
-__global__ void __launch_bounds__(256) hashgrid_bwd_scatter_kernel(uint32_t n_max, const uint32_t* __restrict__ info,
-                                                                   uint32_t B, uint32_t GB, tn_hashgrid_t g, HbPlan p,
-                                                                   const float* __restrict__ pos,
-                                                                   const TG* __restrict__ dfeat,
-                                                                   const uint32_t* __restrict__ offsets,
-                                                                   const uint32_t* __restrict__ start,
-                                                                   uint32_t* __restrict__ binmax,
-                                                                   HbRecord* __restrict__ rec, uint32_t l0,
-                                                                   uint32_t Lg) {
-    __shared__ uint32_t cur[kHbMaxLevelBins], bmax[kHbMaxLevelBins];
+__global__ void __launch_bounds__(kHbScatterThreads) hashgrid_bwd_scatter_kernel(
+    uint32_t n_max, const uint32_t* __restrict__ info, uint32_t B, uint32_t GB, tn_hashgrid_t g, HbPlan p,
+    const float* __restrict__ pos, const TG* __restrict__ dfeat, const uint32_t* __restrict__ offsets,
+    const uint32_t* __restrict__ start, uint32_t* __restrict__ binmax, HbRecord* __restrict__ rec, uint32_t l0,
+    uint32_t Lg) {
+    // cur: the bin's next global slot; cnt: the chunk's records per bin;
+    // lstart: the chunk's first LDS place of the bin; gdel: global slot minus
+    // LDS place for the bin's records of the chunk
+    __shared__ uint32_t cur[kHbMaxLevelBins], bmax[kHbMaxLevelBins], cnt[kHbMaxLevelBins];
+    __shared__ uint32_t lstart[kHbMaxLevelBins], gdel[kHbMaxLevelBins];
+    __shared__ uint2 srec[kHbChunkRecords];
+    __shared__ uint32_t sslot[kHbChunkRecords];
+    __shared__ uint32_t wtot[kHbScatterThreads / 64];
     const uint32_t L = (uint32_t)g.n_levels;
     // this launch covers levels [l0, l0 + Lg); block (l, j) is block l + L j
     // of the count pass
     const int l = (int)(l0 + blockIdx.x % Lg);
     const uint32_t j = blockIdx.x / Lg, b = (uint32_t)l + L * j;
     const uint32_t nb = p.count[l];
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
+    const uint32_t tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
+    for (uint32_t k = tid; k < nb; k += kHbScatterThreads) {
         cur[k] = start[p.base[l] + k] + offsets[(uint64_t)b * p.row + k];
         bmax[k] = 0u;
+        cnt[k] = 0u;
     }
-    __syncthreads();
+    const uint32_t off = g.level_offset[l];
     const uint32_t n = info ? min(info[1], n_max) : n_max;
     const uint32_t chunks = (n + 255) / 256, step = p.level_blocks;
     uint32_t c = j;
-    HbIn next = hb_load<TG>(l, c * 256 + threadIdx.x, n, B, GB, pos, dfeat);
+    HbIn next = hb_load<TG>(l, c * 256 + tid, n, B, GB, pos, dfeat);
     for (; c < chunks; c += step) {
-        const uint32_t i = c * 256 + threadIdx.x;
+        __syncthreads();  // cnt cleared, srec and wtot free
+        const uint32_t i = c * 256 + tid;
         const HbIn v = next;
-        next = hb_load<TG>(l, (c + step) * 256 + threadIdx.x, n, B, GB, pos, dfeat);
+        next = hb_load<TG>(l, (c + step) * 256 + tid, n, B, GB, pos, dfeat);
         HbCorners hc;
         hb_corners(g, l, hb_active(i, n, v), v, hc);
-        if (!hc.emit) continue;
-        uint32_t slot[8], r0[8], r1[8];
+        uint32_t bin[8], rank[8], r0[8], r1[8];
+        if (hc.emit) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t bin = (hc.idx[k] - g.level_offset[l]) >> kHbBinLog;
-            slot[k] = atomicAdd(&cur[bin], 1u);
-            r0[k] = hb_round17(hc.a0[k]);
-            r1[k] = hb_round17(hc.a1[k]);
-            const uint32_t m = max(r0[k] & 0x7FFFFFFFu, r1[k] & 0x7FFFFFFFu);  // of the rounded values
-            if (m != 0u) atomicMax(&bmax[bin], m);
+            for (int k = 0; k < 8; ++k) {
+                bin[k] = (hc.idx[k] - off) >> kHbBinLog;
+                rank[k] = atomicAdd(&cnt[bin[k]], 1u);
+                r0[k] = hb_round17(hc.a0[k]);
+                r1[k] = hb_round17(hc.a1[k]);
+                const uint32_t m = max(r0[k] & 0x7FFFFFFFu, r1[k] & 0x7FFFFFFFu);  // of the rounded values
+                if (m != 0u) atomicMax(&bmax[bin[k]], m);
+            }
         }
+        __syncthreads();
+        // exclusive scan of the chunk's counts: thread t takes bins 2t, 2t + 1
+        const uint32_t k0 = 2 * tid, k1 = k0 + 1;
+        const uint32_t c0 = k0 < nb ? cnt[k0] : 0u, c1 = k1 < nb ? cnt[k1] : 0u;
+        uint32_t inc = c0 + c1;
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (slot[k] < p.cap)
-                rec[slot[k]] = hb_pack((hc.idx[k] - g.level_offset[l]) & (kHbBinEntries - 1u), r0[k], r1[k]);
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= (uint32_t)d) inc += o;
+        }
+        if (lane == 63) wtot[wid] = inc;
+        __syncthreads();
+        uint32_t pre = 0, total = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kHbScatterThreads / 64; ++w) {
+            const uint32_t t = wtot[w];
+            pre += w < wid ? t : 0u;
+            total += t;
+        }
+        const uint32_t e0 = pre + inc - c0 - c1;
+        if (k0 < nb) {
+            lstart[k0] = e0;
+            gdel[k0] = cur[k0] - e0;
+            cur[k0] += c0;
+            cnt[k0] = 0u;
+        }
+        if (k1 < nb) {
+            lstart[k1] = e0 + c0;
+            gdel[k1] = cur[k1] - (e0 + c0);
+            cur[k1] += c1;
+            cnt[k1] = 0u;
+        }
+        __syncthreads();
+        if (hc.emit) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t q = lstart[bin[k]] + rank[k];
+                const HbRecord r = hb_pack((hc.idx[k] - off) & (kHbBinEntries - 1u), r0[k], r1[k]);
+                srec[q] = make_uint2(r.a, r.b);
+                sslot[q] = q + gdel[bin[k]];
+            }
+        }
+        __syncthreads();
+        for (uint32_t q = tid; q < total; q += kHbScatterThreads) {
+            const uint32_t slot = sslot[q];
+            const uint2 r = srec[q];
+            if (slot < p.cap) reinterpret_cast<uint2*>(rec)[slot] = r;  // rec is 8-byte aligned
+        }
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x)
+    for (uint32_t k = tid; k < nb; k += kHbScatterThreads)
         binmax[(uint64_t)(p.base[l] + k) * p.level_blocks + j] = bmax[k];
 }
 
@@ -816,7 +877,7 @@ void hashgrid_backward_binned(uint32_t n_max, const uint32_t* info, uint32_t B, 
         // chunks of the group's bins: at most one per bin plus one per kHbChunk records
         const uint64_t cap_g = (uint64_t)n_max * Lg * 8;
         const uint32_t acc_blocks = (uint32_t)((k_hi - k_lo) + (cap_g + kHbChunk - 1) / kHbChunk);
-        hashgrid_bwd_scatter_kernel<TG><<<p.level_blocks * Lg, 256, 0, stream>>>(
+        hashgrid_bwd_scatter_kernel<TG><<<p.level_blocks * Lg, kHbScatterThreads, 0, stream>>>(
             n_max, info, B, GB, g, p, pos, dfeat, ws.offsets, ws.start, ws.binmax, ws.rec, l0, Lg);
         hashgrid_bwd_accumulate_kernel<<<acc_blocks, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax,
                                                                        ws.slot_base, ws.rec, ws.slots, dtable, k_lo,

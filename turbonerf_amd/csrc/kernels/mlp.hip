@@ -236,6 +236,25 @@ TN_DEVICE OpTile<P> to_ops(const f16v& D) {
     return t;
 }
 
+// to_ops(relu(D)). Mixed precision rounds first and takes the maximum with
+// zero on the fp16 pairs (a packed max per two values instead of a max per
+// value and its NaN-quieting twin); fp16 rounding is monotonic and keeps the
+// sign, so the values are those of rounding relu(D), up to the sign of
+// zero, which no later use distinguishes (relu' tests the magnitude bits,
+// and an MFMA operand of -0 adds nothing, as +0 does)
+template <class P>
+TN_DEVICE OpTile<P> relu_ops(const f16v& D) {
+    if constexpr (P::SL == 8) {
+        OpTile<P> t = to_ops<P>(D);
+        const typename P::Op z = {};
+#pragma unroll
+        for (int ks = 0; ks < kKsPerTile<P>; ++ks) t.k[ks] = __builtin_elementwise_max(t.k[ks], z);
+        return t;
+    } else {
+        return to_ops<P>(relu(D));
+    }
+}
+
 // lo part of a tile whose hi part is `hi` (mixed precision only)
 template <class P>
 TN_DEVICE OpTile<P> lo_ops(const f16v& D, const OpTile<P>& hi) {
@@ -354,15 +373,26 @@ TN_DEVICE void load_inputs(Tile<P>& tl, TileLo<P>& lo, bool valid, uint32_t samp
     if (need_sh) {
         float shv[16];
         sh4(dir[sl], dir[sl + B], dir[sl + 2 * B], shv);
-        if (!valid)
-            for (int k = 0; k < 16; ++k) shv[k] = 0.0f;
+        if constexpr (P::SL == 8) {
+            // both halves' operands are formed and the lane half's pair is
+            // selected as values: an array index that depends on h becomes
+            // a compare-and-select chain over all 16 coefficients
+            typename P::Op hi0, lo0, hi1, lo1;
+            P::split(shv, hi0, lo0);
+            P::split(shv + 8, hi1, lo1);
+            const typename P::Op z = {};
+            tl.sh[0] = valid ? (h ? hi1 : hi0) : z;
+            lo.sh[0] = valid ? (h ? lo1 : lo0) : z;
+        } else {
+            if (!valid)
+                for (int k = 0; k < 16; ++k) shv[k] = 0.0f;
 #pragma unroll
-        for (int ks = 0; ks < 16 / P::KC; ++ks) {
-            float v[P::SL];
+            for (int ks = 0; ks < 16 / P::KC; ++ks) {
+                float v[P::SL];
 #pragma unroll
-            for (int j = 0; j < P::SL; ++j) v[j] = shv[nat_in<P>(ks, h, j)];
-            if constexpr (P::SL == 8) P::split(v, tl.sh[ks], lo.sh[ks]);
-            else tl.sh[ks] = P::from(v);
+                for (int j = 0; j < P::SL; ++j) v[j] = shv[nat_in<P>(ks, h, j)];
+                tl.sh[ks] = P::from(v);
+            }
         }
     }
 }
@@ -429,10 +459,10 @@ TN_DEVICE void forward_tile(Tile<P>& tl, const TileLo<P>& lo, const FwdLds<P>& W
             acc = P::mma(a, tl.sh[ks], acc);
             if constexpr (kSplit) acc = P::mma(a, lo.sh[ks], acc);
         }
-        tl.h2[mt] = to_ops<P>(relu(acc));
+        tl.h2[mt] = relu_ops<P>(acc);
     }
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) tl.h3[mt] = to_ops<P>(relu(gemm_op<P, 2>(W.l4, N::L4, mt, tl.h2, lane, N::L4.n_ks)));
+    for (int mt = 0; mt < 2; ++mt) tl.h3[mt] = relu_ops<P>(gemm_op<P, 2>(W.l4, N::L4, mt, tl.h2, lane, N::L4.n_ks));
     tl.o = gemm_op<P, 2>(W.l5, N::L5, 0, tl.h3, lane, N::L5.n_ks);
 }
 
@@ -566,6 +596,35 @@ TN_DEVICE f16v relu_mask(f16v d, const OpTile<P>& act) {
     return d;
 }
 
+// to_ops(relu_mask(d, act)) with the mask applied to the packed operands:
+// in mixed precision an fp16 activation is > 0 exactly when its magnitude
+// bits m are non-zero, and m + 0x7FFF (no carry out of the half, m <= 0x7FFF)
+// has its top bit set exactly then; an arithmetic shift by 15 spreads that
+// bit over the half, giving an AND mask for d's half (four ops per pair of
+// values instead of a convert, a compare and a select per value)
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+template <class P>
+TN_DEVICE OpTile<P> masked_ops(const f16v& d, const OpTile<P>& act) {
+    if constexpr (P::SL == 8) {
+        OpTile<P> t = to_ops<P>(d);
+#pragma unroll
+        for (int ks = 0; ks < kKsPerTile<P>; ++ks) {
+            typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+            u4v dv = __builtin_bit_cast(u4v, t.k[ks]);
+            const u4v av = __builtin_bit_cast(u4v, act.k[ks]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const i16x2 top = __builtin_bit_cast(i16x2, (av[q] & 0x7FFF7FFFu) + 0x7FFF7FFFu);
+                dv[q] &= __builtin_bit_cast(uint32_t, top >> (i16x2){15, 15});
+            }
+            t.k[ks] = __builtin_bit_cast(typename P::Op, dv);
+        }
+        return t;
+    } else {
+        return to_ops<P>(relu_mask<P>(d, act));
+    }
+}
+
 // add tile C (rows 32*mt.., cols 32*nt..) of a [n_out][n_in] gradient into LDS
 TN_DEVICE void add_dw_tile(float* acc, const f16v& C, int mt, int nt, int n_out, int n_in, int lane) {
     const int col = 32 * nt + (lane & 31), h = lane >> 5;
@@ -636,7 +695,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         OpTile<P> dH3[2];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
-            dH3[mt] = to_ops<P>(relu_mask<P>(gemm_dt<P, 1>(WB.b5, N::B5, mt, dOt, lane, N::B5.n_ks), tl.h3[mt]));
+            dH3[mt] = masked_ops<P>(gemm_dt<P, 1>(WB.b5, N::B5, mt, dOt, lane, N::B5.n_ks), tl.h3[mt]);
         put_grp<P>(dbuf, 0, wid, dO, lane);
         put_grp<P>(abuf, 0, wid, tl.h3[0], lane);
         put_grp<P>(abuf, 1, wid, tl.h3[1], lane);
@@ -647,10 +706,11 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         OpTile<P> dH2[2];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
-            const f16v d = relu_mask<P>(gemm_op<P, 2>(WB.b4, N::B4, mt, dH3, lane, N::B4.n_ks), tl.h2[mt]);
-            dH2[mt] = to_ops<P>(d);
+            const f16v g = gemm_op<P, 2>(WB.b4, N::B4, mt, dH3, lane, N::B4.n_ks);
+            dH2[mt] = masked_ops<P>(g, tl.h2[mt]);
             // the appearance bias's gradient per sample (summed per image later)
             if (app.dz && valid) {
+                const f16v d = relu_mask<P>(g, tl.h2[mt]);
 #pragma unroll
                 for (int r = 0; r < 16; ++r) app.dz[(uint32_t)acc_row(mt, r, h) * B + sample] = d[r];
             }
@@ -689,7 +749,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         OpTile<P> dH1[2];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
-            dH1[mt] = to_ops<P>(relu_mask<P>(gemm_dt<P, 1>(WB.b2, N::B2, mt, dSt, lane, N::B2.n_ks), tl.h1[mt]));
+            dH1[mt] = masked_ops<P>(gemm_dt<P, 1>(WB.b2, N::B2, mt, dSt, lane, N::B2.n_ks), tl.h1[mt]);
         put_grp<P>(dbuf, 0, wid, dS, lane);
         put_grp<P>(abuf, 0, wid, tl.h1[0], lane);
         put_grp<P>(abuf, 1, wid, tl.h1[1], lane);
