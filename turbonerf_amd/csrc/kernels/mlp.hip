@@ -529,55 +529,116 @@ TN_DEVICE BwdLds<P> stage_backward(typename P::T* lds, const typename P::T* __re
 // wave w the 32-sample tile w. Each wave recomputes its tile's forward pass,
 // keeps the activations as operand tiles and runs the delta chain down to
 // dfeat. Weight gradients dW = delta * act^T take the samples as K: each
-// layer's deltas and activations of all four tiles go to LDS in
-// [row][sample] layout, and the four waves share the layer's 32x32 dW tiles
-// -- with four tiles (W4) one each, with two tiles (W1, W2, W3, W5) each
-// tile's 128 samples split between two waves -- so a wave accumulates five
-// dW tiles across the batch instead of twelve.
+// layer's deltas and activations of all four tiles go to LDS, and the four
+// waves share the layer's 32x32 dW tiles -- with four tiles (W4) one each,
+// with two tiles (W1, W2, W3, W5) each tile's 128 samples split between two
+// waves -- so a wave accumulates five dW tiles across the batch instead of
+// twelve.
+//
+// The MFMA operands of dW hold 8 samples of one row per lane, the
+// accumulator tiles 16 rows of one sample, so the group buffers transpose.
+// Mixed precision stores a sample's rows contiguously ([sample][row], a
+// lane's four consecutive accumulator rows as one 8-byte write) and reads
+// the operands with gfx950's transposing LDS load ds_read_b64_tr_b16: in
+// each 16-lane group, lane t's element e is the 16-bit value at (address of
+// lane 4e + t/4) + t%4, so lanes that address (sample e, rows 4q..4q+3) give
+// lane 4q + i of the group the row 4q + i of samples 0..3. Plain fp32 keeps
+// [row][sample] buffers written one value per lane.
 constexpr int kGrp = 32 * kMlpWaves;  // samples per workgroup step
 static_assert(kMlpWaves == 4, "the dW split assumes four waves");
 
-// LDS row stride of the group buffers: 136 fp16 (a row's 8 consecutive
-// samples, one MFMA operand, are one aligned 16-byte read), 129 fp32 (the +1
-// spreads banks)
+// fp32 buffers: [row][sample], row stride 129 (the +1 spreads banks)
 template <class P>
-constexpr int kGW = P::SL == 8 ? kGrp + 8 : kGrp + 1;
+constexpr int kGW = kGrp + 1;
+// fp16 buffers: [sample][row], sample stride 100 fp16 = 200 bytes (64 rows
+// and padding): 200 = 8 * 25 with 25 odd puts the 8-byte writes of 32
+// consecutive samples in 32 distinct bank pairs, and the four samples of a
+// transposing read 7 or more 8-byte slots apart
+constexpr int kTrRows = 100;
 
-// tile t of a wave's samples (columns 32w..32w+31); HALF: rows 32t..32t+15
+template <class P>
+constexpr size_t grp_buf_elems() {
+    return P::SL == 8 ? (size_t)kGrp * kTrRows : (size_t)64 * kGW<P>;
+}
+
+typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+// this lane's first element of a [sample][row] buffer: sample 32w + (lane &
+// 31), rows from 4 * (lane half)
+template <class P>
+TN_DEVICE typename P::T* tr_put_base(typename P::T* buf, int w, int lane) {
+    return buf + (32 * w + (lane & 31)) * kTrRows + 4 * ((lane >> 5) & 1);
+}
+
+// tile t of a wave's samples (samples 32w..32w+31); HALF: rows 32t..32t+15
 // only (registers 0..7)
 template <class P, bool HALF = false>
 TN_DEVICE void put_grp(typename P::T* buf, int t, int w, const OpTile<P>& A, int lane) {
-    const int n = 32 * w + (lane & 31), h = (lane >> 5) & 1;
+    if constexpr (P::SL == 8) {
+        // operand k[ks] holds accumulator registers 8ks..8ks+7: rows
+        // 32t + 16ks + 4h + (0..3) and 32t + 16ks + 8 + 4h + (0..3)
+        typename P::T* p = tr_put_base<P>(buf, w, lane) + 32 * t;
 #pragma unroll
-    for (int r = 0; r < (HALF ? 8 : 16); ++r) buf[acc_row(t, r, h) * kGW<P> + n] = (typename P::T)op_get<P>(A, r);
+        for (int ks = 0; ks < (HALF ? 1 : 2); ++ks) {
+            *reinterpret_cast<h4v*>(p + 16 * ks) = __builtin_shufflevector(A.k[ks], A.k[ks], 0, 1, 2, 3);
+            *reinterpret_cast<h4v*>(p + 16 * ks + 8) = __builtin_shufflevector(A.k[ks], A.k[ks], 4, 5, 6, 7);
+        }
+    } else {
+        const int n = 32 * w + (lane & 31), h = (lane >> 5) & 1;
+#pragma unroll
+        for (int r = 0; r < (HALF ? 8 : 16); ++r) buf[acc_row(t, r, h) * kGW<P> + n] = op_get<P>(A, r);
+    }
 }
 
 template <class P>
 TN_DEVICE void put_grp(typename P::T* buf, int t, int w, const f16v& D, int lane) {
-    const int n = 32 * w + (lane & 31), h = (lane >> 5) & 1;
+    if constexpr (P::SL == 8) {
+        put_grp<P>(buf, t, w, to_ops<P>(D), lane);
+    } else {
+        const int n = 32 * w + (lane & 31), h = (lane >> 5) & 1;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) buf[acc_row(t, r, h) * kGW<P> + n] = (typename P::T)D[r];
+        for (int r = 0; r < 16; ++r) buf[acc_row(t, r, h) * kGW<P> + n] = D[r];
+    }
 }
 
 // natural-order operands x[ks] as rows base + ks*KC + h*SL + j
 template <class P, int NKS>
 TN_DEVICE void put_grp_nat(typename P::T* buf, int base, int w, const typename P::Op (&x)[NKS], int lane) {
-    const int n = 32 * w + (lane & 31), h = (lane >> 5) & 1;
+    if constexpr (P::SL == 8) {
+        // rows base + 16ks + 8h + (0..7): two 8-byte writes per K-step (a
+        // sample's rows start 8-byte aligned only)
+        typename P::T* p = tr_put_base<P>(buf, w, lane) + base + 4 * ((lane >> 5) & 1);
 #pragma unroll
-    for (int ks = 0; ks < NKS; ++ks)
-#pragma unroll
-        for (int j = 0; j < P::SL; ++j) {
-            const int row = base + nat_in<P>(ks, h, j);
-            if constexpr (P::SL == 1) buf[row * kGW<P> + n] = x[ks];
-            else buf[row * kGW<P> + n] = x[ks][j];
+        for (int ks = 0; ks < NKS; ++ks) {
+            *reinterpret_cast<h4v*>(p + 16 * ks) = __builtin_shufflevector(x[ks], x[ks], 0, 1, 2, 3);
+            *reinterpret_cast<h4v*>(p + 16 * ks + 4) = __builtin_shufflevector(x[ks], x[ks], 4, 5, 6, 7);
         }
+    } else {
+        const int n = 32 * w + (lane & 31), h = (lane >> 5) & 1;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) buf[(base + nat_in<P>(ks, h, 0)) * kGW<P> + n] = x[ks];
+    }
 }
 
 // operand holding group samples [ks*KC + h*SL, +SL) of row 32*t + (lane & 31)
 template <class P>
 TN_DEVICE typename P::Op grp_rows(const typename P::T* buf, int t, int ks, int lane) {
-    const int row = 32 * t + (lane & 31), h = lane >> 5;
-    return *reinterpret_cast<const typename P::Op*>(buf + row * kGW<P> + ks * P::KC + h * P::SL);
+    if constexpr (P::SL == 8) {
+        // group G = lane / 16 reads rows 32t + 16(G % 2) + (0..15) of samples
+        // 16ks + 8(G / 2) + (0..7); its lane 4e + q addresses sample e (and
+        // e + 4 in the second read), rows 4q..4q+3 of those 16
+        const int G = lane >> 4, jj = lane & 15;
+        const typename P::T* p =
+            buf + (16 * ks + 8 * (G >> 1) + (jj >> 2)) * kTrRows + 32 * t + 16 * (G & 1) + 4 * (jj & 3);
+        typedef s4v __attribute__((address_space(3))) * lds_s4v;
+        const s4v a = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s4v)(p));
+        const s4v b = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s4v)(p + 4 * kTrRows));
+        return __builtin_bit_cast(typename P::Op, __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7));
+    } else {
+        const int row = 32 * t + (lane & 31), h = lane >> 5;
+        return buf[row * kGW<P> + ks * P::KC + h * P::SL];
+    }
 }
 
 // C += delta rows (tile mt) x act rows (tile nt)^T over group K-steps [ks0, ks1)
@@ -651,7 +712,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
     const FwdLds<P> W = stage_forward<P>(lds, params);
     const BwdLds<P> WB = stage_backward<P>(lds + N::kFwdElems, params);
     T* dbuf = lds + N::kFwdElems + N::kBwdElems;  // deltas, up to 64 rows x 128 samples
-    T* abuf = dbuf + 64 * kGW<P>;                 // activations, up to 64 rows x 128 samples
+    T* abuf = dbuf + grp_buf_elems<P>();          // activations, up to 64 rows x 128 samples
     __syncthreads();
 
     const uint32_t n = info ? min(info[1], n_max) : n_max;
@@ -820,7 +881,7 @@ constexpr size_t fwd_lds_bytes() {
 template <class P>
 constexpr size_t bwd_lds_bytes() {
     const size_t w = (size_t)(Net<P>::kFwdElems + Net<P>::kBwdElems) * sizeof(typename P::T);
-    const size_t t = (size_t)2 * 64 * kGW<P> * sizeof(typename P::T);
+    const size_t t = 2 * grp_buf_elems<P>() * sizeof(typename P::T);
     const size_t r = (size_t)kNetParams * sizeof(float);
     return (w + t) > r ? (w + t) : r;
 }
