@@ -350,21 +350,39 @@ struct TileLo {
     typename P::Op sh[16 / P::KC];
 };
 
+// 32-bit loads and stores through a buffer resource over [base, base +
+// bytes): the per-lane byte offset goes in a VGPR and the wave-uniform one
+// (a row times the row stride) in an SGPR, so the row loops below spend no
+// vector arithmetic on 64-bit addresses. Descriptor word 3 of the gfx9
+// family: 32-bit data format, no swizzle, no stride (byte-range checked).
+TN_DEVICE __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+TN_DEVICE float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t uni_off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane_off, (int)uni_off, 0));
+}
+TN_DEVICE void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t uni_off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)lane_off, (int)uni_off, 0);
+}
+
 // Inputs of one tile. Lanes past the batch load sample 0 and zero what they
 // loaded (no divergent branches around the loads); element offsets are
-// 32-bit (the host checks 32 * stride < 2^32).
+// 32-bit (the host checks that 32 rows of fp32 features span under 2^32 bytes).
 template <class P, typename TF>
 TN_DEVICE void load_inputs(Tile<P>& tl, TileLo<P>& lo, bool valid, uint32_t sample, int h,
                            const TF* __restrict__ feat, uint32_t FS, const float* __restrict__ dir, uint32_t B,
                            bool need_sh) {
     const uint32_t sl = valid ? sample : 0u;
+    static_assert(sizeof(TF) == 4, "features are fp32");
+    // row nat_in(ks, h, j) = (ks*KC + j) + h*SL: the first part is uniform
+    const __amdgpu_buffer_rsrc_t fr = buf_rsrc(feat, 32u * FS * 4u);
+    const uint32_t lane_off = ((uint32_t)(h * P::SL) * FS + sl) * 4u;
 #pragma unroll
     for (int ks = 0; ks < 32 / P::KC; ++ks) {
         float v[P::SL];
 #pragma unroll
         for (int j = 0; j < P::SL; ++j) {
-            const uint32_t row = (uint32_t)nat_in<P>(ks, h, j);
-            const float x = (float)feat[row * FS + sl];
+            const float x = buf_ld(fr, lane_off, (uint32_t)(ks * P::KC + j) * FS * 4u);
             v[j] = valid ? x : 0.0f;
         }
         if constexpr (P::SL == 8) P::split(v, tl.x[ks], lo.x[ks]);
@@ -772,8 +790,11 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
             // the appearance bias's gradient per sample (summed per image later)
             if (app.dz && valid) {
                 const f16v d = relu_mask<P>(g, tl.h2[mt]);
+                const __amdgpu_buffer_rsrc_t zr = buf_rsrc(app.dz, 64u * B * 4u);
+                const uint32_t lane_off = ((uint32_t)(4 * h) * B + sample) * 4u;
 #pragma unroll
-                for (int r = 0; r < 16; ++r) app.dz[(uint32_t)acc_row(mt, r, h) * B + sample] = d[r];
+                for (int r = 0; r < 16; ++r)
+                    buf_st(zr, lane_off, (uint32_t)(32 * mt + 8 * (r >> 2) + (r & 3)) * B * 4u, d[r]);
             }
         }
         put_grp<P>(dbuf, 0, wid, dH3[0], lane);
@@ -820,8 +841,11 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         // dX = W1^T dH1 -> feature gradients; dW1 += dH1 x^T
         const f16v dX = gemm_op<P, 2>(WB.b1, N::B1, 0, dH1, lane, N::B1.n_ks);
         if (valid) {
+            // row acc_row(0, r, h) = (8(r/4) + r%4) + 4h: the first part is uniform
+            const __amdgpu_buffer_rsrc_t dr = buf_rsrc(dfeat, 32u * FS * 4u);
+            const uint32_t lane_off = ((uint32_t)(4 * h) * FS + sample) * 4u;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) dfeat[(uint32_t)acc_row(0, r, h) * FS + sample] = (TF)dX[r];
+            for (int r = 0; r < 16; ++r) buf_st(dr, lane_off, (uint32_t)(8 * (r >> 2) + (r & 3)) * FS * 4u, dX[r]);
         }
         put_grp<P>(dbuf, 0, wid, dH1[0], lane);
         put_grp<P>(dbuf, 1, wid, dH1[1], lane);
@@ -946,7 +970,7 @@ extern "C" int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* inf
     const tn_appearance_t app = appearance && color ? *appearance : tn_appearance_t{};
     TN_CHECK_ARG(!app.bias || (app.n_appearances > 0 && (reinterpret_cast<uintptr_t>(app.bias) & 15) == 0));
     TN_CHECK_ARG(n_max <= stride && n_max <= feat_stride && n_max <= out_stride);
-    TN_CHECK_ARG(32ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * out_stride < (1ull << 32));
+    TN_CHECK_ARG(128ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * out_stride < (1ull << 32));
     if (n_max == 0) return TN_OK;
     const unsigned blocks = mlp_blocks(n_max, (unsigned)tune_value(TN_TUNE_MLP_FWD_BLOCKS_PER_CU));
     if (precision == 1) {
@@ -991,7 +1015,8 @@ extern "C" int tn_mlp_backward(int precision, uint32_t n_max, const uint32_t* in
     TN_CHECK_ARG(!app.dz || app.bias);
     TN_CHECK_ARG(!scale_by_rays || info);
     TN_CHECK_ARG(n_max <= stride && n_max <= feat_stride && n_max <= grad_stride);
-    TN_CHECK_ARG(32ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * grad_stride < (1ull << 32));
+    TN_CHECK_ARG(128ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * grad_stride < (1ull << 32));
+    TN_CHECK_ARG(!app.dz || 256ull * stride < (1ull << 32));
     if (workspace_bytes < tn_mlp_backward_workspace_bytes(n_max)) return TN_ERR_OUT_OF_MEMORY;
     if (n_max == 0) return TN_OK;
     const unsigned blocks = mlp_blocks(n_max, (unsigned)tune_value(TN_TUNE_MLP_BWD_BLOCKS_PER_CU));
