@@ -714,6 +714,27 @@ TN_DEVICE void add_dw_tile(float* acc, const f16v& C, int mt, int nt, int n_out,
     }
 }
 
+// sets of group buffers: two in mixed precision; plain fp32 (twice the
+// bytes) fits one in LDS and waits at a second barrier per layer instead
+template <class P>
+constexpr int kGrpSets = P::SL == 8 ? 2 : 1;
+
+// after a layer's dW reads: the other set for the next layer, or (one set)
+// a barrier before it is written again
+template <class P>
+TN_DEVICE void next_layer(typename P::T*& d, typename P::T*& a, typename P::T*& dn, typename P::T*& an) {
+    if constexpr (kGrpSets<P> == 2) {
+        typename P::T* t = d;
+        d = dn;
+        dn = t;
+        t = a;
+        a = an;
+        an = t;
+    } else {
+        __syncthreads();
+    }
+}
+
 template <class P, typename TF, typename TG>
 __global__ void __launch_bounds__(64 * kMlpWaves)
 mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t B, uint32_t FS,
@@ -729,8 +750,18 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
     T* lds = reinterpret_cast<T*>(smem_raw);
     const FwdLds<P> W = stage_forward<P>(lds, params);
     const BwdLds<P> WB = stage_backward<P>(lds + N::kFwdElems, params);
-    T* dbuf = lds + N::kFwdElems + N::kBwdElems;  // deltas, up to 64 rows x 128 samples
-    T* abuf = dbuf + grp_buf_elems<P>();          // activations, up to 64 rows x 128 samples
+    // two sets of group buffers (deltas and activations, up to 64 rows x 128
+    // samples each), used by the layers in turn: a layer's writes go to the
+    // set the layer before last read, and every wave has passed the barrier
+    // of the layer in between only after its reads of that set, so one
+    // barrier per layer orders both the writes before the reads and the
+    // reads before the next writes
+    T* const bufs = lds + N::kFwdElems + N::kBwdElems;
+    constexpr size_t E = grp_buf_elems<P>();
+    T* dbuf = bufs;
+    T* abuf = bufs + E;
+    T* dnext = bufs + (kGrpSets<P> == 2 ? 2 : 0) * E;
+    T* anext = bufs + (kGrpSets<P> == 2 ? 3 : 1) * E;
     __syncthreads();
 
     const uint32_t n = info ? min(info[1], n_max) : n_max;
@@ -780,7 +811,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         put_grp<P>(abuf, 1, wid, tl.h3[1], lane);
         __syncthreads();
         dW5 = dw_tile<P>(dW5, dbuf, abuf, 0, t2, k0, k1, lane);
-        __syncthreads();
+        next_layer<P>(dbuf, abuf, dnext, anext);
         // dH2 = relu'(h2) (W4^T dH3); dW4 += dH3 h2^T
         OpTile<P> dH2[2];
 #pragma unroll
@@ -803,7 +834,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         put_grp<P>(abuf, 1, wid, tl.h2[1], lane);
         __syncthreads();
         dW4 = dw_tile<P>(dW4, dbuf, abuf, wid >> 1, t2, 0, KS, lane);
-        __syncthreads();
+        next_layer<P>(dbuf, abuf, dnext, anext);
         // dXc = W3^T dH2 (rows 0..15 = d density outputs); dW3 += dH2 xc^T,
         // xc = [density outputs (rows 0..15) | SH (rows 16..31)]
         const f16v dXc = gemm_op<P, 2>(WB.b3, N::B3, 0, dH2, lane, N::B3.n_ks);
@@ -813,7 +844,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         put_grp_nat<P, 16 / P::KC>(abuf, 16, wid, tl.sh, lane);
         __syncthreads();
         dW3 = dw_tile<P>(dW3, dbuf, abuf, t2, 0, k0, k1, lane);
-        __syncthreads();
+        next_layer<P>(dbuf, abuf, dnext, anext);
         // density-output gradient: row 0 also receives the loss-scaled
         // compositing (and distortion) gradient, then the whole row is scaled
         // by t^2 where t < 1 (reference nerf-network.cu:734-758)
@@ -837,7 +868,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         put_grp<P>(abuf, 1, wid, tl.h1[1], lane);
         __syncthreads();
         dW2 = dw_tile<P>(dW2, dbuf, abuf, 0, t2, k0, k1, lane);
-        __syncthreads();
+        next_layer<P>(dbuf, abuf, dnext, anext);
         // dX = W1^T dH1 -> feature gradients; dW1 += dH1 x^T
         const f16v dX = gemm_op<P, 2>(WB.b1, N::B1, 0, dH1, lane, N::B1.n_ks);
         if (valid) {
@@ -852,7 +883,7 @@ mlp_backward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t 
         put_grp_nat<P, 32 / P::KC>(abuf, 0, wid, tl.x, lane);
         __syncthreads();
         dW1 = dw_tile<P>(dW1, dbuf, abuf, t2, 0, k0, k1, lane);
-        __syncthreads();
+        next_layer<P>(dbuf, abuf, dnext, anext);
     }
 
     // workgroup sum of the weight gradients (waves in turn, so the sum order
@@ -905,7 +936,7 @@ constexpr size_t fwd_lds_bytes() {
 template <class P>
 constexpr size_t bwd_lds_bytes() {
     const size_t w = (size_t)(Net<P>::kFwdElems + Net<P>::kBwdElems) * sizeof(typename P::T);
-    const size_t t = 2 * grp_buf_elems<P>() * sizeof(typename P::T);
+    const size_t t = 2 * kGrpSets<P> * grp_buf_elems<P>() * sizeof(typename P::T);
     const size_t r = (size_t)kNetParams * sizeof(float);
     return (w + t) > r ? (w + t) : r;
 }
