@@ -353,9 +353,11 @@ MULTI_BATCH = [8192, 16384, 32768, 65536, 131072, 131072, 65536, 73728]
 
 def multi_scene(tn, synthetic, steps):
     """Eight NeRFs trained side by side: every round is one training step of
-    each, launched as one captured HIP graph (MultiTrainer). Aggregate
-    samples/s after 128 warm-up steps each; the same rounds enqueued as eight
-    eager steps one after another are timed for comparison."""
+    each as replays of captured HIP graphs, each NeRF's own on its own
+    streams (MultiTrainer). Aggregate samples/s after 128 warm-up steps each;
+    the same rounds as one graph launch per round (MultiTrainer with
+    one_launch) and as eight eager steps one after another are timed for
+    comparison."""
     assert sum(MULTI_BATCH) == 1 << 19
     mgr = tn.NeRFManager()
     trainers = []
@@ -382,12 +384,15 @@ def multi_scene(tn, synthetic, steps):
     mt.train_steps(2)  # both graphs captured before the clock starts
     rays = [tr.train_steps(0, read_metrics=True).n_rays for tr in trainers]
 
-    def timed(graph):
+    one = tn.MultiTrainer(trainers, one_launch=True)
+    one.train_steps(2)  # both round graphs captured before the clock starts
+
+    def timed(runner):
         mt.synchronize()
         s0 = sum(tr.samples_trained() for tr in trainers)
         t0 = time.perf_counter()
-        if graph:
-            mt.train_steps(steps)
+        if runner is not None:
+            runner.train_steps(steps)
         else:
             for _ in range(steps):
                 for tr in trainers:
@@ -396,19 +401,21 @@ def multi_scene(tn, synthetic, steps):
         wall = time.perf_counter() - t0
         return sum(tr.samples_trained() for tr in trainers) - s0, wall
 
-    samples, wall = timed(True)
-    samples_e, wall_e = timed(False)
+    samples, wall = timed(mt)
+    samples_o, wall_o = timed(one)
+    samples_e, wall_e = timed(None)
     captures = mt._graph_captures
-    del mt
+    del mt, one
     for tr in trainers:
         tr.teardown()
     log(f"[bench multi-scene] {samples / wall / 1e6:.2f} M samples/s over 8 scenes ({1e3 * wall / steps:.3f} ms per "
-        f"round, one graph launch; eight eager steps: {samples_e / wall_e / 1e6:.2f} M samples/s), "
-        f"{samples / steps:.0f} samples per round, rays per scene {rays}, {captures} captures, autotuned "
-        f"{tune['picks']}")
+        f"round, each NeRF's step graphs; one graph launch per round: {samples_o / wall_o / 1e6:.2f}; eight eager "
+        f"steps: {samples_e / wall_e / 1e6:.2f} M samples/s), {samples / steps:.0f} samples per round, rays per "
+        f"scene {rays}, {captures} captures, autotuned {tune['picks']}")
     return dict(samples_per_s=round(samples / wall, 1), ms_per_round=round(1e3 * wall / steps, 4),
-                samples_per_round=round(samples / steps, 1), eager_samples_per_s=round(samples_e / wall_e, 1),
-                rays_per_scene=rays, log2_hashmap_size=MULTI_LOG2_T, batch=MULTI_BATCH, autotune=tune["picks"])
+                samples_per_round=round(samples / steps, 1), one_launch_samples_per_s=round(samples_o / wall_o, 1),
+                eager_samples_per_s=round(samples_e / wall_e, 1), rays_per_scene=rays,
+                log2_hashmap_size=MULTI_LOG2_T, batch=MULTI_BATCH, autotune=tune["picks"])
 
 
 def large_scene(tn, synthetic):

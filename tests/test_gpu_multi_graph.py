@@ -1,13 +1,15 @@
 """Several NeRFs trained side by side (MultiTrainer: one training step of every
-NeRF per round, the round one launch of a captured HIP graph in which the
-NeRFs' steps run concurrently on their own streams) against the same NeRFs
-stepped one after another with eager steps, as the reference's controllers
-are (one NeRFTrainingController per NeRF, src/controllers/
+NeRF per round as replays of captured HIP graphs -- each NeRF's own step
+graphs on its own streams, or with one_launch one launch of a graph holding
+every NeRF's step -- so the NeRFs' steps may run concurrently) against the
+same NeRFs stepped one after another with eager steps, as the reference's
+controllers are (one NeRFTrainingController per NeRF, src/controllers/
 nerf-training-controller.cu:212-260). Three scenes of different seeds; after
 every round the parameters, Adam state and losses agree bit for bit, across
 an occupancy update of one NeRF, eager steps of one NeRF between rounds (the
-NeRFs then sit at different steps and batch slots: the graph is captured
-again) and a settings change of one NeRF."""
+NeRFs then sit at different steps and batch slots: a one-launch graph is
+captured again) and a settings change of one NeRF (its graphs are captured
+again)."""
 import numpy as np
 import pytest
 
@@ -40,13 +42,14 @@ def _same(a, b, tag):
         assert np.array_equal(sa[k].view(np.uint32), sb[k].view(np.uint32)), (tag, k)
 
 
-@pytest.mark.parametrize("precision", [1, 0])
-def test_multi_nerf_graph_rounds_equal_eager_steps(cuda, precision):
+@pytest.mark.parametrize("precision,one_launch", [(1, False), (0, False), (1, True)])
+def test_multi_nerf_graph_rounds_equal_eager_steps(cuda, precision, one_launch):
     from turbonerf_amd import PyTurboNeRF as tn
 
     eager = [_trainer(s, precision) for s in SEEDS]
     multi = [_trainer(s, precision) for s in SEEDS]
-    mt = tn.MultiTrainer([t for _, _, t in multi])
+    mt = tn.MultiTrainer([t for _, _, t in multi], one_launch=one_launch)
+    assert mt.one_launch == one_launch
     for r in range(6):
         if r == 4:  # a settings change of one NeRF
             for _, _, t in (eager[2], multi[2]):
@@ -70,8 +73,13 @@ def test_multi_nerf_graph_rounds_equal_eager_steps(cuda, precision):
             eager[0][2].train_step()
             multi[0][2].train_step()
             _same(eager[0][2], multi[0][2], "eager step between rounds")
-    # captured: both slots in the first round, after the slot shift and after the settings change
-    assert mt._graph_captures == 6
+    if one_launch:
+        # round graphs for both slots in the first round, after the slot shift
+        # and after the settings change
+        assert mt._graph_captures == 6
+    else:
+        # each NeRF's step graphs for both slots, and the changed NeRF's again
+        assert mt._graph_captures == 2 * len(SEEDS) + 2
     for (_, _, te), (_, _, tm) in zip(eager, multi):
         assert te.samples_trained() == tm.samples_trained() > 0
     del mt

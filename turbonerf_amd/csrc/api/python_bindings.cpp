@@ -472,15 +472,17 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
 
     // several NeRFs trained side by side, one captured launch per round
     py::class_<NeRFMultiTrainingController>(m, "MultiTrainer")
-        .def(py::init([](const std::vector<std::shared_ptr<NeRFTrainingController>>& ts) {
+        .def(py::init([](const std::vector<std::shared_ptr<NeRFTrainingController>>& ts, bool one_launch) {
                  std::vector<NeRFTrainingController*> raw;
                  for (const auto& t : ts) raw.push_back(t.get());
-                 return std::make_unique<NeRFMultiTrainingController>(raw);
+                 return std::make_unique<NeRFMultiTrainingController>(raw, one_launch);
              }),
-             py::arg("trainers"), py::keep_alive<1, 2>())
+             py::arg("trainers"), py::arg("one_launch") = false, py::keep_alive<1, 2>())
         .def("train_steps", &NeRFMultiTrainingController::train_steps, py::arg("n_rounds"),
-             "One training step of every NeRF per round, each round one graph launch.",
+             "One training step of every NeRF per round, as replays of captured graphs (each NeRF's own, or "
+             "with one_launch one graph per round).",
              py::call_guard<py::gil_scoped_release>())
+        .def_readonly("one_launch", &NeRFMultiTrainingController::one_launch)
         .def("synchronize", &NeRFMultiTrainingController::synchronize, py::call_guard<py::gil_scoped_release>())
         .def("samples_trained", &NeRFMultiTrainingController::samples_trained)
         .def(

@@ -322,8 +322,9 @@ void NeRFTrainingController::synchronize() {
 }
 
 // ------------------------------------------------------------------ several NeRFs
-NeRFMultiTrainingController::NeRFMultiTrainingController(std::vector<NeRFTrainingController*> trainers)
-    : trainers_(std::move(trainers)) {
+NeRFMultiTrainingController::NeRFMultiTrainingController(std::vector<NeRFTrainingController*> trainers,
+                                                         bool one_launch_)
+    : one_launch(one_launch_), trainers_(std::move(trainers)) {
     if (trainers_.empty()) throw std::runtime_error("NeRFMultiTrainingController: no trainers");
     for (size_t i = 0; i < trainers_.size(); ++i) {
         if (!trainers_[i]) throw std::runtime_error("NeRFMultiTrainingController: null trainer");
@@ -387,6 +388,32 @@ void NeRFMultiTrainingController::capture(int slot, const std::vector<uint64_t>&
 
 void NeRFMultiTrainingController::train_steps(uint32_t n_rounds) {
     if (n_rounds == 0) return;
+    if (one_launch) {
+        train_steps_one_launch(n_rounds);
+        return;
+    }
+    // every NeRF replays its own captured step graphs on its own streams,
+    // the NeRFs in turn within a round
+    std::vector<uint32_t> caps0;
+    for (auto* t : trainers_) {
+        t->sync_settings();
+        t->begin_work();
+        t->trainer_.begin_graph_steps(*t->ctx_, t->proxy->training_step);
+        caps0.push_back(t->ctx_->n_captures);
+    }
+    for (uint32_t r = 0; r < n_rounds; ++r)
+        for (auto* t : trainers_) {
+            t->trainer_.enqueue_graph_step(*t->ctx_);
+            t->proxy->training_step++;
+            if (!t->proxy->can_render) t->proxy->can_render = true;
+        }
+    for (size_t k = 0; k < trainers_.size(); ++k) {
+        trainers_[k]->end_work();
+        n_captures += trainers_[k]->ctx_->n_captures - caps0[k];
+    }
+}
+
+void NeRFMultiTrainingController::train_steps_one_launch(uint32_t n_rounds) {
     for (size_t k = 0; k < trainers_.size(); ++k) {
         NeRFTrainingController& t = *trainers_[k];
         t.sync_settings();

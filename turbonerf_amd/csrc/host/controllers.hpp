@@ -165,23 +165,29 @@ private:
 
 // Several NeRFs trained side by side (reference: one training controller per
 // NeRF, stepped in turn). Each round enqueues one training step of every NeRF
-// as one launch of a HIP graph that holds every NeRF's captured step as an
-// independent child graph, so the steps may run concurrently; results are
-// those of stepping the controllers one after another. A controller may
-// still be stepped on its own between rounds.
+// as replays of captured HIP graphs, so the NeRFs' steps may run
+// concurrently; results are those of stepping the controllers one after
+// another. By default every NeRF replays its own step graphs on its own
+// streams (two launches per NeRF and round); with one_launch the round is
+// one launch of a graph holding every NeRF's captured step as an independent
+// child graph (slower on this runtime: its child-graph launches leave longer
+// gaps between kernels). A controller may still be stepped on its own
+// between rounds.
 class NeRFMultiTrainingController {
 public:
-    explicit NeRFMultiTrainingController(std::vector<NeRFTrainingController*> trainers);
+    explicit NeRFMultiTrainingController(std::vector<NeRFTrainingController*> trainers, bool one_launch = false);
     ~NeRFMultiTrainingController();
     NeRFMultiTrainingController(const NeRFMultiTrainingController&) = delete;
     NeRFMultiTrainingController& operator=(const NeRFMultiTrainingController&) = delete;
     void train_steps(uint32_t n_rounds);
     void synchronize();
     uint64_t samples_trained();  // summed over the NeRFs (synchronises)
-    uint32_t n_captures = 0;
+    uint32_t n_captures = 0;  // graphs captured for the rounds so far
+    const bool one_launch;
 
 private:
     std::vector<NeRFTrainingController*> trainers_;
+    void train_steps_one_launch(uint32_t n_rounds);
     hipStream_t root_ = nullptr;
     std::vector<hipEvent_t> ev_pre_;
     hipGraphExec_t graph_[2] = {nullptr, nullptr};

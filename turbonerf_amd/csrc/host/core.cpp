@@ -528,7 +528,12 @@ void Trainer::instantiate_graphs(Context& ctx, int c) {
 
 void Trainer::enqueue_train_steps_graph(Context& ctx, uint32_t first_step, uint32_t n) {
     if (n == 0) return;
-    const hipStream_t net = ctx.stream, gen = ctx.gen_stream;
+    begin_graph_steps(ctx, first_step);
+    for (uint32_t k = 0; k < n; ++k) enqueue_graph_step(ctx);
+}
+
+void Trainer::begin_graph_steps(Context& ctx, uint32_t first_step) {
+    const hipStream_t net = ctx.stream;
     // a captured step may not allocate: size every buffer first
     ctx.nerf->network.prepare_workspace(ctx.batches[0].batch_size, ctx.batches[0].ray_capacity);
     const std::vector<uint64_t> key = graph_key(ctx);
@@ -540,27 +545,29 @@ void Trainer::enqueue_train_steps_graph(Context& ctx, uint32_t first_step, uint3
     TN_HIP(hipStreamSynchronize(net));  // the pinned staging value is free
     ctx.step_host[0] = first_step;
     TN_HIP(hipMemcpyAsync(ctx.d_step.data(), ctx.step_host.data(), sizeof(uint32_t), hipMemcpyHostToDevice, net));
-    for (uint32_t k = 0; k < n; ++k) {
-        const int c = ctx.cur, nx = c ^ 1;
-        if (!ctx.prefetched) {
-            TN_HIP(hipStreamWaitEvent(gen, ctx.ev_grid, 0));
-            TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[c], 0));
-            generate_next_training_batch(ctx, ctx.batches[c], gen, &ctx.timer);
-            TN_HIP(hipEventRecord(ctx.ev_ready[c], gen));
-        }
-        if (!ctx.train_graph[c]) instantiate_graphs(ctx, c);
-        // the next batch, once its last training has consumed it
-        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[nx], 0));
-        TN_HIP(hipGraphLaunch(ctx.gen_graph[nx], gen));
-        TN_HIP(hipEventRecord(ctx.ev_ready[nx], gen));
-        // this batch, once it is ready
-        TN_HIP(hipStreamWaitEvent(net, ctx.ev_ready[c], 0));
-        TN_HIP(hipGraphLaunch(ctx.train_graph[c], net));
-        TN_HIP(hipEventRecord(ctx.ev_consumed[c], net));
-        ctx.last = c;
-        ctx.cur = nx;
-        ctx.prefetched = true;
+}
+
+void Trainer::enqueue_graph_step(Context& ctx) {
+    const hipStream_t net = ctx.stream, gen = ctx.gen_stream;
+    const int c = ctx.cur, nx = c ^ 1;
+    if (!ctx.prefetched) {
+        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_grid, 0));
+        TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[c], 0));
+        generate_next_training_batch(ctx, ctx.batches[c], gen, &ctx.timer);
+        TN_HIP(hipEventRecord(ctx.ev_ready[c], gen));
     }
+    if (!ctx.train_graph[c]) instantiate_graphs(ctx, c);
+    // the next batch, once its last training has consumed it
+    TN_HIP(hipStreamWaitEvent(gen, ctx.ev_consumed[nx], 0));
+    TN_HIP(hipGraphLaunch(ctx.gen_graph[nx], gen));
+    TN_HIP(hipEventRecord(ctx.ev_ready[nx], gen));
+    // this batch, once it is ready
+    TN_HIP(hipStreamWaitEvent(net, ctx.ev_ready[c], 0));
+    TN_HIP(hipGraphLaunch(ctx.train_graph[c], net));
+    TN_HIP(hipEventRecord(ctx.ev_consumed[c], net));
+    ctx.last = c;
+    ctx.cur = nx;
+    ctx.prefetched = true;
 }
 
 void Trainer::read_metrics(Context& ctx, float& loss) {

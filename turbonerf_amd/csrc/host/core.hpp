@@ -334,6 +334,12 @@ struct Trainer {
     // launch per step, the next batch generated inside it); training steps
     // first_step .. first_step + n - 1. Same results as enqueue_train_step.
     void enqueue_train_steps_graph(Context& ctx, uint32_t first_step, uint32_t n);
+    // enqueue_train_steps_graph in two parts, for callers that interleave the
+    // steps of several trainers: begin_graph_steps once (workspace sizes, the
+    // captured graphs' key, the device step counter; synchronises the
+    // trainer's network stream), then enqueue_graph_step once per step.
+    void begin_graph_steps(Context& ctx, uint32_t first_step);
+    void enqueue_graph_step(Context& ctx);
     // One training step, synchronising like the reference: throws when the
     // batch has no rays/samples, returns the loss (reference trainer.cu:313-349).
     float train_step(Context& ctx, uint32_t training_step);
