@@ -42,15 +42,21 @@ ALGO = {
     "march_count": ("hbm", 0, 38, 0),
     "scan_select": ("hbm", 0, 8, 0),
     "march_generate": ("hbm", 44, 38, 0),
-    "hashgrid_forward": ("hbm", 588, 0, 0),
+    # positions 12 B + 16 levels x 8 corners x the table's feature pair
+    # (4 B fp16, 8 B fp32) + 32 fp32 features out (128 B)
+    "hashgrid_forward": ("hbm", 12 + 512 + 128, 0, 0),
     "mlp_forward": ("mfma", 18816, 0, 0),
     "composite": ("hbm", 34, 60, 0),
     "distortion": ("hbm", 24, 8, 0),
     "mlp_backward": ("mfma", 56448 - 18816, 0, 0),
-    "hashgrid_backward": ("hbm", 1100, 0, 0),
+    # positions 12 B + 32 fp32 feature gradients 128 B + 16 levels x 8
+    # corners x the fp32 gradient pair read and written (8 B)
+    "hashgrid_backward": ("hbm", 12 + 128 + 1024, 0, 0),
     "adam": ("hbm", 0, 0, 36),
     "loss_reduce": ("hbm", 0, 16, 0),
 }
+# phases whose bytes depend on the network precision (fp32 tables)
+ALGO_FP32 = {"hashgrid_forward": ("hbm", 12 + 1024 + 128, 0, 0)}
 # the kernels each phase launches (rocprofv3 kernel names)
 KERNEL = {
     "hashgrid_forward": ("hashgrid_fwd_kernel",),
@@ -256,7 +262,7 @@ def roofline(res, n_params, dtype):
     for ph, v in res["phases"].items():
         if ph not in ALGO:
             continue
-        bound, ps, pr, pp = ALGO[ph]
+        bound, ps, pr, pp = ALGO_FP32.get(ph, ALGO[ph]) if dtype == "fp32" else ALGO[ph]
         units = ps * S + pr * R + pp * n_params
         sec = v["ms"] * 1e-3
         if bound == "hbm":
