@@ -77,6 +77,85 @@ struct Pair<_Float16> {
     TN_DEVICE static float b(half2v v) { return (float)v.y; }
 };
 
+// Buffer loads of one entry or of the aligned pair of entries e & ~1, e | 1.
+// A byte offset at or past the resource's size reads zeros without a memory
+// request (the buffer range check), so a lane that needs no load skips it
+// without a branch.
+template <typename TT>
+struct Gather;
+template <>
+struct Gather<_Float16> {
+    using V = half2v;
+    typedef uint32_t W __attribute__((ext_vector_type(2)));  // a pair of entries
+    TN_DEVICE static W pair(__amdgpu_buffer_rsrc_t r, uint32_t e) {
+        return __builtin_amdgcn_raw_buffer_load_b64(r, (int)((e >> 1) * 8u), 0, 0);
+    }
+    // (the component is copied out first: __builtin_bit_cast of a vector
+    // component read the vector's first element in this compiler)
+    TN_DEVICE static V even(const W& w) {
+        const uint32_t x = w.x;
+        return __builtin_bit_cast(V, x);
+    }
+    TN_DEVICE static V odd(const W& w) {
+        const uint32_t y = w.y;
+        return __builtin_bit_cast(V, y);
+    }
+    TN_DEVICE static V one(__amdgpu_buffer_rsrc_t r, uint32_t e, bool need) {
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, need ? (int)(e * 4u) : -1, 0, 0));
+    }
+};
+template <>
+struct Gather<float> {
+    using V = float2;
+    typedef uint32_t W __attribute__((ext_vector_type(4)));
+    TN_DEVICE static W pair(__amdgpu_buffer_rsrc_t r, uint32_t e) {
+        return __builtin_amdgcn_raw_buffer_load_b128(r, (int)((e >> 1) * 16u), 0, 0);
+    }
+    TN_DEVICE static V even(const W& w) { return make_float2(__uint_as_float(w.x), __uint_as_float(w.y)); }
+    TN_DEVICE static V odd(const W& w) { return make_float2(__uint_as_float(w.z), __uint_as_float(w.w)); }
+    TN_DEVICE static V one(__amdgpu_buffer_rsrc_t r, uint32_t e, bool need) {
+        typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+        const u2v w = __builtin_amdgcn_raw_buffer_load_b64(r, need ? (int)(e * 8u) : -1, 0, 0);
+        return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+    }
+};
+
+// The eight corner entries of a sample-level, read with fewer requests: the
+// corners (0, y, z) and (1, y, z) are adjacent entries when the lower one is
+// even -- a dense level's x + 1, or a hashed level's even x, whose x + 1 only
+// sets the hash's lowest bit -- so one load of the aligned entry pair
+// holding the lower corner also yields the upper one; the upper corner is
+// loaded on its own only where it lies elsewhere. Same values, and the
+// trilinear sums keep their order. `table` spans `bytes` (< 2^32). The
+// pairs pass through an empty asm statement once every load is issued:
+// without it the compiler narrows each pair load to the half it sees used
+// and loads the other corner separately again (eight loads).
+template <typename TT>
+TN_DEVICE void gather_corners(const TT* __restrict__ table, uint32_t bytes, const HashCorners& hc,
+                              typename Pair<TT>::V (&v)[8]) {
+    using V = typename Pair<TT>::V;
+    using W = typename Gather<TT>::W;
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<TT*>(table), 0, (int)bytes, 0x00020000);
+    W w[4];
+    V up[4];
+    bool own[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = Gather<TT>::pair(r, hc.idx[2 * q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        own[q] = hc.idx[2 * q + 1] != (hc.idx[2 * q] ^ 1u);
+        up[q] = Gather<TT>::one(r, hc.idx[2 * q + 1], own[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        asm volatile("" : "+v"(w[q]));
+        const V even = Gather<TT>::even(w[q]), odd = Gather<TT>::odd(w[q]);
+        const bool odd0 = (hc.idx[2 * q] & 1u) != 0u;
+        v[2 * q] = odd0 ? odd : even;
+        v[2 * q + 1] = own[q] ? up[q] : (odd0 ? even : odd);
+    }
+}
+
 template <typename TT, typename TO>
 __global__ void __launch_bounds__(256) hashgrid_fwd_kernel(uint32_t n_max, const uint32_t* __restrict__ info,
                                                            uint32_t B, uint32_t OB, tn_hashgrid_t g,
@@ -88,12 +167,15 @@ __global__ void __launch_bounds__(256) hashgrid_fwd_kernel(uint32_t n_max, const
     if (i >= n) return;
     HashCorners hc;
     hash_corners(g, l, pos[i], pos[i + B], pos[i + 2 * (uint64_t)B], hc);
+    // the table's entries (every corner index lies below: levels wrap their indices)
+    const uint32_t n_entries = g.level_offset[g.n_levels - 1] + g.level_size[g.n_levels - 1];
+    typename Pair<TT>::V v[8];
+    gather_corners<TT>(table, n_entries * (uint32_t)(2 * sizeof(TT)), hc, v);
     float f0 = 0.0f, f1 = 0.0f;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        const auto v = Pair<TT>::ld(table, hc.idx[c]);
-        f0 += hc.w[c] * Pair<TT>::a(v);
-        f1 += hc.w[c] * Pair<TT>::b(v);
+        f0 += hc.w[c] * Pair<TT>::a(v[c]);
+        f1 += hc.w[c] * Pair<TT>::b(v[c]);
     }
     out[(uint64_t)(2 * l) * OB + i] = (TO)f0;
     out[(uint64_t)(2 * l + 1) * OB + i] = (TO)f1;
