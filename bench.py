@@ -60,9 +60,11 @@ ALGO_FP32 = {"hashgrid_forward": ("hbm", 12 + 1024 + 128, 0, 0)}
 # the kernels each phase launches (rocprofv3 kernel names)
 KERNEL = {
     "hashgrid_forward": ("hashgrid_fwd_kernel",),
-    "hashgrid_backward": ("hashgrid_bwd_count_kernel", "hashgrid_bwd_reduce_kernel", "hashgrid_bwd_scan_kernel",
-                          "hashgrid_bwd_scatter_kernel", "hashgrid_bwd_accumulate_kernel",
+    # the plan (count, reduce, scan) is formed with the batch on the
+    # generation stream; the network stream's phase runs the gradient passes
+    "hashgrid_backward": ("hashgrid_bwd_scatter_kernel", "hashgrid_bwd_accumulate_kernel",
                           "hashgrid_bwd_finish_kernel"),
+    "hashgrid_plan": ("hashgrid_bwd_count_kernel", "hashgrid_bwd_reduce_kernel", "hashgrid_bwd_scan_kernel"),
     "mlp_forward": ("mlp_forward_kernel",),
     "mlp_backward": ("mlp_backward_kernel", "reduce_partials_kernel"),
     "composite": ("composite_chunk_kernel", "composite_ray_kernel", "composite_grad_kernel"),

@@ -246,6 +246,25 @@ int tn_hashgrid_backward_binned(int dfeat_dtype, uint32_t n_max, const uint32_t*
     uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos, const void* dfeat, float* dtable,
     void* workspace, size_t workspace_bytes, tn_stream_t stream);
 
+/* tn_hashgrid_backward_binned in two calls (same result bit for bit), for a
+ * caller that has the positions before the gradients (the reference's
+ * trainer does: the batch's sample positions at trainer.cu:184-216, the
+ * gradients at nerf-network.cu:760). tn_hashgrid_backward_plan forms, from
+ * the positions alone, the record counts and offsets of every bin into
+ * `plan` (tn_hashgrid_backward_plan_bytes, 256-byte aligned); the trainer
+ * runs it on its batch-generation stream. tn_hashgrid_backward_planned then
+ * runs the passes that need the gradients with that plan and a work area
+ * (tn_hashgrid_backward_work_bytes, 256-byte aligned). Both take the same
+ * batch (n_max, info, stride, grid, pos); 0 bytes = layout beyond the bin
+ * limits. */
+size_t tn_hashgrid_backward_plan_bytes(uint32_t n_max, tn_hashgrid_t grid);
+size_t tn_hashgrid_backward_work_bytes(uint32_t n_max, tn_hashgrid_t grid);
+int tn_hashgrid_backward_plan(uint32_t n_max, const uint32_t* info, uint32_t stride, tn_hashgrid_t grid,
+    const float* pos, void* plan, size_t plan_bytes, tn_stream_t stream);
+int tn_hashgrid_backward_planned(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
+    uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos, const void* dfeat, float* dtable,
+    const void* plan, size_t plan_bytes, void* work, size_t work_bytes, tn_stream_t stream);
+
 
 /* ------------------------------------------------------------------- MLPs */
 

@@ -4,7 +4,12 @@ kernel, on the default 16-level layout (T = 2^19):
 
 * samples along rays (runs of equal coarse entries merged before sorting),
   fp32 and fp16 feature gradients, a share of samples with zero gradient
-  (skipped), batch size from the device count (info);
+  (their records carry zeros: the record counts follow from the positions
+  alone), batch size from the device count (info);
+* the same in two calls, the plan from the positions first
+  (tn_hashgrid_backward_plan) and then the gradient passes
+  (tn_hashgrid_backward_planned), as the trainer runs them: equal to the
+  one-call backward bit for bit;
 * random points (no merging: level 0's single bin holds 16 chunks of
   records, summed in parts);
 * accumulation into a non-zero table, values spanning 1e-6..1e3 (the
@@ -223,5 +228,35 @@ def test_binned_level_groups_and_blocks_give_the_same_bits(cuda, groups, level_b
             runs.append(got.cpu().numpy())
     finally:
         assert lib.tn_tune_set(6, 1) == 0 and lib.tn_tune_set(1, 128) == 0
+    assert np.abs(runs[0]).max() > 0
+    assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_planned_equals_one_call(cuda, half):
+    """Plan from the positions, then the gradient passes: the one-call binned
+    backward's gradient bit for bit (rays with merged runs, random points, a
+    share of zero gradients, the batch bounded by a device count)."""
+    import torch
+
+    from turbonerf_amd import ops
+
+    rng = np.random.default_rng(18)
+    g = ops.hashgrid_layout()
+    pos = np.concatenate([_rays(rng, 512, 64, 0.004), rng.uniform(0, 1, size=(3, 16384)).astype(np.float32)], axis=1)
+    n = pos.shape[1]
+    df = rng.normal(0.0, 1.0, size=(32, n)).astype(np.float32)
+    df[:, rng.uniform(size=n) < 0.3] = 0.0
+    pos_d = torch.from_numpy(np.ascontiguousarray(pos)).cuda().view(-1)
+    df_d = torch.from_numpy(df).cuda().view(-1)
+    if half:
+        df_d = df_d.half()
+    info = torch.tensor([0, n - 1000, 0, 0], dtype=torch.int32, device="cuda")
+    runs = []
+    for planned in (False, True):
+        got = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
+        ops.hashgrid_backward(g, n, pos_d, n, df_d, got, info=info, binned=not planned, planned=planned)
+        runs.append(got.cpu().numpy())
+    print(f"[hash bwd planned half={half}] touched {np.count_nonzero(runs[0])}", flush=True)
     assert np.abs(runs[0]).max() > 0
     assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))

@@ -150,6 +150,11 @@ _SIGS: dict[str, tuple] = {
     "tn_hashgrid_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp]),
     "tn_hashgrid_backward_workspace_bytes": (C.c_size_t, [u32, HashGrid]),
     "tn_hashgrid_backward_binned": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp, C.c_size_t, vp]),
+    "tn_hashgrid_backward_plan_bytes": (C.c_size_t, [u32, HashGrid]),
+    "tn_hashgrid_backward_work_bytes": (C.c_size_t, [u32, HashGrid]),
+    "tn_hashgrid_backward_plan": (C.c_int, [u32, vp, u32, HashGrid, vp, vp, C.c_size_t, vp]),
+    "tn_hashgrid_backward_planned": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp, C.c_size_t, vp,
+                                               C.c_size_t, vp]),
     "tn_mlp_forward": (C.c_int, [C.c_int, u32, vp, u32, u32, u32, vp, u32, vp, vp, C.c_int, vp, vp, vp, vp]),
     "tn_mlp_backward_workspace_bytes": (sz, [u32]),
     "tn_mlp_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, vp, u32, vp, vp, vp, vp, u32, vp, vp, f32, C.c_int, vp,

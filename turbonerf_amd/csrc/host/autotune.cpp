@@ -11,11 +11,13 @@ namespace turbo {
 
 namespace {
 
-// order of the coordinate search: the knobs of the longest kernels first
-constexpr int kOrder[] = {TN_TUNE_HASH_BWD_LEVEL_GROUPS, TN_TUNE_HASH_BWD_LEVEL_BLOCKS,
-                          TN_TUNE_MLP_BWD_BLOCKS_PER_CU, TN_TUNE_MLP_FWD_BLOCKS_PER_CU,
-                          TN_TUNE_HASH_FWD_BLOCK,        TN_TUNE_MARCH_MAX_BLOCKS,
-                          TN_TUNE_ADAM_BLOCK};
+// order of the coordinate search: the knobs of the longest kernels first.
+// The hash-grid backward's workgroups per level are not searched: the
+// trainer forms each batch's backward plan ahead with the batch, at a fixed
+// 256 per level (the knob shapes the one-call tn_hashgrid_backward_binned).
+constexpr int kOrder[] = {TN_TUNE_HASH_BWD_LEVEL_GROUPS, TN_TUNE_MLP_BWD_BLOCKS_PER_CU,
+                          TN_TUNE_MLP_FWD_BLOCKS_PER_CU, TN_TUNE_HASH_FWD_BLOCK,
+                          TN_TUNE_MARCH_MAX_BLOCKS,      TN_TUNE_ADAM_BLOCK};
 
 int get(int knob) {
     int v = 0;

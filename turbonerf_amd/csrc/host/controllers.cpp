@@ -364,6 +364,7 @@ void NeRFMultiTrainingController::capture(int slot, const std::vector<uint64_t>&
     graph_[slot] = nullptr;
     hipGraph_t parent = nullptr;
     TN_HIP(hipGraphCreate(&parent, 0));
+    for (auto* t : trainers_) t->trainer_.prepare_plans(*t->ctx_);
     try {
         for (auto* t : trainers_) {
             Trainer::Context& ctx = *t->ctx_;
@@ -422,6 +423,7 @@ void NeRFMultiTrainingController::train_steps_one_launch(uint32_t n_rounds) {
         TN_HIP(hipStreamSynchronize(ctx.stream));  // the pinned step value below is free
         TN_HIP(hipStreamSynchronize(root_));
         ctx.nerf->network.prepare_workspace(ctx.batches[0].batch_size, ctx.batches[0].ray_capacity);
+        t.trainer_.prepare_plans(ctx);
         const int c = ctx.cur;
         if (!ctx.prefetched) {
             TN_HIP(hipStreamWaitEvent(ctx.gen_stream, ctx.ev_grid, 0));

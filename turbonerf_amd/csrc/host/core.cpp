@@ -221,11 +221,18 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
     }
     {
         ScopedPhase ph(tm, "hashgrid_backward", s);
-        // the bin-sort scatter (its workspace follows the batch size and the layout)
+        // the bin-sort scatter (its workspace follows the batch size and the
+        // layout); with the batch's plan formed ahead, only the passes that
+        // need the gradients run here
         const size_t hb = tn_hashgrid_backward_workspace_bytes(B, grid);
         w.hash_ws.reserve(hb);
-        TN_TRY(tn_hashgrid_backward_binned(0, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(),
-                                           w.hash_ws.data(), w.hash_ws.size(), ts));
+        if (st.hash_plan)
+            TN_TRY(tn_hashgrid_backward_planned(0, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(),
+                                                st.hash_plan, st.hash_plan_bytes, w.hash_ws.data(), w.hash_ws.size(),
+                                                ts));
+        else
+            TN_TRY(tn_hashgrid_backward_binned(0, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(),
+                                               w.hash_ws.data(), w.hash_ws.size(), ts));
     }
     // optimiser (reference :770-781, lr schedule :256; from the device step
     // counter in a captured step)
@@ -305,7 +312,7 @@ size_t TrainingWorkspace::bytes() const {
     return random_float.bytes() + pix_rgba.bytes() + ray_origin.bytes() + ray_dir.bytes() + ray_t.bytes() +
            ray_t_max.bytes() + ray_img_id.bytes() + ray_step.bytes() + ray_offset.bytes() + ray_alive.bytes() +
            sample_img_id.bytes() + sample_t.bytes() + sample_pos.bytes() + sample_dir.bytes() + sample_dt.bytes() +
-           sample_m_norm.bytes() + sample_dt_norm.bytes() + scan_ws.bytes();
+           sample_m_norm.bytes() + sample_dt_norm.bytes() + scan_ws.bytes() + hash_plan.bytes();
 }
 
 void TrainingWorkspace::free_allocations() { *this = TrainingWorkspace(); }
@@ -411,6 +418,21 @@ void Trainer::generate_next_training_batch(Context& ctx, TrainingWorkspace& ws, 
                                  ws.sample_dt_norm.data(), ts));
         TN_TRY(tn_estimate_rays(ws.info.data(), B, R, 64, ctx.n_rays_next.data(), ts));
     }
+    {
+        // the hash-grid backward's plan needs only the positions: formed here,
+        // beside the network stream's work, rather than on it
+        ScopedPhase ph(tm, "hashgrid_plan", s);
+        const tn_hashgrid_t& g = nerf.network.grid;
+        const size_t hp = tn_hashgrid_backward_plan_bytes(B, g);
+        ws.hash_plan.reserve(hp);  // sized ahead of any capture (prepare_plans)
+        if (hp)
+            TN_TRY(tn_hashgrid_backward_plan(B, ws.info.data(), B, g, ws.sample_pos.data(), ws.hash_plan.data(),
+                                             ws.hash_plan.size(), ts));
+    }
+}
+
+void Trainer::prepare_plans(Context& ctx) {
+    for (auto& b : ctx.batches) b.hash_plan.reserve(tn_hashgrid_backward_plan_bytes(b.batch_size, ctx.nerf->network.grid));
 }
 
 // Train on batches[c] on the network stream; with `graph` the work is being
@@ -424,6 +446,10 @@ void Trainer::train_network(Context& ctx, int c, uint32_t training_step, bool gr
     st.samples_total = ctx.samples_total.size() ? ctx.samples_total.data() : nullptr;
     st.step_counter = ctx.d_step.data();
     st.step_for_lr = graph;
+    if (ws.hash_plan.size() && ws.hash_plan.size() >= tn_hashgrid_backward_plan_bytes(ws.batch_size, ctx.nerf->network.grid)) {
+        st.hash_plan = ws.hash_plan.data();
+        st.hash_plan_bytes = ws.hash_plan.size();
+    }
     ctx.nerf->network.train(ctx.stream, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity,
                             ws.info.data(), ws.random_float.data() + ws.ray_capacity, ws.ray_step.data(),
                             ws.ray_offset.data(), ws.sample_img_id.data(), ws.sample_t.data(), ws.sample_pos.data(),
@@ -480,7 +506,10 @@ std::vector<uint64_t> Trainer::graph_key(const Context& ctx) const {
         ptr(n.network.workspace.mlp_ws.data()), ptr(ctx.samples_total.data()), tn_tune_generation(),
         n.params.n_appearances, ptr(n.network.app_bias.data()), ptr(n.network.workspace.app_dz.data()),
         ptr(n.network.workspace.app_ws.data())};
-    for (const auto& b : ctx.batches) k.push_back(ptr(b.random_float.data()));
+    for (const auto& b : ctx.batches) {
+        k.push_back(ptr(b.random_float.data()));
+        k.push_back(ptr(b.hash_plan.data()));
+    }
     return k;
 }
 
@@ -521,6 +550,7 @@ hipGraph_t Trainer::capture_gen(Context& ctx, int c) {
 }
 
 void Trainer::instantiate_graphs(Context& ctx, int c) {
+    prepare_plans(ctx);
     ctx.train_graph[c] = instantiate(capture_train(ctx, c));
     ctx.gen_graph[c ^ 1] = instantiate(capture_gen(ctx, c ^ 1));
     ++ctx.n_captures;
@@ -536,6 +566,7 @@ void Trainer::begin_graph_steps(Context& ctx, uint32_t first_step) {
     const hipStream_t net = ctx.stream;
     // a captured step may not allocate: size every buffer first
     ctx.nerf->network.prepare_workspace(ctx.batches[0].batch_size, ctx.batches[0].ray_capacity);
+    prepare_plans(ctx);
     const std::vector<uint64_t> key = graph_key(ctx);
     if (key != ctx.graph_key) {
         ctx.destroy_graphs();

@@ -165,6 +165,10 @@ struct NerfNetwork {
         // reads (a captured step replays with it advancing on the device)
         uint32_t* step_counter = nullptr;
         bool step_for_lr = false;
+        // the batch's hash-grid backward plan (tn_hashgrid_backward_plan),
+        // formed ahead from the positions; null: the backward forms it
+        const void* hash_plan = nullptr;
+        size_t hash_plan_bytes = 0;
     };
 
     NetworkConfig config;
@@ -227,6 +231,9 @@ struct TrainingWorkspace {
     DeviceBuffer<float> sample_t, sample_pos, sample_dir, sample_dt, sample_m_norm, sample_dt_norm;
     DeviceBuffer<uint32_t> info;
     DeviceBuffer<uint8_t> scan_ws;
+    // the hash-grid backward's plan for this batch (record counts and offsets
+    // per bin, from the sample positions), formed with the batch
+    DeviceBuffer<uint8_t> hash_plan;
     void enlarge(uint32_t B, uint32_t R);
     size_t bytes() const;
     void free_allocations();
@@ -362,6 +369,8 @@ struct Trainer {
     std::vector<uint64_t> graph_key(const Context& ctx) const;
     // Eager step on batches[c] while batches[c ^ 1] is generated.
     void train_batch(Context& ctx, int c, uint32_t training_step);
+    // sizes both batches' hash-grid backward plans (before any capture)
+    void prepare_plans(Context& ctx);
 
 private:
     void instantiate_graphs(Context& ctx, int c);

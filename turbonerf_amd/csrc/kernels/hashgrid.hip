@@ -392,8 +392,21 @@ TN_DEVICE HbIn hb_load(int l, uint32_t i, uint32_t n, uint32_t B, uint32_t GB, c
     return v;
 }
 
-// a sample takes part when it is in the batch and has a gradient
-TN_DEVICE bool hb_active(uint32_t i, uint32_t n, const HbIn& v) { return i < n && (v.g0 != 0.0f || v.g1 != 0.0f); }
+// every sample of the batch takes part (one with a zero gradient emits
+// zero-valued records), so the record counts follow from the positions
+// alone and can be formed before the gradients exist
+TN_DEVICE bool hb_active(uint32_t i, uint32_t n) { return i < n; }
+
+// position of sample i (zero past the batch), for the count pass
+TN_DEVICE HbIn hb_load_pos(uint32_t i, uint32_t n, uint32_t B, const float* __restrict__ pos) {
+    HbIn v = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (i < n) {
+        v.x = pos[i];
+        v.y = pos[i + B];
+        v.z = pos[i + 2 * (uint64_t)B];
+    }
+    return v;
+}
 
 // whether this lane ends its run (emits the run's records)
 TN_DEVICE bool hb_emits(bool active, uint64_t heads) {
@@ -432,12 +445,10 @@ TN_DEVICE uint32_t abs_bits(float a, float b) {
 }
 
 // (1) per-block record counts per bin of the block's level (the run
-// structure only: no values are formed here)
-template <typename TG>
+// structure of the positions only: no gradients are read)
 __global__ void __launch_bounds__(256) hashgrid_bwd_count_kernel(uint32_t n_max, const uint32_t* __restrict__ info,
-                                                                 uint32_t B, uint32_t GB, tn_hashgrid_t g, HbPlan p,
+                                                                 uint32_t B, tn_hashgrid_t g, HbPlan p,
                                                                  const float* __restrict__ pos,
-                                                                 const TG* __restrict__ dfeat,
                                                                  uint32_t* __restrict__ counts) {
     __shared__ uint32_t hist[kHbMaxLevelBins];
     const uint32_t L = (uint32_t)g.n_levels;
@@ -448,12 +459,12 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_count_kernel(uint32_t n_max,
     const uint32_t n = info ? min(info[1], n_max) : n_max;
     const uint32_t chunks = (n + 255) / 256, step = p.blocks / L;
     uint32_t c = blockIdx.x / L;
-    HbIn next = hb_load<TG>(l, c * 256 + threadIdx.x, n, B, GB, pos, dfeat);
+    HbIn next = hb_load_pos(c * 256 + threadIdx.x, n, B, pos);
     for (; c < chunks; c += step) {
         const uint32_t i = c * 256 + threadIdx.x;
         const HbIn v = next;
-        next = hb_load<TG>(l, (c + step) * 256 + threadIdx.x, n, B, GB, pos, dfeat);
-        const bool active = hb_active(i, n, v);
+        next = hb_load_pos((c + step) * 256 + threadIdx.x, n, B, pos);
+        const bool active = hb_active(i, n);
         if (__ballot(active) == 0) continue;
         HashCorners c8;
         uint64_t hm;
@@ -626,7 +637,7 @@ __global__ void __launch_bounds__(kHbScatterThreads) hashgrid_bwd_scatter_kernel
         const HbIn v = next;
         next = hb_load<TG>(l, (c + step) * 256 + tid, n, B, GB, pos, dfeat);
         HbCorners hc;
-        hb_corners(g, l, hb_active(i, n, v), v, hc);
+        hb_corners(g, l, hb_active(i, n), v, hc);
         uint32_t bin[8], rank[8], r0[8], r1[8];
         if (hc.emit) {
 #pragma unroll
@@ -902,52 +913,78 @@ inline bool hb_plan(const tn_hashgrid_t& g, uint32_t n_max, uint32_t level_block
     return true;
 }
 
-struct HbWorkspace {
-    uint32_t *counts, *offsets, *binmax, *totals, *start, *chunk_start, *slot_base, *split_list;
+// The workspace in two parts: the plan (what the positions decide: record
+// counts and offsets per block and bin, bin starts, chunking) and the work
+// area of the gradient passes (per-block bin maxima, the records, partial
+// sums of split bins).
+struct HbPlanArea {
+    uint32_t *counts, *offsets, *totals, *start, *chunk_start, *slot_base, *split_list;
+};
+struct HbWorkArea {
+    uint32_t* binmax;
     unsigned long long* slots;
     HbRecord* rec;
 };
 
 inline size_t hb_align(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// workspace bytes for a plan, and its carve-up when `base` is given
-inline size_t hb_workspace(const HbPlan& p, void* base, HbWorkspace* ws) {
-    const size_t mat = (size_t)p.blocks * p.row * sizeof(uint32_t);
-    const size_t bins1 = ((size_t)p.n_bins + 1) * sizeof(uint32_t);
-    unsigned char* b = static_cast<unsigned char*>(base);
+struct HbCarver {
+    unsigned char* b;
     size_t off = 0;
-    auto take = [&](size_t bytes) {
+    unsigned char* take(size_t bytes) {
         unsigned char* q = b ? b + off : nullptr;
         off += hb_align(bytes);
         return q;
-    };
-    HbWorkspace w;
-    w.counts = reinterpret_cast<uint32_t*>(take(mat));
-    w.offsets = reinterpret_cast<uint32_t*>(take(mat));
-    w.binmax = reinterpret_cast<uint32_t*>(take((size_t)p.n_bins * p.level_blocks * sizeof(uint32_t)));
-    w.totals = reinterpret_cast<uint32_t*>(take(bins1));
-    w.start = reinterpret_cast<uint32_t*>(take(bins1));
-    w.chunk_start = reinterpret_cast<uint32_t*>(take(bins1));
-    w.slot_base = reinterpret_cast<uint32_t*>(take(bins1));
-    w.split_list = reinterpret_cast<uint32_t*>(take(bins1 + sizeof(uint32_t)));
-    w.slots = reinterpret_cast<unsigned long long*>(
-        take((size_t)p.split_slots * 2 * kHbBinEntries * sizeof(unsigned long long)));
-    w.rec = reinterpret_cast<HbRecord*>(take((size_t)p.cap * sizeof(HbRecord)));
-    if (ws) *ws = w;
-    return off;
+    }
+};
+
+// plan bytes, and the carve-up when `base` is given
+inline size_t hb_plan_area(const HbPlan& p, void* base, HbPlanArea* pa) {
+    const size_t mat = (size_t)p.blocks * p.row * sizeof(uint32_t);
+    const size_t bins1 = ((size_t)p.n_bins + 1) * sizeof(uint32_t);
+    HbCarver c{static_cast<unsigned char*>(base)};
+    HbPlanArea a;
+    a.counts = reinterpret_cast<uint32_t*>(c.take(mat));
+    a.offsets = reinterpret_cast<uint32_t*>(c.take(mat));
+    a.totals = reinterpret_cast<uint32_t*>(c.take(bins1));
+    a.start = reinterpret_cast<uint32_t*>(c.take(bins1));
+    a.chunk_start = reinterpret_cast<uint32_t*>(c.take(bins1));
+    a.slot_base = reinterpret_cast<uint32_t*>(c.take(bins1));
+    a.split_list = reinterpret_cast<uint32_t*>(c.take(bins1 + sizeof(uint32_t)));
+    if (pa) *pa = a;
+    return c.off;
 }
 
+// work-area bytes, and the carve-up when `base` is given
+inline size_t hb_work_area(const HbPlan& p, void* base, HbWorkArea* wa) {
+    HbCarver c{static_cast<unsigned char*>(base)};
+    HbWorkArea a;
+    a.binmax = reinterpret_cast<uint32_t*>(c.take((size_t)p.n_bins * p.level_blocks * sizeof(uint32_t)));
+    a.slots = reinterpret_cast<unsigned long long*>(
+        c.take((size_t)p.split_slots * 2 * kHbBinEntries * sizeof(unsigned long long)));
+    a.rec = reinterpret_cast<HbRecord*>(c.take((size_t)p.cap * sizeof(HbRecord)));
+    if (wa) *wa = a;
+    return c.off;
+}
+
+// both parts in one buffer (the plan first)
+inline size_t hb_workspace(const HbPlan& p) { return hb_plan_area(p, nullptr, nullptr) + hb_work_area(p, nullptr, nullptr); }
+
+// (1)-(2b): the plan from the positions
+inline void hashgrid_backward_plan(uint32_t n_max, const uint32_t* info, uint32_t B, const tn_hashgrid_t& g,
+                                   const HbPlan& p, const float* pos, const HbPlanArea& pa, hipStream_t stream) {
+    hashgrid_bwd_count_kernel<<<p.blocks, 256, 0, stream>>>(n_max, info, B, g, p, pos, pa.counts);
+    hashgrid_bwd_reduce_kernel<<<g.n_levels * ((p.row + 63) / 64), 1024, 0, stream>>>(g, p, pa.counts, pa.offsets,
+                                                                                     pa.totals);
+    hashgrid_bwd_scan_kernel<<<1, 1024, 0, stream>>>(p, pa.totals, pa.start, pa.chunk_start, pa.slot_base,
+                                                     pa.split_list);
+}
+
+// (3)-(5): records, sums and the table update, from the plan and the gradients
 template <typename TG>
-void hashgrid_backward_binned(uint32_t n_max, const uint32_t* info, uint32_t B, uint32_t GB, const tn_hashgrid_t& g,
-                              const HbPlan& p, const float* pos, const TG* dfeat, float* dtable, void* workspace,
-                              hipStream_t stream) {
-    HbWorkspace ws;
-    hb_workspace(p, workspace, &ws);
-    hashgrid_bwd_count_kernel<TG><<<p.blocks, 256, 0, stream>>>(n_max, info, B, GB, g, p, pos, dfeat, ws.counts);
-    hashgrid_bwd_reduce_kernel<<<g.n_levels * ((p.row + 63) / 64), 1024, 0, stream>>>(g, p, ws.counts, ws.offsets,
-                                                                                     ws.totals);
-    hashgrid_bwd_scan_kernel<<<1, 1024, 0, stream>>>(p, ws.totals, ws.start, ws.chunk_start, ws.slot_base,
-                                                     ws.split_list);
+void hashgrid_backward_planned(uint32_t n_max, const uint32_t* info, uint32_t B, uint32_t GB, const tn_hashgrid_t& g,
+                               const HbPlan& p, const float* pos, const TG* dfeat, float* dtable,
+                               const HbPlanArea& pa, const HbWorkArea& wa, hipStream_t stream) {
     // scatter and accumulate by groups of levels, so a group's records are
     // read back while they are still in the device's last-level cache
     const uint32_t L = (uint32_t)g.n_levels;
@@ -960,13 +997,13 @@ void hashgrid_backward_binned(uint32_t n_max, const uint32_t* info, uint32_t B, 
         const uint64_t cap_g = (uint64_t)n_max * Lg * 8;
         const uint32_t acc_blocks = (uint32_t)((k_hi - k_lo) + (cap_g + kHbChunk - 1) / kHbChunk);
         hashgrid_bwd_scatter_kernel<TG><<<p.level_blocks * Lg, kHbScatterThreads, 0, stream>>>(
-            n_max, info, B, GB, g, p, pos, dfeat, ws.offsets, ws.start, ws.binmax, ws.rec, l0, Lg);
-        hashgrid_bwd_accumulate_kernel<<<acc_blocks, 512, 0, stream>>>(g, p, ws.start, ws.chunk_start, ws.binmax,
-                                                                       ws.slot_base, ws.rec, ws.slots, dtable, k_lo,
+            n_max, info, B, GB, g, p, pos, dfeat, pa.offsets, pa.start, wa.binmax, wa.rec, l0, Lg);
+        hashgrid_bwd_accumulate_kernel<<<acc_blocks, 512, 0, stream>>>(g, p, pa.start, pa.chunk_start, wa.binmax,
+                                                                       pa.slot_base, wa.rec, wa.slots, dtable, k_lo,
                                                                        k_hi);
     }
     hashgrid_bwd_finish_kernel<<<kHbFinishBlocks, kHbFinishThreads, 0, stream>>>(
-        g, p, ws.start, ws.chunk_start, ws.binmax, ws.slot_base, ws.split_list, ws.slots, dtable);
+        g, p, pa.start, pa.chunk_start, wa.binmax, pa.slot_base, pa.split_list, wa.slots, dtable);
 }
 
 inline bool hb_layout_ok(const tn_hashgrid_t& g) {
@@ -1065,7 +1102,7 @@ extern "C" int tn_hashgrid_backward(int dfeat_dtype, uint32_t n_max, const uint3
 extern "C" size_t tn_hashgrid_backward_workspace_bytes(uint32_t n_max, tn_hashgrid_t grid) {
     HbPlan p;
     if (!hb_layout_ok(grid) || n_max == 0 || !hb_plan(grid, n_max, kHbBlocksPerLevel, p)) return 0;
-    return hb_workspace(p, nullptr, nullptr);
+    return hb_workspace(p);
 }
 
 // The scatter-add of tn_hashgrid_backward through the bin sort (same
@@ -1080,15 +1117,80 @@ extern "C" int tn_hashgrid_backward_binned(int dfeat_dtype, uint32_t n_max, cons
     if (n_max == 0) return TN_OK;
     HbPlan p;
     const uint32_t level_blocks = (uint32_t)tune_value(TN_TUNE_HASH_BWD_LEVEL_BLOCKS);
-    if (!workspace || !hb_plan(grid, n_max, level_blocks, p) || workspace_bytes < hb_workspace(p, nullptr, nullptr))
+    if (!workspace || !hb_plan(grid, n_max, level_blocks, p) || workspace_bytes < hb_workspace(p))
         return tn_hashgrid_backward(dfeat_dtype, n_max, info, stride, dfeat_stride, grid, pos, dfeat, dtable, stream);
     TN_CHECK_ARG((reinterpret_cast<uintptr_t>(workspace) & 255) == 0 && (reinterpret_cast<uintptr_t>(dtable) & 7) == 0);
+    HbPlanArea pa;
+    HbWorkArea wa;
+    const size_t plan_bytes = hb_plan_area(p, workspace, &pa);
+    hb_work_area(p, static_cast<unsigned char*>(workspace) + plan_bytes, &wa);
+    hashgrid_backward_plan(n_max, info, stride, grid, p, pos, pa, stream);
     if (dfeat_dtype == 0)
-        hashgrid_backward_binned<float>(n_max, info, stride, dfeat_stride, grid, p, pos,
-                                        static_cast<const float*>(dfeat), dtable, workspace, stream);
+        hashgrid_backward_planned<float>(n_max, info, stride, dfeat_stride, grid, p, pos,
+                                         static_cast<const float*>(dfeat), dtable, pa, wa, stream);
     else
-        hashgrid_backward_binned<_Float16>(n_max, info, stride, dfeat_stride, grid, p, pos,
-                                           static_cast<const _Float16*>(dfeat), dtable, workspace, stream);
+        hashgrid_backward_planned<_Float16>(n_max, info, stride, dfeat_stride, grid, p, pos,
+                                            static_cast<const _Float16*>(dfeat), dtable, pa, wa, stream);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+// tn_hashgrid_backward_binned in two calls, for callers that know the
+// positions before the gradients: the plan (record counts and offsets per
+// bin, from the positions alone) into `plan`, then the gradient passes into
+// `work`. Both calls take the batch's positions and bound, and use 256
+// count/scatter workgroups per level (not the tuning knob, so that a plan
+// made earlier always fits the passes that use it).
+static bool hb_split_plan(const tn_hashgrid_t& g, uint32_t n_max, HbPlan& p) {
+    return hb_layout_ok(g) && n_max > 0 && hb_plan(g, n_max, kHbBlocksPerLevel, p);
+}
+
+extern "C" size_t tn_hashgrid_backward_plan_bytes(uint32_t n_max, tn_hashgrid_t grid) {
+    HbPlan p;
+    return hb_split_plan(grid, n_max, p) ? hb_plan_area(p, nullptr, nullptr) : 0;
+}
+
+extern "C" size_t tn_hashgrid_backward_work_bytes(uint32_t n_max, tn_hashgrid_t grid) {
+    HbPlan p;
+    return hb_split_plan(grid, n_max, p) ? hb_work_area(p, nullptr, nullptr) : 0;
+}
+
+extern "C" int tn_hashgrid_backward_plan(uint32_t n_max, const uint32_t* info, uint32_t stride, tn_hashgrid_t grid,
+                                         const float* pos, void* plan, size_t plan_bytes, hipStream_t stream) {
+    TN_CHECK_ARG(pos && plan && n_max <= stride);
+    HbPlan p;
+    TN_CHECK_ARG(n_max == 0 || hb_split_plan(grid, n_max, p));
+    if (n_max == 0) return TN_OK;
+    TN_CHECK_ARG(plan_bytes >= hb_plan_area(p, nullptr, nullptr) && (reinterpret_cast<uintptr_t>(plan) & 255) == 0);
+    HbPlanArea pa;
+    hb_plan_area(p, plan, &pa);
+    hashgrid_backward_plan(n_max, info, stride, grid, p, pos, pa, stream);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+extern "C" int tn_hashgrid_backward_planned(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
+                                            uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos,
+                                            const void* dfeat, float* dtable, const void* plan, size_t plan_bytes,
+                                            void* work, size_t work_bytes, hipStream_t stream) {
+    TN_CHECK_ARG(pos && dfeat && dtable && plan && work && n_max <= stride && n_max <= dfeat_stride);
+    TN_CHECK_ARG(dfeat_dtype == 0 || dfeat_dtype == 1);
+    HbPlan p;
+    TN_CHECK_ARG(n_max == 0 || hb_split_plan(grid, n_max, p));
+    if (n_max == 0) return TN_OK;
+    TN_CHECK_ARG(plan_bytes >= hb_plan_area(p, nullptr, nullptr) && work_bytes >= hb_work_area(p, nullptr, nullptr));
+    TN_CHECK_ARG(((reinterpret_cast<uintptr_t>(plan) | reinterpret_cast<uintptr_t>(work)) & 255) == 0 &&
+                 (reinterpret_cast<uintptr_t>(dtable) & 7) == 0);
+    HbPlanArea pa;
+    HbWorkArea wa;
+    hb_plan_area(p, const_cast<void*>(plan), &pa);
+    hb_work_area(p, work, &wa);
+    if (dfeat_dtype == 0)
+        hashgrid_backward_planned<float>(n_max, info, stride, dfeat_stride, grid, p, pos,
+                                         static_cast<const float*>(dfeat), dtable, pa, wa, stream);
+    else
+        hashgrid_backward_planned<_Float16>(n_max, info, stride, dfeat_stride, grid, p, pos,
+                                            static_cast<const _Float16*>(dfeat), dtable, pa, wa, stream);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }

@@ -234,10 +234,23 @@ def hashgrid_forward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: in
 
 def hashgrid_backward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: int, dfeat: torch.Tensor,
                       dtable: torch.Tensor, info: torch.Tensor | None = None, dfeat_stride: int | None = None,
-                      binned: bool = False):
+                      binned: bool = False, planned: bool = False):
     """dtable += the table gradient. binned: the bin-sort kernels (with their
-    workspace) instead of the atomic scatter-add."""
+    workspace) instead of the atomic scatter-add; planned: the same in two
+    calls, the plan from the positions first (tn_hashgrid_backward_plan),
+    then the gradient passes."""
     dfeat_stride = stride if dfeat_stride is None else dfeat_stride
+    if planned:
+        pb = query("tn_hashgrid_backward_plan_bytes", n, grid)
+        wb = query("tn_hashgrid_backward_work_bytes", n, grid)
+        if pb == 0 or wb == 0:
+            raise ValueError("hash-grid layout exceeds the bin limits")
+        plan = torch.empty(pb, dtype=torch.uint8, device="cuda")
+        work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+        call("tn_hashgrid_backward_plan", n, ptr(info), stride, grid, ptr(pos), ptr(plan), pb, stream_ptr())
+        call("tn_hashgrid_backward_planned", _PREC[dfeat.dtype], n, ptr(info), stride, dfeat_stride, grid, ptr(pos),
+             ptr(dfeat), ptr(dtable), ptr(plan), pb, ptr(work), wb, stream_ptr())
+        return
     if binned:
         nb = query("tn_hashgrid_backward_workspace_bytes", n, grid)
         if nb == 0:
