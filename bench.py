@@ -105,6 +105,9 @@ def parse():
                     help="summarise rocprofv3 CSVs of a --profile-steps run: kernel_trace.csv [FETCH_SIZE "
                          "counter_collection.csv WRITE_SIZE counter_collection.csv copy-probe FETCH_SIZE csv "
                          "copy-probe WRITE_SIZE csv]; prints markdown (no GPU needed)")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="CSV",
+                    help="summarise a --profile-steps run's kernel_trace.csv and counter_collection.csv files "
+                         "(any counters) for the MLP kernels; prints markdown (no GPU needed)")
     ap.add_argument("--profile-render", type=int, default=0, metavar="FRAMES",
                     help="profiling run: --warmup mixed-precision training steps (occupancy update every 16), "
                          "then FRAMES renders of an 800x800 frame of the trained NeRF, and exit")
@@ -675,8 +678,53 @@ def summarize(paths, last=0, sps=0.0, after="write_rgba_kernel", units=0):
             print(row + (f" {(rd + wr) / sps:.0f} |" if sps else ""))
 
 
+def summarize_counters(paths, last=0, kernels=("mlp_forward_kernel", "mlp_backward_kernel"), n_cu=256):
+    """Markdown table of the mean per-dispatch value of every counter in the
+    rocprofv3 counter CSVs paths[1:] for `kernels`, over the same window as
+    `summarize` (the last `last` training steps), with their average duration
+    from the kernel trace paths[0]. Derived, where the counters are present:
+    executed MFMA FLOP/s (SQ_INSTS_VALU_MFMA_MOPS_F16/_F32 count 512 FLOPs
+    each) and MFMA busy cycles per SIMD-cycle of the kernel's duration
+    (SQ_VALU_MFMA_BUSY_CYCLES over GRBM_GUI_ACTIVE x 4 SIMDs x n_cu)."""
+    import csv
+    import statistics as st
+
+    def window(rows):
+        adam = sorted(int(r["End_Timestamp"]) for r in rows if "adam_kernel" in r["Kernel_Name"])
+        start = adam[-last - 1] if last and len(adam) > last else 0
+        return [r for r in rows if int(r["Start_Timestamp"]) > start]
+
+    dur = {}
+    for r in window(list(csv.DictReader(open(paths[0])))):
+        dur.setdefault(_short(r["Kernel_Name"]), []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    vals = {}
+    for p in paths[1:]:
+        for r in window(list(csv.DictReader(open(p)))):
+            k = _short(r["Kernel_Name"])
+            if k in kernels:
+                vals.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    names = sorted({c for v in vals.values() for c in v})
+    print("| kernel | avg us | " + " | ".join(names) + " | MFMA TFLOP/s executed | MFMA busy |")
+    print("|---|---:|" + "---:|" * len(names) + "---:|---:|")
+    for k in kernels:
+        if k not in vals:
+            continue
+        c = {n: st.mean(v) for n, v in vals[k].items()}
+        us = st.mean(dur[k]) / 1e3 if k in dur else float("nan")
+        mops = c.get("SQ_INSTS_VALU_MFMA_MOPS_F16", 0.0) + c.get("SQ_INSTS_VALU_MFMA_MOPS_F32", 0.0)
+        tflops = f"{mops * 512 / (us * 1e-6) / 1e12:.1f}" if mops and us == us else "-"
+        busy = "-"
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in c and c.get("GRBM_GUI_ACTIVE"):
+            busy = f"{100 * c['SQ_VALU_MFMA_BUSY_CYCLES'] / (c['GRBM_GUI_ACTIVE'] * 4 * n_cu):.1f}%"
+        print(f"| {k} | {us:.2f} | " + " | ".join(f"{c[n]:.4g}" if n in c else "-" for n in names)
+              + f" | {tflops} | {busy} |")
+
+
 def main():
     args = parse()
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters, args.summarize_last)
+        return
     if args.summarize:
         summarize(args.summarize, args.summarize_last, args.samples_per_step, args.summarize_after,
                   args.summarize_units)

@@ -1,7 +1,8 @@
 """The Python binding on the host (no GPU): the classes and members the
 reference module exposes for training and rendering
-(src/api/python_bindings.cu:56-693; the Blender bridge, OpenGL/CUDA render
-surfaces and the CUDA runtime checks are not part of this build), the
+(src/api/python_bindings.cu:56-693; the Blender bridge and OpenGL/CUDA render
+surfaces are not part of this build, and the runtime checks name HIP and
+gfx950 instead of CUDA versions and compute capabilities), the
 UpdatableProperty wrappers, and object lifetimes: a NeRF returned by
 NeRFManager.create keeps its manager alive, and a Trainer or a Renderable
 keeps its NeRF alive, so dropping the Python names of the manager and the
@@ -49,6 +50,9 @@ SURFACE = {
     "RenderTarget": ["save_image", "set_size", "width", "height", "free"],
     "RenderModifiers": [], "RenderFlags": [], "RenderPattern": [],
     "FileManager": ["load", "save"],
+    "RuntimeManager": ["gpu_archs", "required_runtime_version", "current_runtime_version",
+                       "is_driver_version_sufficient", "get_device_architecture", "check_runtime"],
+    "RuntimeVersion": ["major", "minor", "patch", "to_string"],
     "UpdatableBoundingBox": ["get", "set", "set_dirty", "is_dirty"],
     "UpdatableTransform4f": ["get", "set", "set_dirty", "is_dirty"],
 }
@@ -80,3 +84,22 @@ def test_destroyed_slot_is_reused_in_place():
     mgr.destroy(a)
     b = mgr.create()
     assert b.id == slot and a.id == slot and a.training_step == 0 and not a.can_render
+
+
+def test_runtime_manager_names_the_build():
+    """The build's architecture list and required runtime (the HIP version
+    the library was compiled with, from the installed hip_version.h)."""
+    import re
+    from pathlib import Path
+
+    from turbonerf_amd import PyTurboNeRF as tn
+
+    rm = tn.RuntimeManager()
+    assert list(rm.gpu_archs) == ["gfx950"]
+    hdr = Path("/opt/rocm/include/hip/hip_version.h")
+    if hdr.exists():
+        v = {k: int(re.search(rf"#define HIP_VERSION_{k} (\d+)", hdr.read_text()).group(1))
+             for k in ("MAJOR", "MINOR", "PATCH")}
+        r = tn.RuntimeManager.required_runtime_version()
+        assert (r.major, r.minor, r.patch) == (v["MAJOR"], v["MINOR"], v["PATCH"])
+        assert str(r) == f"{v['MAJOR']}.{v['MINOR']}.{v['PATCH']}"

@@ -193,3 +193,14 @@ def test_trainer_survives_slot_reuse(scene):
         m = tr.train_step()
         assert np.isfinite(m.loss) and m.n_samples > 0
     assert again.training_step == 3
+
+
+def test_runtime_check_accepts_this_device(cuda):
+    """RuntimeManager (reference src/services/runtime-manager.cuh:159-200):
+    the device is gfx950 and the runtime at least the build's HIP version."""
+    from turbonerf_amd import PyTurboNeRF as tn
+
+    assert tn.RuntimeManager.get_device_architecture(0) == "gfx950"
+    assert tn.RuntimeManager.is_driver_version_sufficient()
+    assert tn.RuntimeManager.current_runtime_version() >= tn.RuntimeManager.required_runtime_version()
+    assert tn.RuntimeManager().check_runtime()

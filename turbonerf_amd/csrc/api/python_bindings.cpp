@@ -88,6 +88,24 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
     m.attr("__build__") = 1;
     m.def("teardown", []() { DeviceManager::teardown(); });
 
+    py::class_<RuntimeVersion>(m, "RuntimeVersion")
+        .def_readonly("major", &RuntimeVersion::major)
+        .def_readonly("minor", &RuntimeVersion::minor)
+        .def_readonly("patch", &RuntimeVersion::patch)
+        .def("to_string", &RuntimeVersion::to_string)
+        .def("__str__", &RuntimeVersion::to_string)
+        .def("__ge__", &RuntimeVersion::operator>=);
+
+    // reference src/api/python_bindings.cu:685-692 (cuda_archs becomes gpu_archs)
+    py::class_<RuntimeManager>(m, "RuntimeManager")
+        .def(py::init<>())
+        .def_readonly("gpu_archs", &RuntimeManager::gpu_archs)
+        .def_static("required_runtime_version", &RuntimeManager::required_runtime_version)
+        .def_static("current_runtime_version", &RuntimeManager::current_runtime_version)
+        .def_static("is_driver_version_sufficient", &RuntimeManager::is_driver_version_sufficient)
+        .def_static("get_device_architecture", &RuntimeManager::get_device_architecture, py::arg("device_id"))
+        .def("check_runtime", &RuntimeManager::check_runtime);
+
     py::class_<Transform4f>(m, "Transform4f", py::buffer_protocol())
         .def(py::init([](py::array_t<float, py::array::c_style | py::array::forcecast> a) { return to_transform(a); }),
              py::arg("matrix"))

@@ -5,12 +5,15 @@
 #pragma once
 
 #include <hip/hip_runtime_api.h>
+#include <hip/hip_version.h>
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "turbonerf/tn_c_api.h"
 
@@ -118,13 +121,96 @@ private:
     size_t n_ = 0;
 };
 
+// HIP runtime version as major.minor.patch (HIP_VERSION's encoding,
+// major * 10^7 + minor * 10^5 + patch).
+struct RuntimeVersion {
+    int major = 0, minor = 0, patch = 0;
+    static RuntimeVersion from_hip(int v) { return {v / 10000000, (v / 100000) % 100, v % 100000}; }
+    static RuntimeVersion compiled() { return from_hip(HIP_VERSION); }
+    bool operator>=(const RuntimeVersion& o) const {
+        if (major != o.major) return major > o.major;
+        if (minor != o.minor) return minor > o.minor;
+        return patch >= o.patch;
+    }
+    std::string to_string() const {
+        return std::to_string(major) + "." + std::to_string(minor) + "." + std::to_string(patch);
+    }
+};
+
+// Runtime and device checks (reference RuntimeManager,
+// src/services/runtime-manager.cuh:94-202, which checks the CUDA driver
+// version and each device's compute capability against the compiled
+// architectures). The kernels here are built for one architecture, gfx950
+// (MI355X); the runtime must be at least the HIP version they were built
+// with.
+class RuntimeManager {
+public:
+    const std::vector<std::string> gpu_archs{"gfx950"};
+
+    static RuntimeVersion required_runtime_version() { return RuntimeVersion::compiled(); }
+    static RuntimeVersion current_runtime_version() {
+        int v = 0;
+        TN_HIP(hipRuntimeGetVersion(&v));
+        return RuntimeVersion::from_hip(v);
+    }
+    static bool is_driver_version_sufficient() { return current_runtime_version() >= required_runtime_version(); }
+    // the device's architecture name without target features ("gfx950" of "gfx950:sramecc+:xnack-")
+    static std::string get_device_architecture(int device_id) {
+        hipDeviceProp_t p;
+        TN_HIP(hipGetDeviceProperties(&p, device_id));
+        std::string a(p.gcnArchName);
+        return a.substr(0, a.find(':'));
+    }
+    static bool is_architecture_supported(const std::string& arch) { return arch == "gfx950"; }
+
+    // Prints what it finds, as the reference does; false when the runtime is
+    // older than the build's or a device is not gfx950.
+    bool check_runtime() const {
+        std::printf("Checking HIP runtime and device capabilities...\n");
+        if (!is_driver_version_sufficient()) {
+            std::printf("HIP runtime version is %s which is not sufficient.  This binary was compiled for HIP "
+                        "runtime version %s.\n", current_runtime_version().to_string().c_str(),
+                        required_runtime_version().to_string().c_str());
+            return false;
+        }
+        int n = 0;
+        TN_HIP(hipGetDeviceCount(&n));
+        for (int d = 0; d < n; ++d) {
+            const std::string a = get_device_architecture(d);
+            if (!is_architecture_supported(a)) {
+                std::printf("Device %d (%s) is not supported.\nSupported architecture(s) are: \ngfx950 (MI355X)\n",
+                            d, a.c_str());
+                return false;
+            }
+            std::printf("Device %d (%s) is supported!\n", d, a.c_str());
+        }
+        std::printf("HIP runtime is %s which is sufficient to run this binary which was built for %s. We're good "
+                    "to go!\n", current_runtime_version().to_string().c_str(),
+                    required_runtime_version().to_string().c_str());
+        return true;
+    }
+
+    // Throws unless the current device is gfx950: the kernel library holds
+    // no code for any other architecture, so a launch there would fail.
+    static void require_current_device() {
+        int d = 0;
+        TN_HIP(hipGetDevice(&d));
+        const std::string a = get_device_architecture(d);
+        if (!is_architecture_supported(a))
+            throw std::runtime_error("device " + std::to_string(d) + " is " + a +
+                                     "; this build's kernels are compiled for gfx950 (MI355X) only");
+    }
+};
+
 // One device and one stream (reference DeviceManager,
 // src/services/device-manager.cuh:9-113, which also fixes the device count to 1).
+// Creating the stream checks that the device is gfx950.
 class DeviceManager {
 public:
     static int get_device_count() { return 1; }
     static hipStream_t stream() {
         static hipStream_t s = [] {
+            RuntimeManager::require_current_device();
             hipStream_t st;
             TN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
             return st;
