@@ -61,7 +61,7 @@ ALGO_FP32 = {"hashgrid_forward": ("hbm", 12 + 1024 + 128, 0, 0)}
 KERNEL = {
     "hashgrid_forward": ("hashgrid_fwd_kernel",),
     # the plan (count, reduce, scan) is formed with the batch on the
-    # generation stream; the network stream's phase runs the gradient passes
+    # generation stream; the network streams' phase runs the gradient passes
     "hashgrid_backward": ("hashgrid_bwd_scatter_kernel", "hashgrid_bwd_accumulate_kernel",
                           "hashgrid_bwd_finish_kernel"),
     "hashgrid_plan": ("hashgrid_bwd_count_kernel", "hashgrid_bwd_reduce_kernel", "hashgrid_bwd_scan_kernel"),
@@ -238,7 +238,7 @@ def run_precision(tn, synthetic, scene, precision, args):
         step += 1
     res["rays_per_step"] = statistics.mean(rays)
     res["loss"] = m.loss
-    res["phases"] = {k: dict(ms=v[0] / max(v[1], 1), launches=v[1]) for k, v in phases.items()}
+    res["phases"] = fold_phases(phases)
 
     # occupancy update, timed on its own
     tr.synchronize()
@@ -257,6 +257,22 @@ def run_precision(tn, synthetic, scene, precision, args):
         f"(captured step; eager {res['eager_samples_per_s'] / 1e6:.2f} M samples/s), "
         f"{res['samples_per_step']:.0f} samples/step, {res['rays_per_step']:.0f} rays/step")
     return res
+
+
+def fold_phases(phases):
+    """ms per step of each phase from the trainer's timings (name -> (total
+    ms, launches)). A step over the batch's two halves times the second
+    half's phases as "<phase>/2" on its own stream and each half's hash-grid
+    record scatter as "hashgrid_scatter"; they are added into the phase
+    (so a phase's time is the sum of its kernels' spans on both streams)."""
+    out = {}
+    for k, (ms, n) in phases.items():
+        base = k[:-2] if k.endswith("/2") else k
+        base = "hashgrid_backward" if base == "hashgrid_scatter" else base
+        d = out.setdefault(base, dict(ms=0.0, launches=0))
+        d["ms"] += ms / max(n, 1)
+        d["launches"] = max(d["launches"], n)
+    return out
 
 
 def roofline(res, n_params, dtype):
@@ -678,14 +694,16 @@ def summarize(paths, last=0, sps=0.0, after="write_rgba_kernel", units=0):
             print(row + (f" {(rd + wr) / sps:.0f} |" if sps else ""))
 
 
-def summarize_counters(paths, last=0, kernels=("mlp_forward_kernel", "mlp_backward_kernel"), n_cu=256):
+def summarize_counters(paths, last=0, kernels=("mlp_forward_kernel", "mlp_backward_kernel"), n_cu=256, n_xcd=8):
     """Markdown table of the mean per-dispatch value of every counter in the
     rocprofv3 counter CSVs paths[1:] for `kernels`, over the same window as
     `summarize` (the last `last` training steps), with their average duration
     from the kernel trace paths[0]. Derived, where the counters are present:
     executed MFMA FLOP/s (SQ_INSTS_VALU_MFMA_MOPS_F16/_F32 count 512 FLOPs
     each) and MFMA busy cycles per SIMD-cycle of the kernel's duration
-    (SQ_VALU_MFMA_BUSY_CYCLES over GRBM_GUI_ACTIVE x 4 SIMDs x n_cu)."""
+    (SQ_VALU_MFMA_BUSY_CYCLES, summed over the SIMDs, over the GPU's busy
+    cycles x 4 SIMDs x n_cu; GRBM_GUI_ACTIVE reads the busy cycles summed
+    over the n_xcd XCDs)."""
     import csv
     import statistics as st
 
@@ -715,7 +733,7 @@ def summarize_counters(paths, last=0, kernels=("mlp_forward_kernel", "mlp_backwa
         tflops = f"{mops * 512 / (us * 1e-6) / 1e12:.1f}" if mops and us == us else "-"
         busy = "-"
         if "SQ_VALU_MFMA_BUSY_CYCLES" in c and c.get("GRBM_GUI_ACTIVE"):
-            busy = f"{100 * c['SQ_VALU_MFMA_BUSY_CYCLES'] / (c['GRBM_GUI_ACTIVE'] * 4 * n_cu):.1f}%"
+            busy = f"{100 * c['SQ_VALU_MFMA_BUSY_CYCLES'] / (c['GRBM_GUI_ACTIVE'] / n_xcd * 4 * n_cu):.1f}%"
         print(f"| {k} | {us:.2f} | " + " | ".join(f"{c[n]:.4g}" if n in c else "-" for n in names)
               + f" | {tflops} | {busy} |")
 

@@ -102,6 +102,14 @@ int tn_march_generate(uint32_t n_rays_max, const uint32_t* info, uint32_t stride
  * reads n_samples back every step, src/core/trainer.cu:169-179). */
 int tn_accumulate_samples(const uint32_t* info, unsigned long long* total, tn_stream_t stream);
 
+/* tail_info = info with the sample count less `first` (0 when the batch has
+ * no more samples): the descriptor of the batch's samples from `first` on,
+ * for running the per-sample kernels over a batch in two parts (pointers
+ * advanced by `first`, same strides); the ray count (the loss scale's) is
+ * kept. No reference counterpart: the reference runs each step's kernels
+ * over the whole batch (src/core/nerf-network.cu:223-317). */
+int tn_batch_tail(const uint32_t* info, uint32_t first, uint32_t* tail_info, tn_stream_t stream);
+
 /* Next step's ray count from this step's samples per ray (device-side
  * replacement of reference src/core/trainer.cu:133-140). */
 int tn_estimate_rays(const uint32_t* info, uint32_t max_samples, uint32_t cap, uint32_t min_rays,
@@ -264,6 +272,22 @@ int tn_hashgrid_backward_plan(uint32_t n_max, const uint32_t* info, uint32_t str
 int tn_hashgrid_backward_planned(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
     uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos, const void* dfeat, float* dtable,
     const void* plan, size_t plan_bytes, void* work, size_t work_bytes, tn_stream_t stream);
+/* The passes of tn_hashgrid_backward_planned in parts, for a caller whose
+ * feature gradients become ready by ranges of samples (the trainer's two
+ * halves of a batch): tn_hashgrid_backward_scatter writes the records of
+ * samples [sample_begin, sample_end) of every level (whole count blocks of
+ * tn_hashgrid_backward_part_samples samples; sample_end may be n_max, and
+ * the part reaching n_max must be scattered); once every part is,
+ * tn_hashgrid_backward_accumulate adds the sums to dtable. Same result as
+ * tn_hashgrid_backward_planned bit for bit. Same reference call site
+ * (src/core/nerf-network.cu:760). */
+uint32_t tn_hashgrid_backward_part_samples(uint32_t n_max, tn_hashgrid_t grid);
+int tn_hashgrid_backward_scatter(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
+    uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos, const void* dfeat, const void* plan,
+    size_t plan_bytes, void* work, size_t work_bytes, uint32_t sample_begin, uint32_t sample_end,
+    tn_stream_t stream);
+int tn_hashgrid_backward_accumulate(uint32_t n_max, tn_hashgrid_t grid, float* dtable, const void* plan,
+    size_t plan_bytes, void* work, size_t work_bytes, tn_stream_t stream);
 
 
 /* ------------------------------------------------------------------- MLPs */

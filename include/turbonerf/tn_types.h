@@ -28,9 +28,9 @@ enum tn_status {
 
 /* Launch-shape knobs of the kernel launchers (tn_tune_set / tn_tune_get):
  * process-wide, read when a kernel is enqueued. Every value gives the same
- * results except TN_TUNE_MLP_BWD_BLOCKS_PER_CU, which changes the order in
- * which the MLP weight gradient's per-workgroup partial sums are added (fp32
- * rounding only). Defaults in parentheses. */
+ * results except TN_TUNE_MLP_BWD_BLOCKS_PER_CU and TN_TUNE_STEP_HALVES, which
+ * change the order in which the MLP weight gradient's per-workgroup partial
+ * sums are added (fp32 rounding only). Defaults in parentheses. */
 enum tn_tune_knob {
     TN_TUNE_HASH_FWD_BLOCK = 0,        /* threads per hash-forward workgroup: 64, 128 or 256 (256) */
     TN_TUNE_HASH_BWD_LEVEL_BLOCKS = 1, /* binned hash backward: count/scatter workgroups per level, 8..256 (128) */
@@ -39,7 +39,11 @@ enum tn_tune_knob {
     TN_TUNE_MARCH_MAX_BLOCKS = 4,      /* ray-marching workgroups at most, 256..8192 (2048) */
     TN_TUNE_ADAM_BLOCK = 5,            /* threads per Adam workgroup, 64..1024, a multiple of 64 (256) */
     TN_TUNE_HASH_BWD_LEVEL_GROUPS = 6, /* binned hash backward: levels scattered and summed in this many groups, 1..16 (1) */
-    TN_TUNE_COUNT = 7
+    TN_TUNE_HASH_FWD_LDS_LEVELS = 7,   /* hash forward: coarse dense levels read from a copy in LDS, at most, 0..4 (0);
+                                          only those whose tables fit 64 KiB together are staged */
+    TN_TUNE_STEP_HALVES = 8,           /* training step: the per-sample kernels over the batch's two halves on two
+                                          streams, so one half's MLP passes overlap the other's hash-grid passes, 0..1 (0) */
+    TN_TUNE_COUNT = 9
 };
 
 /* Axis-aligned box; same field order as BoundingBox

@@ -50,7 +50,7 @@ SURFACE = {
     "RenderTarget": ["save_image", "set_size", "width", "height", "free"],
     "RenderModifiers": [], "RenderFlags": [], "RenderPattern": [],
     "FileManager": ["load", "save"],
-    "RuntimeManager": ["gpu_archs", "required_runtime_version", "current_runtime_version",
+    "RuntimeManager": ["gpu_archs", "required_runtime_version", "compiled_runtime_version", "current_runtime_version",
                        "is_driver_version_sufficient", "get_device_architecture", "check_runtime"],
     "RuntimeVersion": ["major", "minor", "patch", "to_string"],
     "UpdatableBoundingBox": ["get", "set", "set_dirty", "is_dirty"],
@@ -87,8 +87,9 @@ def test_destroyed_slot_is_reused_in_place():
 
 
 def test_runtime_manager_names_the_build():
-    """The build's architecture list and required runtime (the HIP version
-    the library was compiled with, from the installed hip_version.h)."""
+    """The build's architecture list, the HIP version it was compiled with
+    (the installed hip_version.h) and the runtime it requires (that major
+    version)."""
     import re
     from pathlib import Path
 
@@ -100,6 +101,8 @@ def test_runtime_manager_names_the_build():
     if hdr.exists():
         v = {k: int(re.search(rf"#define HIP_VERSION_{k} (\d+)", hdr.read_text()).group(1))
              for k in ("MAJOR", "MINOR", "PATCH")}
+        c = tn.RuntimeManager.compiled_runtime_version()
+        assert (c.major, c.minor, c.patch) == (v["MAJOR"], v["MINOR"], v["PATCH"])
+        assert str(c) == f"{v['MAJOR']}.{v['MINOR']}.{v['PATCH']}"
         r = tn.RuntimeManager.required_runtime_version()
-        assert (r.major, r.minor, r.patch) == (v["MAJOR"], v["MINOR"], v["PATCH"])
-        assert str(r) == f"{v['MAJOR']}.{v['MINOR']}.{v['PATCH']}"
+        assert (r.major, r.minor, r.patch) == (v["MAJOR"], 0, 0)

@@ -8,7 +8,8 @@ kernel, on the default 16-level layout (T = 2^19):
   alone), batch size from the device count (info);
 * the same in two calls, the plan from the positions first
   (tn_hashgrid_backward_plan) and then the gradient passes
-  (tn_hashgrid_backward_planned), as the trainer runs them: equal to the
+  (tn_hashgrid_backward_planned), as the trainer runs them, or the records
+  of two sample ranges scattered apart and summed once: equal to the
   one-call backward bit for bit;
 * random points (no merging: level 0's single bin holds 16 chunks of
   records, summed in parts);
@@ -252,11 +253,20 @@ def test_planned_equals_one_call(cuda, half):
     if half:
         df_d = df_d.half()
     info = torch.tensor([0, n - 1000, 0, 0], dtype=torch.int32, device="cuda")
+    # records scattered in two parts (tn_hashgrid_backward_scatter, the later
+    # samples first) and summed once (tn_hashgrid_backward_accumulate): at
+    # the middle count block, and at the last block, which lies past the
+    # device count (its part scatters nothing but its zero bin maxima)
+    part = ops.query("tn_hashgrid_backward_part_samples", n, g)
+    blocks = -(-n // part)
+    assert part % 256 == 0 and blocks >= 2
+    splits = [None, None, part * (blocks // 2), part * (blocks - 1)]
     runs = []
-    for planned in (False, True):
+    for k, split in enumerate(splits):
         got = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
-        ops.hashgrid_backward(g, n, pos_d, n, df_d, got, info=info, binned=not planned, planned=planned)
+        ops.hashgrid_backward(g, n, pos_d, n, df_d, got, info=info, binned=k == 0, planned=k > 0, split=split)
         runs.append(got.cpu().numpy())
-    print(f"[hash bwd planned half={half}] touched {np.count_nonzero(runs[0])}", flush=True)
+    print(f"[hash bwd planned half={half}] touched {np.count_nonzero(runs[0])}, parts of {part} samples", flush=True)
     assert np.abs(runs[0]).max() > 0
-    assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))
+    for r in runs[1:]:
+        assert np.array_equal(runs[0].view(np.uint32), r.view(np.uint32))

@@ -82,6 +82,46 @@ def test_hashgrid_forward_backward(cuda):
     assert np.array_equal(got != 0, ref_b != 0)
 
 
+@pytest.mark.parametrize("log2_t", [15, 19])
+def test_hashgrid_forward_lds_levels_equal_global_gathers(cuda, log2_t):
+    """The coarse dense levels read from a copy of their tables in LDS give
+    the features of the global-memory gathers bit for bit, for fp32 and fp16
+    tables and fp16 features, with the sample count read on the device (the
+    trailing samples are left untouched)."""
+    import ctypes
+
+    import torch
+
+    from turbonerf_amd import _capi, ops
+
+    knob = 7  # TN_TUNE_HASH_FWD_LDS_LEVELS
+    lib = _capi.lib()
+    old = ctypes.c_int()
+    assert lib.tn_tune_get(knob, ctypes.byref(old)) == 0
+    g = ops.hashgrid_layout(log2_T=log2_t)
+    rng = np.random.default_rng(log2_t)
+    n_max, n = 70000, 65521
+    pos = rng.uniform(-0.05, 1.05, size=(3, n_max)).astype(np.float32)
+    pos[:, :6] = [[0, 1, 0.5, 0.999999, 0, 1], [0, 1, 0.5, 0.0, 1, 0], [0, 1, 0.5, 1, 1, 0]]
+    pos_d = torch.from_numpy(pos.ravel()).cuda()
+    info = torch.tensor([1, n, 0, 0], dtype=torch.int32, device="cuda")
+    table = torch.from_numpy(rng.uniform(-1, 1, size=g.n_params).astype(np.float32)).cuda()
+    try:
+        for tab, fdt in ((table, torch.float32), (table.half(), torch.float32), (table.half(), torch.float16)):
+            outs = []
+            for ls in (0, 1, 2, 4):
+                assert lib.tn_tune_set(knob, ls) == 0
+                outs.append(ops.hashgrid_forward(g, n_max, pos_d, n_max, tab, feat_dtype=fdt, info=info))
+            torch.cuda.synchronize()
+            ref = outs[0].view(32, n_max)
+            assert not ref[:, n:].any()
+            for o in outs[1:]:
+                assert torch.equal(o.view(32, n_max).view(torch.int16 if fdt == torch.float16 else torch.int32),
+                                   ref.view(torch.int16 if fdt == torch.float16 else torch.int32))
+    finally:
+        lib.tn_tune_set(knob, old.value)
+
+
 def _mlp_case(seed, n, dtype):
     import torch
 

@@ -197,9 +197,12 @@ def test_trainer_survives_slot_reuse(scene):
 
 def test_runtime_check_accepts_this_device(cuda):
     """RuntimeManager (reference src/services/runtime-manager.cuh:159-200):
-    the device is gfx950 and the runtime at least the build's HIP version."""
+    the device is gfx950 and the HIP runtime loaded in this process (PyTorch's
+    own may be another minor release) is of the build's major version."""
     from turbonerf_amd import PyTurboNeRF as tn
 
+    print(f"[runtime] current {tn.RuntimeManager.current_runtime_version()}, compiled "
+          f"{tn.RuntimeManager.compiled_runtime_version()}", flush=True)
     assert tn.RuntimeManager.get_device_architecture(0) == "gfx950"
     assert tn.RuntimeManager.is_driver_version_sufficient()
     assert tn.RuntimeManager.current_runtime_version() >= tn.RuntimeManager.required_runtime_version()

@@ -10,7 +10,9 @@
   when a knob changes.
 * Two MLP backward workgroups per CU only reorder the fp32 sum of the
   per-workgroup weight-gradient partials: the parameters after one step
-  agree to within a few fp32 ulps of the update.
+  agree to within a few fp32 ulps of the update. The same holds for a step
+  over the batch's two halves on two streams, whose hash-grid update is
+  the whole batch's bit for bit.
 * Trainer.autotune and MultiTrainer.autotune return a pick from each
   searched knob's candidates with a timing for every candidate, and
   training goes on.
@@ -21,9 +23,10 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"hash_fwd_block": 256, "hash_bwd_level_blocks": 128, "mlp_fwd_blocks_per_cu": 2,
-            "mlp_bwd_blocks_per_cu": 1, "march_max_blocks": 2048, "adam_block": 256, "hash_bwd_level_groups": 1}
+            "mlp_bwd_blocks_per_cu": 1, "march_max_blocks": 2048, "adam_block": 256, "hash_bwd_level_groups": 1,
+            "hash_fwd_lds_levels": 0, "step_halves": 0}
 OTHER = {"hash_fwd_block": 64, "hash_bwd_level_blocks": 16, "mlp_fwd_blocks_per_cu": 1, "march_max_blocks": 512,
-         "adam_block": 1024, "hash_bwd_level_groups": 3}
+         "adam_block": 1024, "hash_bwd_level_groups": 3, "hash_fwd_lds_levels": 2, "step_halves": 0}
 
 
 def _set(tn, knobs):
@@ -88,6 +91,37 @@ def test_launch_shapes_leave_results_unchanged(tuning, precision):
         t[2].teardown()
 
 
+@pytest.mark.parametrize("precision", [1, 0])
+def test_step_halves_reorder_the_weight_gradient_sums_only(tuning, precision):
+    """A step over the batch's two halves on two streams: every per-sample
+    result is the whole batch's, so the hash-grid gradient, and with it the
+    table's Adam update, is the same bit for bit; the MLP weight gradient is
+    the first half's sum plus the second's, a reordered fp32 sum. Then eager
+    and captured steps of the split trainer go on to agree bit for bit."""
+    tn = tuning
+    ref, alt = _trainer(47, precision), _trainer(47, precision)
+    tn.set_tuning("step_halves", 0)
+    mr = ref[2].train_steps(1, read_metrics=True, graph=False)
+    tn.set_tuning("step_halves", 1)
+    ma = alt[2].train_steps(1, read_metrics=True, graph=False)
+    p0, p1 = ref[2].get_params(), alt[2].get_params()
+    n_grid = len(p0) - 9408
+    d = np.abs(p1 - p0)
+    print(f"[tuning prec={precision}] step halves: loss {mr.loss:.6f} / {ma.loss:.6f}, network max |dp| "
+          f"{d[n_grid:].max():.2e}", flush=True)
+    assert np.float32(mr.loss) == np.float32(ma.loss)
+    assert np.array_equal(p0[:n_grid].view(np.uint32), p1[:n_grid].view(np.uint32))
+    assert d[n_grid:].max() < 1e-4
+    # split steps, eager against captured
+    eag, cap = _trainer(48, precision), _trainer(48, precision)
+    for r in range(3):
+        eag[2].train_steps(4, read_metrics=False, graph=False)
+        cap[2].train_steps(4, read_metrics=False, graph=True)
+        _same(eag[2], cap[2], f"halves round {r}")
+    for t in (ref, alt, eag, cap):
+        t[2].teardown()
+
+
 def test_two_mlp_backward_blocks_per_cu_reorders_the_partial_sums_only(tuning):
     tn = tuning
     ref, alt = _trainer(43), _trainer(43)
@@ -118,10 +152,10 @@ def test_autotune_picks_from_the_candidates_and_training_goes_on(tuning):
     print(f"[tuning] single: {res['picks']} {res['ns_per_sample_before']:.3f} -> "
           f"{res['ns_per_sample_after']:.3f} ns/sample", flush=True)
     # the hash-grid backward's workgroups per level are not searched (the
-    # trainer's backward plans use a fixed 256), the other six are
+    # trainer's backward plans use a fixed 256), the other eight are
     cands = {"hash_fwd_block": [64, 128, 256], "mlp_fwd_blocks_per_cu": [1, 2, 3, 4], "mlp_bwd_blocks_per_cu": [1, 2],
              "march_max_blocks": [512, 1024, 2048, 4096], "adam_block": [128, 256, 512, 1024],
-             "hash_bwd_level_groups": [1, 2, 4, 8]}
+             "hash_bwd_level_groups": [1, 2, 4, 8], "hash_fwd_lds_levels": [0, 1, 2], "step_halves": [0, 1]}
     assert set(res["picks"]) == set(DEFAULTS) and tn.get_tuning() == res["picks"]
     assert set(res["trials"]) == set(cands)
     for k, c in cands.items():

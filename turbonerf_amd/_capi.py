@@ -128,6 +128,7 @@ _SIGS: dict[str, tuple] = {
                                     vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "tn_estimate_rays": (C.c_int, [vp, u32, u32, u32, vp, vp]),
     "tn_accumulate_samples": (C.c_int, [vp, vp, vp]),
+    "tn_batch_tail": (C.c_int, [vp, u32, vp, vp]),
     "tn_composite_scratch_bytes": (sz, [u32, u32]),
     "tn_composite_train": (C.c_int, [C.c_int, vp, u32, u32, u32, f32, C.c_int, f32, vp, vp, vp, vp, vp, vp, vp, vp,
                                      vp, vp, vp, sz, vp]),
@@ -155,6 +156,10 @@ _SIGS: dict[str, tuple] = {
     "tn_hashgrid_backward_plan": (C.c_int, [u32, vp, u32, HashGrid, vp, vp, C.c_size_t, vp]),
     "tn_hashgrid_backward_planned": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp, C.c_size_t, vp,
                                                C.c_size_t, vp]),
+    "tn_hashgrid_backward_part_samples": (u32, [u32, HashGrid]),
+    "tn_hashgrid_backward_scatter": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, C.c_size_t, vp,
+                                               C.c_size_t, u32, u32, vp]),
+    "tn_hashgrid_backward_accumulate": (C.c_int, [u32, HashGrid, vp, vp, C.c_size_t, vp, C.c_size_t, vp]),
     "tn_mlp_forward": (C.c_int, [C.c_int, u32, vp, u32, u32, u32, vp, u32, vp, vp, C.c_int, vp, vp, vp, vp]),
     "tn_mlp_backward_workspace_bytes": (sz, [u32]),
     "tn_mlp_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, vp, u32, vp, vp, vp, vp, u32, vp, vp, f32, C.c_int, vp,

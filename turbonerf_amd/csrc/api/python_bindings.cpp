@@ -101,6 +101,7 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
         .def(py::init<>())
         .def_readonly("gpu_archs", &RuntimeManager::gpu_archs)
         .def_static("required_runtime_version", &RuntimeManager::required_runtime_version)
+        .def_static("compiled_runtime_version", &RuntimeManager::compiled_runtime_version)
         .def_static("current_runtime_version", &RuntimeManager::current_runtime_version)
         .def_static("is_driver_version_sufficient", &RuntimeManager::is_driver_version_sufficient)
         .def_static("get_device_architecture", &RuntimeManager::get_device_architecture, py::arg("device_id"))

@@ -15,9 +15,10 @@ namespace {
 // The hash-grid backward's workgroups per level are not searched: the
 // trainer forms each batch's backward plan ahead with the batch, at a fixed
 // 256 per level (the knob shapes the one-call tn_hashgrid_backward_binned).
-constexpr int kOrder[] = {TN_TUNE_HASH_BWD_LEVEL_GROUPS, TN_TUNE_MLP_BWD_BLOCKS_PER_CU,
-                          TN_TUNE_MLP_FWD_BLOCKS_PER_CU, TN_TUNE_HASH_FWD_BLOCK,
-                          TN_TUNE_MARCH_MAX_BLOCKS,      TN_TUNE_ADAM_BLOCK};
+constexpr int kOrder[] = {TN_TUNE_STEP_HALVES,           TN_TUNE_HASH_BWD_LEVEL_GROUPS,
+                          TN_TUNE_MLP_BWD_BLOCKS_PER_CU, TN_TUNE_MLP_FWD_BLOCKS_PER_CU,
+                          TN_TUNE_HASH_FWD_BLOCK,        TN_TUNE_MARCH_MAX_BLOCKS,
+                          TN_TUNE_ADAM_BLOCK,            TN_TUNE_HASH_FWD_LDS_LEVELS};
 
 int get(int knob) {
     int v = 0;
@@ -36,6 +37,8 @@ const char* Autotuner::knob_name(int knob) {
     case TN_TUNE_MARCH_MAX_BLOCKS: return "march_max_blocks";
     case TN_TUNE_ADAM_BLOCK: return "adam_block";
     case TN_TUNE_HASH_BWD_LEVEL_GROUPS: return "hash_bwd_level_groups";
+    case TN_TUNE_HASH_FWD_LDS_LEVELS: return "hash_fwd_lds_levels";
+    case TN_TUNE_STEP_HALVES: return "step_halves";
     default: throw std::runtime_error("autotune: unknown knob");
     }
 }
@@ -49,6 +52,8 @@ std::vector<int> Autotuner::candidates(int knob) {
     case TN_TUNE_MARCH_MAX_BLOCKS: return {512, 1024, 2048, 4096};
     case TN_TUNE_ADAM_BLOCK: return {128, 256, 512, 1024};
     case TN_TUNE_HASH_BWD_LEVEL_GROUPS: return {1, 2, 4, 8};
+    case TN_TUNE_HASH_FWD_LDS_LEVELS: return {0, 1, 2};
+    case TN_TUNE_STEP_HALVES: return {0, 1};
     default: throw std::runtime_error("autotune: unknown knob");
     }
 }

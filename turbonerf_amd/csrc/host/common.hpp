@@ -141,19 +141,25 @@ struct RuntimeVersion {
 // src/services/runtime-manager.cuh:94-202, which checks the CUDA driver
 // version and each device's compute capability against the compiled
 // architectures). The kernels here are built for one architecture, gfx950
-// (MI355X); the runtime must be at least the HIP version they were built
-// with.
+// (MI355X). The HIP runtime keeps its interface within a major version (the
+// library's soname), and a process may load another minor release than the
+// one the build compiled against (PyTorch ships its own), so the runtime must
+// be of the build's major version.
 class RuntimeManager {
 public:
     const std::vector<std::string> gpu_archs{"gfx950"};
 
-    static RuntimeVersion required_runtime_version() { return RuntimeVersion::compiled(); }
+    static RuntimeVersion compiled_runtime_version() { return RuntimeVersion::compiled(); }
+    static RuntimeVersion required_runtime_version() { return {RuntimeVersion::compiled().major, 0, 0}; }
     static RuntimeVersion current_runtime_version() {
         int v = 0;
         TN_HIP(hipRuntimeGetVersion(&v));
         return RuntimeVersion::from_hip(v);
     }
-    static bool is_driver_version_sufficient() { return current_runtime_version() >= required_runtime_version(); }
+    static bool is_driver_version_sufficient() {
+        const RuntimeVersion c = current_runtime_version(), r = required_runtime_version();
+        return c.major == r.major && c >= r;
+    }
     // the device's architecture name without target features ("gfx950" of "gfx950:sramecc+:xnack-")
     static std::string get_device_architecture(int device_id) {
         hipDeviceProp_t p;
@@ -170,7 +176,7 @@ public:
         if (!is_driver_version_sufficient()) {
             std::printf("HIP runtime version is %s which is not sufficient.  This binary was compiled for HIP "
                         "runtime version %s.\n", current_runtime_version().to_string().c_str(),
-                        required_runtime_version().to_string().c_str());
+                        compiled_runtime_version().to_string().c_str());
             return false;
         }
         int n = 0;
@@ -186,7 +192,7 @@ public:
         }
         std::printf("HIP runtime is %s which is sufficient to run this binary which was built for %s. We're good "
                     "to go!\n", current_runtime_version().to_string().c_str(),
-                    required_runtime_version().to_string().c_str());
+                    compiled_runtime_version().to_string().c_str());
         return true;
     }
 

@@ -79,6 +79,7 @@ void NetworkWorkspace::enlarge(uint32_t B, uint32_t R, int prec) {
     ray_loss.resize(4 * (size_t)ray_capacity);
     ray_dist_loss.resize(ray_capacity);
     loss_out.resize(2);
+    info_tail.resize(4);
     composite_scratch.resize(tn_composite_scratch_bytes(ray_capacity, batch_size));
     distortion_scratch.resize(tn_distortion_scratch_bytes(ray_capacity, batch_size));
     mlp_ws.resize(std::max<size_t>(tn_mlp_backward_workspace_bytes(batch_size), 4));
@@ -178,15 +179,53 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
         app_fwd = tn_appearance_t{app_bias.data(), appearance_id_batch, n_app, nullptr};
         app_bwd = tn_appearance_t{app_bias.data(), appearance_id_batch, n_app, w.app_dz.data()};
     }
-    // forward: encoding, density + colour MLPs (reference :431-545)
-    {
-        ScopedPhase ph(tm, "hashgrid_forward", s);
-        TN_TRY(tn_hashgrid_forward(prec, 0, B, info, B, B, grid, pos_batch, params, w.features.data(), ts));
+    // The per-sample passes run over samples [0, h) on s and, when the batch
+    // is split, over [h, B) on the auxiliary stream: the second half's
+    // encoding overlaps the first half's MLP forward, and the second half's
+    // MLP backward overlaps the first half's hash-grid scatter (MFMA-bound
+    // and memory-bound kernels side by side). h is a whole number of the
+    // hash-grid backward's count blocks, so each half scatters its own
+    // records into the batch's plan; compositing, the sums and Adam run once.
+    int halves_knob = 0;
+    TN_TRY(tn_tune_get(TN_TUNE_STEP_HALVES, &halves_knob));
+    const uint32_t part = st.aux && st.hash_plan && halves_knob ? tn_hashgrid_backward_part_samples(B, grid) : 0u;
+    const uint32_t h = part ? ((B + part - 1) / part / 2) * part : 0u;
+    const bool split = h > 0 && h < B;
+    const size_t esz = prec == 1 ? 2 : 4;
+    auto at = [](auto* p, size_t bytes) { return reinterpret_cast<decltype(p)>(reinterpret_cast<uintptr_t>(p) + bytes); };
+    // half k: its first sample, sample bound, descriptor and stream
+    struct Half { uint32_t first, n; const uint32_t* info; hipStream_t s; const char* tag; };
+    const Half halves[2] = {{0, split ? h : B, info, s, ""}, {h, B - h, w.info_tail.data(), st.aux, "/2"}};
+    const int n_halves = split ? 2 : 1;
+    // the second half's phases are timed under their own names ("…/2")
+    auto phase = [](const Half& hf, const char* name) { return std::string(name) + hf.tag; };
+    if (split) {
+        TN_TRY(tn_batch_tail(info, h, w.info_tail.data(), ts));
+        TN_HIP(hipEventRecord(st.split_events[0], s));
+        TN_HIP(hipStreamWaitEvent(st.aux, st.split_events[0], 0));
     }
-    {
-        ScopedPhase ph(tm, "mlp_forward", s);
-        TN_TRY(tn_mlp_forward(prec, B, info, B, B, B, params, p.n_grid, w.features.data(), dir_batch, 1,
-                              w.density.data(), w.rgb.data(), n_app ? &app_fwd : nullptr, ts));
+    // forward: encoding, density + colour MLPs (reference :431-545)
+    for (int k = 0; k < n_halves; ++k) {
+        const Half& hf = halves[k];
+        const tn_stream_t hs = as_tn(hf.s);
+        tn_appearance_t af = app_fwd;
+        if (n_app) af.image_id = appearance_id_batch + hf.first;
+        {
+            ScopedPhase ph(tm, phase(hf, "hashgrid_forward").c_str(), hf.s);
+            TN_TRY(tn_hashgrid_forward(prec, 0, hf.n, hf.info, B, B, grid, pos_batch + hf.first, params,
+                                       at(w.features.data(), 4 * (size_t)hf.first), hs));
+        }
+        {
+            ScopedPhase ph(tm, phase(hf, "mlp_forward").c_str(), hf.s);
+            TN_TRY(tn_mlp_forward(prec, hf.n, hf.info, B, B, B, params, p.n_grid,
+                                  at(w.features.data(), 4 * (size_t)hf.first), dir_batch + hf.first, 1,
+                                  at(w.density.data(), esz * hf.first), at(w.rgb.data(), esz * hf.first),
+                                  n_app ? &af : nullptr, hs));
+        }
+    }
+    if (split) {
+        TN_HIP(hipEventRecord(st.split_events[1], st.aux));
+        TN_HIP(hipStreamWaitEvent(s, st.split_events[1], 0));
     }
     // distortion loss (reference :273-283)
     if (st.use_distortion_loss) {
@@ -204,13 +243,43 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
                                   w.ray_loss.data(), w.dL_ds.data(), w.dL_drgb.data(), w.composite_scratch.data(),
                                   w.composite_scratch.size(), ts));
     }
-    // network backward (reference :668-768)
-    {
-        ScopedPhase ph(tm, "mlp_backward", s);
-        TN_TRY(tn_mlp_backward(prec, B, info, B, B, params, p.n_grid, w.features.data(), dir_batch, t_batch,
-                               w.dL_drgb.data(), B, w.dL_ds.data(), st.use_distortion_loss ? w.dL_dist.data() : nullptr,
-                               config.loss_scale(), 1, w.dfeat.data(), p.gradients.data() + p.n_grid, 0,
-                               w.mlp_ws.data(), w.mlp_ws.size(), n_app ? &app_bwd : nullptr, ts));
+    // network backward (reference :668-768) and the hash-grid records of
+    // each half; the second half's weight gradients add to the first's
+    const size_t hb = tn_hashgrid_backward_workspace_bytes(B, grid);
+    w.hash_ws.reserve(hb);
+    for (int k = n_halves - 1; k >= 0; --k) {
+        // the second half's MLP backward waits for the first's (one weight
+        // gradient, one partial-sum workspace), then scatters its records
+        const Half& hf = halves[split ? 1 - k : 0];
+        const tn_stream_t hs = as_tn(hf.s);
+        tn_appearance_t ab = app_bwd;
+        if (n_app) {
+            ab.image_id = appearance_id_batch + hf.first;
+            ab.dz = w.app_dz.data() + hf.first;
+        }
+        if (split && hf.first) TN_HIP(hipStreamWaitEvent(hf.s, st.split_events[2], 0));
+        {
+            ScopedPhase ph(tm, phase(hf, "mlp_backward").c_str(), hf.s);
+            TN_TRY(tn_mlp_backward(prec, hf.n, hf.info, B, B, params, p.n_grid,
+                                   at(w.features.data(), 4 * (size_t)hf.first), dir_batch + hf.first,
+                                   t_batch + hf.first, at(w.dL_drgb.data(), esz * hf.first), B,
+                                   w.dL_ds.data() + hf.first,
+                                   st.use_distortion_loss ? w.dL_dist.data() + hf.first : nullptr, config.loss_scale(),
+                                   1, at(w.dfeat.data(), 4 * (size_t)hf.first), p.gradients.data() + p.n_grid,
+                                   hf.first ? 1 : 0, w.mlp_ws.data(), w.mlp_ws.size(), n_app ? &ab : nullptr, hs));
+        }
+        if (split && !hf.first) TN_HIP(hipEventRecord(st.split_events[2], s));
+        if (split) {
+            // the records of this half's samples into the batch's plan
+            ScopedPhase ph(tm, phase(hf, "hashgrid_scatter").c_str(), hf.s);
+            TN_TRY(tn_hashgrid_backward_scatter(0, B, info, B, B, grid, pos_batch, w.dfeat.data(), st.hash_plan,
+                                                st.hash_plan_bytes, w.hash_ws.data(), w.hash_ws.size(), hf.first,
+                                                hf.first ? B : h, hs));
+        }
+    }
+    if (split) {
+        TN_HIP(hipEventRecord(st.split_events[3], st.aux));
+        TN_HIP(hipStreamWaitEvent(s, st.split_events[3], 0));
     }
     if (n_app) {
         ScopedPhase ph(tm, "appearance", s);
@@ -224,9 +293,10 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
         // the bin-sort scatter (its workspace follows the batch size and the
         // layout); with the batch's plan formed ahead, only the passes that
         // need the gradients run here
-        const size_t hb = tn_hashgrid_backward_workspace_bytes(B, grid);
-        w.hash_ws.reserve(hb);
-        if (st.hash_plan)
+        if (split)
+            TN_TRY(tn_hashgrid_backward_accumulate(B, grid, p.gradients.data(), st.hash_plan, st.hash_plan_bytes,
+                                                   w.hash_ws.data(), w.hash_ws.size(), ts));
+        else if (st.hash_plan)
             TN_TRY(tn_hashgrid_backward_planned(0, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(),
                                                 st.hash_plan, st.hash_plan_bytes, w.hash_ws.data(), w.hash_ws.size(),
                                                 ts));
@@ -337,6 +407,8 @@ void DatasetWorkspace::upload(const Dataset& ds, hipStream_t s) {
 Trainer::Context::Context(const Dataset* ds, NeRF* n, uint32_t B) : dataset(ds), nerf(n), batch_size(B) {
     TN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     TN_HIP(hipStreamCreateWithFlags(&gen_stream, hipStreamNonBlocking));
+    TN_HIP(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
+    for (auto& e : ev_split) TN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (int k = 0; k < 2; ++k) {
         TN_HIP(hipEventCreateWithFlags(&ev_ready[k], hipEventDisableTiming));
         TN_HIP(hipEventCreateWithFlags(&ev_consumed[k], hipEventDisableTiming));
@@ -357,14 +429,17 @@ void Trainer::Context::destroy_graphs() {
 Trainer::Context::~Context() {
     (void)hipStreamSynchronize(gen_stream);
     (void)hipStreamSynchronize(stream);
+    (void)hipStreamSynchronize(aux_stream);
     for (int k = 0; k < 2; ++k) {
         (void)hipEventDestroy(ev_ready[k]);
         (void)hipEventDestroy(ev_consumed[k]);
     }
+    for (auto e : ev_split) (void)hipEventDestroy(e);
     destroy_graphs();
     (void)hipEventDestroy(ev_grid);
     (void)hipEventDestroy(ev_gen_idle);
     (void)hipStreamDestroy(gen_stream);
+    (void)hipStreamDestroy(aux_stream);
     (void)hipStreamDestroy(stream);
 }
 
@@ -449,6 +524,8 @@ void Trainer::train_network(Context& ctx, int c, uint32_t training_step, bool gr
     if (ws.hash_plan.size() && ws.hash_plan.size() >= tn_hashgrid_backward_plan_bytes(ws.batch_size, ctx.nerf->network.grid)) {
         st.hash_plan = ws.hash_plan.data();
         st.hash_plan_bytes = ws.hash_plan.size();
+        st.aux = ctx.aux_stream;
+        st.split_events = ctx.ev_split;
     }
     ctx.nerf->network.train(ctx.stream, ctx.nerf->params, training_step, ws.batch_size, ws.ray_capacity,
                             ws.info.data(), ws.random_float.data() + ws.ray_capacity, ws.ray_step.data(),

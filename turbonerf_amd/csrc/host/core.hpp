@@ -149,6 +149,7 @@ struct NetworkWorkspace {
     DeviceBuffer<float> dL_ds, dL_dist, ray_loss, ray_dist_loss, loss_out;
     DeviceBuffer<uint8_t> composite_scratch, distortion_scratch, mlp_ws, hash_ws;
     DeviceBuffer<float> app_dz, app_ws;  // appearance embedding: colour-layer dz [64][B], per-image sums
+    DeviceBuffer<uint32_t> info_tail;    // batch descriptor of the second half (tn_batch_tail)
     void enlarge(uint32_t B, uint32_t R, int precision);
     size_t bytes() const;
 };
@@ -169,6 +170,10 @@ struct NerfNetwork {
         // formed ahead from the positions; null: the backward forms it
         const void* hash_plan = nullptr;
         size_t hash_plan_bytes = 0;
+        // with a plan and TN_TUNE_STEP_HALVES: a second stream and four
+        // events for running the batch's two halves side by side
+        hipStream_t aux = nullptr;
+        const hipEvent_t* split_events = nullptr;
     };
 
     NetworkConfig config;
@@ -281,6 +286,10 @@ struct Trainer {
     struct Context {
         hipStream_t stream;      // network, optimiser and occupancy update (the trainer's own)
         hipStream_t gen_stream;  // batch generation
+        // the second half of a batch's per-sample network passes, forked
+        // from and joined back into `stream` within each step
+        hipStream_t aux_stream;
+        hipEvent_t ev_split[4]{};
         TrainingWorkspace batches[2];
         int cur = 0;              // batch the next step trains on
         int last = 0;             // batch the last step trained on

@@ -37,6 +37,32 @@ inline unsigned blocks_for(uint64_t n, unsigned threads) {
     return (unsigned)(b == 0 ? 1 : b);
 }
 
+// kernels using more than the default 64 KiB of dynamic LDS opt in once
+template <typename K>
+inline bool allow_lds(K kernel, size_t bytes) {
+    if (bytes <= 65536) return true;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes) == hipSuccess)
+        return true;
+    (void)hipGetLastError();  // do not leave the failure for the next launch check
+    return false;
+}
+
+// multiprocessors (CUs) of the current device, cached per device
+inline unsigned compute_units() {
+    static int cached_dev = -1;
+    static unsigned cached_cus = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (dev != cached_dev) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        cached_cus = (unsigned)cus;
+        cached_dev = dev;
+    }
+    return cached_cus;
+}
+
 // Launch-shape knobs (enum tn_tune_knob), defined in tuning.hip; read by the
 // launchers on the host when they enqueue a kernel.
 int tune_value(int knob);

@@ -156,13 +156,14 @@ TN_DEVICE void gather_corners(const TT* __restrict__ table, uint32_t bytes, cons
     }
 }
 
+// Levels [l0, l0 + gridDim.y) gathered from global memory.
 template <typename TT, typename TO>
 __global__ void __launch_bounds__(256) hashgrid_fwd_kernel(uint32_t n_max, const uint32_t* __restrict__ info,
                                                            uint32_t B, uint32_t OB, tn_hashgrid_t g,
                                                            const float* __restrict__ pos, const TT* __restrict__ table,
-                                                           TO* __restrict__ out) {
+                                                           TO* __restrict__ out, int l0) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
+    const int l = l0 + (int)blockIdx.y;
     const uint32_t n = info ? min(info[1], n_max) : n_max;
     if (i >= n) return;
     HashCorners hc;
@@ -179,6 +180,61 @@ __global__ void __launch_bounds__(256) hashgrid_fwd_kernel(uint32_t n_max, const
     }
     out[(uint64_t)(2 * l) * OB + i] = (TO)f0;
     out[(uint64_t)(2 * l + 1) * OB + i] = (TO)f1;
+}
+
+// The coarse dense levels [0, Ls) from a copy of their tables in LDS. Their
+// entries are the first `stage_bytes` of the table (levels are stored in
+// order), small enough to copy into every workgroup: a wave's eight corner
+// reads per level then go to LDS instead of each lane requesting its own
+// cache line. A workgroup per CU copies the tables once and strides over the
+// samples, reading each position once for all Ls levels. The corners, weights
+// and the order of the trilinear sums are those of hashgrid_fwd_kernel, so
+// the features are the same bit for bit.
+constexpr uint32_t kHfLdsThreads = 1024;
+constexpr uint32_t kHfLdsBytes = 64 * 1024;
+template <typename TT, typename TO>
+__global__ void __launch_bounds__(kHfLdsThreads) hashgrid_fwd_lds_kernel(
+    uint32_t n_max, const uint32_t* __restrict__ info, uint32_t B, uint32_t OB, tn_hashgrid_t g,
+    const float* __restrict__ pos, const TT* __restrict__ table, TO* __restrict__ out, int Ls, uint32_t stage_bytes) {
+    extern __shared__ uint4 stage[];
+    const uint4* src = reinterpret_cast<const uint4*>(table);
+    for (uint32_t k = threadIdx.x; k < stage_bytes / 16u; k += kHfLdsThreads) stage[k] = src[k];
+    __syncthreads();
+    const auto* st = reinterpret_cast<const typename Pair<TT>::V*>(stage);
+    const uint32_t n = info ? min(info[1], n_max) : n_max;
+    for (uint32_t i = blockIdx.x * kHfLdsThreads + threadIdx.x; i < n; i += gridDim.x * kHfLdsThreads) {
+        const float px = pos[i], py = pos[i + B], pz = pos[i + 2 * (uint64_t)B];
+        for (int l = 0; l < Ls; ++l) {
+            HashCorners hc;
+            hash_corners(g, l, px, py, pz, hc);
+            float f0 = 0.0f, f1 = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const typename Pair<TT>::V v = st[hc.idx[c]];
+                f0 += hc.w[c] * Pair<TT>::a(v);
+                f1 += hc.w[c] * Pair<TT>::b(v);
+            }
+            out[(uint64_t)(2 * l) * OB + i] = (TO)f0;
+            out[(uint64_t)(2 * l + 1) * OB + i] = (TO)f1;
+        }
+    }
+}
+
+// Levels staged in LDS: the longest prefix of dense levels, at most `want`,
+// whose entries (entry_bytes each) fit kHfLdsBytes; 0 when the table is not
+// 16-byte aligned. *bytes gets the staged size (a multiple of 16: level
+// sizes are multiples of 8 entries of at least 4 bytes).
+inline int hf_lds_levels(const tn_hashgrid_t& g, uint32_t entry_bytes, const void* table, int want,
+                         uint32_t* bytes) {
+    *bytes = 0;
+    if (reinterpret_cast<uintptr_t>(table) & 15u) return 0;
+    int Ls = 0;
+    while (Ls < want && Ls < g.n_levels - 1 && g.level_dense[Ls] &&
+           (uint64_t)(g.level_offset[Ls] + g.level_size[Ls]) * entry_bytes <= kHfLdsBytes) {
+        *bytes = (g.level_offset[Ls] + g.level_size[Ls]) * entry_bytes;
+        ++Ls;
+    }
+    return Ls;
 }
 
 // dtable[e] += sum over samples of w * dL/dfeature. Two lanes per sample,
@@ -263,8 +319,10 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_kernel(uint32_t n_max, const
 // and the sum of the records is exact up to the rounding of each value to a
 // multiple of 2^-S (relative 2^-38 of the bin's largest value or better);
 // each record carries its value to 17 significant bits (HbRecord). Blocks of count and
-// scatter work on one level each: block b serves level b % L and chunks
-// b / L + k * (blocks / L) of 256 samples.
+// scatter work on one level each: block b serves level b % L and the
+// contiguous 256-sample chunks [j, j + 1) * chunk_blocks, j = b / L, so the
+// records of a range of samples can be scattered on their own (the slots of
+// every (block, bin) are fixed by the plan) once their gradients exist.
 // ---------------------------------------------------------------------------
 constexpr int kHbBinLog = 12;
 constexpr uint32_t kHbBinEntries = 1u << kHbBinLog;
@@ -280,6 +338,7 @@ struct HbPlan {
     uint32_t row;         // per-block stride of the count/offset matrices (max bins of a level)
     uint32_t blocks;      // count/scatter grid (a multiple of the level count)
     uint32_t level_blocks; // count/scatter blocks per level: blocks / L
+    uint32_t chunk_blocks; // 256-sample chunks per count/scatter block (block j: chunks [j, j + 1) * chunk_blocks)
     uint32_t acc_blocks;  // accumulate grid: an upper bound on the number of chunks
     uint32_t split_slots; // partial-sum slots for bins summed by several blocks (upper bound)
     uint64_t cap;         // record capacity: n_max * L * 8
@@ -457,13 +516,13 @@ __global__ void __launch_bounds__(256) hashgrid_bwd_count_kernel(uint32_t n_max,
     for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) hist[k] = 0;
     __syncthreads();
     const uint32_t n = info ? min(info[1], n_max) : n_max;
-    const uint32_t chunks = (n + 255) / 256, step = p.blocks / L;
-    uint32_t c = blockIdx.x / L;
-    HbIn next = hb_load_pos(c * 256 + threadIdx.x, n, B, pos);
-    for (; c < chunks; c += step) {
+    const uint32_t c0 = (blockIdx.x / L) * p.chunk_blocks;
+    const uint32_t c1 = min(c0 + p.chunk_blocks, (n + 255) / 256);
+    HbIn next = hb_load_pos(c0 * 256 + threadIdx.x, n, B, pos);
+    for (uint32_t c = c0; c < c1; ++c) {
         const uint32_t i = c * 256 + threadIdx.x;
         const HbIn v = next;
-        next = hb_load_pos((c + step) * 256 + threadIdx.x, n, B, pos);
+        next = hb_load_pos((c + 1) * 256 + threadIdx.x, n, B, pos);
         const bool active = hb_active(i, n);
         if (__ballot(active) == 0) continue;
         HashCorners c8;
@@ -605,7 +664,7 @@ __global__ void __launch_bounds__(kHbScatterThreads) hashgrid_bwd_scatter_kernel
     uint32_t n_max, const uint32_t* __restrict__ info, uint32_t B, uint32_t GB, tn_hashgrid_t g, HbPlan p,
     const float* __restrict__ pos, const TG* __restrict__ dfeat, const uint32_t* __restrict__ offsets,
     const uint32_t* __restrict__ start, uint32_t* __restrict__ binmax, HbRecord* __restrict__ rec, uint32_t l0,
-    uint32_t Lg) {
+    uint32_t Lg, uint32_t j0) {
     // cur: the bin's next global slot; cnt: the chunk's records per bin;
     // lstart: the chunk's first LDS place of the bin; gdel: global slot minus
     // LDS place for the bin's records of the chunk
@@ -615,10 +674,10 @@ __global__ void __launch_bounds__(kHbScatterThreads) hashgrid_bwd_scatter_kernel
     __shared__ uint32_t sslot[kHbChunkRecords];
     __shared__ uint32_t wtot[kHbScatterThreads / 64];
     const uint32_t L = (uint32_t)g.n_levels;
-    // this launch covers levels [l0, l0 + Lg); block (l, j) is block l + L j
-    // of the count pass
+    // this launch covers levels [l0, l0 + Lg) and count blocks j >= j0;
+    // block (l, j) is block l + L j of the count pass
     const int l = (int)(l0 + blockIdx.x % Lg);
-    const uint32_t j = blockIdx.x / Lg, b = (uint32_t)l + L * j;
+    const uint32_t j = j0 + blockIdx.x / Lg, b = (uint32_t)l + L * j;
     const uint32_t nb = p.count[l];
     const uint32_t tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
     for (uint32_t k = tid; k < nb; k += kHbScatterThreads) {
@@ -628,14 +687,13 @@ __global__ void __launch_bounds__(kHbScatterThreads) hashgrid_bwd_scatter_kernel
     }
     const uint32_t off = g.level_offset[l];
     const uint32_t n = info ? min(info[1], n_max) : n_max;
-    const uint32_t chunks = (n + 255) / 256, step = p.level_blocks;
-    uint32_t c = j;
-    HbIn next = hb_load<TG>(l, c * 256 + tid, n, B, GB, pos, dfeat);
-    for (; c < chunks; c += step) {
+    const uint32_t c0 = j * p.chunk_blocks, c1 = min(c0 + p.chunk_blocks, (n + 255) / 256);
+    HbIn next = hb_load<TG>(l, c0 * 256 + tid, n, B, GB, pos, dfeat);
+    for (uint32_t c = c0; c < c1; ++c) {
         __syncthreads();  // cnt cleared, srec and wtot free
         const uint32_t i = c * 256 + tid;
         const HbIn v = next;
-        next = hb_load<TG>(l, (c + step) * 256 + tid, n, B, GB, pos, dfeat);
+        next = hb_load<TG>(l, (c + 1) * 256 + tid, n, B, GB, pos, dfeat);
         HbCorners hc;
         hb_corners(g, l, hb_active(i, n), v, hc);
         uint32_t bin[8], rank[8], r0[8], r1[8];
@@ -905,6 +963,7 @@ inline bool hb_plan(const tn_hashgrid_t& g, uint32_t n_max, uint32_t level_block
     p.row = row;
     p.blocks = (uint32_t)(per_level * L);
     p.level_blocks = (uint32_t)per_level;
+    p.chunk_blocks = (uint32_t)((chunks + per_level - 1) / per_level);
     p.cap = (uint64_t)n_max * L * 8;
     if (p.cap >= (1ull << 32)) return false;  // record offsets are 32-bit
     p.acc_blocks = (uint32_t)((uint64_t)n_bins + (p.cap + kHbChunk - 1) / kHbChunk);
@@ -980,6 +1039,35 @@ inline void hashgrid_backward_plan(uint32_t n_max, const uint32_t* info, uint32_
                                                      pa.split_list);
 }
 
+// (3) for levels [l0, l0 + Lg) and the samples of count blocks [j0, j1)
+template <typename TG>
+void hb_scatter(uint32_t n_max, const uint32_t* info, uint32_t B, uint32_t GB, const tn_hashgrid_t& g, const HbPlan& p,
+                const float* pos, const TG* dfeat, const HbPlanArea& pa, const HbWorkArea& wa, uint32_t l0,
+                uint32_t Lg, uint32_t j0, uint32_t j1, hipStream_t stream) {
+    if (j1 > j0)
+        hashgrid_bwd_scatter_kernel<TG><<<(j1 - j0) * Lg, kHbScatterThreads, 0, stream>>>(
+            n_max, info, B, GB, g, p, pos, dfeat, pa.offsets, pa.start, wa.binmax, wa.rec, l0, Lg, j0);
+}
+
+// (4) for levels [l0, l0 + Lg), every record of them scattered
+inline void hb_accumulate(uint32_t n_max, const tn_hashgrid_t& g, const HbPlan& p, float* dtable,
+                          const HbPlanArea& pa, const HbWorkArea& wa, uint32_t l0, uint32_t Lg, hipStream_t stream) {
+    const uint32_t L = (uint32_t)g.n_levels;
+    const uint32_t k_lo = p.base[l0], k_hi = l0 + Lg < L ? p.base[l0 + Lg] : p.n_bins;
+    // chunks of the group's bins: at most one per bin plus one per kHbChunk records
+    const uint64_t cap_g = (uint64_t)n_max * Lg * 8;
+    const uint32_t acc_blocks = (uint32_t)((k_hi - k_lo) + (cap_g + kHbChunk - 1) / kHbChunk);
+    hashgrid_bwd_accumulate_kernel<<<acc_blocks, 512, 0, stream>>>(g, p, pa.start, pa.chunk_start, wa.binmax,
+                                                                   pa.slot_base, wa.rec, wa.slots, dtable, k_lo, k_hi);
+}
+
+// (5)
+inline void hb_finish(const tn_hashgrid_t& g, const HbPlan& p, float* dtable, const HbPlanArea& pa,
+                      const HbWorkArea& wa, hipStream_t stream) {
+    hashgrid_bwd_finish_kernel<<<kHbFinishBlocks, kHbFinishThreads, 0, stream>>>(
+        g, p, pa.start, pa.chunk_start, wa.binmax, pa.slot_base, pa.split_list, wa.slots, dtable);
+}
+
 // (3)-(5): records, sums and the table update, from the plan and the gradients
 template <typename TG>
 void hashgrid_backward_planned(uint32_t n_max, const uint32_t* info, uint32_t B, uint32_t GB, const tn_hashgrid_t& g,
@@ -992,18 +1080,10 @@ void hashgrid_backward_planned(uint32_t n_max, const uint32_t* info, uint32_t B,
     const uint32_t per = (L + G - 1) / G;
     for (uint32_t l0 = 0; l0 < L; l0 += per) {
         const uint32_t Lg = std::min(per, L - l0);
-        const uint32_t k_lo = p.base[l0], k_hi = l0 + Lg < L ? p.base[l0 + Lg] : p.n_bins;
-        // chunks of the group's bins: at most one per bin plus one per kHbChunk records
-        const uint64_t cap_g = (uint64_t)n_max * Lg * 8;
-        const uint32_t acc_blocks = (uint32_t)((k_hi - k_lo) + (cap_g + kHbChunk - 1) / kHbChunk);
-        hashgrid_bwd_scatter_kernel<TG><<<p.level_blocks * Lg, kHbScatterThreads, 0, stream>>>(
-            n_max, info, B, GB, g, p, pos, dfeat, pa.offsets, pa.start, wa.binmax, wa.rec, l0, Lg);
-        hashgrid_bwd_accumulate_kernel<<<acc_blocks, 512, 0, stream>>>(g, p, pa.start, pa.chunk_start, wa.binmax,
-                                                                       pa.slot_base, wa.rec, wa.slots, dtable, k_lo,
-                                                                       k_hi);
+        hb_scatter<TG>(n_max, info, B, GB, g, p, pos, dfeat, pa, wa, l0, Lg, 0, p.level_blocks, stream);
+        hb_accumulate(n_max, g, p, dtable, pa, wa, l0, Lg, stream);
     }
-    hashgrid_bwd_finish_kernel<<<kHbFinishBlocks, kHbFinishThreads, 0, stream>>>(
-        g, p, pa.start, pa.chunk_start, wa.binmax, pa.slot_base, pa.split_list, wa.slots, dtable);
+    hb_finish(g, p, dtable, pa, wa, stream);
 }
 
 inline bool hb_layout_ok(const tn_hashgrid_t& g) {
@@ -1055,22 +1135,44 @@ extern "C" int tn_hashgrid_forward(int table_dtype, int feat_dtype, uint32_t n_m
     TN_CHECK_ARG(pos && table && features && n_max <= stride && n_max <= out_stride && grid.n_features == 2);
     TN_CHECK_ARG(grid.n_levels >= 1 && grid.n_levels <= TN_HASHGRID_MAX_LEVELS);
     if (n_max == 0) return TN_OK;
+    if (!((table_dtype == 0 && feat_dtype == 0) || (table_dtype == 1 && (feat_dtype == 0 || feat_dtype == 1))))
+        return TN_ERR_INVALID_ARGUMENT;
     const unsigned tb = (unsigned)tune_value(TN_TUNE_HASH_FWD_BLOCK);  // <= 256, the kernel's bound
-    const dim3 blocks(blocks_for(n_max, tb), grid.n_levels);
-    if (table_dtype == 0 && feat_dtype == 0)
+    uint32_t stage = 0;
+    const int Ls = hf_lds_levels(grid, table_dtype == 1 ? 4u : 8u, table,
+                                 tune_value(TN_TUNE_HASH_FWD_LDS_LEVELS), &stage);
+    if (Ls > 0) {
+        const unsigned lb = std::min<unsigned>(blocks_for(n_max, kHfLdsThreads), compute_units());
+        if (table_dtype == 0) {
+            if (!allow_lds(hashgrid_fwd_lds_kernel<float, float>, stage)) return TN_ERR_LAUNCH;
+            hashgrid_fwd_lds_kernel<float, float><<<lb, kHfLdsThreads, stage, stream>>>(
+                n_max, info, stride, out_stride, grid, pos, static_cast<const float*>(table),
+                static_cast<float*>(features), Ls, stage);
+        } else if (feat_dtype == 1) {
+            if (!allow_lds(hashgrid_fwd_lds_kernel<_Float16, _Float16>, stage)) return TN_ERR_LAUNCH;
+            hashgrid_fwd_lds_kernel<_Float16, _Float16><<<lb, kHfLdsThreads, stage, stream>>>(
+                n_max, info, stride, out_stride, grid, pos, static_cast<const _Float16*>(table),
+                static_cast<_Float16*>(features), Ls, stage);
+        } else {
+            if (!allow_lds(hashgrid_fwd_lds_kernel<_Float16, float>, stage)) return TN_ERR_LAUNCH;
+            hashgrid_fwd_lds_kernel<_Float16, float><<<lb, kHfLdsThreads, stage, stream>>>(
+                n_max, info, stride, out_stride, grid, pos, static_cast<const _Float16*>(table),
+                static_cast<float*>(features), Ls, stage);
+        }
+    }
+    const dim3 blocks(blocks_for(n_max, tb), grid.n_levels - Ls);
+    if (table_dtype == 0)
         hashgrid_fwd_kernel<float, float><<<blocks, tb, 0, stream>>>(n_max, info, stride, out_stride, grid, pos,
                                                                       static_cast<const float*>(table),
-                                                                      static_cast<float*>(features));
-    else if (table_dtype == 1 && feat_dtype == 1)
+                                                                      static_cast<float*>(features), Ls);
+    else if (feat_dtype == 1)
         hashgrid_fwd_kernel<_Float16, _Float16><<<blocks, tb, 0, stream>>>(
             n_max, info, stride, out_stride, grid, pos, static_cast<const _Float16*>(table),
-            static_cast<_Float16*>(features));
-    else if (table_dtype == 1 && feat_dtype == 0)
+            static_cast<_Float16*>(features), Ls);
+    else
         hashgrid_fwd_kernel<_Float16, float><<<blocks, tb, 0, stream>>>(n_max, info, stride, out_stride, grid, pos,
                                                                          static_cast<const _Float16*>(table),
-                                                                         static_cast<float*>(features));
-    else
-        return TN_ERR_INVALID_ARGUMENT;
+                                                                         static_cast<float*>(features), Ls);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }
@@ -1191,6 +1293,72 @@ extern "C" int tn_hashgrid_backward_planned(int dfeat_dtype, uint32_t n_max, con
     else
         hashgrid_backward_planned<_Float16>(n_max, info, stride, dfeat_stride, grid, p, pos,
                                             static_cast<const _Float16*>(dfeat), dtable, pa, wa, stream);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+// The gradient passes of tn_hashgrid_backward_planned in parts, for a caller
+// whose feature gradients of a batch become ready in sample ranges: the
+// records of samples [sample_begin, sample_end) of every level are scattered
+// as soon as their gradients exist (ranges are whole count blocks of
+// tn_hashgrid_backward_part_samples samples each; the end may be n_max), and
+// tn_hashgrid_backward_accumulate sums them once every range of the batch is
+// scattered. Same result as tn_hashgrid_backward_planned bit for bit.
+extern "C" uint32_t tn_hashgrid_backward_part_samples(uint32_t n_max, tn_hashgrid_t grid) {
+    HbPlan p;
+    return hb_split_plan(grid, n_max, p) ? p.chunk_blocks * 256u : 0u;
+}
+
+extern "C" int tn_hashgrid_backward_scatter(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
+                                            uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos,
+                                            const void* dfeat, const void* plan, size_t plan_bytes, void* work,
+                                            size_t work_bytes, uint32_t sample_begin, uint32_t sample_end,
+                                            hipStream_t stream) {
+    TN_CHECK_ARG(pos && dfeat && plan && work && n_max <= stride && n_max <= dfeat_stride);
+    TN_CHECK_ARG(dfeat_dtype == 0 || dfeat_dtype == 1);
+    HbPlan p;
+    TN_CHECK_ARG(n_max == 0 || hb_split_plan(grid, n_max, p));
+    if (n_max == 0) return TN_OK;
+    const uint32_t part = p.chunk_blocks * 256u;
+    TN_CHECK_ARG(sample_begin <= sample_end && sample_begin % part == 0 &&
+                 (sample_end % part == 0 || sample_end >= n_max));
+    TN_CHECK_ARG(plan_bytes >= hb_plan_area(p, nullptr, nullptr) && work_bytes >= hb_work_area(p, nullptr, nullptr));
+    TN_CHECK_ARG(((reinterpret_cast<uintptr_t>(plan) | reinterpret_cast<uintptr_t>(work)) & 255) == 0);
+    const uint32_t j0 = std::min(sample_begin / part, p.level_blocks);
+    // the last part runs every remaining block: blocks past the batch still
+    // write their (zero) bin maxima, which the sums read
+    const uint32_t j1 = sample_end >= n_max ? p.level_blocks : std::min(sample_end / part, p.level_blocks);
+    HbPlanArea pa;
+    HbWorkArea wa;
+    hb_plan_area(p, const_cast<void*>(plan), &pa);
+    hb_work_area(p, work, &wa);
+    const uint32_t L = (uint32_t)grid.n_levels;
+    if (dfeat_dtype == 0)
+        hb_scatter<float>(n_max, info, stride, dfeat_stride, grid, p, pos, static_cast<const float*>(dfeat), pa, wa,
+                          0, L, j0, j1, stream);
+    else
+        hb_scatter<_Float16>(n_max, info, stride, dfeat_stride, grid, p, pos, static_cast<const _Float16*>(dfeat),
+                             pa, wa, 0, L, j0, j1, stream);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+extern "C" int tn_hashgrid_backward_accumulate(uint32_t n_max, tn_hashgrid_t grid, float* dtable, const void* plan,
+                                               size_t plan_bytes, void* work, size_t work_bytes,
+                                               hipStream_t stream) {
+    TN_CHECK_ARG(dtable && plan && work);
+    HbPlan p;
+    TN_CHECK_ARG(n_max == 0 || hb_split_plan(grid, n_max, p));
+    if (n_max == 0) return TN_OK;
+    TN_CHECK_ARG(plan_bytes >= hb_plan_area(p, nullptr, nullptr) && work_bytes >= hb_work_area(p, nullptr, nullptr));
+    TN_CHECK_ARG(((reinterpret_cast<uintptr_t>(plan) | reinterpret_cast<uintptr_t>(work)) & 255) == 0 &&
+                 (reinterpret_cast<uintptr_t>(dtable) & 7) == 0);
+    HbPlanArea pa;
+    HbWorkArea wa;
+    hb_plan_area(p, const_cast<void*>(plan), &pa);
+    hb_work_area(p, work, &wa);
+    hb_accumulate(n_max, grid, p, dtable, pa, wa, 0, (uint32_t)grid.n_levels, stream);
+    hb_finish(grid, p, dtable, pa, wa, stream);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }

@@ -638,6 +638,14 @@ __global__ void accumulate_samples_kernel(const uint32_t* __restrict__ info, uns
     total[0] += (unsigned long long)info[1];
 }
 
+__global__ void batch_tail_kernel(const uint32_t* __restrict__ info, uint32_t first, uint32_t* __restrict__ tail) {
+    const uint32_t n = info[1];
+    tail[0] = info[0];
+    tail[1] = n > first ? n - first : 0u;
+    tail[2] = info[2];
+    tail[3] = info[3];
+}
+
 }  // namespace tn
 
 using namespace tn;
@@ -646,6 +654,13 @@ using namespace tn;
 extern "C" int tn_accumulate_samples(const uint32_t* info, unsigned long long* total, hipStream_t stream) {
     TN_CHECK_ARG(info && total);
     accumulate_samples_kernel<<<1, 1, 0, stream>>>(info, total);
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+extern "C" int tn_batch_tail(const uint32_t* info, uint32_t first, uint32_t* tail_info, hipStream_t stream) {
+    TN_CHECK_ARG(info && tail_info && info != tail_info);
+    batch_tail_kernel<<<1, 1, 0, stream>>>(info, first, tail_info);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }

@@ -945,35 +945,10 @@ constexpr size_t bwd_lds_bytes() {
 static_assert(bwd_lds_bytes<PrecF16>() <= 160 * 1024 && bwd_lds_bytes<PrecF32>() <= 160 * 1024,
               "the backward kernel's LDS must fit one CU (160 KiB)");
 
-// kernels using more than the default 64 KiB of dynamic LDS opt in once
-template <typename K>
-inline bool allow_lds(K kernel, size_t bytes) {
-    if (bytes <= 65536) return true;
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bytes) == hipSuccess)
-        return true;
-    (void)hipGetLastError();  // do not leave the failure for the next launch check
-    return false;
-}
-
 // Persistent launches: as many workgroups as fit on the device at once (the
 // backward kernel holds one workgroup per CU, the forward two), each looping
 // over 32-sample tiles. Fewer workgroups also means fewer weight stagings and
 // fewer gradient partials to reduce.
-inline unsigned compute_units() {
-    static int cached_dev = -1;
-    static unsigned cached_cus = 0;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 256;
-    if (dev != cached_dev) {
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
-        cached_cus = (unsigned)cus;
-        cached_dev = dev;
-    }
-    return cached_cus;
-}
-
 inline unsigned mlp_blocks(uint32_t n_max, unsigned per_cu) {
     const uint32_t tiles = (n_max + 31) / 32;
     const unsigned b = blocks_for(tiles, kMlpWaves);

@@ -234,11 +234,13 @@ def hashgrid_forward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: in
 
 def hashgrid_backward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: int, dfeat: torch.Tensor,
                       dtable: torch.Tensor, info: torch.Tensor | None = None, dfeat_stride: int | None = None,
-                      binned: bool = False, planned: bool = False):
+                      binned: bool = False, planned: bool = False, split: int | None = None):
     """dtable += the table gradient. binned: the bin-sort kernels (with their
     workspace) instead of the atomic scatter-add; planned: the same in two
     calls, the plan from the positions first (tn_hashgrid_backward_plan),
-    then the gradient passes."""
+    then the gradient passes; with `split` (a multiple of
+    tn_hashgrid_backward_part_samples) the records of samples [split, n) are
+    scattered first, then those of [0, split), then summed."""
     dfeat_stride = stride if dfeat_stride is None else dfeat_stride
     if planned:
         pb = query("tn_hashgrid_backward_plan_bytes", n, grid)
@@ -248,8 +250,14 @@ def hashgrid_backward(grid: _capi.HashGrid, n: int, pos: torch.Tensor, stride: i
         plan = torch.empty(pb, dtype=torch.uint8, device="cuda")
         work = torch.empty(wb, dtype=torch.uint8, device="cuda")
         call("tn_hashgrid_backward_plan", n, ptr(info), stride, grid, ptr(pos), ptr(plan), pb, stream_ptr())
-        call("tn_hashgrid_backward_planned", _PREC[dfeat.dtype], n, ptr(info), stride, dfeat_stride, grid, ptr(pos),
-             ptr(dfeat), ptr(dtable), ptr(plan), pb, ptr(work), wb, stream_ptr())
+        if split is None:
+            call("tn_hashgrid_backward_planned", _PREC[dfeat.dtype], n, ptr(info), stride, dfeat_stride, grid,
+                 ptr(pos), ptr(dfeat), ptr(dtable), ptr(plan), pb, ptr(work), wb, stream_ptr())
+            return
+        for lo, hi in ((split, n), (0, split)):
+            call("tn_hashgrid_backward_scatter", _PREC[dfeat.dtype], n, ptr(info), stride, dfeat_stride, grid,
+                 ptr(pos), ptr(dfeat), ptr(plan), pb, ptr(work), wb, lo, hi, stream_ptr())
+        call("tn_hashgrid_backward_accumulate", n, grid, ptr(dtable), ptr(plan), pb, ptr(work), wb, stream_ptr())
         return
     if binned:
         nb = query("tn_hashgrid_backward_workspace_bytes", n, grid)
