@@ -55,6 +55,15 @@ ALGO = {
     "adam": ("hbm", 0, 0, 36),
     "loss_reduce": ("hbm", 0, 16, 0),
 }
+# with Adam on the hash table fused into the hash-grid backward's sum passes
+# (the default): the backward moves the per-sample inputs and, per table
+# parameter, the Adam state (fp32 weight, moments and step read and written,
+# the fp16 copy written: 34 B; 32 B in fp32 training) instead of gradient
+# pairs per corner; Adam's own pass covers the network weights (36 B each)
+ALGO_FUSED = {
+    "hashgrid_backward": ("hbm", 12 + 128, 0, 34),
+    "adam": ("hbm", 0, 0, 36),
+}
 # phases whose bytes depend on the network precision (fp32 tables)
 ALGO_FP32 = {"hashgrid_forward": ("hbm", 12 + 1024 + 128, 0, 0)}
 # the kernels each phase launches (rocprofv3 kernel names)
@@ -275,16 +284,26 @@ def fold_phases(phases):
     return out
 
 
-def roofline(res, n_params, dtype):
+def roofline(res, n_params, dtype, n_net=9408):
     """Achieved rate of every phase from its algorithmic bytes/FLOPs; the
-    roofline object is the phase with the largest share of the step."""
+    roofline object is the phase with the largest share of the step. With
+    Adam fused into the hash-grid backward (the run's tuning pick), that
+    phase's per-parameter bytes count the table's n_params - n_net
+    parameters and Adam's the network's n_net."""
     S, R = res["samples_per_step"], res["rays_per_step"]
+    fused = res.get("autotune", {}).get("picks", {}).get("hash_adam_fused", 0) == 1
     table = {}
     for ph, v in res["phases"].items():
         if ph not in ALGO:
             continue
         bound, ps, pr, pp = ALGO_FP32.get(ph, ALGO[ph]) if dtype == "fp32" else ALGO[ph]
-        units = ps * S + pr * R + pp * n_params
+        n_p = n_params
+        if fused and ph in ALGO_FUSED:
+            bound, ps, pr, pp = ALGO_FUSED[ph]
+            n_p = n_params - n_net if ph == "hashgrid_backward" else n_net
+            if ph == "hashgrid_backward" and dtype == "fp32":
+                pp = 32
+        units = ps * S + pr * R + pp * n_p
         sec = v["ms"] * 1e-3
         if bound == "hbm":
             ach, peak, unit = units / sec / 1e9, PEAK_HBM_GBS, "GB/s"

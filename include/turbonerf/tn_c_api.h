@@ -288,6 +288,23 @@ int tn_hashgrid_backward_scatter(int dfeat_dtype, uint32_t n_max, const uint32_t
     tn_stream_t stream);
 int tn_hashgrid_backward_accumulate(uint32_t n_max, tn_hashgrid_t grid, float* dtable, const void* plan,
     size_t plan_bytes, void* work, size_t work_bytes, tn_stream_t stream);
+/* tn_hashgrid_backward_planned / _accumulate with the optimiser step of the
+ * hash table fused in: each table entry takes NGP Adam (the rule of
+ * tn_adam_step: loss scale divided out, zero-gradient parameters skipped,
+ * per-parameter steps, fp16 copy) where its gradient sum is formed, so the
+ * table's gradient is never written and tn_adam_step then runs over the
+ * network range only. Parameters, moments and steps equal those of the
+ * unfused backward followed by tn_adam_step bit for bit. dtable: the fp32
+ * table-gradient buffer, all zero on entry (only non-finite or oversized
+ * split bins pass through it) and all zero on return. Replaces the hash
+ * range of the optimiser step (reference src/core/adam-optimizer.cuh:48-96,
+ * called at src/core/nerf-network.cu:770-781). */
+int tn_hashgrid_backward_planned_adam(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
+    uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos, const void* dfeat, float* dtable,
+    const void* plan, size_t plan_bytes, void* work, size_t work_bytes, const tn_adam_t* adam,
+    tn_stream_t stream);
+int tn_hashgrid_backward_accumulate_adam(uint32_t n_max, tn_hashgrid_t grid, float* dtable, const void* plan,
+    size_t plan_bytes, void* work, size_t work_bytes, const tn_adam_t* adam, tn_stream_t stream);
 
 
 /* ------------------------------------------------------------------- MLPs */

@@ -43,7 +43,9 @@ enum tn_tune_knob {
                                           only those whose tables fit 64 KiB together are staged */
     TN_TUNE_STEP_HALVES = 8,           /* training step: the per-sample kernels over the batch's two halves on two
                                           streams, so one half's MLP passes overlap the other's hash-grid passes, 0..1 (0) */
-    TN_TUNE_COUNT = 9
+    TN_TUNE_HASH_ADAM_FUSED = 9,       /* training step: Adam on the hash table applied by the hash-grid backward's
+                                          sum passes instead of a pass of its own, 0..1 (0) */
+    TN_TUNE_COUNT = 10
 };
 
 /* Axis-aligned box; same field order as BoundingBox
@@ -146,6 +148,24 @@ typedef struct tn_appearance {
     float* dz;                 /* backward only (may be null): dL/dz of the colour MLP's first layer,
                                   fp32 [64][stride], written for the batch's samples */
 } tn_appearance_t;
+
+/* NGP Adam settings for the hash-table range of the parameters (entries
+ * [0, n_grid/2) of two parameters each), as tn_adam_step takes them: the
+ * hash-grid backward can apply the step where it forms each entry's
+ * gradient (tn_hashgrid_backward_planned_adam / _accumulate_adam). */
+typedef struct tn_adam {
+    float loss_scale;                     /* gradients are divided by it (and by 2^ceil(log2(info[0]))) */
+    const uint32_t* scale_by_rays_info;   /* batch descriptor (may be null: loss_scale only) */
+    float learning_rate;                  /* lr0; with d_step: lr0 / (1 + lr_decay * d_step[0]) */
+    const uint32_t* d_step;
+    float lr_decay, beta1, beta2, epsilon;
+    float* weights;                       /* fp32 master weights (these four arrays 16-byte aligned) */
+    int lp_dtype;                         /* 0: no low-precision copy, 1: fp16 copy in weights_lp */
+    void* weights_lp;                     /* 8-byte aligned */
+    float* first_moments;
+    float* second_moments;
+    uint32_t* param_steps;
+} tn_adam_t;
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,11 @@ kernel, on the default 16-level layout (T = 2^19):
   records, summed in parts);
 * accumulation into a non-zero table, values spanning 1e-6..1e3 (the
   fixed-point scale follows each bin's largest value), and a non-finite
-  gradient (the bin falls back to float sums and the inf reaches the table).
+  gradient (the bin falls back to float sums and the inf reaches the table);
+* Adam on the table fused into the sum passes
+  (tn_hashgrid_backward_planned_adam): the parameters, moments, steps and
+  fp16 copy of the unfused backward followed by tn_adam_step, bit for bit,
+  with split bins and a non-finite gradient.
 
 Bar: norm-wise 1e-6 against fp64 (atomic kernel), 2e-6 (binned); for the
 binned kernel also per entry |err| <= 6e-6 sum|w g| (fp32 products and run
@@ -270,3 +274,82 @@ def test_planned_equals_one_call(cuda, half):
     assert np.abs(runs[0]).max() > 0
     for r in runs[1:]:
         assert np.array_equal(runs[0].view(np.uint32), r.view(np.uint32))
+
+
+@pytest.mark.parametrize("lp", [0, 1])
+@pytest.mark.parametrize("nonfinite", [False, True])
+def test_planned_adam_equals_backward_then_adam(cuda, lp, nonfinite):
+    """Adam fused into the sum passes against tn_hashgrid_backward_planned
+    then tn_adam_step over the table: random points (level 0's single bin
+    holds 8 records per sample, summed in parts and finished by the last
+    pass), rays (merged runs), a share of zero gradients, the batch bounded
+    by a device count; with `nonfinite` an inf gradient puts level 0's split
+    bin on float sums through the table-gradient buffer. Weights, moments,
+    per-parameter steps and the fp16 copy agree bit for bit (that float-summed
+    bin to rounding: its atomic sums take a different order in each run),
+    and the gradient buffer is all zero afterwards in both."""
+    import torch
+
+    from turbonerf_amd import _capi, ops
+
+    rng = np.random.default_rng(31)
+    g = ops.hashgrid_layout()
+    P = g.n_params
+    pos = np.concatenate([_rays(rng, 256, 64, 0.004), rng.uniform(0, 1, size=(3, 16384)).astype(np.float32)], axis=1)
+    n = pos.shape[1]
+    df = rng.normal(0.0, 1.0, size=(32, n)).astype(np.float32) * 1e3
+    df[:, rng.uniform(size=n) < 0.3] = 0.0
+    if nonfinite:
+        df[0, 20000] = np.inf
+    pos_d = torch.from_numpy(np.ascontiguousarray(pos)).cuda().view(-1)
+    df_d = torch.from_numpy(df).cuda().view(-1)
+    info = torch.tensor([1000, n - 500, 0, 0], dtype=torch.int32, device="cuda")
+    w0 = torch.from_numpy(rng.uniform(-1e-2, 1e-2, size=P).astype(np.float32)).cuda()
+    m0 = torch.from_numpy(rng.normal(0.0, 1e-3, size=P).astype(np.float32)).cuda()
+    v0 = torch.from_numpy(np.abs(rng.normal(0.0, 1e-6, size=P)).astype(np.float32)).cuda()
+    s0 = torch.from_numpy(rng.integers(0, 100, size=P).astype(np.int32)).cuda()
+    pb = ops.query("tn_hashgrid_backward_plan_bytes", n, g)
+    wb = ops.query("tn_hashgrid_backward_work_bytes", n, g)
+    plan = torch.empty(pb, dtype=torch.uint8, device="cuda")
+    _capi.call("tn_hashgrid_backward_plan", n, _capi.ptr(info), n, g, _capi.ptr(pos_d), _capi.ptr(plan), pb,
+               _capi.stream_ptr())
+    ls, lr, b1, b2, eps = 1024.0, 1e-2, 0.9, 0.99, 1e-15
+    out = []
+    for fused in (False, True):
+        w, m, v, st = w0.clone(), m0.clone(), v0.clone(), s0.clone()
+        wlp = w0.half() if lp else None
+        dt = torch.zeros(P, dtype=torch.float32, device="cuda")
+        work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+        args = (0, n, _capi.ptr(info), n, n, g, _capi.ptr(pos_d), _capi.ptr(df_d), _capi.ptr(dt), _capi.ptr(plan), pb,
+                _capi.ptr(work), wb)
+        if fused:
+            a = _capi.Adam(ls, _capi.ptr(info), lr, None, 0.0, b1, b2, eps, _capi.ptr(w), lp, _capi.ptr(wlp),
+                           _capi.ptr(m), _capi.ptr(v), _capi.ptr(st))
+            _capi.call("tn_hashgrid_backward_planned_adam", *args, ctypes.byref(a), _capi.stream_ptr())
+        else:
+            _capi.call("tn_hashgrid_backward_planned", *args, _capi.stream_ptr())
+            _capi.call("tn_adam_step", P, P, P, ls, _capi.ptr(info), lr, None, 0.0, b1, b2, eps, 0.0, _capi.ptr(w),
+                       lp, _capi.ptr(wlp), _capi.ptr(dt), _capi.ptr(m), _capi.ptr(v), _capi.ptr(st),
+                       _capi.stream_ptr())
+        torch.cuda.synchronize()
+        out.append([t.cpu().numpy() for t in (w, m, v, st, dt)] + ([wlp.cpu().numpy()] if lp else []))
+    changed = int((out[0][3] != s0.cpu().numpy()).sum())
+    print(f"[hash bwd adam lp={lp} nonfinite={nonfinite}] {changed} parameters stepped, "
+          f"{int(np.isnan(out[0][0]).sum())} NaN weights", flush=True)
+    assert changed > 0.1 * P
+    # with the inf, level 0's bin is summed in float by atomics in both runs
+    # (summation order, so rounding, differs from run to run): close there,
+    # NaNs in the same places (a NaN's sign and payload are not specified),
+    # and bit for bit everywhere else
+    k0 = 2 * g.level_size[0] if nonfinite else 0
+    for a, b, name in zip(out[0], out[1], ("w", "m", "v", "steps", "grad", "w_lp")):
+        a0, b0, a, b = a[:k0], b[:k0], a[k0:], b[k0:]
+        if k0 and a0.dtype.kind == "f":
+            assert np.array_equal(np.isnan(a0), np.isnan(b0)), name
+            ok = ~np.isnan(a0)
+            assert np.allclose(a0[ok].astype(np.float64), b0[ok], rtol=1e-4, atol=1e-9), name
+        elif k0:
+            assert np.array_equal(a0, b0), name
+        view = np.uint16 if a.dtype == np.float16 else np.uint32
+        assert np.array_equal(a.view(view), b.view(view)), name
+    assert not out[1][4].any() and not out[0][4].any()

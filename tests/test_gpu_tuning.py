@@ -1,7 +1,8 @@
 """Launch-shape knobs and the autotuner on the GPU.
 
-* Every knob but the MLP backward's workgroups per CU leaves training
-  results bit for bit unchanged (the hash-grid backward's workgroups per
+* Every knob but the MLP backward's workgroups per CU and the two-half step
+  leaves training results bit for bit unchanged (among them Adam on the hash
+  table fused into the hash-grid backward, against its own pass) (the hash-grid backward's workgroups per
   level shape only the one-call binned backward, not the trainer's planned
   one): two trainers with the same seed and data,
   one stepped under the default launch shapes and one under other values of
@@ -24,9 +25,10 @@ pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"hash_fwd_block": 256, "hash_bwd_level_blocks": 128, "mlp_fwd_blocks_per_cu": 2,
             "mlp_bwd_blocks_per_cu": 1, "march_max_blocks": 2048, "adam_block": 256, "hash_bwd_level_groups": 1,
-            "hash_fwd_lds_levels": 0, "step_halves": 0}
+            "hash_fwd_lds_levels": 0, "step_halves": 0, "hash_adam_fused": 0}
 OTHER = {"hash_fwd_block": 64, "hash_bwd_level_blocks": 16, "mlp_fwd_blocks_per_cu": 1, "march_max_blocks": 512,
-         "adam_block": 1024, "hash_bwd_level_groups": 3, "hash_fwd_lds_levels": 2, "step_halves": 0}
+         "adam_block": 1024, "hash_bwd_level_groups": 3, "hash_fwd_lds_levels": 2, "step_halves": 0,
+         "hash_adam_fused": 1}
 
 
 def _set(tn, knobs):
@@ -79,7 +81,8 @@ def test_launch_shapes_leave_results_unchanged(tuning, precision):
         assert np.float32(mr.loss) == np.float32(ma.loss)
         _same(ref[2], alt[2], f"eager round {r}")
     caps = alt[2]._graph_captures()
-    for r, knobs in enumerate(({"march_max_blocks": 1024}, {"adam_block": 128}, {"hash_bwd_level_groups": 16})):
+    for r, knobs in enumerate(({"march_max_blocks": 1024}, {"adam_block": 128}, {"hash_bwd_level_groups": 16},
+                               {"hash_adam_fused": 1, "hash_bwd_level_groups": 4})):
         _set(tn, DEFAULTS)
         ref[2].train_steps(4, read_metrics=False, graph=True)
         _set(tn, {**OTHER, **knobs})
@@ -152,10 +155,11 @@ def test_autotune_picks_from_the_candidates_and_training_goes_on(tuning):
     print(f"[tuning] single: {res['picks']} {res['ns_per_sample_before']:.3f} -> "
           f"{res['ns_per_sample_after']:.3f} ns/sample", flush=True)
     # the hash-grid backward's workgroups per level are not searched (the
-    # trainer's backward plans use a fixed 256), the other eight are
+    # trainer's backward plans use a fixed 256), the other nine are
     cands = {"hash_fwd_block": [64, 128, 256], "mlp_fwd_blocks_per_cu": [1, 2, 3, 4], "mlp_bwd_blocks_per_cu": [1, 2],
              "march_max_blocks": [512, 1024, 2048, 4096], "adam_block": [128, 256, 512, 1024],
-             "hash_bwd_level_groups": [1, 2, 4, 8], "hash_fwd_lds_levels": [0, 1, 2], "step_halves": [0, 1]}
+             "hash_bwd_level_groups": [1, 2, 4, 8], "hash_fwd_lds_levels": [0, 1, 2], "step_halves": [0, 1],
+             "hash_adam_fused": [0, 1]}
     assert set(res["picks"]) == set(DEFAULTS) and tn.get_tuning() == res["picks"]
     assert set(res["trials"]) == set(cands)
     for k, c in cands.items():

@@ -978,13 +978,13 @@ def test_tuning_knobs_validate_and_count_changes():
 
     lib = _capi.lib()
     v = ctypes.c_int()
-    defaults = {0: 256, 1: 128, 2: 2, 3: 1, 4: 2048, 5: 256, 6: 1, 7: 0, 8: 0}
+    defaults = {0: 256, 1: 128, 2: 2, 3: 1, 4: 2048, 5: 256, 6: 1, 7: 0, 8: 0, 9: 0}
     for k, d in defaults.items():
         assert lib.tn_tune_get(k, ctypes.byref(v)) == 0 and v.value == d
-    assert lib.tn_tune_get(9, ctypes.byref(v)) == 1 and lib.tn_tune_get(-1, ctypes.byref(v)) == 1
+    assert lib.tn_tune_get(10, ctypes.byref(v)) == 1 and lib.tn_tune_get(-1, ctypes.byref(v)) == 1
     g0 = lib.tn_tune_generation()
     bad = {0: [32, 192, 512], 1: [7, 257], 2: [0, 5], 3: [0, 3], 4: [255, 8193], 5: [0, 96, 1088], 6: [0, 17],
-           7: [-1, 5], 8: [-1, 2], 9: [1]}
+           7: [-1, 5], 8: [-1, 2], 9: [-1, 2], 10: [1]}
     for k, vals in bad.items():
         for x in vals:
             assert lib.tn_tune_set(k, x) == 1, (k, x)

@@ -107,6 +107,26 @@ class Appearance(C.Structure):
     ]
 
 
+class Adam(C.Structure):
+    """tn_adam_t: NGP Adam settings for the hash-table range."""
+    _fields_ = [
+        ("loss_scale", C.c_float),
+        ("scale_by_rays_info", C.c_void_p),
+        ("learning_rate", C.c_float),
+        ("d_step", C.c_void_p),
+        ("lr_decay", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("epsilon", C.c_float),
+        ("weights", C.c_void_p),
+        ("lp_dtype", C.c_int),
+        ("weights_lp", C.c_void_p),
+        ("first_moments", C.c_void_p),
+        ("second_moments", C.c_void_p),
+        ("param_steps", C.c_void_p),
+    ]
+
+
 vp = C.c_void_p
 u32 = C.c_uint32
 u64 = C.c_uint64
@@ -160,6 +180,9 @@ _SIGS: dict[str, tuple] = {
     "tn_hashgrid_backward_scatter": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, C.c_size_t, vp,
                                                C.c_size_t, u32, u32, vp]),
     "tn_hashgrid_backward_accumulate": (C.c_int, [u32, HashGrid, vp, vp, C.c_size_t, vp, C.c_size_t, vp]),
+    "tn_hashgrid_backward_planned_adam": (C.c_int, [C.c_int, u32, vp, u32, u32, HashGrid, vp, vp, vp, vp,
+                                                    C.c_size_t, vp, C.c_size_t, vp, vp]),
+    "tn_hashgrid_backward_accumulate_adam": (C.c_int, [u32, HashGrid, vp, vp, C.c_size_t, vp, C.c_size_t, vp, vp]),
     "tn_mlp_forward": (C.c_int, [C.c_int, u32, vp, u32, u32, u32, vp, u32, vp, vp, C.c_int, vp, vp, vp, vp]),
     "tn_mlp_backward_workspace_bytes": (sz, [u32]),
     "tn_mlp_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, vp, u32, vp, vp, vp, vp, u32, vp, vp, f32, C.c_int, vp,

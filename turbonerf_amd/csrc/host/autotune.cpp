@@ -18,7 +18,8 @@ namespace {
 constexpr int kOrder[] = {TN_TUNE_STEP_HALVES,           TN_TUNE_HASH_BWD_LEVEL_GROUPS,
                           TN_TUNE_MLP_BWD_BLOCKS_PER_CU, TN_TUNE_MLP_FWD_BLOCKS_PER_CU,
                           TN_TUNE_HASH_FWD_BLOCK,        TN_TUNE_MARCH_MAX_BLOCKS,
-                          TN_TUNE_ADAM_BLOCK,            TN_TUNE_HASH_FWD_LDS_LEVELS};
+                          TN_TUNE_ADAM_BLOCK,            TN_TUNE_HASH_FWD_LDS_LEVELS,
+                          TN_TUNE_HASH_ADAM_FUSED};
 
 int get(int knob) {
     int v = 0;
@@ -39,6 +40,7 @@ const char* Autotuner::knob_name(int knob) {
     case TN_TUNE_HASH_BWD_LEVEL_GROUPS: return "hash_bwd_level_groups";
     case TN_TUNE_HASH_FWD_LDS_LEVELS: return "hash_fwd_lds_levels";
     case TN_TUNE_STEP_HALVES: return "step_halves";
+    case TN_TUNE_HASH_ADAM_FUSED: return "hash_adam_fused";
     default: throw std::runtime_error("autotune: unknown knob");
     }
 }
@@ -54,6 +56,7 @@ std::vector<int> Autotuner::candidates(int knob) {
     case TN_TUNE_HASH_BWD_LEVEL_GROUPS: return {1, 2, 4, 8};
     case TN_TUNE_HASH_FWD_LDS_LEVELS: return {0, 1, 2};
     case TN_TUNE_STEP_HALVES: return {0, 1};
+    case TN_TUNE_HASH_ADAM_FUSED: return {0, 1};
     default: throw std::runtime_error("autotune: unknown knob");
     }
 }

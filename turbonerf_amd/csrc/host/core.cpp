@@ -288,14 +288,38 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
                                       p.gradients.data() + p.w3e_offset(), p.gradients.data() + p.emb_offset(),
                                       w.app_ws.data(), w.app_ws.bytes(), ts));
     }
+    // optimiser settings (reference :770-781, lr schedule :256; from the
+    // device step counter in a captured step)
+    const bool dev_lr = st.step_for_lr && st.step_counter;
+    const float lr = dev_lr ? config.learning_rate
+                            : config.learning_rate / (1.0f + config.learning_rate_decay * (float)step);
+    // With the batch's plan, the hash table takes its Adam step inside the
+    // hash-grid backward's sum passes (its gradient is never written and
+    // re-read); the separate Adam pass then covers the network weights and
+    // embeddings only. Same parameters, moments and steps bit for bit.
+    int fused_knob = 0;
+    TN_TRY(tn_tune_get(TN_TUNE_HASH_ADAM_FUSED, &fused_knob));
+    const bool fused = fused_knob && st.hash_plan;
+    const tn_adam_t adam_hash{config.loss_scale(), info, lr, dev_lr ? st.step_counter : nullptr,
+                              config.learning_rate_decay, config.beta1, config.beta2, config.epsilon,
+                              p.params_fp.data(), prec, prec == 1 ? p.params_lp.data() : nullptr, p.m1.data(),
+                              p.m2.data(), p.steps.data()};
     {
         ScopedPhase ph(tm, "hashgrid_backward", s);
         // the bin-sort scatter (its workspace follows the batch size and the
         // layout); with the batch's plan formed ahead, only the passes that
         // need the gradients run here
-        if (split)
+        if (split && fused)
+            TN_TRY(tn_hashgrid_backward_accumulate_adam(B, grid, p.gradients.data(), st.hash_plan,
+                                                        st.hash_plan_bytes, w.hash_ws.data(), w.hash_ws.size(),
+                                                        &adam_hash, ts));
+        else if (split)
             TN_TRY(tn_hashgrid_backward_accumulate(B, grid, p.gradients.data(), st.hash_plan, st.hash_plan_bytes,
                                                    w.hash_ws.data(), w.hash_ws.size(), ts));
+        else if (fused)
+            TN_TRY(tn_hashgrid_backward_planned_adam(0, B, info, B, B, grid, pos_batch, w.dfeat.data(),
+                                                     p.gradients.data(), st.hash_plan, st.hash_plan_bytes,
+                                                     w.hash_ws.data(), w.hash_ws.size(), &adam_hash, ts));
         else if (st.hash_plan)
             TN_TRY(tn_hashgrid_backward_planned(0, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(),
                                                 st.hash_plan, st.hash_plan_bytes, w.hash_ws.data(), w.hash_ws.size(),
@@ -304,18 +328,17 @@ void NerfNetwork::train(hipStream_t s, NetworkParamsWorkspace& p, uint32_t step,
             TN_TRY(tn_hashgrid_backward_binned(0, B, info, B, B, grid, pos_batch, w.dfeat.data(), p.gradients.data(),
                                                w.hash_ws.data(), w.hash_ws.size(), ts));
     }
-    // optimiser (reference :770-781, lr schedule :256; from the device step
-    // counter in a captured step)
-    const bool dev_lr = st.step_for_lr && st.step_counter;
-    const float lr = dev_lr ? config.learning_rate
-                            : config.learning_rate / (1.0f + config.learning_rate_decay * (float)step);
     {
         ScopedPhase ph(tm, "adam", s);
-        TN_TRY(tn_adam_step(p.n_total, p.n_grid, p.n_net_end(), config.loss_scale(), info, lr,
+        // the parameters from `first` on: everything, or past the hash table
+        // when its step is fused above
+        const uint32_t first = fused ? p.n_grid : 0u;
+        uint8_t* lp = prec == 1 ? p.params_lp.data() + 2 * (size_t)first : nullptr;
+        TN_TRY(tn_adam_step(p.n_total - first, p.n_grid - first, p.n_net_end() - first, config.loss_scale(), info, lr,
                             dev_lr ? st.step_counter : nullptr, config.learning_rate_decay, config.beta1,
-                            config.beta2, config.epsilon, config.l2_reg, p.params_fp.data(), prec,
-                            prec == 1 ? p.params_lp.data() : nullptr, p.gradients.data(), p.m1.data(), p.m2.data(),
-                            p.steps.data(), ts));
+                            config.beta2, config.epsilon, config.l2_reg, p.params_fp.data() + first, prec, lp,
+                            p.gradients.data() + first, p.m1.data() + first, p.m2.data() + first,
+                            p.steps.data() + first, ts));
     }
     // losses: mean over rays of the four smooth-L1 channels, mean distortion,
     // and the running count of trained samples

@@ -75,13 +75,6 @@ TN_DEVICE void st_nt(uint32_t* p, const uint32_t (&a)[4]) {
 // gradient read only; otherwise all 32 parameters of the line are read and
 // written back whole (skipped entries keep their values bit for bit), since
 // partially written lines cost more than the bytes they save.
-// Learning rate of the step: lr0 / (1 + decay * step) with the step read on
-// the device when d_step is given (reference schedule, nerf-network.cu:256),
-// else lr0 as passed. Contraction off: the same float expression as the host's.
-TN_DEVICE float step_lr(float lr0, const uint32_t* d_step, float decay) {
-#pragma clang fp contract(off)
-    return d_step ? lr0 / (1.0f + decay * (float)(*d_step)) : lr0;
-}
 
 template <typename TL>
 __global__ void adam_kernel(uint32_t n, uint32_t n_grid, uint32_t n_net_end, float inv_scale0,

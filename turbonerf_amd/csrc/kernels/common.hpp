@@ -233,6 +233,14 @@ TN_DEVICE float adam_bias(uint32_t t, float beta1, float beta2) {
     return sqrtf(1.0f - powf(beta2, (float)t)) / (1.0f - powf(beta1, (float)t));
 }
 
+// Learning rate of the step: lr0 / (1 + decay * step) with the step read on
+// the device when d_step is given (reference schedule, nerf-network.cu:256),
+// else lr0 as passed. Contraction off: the same float expression as the host's.
+TN_DEVICE float step_lr(float lr0, const uint32_t* d_step, float decay) {
+#pragma clang fp contract(off)
+    return d_step ? lr0 / (1.0f + decay * (float)(*d_step)) : lr0;
+}
+
 // moments and weight after gradient g with bias factor c (no contraction, so
 // every kernel that inlines it rounds identically)
 TN_DEVICE void adam_apply(float g, float c, float lr, float beta1, float beta2, float eps, float& w, float& m,

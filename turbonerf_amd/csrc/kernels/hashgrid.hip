@@ -785,6 +785,127 @@ TN_DEVICE void hb_finish_entry(float* __restrict__ dtable, uint32_t entry, float
     *d = o;
 }
 
+// NGP Adam on the hash table inside the sum passes (tn_adam_t on the device;
+// w null: the passes add the gradient to dtable instead)
+struct HbAdam {
+    float inv_scale0;  // 1 / loss_scale
+    const uint32_t* info;
+    float lr0;
+    const uint32_t* d_step;
+    float lr_decay, beta1, beta2, eps;
+    float* w;
+    _Float16* w_lp;  // null: no fp16 copy
+    float* m;
+    float* v;
+    uint32_t* steps;
+};
+
+typedef float hb_f2 __attribute__((ext_vector_type(2)));
+typedef uint32_t hb_u2 __attribute__((ext_vector_type(2)));
+
+// Adam on table entry `entry` (its parameters 2 entry and 2 entry + 1) given
+// the raw (loss-scaled) gradient sums (s0, s1), by the rule of tn_adam_step:
+// g = s * inv, and a zero g leaves its parameter, moments and step untouched.
+// Lanes act in groups of 16 consecutive entries (a 128-byte line of each fp32
+// array): a group with any gradient reads and writes all its entries back,
+// the untouched ones unchanged bit for bit, as adam_kernel does with its
+// lines. Every lane of the wave calls it (valid: the lane has an entry).
+// Master weights, moments and steps are streamed non-temporally.
+TN_DEVICE void hb_adam_entry(const HbAdam& a, float inv, float lr, uint32_t entry, float s0, float s1, bool valid) {
+    const float g[2] = {valid ? s0 * inv : 0.0f, valid ? s1 * inv : 0.0f};
+    const uint64_t touched = __ballot(g[0] != 0.0f || g[1] != 0.0f);
+    const bool group = ((touched >> (__lane_id() & 48u)) & 0xFFFFull) != 0ull;
+    if (!valid || !group) return;
+    hb_f2* wp = reinterpret_cast<hb_f2*>(a.w) + entry;
+    hb_f2* mp = reinterpret_cast<hb_f2*>(a.m) + entry;
+    hb_f2* vp = reinterpret_cast<hb_f2*>(a.v) + entry;
+    hb_u2* sp = reinterpret_cast<hb_u2*>(a.steps) + entry;
+    const hb_f2 w2 = __builtin_nontemporal_load(wp), m2 = __builtin_nontemporal_load(mp);
+    const hb_f2 v2 = __builtin_nontemporal_load(vp);
+    const hb_u2 s2 = __builtin_nontemporal_load(sp);
+    float w[2] = {w2.x, w2.y}, m[2] = {m2.x, m2.y}, v[2] = {v2.x, v2.y};
+    uint32_t st[2] = {s2.x, s2.y};
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+        if (g[f] == 0.0f) continue;
+        const uint32_t t = st[f] + 1;
+        adam_apply(g[f], adam_bias(t, a.beta1, a.beta2), lr, a.beta1, a.beta2, a.eps, w[f], m[f], v[f]);
+        st[f] = t;
+    }
+    __builtin_nontemporal_store(hb_f2{w[0], w[1]}, wp);
+    __builtin_nontemporal_store(hb_f2{m[0], m[1]}, mp);
+    __builtin_nontemporal_store(hb_f2{v[0], v[1]}, vp);
+    __builtin_nontemporal_store(hb_u2{st[0], st[1]}, sp);
+    if (a.w_lp) {
+        half2v h;
+        h.x = (_Float16)w[0];
+        h.y = (_Float16)w[1];
+        reinterpret_cast<half2v*>(a.w_lp)[entry] = h;
+    }
+}
+
+typedef float hb_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t hb_u4 __attribute__((ext_vector_type(4)));
+typedef _Float16 hb_h4 __attribute__((ext_vector_type(4)));
+
+// The same rule on P pairs of entries per lane, pair u being entries
+// ent[u], ent[u] + 1 (ent[u] even, so its four parameters are one 16-byte
+// access of each array) with raw sums s[u][0..3]; groups of 8 lanes (16
+// entries, a 128-byte line of each fp32 array) act together. Every pair's
+// loads are issued before any update, so a lane keeps P lines in flight.
+template <int P>
+TN_DEVICE void hb_adam_pairs(const HbAdam& a, float inv, float lr, const uint32_t (&ent)[P], const float (&s)[P][4],
+                             const bool (&valid)[P]) {
+    float g[P][4];
+    bool act[P];
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+        bool any = false;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            g[u][f] = valid[u] ? s[u][f] * inv : 0.0f;
+            any |= g[u][f] != 0.0f;
+        }
+        const uint64_t touched = __ballot(any);
+        act[u] = valid[u] && ((touched >> (__lane_id() & 56u)) & 0xFFull) != 0ull;
+    }
+    hb_f4 w[P], m[P], v[P];
+    hb_u4 st[P];
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+        if (!act[u]) continue;
+        const uint32_t q = ent[u] >> 1;
+        w[u] = __builtin_nontemporal_load(reinterpret_cast<const hb_f4*>(a.w) + q);
+        m[u] = __builtin_nontemporal_load(reinterpret_cast<const hb_f4*>(a.m) + q);
+        v[u] = __builtin_nontemporal_load(reinterpret_cast<const hb_f4*>(a.v) + q);
+        st[u] = __builtin_nontemporal_load(reinterpret_cast<const hb_u4*>(a.steps) + q);
+    }
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+        if (!act[u]) continue;
+        const uint32_t q = ent[u] >> 1;
+        float wv[4] = {w[u].x, w[u].y, w[u].z, w[u].w}, mv[4] = {m[u].x, m[u].y, m[u].z, m[u].w};
+        float vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+        uint32_t sv[4] = {st[u].x, st[u].y, st[u].z, st[u].w};
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            if (g[u][f] == 0.0f) continue;
+            const uint32_t t = sv[f] + 1;
+            adam_apply(g[u][f], adam_bias(t, a.beta1, a.beta2), lr, a.beta1, a.beta2, a.eps, wv[f], mv[f], vv[f]);
+            sv[f] = t;
+        }
+        __builtin_nontemporal_store(hb_f4{wv[0], wv[1], wv[2], wv[3]}, reinterpret_cast<hb_f4*>(a.w) + q);
+        __builtin_nontemporal_store(hb_f4{mv[0], mv[1], mv[2], mv[3]}, reinterpret_cast<hb_f4*>(a.m) + q);
+        __builtin_nontemporal_store(hb_f4{vv[0], vv[1], vv[2], vv[3]}, reinterpret_cast<hb_f4*>(a.v) + q);
+        __builtin_nontemporal_store(hb_u4{sv[0], sv[1], sv[2], sv[3]}, reinterpret_cast<hb_u4*>(a.steps) + q);
+        if (a.w_lp)
+            reinterpret_cast<hb_h4*>(a.w_lp)[q] =
+                hb_h4{(_Float16)wv[0], (_Float16)wv[1], (_Float16)wv[2], (_Float16)wv[3]};
+    }
+}
+
+TN_DEVICE float hb_adam_inv(const HbAdam& a) { return a.info ? a.inv_scale0 / ray_pow2(a.info[0]) : a.inv_scale0; }
+
 // bin k's level, first entry and entry count
 struct HbBin {
     uint32_t l, local0, ne;
@@ -818,7 +939,7 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgri
                                                                       const HbRecord* __restrict__ rec,
                                                                       unsigned long long* __restrict__ slots,
                                                                       float* __restrict__ dtable, uint32_t k_lo,
-                                                                      uint32_t k_hi) {
+                                                                      uint32_t k_hi, HbAdam adam) {
     __shared__ unsigned long long acc[2 * kHbBinEntries];
     __shared__ uint32_t smax;
     // this launch covers the chunks of bins [k_lo, k_hi)
@@ -842,17 +963,18 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgri
     const bool finite = bits < 0x7F800000u;
     const int S = hb_scale(bits, b1 > b0 ? b1 - b0 : 0);
     float* accf = reinterpret_cast<float*>(acc);
-    for (uint64_t r = r0 + threadIdx.x; r < r1; r += 4 * blockDim.x) {
-        HbRecord q[4];
-        bool in[4];
+    constexpr int kRecUnroll = 8;  // records in flight per lane
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += kRecUnroll * blockDim.x) {
+        HbRecord q[kRecUnroll];
+        bool in[kRecUnroll];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kRecUnroll; ++u) {
             const uint64_t rr = r + (uint64_t)u * blockDim.x;
             in[u] = rr < r1;
             q[u] = in[u] ? rec[rr] : HbRecord{0u, 0u};
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kRecUnroll; ++u) {
             const uint32_t e = hb_entry(q[u]);
             if (!in[u] || e >= ne) continue;
             if (finite) {
@@ -874,23 +996,54 @@ __global__ void __launch_bounds__(512) hashgrid_bwd_accumulate_kernel(tn_hashgri
         for (uint32_t e = threadIdx.x; e < 2 * ne; e += blockDim.x) dst[e] = acc[e];
         return;
     }
-    for (uint32_t e = threadIdx.x; e < ne; e += blockDim.x) {
-        float v0, v1;
-        if (finite) {
-            const long long q0 = (long long)acc[2 * e], q1 = (long long)acc[2 * e + 1];
-            if (q0 == 0 && q1 == 0) continue;
-            v0 = (float)ldexp((double)q0, -S);
-            v1 = (float)ldexp((double)q1, -S);
-        } else {
-            v0 = accf[2 * e];
-            v1 = accf[2 * e + 1];
-            if (v0 == 0.0f && v1 == 0.0f) continue;
+    // with Adam, a bin summed by one block takes its step here; a split bin
+    // still summed in float adds into dtable for (5) to take
+    const bool adam_here = adam.w && nparts == 1;
+    const float inv = adam_here ? hb_adam_inv(adam) : 0.0f;
+    const float lr = adam_here ? step_lr(adam.lr0, adam.d_step, adam.lr_decay) : 0.0f;
+    // lanes take pairs of entries (e, e + 1), two pairs a round: bins and
+    // levels start at multiples of 8 entries and ne is one, so pairs never
+    // straddle the bin's end and entry0 + e is even
+    constexpr int kPairs = 2;
+    for (uint32_t e0 = 0; e0 < ne; e0 += 2 * kPairs * blockDim.x) {  // the same trip count in every lane
+        uint32_t ent[kPairs];
+        float sum[kPairs][4];
+        bool valid[kPairs];
+#pragma unroll
+        for (int u = 0; u < kPairs; ++u) {
+            const uint32_t e = e0 + 2 * (u * blockDim.x + threadIdx.x);
+            valid[u] = e < ne;
+            ent[u] = entry0 + e;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                float x = 0.0f;
+                if (valid[u] && finite) {
+                    const long long q = (long long)acc[2 * e + f];
+                    if (q != 0) x = (float)ldexp((double)q, -S);
+                } else if (valid[u]) {
+                    x = accf[2 * e + f];
+                }
+                sum[u][f] = x;
+            }
         }
-        if (nparts == 1) {
-            hb_finish_entry(dtable, entry0 + e, v0, v1);
-        } else {  // non-finite split bin: float partials
-            atomicAdd(&dtable[2 * (entry0 + e)], v0);
-            atomicAdd(&dtable[2 * (entry0 + e) + 1], v1);
+        if (adam_here) {
+            hb_adam_pairs<kPairs>(adam, inv, lr, ent, sum, valid);
+            continue;
+        }
+#pragma unroll
+        for (int u = 0; u < kPairs; ++u) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float v0 = sum[u][2 * h], v1 = sum[u][2 * h + 1];
+                if (!valid[u] || (v0 == 0.0f && v1 == 0.0f)) continue;
+                const uint32_t entry = ent[u] + h;
+                if (nparts == 1) {
+                    hb_finish_entry(dtable, entry, v0, v1);
+                } else {  // a split bin summed in float: partials added atomically
+                    atomicAdd(&dtable[2 * entry], v0);
+                    atomicAdd(&dtable[2 * entry + 1], v1);
+                }
+            }
         }
     }
 }
@@ -904,42 +1057,59 @@ constexpr uint32_t kHbFinishBlocks = 256;
 __global__ void __launch_bounds__(kHbFinishThreads) hashgrid_bwd_finish_kernel(
     tn_hashgrid_t g, HbPlan p, const uint32_t* __restrict__ start, const uint32_t* __restrict__ chunk_start,
     const uint32_t* __restrict__ binmax, const uint32_t* __restrict__ slot_base, const uint32_t* __restrict__ split_list,
-    const unsigned long long* __restrict__ slots, float* __restrict__ dtable) {
+    const unsigned long long* __restrict__ slots, float* __restrict__ dtable, HbAdam adam) {
     __shared__ uint32_t smax;
     const uint32_t items = split_list[p.n_bins] * kHbFinishItems;
+    const float inv = adam.w ? hb_adam_inv(adam) : 0.0f;
+    const float lr = adam.w ? step_lr(adam.lr0, adam.d_step, adam.lr_decay) : 0.0f;
     for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
         const uint32_t k = split_list[w / kHbFinishItems];
         const uint32_t bits = hb_bin_max(p, binmax, k, &smax);
         __syncthreads();  // smax is rewritten by the next item
-        if (bits >= 0x7F800000u) continue;  // non-finite: added atomically in (4)
         const uint32_t nparts = chunk_start[k + 1] - chunk_start[k], s0 = slot_base[k];
-        if ((uint64_t)s0 + nparts > p.split_slots) continue;  // beyond the slots: (4) added it atomically
+        // non-finite, or beyond the slots: (4) added the bin into dtable
+        // atomically; with Adam its step is taken from there
+        const bool in_slots = bits < 0x7F800000u && (uint64_t)s0 + nparts <= p.split_slots;
+        if (!in_slots && !adam.w) continue;
         const HbBin bin = hb_bin(g, p, k);
         const uint32_t e = (w % kHbFinishItems) * kHbFinishThreads + threadIdx.x;
-        if (e >= bin.ne) continue;
-        const uint64_t b0 = start[k], b1 = min((uint64_t)start[k + 1], p.cap);
-        const int S = hb_scale(bits, b1 > b0 ? b1 - b0 : 0);
-        const ulonglong2* src = reinterpret_cast<const ulonglong2*>(slots) + (uint64_t)s0 * kHbBinEntries + e;
-        long long q0 = 0, q1 = 0;
-        uint32_t j = 0;
-        for (; j + 4 <= nparts; j += 4) {  // four loads in flight, added in slot order
-            ulonglong2 t[4];
+        const bool valid = e < bin.ne;
+        const uint32_t entry = g.level_offset[bin.l] + bin.local0 + e;
+        float v0 = 0.0f, v1 = 0.0f;
+        if (valid && in_slots) {
+            const uint64_t b0 = start[k], b1 = min((uint64_t)start[k + 1], p.cap);
+            const int S = hb_scale(bits, b1 > b0 ? b1 - b0 : 0);
+            const ulonglong2* src = reinterpret_cast<const ulonglong2*>(slots) + (uint64_t)s0 * kHbBinEntries + e;
+            long long q0 = 0, q1 = 0;
+            uint32_t j = 0;
+            for (; j + 4 <= nparts; j += 4) {  // four loads in flight, added in slot order
+                ulonglong2 t[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) t[u] = src[(uint64_t)(j + u) * kHbBinEntries];
+                for (int u = 0; u < 4; ++u) t[u] = src[(uint64_t)(j + u) * kHbBinEntries];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                q0 += (long long)t[u].x;
-                q1 += (long long)t[u].y;
+                for (int u = 0; u < 4; ++u) {
+                    q0 += (long long)t[u].x;
+                    q1 += (long long)t[u].y;
+                }
             }
+            for (; j < nparts; ++j) {
+                const ulonglong2 t = src[(uint64_t)j * kHbBinEntries];
+                q0 += (long long)t.x;
+                q1 += (long long)t.y;
+            }
+            if (q0 != 0) v0 = (float)ldexp((double)q0, -S);
+            if (q1 != 0) v1 = (float)ldexp((double)q1, -S);
+        } else if (valid) {  // with Adam: the bin's float sums, taken out of dtable
+            float2* d = reinterpret_cast<float2*>(dtable) + entry;
+            const float2 o = *d;
+            v0 = o.x;
+            v1 = o.y;
+            if (v0 != 0.0f || v1 != 0.0f) *d = make_float2(0.0f, 0.0f);
         }
-        for (; j < nparts; ++j) {
-            const ulonglong2 t = src[(uint64_t)j * kHbBinEntries];
-            q0 += (long long)t.x;
-            q1 += (long long)t.y;
-        }
-        if (q0 == 0 && q1 == 0) continue;
-        hb_finish_entry(dtable, g.level_offset[bin.l] + bin.local0 + e, (float)ldexp((double)q0, -S),
-                        (float)ldexp((double)q1, -S));
+        if (adam.w)
+            hb_adam_entry(adam, inv, lr, entry, v0, v1, valid);
+        else if (valid && (v0 != 0.0f || v1 != 0.0f))
+            hb_finish_entry(dtable, entry, v0, v1);
     }
 }
 
@@ -1051,28 +1221,31 @@ void hb_scatter(uint32_t n_max, const uint32_t* info, uint32_t B, uint32_t GB, c
 
 // (4) for levels [l0, l0 + Lg), every record of them scattered
 inline void hb_accumulate(uint32_t n_max, const tn_hashgrid_t& g, const HbPlan& p, float* dtable,
-                          const HbPlanArea& pa, const HbWorkArea& wa, uint32_t l0, uint32_t Lg, hipStream_t stream) {
+                          const HbPlanArea& pa, const HbWorkArea& wa, uint32_t l0, uint32_t Lg, const HbAdam& adam,
+                          hipStream_t stream) {
     const uint32_t L = (uint32_t)g.n_levels;
     const uint32_t k_lo = p.base[l0], k_hi = l0 + Lg < L ? p.base[l0 + Lg] : p.n_bins;
     // chunks of the group's bins: at most one per bin plus one per kHbChunk records
     const uint64_t cap_g = (uint64_t)n_max * Lg * 8;
     const uint32_t acc_blocks = (uint32_t)((k_hi - k_lo) + (cap_g + kHbChunk - 1) / kHbChunk);
     hashgrid_bwd_accumulate_kernel<<<acc_blocks, 512, 0, stream>>>(g, p, pa.start, pa.chunk_start, wa.binmax,
-                                                                   pa.slot_base, wa.rec, wa.slots, dtable, k_lo, k_hi);
+                                                                   pa.slot_base, wa.rec, wa.slots, dtable, k_lo, k_hi,
+                                                                   adam);
 }
 
 // (5)
 inline void hb_finish(const tn_hashgrid_t& g, const HbPlan& p, float* dtable, const HbPlanArea& pa,
-                      const HbWorkArea& wa, hipStream_t stream) {
+                      const HbWorkArea& wa, const HbAdam& adam, hipStream_t stream) {
     hashgrid_bwd_finish_kernel<<<kHbFinishBlocks, kHbFinishThreads, 0, stream>>>(
-        g, p, pa.start, pa.chunk_start, wa.binmax, pa.slot_base, pa.split_list, wa.slots, dtable);
+        g, p, pa.start, pa.chunk_start, wa.binmax, pa.slot_base, pa.split_list, wa.slots, dtable, adam);
 }
 
 // (3)-(5): records, sums and the table update, from the plan and the gradients
 template <typename TG>
 void hashgrid_backward_planned(uint32_t n_max, const uint32_t* info, uint32_t B, uint32_t GB, const tn_hashgrid_t& g,
                                const HbPlan& p, const float* pos, const TG* dfeat, float* dtable,
-                               const HbPlanArea& pa, const HbWorkArea& wa, hipStream_t stream) {
+                               const HbPlanArea& pa, const HbWorkArea& wa, hipStream_t stream,
+                               const HbAdam& adam = HbAdam{}) {
     // scatter and accumulate by groups of levels, so a group's records are
     // read back while they are still in the device's last-level cache
     const uint32_t L = (uint32_t)g.n_levels;
@@ -1081,9 +1254,9 @@ void hashgrid_backward_planned(uint32_t n_max, const uint32_t* info, uint32_t B,
     for (uint32_t l0 = 0; l0 < L; l0 += per) {
         const uint32_t Lg = std::min(per, L - l0);
         hb_scatter<TG>(n_max, info, B, GB, g, p, pos, dfeat, pa, wa, l0, Lg, 0, p.level_blocks, stream);
-        hb_accumulate(n_max, g, p, dtable, pa, wa, l0, Lg, stream);
+        hb_accumulate(n_max, g, p, dtable, pa, wa, l0, Lg, adam, stream);
     }
-    hb_finish(g, p, dtable, pa, wa, stream);
+    hb_finish(g, p, dtable, pa, wa, adam, stream);
 }
 
 inline bool hb_layout_ok(const tn_hashgrid_t& g) {
@@ -1271,10 +1444,24 @@ extern "C" int tn_hashgrid_backward_plan(uint32_t n_max, const uint32_t* info, u
     return TN_OK;
 }
 
-extern "C" int tn_hashgrid_backward_planned(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
-                                            uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos,
-                                            const void* dfeat, float* dtable, const void* plan, size_t plan_bytes,
-                                            void* work, size_t work_bytes, hipStream_t stream) {
+// tn_adam_t on the device; false when its pointers are missing or misaligned
+// (16-byte accesses of four parameters, 8 bytes of the fp16 copy)
+static bool hb_adam_from(const tn_adam_t* a, HbAdam& h) {
+    auto al = [](const void* q, uintptr_t n) { return q && (reinterpret_cast<uintptr_t>(q) & (n - 1)) == 0; };
+    if (!a || !(a->loss_scale > 0.0f) || (a->lp_dtype != 0 && a->lp_dtype != 1)) return false;
+    if (!al(a->weights, 16) || !al(a->first_moments, 16) || !al(a->second_moments, 16) || !al(a->param_steps, 16) ||
+        (a->lp_dtype == 1 && !al(a->weights_lp, 8)))
+        return false;
+    h = HbAdam{1.0f / a->loss_scale, a->scale_by_rays_info, a->learning_rate, a->d_step, a->lr_decay, a->beta1,
+               a->beta2, a->epsilon, a->weights,
+               a->lp_dtype == 1 ? static_cast<_Float16*>(a->weights_lp) : nullptr, a->first_moments,
+               a->second_moments, a->param_steps};
+    return true;
+}
+
+static int hb_planned(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride, uint32_t dfeat_stride,
+                      tn_hashgrid_t grid, const float* pos, const void* dfeat, float* dtable, const void* plan,
+                      size_t plan_bytes, void* work, size_t work_bytes, const HbAdam& adam, hipStream_t stream) {
     TN_CHECK_ARG(pos && dfeat && dtable && plan && work && n_max <= stride && n_max <= dfeat_stride);
     TN_CHECK_ARG(dfeat_dtype == 0 || dfeat_dtype == 1);
     HbPlan p;
@@ -1289,12 +1476,36 @@ extern "C" int tn_hashgrid_backward_planned(int dfeat_dtype, uint32_t n_max, con
     hb_work_area(p, work, &wa);
     if (dfeat_dtype == 0)
         hashgrid_backward_planned<float>(n_max, info, stride, dfeat_stride, grid, p, pos,
-                                         static_cast<const float*>(dfeat), dtable, pa, wa, stream);
+                                         static_cast<const float*>(dfeat), dtable, pa, wa, stream, adam);
     else
         hashgrid_backward_planned<_Float16>(n_max, info, stride, dfeat_stride, grid, p, pos,
-                                            static_cast<const _Float16*>(dfeat), dtable, pa, wa, stream);
+                                            static_cast<const _Float16*>(dfeat), dtable, pa, wa, stream, adam);
     TN_CHECK_LAUNCH();
     return TN_OK;
+}
+
+extern "C" int tn_hashgrid_backward_planned(int dfeat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
+                                            uint32_t dfeat_stride, tn_hashgrid_t grid, const float* pos,
+                                            const void* dfeat, float* dtable, const void* plan, size_t plan_bytes,
+                                            void* work, size_t work_bytes, hipStream_t stream) {
+    return hb_planned(dfeat_dtype, n_max, info, stride, dfeat_stride, grid, pos, dfeat, dtable, plan, plan_bytes,
+                      work, work_bytes, HbAdam{}, stream);
+}
+
+// tn_hashgrid_backward_planned with NGP Adam applied to the hash table where
+// each entry's gradient is formed, in place of tn_adam_step over the table's
+// range (same parameters, moments and steps bit for bit); dtable is a
+// zero-filled gradient scratch that only non-finite or oversized split bins
+// pass through, left zero again.
+extern "C" int tn_hashgrid_backward_planned_adam(int dfeat_dtype, uint32_t n_max, const uint32_t* info,
+                                                 uint32_t stride, uint32_t dfeat_stride, tn_hashgrid_t grid,
+                                                 const float* pos, const void* dfeat, float* dtable, const void* plan,
+                                                 size_t plan_bytes, void* work, size_t work_bytes,
+                                                 const tn_adam_t* adam, hipStream_t stream) {
+    HbAdam a;
+    TN_CHECK_ARG(hb_adam_from(adam, a));
+    return hb_planned(dfeat_dtype, n_max, info, stride, dfeat_stride, grid, pos, dfeat, dtable, plan, plan_bytes,
+                      work, work_bytes, a, stream);
 }
 
 // The gradient passes of tn_hashgrid_backward_planned in parts, for a caller
@@ -1343,9 +1554,8 @@ extern "C" int tn_hashgrid_backward_scatter(int dfeat_dtype, uint32_t n_max, con
     return TN_OK;
 }
 
-extern "C" int tn_hashgrid_backward_accumulate(uint32_t n_max, tn_hashgrid_t grid, float* dtable, const void* plan,
-                                               size_t plan_bytes, void* work, size_t work_bytes,
-                                               hipStream_t stream) {
+static int hb_accumulate_all(uint32_t n_max, tn_hashgrid_t grid, float* dtable, const void* plan, size_t plan_bytes,
+                             void* work, size_t work_bytes, const HbAdam& adam, hipStream_t stream) {
     TN_CHECK_ARG(dtable && plan && work);
     HbPlan p;
     TN_CHECK_ARG(n_max == 0 || hb_split_plan(grid, n_max, p));
@@ -1357,8 +1567,24 @@ extern "C" int tn_hashgrid_backward_accumulate(uint32_t n_max, tn_hashgrid_t gri
     HbWorkArea wa;
     hb_plan_area(p, const_cast<void*>(plan), &pa);
     hb_work_area(p, work, &wa);
-    hb_accumulate(n_max, grid, p, dtable, pa, wa, 0, (uint32_t)grid.n_levels, stream);
-    hb_finish(grid, p, dtable, pa, wa, stream);
+    hb_accumulate(n_max, grid, p, dtable, pa, wa, 0, (uint32_t)grid.n_levels, adam, stream);
+    hb_finish(grid, p, dtable, pa, wa, adam, stream);
     TN_CHECK_LAUNCH();
     return TN_OK;
+}
+
+extern "C" int tn_hashgrid_backward_accumulate(uint32_t n_max, tn_hashgrid_t grid, float* dtable, const void* plan,
+                                               size_t plan_bytes, void* work, size_t work_bytes,
+                                               hipStream_t stream) {
+    return hb_accumulate_all(n_max, grid, dtable, plan, plan_bytes, work, work_bytes, HbAdam{}, stream);
+}
+
+// tn_hashgrid_backward_accumulate with Adam on the hash table, as
+// tn_hashgrid_backward_planned_adam
+extern "C" int tn_hashgrid_backward_accumulate_adam(uint32_t n_max, tn_hashgrid_t grid, float* dtable,
+                                                    const void* plan, size_t plan_bytes, void* work,
+                                                    size_t work_bytes, const tn_adam_t* adam, hipStream_t stream) {
+    HbAdam a;
+    TN_CHECK_ARG(hb_adam_from(adam, a));
+    return hb_accumulate_all(n_max, grid, dtable, plan, plan_bytes, work, work_bytes, a, stream);
 }

@@ -12,7 +12,7 @@ namespace tn {
 
 namespace {
 
-std::atomic<int> g_tune[TN_TUNE_COUNT] = {256, 128, 2, 1, 2048, 256, 1, 0, 0};
+std::atomic<int> g_tune[TN_TUNE_COUNT] = {256, 128, 2, 1, 2048, 256, 1, 0, 0, 0};
 std::atomic<uint32_t> g_generation{0};
 
 bool tune_valid(int knob, int v) {
@@ -26,6 +26,7 @@ bool tune_valid(int knob, int v) {
     case TN_TUNE_HASH_BWD_LEVEL_GROUPS: return v >= 1 && v <= 16;
     case TN_TUNE_HASH_FWD_LDS_LEVELS: return v >= 0 && v <= 4;
     case TN_TUNE_STEP_HALVES: return v == 0 || v == 1;
+    case TN_TUNE_HASH_ADAM_FUSED: return v == 0 || v == 1;
     default: return false;
     }
 }
