@@ -790,13 +790,12 @@ def main():
         profile_render(tn, synthetic, scene, args)
         return
     precs = {"mixed": [1], "fp32": [0], "both": [1, 0]}[args.precision]
+    defaults = tn.get_tuning()  # the launch shapes before any tuning
     results = {}
     for p in precs:
         results[p] = run_precision(tn, synthetic, scene, p, args)
     main_p = precs[0]
     main = results[main_p]
-    for k, v in main["autotune"]["picks"].items():  # the later legs train and render the main NeRF
-        tn.set_tuning(k, v)
     dtype = "fp16" if main_p == 1 else "fp32"
     n_params = ops.hashgrid_layout().n_params + 9408
     roof, table = roofline(main, n_params, dtype)
@@ -813,6 +812,11 @@ def main():
         extra["fp32_roofline"] = roofline(f, n_params, "fp32")[0]
         extra["fp32_phases_ms_per_step"] = roofline(f, n_params, "fp32")[1]
         extra["fp32_autotune"] = f["autotune"]
+    # the renders run with the default launch shapes: the training picks are
+    # tuned for 2^18-sample steps (e.g. a smaller marching grid), not for
+    # render passes of up to 2^22 samples over 640K-2M rays
+    for k, v in defaults.items():
+        tn.set_tuning(k, v)
     if not args.no_render:
         mr, ms = render_rate(tn, synthetic, scene, main["trainer"])
         extra["render_mrays_per_s"] = round(mr, 2)
@@ -820,6 +824,8 @@ def main():
         log(f"[bench] render {mr:.1f} Mrays/s ({ms:.2f} ms per {RES}x{RES} frame)")
     if not args.no_render:
         extra["large_scene_aabb16_5cascades_1920x1080"] = large_scene(tn, synthetic)
+    for k, v in main["autotune"]["picks"].items():  # the distortion leg trains the main NeRF
+        tn.set_tuning(k, v)
     if not args.no_extra_configs:
         if main_p == 1:
             extra["distortion_loss_mixed"] = distortion_rate(main["trainer"][2], args.steps)

@@ -109,7 +109,7 @@ def test_distortion_long_rays(cuda):
 def test_distortion_fd(cuda):
     import oracle
 
-    b = make(seed=22, n_rays=16)
+    b = make(seed=22, n_rays=64)
     _, g = run_gpu(b)
     rng = np.random.default_rng(0)
     h = 1e-6

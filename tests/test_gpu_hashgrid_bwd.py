@@ -16,6 +16,10 @@ kernel, on the default 16-level layout (T = 2^19):
 * accumulation into a non-zero table, values spanning 1e-6..1e3 (the
   fixed-point scale follows each bin's largest value), and a non-finite
   gradient (the bin falls back to float sums and the inf reaches the table);
+* FP64 central differences (h = 1e-6) of the oracle's fp64 forward on
+  samples along 64 rays (geometric(48) samples per ray): the binned
+  gradient of the loss sum(dfeat * features) at table entries those samples
+  touch;
 * Adam on the table fused into the sum passes
   (tn_hashgrid_backward_planned_adam): the parameters, moments, steps and
   fp16 copy of the unfused backward followed by tn_adam_step, bit for bit,
@@ -33,6 +37,11 @@ import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+
+def norm_rel(a, b):
+    a, b = np.ravel(np.asarray(a, np.float64)), np.ravel(np.asarray(b, np.float64))
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
 def _rays(rng, n_rays, per, spacing):
@@ -353,3 +362,62 @@ def test_planned_adam_equals_backward_then_adam(cuda, lp, nonfinite):
         view = np.uint16 if a.dtype == np.float16 else np.uint32
         assert np.array_equal(a.view(view), b.view(view)), name
     assert not out[1][4].any() and not out[0][4].any()
+
+
+def test_binned_vs_fp64_central_differences(cuda):
+    """The trainer's backward (plan, then the gradient passes) against FP64
+    central differences, h = 1e-6, of L(table) = sum(dfeat * features) with
+    the oracle's fp64 encoding, at 64 touched entries of every level, on
+    samples along 64 rays with geometric(48) samples each (the compositing
+    and distortion tests' batch shape). Norm-wise 1e-5 per level."""
+    import torch
+
+    import oracle
+    from turbonerf_amd import ops
+
+    rng = np.random.default_rng(64)
+    g = ops.hashgrid_layout()
+    steps = np.clip(rng.geometric(1 / 48, 64), 1, 1024)
+    pts = []
+    for k in steps:
+        o = rng.normal(size=3)
+        o = 0.5 + 1.2 * o / np.linalg.norm(o)
+        d = rng.uniform(0.3, 0.7, size=3) - o
+        ln = np.linalg.norm(d)
+        t = (ln - 0.45) + (np.sqrt(3.0) / 1024) * np.arange(k)
+        pts.append(np.clip(o[None, :] + t[:, None] * (d / ln)[None, :], 0.0, 1.0))
+    pos = np.concatenate(pts).astype(np.float32).T.copy()  # [3][n]
+    n = pos.shape[1]
+    table = rng.uniform(-1e-1, 1e-1, size=g.n_params)
+    df = rng.normal(size=(32, n)).astype(np.float32)
+    pos_d = torch.from_numpy(pos).cuda().view(-1)
+    got = torch.zeros(g.n_params, dtype=torch.float32, device="cuda")
+    ops.hashgrid_backward(g, n, pos_d, n, torch.from_numpy(df).cuda().view(-1), got, planned=True)
+    got = got.cpu().numpy().astype(np.float64)
+    lib = oracle.lib()
+    df64 = df.astype(np.float64).ravel()
+    feat = np.zeros(32 * n)
+
+    def loss(t):
+        lib.oc_hashgrid_fwd_f64(ctypes.addressof(g), n, n, pos.ravel(), t, feat)
+        return float(feat @ df64)
+
+    h = 1e-6
+    worst = 0.0
+    for l in range(g.n_levels):
+        lo, hi = 2 * g.level_offset[l], 2 * (g.level_offset[l] + g.level_size[l])
+        touched = lo + np.flatnonzero(got[lo:hi])
+        pick = rng.choice(touched, min(64, len(touched)), replace=False)
+        fd = []
+        for k in pick:
+            t0 = table[k]
+            table[k] = t0 + h
+            lp = loss(table)
+            table[k] = t0 - h
+            lm = loss(table)
+            table[k] = t0
+            fd.append((lp - lm) / (2 * h))
+        e = norm_rel(got[pick], np.array(fd))
+        worst = max(worst, e)
+        assert e < 1e-5, (l, e)
+    print(f"[hash bwd fd] {n} samples on 64 rays, worst level rel err {worst:.2e}", flush=True)
