@@ -37,6 +37,14 @@ inline unsigned blocks_for(uint64_t n, unsigned threads) {
     return (unsigned)(b == 0 ? 1 : b);
 }
 
+// Grid of a kernel whose waves loop over the device-side ray count (a wave
+// per ray): at most 1024 workgroups, since a training batch uses a small
+// part of its ray capacity and idle workgroups still cost their dispatch
+inline unsigned ray_wave_blocks(uint64_t n_rays_max, unsigned waves_per_block) {
+    const unsigned b = blocks_for(n_rays_max, waves_per_block);
+    return b < 1024u ? b : 1024u;
+}
+
 // kernels using more than the default 64 KiB of dynamic LDS opt in once
 template <typename K>
 inline bool allow_lds(K kernel, size_t bytes) {

@@ -227,13 +227,11 @@ composite_chunk_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t P
 // in order up to the stop by that lane), the totals, loss and dL/dRc, and the
 // prefixes of w g at the chunk starts by a second lane scan
 template <typename TIn, typename R>
-__global__ void __launch_bounds__(64 * kCompositeWaves)
-composite_ray_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB, uint32_t SB, float min_T,
+TN_DEVICE void composite_one_ray(uint32_t r, const uint32_t* __restrict__ info, uint32_t B, uint32_t PB, uint32_t SB, float min_T,
                      const uint32_t* __restrict__ steps, const uint32_t* __restrict__ offs,
                      const float* __restrict__ bg, const float* __restrict__ target, const TIn* __restrict__ density,
                      const TIn* __restrict__ color, const float* __restrict__ dtb, float* __restrict__ loss,
                      R* __restrict__ scratch, const Acc<R>* __restrict__ rec, Acc<R>* __restrict__ carry) {
-    const uint32_t r = blockIdx.x * kCompositeWaves + (threadIdx.x >> 6);
     const uint32_t lane = __lane_id();
     const uint32_t n_rays = info[0];
     if (r >= n_rays) return;
@@ -331,6 +329,20 @@ composite_ray_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB,
         Q0 = (Qin + wg).bcast(63);
         P0 = (Pin + Pk).bcast(63);
     }
+}
+
+// rays r = wave, wave + the grid's waves, ...: the grid is capped (most of a
+// training batch's ray capacity is unused), so idle workgroups are not launched
+template <typename TIn, typename R>
+__global__ void __launch_bounds__(64 * kCompositeWaves)
+composite_ray_kernel(const uint32_t* __restrict__ info, uint32_t B, uint32_t PB, uint32_t SB, float min_T,
+                     const uint32_t* __restrict__ steps, const uint32_t* __restrict__ offs,
+                     const float* __restrict__ bg, const float* __restrict__ target, const TIn* __restrict__ density,
+                     const TIn* __restrict__ color, const float* __restrict__ dtb, float* __restrict__ loss,
+                     R* __restrict__ scratch, const Acc<R>* __restrict__ rec, Acc<R>* __restrict__ carry) {
+    const uint32_t n_rays = info[0];
+    for (uint32_t r = blockIdx.x * kCompositeWaves + (threadIdx.x >> 6); r < n_rays; r += gridDim.x * kCompositeWaves)
+        composite_one_ray<TIn, R>(r, info, B, PB, SB, min_T, steps, offs, bg, target, density, color, dtb, loss, scratch, rec, carry);
 }
 
 // (3) gradients per chunk with the chained carry-ins
@@ -557,7 +569,7 @@ int launch_composite(const uint32_t* info, uint32_t n_rays_max, uint32_t B, uint
     const TIn* c = static_cast<const TIn*>(color);
     composite_chunk_kernel<TIn, R><<<chunk_blocks, 64 * kCompositeWaves, 0, stream>>>(
         info, B, PB, n_rays_max, min_T, steps, offs, bg, target, d, c, dt, loss, rays, rec);
-    composite_ray_kernel<TIn, R><<<blocks_for(n_rays_max, kCompositeWaves), 64 * kCompositeWaves, 0, stream>>>(
+    composite_ray_kernel<TIn, R><<<ray_wave_blocks(n_rays_max, kCompositeWaves), 64 * kCompositeWaves, 0, stream>>>(
         info, B, PB, n_rays_max, min_T, steps, offs, bg, target, d, c, dt, loss, rays, rec, carry);
     composite_grad_kernel<TIn, TGrad, R><<<chunk_blocks, 64 * kCompositeWaves, 0, stream>>>(
         info, B, PB, n_rays_max, grad_scale, scale_by_rays, min_T, steps, offs, bg, d, c, dt, dL_ddensity,

@@ -136,11 +136,9 @@ TN_DEVICE const Acc<R>* dist_rec(const Acc<R>* rec, int per, uint32_t k, uint32_
 
 // (2) a wave per ray crossing chunks: prefixes at the chunk starts, totals
 template <typename R>
-__global__ void __launch_bounds__(64 * kDistWaves)
-dist_prefix_kernel(const uint32_t* __restrict__ info, const uint32_t* __restrict__ steps,
+TN_DEVICE void dist_prefix_one_ray(uint32_t r, const uint32_t* __restrict__ info, const uint32_t* __restrict__ steps,
                    const uint32_t* __restrict__ offs, float* __restrict__ ray_loss, R* __restrict__ rays, uint32_t SB,
                    const Acc<R>* __restrict__ rec, Acc<R>* __restrict__ carry) {
-    const uint32_t r = blockIdx.x * kDistWaves + (threadIdx.x >> 6);
     const uint32_t lane = __lane_id();
     if (r >= info[0]) return;
     const uint32_t o = offs[r], n = steps[r];
@@ -175,6 +173,18 @@ dist_prefix_kernel(const uint32_t* __restrict__ info, const uint32_t* __restrict
         rays[r] = W0.val();
         rays[r + SB] = WM0.val();
     }
+}
+
+// rays r = wave, wave + the grid's waves, ...: the grid is capped (most of a
+// training batch's ray capacity is unused), so idle workgroups are not launched
+template <typename R>
+__global__ void __launch_bounds__(64 * kDistWaves)
+dist_prefix_kernel(const uint32_t* __restrict__ info, const uint32_t* __restrict__ steps,
+                   const uint32_t* __restrict__ offs, float* __restrict__ ray_loss, R* __restrict__ rays, uint32_t SB,
+                   const Acc<R>* __restrict__ rec, Acc<R>* __restrict__ carry) {
+    const uint32_t n_rays = info[0];
+    for (uint32_t r = blockIdx.x * kDistWaves + (threadIdx.x >> 6); r < n_rays; r += gridDim.x * kDistWaves)
+        dist_prefix_one_ray<R>(r, info, steps, offs, ray_loss, rays, SB, rec, carry);
 }
 
 // per-sample terms of pass (3) and (5): state with the carry-ins, the loss
@@ -246,11 +256,9 @@ dist_terms_kernel(const uint32_t* __restrict__ info, float lambda, const uint32_
 
 // (4) a wave per ray crossing chunks: loss, sum w g, its prefix at chunk starts
 template <typename R>
-__global__ void __launch_bounds__(64 * kDistWaves)
-dist_totals_kernel(const uint32_t* __restrict__ info, float lambda, const uint32_t* __restrict__ steps,
+TN_DEVICE void dist_totals_one_ray(uint32_t r, const uint32_t* __restrict__ info, float lambda, const uint32_t* __restrict__ steps,
                    const uint32_t* __restrict__ offs, float* __restrict__ ray_loss, R* __restrict__ rays, uint32_t SB,
                    const Acc<R>* __restrict__ rec2, Acc<R>* __restrict__ carry) {
-    const uint32_t r = blockIdx.x * kDistWaves + (threadIdx.x >> 6);
     const uint32_t lane = __lane_id();
     if (r >= info[0]) return;
     const uint32_t o = offs[r], n = steps[r];
@@ -272,6 +280,18 @@ dist_totals_kernel(const uint32_t* __restrict__ info, float lambda, const uint32
         ray_loss[r] = (float)((R)lambda / (R)n * V0.val());
         rays[r + 2 * SB] = S0.val();
     }
+}
+
+// rays r = wave, wave + the grid's waves, ...: the grid is capped (most of a
+// training batch's ray capacity is unused), so idle workgroups are not launched
+template <typename R>
+__global__ void __launch_bounds__(64 * kDistWaves)
+dist_totals_kernel(const uint32_t* __restrict__ info, float lambda, const uint32_t* __restrict__ steps,
+                   const uint32_t* __restrict__ offs, float* __restrict__ ray_loss, R* __restrict__ rays, uint32_t SB,
+                   const Acc<R>* __restrict__ rec2, Acc<R>* __restrict__ carry) {
+    const uint32_t n_rays = info[0];
+    for (uint32_t r = blockIdx.x * kDistWaves + (threadIdx.x >> 6); r < n_rays; r += gridDim.x * kDistWaves)
+        dist_totals_one_ray<R>(r, info, lambda, steps, offs, ray_loss, rays, SB, rec2, carry);
 }
 
 // (5) per chunk: the gradient
@@ -318,7 +338,7 @@ void launch_distortion(const uint32_t* info, uint32_t n_rays_max, uint32_t sampl
     Acc<R>* carry = reinterpret_cast<Acc<R>*>(base + off);
     const uint32_t SB = n_rays_max;
     const unsigned chunk_blocks = (dist_chunks(sample_stride) + kDistWaves - 1) / kDistWaves;
-    const unsigned ray_blocks = blocks_for(n_rays_max, kDistWaves);
+    const unsigned ray_blocks = ray_wave_blocks(n_rays_max, kDistWaves);
     const unsigned threads = 64 * kDistWaves;
     dist_chunk_kernel<TIn, R><<<chunk_blocks, threads, 0, stream>>>(info, steps, offs, density, mn, dtn, rays, SB,
                                                                      rec1);
