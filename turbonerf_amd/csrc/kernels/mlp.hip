@@ -916,8 +916,18 @@ reduce_partials_kernel(uint32_t n_part, const float* __restrict__ partial, float
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const uint32_t e = blockIdx.x * 64u + lane;
     float s = 0.0f;
-    if (e < (uint32_t)kNetParams)
-        for (uint32_t p = w; p < n_part; p += kReduceWaves) s += partial[(uint64_t)p * kNetParams + e];
+    if (e < (uint32_t)kNetParams) {
+        // eight rows in flight, added in row order
+        uint32_t p = w;
+        for (; p + 7 * kReduceWaves < n_part; p += 8 * kReduceWaves) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = partial[(uint64_t)(p + u * kReduceWaves) * kNetParams + e];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; p < n_part; p += kReduceWaves) s += partial[(uint64_t)p * kNetParams + e];
+    }
     red[w][lane] = s;
     __syncthreads();
     if (w == 0 && e < (uint32_t)kNetParams) {
