@@ -648,7 +648,7 @@ def _short(name):
     keys = {k for ks in KERNEL.values() for k in ks} | {
         "scan_tile_sums_kernel", "scan_tile_prefix_kernel", "scan_write_select_kernel", "uniform_kernel",
         "init_training_rays_kernel", "estimate_rays_kernel", "reduce_partials_kernel",
-        "write_rgba_kernel", "cast_kernel"}
+        "write_rgba_kernel", "cast_kernel", "composite_forward_kernel", "render_rays_kernel"}
     for key in sorted(keys, key=len, reverse=True):
         if key in base:
             return key

@@ -322,6 +322,16 @@ int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* info, uint32_t
     const void* features, const float* dir, int color, void* density_out, void* rgb_out,
     const tn_appearance_t* appearance, tn_stream_t stream);
 
+/* tn_mlp_forward for inference (rendering, the occupancy update):
+ * feat_dtype 1 takes fp16 features [32][feat_stride] (precision 1 only) and
+ * rounds every layer's inputs to fp16 once, as the reference's fp16 network
+ * does (tcnn FullyFusedMLP inference via src/core/nerf-network.cu:431-545);
+ * feat_dtype 0 is tn_mlp_forward. */
+int tn_mlp_inference(int precision, int feat_dtype, uint32_t n_max, const uint32_t* info, uint32_t stride,
+    uint32_t feat_stride, uint32_t out_stride, const void* params, uint32_t net_offset, const void* features,
+    const float* dir, int color, void* density_out, void* rgb_out, const tn_appearance_t* appearance,
+    tn_stream_t stream);
+
 /* Backward of both MLPs with the reference's gradient plumbing (colour-MLP
  * backward, density row 0 += loss_scale*(dL_ds + dL_ds_extra), t^2 scaling for
  * t < 1, density-MLP backward; reference src/core/nerf-network.cu:668-768,

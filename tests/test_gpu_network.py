@@ -164,6 +164,20 @@ def test_mlp_forward(cuda, prec):
     # density-only inference gives the same density
     dens2, _ = ops.mlp_forward(params, n_grid, n, feat_d, n, None, n, color=False)
     assert np.array_equal(dens2.float().cpu().numpy(), dens)
+    if prec == "fp16":
+        # inference on fp16 features (tn_mlp_inference): every layer's inputs
+        # rounded to fp16 once, as the reference's fp16 network computes;
+        # against fp64 on the same fp16 features
+        f16 = feat.astype(np.float16).astype(np.float64)
+        d16, c16 = ops.mlp_forward(params, n_grid, n, feat_d.half(), n, dir_d, n, color=True)
+        fw16 = np_forward(net, f16, dir01)
+        d16 = d16.float().cpu().numpy()
+        c16 = c16.float().cpu().numpy().reshape(3, n).T
+        e3, e4 = norm_rel(d16, fw16["s16"][:, 0]), norm_rel(c16, fw16["rgb"])
+        print(f"[mlp inference fp16 features] density {e3:.2e} rgb {e4:.2e}", flush=True)
+        assert e3 < 1e-2 and e4 < 1e-2
+        d16b, _ = ops.mlp_forward(params, n_grid, n, feat_d.half(), n, None, n, color=False)
+        assert np.array_equal(d16b.float().cpu().numpy(), d16)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp16"])

@@ -184,6 +184,8 @@ _SIGS: dict[str, tuple] = {
                                                     C.c_size_t, vp, C.c_size_t, vp, vp]),
     "tn_hashgrid_backward_accumulate_adam": (C.c_int, [u32, HashGrid, vp, vp, C.c_size_t, vp, C.c_size_t, vp, vp]),
     "tn_mlp_forward": (C.c_int, [C.c_int, u32, vp, u32, u32, u32, vp, u32, vp, vp, C.c_int, vp, vp, vp, vp]),
+    "tn_mlp_inference": (C.c_int, [C.c_int, C.c_int, u32, vp, u32, u32, u32, vp, u32, vp, vp, C.c_int, vp, vp, vp,
+                                   vp]),
     "tn_mlp_backward_workspace_bytes": (sz, [u32]),
     "tn_mlp_backward": (C.c_int, [C.c_int, u32, vp, u32, u32, vp, u32, vp, vp, vp, vp, u32, vp, vp, f32, C.c_int, vp,
                                   vp, C.c_int, vp, sz, vp, vp]),

@@ -306,6 +306,9 @@ PYBIND11_MODULE(PyTurboNeRF, m) {
         .def_readwrite("beta2", &NetworkConfig::beta2)
         .def_readwrite("epsilon", &NetworkConfig::epsilon)
         .def_readwrite("l2_reg", &NetworkConfig::l2_reg)
+        .def_readwrite("inference_fp16_features", &NetworkConfig::inference_fp16_features,
+                       "mixed precision: render and update the occupancy grid on fp16 features, inputs rounded "
+                       "to fp16 once (the reference's fp16 network); False: the training forward's split inputs")
         .def_property(
             "appearance_embedding_dims", [](const NetworkConfig& c) { return c.appearance_dims; },
             [](NetworkConfig& c, int v) {

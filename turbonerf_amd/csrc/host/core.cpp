@@ -364,15 +364,16 @@ void NerfNetwork::inference(hipStream_t s, const NetworkParamsWorkspace& p, uint
                             void* density_out, void* rgb_out, bool use_color_network) {
     const int prec = config.precision;
     const void* params = params_for_kernels(p);
-    TN_TRY(tn_hashgrid_forward(prec, 0, n, info, stride, stride, grid, pos_batch, params, features, as_tn(s)));
+    const int fdt = prec == 1 && config.inference_fp16_features ? 1 : 0;  // feature dtype
+    TN_TRY(tn_hashgrid_forward(prec, fdt, n, info, stride, stride, grid, pos_batch, params, features, as_tn(s)));
     tn_appearance_t app{};
     if (use_color_network && p.n_appearances) {
         TN_TRY(tn_appearance_bias(1, p.params_fp.data() + p.w3e_offset(), p.params_fp.data() + p.emb_offset(),
                                   app_bias.data(), as_tn(s)));
         app = tn_appearance_t{app_bias.data(), nullptr, 1, nullptr};
     }
-    TN_TRY(tn_mlp_forward(prec, n, info, stride, stride, stride, params, p.n_grid, features, dir_batch,
-                          use_color_network ? 1 : 0, density_out, rgb_out, app.bias ? &app : nullptr, as_tn(s)));
+    TN_TRY(tn_mlp_inference(prec, fdt, n, info, stride, stride, stride, params, p.n_grid, features, dir_batch,
+                            use_color_network ? 1 : 0, density_out, rgb_out, app.bias ? &app : nullptr, as_tn(s)));
 }
 
 // ------------------------------------------------------------------ workspaces

@@ -114,6 +114,11 @@ struct NetworkConfig {
     // TN_APPEARANCE_DIMS per training image, as the reference always trains
     // (src/core/nerf-network.cu:27-72, nerf-constants.cuh:31)
     int appearance_dims = 0;
+    // mixed precision: inference (rendering, the occupancy update) encodes to
+    // fp16 features and runs the network on inputs rounded to fp16 once, as
+    // the reference's fp16 network does; false: the training forward's fp32
+    // features and split fp16 first-layer inputs
+    bool inference_fp16_features = true;
     float loss_scale() const { return precision == 1 ? 1024.0f : 1.0f; }
 };
 

@@ -367,31 +367,52 @@ TN_DEVICE void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t uni_
 
 // Inputs of one tile. Lanes past the batch load sample 0 and zero what they
 // loaded (no divergent branches around the loads); element offsets are
-// 32-bit (the host checks that 32 rows of fp32 features span under 2^32 bytes).
-template <class P, typename TF>
+// 32-bit (the host checks that 32 rows of features span under 2^32 bytes).
+// kSplit: the first layers' inputs as fp16 hi + lo pairs (mixed-precision
+// training); otherwise rounded once. fp16 features (TF = _Float16, for
+// inference) are the operands as loaded.
+template <class P, typename TF, bool kSplit = (P::SL == 8)>
 TN_DEVICE void load_inputs(Tile<P>& tl, TileLo<P>& lo, bool valid, uint32_t sample, int h,
                            const TF* __restrict__ feat, uint32_t FS, const float* __restrict__ dir, uint32_t B,
                            bool need_sh) {
     const uint32_t sl = valid ? sample : 0u;
-    static_assert(sizeof(TF) == 4, "features are fp32");
+    static_assert(sizeof(TF) == 4 || (sizeof(TF) == 2 && P::SL == 8 && !kSplit),
+                  "features are fp32, or fp16 unsplit for the fp16 network");
     // row nat_in(ks, h, j) = (ks*KC + j) + h*SL: the first part is uniform
-    const __amdgpu_buffer_rsrc_t fr = buf_rsrc(feat, 32u * FS * 4u);
-    const uint32_t lane_off = ((uint32_t)(h * P::SL) * FS + sl) * 4u;
+    constexpr uint32_t kFB = sizeof(TF);
+    const __amdgpu_buffer_rsrc_t fr = buf_rsrc(feat, 32u * FS * kFB);
+    const uint32_t lane_off = ((uint32_t)(h * P::SL) * FS + sl) * kFB;
 #pragma unroll
     for (int ks = 0; ks < 32 / P::KC; ++ks) {
-        float v[P::SL];
+        if constexpr (sizeof(TF) == 2) {
+            typename P::Op o;
 #pragma unroll
-        for (int j = 0; j < P::SL; ++j) {
-            const float x = buf_ld(fr, lane_off, (uint32_t)(ks * P::KC + j) * FS * 4u);
-            v[j] = valid ? x : 0.0f;
+            for (int j = 0; j < P::SL; ++j) {
+                const uint16_t x = __builtin_amdgcn_raw_buffer_load_b16(fr, (int)lane_off,
+                                                                        (int)((uint32_t)(ks * P::KC + j) * FS * 2u), 0);
+                o[j] = valid ? __builtin_bit_cast(_Float16, x) : (_Float16)0.0f;
+            }
+            tl.x[ks] = o;
+        } else {
+            float v[P::SL];
+#pragma unroll
+            for (int j = 0; j < P::SL; ++j) {
+                const float x = buf_ld(fr, lane_off, (uint32_t)(ks * P::KC + j) * FS * 4u);
+                v[j] = valid ? x : 0.0f;
+            }
+            if constexpr (kSplit) P::split(v, tl.x[ks], lo.x[ks]);
+            else tl.x[ks] = P::from(v);
         }
-        if constexpr (P::SL == 8) P::split(v, tl.x[ks], lo.x[ks]);
-        else tl.x[ks] = P::from(v);
     }
     if (need_sh) {
         float shv[16];
         sh4(dir[sl], dir[sl + B], dir[sl + 2 * B], shv);
-        if constexpr (P::SL == 8) {
+        if constexpr (P::SL == 8 && !kSplit) {
+            // the lane half's eight coefficients, selected as values
+            const typename P::Op o0 = P::from(shv), o1 = P::from(shv + 8);
+            const typename P::Op z = {};
+            tl.sh[0] = valid ? (h ? o1 : o0) : z;
+        } else if constexpr (P::SL == 8) {
             // both halves' operands are formed and the lane half's pair is
             // selected as values: an array index that depends on h becomes
             // a compare-and-select chain over all 16 coefficients
@@ -435,11 +456,10 @@ TN_DEVICE f16v load_rows(const float* __restrict__ v, int t, int h) {
 // Forward of one tile. app_bias (may be null): this lane's sample's
 // appearance row, W3[:, 32:48] times its image's embedding, which starts the
 // colour MLP's first-layer accumulator.
-template <class P>
+template <class P, bool kSplit = (P::SL == 8)>
 TN_DEVICE void forward_tile(Tile<P>& tl, const TileLo<P>& lo, const FwdLds<P>& W, int lane, bool color,
                             const float* __restrict__ app_bias = nullptr) {
     using N = Net<P>;
-    constexpr bool kSplit = P::SL == 8;
     OpTile<P> h1lo[2];
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
@@ -488,7 +508,7 @@ TN_DEVICE float sigmoidf(float x) { return 1.0f / (1.0f + __expf(-x)); }
 
 constexpr int kMlpWaves = 4;
 
-template <class P, typename TF>
+template <class P, typename TF, bool kSplit = (P::SL == 8)>
 __global__ void __launch_bounds__(64 * kMlpWaves)
 mlp_forward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t B, uint32_t FS,
                    const typename P::T* __restrict__ params, const TF* __restrict__ feat,
@@ -509,8 +529,8 @@ mlp_forward_kernel(uint32_t n_max, const uint32_t* __restrict__ info, uint32_t B
         const bool valid = sample < n;
         Tile<P> tl;
         TileLo<P> lo;
-        load_inputs<P>(tl, lo, valid, sample, h, feat, FS, dir, B, color != 0);
-        forward_tile<P>(tl, lo, W, lane, color != 0, app_row(app, valid, sample));
+        load_inputs<P, TF, kSplit>(tl, lo, valid, sample, h, feat, FS, dir, B, color != 0);
+        forward_tile<P, kSplit>(tl, lo, W, lane, color != 0, app_row(app, valid, sample));
         if (valid && h == 0) {
             density_out[sample] = (typename P::T)tl.s0;
             if (color) {
@@ -1004,6 +1024,35 @@ extern "C" int tn_mlp_forward(int precision, uint32_t n_max, const uint32_t* inf
     } else {
         return TN_ERR_INVALID_ARGUMENT;
     }
+    TN_CHECK_LAUNCH();
+    return TN_OK;
+}
+
+// tn_mlp_forward for inference: feat_dtype 1 takes fp16 features (precision
+// 1 only) as the first layer's operands, and every layer's inputs are
+// rounded to fp16 once, as the reference's fp16 network computes; feat_dtype
+// 0 is tn_mlp_forward.
+extern "C" int tn_mlp_inference(int precision, int feat_dtype, uint32_t n_max, const uint32_t* info,
+                                uint32_t stride, uint32_t feat_stride, uint32_t out_stride, const void* params,
+                                uint32_t net_offset, const void* features, const float* dir, int color,
+                                void* density_out, void* rgb_out, const tn_appearance_t* appearance,
+                                hipStream_t stream) {
+    if (feat_dtype == 0)
+        return tn_mlp_forward(precision, n_max, info, stride, feat_stride, out_stride, params, net_offset, features,
+                              dir, color, density_out, rgb_out, appearance, stream);
+    TN_CHECK_ARG(feat_dtype == 1 && precision == 1);
+    TN_CHECK_ARG(params && features && density_out && (!color || (dir && rgb_out)));
+    const tn_appearance_t app = appearance && color ? *appearance : tn_appearance_t{};
+    TN_CHECK_ARG(!app.bias || (app.n_appearances > 0 && (reinterpret_cast<uintptr_t>(app.bias) & 15) == 0));
+    TN_CHECK_ARG(n_max <= stride && n_max <= feat_stride && n_max <= out_stride);
+    TN_CHECK_ARG(64ull * feat_stride < (1ull << 32) && 3ull * stride < (1ull << 32) && 3ull * out_stride < (1ull << 32));
+    if (n_max == 0) return TN_OK;
+    const unsigned blocks = mlp_blocks(n_max, (unsigned)tune_value(TN_TUNE_MLP_FWD_BLOCKS_PER_CU));
+    using P = PrecF16;
+    const auto* w = static_cast<const P::T*>(params) + net_offset;
+    mlp_forward_kernel<P, _Float16, false><<<blocks, 64 * kMlpWaves, fwd_lds_bytes<P>(), stream>>>(
+        n_max, info, stride, feat_stride, w, static_cast<const _Float16*>(features), dir, color,
+        static_cast<P::T*>(density_out), static_cast<P::T*>(rgb_out), out_stride, app);
     TN_CHECK_LAUNCH();
     return TN_OK;
 }

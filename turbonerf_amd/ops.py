@@ -287,13 +287,17 @@ def _app_ref(app):
 def mlp_forward(params: torch.Tensor, net_offset: int, n: int, feat: torch.Tensor, feat_stride: int,
                 dir: torch.Tensor | None, stride: int, color: bool = True, info: torch.Tensor | None = None,
                 app=None):
+    """Density and colour; fp16 features (an fp16 network only) run the
+    inference form, every layer's inputs rounded to fp16 once
+    (tn_mlp_inference)."""
     prec = _PREC[params.dtype]
-    if feat.dtype != torch.float32 or feat.numel() < 32 * feat_stride:
-        raise ValueError("the MLP kernels take fp32 features [32][feat_stride]")
+    f16 = feat.dtype == torch.float16
+    if (feat.dtype != torch.float32 and not (f16 and prec == 1)) or feat.numel() < 32 * feat_stride:
+        raise ValueError("the MLP kernels take fp32 features [32][feat_stride] (fp16 for an fp16 network)")
     dens = torch.zeros(stride, dtype=params.dtype, device="cuda")
     rgb = torch.zeros(3 * stride, dtype=params.dtype, device="cuda") if color else None
-    call("tn_mlp_forward", prec, n, ptr(info), stride, feat_stride, stride, ptr(params), net_offset, ptr(feat),
-         ptr(dir), int(color), ptr(dens), ptr(rgb), _app_ref(app), stream_ptr())
+    call("tn_mlp_inference", prec, int(f16), n, ptr(info), stride, feat_stride, stride, ptr(params), net_offset,
+         ptr(feat), ptr(dir), int(color), ptr(dens), ptr(rgb), _app_ref(app), stream_ptr())
     return dens, rgb
 
 
