@@ -121,7 +121,15 @@ def test_step_halves_reorder_the_weight_gradient_sums_only(tuning, precision):
         eag[2].train_steps(4, read_metrics=False, graph=False)
         cap[2].train_steps(4, read_metrics=False, graph=True)
         _same(eag[2], cap[2], f"halves round {r}")
-    for t in (ref, alt, eag, cap):
+    # split steps with the table's Adam fused into the sum pass: the same bits
+    unf, fus = _trainer(49, precision), _trainer(49, precision)
+    for r in range(2):
+        tn.set_tuning("hash_adam_fused", 0)
+        unf[2].train_steps(3, read_metrics=False, graph=r == 1)
+        tn.set_tuning("hash_adam_fused", 1)
+        fus[2].train_steps(3, read_metrics=False, graph=r == 1)
+        _same(unf[2], fus[2], f"halves fused round {r}")
+    for t in (ref, alt, eag, cap, unf, fus):
         t[2].teardown()
 
 
