@@ -5,9 +5,8 @@
   oracle's fp64 adjoint (norm-wise 1e-5; atomics change the summation order);
 * MLPs: fp32 MFMA path vs fp64 numpy (norm-wise 1e-5); fp16 MFMA path: forward
   vs fp64 numpy on fp16-rounded inputs (1e-2), backward vs fp64 numpy that
-  rounds each matrix operand to fp16 as the kernel does, within 3x the spread
-  that numpy's fp32-accumulating restatement shows against it (ReLU-mask flips
-  make fp16 weight gradients sensitive at ~5e-3; see the test);
+  rounds each matrix operand to fp16 as the kernel does, 1e-4 per weight
+  block and for dX;
 * Adam: vs the oracle (rtol 1e-6), including skipped zero-gradient hash entries.
 """
 import numpy as np
@@ -214,19 +213,12 @@ def test_mlp_backward(cuda, prec):
     got_x = dfeat.float().cpu().numpy().reshape(32, n).T
     blocks = (("W1", slice(0, 2048)), ("W2", slice(2048, 3072)), ("W3", slice(3072, 5120)),
               ("W4", slice(5120, 9216)), ("W5", slice(9216, 9408)))
-    if r16:
-        # With fp16 operands, a last-bit change of a forward activation (fp32
-        # vs fp64 accumulation) flips an fp16 rounding now and then, and a
-        # flipped ReLU mask moves one sample's whole contribution to a weight
-        # row: dW error then scales with sqrt(#flips), ~5e-3 on W3 for this
-        # case. The bar is the spread that the same arithmetic with numpy's
-        # fp32 accumulators shows against the fp64 reference.
-        fw32 = np_forward(net, feat, dir01, round16=True, acc=np.float32)
-        dX32, g32 = np_backward(net, fw32, G_rgb, dL_ds, t, ls, extra)
-        tol = {name: max(3.0 * norm_rel(g32[sl], g_ref[sl]), 1e-3) for name, sl in blocks}
-        tol["dX"] = max(3.0 * norm_rel(dX32, dX), 1e-3)
-    else:
-        tol = {name: 1e-5 for name in ("W1", "W2", "W3", "W4", "W5", "dX")}
+    # fp16: against fp64 that rounds every matrix operand to fp16 as the
+    # kernel does (the first layers' inputs as hi + lo pairs), what remains
+    # is fp32 against fp64 accumulation and the operand roundings it flips:
+    # 1.0e-5 to 2.2e-5 per block measured, a fixed bar of 1e-4 (the
+    # training configuration's gradient bar is 1e-3)
+    tol = {name: (1e-4 if r16 else 1e-5) for name in ("W1", "W2", "W3", "W4", "W5", "dX")}
     eg, ex = norm_rel(got_g, g_ref), norm_rel(got_x, dX)
     print(f"[mlp bwd {prec}] dW {eg:.2e} dX {ex:.2e}", flush=True)
     for name, sl in blocks:
