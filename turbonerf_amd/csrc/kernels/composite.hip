@@ -648,7 +648,7 @@ extern "C" int tn_composite_forward(int dtype, const uint32_t* info, uint32_t st
                                     const void* color, const float* dt, float* rgba, hipStream_t stream) {
     TN_CHECK_ARG(info && ray_steps && ray_offset && density && color && dt && rgba && max_samples <= stride);
     TN_CHECK_ARG(max_samples < (1u << 31));  // sample offsets fit an int
-    const uint32_t S = 128;
+    const uint32_t S = 256;
     const uint32_t waves = (max_samples + S - 1) / S + 1;
     const uint32_t blocks = (waves + kCompositeWaves - 1) / kCompositeWaves;
     switch (dtype) {
