@@ -34,7 +34,7 @@ enum tn_status {
 enum tn_tune_knob {
     TN_TUNE_HASH_FWD_BLOCK = 0,        /* threads per hash-forward workgroup: 64, 128 or 256 (256) */
     TN_TUNE_HASH_BWD_LEVEL_BLOCKS = 1, /* binned hash backward: count/scatter workgroups per level, 8..256 (128) */
-    TN_TUNE_MLP_FWD_BLOCKS_PER_CU = 2, /* MLP forward workgroups per compute unit, 1..4 (2) */
+    TN_TUNE_MLP_FWD_BLOCKS_PER_CU = 2, /* MLP forward workgroups per compute unit, 1..4 (3) */
     TN_TUNE_MLP_BWD_BLOCKS_PER_CU = 3, /* MLP backward workgroups per compute unit, 1..2 (1) */
     TN_TUNE_MARCH_MAX_BLOCKS = 4,      /* ray-marching workgroups at most, 256..8192 (2048) */
     TN_TUNE_ADAM_BLOCK = 5,            /* threads per Adam workgroup, 64..1024, a multiple of 64 (256) */

@@ -23,7 +23,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-DEFAULTS = {"hash_fwd_block": 256, "hash_bwd_level_blocks": 128, "mlp_fwd_blocks_per_cu": 2,
+DEFAULTS = {"hash_fwd_block": 256, "hash_bwd_level_blocks": 128, "mlp_fwd_blocks_per_cu": 3,
             "mlp_bwd_blocks_per_cu": 1, "march_max_blocks": 2048, "adam_block": 256, "hash_bwd_level_groups": 1,
             "hash_fwd_lds_levels": 0, "step_halves": 0, "hash_adam_fused": 0}
 OTHER = {"hash_fwd_block": 64, "hash_bwd_level_blocks": 16, "mlp_fwd_blocks_per_cu": 1, "march_max_blocks": 512,

@@ -978,7 +978,7 @@ def test_tuning_knobs_validate_and_count_changes():
 
     lib = _capi.lib()
     v = ctypes.c_int()
-    defaults = {0: 256, 1: 128, 2: 2, 3: 1, 4: 2048, 5: 256, 6: 1, 7: 0, 8: 0, 9: 0}
+    defaults = {0: 256, 1: 128, 2: 3, 3: 1, 4: 2048, 5: 256, 6: 1, 7: 0, 8: 0, 9: 0}
     for k, d in defaults.items():
         assert lib.tn_tune_get(k, ctypes.byref(v)) == 0 and v.value == d
     assert lib.tn_tune_get(10, ctypes.byref(v)) == 1 and lib.tn_tune_get(-1, ctypes.byref(v)) == 1

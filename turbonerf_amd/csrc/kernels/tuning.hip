@@ -12,7 +12,7 @@ namespace tn {
 
 namespace {
 
-std::atomic<int> g_tune[TN_TUNE_COUNT] = {256, 128, 2, 1, 2048, 256, 1, 0, 0, 0};
+std::atomic<int> g_tune[TN_TUNE_COUNT] = {256, 128, 3, 1, 2048, 256, 1, 0, 0, 0};
 std::atomic<uint32_t> g_generation{0};
 
 bool tune_valid(int knob, int v) {
